@@ -1,0 +1,366 @@
+// gegs_capi.hip -- C ABI of the transport library (include/gegs.h).
+//
+// Replaces the host <-> device contract of the reference (launch_kernels /
+// runSimulation, CUDA_EGS.cu:226-900; uploads in copy.cu, init.cu:909-1051,
+// media.cu:1030-1048): state lives in a context object, every call returns a
+// status code, nothing calls exit().
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gegs.h"
+#include "event.hpp"
+#include "plain.hpp"
+
+using namespace gegs;
+
+struct gegs_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    gegs_media_tables *d_media = nullptr;
+    gegs_region *d_regions = nullptr;
+    float *d_xb = nullptr, *d_yb = nullptr, *d_zb = nullptr;
+    uint32_t nx = 0, ny = 0, nz = 0;
+    size_t nvox = 0;
+    gegs_field field{};
+    bool have_media = false, have_phantom = false, have_field = false;
+    gegs_source *d_src = nullptr;
+    uint64_t nsrc = 0;
+    double *d_chunk = nullptr, *d_dose = nullptr, *d_dose2 = nullptr;
+    size_t tally_n = 0;
+    float *d_stack = nullptr;
+    size_t stack_bytes = 0;
+    uint32_t *d_next = nullptr;
+    unsigned long long *d_counters = nullptr;
+    bool hist_tally = false;
+    double *d_hist_edep = nullptr;
+    uint32_t *d_hist_steps = nullptr;
+    uint64_t hist_n = 0;
+    gegs_counters counters{};
+    float last_ms = 0.0f;
+    EventBanks banks;
+};
+
+static int fail(gegs_ctx *c, int code, const char *what, hipError_t e = hipSuccess) {
+    if (c) {
+        c->err = what;
+        if (e != hipSuccess) {
+            c->err += ": ";
+            c->err += hipGetErrorString(e);
+        }
+    }
+    return code;
+}
+
+#define HIPCHECK(c, call, what)                                        \
+    do {                                                               \
+        hipError_t e_ = (call);                                        \
+        if (e_ != hipSuccess) return fail((c), GEGS_ERR_DEVICE, (what), e_); \
+    } while (0)
+
+template <class T>
+static hipError_t realloc_dev(T *&p, size_t &have, size_t need_elems) {
+    if (p && have >= need_elems) return hipSuccess;
+    if (p) {
+        hipError_t e = hipFree(p);
+        if (e != hipSuccess) return e;
+        p = nullptr;
+    }
+    have = 0;
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&p), need_elems * sizeof(T));
+    if (e == hipSuccess) have = need_elems;
+    return e;
+}
+
+extern "C" {
+
+size_t gegs_sizeof_media_tables(void) { return sizeof(gegs_media_tables); }
+
+gegs_ctx *gegs_create(int device) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return nullptr;
+    if (hipSetDevice(device) != hipSuccess) return nullptr;
+    gegs_ctx *c = new gegs_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&c->d_next, sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&c->d_counters, 16 * sizeof(unsigned long long)) != hipSuccess) {
+        delete c;
+        return nullptr;
+    }
+    return c;
+}
+
+void gegs_destroy(gegs_ctx *c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    hipFree(c->d_media);
+    hipFree(c->d_regions);
+    hipFree(c->d_xb);
+    hipFree(c->d_yb);
+    hipFree(c->d_zb);
+    hipFree(c->d_src);
+    hipFree(c->d_chunk);
+    hipFree(c->d_dose);
+    hipFree(c->d_dose2);
+    hipFree(c->d_stack);
+    hipFree(c->d_next);
+    hipFree(c->d_counters);
+    hipFree(c->d_hist_edep);
+    hipFree(c->d_hist_steps);
+    c->banks.release();
+    hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *gegs_last_error(gegs_ctx *c) { return c ? c->err.c_str() : "no context"; }
+
+int gegs_load_media(gegs_ctx *c, const gegs_media_tables *t) {
+    if (!c || !t) return fail(c, GEGS_ERR_ARG, "gegs_load_media: null argument");
+    if (t->nmed < 1 || t->nmed > GEGS_MXMED) return fail(c, GEGS_ERR_MEDIA, "gegs_load_media: nmed out of range");
+    hipSetDevice(c->device);
+    if (!c->d_media) HIPCHECK(c, hipMalloc(&c->d_media, sizeof(gegs_media_tables)), "hipMalloc media");
+    HIPCHECK(c, hipMemcpyAsync(c->d_media, t, sizeof(gegs_media_tables), hipMemcpyHostToDevice, c->stream), "copy media");
+    HIPCHECK(c, hipStreamSynchronize(c->stream), "sync media");
+    c->have_media = true;
+    return GEGS_OK;
+}
+
+int gegs_load_phantom(gegs_ctx *c, uint32_t nx, uint32_t ny, uint32_t nz, const float *xb, const float *yb,
+                      const float *zb, const gegs_region *regions) {
+    if (!c || !xb || !yb || !zb || !regions) return fail(c, GEGS_ERR_ARG, "gegs_load_phantom: null argument");
+    if (nx < 2 || ny < 2 || nz < 2) return fail(c, GEGS_ERR_PHANTOM, "gegs_load_phantom: need at least 2 voxels per axis");
+    const size_t nvox = (size_t)nx * ny * nz;
+    if (nvox + VAC_AND_OTH > 0xFFFFFFFFull) return fail(c, GEGS_ERR_PHANTOM, "gegs_load_phantom: too many voxels");
+    // voxel media must exist in the loaded tables
+    for (size_t i = 0; i < nvox + VAC_AND_OTH; ++i) {
+        const uint16_t m = regions[i].med;
+        if (m != GEGS_VACUUM && m >= GEGS_MXMED) return fail(c, GEGS_ERR_PHANTOM, "gegs_load_phantom: region medium out of range");
+    }
+    const float *b[3] = {xb, yb, zb};
+    const uint32_t n[3] = {nx, ny, nz};
+    float **d[3] = {&c->d_xb, &c->d_yb, &c->d_zb};
+    hipSetDevice(c->device);
+    for (int a = 0; a < 3; ++a) {
+        for (uint32_t i = 0; i < n[a]; ++i)
+            if (!(b[a][i + 1] > b[a][i])) return fail(c, GEGS_ERR_PHANTOM, "gegs_load_phantom: bounds must increase");
+        // one extrapolated bound past the end keeps the neighbouring-voxel read
+        // of the reference's hownear defined
+        std::vector<float> pad(b[a], b[a] + n[a] + 1);
+        pad.push_back(b[a][n[a]] + (b[a][n[a]] - b[a][n[a] - 1]));
+        hipFree(*d[a]);
+        *d[a] = nullptr;
+        HIPCHECK(c, hipMalloc(d[a], pad.size() * sizeof(float)), "hipMalloc bounds");
+        HIPCHECK(c, hipMemcpy(*d[a], pad.data(), pad.size() * sizeof(float), hipMemcpyHostToDevice), "copy bounds");
+    }
+    hipFree(c->d_regions);
+    c->d_regions = nullptr;
+    HIPCHECK(c, hipMalloc(&c->d_regions, (nvox + VAC_AND_OTH) * sizeof(gegs_region)), "hipMalloc regions");
+    HIPCHECK(c, hipMemcpy(c->d_regions, regions, (nvox + VAC_AND_OTH) * sizeof(gegs_region), hipMemcpyHostToDevice),
+             "copy regions");
+    c->nx = nx;
+    c->ny = ny;
+    c->nz = nz;
+    c->nvox = nvox;
+    c->have_phantom = true;
+    return GEGS_OK;
+}
+
+int gegs_set_field(gegs_ctx *c, const gegs_field *f) {
+    if (!c || !f) return fail(c, GEGS_ERR_ARG, "gegs_set_field: null argument");
+    if (f->module_first != GEGS_M_PHANTOM) return fail(c, GEGS_ERR_ARG, "gegs_set_field: only the phantom module is built");
+    c->field = *f;
+    c->have_field = true;
+    return GEGS_OK;
+}
+
+int gegs_set_source(gegs_ctx *c, const gegs_source *src, uint64_t n) {
+    if (!c || (!src && n)) return fail(c, GEGS_ERR_ARG, "gegs_set_source: null argument");
+    if (n > 0xFFFFFFF0ull) return fail(c, GEGS_ERR_ARG, "gegs_set_source: at most 2^32-16 records per call");
+    hipSetDevice(c->device);
+    hipFree(c->d_src);
+    c->d_src = nullptr;
+    c->nsrc = n;
+    if (n == 0) return GEGS_OK;
+    HIPCHECK(c, hipMalloc(&c->d_src, n * sizeof(gegs_source)), "hipMalloc source");
+    HIPCHECK(c, hipMemcpy(c->d_src, src, n * sizeof(gegs_source), hipMemcpyHostToDevice), "copy source");
+    return GEGS_OK;
+}
+
+int gegs_set_history_tally(gegs_ctx *c, int enable) {
+    if (!c) return GEGS_ERR_ARG;
+    c->hist_tally = enable != 0;
+    return GEGS_OK;
+}
+
+static int prepare_tallies(gegs_ctx *c) {
+    size_t have = c->tally_n;
+    HIPCHECK(c, realloc_dev(c->d_chunk, have, c->nvox), "hipMalloc tally");
+    have = c->tally_n;
+    HIPCHECK(c, realloc_dev(c->d_dose, have, c->nvox), "hipMalloc tally");
+    have = c->tally_n;
+    HIPCHECK(c, realloc_dev(c->d_dose2, have, c->nvox), "hipMalloc tally");
+    c->tally_n = c->nvox;
+    const int blocks = 2048;
+    hipLaunchKernelGGL(fill_zero, dim3(blocks), dim3(256), 0, c->stream, c->d_chunk, c->nvox);
+    hipLaunchKernelGGL(fill_zero, dim3(blocks), dim3(256), 0, c->stream, c->d_dose, c->nvox);
+    hipLaunchKernelGGL(fill_zero, dim3(blocks), dim3(256), 0, c->stream, c->d_dose2, c->nvox);
+    HIPCHECK(c, hipMemsetAsync(c->d_counters, 0, 16 * sizeof(unsigned long long), c->stream), "clear counters");
+    if (c->hist_tally) {
+        if (c->hist_n < c->nsrc) {
+            hipFree(c->d_hist_edep);
+            hipFree(c->d_hist_steps);
+            c->d_hist_edep = nullptr;
+            c->d_hist_steps = nullptr;
+            HIPCHECK(c, hipMalloc(&c->d_hist_edep, c->nsrc * sizeof(double)), "hipMalloc history tally");
+            HIPCHECK(c, hipMalloc(&c->d_hist_steps, c->nsrc * sizeof(uint32_t)), "hipMalloc history tally");
+            c->hist_n = c->nsrc;
+        }
+    }
+    return GEGS_OK;
+}
+
+static Env make_env(gegs_ctx *c, uint64_t seed) {
+    Env env;
+    env.mt = c->d_media;
+    env.reg = c->d_regions;
+    env.xb = c->d_xb;
+    env.yb = c->d_yb;
+    env.zb = c->d_zb;
+    env.nx = c->nx;
+    env.ny = c->ny;
+    env.nz = c->nz;
+    env.f = c->field;
+    env.seed = seed;
+    return env;
+}
+
+static int finish_run(gegs_ctx *c, double *dose, double *dose2, hipEvent_t t0, hipEvent_t t1) {
+    HIPCHECK(c, hipEventSynchronize(t1), "run");
+    hipEventElapsedTime(&c->last_ms, t0, t1);
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+    unsigned long long h[16];
+    HIPCHECK(c, hipMemcpy(h, c->d_counters, sizeof(h), hipMemcpyDeviceToHost), "read counters");
+    c->counters.photon_steps = h[0];
+    c->counters.electron_steps = h[1];
+    c->counters.region_gathers = h[2];
+    c->counters.deposits = h[3];
+    for (int k = 0; k < 10; ++k) c->counters.interactions[k] = h[4 + k];
+    c->counters.histories = h[14];
+    c->counters.overflow = h[15];
+    if (dose) HIPCHECK(c, hipMemcpy(dose, c->d_dose, c->nvox * sizeof(double), hipMemcpyDeviceToHost), "read dose");
+    if (dose2) HIPCHECK(c, hipMemcpy(dose2, c->d_dose2, c->nvox * sizeof(double), hipMemcpyDeviceToHost), "read dose2");
+    if (c->counters.overflow) return fail(c, GEGS_ERR_QUEUE_OVERFLOW, "particle stack overflow: particles were lost");
+    return GEGS_OK;
+}
+
+static int check_ready(gegs_ctx *c, const gegs_run_options *o) {
+    if (!c || !o) return fail(c, GEGS_ERR_ARG, "run: null argument");
+    if (!c->have_media || !c->have_phantom || !c->have_field) return fail(c, GEGS_ERR_STATE, "run: load media, phantom and field first");
+    if (c->nsrc == 0) return fail(c, GEGS_ERR_STATE, "run: no source particles");
+    if (o->n_batches < 2 || o->n_batches > 4096) return fail(c, GEGS_ERR_ARG, "run: n_batches must be in [2, 4096]");
+    return GEGS_OK;
+}
+
+int gegs_run_plain(gegs_ctx *c, const gegs_run_options *o, double *dose, double *dose2) {
+    int rc = check_ready(c, o);
+    if (rc) return rc;
+    hipSetDevice(c->device);
+    rc = prepare_tallies(c);
+    if (rc) return rc;
+    const uint32_t block = o->block_size ? o->block_size : 128;
+    if (block != 128) return fail(c, GEGS_ERR_ARG, "run_plain: block_size must be 128");
+    const uint32_t grid = o->grid_size ? o->grid_size : 2048;
+    const int depth = o->stack_depth ? (int)o->stack_depth : 40;
+    const size_t nthreads = (size_t)grid * block;
+    size_t have = c->stack_bytes / sizeof(float);
+    HIPCHECK(c, realloc_dev(c->d_stack, have, nthreads * (size_t)depth * PART_WORDS), "hipMalloc stack");
+    c->stack_bytes = have * sizeof(float);
+    PlainArgs a;
+    a.env = make_env(c, o->seed);
+    a.src = c->d_src;
+    a.hist_offset = o->history_offset;
+    a.next = c->d_next;
+    a.stack = c->d_stack;
+    a.depth = depth;
+    a.dose = c->d_chunk;
+    a.hist_edep = c->hist_tally ? c->d_hist_edep : nullptr;
+    a.hist_steps = c->hist_tally ? c->d_hist_steps : nullptr;
+    a.counters = c->d_counters;
+    hipEvent_t t0, t1;
+    hipEventCreate(&t0);
+    hipEventCreate(&t1);
+    hipEventRecord(t0, c->stream);
+    const uint32_t nb = o->n_batches;
+    for (uint32_t b = 0; b < nb; ++b) {
+        a.first = (uint32_t)((c->nsrc * b) / nb);
+        a.last = (uint32_t)((c->nsrc * (b + 1)) / nb);
+        HIPCHECK(c, hipMemcpyAsync(c->d_next, &a.first, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream), "set counter");
+        hipLaunchKernelGGL(plain_kernel, dim3(grid), dim3(block), 0, c->stream, a);
+        hipLaunchKernelGGL(fold_batch, dim3(2048), dim3(256), 0, c->stream, c->d_chunk, c->d_dose, c->d_dose2, c->nvox);
+    }
+    hipEventRecord(t1, c->stream);
+    HIPCHECK(c, hipGetLastError(), "launch plain_kernel");
+    return finish_run(c, dose, dose2, t0, t1);
+}
+
+int gegs_run_event(gegs_ctx *c, const gegs_run_options *o, double *dose, double *dose2) {
+    int rc = check_ready(c, o);
+    if (rc) return rc;
+    hipSetDevice(c->device);
+    rc = prepare_tallies(c);
+    if (rc) return rc;
+    hipEvent_t t0, t1;
+    hipEventCreate(&t0);
+    hipEventCreate(&t1);
+    EventConfig cfg;
+    cfg.block = o->block_size ? o->block_size : 256;
+    cfg.capacity = o->stack_depth ? (size_t)o->stack_depth * 65536 : 0;
+    const uint32_t nb = o->n_batches;
+    Env env = make_env(c, o->seed);
+    hipError_t e = c->banks.reserve(c->nsrc, nb, cfg);
+    if (e != hipSuccess) return fail(c, GEGS_ERR_DEVICE, "event banks", e);
+    hipEventRecord(t0, c->stream);
+    for (uint32_t b = 0; b < nb; ++b) {
+        const uint32_t first = (uint32_t)((c->nsrc * b) / nb);
+        const uint32_t last = (uint32_t)((c->nsrc * (b + 1)) / nb);
+        e = run_event_batch(c->banks, env, c->d_src, first, last, o->history_offset, c->d_chunk,
+                            c->hist_tally ? c->d_hist_edep : nullptr, c->hist_tally ? c->d_hist_steps : nullptr,
+                            c->d_counters, cfg, c->stream);
+        if (e != hipSuccess) return fail(c, GEGS_ERR_DEVICE, "event batch", e);
+        hipLaunchKernelGGL(fold_batch, dim3(2048), dim3(256), 0, c->stream, c->d_chunk, c->d_dose, c->d_dose2, c->nvox);
+    }
+    hipEventRecord(t1, c->stream);
+    HIPCHECK(c, hipGetLastError(), "launch event kernels");
+    return finish_run(c, dose, dose2, t0, t1);
+}
+
+int gegs_history_tally(gegs_ctx *c, double *edep, uint32_t *steps) {
+    if (!c) return GEGS_ERR_ARG;
+    if (!c->hist_tally || !c->d_hist_edep) return fail(c, GEGS_ERR_STATE, "history tally not enabled");
+    hipSetDevice(c->device);
+    if (edep) HIPCHECK(c, hipMemcpy(edep, c->d_hist_edep, c->nsrc * sizeof(double), hipMemcpyDeviceToHost), "read edep");
+    if (steps) HIPCHECK(c, hipMemcpy(steps, c->d_hist_steps, c->nsrc * sizeof(uint32_t), hipMemcpyDeviceToHost), "read steps");
+    return GEGS_OK;
+}
+
+int gegs_get_counters(gegs_ctx *c, gegs_counters *out) {
+    if (!c || !out) return GEGS_ERR_ARG;
+    *out = c->counters;
+    return GEGS_OK;
+}
+
+int gegs_last_kernel_ms(gegs_ctx *c, float *ms) {
+    if (!c || !ms) return GEGS_ERR_ARG;
+    *ms = c->last_ms;
+    return GEGS_OK;
+}
+
+}  // extern "C"

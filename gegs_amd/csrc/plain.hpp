@@ -1,0 +1,160 @@
+// plain.hpp -- the plain per-history kernel (the baseline method).
+//
+// One thread transports one history at a time with its own LIFO particle
+// stack, exactly the order the oracle uses; the stack lives in HBM in a
+// slot-major, thread-minor layout so that the lanes of a wavefront touching
+// the same slot depth hit consecutive words.  Histories are handed out by an
+// atomic counter, so threads that finish early pick up more work.
+#pragma once
+#include "transport.hpp"
+
+namespace gegs {
+
+constexpr int PART_WORDS = 16;
+
+struct StackSink {
+    float *base;          // [slot][word][thread]
+    uint32_t stride;      // threads in the grid
+    uint32_t tid;
+    int sp;
+    int depth;
+    uint32_t overflow;
+
+    __device__ float *at(int slot, int word) const {
+        return base + ((size_t)slot * PART_WORDS + word) * stride + tid;
+    }
+    __device__ void push(const Part &p) {
+        if (sp >= depth) {
+            overflow++;
+            return;
+        }
+        const int k = sp++;
+        *at(k, 0) = p.e;
+        *at(k, 1) = p.x;
+        *at(k, 2) = p.y;
+        *at(k, 3) = p.z;
+        *at(k, 4) = p.u;
+        *at(k, 5) = p.v;
+        *at(k, 6) = p.w;
+        *at(k, 7) = p.wt;
+        *at(k, 8) = p.dnear;
+        *at(k, 9) = __uint_as_float(p.region);
+        *at(k, 10) = __uint_as_float((uint32_t)(p.charge + 1) | ((uint32_t)p.status << 8) | ((uint32_t)p.module << 16));
+        *at(k, 11) = __uint_as_float((uint32_t)p.lin);
+        *at(k, 12) = __uint_as_float((uint32_t)(p.lin >> 32));
+        *at(k, 13) = __uint_as_float(p.blk);
+    }
+    __device__ Part pop() {
+        const int k = --sp;
+        Part p;
+        p.e = *at(k, 0);
+        p.x = *at(k, 1);
+        p.y = *at(k, 2);
+        p.z = *at(k, 3);
+        p.u = *at(k, 4);
+        p.v = *at(k, 5);
+        p.w = *at(k, 6);
+        p.wt = *at(k, 7);
+        p.dnear = *at(k, 8);
+        p.region = __float_as_uint(*at(k, 9));
+        const uint32_t pk = __float_as_uint(*at(k, 10));
+        p.charge = (int32_t)(pk & 0xffu) - 1;
+        p.status = (int32_t)((pk >> 8) & 0xffu);
+        p.module = (int32_t)((pk >> 16) & 0xffu);
+        p.lin = (uint64_t)__float_as_uint(*at(k, 11)) | ((uint64_t)__float_as_uint(*at(k, 12)) << 32);
+        p.blk = __float_as_uint(*at(k, 13));
+        return p;
+    }
+};
+
+struct PlainArgs {
+    Env env;
+    const gegs_source *src;
+    uint32_t first, last;        // source records [first, last)
+    uint64_t hist_offset;        // global index of record 0
+    uint32_t *next;              // work counter, starts at first
+    float *stack;
+    int depth;
+    double *dose;
+    double *hist_edep;           // optional, indexed by record
+    uint32_t *hist_steps;        // optional
+    unsigned long long *counters;   // gegs_counters layout
+};
+
+__device__ inline Part source_particle(const gegs_source &r, int module) {
+    Part p;
+    p.e = r.e; p.x = r.x; p.y = r.y; p.z = r.z;
+    p.u = r.u; p.v = r.v; p.w = r.w; p.wt = r.wt;
+    p.charge = r.charge;
+    p.region = 0;
+    p.module = module;
+    p.dnear = INFI_DIST;
+    p.blk = 0;
+    p.status = r.charge == 0 ? P_PHOTON_STEP : E_ELECTRON_STEP;
+    return p;
+}
+
+__global__ void __launch_bounds__(128) plain_kernel(PlainArgs a) {
+    StackSink sink;
+    sink.base = a.stack;
+    sink.stride = gridDim.x * blockDim.x;
+    sink.tid = blockIdx.x * blockDim.x + threadIdx.x;
+    sink.depth = a.depth;
+    sink.overflow = 0;
+    Tally t;
+    t.dose = a.dose;
+    t.photon_steps = t.electron_steps = t.gathers = t.deposits = 0;
+    uint32_t pops[N_EVENTS];
+#pragma unroll
+    for (int k = 0; k < N_EVENTS; ++k) pops[k] = 0;
+    uint32_t nhist = 0;
+    for (;;) {
+        const uint32_t i = atomicAdd(a.next, 1u);
+        if (i >= a.last) break;
+        Part p = source_particle(a.src[i], a.env.f.module_first);
+        p.lin = primary_lineage(a.hist_offset + i);
+        t.hist_edep = 0.0;
+        t.steps = 0;
+        sink.sp = 0;
+        sink.push(p);
+        while (sink.sp > 0) {
+            Part q = sink.pop();
+            Stream s;
+            s.begin(q, a.env.seed);
+#pragma unroll
+            for (int k = 0; k < N_EVENTS; ++k) pops[k] += (q.status == k);
+            handle(a.env, s, q, sink, t);
+        }
+        if (a.hist_edep) a.hist_edep[i] = t.hist_edep;
+        if (a.hist_steps) a.hist_steps[i] = t.steps;
+        nhist++;
+    }
+    unsigned long long *c = a.counters;
+    atomicAdd(c + 0, (unsigned long long)t.photon_steps);
+    atomicAdd(c + 1, (unsigned long long)t.electron_steps);
+    atomicAdd(c + 2, (unsigned long long)t.gathers);
+    atomicAdd(c + 3, (unsigned long long)t.deposits);
+#pragma unroll
+    for (int k = 0; k < N_EVENTS; ++k) atomicAdd(c + 4 + k, (unsigned long long)pops[k]);
+    atomicAdd(c + 14, (unsigned long long)nhist);
+    atomicAdd(c + 15, (unsigned long long)sink.overflow);
+}
+
+// batch boundary: fold the batch tally into the run totals and clear it
+__global__ void fold_batch(double *chunk, double *dose, double *dose2, size_t n) {
+    for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x) {
+        const double d = chunk[v];
+        if (d != 0.0) {
+            dose[v] += d;
+            dose2[v] += d * d;
+            chunk[v] = 0.0;
+        }
+    }
+}
+
+__global__ void fill_zero(double *p, size_t n) {
+    for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x)
+        p[v] = 0.0;
+}
+
+}  // namespace gegs

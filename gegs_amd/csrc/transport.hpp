@@ -1,0 +1,1499 @@
+// transport.hpp -- device-side coupled electron/photon transport for gfx950.
+//
+// Event handlers of the reference's event kernels, one device function per
+// event type, shared by the plain per-history kernel (plain.hip) and the
+// event-queued kernels (event.hip):
+//   photon_step   photon.cu:6-292       compton   compton.cu:17-492
+//   photo         photo.cu:1-279        rayleigh  rayleigh.cu:4-157
+//   pair          pair.cu:14-431        electron  electron.cu:888-2622
+//   moller        moller.cu:16-210      brems     brems.cu:36-308
+//   bhabha        bhabha.cu:12-172      annih     annih.cu:16-206
+//   annih_at_rest annihatrest.cu:10-68  mscat / msdist_pI electron.cu:77-882
+//   howfar/hownear_phantom  geom.cu:54-307, geom.cu:1422-1543
+// A handler consumes one particle from its event type and hands every
+// particle it creates, or forwards to another event type, to a Sink: the
+// per-thread stack of the plain kernel or the SoA queues of the event path.
+//
+// Random numbers: every particle owns a Philox-4x32-10 stream keyed by
+// (seed, 64-bit lineage) with a 32-bit block counter; four uniforms per
+// block, in (0, 1].  The buffer lives only while the particle is being
+// handled; a forwarded particle keeps its lineage and next block, a created
+// particle gets a lineage derived from its parent's lineage and stream
+// position.  Results therefore do not depend on processing order, which is
+// what lets the plain kernel, the event-queued kernels and the adjoint replay
+// reproduce each other particle by particle.
+//
+// Arithmetic is binary32 with explicit float literals except where the
+// reference keeps double (prm, the FP64 tallies).  Build with
+// -ffp-contract=off and correctly rounded division/sqrt.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/gegs.h"
+#include "gmath.hpp"
+
+namespace gegs {
+
+constexpr float RM = 0.5110034f;
+constexpr double PRM = 0.5110034;
+constexpr float RMT2 = 2.0f * RM;
+constexpr float RMSQ = RM * RM;
+constexpr float VACUUM_STEP = 1e8f;
+constexpr double EPSGMFP = 1e-5;
+constexpr double EPSEMFP = 1e-5;
+constexpr float EPSTEP = 0.0005f;
+constexpr float INFI_DIST = 1e5f;
+constexpr uint32_t VAC_AND_OTH = 6;
+constexpr uint32_t VACUUM = GEGS_VACUUM;
+constexpr int MXMED = GEGS_MXMED;
+constexpr int MXGE = GEGS_MXGE;
+constexpr int MXEKE = GEGS_MXEKE;
+constexpr int MXRAYFF = GEGS_MXRAYFF;
+constexpr int RAYCDFSIZE = GEGS_RAYCDFSIZE;
+constexpr int MAXL_MS = GEGS_MAXL_MS;
+constexpr int MAXQ_MS = GEGS_MAXQ_MS;
+constexpr int MAXU_MS = GEGS_MAXU_MS;
+constexpr float HC_INVERSE = 80.65506856998f;
+constexpr float TWICE_HC2 = 0.000307444456f;
+
+// particle_status of the reference (CUDA_EGS.h:416-427)
+enum Event : int {
+    P_PHOTON_STEP = 0, P_COMPTON = 1, P_PHOTO = 2, P_RAYLEIGH = 3, P_PAIR = 4,
+    E_ELECTRON_STEP = 5, E_MOLLER = 6, E_BREMS = 7, E_BHABHA = 8, E_ANNIH = 9,
+    N_EVENTS = 10
+};
+
+struct Part {
+    float e, x, y, z, u, v, w, wt, dnear;
+    uint32_t region;
+    int32_t charge;
+    int32_t status;
+    int32_t module;
+    uint64_t lin;
+    uint32_t blk;
+};
+
+// ------------------------------------------------------------------ Philox
+__host__ __device__ inline void philox10(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3,
+                                         uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t m0 = (uint64_t)0xD2511F53u * c0;
+        const uint64_t m1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(m1 >> 32) ^ c1 ^ k0;
+        const uint32_t n2 = (uint32_t)(m0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)m1;
+        c3 = (uint32_t)m0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+__host__ __device__ inline uint64_t splitmix(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__host__ __device__ inline uint64_t primary_lineage(uint64_t hist) {
+    return splitmix(hist ^ 0x5851F42D4C957F2Dull);
+}
+
+struct Stream {
+    uint64_t lin;
+    uint32_t blk;
+    int pos;
+    int nchild;
+    float b0, b1, b2, b3;
+    uint32_t k0, k1;
+
+    __device__ void begin(const Part &p, uint64_t seed) {
+        lin = p.lin;
+        blk = p.blk;
+        pos = 4;
+        nchild = 0;
+        k0 = (uint32_t)seed;
+        k1 = (uint32_t)(seed >> 32);
+    }
+    __device__ float next() {
+        if (pos == 4) {
+            uint32_t c0 = blk, c1 = (uint32_t)lin, c2 = (uint32_t)(lin >> 32), c3 = 0x47454753u;
+            philox10(c0, c1, c2, c3, k0, k1);
+            const float s = 5.96046448e-08f;
+            b0 = (float)((c0 >> 8) + 1u) * s;
+            b1 = (float)((c1 >> 8) + 1u) * s;
+            b2 = (float)((c2 >> 8) + 1u) * s;
+            b3 = (float)((c3 >> 8) + 1u) * s;
+            blk += 1;
+            pos = 0;
+        }
+        const int k = pos++;
+        return k == 0 ? b0 : (k == 1 ? b1 : (k == 2 ? b2 : b3));
+    }
+    __device__ uint64_t child() {
+        const uint64_t at = (uint64_t)blk * 4u + (uint64_t)pos;
+        const uint64_t k = (uint64_t)(nchild++);
+        return lin ^ splitmix((at << 8) | k);
+    }
+};
+
+__device__ inline uint64_t child_lineage(Stream &s) { return splitmix(s.child()); }
+
+// ------------------------------------------------------------------ environment
+struct Env {
+    const gegs_media_tables *mt;
+    const gegs_region *reg;
+    const float *xb, *yb, *zb;   // N+2 entries each
+    uint32_t nx, ny, nz;
+    gegs_field f;
+    uint64_t seed;
+};
+
+struct Tally {
+    double *dose;          // per voxel, FP64
+    double hist_edep;
+    uint32_t steps;
+    uint32_t photon_steps, electron_steps, gathers, deposits;
+
+    __device__ void score(uint32_t region, float energy) {
+        if (region < VAC_AND_OTH) return;
+        const double v = (double)energy;
+        unsafeAtomicAdd(dose + (region - VAC_AND_OTH), v);
+        hist_edep += v;
+        deposits++;
+    }
+};
+
+template <class Sink>
+__device__ inline void forward(Sink &sink, Stream &s, Part &p, int status) {
+    p.status = status;
+    p.lin = s.lin;
+    p.blk = s.blk;
+    sink.push(p);
+}
+
+template <class Sink>
+__device__ inline void create(Sink &sink, Stream &s, Part &p, int status) {
+    p.status = status;
+    p.lin = child_lineage(s);
+    p.blk = 0;
+    sink.push(p);
+}
+
+// ------------------------------------------------------------------ geometry
+struct Local {
+    float x, y, z, u, v, w;
+};
+
+__device__ inline Local phantom_frame(const gegs_field &f, const Part &p) {
+    float x, y, z, u, v, w;
+    if (f.change_xyz) {
+        x = p.y; y = p.z - f.source_to_isocenter; z = p.x;
+        u = p.v; v = p.w; w = p.u;
+    } else {
+        x = p.x; y = p.y; z = p.z - f.source_to_isocenter;
+        u = p.u; v = p.v; w = p.w;
+    }
+    Local q;
+    const float rx = x * f.cos_angle - y * f.sin_angle;
+    const float ry = x * f.sin_angle + y * f.cos_angle;
+    const float ru = u * f.cos_angle - v * f.sin_angle;
+    const float rv = u * f.sin_angle + v * f.cos_angle;
+    q.x = rx + f.iso_x;
+    q.y = ry + f.iso_y;
+    q.z = z + f.iso_z;
+    q.u = ru;
+    q.v = rv;
+    q.w = w;
+    return q;
+}
+
+__device__ inline int locate(float p, uint32_t n, const float *b) {
+    if (p < b[0] || p > b[n]) return -1;
+    if (n == 1) return 0;
+    int lo = 0, hi = (int)n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (p <= b[mid]) hi = mid; else lo = mid;
+    }
+    return hi - 1;
+}
+
+// distance to the next voxel face along the flight direction; returns the
+// region entered (0 = outside) and shortens t (geom.cu:54-307)
+__device__ inline uint32_t howfar_phantom(const Env &env, const Part &p, float &t) {
+    const Local q = phantom_frame(env.f, p);
+    const uint32_t nx = env.nx, ny = env.ny, nz = env.nz, nxy = nx * ny;
+    const float *xb = env.xb, *yb = env.yb, *zb = env.zb;
+    if (p.region >= VAC_AND_OTH) {
+        uint32_t ir = p.region - VAC_AND_OTH;
+        int iz = (int)(ir / nxy);
+        ir -= (uint32_t)iz * nxy;
+        int iy = (int)(ir / nx);
+        int ix = (int)(ir - (uint32_t)iy * nx);
+        uint32_t inew = p.region;
+        if (q.u > 0.0f) {
+            const float d = (xb[ix + 1] - q.x) / q.u;
+            if (d <= t) { t = d; inew = (++ix < (int)nx) ? p.region + 1 : 0u; }
+        } else if (q.u < 0.0f) {
+            const float d = (xb[ix] - q.x) / q.u;
+            if (d <= t) { t = d; inew = (--ix >= 0) ? p.region - 1 : 0u; }
+        }
+        if (q.v > 0.0f) {
+            const float d = (yb[iy + 1] - q.y) / q.v;
+            if (d <= t) { t = d; inew = (++iy < (int)ny) ? p.region + nx : 0u; }
+        } else if (q.v < 0.0f) {
+            const float d = (yb[iy] - q.y) / q.v;
+            if (d <= t) { t = d; inew = (--iy >= 0) ? p.region - nx : 0u; }
+        }
+        if (q.w > 0.0f) {
+            const float d = (zb[iz + 1] - q.z) / q.w;
+            if (d <= t) { t = d; inew = (++iz < (int)nz) ? p.region + nxy : 0u; }
+        } else if (q.w < 0.0f) {
+            const float d = (zb[iz] - q.z) / q.w;
+            if (d <= t) { t = d; inew = (--iz >= 0) ? p.region - nxy : 0u; }
+        }
+        if (t < 0.0f) t = EPSTEP;
+        return inew;
+    }
+    // from outside: try the x, y, z entry faces in turn
+    float t1 = 0.0f;
+    int ix = -1, iy, iz;
+    if (q.x <= xb[0] && q.u > 0.0f) { t1 = (xb[0] - q.x) / q.u; ix = 0; }
+    else if (q.x >= xb[nx] && q.u < 0.0f) { t1 = (xb[nx] - q.x) / q.u; ix = (int)nx - 1; }
+    if (ix >= 0 && t1 <= t) {
+        iy = locate(q.y + q.v * t1, ny, yb);
+        if (iy >= 0) {
+            iz = locate(q.z + q.w * t1, nz, zb);
+            if (iz >= 0) { t = t1; return (uint32_t)(ix + iy * (int)nx + iz * (int)nxy) + VAC_AND_OTH; }
+        }
+    }
+    iy = -1;
+    if (q.y <= yb[0] && q.v > 0.0f) { t1 = (yb[0] - q.y) / q.v; iy = 0; }
+    else if (q.y >= yb[ny] && q.v < 0.0f) { t1 = (yb[ny] - q.y) / q.v; iy = (int)ny - 1; }
+    if (iy >= 0 && t1 <= t) {
+        ix = locate(q.x + q.u * t1, nx, xb);
+        if (ix >= 0) {
+            iz = locate(q.z + q.w * t1, nz, zb);
+            if (iz >= 0) { t = t1; return (uint32_t)(ix + iy * (int)nx + iz * (int)nxy) + VAC_AND_OTH; }
+        }
+    }
+    iz = -1;
+    if (q.z <= zb[0] && q.w > 0.0f) { t1 = (zb[0] - q.z) / q.w; iz = 0; }
+    else if (q.z >= zb[nz] && q.w < 0.0f) { t1 = (zb[nz] - q.z) / q.w; iz = (int)nz - 1; }
+    if (iz >= 0 && t1 <= t) {
+        ix = locate(q.x + q.u * t1, nx, xb);
+        if (ix >= 0) {
+            iy = locate(q.y + q.v * t1, ny, yb);
+            if (iy >= 0) { t = t1; return (uint32_t)(ix + iy * (int)nx + iz * (int)nxy) + VAC_AND_OTH; }
+        }
+    }
+    return 0u;
+}
+
+// perpendicular distance to the closest face; keeps the reference's
+// region-5 voxel offset (geom.cu:1487-1490), reading the neighbouring voxel
+__device__ inline float hownear_phantom(const Env &env, const Part &p) {
+    const Local q = phantom_frame(env.f, p);
+    const uint32_t nx = env.nx, ny = env.ny, nz = env.nz;
+    const float *xb = env.xb, *yb = env.yb, *zb = env.zb;
+    if (p.region > 4) {
+        uint32_t ir = p.region - 5u;
+        const int iz = (int)(ir / (nx * ny));
+        ir -= (uint32_t)iz * nx * ny;
+        const int iy = (int)(ir / nx);
+        const int ix = (int)(ir - (uint32_t)iy * nx);
+        float a, dx, dy, dz;
+        a = xb[ix + 1] - q.x; dx = q.x - xb[ix]; if (a < dx) dx = a;
+        a = yb[iy + 1] - q.y; dy = q.y - yb[iy]; if (a < dy) dy = a;
+        a = zb[iz + 1] - q.z; dz = q.z - zb[iz]; if (a < dz) dz = a;
+        float tp = dx < dy ? dx : dy;
+        if (tp > dz) tp = dz;
+        return tp;
+    }
+    const float lx = xb[0], ly = yb[0], lz = zb[0];
+    float hx = xb[nx], hy = yb[ny], hz = zb[nz];
+    if (q.x - lx > 0 && q.x - hx > 0) {} else if (q.x - lx < 0 && q.x - hx < 0) hx = lx; else hx = q.x;
+    if (q.y - ly > 0 && q.y - hy > 0) {} else if (q.y - ly < 0 && q.y - hy < 0) hy = ly; else hy = q.y;
+    if (q.z - lz > 0 && q.z - hz > 0) {} else if (q.z - lz < 0 && q.z - hz < 0) hz = lz; else hz = q.z;
+    const float ddx = q.x - hx, ddy = q.y - hy, ddz = q.z - hz;
+    return sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);
+}
+
+// ------------------------------------------------------------------ rotations (utility.cu:94-218)
+__device__ inline void turn(float costhe, float sinthe, float cosphi, float sinphi, Part &p) {
+    const float A = p.u, B = p.v, C = p.w;
+    const float s2 = A * A + B * B;
+    if (s2 < 1e-20f) {
+        p.u = sinthe * cosphi;
+        p.v = sinthe * sinphi;
+        p.w = C * costhe;
+        return;
+    }
+    const float sp = sqrtf(s2);
+    const float us = sinthe * cosphi, vs = sinthe * sinphi;
+    const float sd = B / sp, cd = A / sp;
+    p.u = C * cd * us - sd * vs + A * costhe;
+    p.v = C * sd * us + cd * vs + B * costhe;
+    p.w = -sp * us + C * costhe;
+}
+
+__device__ inline void uphi21(Stream &s, float costhe, float sinthe, float &cosphi, float &sinphi, Part &p) {
+    const float r = s.next();
+    gsincos2pi(r, sinphi, cosphi);
+    turn(costhe, sinthe, cosphi, sinphi, p);
+}
+
+__device__ inline void azimuth_box(Stream &s, float &cphi, float &sphi) {
+    float xp, yp, r2;
+    do {
+        xp = s.next();
+        xp = 2.0f * xp - 1.0f;
+        yp = s.next();
+        r2 = xp * xp + yp * yp;
+    } while (r2 > 1.0f);
+    r2 = 1.0f / r2;
+    cphi = (xp * xp - yp * yp) * r2;
+    sphi = 2.0f * xp * yp * r2;
+}
+
+__device__ inline float to_kinetic(float etot) { return (float)((double)etot - PRM); }
+
+// ------------------------------------------------------------------ annihilation at rest
+template <class Sink>
+__device__ void annih_at_rest(const Env &env, Stream &s, Part &p, const gegs_region &rd, Sink &sink, Tally &t) {
+    const int nsplit = env.mt->nbr_split;
+    float edep = 0.0f;
+    if (nsplit > 1) p.wt = p.wt / (float)nsplit;
+    p.e = (float)PRM;
+    p.charge = 0;
+    for (int i = 1; i <= nsplit; ++i) {
+        float costhe = s.next();
+        costhe = 2.0f * costhe - 1.0f;
+        const float sinthe = sqrtf(fmaxf(0.0f, (1.0f - costhe) * (1.0f + costhe)));
+        float cphi, sphi;
+        azimuth_box(s, cphi, sphi);
+        p.u = sinthe * cphi;
+        p.v = sinthe * sphi;
+        p.w = costhe;
+        if (p.e < rd.pcut) edep += p.e; else create(sink, s, p, P_PHOTON_STEP);
+        p.u = -p.u;
+        p.v = -p.v;
+        p.w = -p.w;
+        if (p.e < rd.pcut) edep += p.e; else create(sink, s, p, P_PHOTON_STEP);
+    }
+    t.score(p.region, edep * p.wt);
+}
+
+// ------------------------------------------------------------------ photon flight
+template <class Sink>
+__device__ void photon_step(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+    const gegs_media_tables *mt = env.mt;
+    for (;;) {
+        t.photon_steps++;
+        t.steps++;
+        gegs_region rd = env.reg[p.region];
+        if (p.e <= rd.pcut) { t.score(p.region, p.e * p.wt); return; }
+        if (p.wt <= 0.0f) return;
+        float r1 = s.next();
+        float dpmfp = -glog(r1);
+        const float gle = glog(p.e);
+        int lgle = 0;
+        float tstep, gmfp_val = 0.0f, cohfac = 0.0f;
+        if (rd.med == VACUUM) {
+            tstep = VACUUM_STEP;
+        } else {
+            const gegs_float2 ge = mt->ge[rd.med];
+            lgle = (int)(ge.x + ge.y * gle);
+            const gegs_float2 gm = mt->gmfp[rd.med * MXGE + lgle];
+            gmfp_val = (gm.x + gm.y * gle) / rd.rhof;
+            if (rd.flags & GEGS_F_RAYLEIGH) {
+                const gegs_float2 ch = mt->cohe[rd.med * MXGE + lgle];
+                cohfac = ch.x + ch.y * gle;
+                gmfp_val *= cohfac;
+            }
+            tstep = gmfp_val * dpmfp;
+        }
+        const uint32_t nreg = howfar_phantom(env, p, tstep);
+        int idisc = 0;
+        if (nreg == 0) idisc = rd.med == VACUUM ? 1 : -1;
+        if (idisc > 0) { p.region = 0; return; }
+        p.x += p.u * tstep;
+        p.y += p.v * tstep;
+        p.z += p.w * tstep;
+        if (rd.med != VACUUM) dpmfp = fmaxf(0.0f, dpmfp - tstep / gmfp_val);
+        const uint32_t old_med = rd.med;
+        if (nreg != p.region) {
+            p.region = nreg;
+            rd = env.reg[nreg];
+            t.gathers++;
+        }
+        if (p.e <= rd.pcut || idisc < 0) { t.score(p.region, p.e * p.wt); return; }
+        if (rd.med != old_med || rd.med == VACUUM || (double)dpmfp >= EPSGMFP) continue;
+        if (rd.flags & GEGS_F_RAYLEIGH) {
+            r1 = s.next();
+            if (r1 < 1.0f - cohfac) { forward(sink, s, p, P_RAYLEIGH); return; }
+        }
+        const gegs_float2 g1 = mt->gbr1[rd.med * MXGE + lgle];
+        const gegs_float2 g2 = mt->gbr2[rd.med * MXGE + lgle];
+        const float gbr1 = g1.x + g1.y * gle;
+        const float gbr2 = g2.x + g2.y * gle;
+        r1 = s.next();
+        if (r1 <= gbr1 && p.e > 2.0f * RM) forward(sink, s, p, P_PAIR);
+        else if (r1 < gbr2) forward(sink, s, p, P_COMPTON);
+        else forward(sink, s, p, P_PHOTO);
+        return;
+    }
+}
+
+// ------------------------------------------------------------------ Compton (Klein-Nishina)
+template <class Sink>
+__device__ void compton(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+    const gegs_region rd = env.reg[p.region];
+    float edep = 0.0f;
+    Part e1 = p;
+    const float peig = p.e;
+    const float ko = peig / RM;
+    const float broi = 1.0f + 2.0f * ko;
+    float bro = 0.0f, br = 0.0f, temp = 0.0f, sinthe = 0.0f;
+    float alph1 = 0.0f, alpha = 0.0f, rejmax = 0.0f;
+    bool first = true;
+    do {
+        if (ko > 2.0f) {
+            if (first) {
+                alph1 = glog(broi);
+                bro = 1.0f / broi;
+                alpha = alph1 + ko * (broi + 1.0f) * bro * bro;
+            }
+            float aux, rejf3, r19;
+            int n = 0;
+            do {
+                const float r15 = s.next();
+                const float r16 = s.next();
+                if (r15 * alpha < alph1) br = gexp(alph1 * r16) * bro;
+                else br = sqrtf(r16 * broi * broi + (1.0f - r16)) * bro;
+                temp = (1.0f - br) / (ko * br);
+                sinthe = fmaxf(0.0f, temp * (2.0f - temp));
+                aux = 1.0f + br * br;
+                rejf3 = aux - br * sinthe;
+                r19 = s.next();
+                if (++n > 100) break;
+            } while (r19 * aux > rejf3);
+        } else {
+            if (first) {
+                bro = 1.0f / broi;
+                rejmax = broi + bro;
+            }
+            float r16, rejf3;
+            int n = 0;
+            do {
+                const float r15 = s.next();
+                r16 = s.next();
+                br = bro + (1.0f - bro) * r15;
+                temp = (1.0f - br) / (ko * br);
+                sinthe = fmaxf(0.0f, temp * (2.0f - temp));
+                rejf3 = 1.0f + br * br - br * sinthe;
+                if (n++ > 100) break;
+            } while (r16 * br * rejmax > rejf3);
+        }
+        first = false;
+    } while (br < bro || br > 1.0f);
+    float costhe = 1.0f - temp;
+    const float pesg = br * peig;
+    const float pese = (float)((double)(peig - pesg) + PRM);
+    sinthe = sqrtf(sinthe);
+    p.e = pesg;
+    float sinphi, cosphi;
+    uphi21(s, costhe, sinthe, cosphi, sinphi, p);
+    if (p.e < rd.pcut) edep += p.e; else create(sink, s, p, P_PHOTON_STEP);
+    const float aux = 1.0f + br * br - 2.0f * br * costhe;
+    if (aux > 1e-8f) {
+        costhe = (1.0f - br * costhe) / sqrtf(aux);
+        sinthe = (1.0f - costhe) * (1.0f + costhe);
+        sinthe = sinthe > 0.0f ? -sqrtf(sinthe) : 0.0f;
+    } else {
+        costhe = 0.0f;
+        sinthe = -1.0f;
+    }
+    e1.e = pese;
+    e1.charge = -1;
+    turn(costhe, sinthe, cosphi, sinphi, e1);
+    if (e1.e < rd.ecut) edep += to_kinetic(e1.e); else create(sink, s, e1, E_ELECTRON_STEP);
+    t.score(p.region, edep * p.wt);
+}
+
+// ------------------------------------------------------------------ photo-electric
+template <class Sink>
+__device__ void photo(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+    const gegs_region rd = env.reg[p.region];
+    float edep = 0.0f;
+    p.e = (float)((double)p.e + PRM);
+    p.charge = -1;
+    if (p.e > rd.ecut) {
+        if (rd.flags & GEGS_F_PHOTOELECTRON_ANGLE) {
+            // Sauter distribution (photo.cu:197-244)
+            const float beta = sqrtf((p.e - RM) * (p.e + RM)) / p.e;
+            const float gamma = p.e / RM;
+            const float alpha = 0.5f * gamma - 0.5f + 1.0f / gamma;
+            const float ratio = beta / alpha;
+            float xi, r2, sinth2, costhe;
+            do {
+                float rn = s.next();
+                rn = 2.0f * rn - 1.0f;
+                if (ratio <= 0.2f) {
+                    const float fk = rn + 0.5f * ratio * (1.0f - rn) * (1.0f + rn);
+                    if (gamma < 100.0f) {
+                        costhe = (beta + fk) / (1.0f + beta * fk);
+                    } else if (fk > 0.0f) {
+                        costhe = 1.0f - (1.0f - fk) * (gamma - 3.0f) /
+                                 (2.0f * (1.0f + fk) * (gamma - 1.0f) * (gamma - 1.0f) * (gamma - 1.0f));
+                    } else {
+                        costhe = (beta + fk) / (1.0f + beta * fk);
+                    }
+                    xi = (1.0f + beta * fk) * gamma * gamma;
+                } else {
+                    xi = gamma * gamma * (1.0f + alpha * (sqrtf(1.0f + ratio * (2.0f * rn + ratio)) - 1.0f));
+                    costhe = (1.0f - 1.0f / xi) / beta;
+                }
+                sinth2 = fmaxf(0.0f, (1.0f - costhe) * (1.0f + costhe));
+                r2 = s.next();
+            } while (r2 > 0.5f * (1.0f + gamma) * sinth2 * xi / gamma);
+            float sinphi, cosphi;
+            uphi21(s, costhe, sqrtf(sinth2), cosphi, sinphi, p);
+        }
+        create(sink, s, p, E_ELECTRON_STEP);
+    } else {
+        edep += to_kinetic(p.e);
+    }
+    t.score(p.region, edep * p.wt);
+}
+
+// ------------------------------------------------------------------ Rayleigh
+template <class Sink>
+__device__ void rayleigh(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+    const gegs_media_tables *mt = env.mt;
+    const gegs_region rd = env.reg[p.region];
+    const float gle = glog(p.e);
+    const gegs_float2 ge = mt->ge[rd.med];
+    const int lgle = (int)(ge.x + ge.y * gle);
+    const gegs_float2 pm = mt->pmax[rd.med * MXGE + lgle];
+    const float pmax_val = pm.x + pm.y * gle;
+    const float xmax = HC_INVERSE * p.e;
+    const uint32_t *iarr = mt->i_array + rd.med * RAYCDFSIZE;
+    const gegs_rayleigh *rdat = mt->rayleigh_data + rd.med * MXRAYFF;
+    float costhe, costhe2;
+    for (;;) {
+        float xv;
+        for (;;) {
+            const float r1 = s.next();
+            float temp = r1 * pmax_val;
+            const int ibin = (int)(temp * (float)(RAYCDFSIZE - 1));
+            int ib = (int)iarr[ibin] - 1;
+            const int next_ib = (int)iarr[ibin + 1] - 1;
+            if (next_ib > ib) {
+                while (!(temp < rdat[ib + 1].fcum || ib >= RAYCDFSIZE - 2)) ib++;
+            }
+            const gegs_rayleigh rb = rdat[ib];
+            temp = (temp - rb.fcum) * rb.c_array;
+            xv = rb.xgrid * gexp(glog(1.0f + temp) * rb.b_array);
+            if (xv < xmax) break;
+        }
+        const float r2 = s.next();
+        xv = xv / p.e;
+        costhe = 1.0f - TWICE_HC2 * xv * xv;
+        costhe2 = costhe * costhe;
+        if (2.0f * r2 < 1.0f + costhe2) break;
+    }
+    float cosphi, sinphi;
+    uphi21(s, costhe, sqrtf(1.0f - costhe2), cosphi, sinphi, p);
+    if (p.e < rd.pcut) t.score(p.region, p.e * p.wt); else create(sink, s, p, P_PHOTON_STEP);
+}
+
+// ------------------------------------------------------------------ pair production
+template <class Sink>
+__device__ void pair(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+    const gegs_media_tables *mt = env.mt;
+    const gegs_region rd = env.reg[p.region];
+    const int m8 = 8 * (int)rd.med;
+    float edep = 0.0f;
+    Part p1 = p;
+    const float eig = p.e;
+    float ese1, ese2;
+    int q1, q2;
+    if (eig <= 2.1f) {
+        float r = s.next();
+        ese2 = (float)(PRM + (double)(0.5f * r * (eig - 2.0f * RM)));
+        r = s.next();
+        ese1 = eig - ese2;
+        q1 = r < 0.5f ? -1 : 1;
+    } else {
+        float amax, bmax, aux1;
+        int l;
+        if (eig < 50.0f) {
+            l = 5;
+            const float delta = 4.0f * mt->delcm[rd.med] / eig;
+            if (delta < 1.0f) {
+                amax = mt->dl1[m8 + l - 1] + delta * (mt->dl2[m8 + l - 1] + delta * mt->dl3[m8 + l - 1]);
+                bmax = mt->dl1[m8 + l] + delta * (mt->dl2[m8 + l] + delta * mt->dl3[m8 + l]);
+            } else {
+                const float a2 = glog(delta + mt->dl6[m8 + l - 1]);
+                amax = mt->dl4[m8 + l - 1] + mt->dl5[m8 + l - 1] * a2;
+                bmax = mt->dl4[m8 + l] + mt->dl5[m8 + l] * a2;
+            }
+            aux1 = 1.0f - RMT2 / eig;
+            aux1 = aux1 * aux1;
+            aux1 = aux1 * amax / 3.0f;
+            aux1 = aux1 / (bmax + aux1);
+        } else {
+            l = 7;
+            amax = mt->dl1[m8 + l - 1];
+            bmax = mt->dl1[m8 + l];
+            aux1 = mt->bpar[2 * rd.med + 1] * (1.0f - mt->bpar[8 * rd.med] * RM / eig);
+        }
+        const float del0 = eig * mt->delcm[rd.med];
+        const float eavail = eig - RMT2;
+        float r30, rejmax, rejf, eminus;
+        do {
+            r30 = s.next();
+            float r31 = s.next();
+            float br;
+            int l1;
+            if (r30 > aux1) {
+                br = 0.5f * r31;
+                rejmax = bmax;
+                l1 = l + 1;
+            } else {
+                r30 = s.next();
+                r30 = fmaxf(r30, r31);
+                r31 = s.next();
+                br = 0.5f * (1.0f - fmaxf(r30, r31));
+                rejmax = amax;
+                l1 = l;
+            }
+            eminus = br * eavail + RM;
+            const float delta = del0 / (eminus * (eig - eminus));
+            if (delta < 1.0f)
+                rejf = mt->dl1[m8 + l1 - 1] + delta * (mt->dl2[m8 + l1 - 1] + delta * mt->dl3[m8 + l1 - 1]);
+            else
+                rejf = mt->dl4[m8 + l1 - 1] + mt->dl5[m8 + l1 - 1] * glog(delta + mt->dl6[m8 + l1 - 1]);
+            r30 = s.next();
+        } while (r30 * rejmax > rejf);
+        ese2 = eminus;
+        ese1 = eig - ese2;
+        q1 = s.next() < 0.5f ? -1 : 1;
+    }
+    q2 = -q1;
+    p.e = ese1;
+    p1.e = ese2;
+    p.charge = q1;
+    p1.charge = q2;
+    float cosphi = 0.0f, sinphi = 0.0f;
+    // iprdst = 1: leading-order angular distribution for each lepton
+    for (int k = 1; k <= 2; ++k) {
+        const float ese = k == 1 ? ese1 : ese2;
+        const float pse = sqrtf(fmaxf(0.0f, (ese - RM) * (ese + RM)));
+        float costhe = s.next();
+        costhe = 1.0f - 2.0f * costhe;
+        float sinthe = RM * sqrtf((1.0f - costhe) * (1.0f + costhe)) / (pse * costhe + ese);
+        costhe = (ese * costhe + pse) / (pse * costhe + ese);
+        Part &q = k == 1 ? p : p1;
+        if (k == 1) {
+            uphi21(s, costhe, sinthe, cosphi, sinphi, q);
+        } else {
+            turn(costhe, -sinthe, cosphi, sinphi, q);
+        }
+        if (q.e < rd.ecut) {
+            edep += to_kinetic(q.e);
+            if (q.charge > 0) annih_at_rest(env, s, q, rd, sink, t);
+        } else {
+            create(sink, s, q, E_ELECTRON_STEP);
+        }
+    }
+    t.score(p.region, edep * p.wt);
+}
+
+// ------------------------------------------------------------------ multiple scattering
+__device__ inline void mscat(const gegs_media_tables *mt, Stream &s, float lambda, float chia2, float q1,
+                             float &cost, float &sint) {
+    if (lambda <= 13.8f) {
+        const float sprob = s.next();
+        const float explambda = gexp(-lambda);
+        if (sprob < explambda) { cost = 1.0f; sint = 0.0f; return; }
+        float wsum = (1.0f + lambda) * explambda;
+        if (sprob < wsum) {
+            float xi = s.next();
+            xi = 2.0f * chia2 * xi / (1.0f - xi + chia2);
+            cost = 1.0f - xi;
+            sint = sqrtf(xi * (2.0f - xi));
+            return;
+        }
+        if (lambda <= 1.0f) {
+            float wprob = explambda;
+            wsum = explambda;
+            cost = 1.0f;
+            sint = 0.0f;
+            int icount = 0;
+            do {
+                if (++icount > 20) break;
+                wprob = wprob * lambda / (float)icount;
+                wsum = wsum + wprob;
+                float xi = s.next();
+                xi = 2.0f * chia2 * xi / (1.0f - xi + chia2);
+                const float cosz = 1.0f - xi;
+                float sinz = xi * (2.0f - xi);
+                if (sinz > 1.0e-20f) {
+                    sinz = sqrtf(sinz);
+                    xi = s.next();
+                    float sphi, cphi;
+                    gsincos2pi(xi, sphi, cphi);
+                    cost = cost * cosz - sint * sinz * cphi;
+                    sint = sqrtf(fmaxf(0.0f, (1.0f - cost) * (1.0f + cost)));
+                }
+            } while (wsum <= sprob);
+            return;
+        }
+    }
+    if (lambda <= 1e5f) {
+        const float ll = glog(lambda);
+        float ai = ll * mt->dllambi;
+        int msi = (int)ai;
+        ai = ai - (float)msi;
+        float xi = s.next();
+        if (xi < ai) msi = msi + 1;
+        int msj;
+        if (q1 < 1e-3f) {
+            msj = 0;
+        } else if (q1 < 0.5f) {
+            float aj = q1 * mt->dqmsi;
+            msj = (int)aj;
+            aj = aj - (float)msj;
+            xi = s.next();
+            if (xi < aj) msj = msj + 1;
+        } else {
+            msj = MAXQ_MS;
+        }
+        float omega2;
+        if (ll < 2.2299f)
+            omega2 = chia2 * (lambda + 4.0f) * (1.347006f + ll * (0.209364f - ll * (0.45525f - ll * (0.50142f - 0.081234f * ll))));
+        else
+            omega2 = chia2 * (lambda + 4.0f) * (-2.77164f + ll * (2.94874f - ll * (0.1535754f - ll * 0.00552888f)));
+        xi = s.next();
+        float ak = xi * (float)MAXU_MS;
+        int msk = (int)ak;
+        ak = ak - (float)msk;
+        const int stride_k = (MAXQ_MS + 1) * (MAXL_MS + 1);
+        const int off = msj * (MAXL_MS + 1) + msi;
+        if (ak > mt->wms[msk * stride_k + off]) msk = mt->ims[msk * stride_k + off];
+        const float a = mt->fms[msk * stride_k + off];
+        float u = mt->ums[msk * stride_k + off];
+        const float du = mt->ums[(msk + 1) * stride_k + off] - u;
+        xi = s.next();
+        if (fabsf(a) < 0.2f) {
+            const float x1 = 0.5f * (1.0f - xi) * a;
+            u = u + xi * du * (1.0f + x1 * (1.0f - xi * a));
+        } else {
+            u = u - du / a * (1.0f - sqrtf(1.0f + xi * a * (2.0f + a)));
+        }
+        xi = omega2 * u / (1.0f + 0.5f * omega2 - u);
+        if (xi > 1.99999f) xi = 1.99999f;
+        cost = 1.0f - xi;
+        sint = sqrtf(xi * (2.0f - xi));
+        return;
+    }
+    cost = 1.0f;
+    sint = 0.0f;
+}
+
+__device__ inline float msdist_pI(const gegs_media_tables *mt, Stream &s, float eloss, float tustep, Part &p,
+                                  const gegs_region &rd) {
+    const int m = rd.med;
+    const float e0 = p.e - RM;
+    const float blc = mt->blcc[m];
+    const float xcc = mt->xcc[m];
+    const float e = e0 - 0.5f * eloss;
+    const float p2 = e * (e + RMT2);
+    const float beta2 = p2 / (p2 + RMSQ);
+    float lambda = tustep * rd.rhof * blc / beta2;
+    const float factor = 1.0f / (1.0f + 0.9784671f * e);
+    float eps = eloss / e0;
+    eps = eps / (1.0f - 0.5f * eps);
+    const float temp = 0.25f * (1.0f - factor * (1.0f - 0.333333f * factor)) * eps * eps;
+    lambda = lambda * (1.0f + temp);
+    const float chia2 = xcc / p2 / (4.0f * blc);
+    lambda = lambda / (1.0f + chia2);
+    const float chilog = glog(1.0f + 1.0f / chia2);
+    const float q1 = 2.0f * chia2 * (chilog * (1.0f + chia2) - 1.0f);
+    const float xi = q1 * lambda;
+    float ws, sint;
+    mscat(mt, s, lambda, chia2, xi, ws, sint);
+    float cphi, sphi;
+    azimuth_box(s, cphi, sphi);
+    float us = sint * cphi, vs = sint * sphi;
+    float z;
+    if (xi < 0.1f) z = 1.0f - xi * (0.5f - xi * (0.166666667f - 0.041666667f * xi));
+    else z = (1.0f - gexp(-xi)) / xi;
+    float r = 0.5f * sint;
+    float r2 = r * r;
+    const float z2 = z * z;
+    const float r2max = 1.0f - z2;
+    if (r2max < r2) {
+        r2 = r2max;
+        r = sqrtf(r2);
+    }
+    float ut = r * cphi, vt = r * sphi, wt = z;
+    const float ustep = sqrtf(z2 + r2) * tustep;
+    const float s02 = p.u * p.u + p.v * p.v;
+    if (s02 > 1e-20f) {
+        const float s0 = sqrtf(s02);
+        const float cp0 = p.u / s0, sp0 = p.v / s0;
+        float u2p = p.w * us + s0 * ws;
+        ws = p.w * ws - s0 * us;
+        us = u2p * cp0 - vs * sp0;
+        vs = u2p * sp0 + vs * cp0;
+        u2p = p.w * ut + s0 * wt;
+        wt = p.w * wt - s0 * ut;
+        ut = u2p * cp0 - vt * sp0;
+        vt = u2p * sp0 + vt * cp0;
+    } else {
+        wt = p.w * wt;
+        ws = p.w * ws;
+    }
+    p.x = p.x + tustep * ut;
+    p.y = p.y + tustep * vt;
+    p.z = p.z + tustep * wt;
+    p.u = us;
+    p.v = vs;
+    p.w = ws;
+    return ustep;
+}
+
+// energy lost over a path (the reference's $COMPUTE-ELOSS-G)
+__device__ inline float eloss_over(const gegs_media_tables *mt, const gegs_float2 *dedx, int qel, int m,
+                                   float range, float rhof, float step, float eke, float elke, int lelke) {
+    const float *rep = mt->range_ep;
+    float tuss = range - rep[(m * MXEKE + lelke - 1) * 2 + qel] / rhof;
+    float de;
+    if (tuss >= step) {
+        const gegs_float2 dd = dedx[lelke - 1];
+        const float dmid = dd.x + dd.y * elke;
+        const float aux = dd.y / dmid;
+        de = dmid * step * rhof;
+        const float fe = de / eke;
+        de = de * (1.0f - 0.5f * fe * aux * (1.0f - 0.333333f * fe * (aux - 1.0f - 0.25f * fe * (2.0f - aux * (4.0f - aux)))));
+        return de;
+    }
+    tuss = (range - step) * rhof;
+    if (tuss <= 0.0f) return eke - mt->te[m] * 0.99f;
+    int l = lelke;
+    while (tuss < rep[(m * MXEKE + l - 1) * 2 + qel]) l = l - 1;
+    const gegs_float2 ek = mt->eke01[m];
+    const float elk = ((float)(l + 1) - ek.x) / ek.y;
+    const float ekt = mt->e_array[m * MXEKE + l];
+    tuss = (rep[(m * MXEKE + l) * 2 + qel] - tuss) / rhof;
+    const gegs_float2 dd = dedx[l - 1];
+    const float dmid = dd.x + dd.y * elk;
+    const float aux = dd.y / dmid;
+    de = dmid * tuss * rhof;
+    const float fe = de / ekt;
+    de = de * (1.0f - 0.5f * fe * aux * (1.0f - 0.333333f * fe * (aux - 1.0f - 0.25f * fe * (2.0f - aux * (4.0f - aux)))));
+    return de + eke - ekt;
+}
+
+// path length from eke down to e1 within one interpolation bin (the
+// reference's $COMPUTE-DRANGE)
+__device__ inline float path_in_bin(const gegs_float2 *dedx, float eke, float elke, float e1, float elk1, int lbin) {
+    const float fe = 1.0f - e1 / eke;
+    const float elmid = 0.5f * (elke + elk1 + 0.25f * fe * fe * (1.0f + fe * (1.0f + 0.875f * fe)));
+    const gegs_float2 dd = dedx[lbin - 1];
+    float dmid = dd.x + dd.y * elmid;
+    dmid = 1.0f / dmid;
+    float aux = dd.y * dmid;
+    const float tq = fe / (2.0f - fe);
+    aux = aux * (1.0f + 2.0f * aux) * tq * tq / 6.0f;
+    return fe * eke * dmid * (1.0f + aux);
+}
+
+// ------------------------------------------------------------------ condensed-history electron step
+template <class Sink>
+__device__ void electron_step(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+    const gegs_media_tables *mt = env.mt;
+    float elke = 0.0f, demfp = 0.0f, sig0 = 0.0f, total_tstep = 0.0f;
+    int lelke = 0;
+    bool compute_tstep = true;
+    bool is_tstep = true;
+    for (;;) {
+        uint32_t irl = p.region;
+        gegs_region rd = env.reg[irl];
+        int medium = rd.med;
+        const float rhof = rd.rhof;
+        const int lelec = p.charge;
+        const int qel = (1 + lelec) / 2;
+        float eie = p.e;
+        t.electron_steps++;
+        t.steps++;
+        if (eie <= rd.ecut) {
+            t.score(p.region, to_kinetic(p.e) * p.wt);
+            if (p.charge > 0) annih_at_rest(env, s, p, rd, sink, t);
+            return;
+        }
+        if (p.wt <= 0.0f) return;
+        float eke = eie - RM;
+        const int mq = medium == (int)VACUUM ? 0 : medium;
+        const gegs_float2 *dedx = mt->dedx + qel * MXEKE * MXMED + mq * MXEKE;
+        if (is_tstep) {
+            compute_tstep = true;
+            if (medium != (int)VACUUM) {
+                float rn = s.next();
+                if (rn == 0.0f) rn = 1.0e-30f;
+                demfp = fmaxf(-glog(rn), (float)EPSEMFP);
+                elke = glog(eke);
+                const gegs_float2 ek = mt->eke01[medium];
+                lelke = (int)(ek.x + ek.y * elke);
+                if (mt->sig_ismonotone[medium * 2 + qel]) {
+                    const gegs_float2 sd = mt->sig[qel * MXEKE * MXMED + medium * MXEKE + lelke - 1];
+                    const gegs_float2 dd = dedx[lelke - 1];
+                    float sigf = sd.x + sd.y * elke;
+                    sigf /= dd.x + dd.y * elke;
+                    sig0 = sigf;
+                } else {
+                    sig0 = mt->sig_e[qel * MXMED + medium];
+                }
+            }
+        }
+        float sig_val = 0.0f, dedx_val = 0.0f, tstep = 0.0f, ustep = 0.0f, tustep = 0.0f;
+        float tvstep = 0.0f, vstep = 0.0f, range = 0.0f, de = 0.0f;
+        float blccl = 0.0f, xccl = 0.0f, beta2 = 0.0f, chia2 = 0.0f, xi = 0.0f, tperp = 0.0f;
+        bool callhowfar = false, domultiple = false, callmsdist = false;
+        if (medium == (int)VACUUM) {
+            tstep = VACUUM_STEP;
+            ustep = tstep;
+            tustep = ustep;
+            callhowfar = true;
+        } else {
+            const gegs_float2 ek = mt->eke01[medium];
+            sig_val = sig0;
+            if (sig_val <= 0.0f) {
+                tstep = VACUUM_STEP;
+                sig0 = 1.0e-15f;
+            } else {
+                if (compute_tstep) {
+                    const float ekef = eke - demfp / sig_val;
+                    if (ekef <= mt->e_array[medium * MXEKE]) {
+                        tstep = VACUUM_STEP;
+                    } else {
+                        const float elkef = glog(ekef);
+                        const int lelkef = (int)(ek.x + ek.y * elkef);
+                        if (lelkef == lelke) {
+                            tstep = path_in_bin(dedx, eke, elke, ekef, elkef, lelke);
+                        } else {
+                            const float ekei = mt->e_array[medium * MXEKE + lelke - 1];
+                            const float elkei = ((float)lelke - ek.x) / ek.y;
+                            const float tuss = path_in_bin(dedx, eke, elke, ekei, elkei, lelke);
+                            const float ekej = mt->e_array[medium * MXEKE + lelkef];
+                            const float elkej = ((float)(lelkef + 1) - ek.x) / ek.y;
+                            tstep = path_in_bin(dedx, ekej, elkej, ekef, elkef, lelkef);
+                            tstep = tstep + tuss + mt->range_ep[(medium * MXEKE + lelke - 1) * 2 + qel] -
+                                    mt->range_ep[(medium * MXEKE + lelkef) * 2 + qel];
+                        }
+                    }
+                    total_tstep = tstep;
+                    compute_tstep = false;
+                }
+                tstep = total_tstep / rhof;
+            }
+            const gegs_float2 dd = dedx[lelke - 1];
+            dedx_val = rhof * (dd.x + dd.y * elke);
+            const gegs_float2 tm = mt->tmxs[medium * MXEKE + lelke - 1];
+            float tmxs_val = tm.x + tm.y * elke;
+            tmxs_val /= rhof;
+            const float ekei = mt->e_array[medium * MXEKE + lelke - 1];
+            const float elkei = ((float)lelke - ek.x) / ek.y;
+            range = path_in_bin(dedx, eke, elke, ekei, elkei, lelke);
+            range = (range + mt->range_ep[(medium * MXEKE + lelke - 1) * 2 + qel]) / rhof;
+            tmxs_val = fminf(tmxs_val, mt->smaxir[medium]);
+            tustep = fminf(tstep, fminf(tmxs_val, range));
+            tperp = hownear_phantom(env, p);
+            p.dnear = tperp;
+            if (mt->i_do_rr[medium] == 1 && p.e < mt->e_max_rr[medium] && tperp >= range) {
+                t.score(p.region, to_kinetic(p.e) * p.wt);
+                if (lelec > 0) annih_at_rest(env, s, p, rd, sink, t);
+                return;
+            }
+            blccl = rhof * mt->blcc[medium];
+            xccl = rhof * mt->xcc[medium];
+            const float p2 = eke * (eke + RMT2);
+            beta2 = p2 / (p2 + RMSQ);
+            const float ssmfp = beta2 / blccl;
+            const float skindepth = mt->skindepth_for_bca * ssmfp;
+            tustep = fminf(tustep, fmaxf(tperp, skindepth));
+            if (tustep <= tperp && (!mt->exact_bca || tustep > skindepth)) {
+                callmsdist = true;
+                de = eloss_over(mt, dedx, qel, medium, range, rhof, tustep, eke, elke, lelke);
+                tvstep = tustep;
+                ustep = msdist_pI(mt, s, de, tustep, p, rd);
+            } else {
+                // inexact boundary-crossing algorithm
+                domultiple = true;
+                const float ekems = eke - 0.5f * tustep * dedx_val;
+                const float p2m = ekems * (ekems + RMT2);
+                beta2 = p2m / (p2m + RMSQ);
+                chia2 = xccl / (4.0f * blccl * p2m);
+                xi = 0.5f * xccl / p2m / beta2 * tustep;
+                xi = xi * (glog(1.0f + 1.0f / chia2) - 1.0f / (1.0f + chia2));
+                if (xi < 0.1f) ustep = tustep * (1.0f - xi * (0.5f - xi * 0.166667f));
+                else ustep = tustep * (1.0f - gexp(-xi)) / xi;
+                callhowfar = !(ustep < tperp);
+            }
+        }
+        uint32_t irnew = p.region;
+        int idisc = 0;
+        const float ustep0 = ustep;
+        if (callhowfar || p.wt <= 0.0f) irnew = howfar_phantom(env, p, ustep);
+        if (irnew == 0) idisc = rd.med == VACUUM ? 1 : -1;
+        if (idisc > 0) {
+            p.region = 0;
+            return;
+        }
+        if (ustep <= 0.0f) ustep = 0.0f;
+        if (ustep == 0.0f || medium == (int)VACUUM) {
+            if (ustep != 0.0f) {
+                vstep = ustep;
+                p.x = p.x + p.u * vstep;
+                p.y = p.y + p.v * vstep;
+                p.z = p.z + p.w * vstep;
+                p.dnear = p.dnear - vstep;
+            }
+            if (p.region != irnew) t.gathers++;
+            p.region = irnew;
+            rd = env.reg[irnew];
+            if (eie <= rd.ecut) {
+                t.score(p.region, to_kinetic(p.e) * p.wt);
+                if (p.charge > 0) annih_at_rest(env, s, p, rd, sink, t);
+                return;
+            }
+            if (ustep != 0.0f && idisc < 0) {
+                t.score(p.region, (float)((double)p.e + (double)p.charge * PRM) * p.wt);
+                return;
+            }
+            is_tstep = true;
+            continue;
+        }
+        vstep = ustep;
+        if (callhowfar) {
+            if (vstep < ustep0) {
+                const float ekems = eke - 0.5f * tustep * vstep / ustep0 * dedx_val;
+                const float p2m = ekems * (ekems + RMT2);
+                beta2 = p2m / (p2m + RMSQ);
+                chia2 = xccl / (4.0f * blccl * p2m);
+                xi = 0.5f * xccl / p2m / beta2 * vstep;
+                xi = xi * (glog(1.0f + 1.0f / chia2) - 1.0f / (1.0f + chia2));
+                if (xi < 0.1f) {
+                    tvstep = vstep * (1.0f + xi * (0.5f + xi * 0.333333f));
+                } else if (xi < 0.999999f) {
+                    tvstep = -vstep * glog(1.0f - xi) / xi;
+                } else {
+                    return;
+                }
+            } else {
+                tvstep = tustep;
+            }
+            de = eloss_over(mt, dedx, qel, medium, range, rhof, tvstep, eke, elke, lelke);
+        } else {
+            tvstep = tustep;
+            if (!callmsdist) de = eloss_over(mt, dedx, qel, medium, range, rhof, tvstep, eke, elke, lelke);
+        }
+        const float save_de = de;
+        if (!callmsdist) {
+            float sinthe = 0.0f, costhe = 1.0f;
+            if (domultiple) {
+                const float lambda = blccl * tvstep / beta2 / (1.0f + chia2);
+                mscat(mt, s, lambda, chia2, xi, costhe, sinthe);
+            }
+            p.x = p.x + p.u * vstep;
+            p.y = p.y + p.v * vstep;
+            p.z = p.z + p.w * vstep;
+            if (domultiple) {
+                float sinphi, cosphi;
+                uphi21(s, costhe, sinthe, cosphi, sinphi, p);
+            }
+        }
+        p.dnear = p.dnear - vstep;
+        const uint32_t irold = p.region;
+        eie = eie - de;
+        p.e = eie;
+        t.score(irold, de * p.wt);
+        if (irnew == irl && eie <= rd.ecut) {
+            t.score(p.region, to_kinetic(p.e) * p.wt);
+            if (p.charge > 0) annih_at_rest(env, s, p, rd, sink, t);
+            return;
+        }
+        const int medold = medium;
+        if (medium != (int)VACUUM) {
+            eke = eie - RM;
+            elke = glog(eke);
+            const gegs_float2 ek = mt->eke01[medium];
+            lelke = (int)(ek.x + ek.y * elke);
+        }
+        if (irnew != irold) {
+            p.region = irnew;
+            rd = env.reg[irnew];
+            medium = rd.med;
+            t.gathers++;
+        }
+        if (eie <= rd.ecut) {
+            t.score(p.region, to_kinetic(p.e) * p.wt);
+            if (p.charge > 0) annih_at_rest(env, s, p, rd, sink, t);
+            return;
+        }
+        if (idisc < 0) {
+            t.score(p.region, (float)((double)p.e + (double)p.charge * PRM) * p.wt);
+            return;
+        }
+        if (medium != medold) {
+            is_tstep = true;
+            continue;
+        }
+        demfp = demfp - save_de * sig_val;
+        total_tstep = total_tstep - tvstep * rhof;
+        if (total_tstep < 1e-9f) demfp = 0.0f;
+        if ((double)demfp >= EPSEMFP) {
+            is_tstep = false;
+            continue;
+        }
+        const gegs_float2 sd = mt->sig[qel * MXEKE * MXMED + medium * MXEKE + lelke - 1];
+        const gegs_float2 dd = mt->dedx[qel * MXEKE * MXMED + medium * MXEKE + lelke - 1];
+        float sigf = sd.x + sd.y * elke;
+        sigf /= dd.x + dd.y * elke;
+        const float sigratio = sigf / sig0;
+        if (s.next() > sigratio) {
+            is_tstep = true;
+            continue;
+        }
+        if (lelec < 0) {
+            const gegs_float2 b1 = mt->ebr1[medium * MXEKE + lelke - 1];
+            const float ebr1 = b1.x + b1.y * elke;
+            if (s.next() <= ebr1) { forward(sink, s, p, E_BREMS); return; }
+            if (p.e <= mt->thmoll[medium] && mt->eii_flag == 0) {
+                if (ebr1 <= 0.0f) { is_tstep = true; continue; }
+                forward(sink, s, p, E_BREMS);
+                return;
+            }
+            forward(sink, s, p, E_MOLLER);
+            return;
+        }
+        const gegs_float2 pb1 = mt->pbr1[medium * MXEKE + lelke - 1];
+        const float pbr1 = pb1.x + pb1.y * elke;
+        const float r25 = s.next();
+        if (r25 < pbr1) { forward(sink, s, p, E_BREMS); return; }
+        const gegs_float2 pb2 = mt->pbr2[medium * MXEKE + lelke - 1];
+        const float pbr2 = pb2.x + pb2.y * elke;
+        forward(sink, s, p, r25 < pbr2 ? E_BHABHA : E_ANNIH);
+        return;
+    }
+}
+
+// ------------------------------------------------------------------ Moller
+template <class Sink>
+__device__ void moller(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+    const gegs_media_tables *mt = env.mt;
+    const gegs_region rd = env.reg[p.region];
+    const int m = rd.med;
+    float edep = 0.0f;
+    Part p1 = p;
+    const float eie = p.e;
+    const float ekin = to_kinetic(eie);
+    const float te = mt->te[m];
+    if (ekin <= 2.0f * te) {
+        t.score(p.region, to_kinetic(p.e) * p.wt);
+        return;
+    }
+    const float t0 = ekin / RM;
+    const float e0 = t0 + 1.0f;
+    const float extrae = eie - mt->thmoll[m];
+    const float e02 = e0 * e0;
+    const float g2 = t0 * t0 / e02;
+    const float g3 = (2.0f * t0 + 1.0f) / e02;
+    const float gmax = 1.0f + 1.25f * g2;
+    float r, rejf4, br, rr;
+    do {
+        r = s.next();
+        br = te / (ekin - extrae * r);
+        rr = br / (1.0f - br);
+        r = s.next();
+        rejf4 = 1.0f + g2 * br * br + rr * (rr - g3);
+        r = gmax * r;
+    } while (r > rejf4);
+    const float ekse2 = br * ekin;
+    const float ese1 = eie - ekse2;
+    const float ese2 = (float)((double)ekse2 + PRM);
+    p.e = ese1;
+    p1.e = ese2;
+    const float h1 = (float)(((double)eie + PRM) / (double)ekin);
+    float dcosth = (float)((double)h1 * ((double)ese1 - PRM) / ((double)ese1 + PRM));
+    float sinthe = sqrtf(1.0f - dcosth);
+    float costhe = sqrtf(dcosth);
+    float sinphi, cosphi;
+    uphi21(s, costhe, sinthe, cosphi, sinphi, p);
+    if (p.e < rd.ecut) edep += to_kinetic(p.e); else create(sink, s, p, E_ELECTRON_STEP);
+    dcosth = (float)((double)h1 * ((double)ese2 - PRM) / ((double)ese2 + PRM));
+    sinthe = -sqrtf(1.0f - dcosth);
+    costhe = sqrtf(dcosth);
+    turn(costhe, sinthe, cosphi, sinphi, p1);
+    if (p1.e < rd.ecut) edep += to_kinetic(p1.e); else create(sink, s, p1, E_ELECTRON_STEP);
+    t.score(p.region, edep * p.wt);
+}
+
+// ------------------------------------------------------------------ Bhabha
+template <class Sink>
+__device__ void bhabha(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+    const gegs_media_tables *mt = env.mt;
+    const gegs_region rd = env.reg[p.region];
+    float edep = 0.0f;
+    Part p1 = p;
+    const float eip = p.e;
+    const float ekin = to_kinetic(eip);
+    const float t0 = ekin / RM;
+    const float e0 = t0 + 1.0f;
+    const float yy = 1.0f / (t0 + 2.0f);
+    const float e02 = e0 * e0;
+    const float beta2 = (e02 - 1.0f) / e02;
+    const float ep0 = mt->te[rd.med] / ekin;
+    const float ep0c = 1.0f - ep0;
+    const float y2 = yy * yy;
+    const float yp = 1.0f - 2.0f * yy;
+    const float yp2 = yp * yp;
+    const float b4 = yp2 * yp;
+    const float b3 = b4 + yp2;
+    const float b2 = yp * (3.0f + y2);
+    const float b1 = 2.0f - y2;
+    float r, rejf2, br;
+    do {
+        r = s.next();
+        br = ep0 / (1.0f - ep0c * r);
+        r = s.next();
+        rejf2 = 1.0f - beta2 * br * (b1 - br * (b2 - br * (b3 - br * b4)));
+    } while (r > rejf2);
+    if (br < 0.5f) {
+        p1.charge = -1;
+    } else {
+        p.charge = -1;
+        p1.charge = 1;
+        br = 1.0f - br;
+    }
+    br = fmaxf(br, 0.0f);
+    const float ekse2 = br * ekin;
+    const float ese1 = eip - ekse2;
+    const float ese2 = (float)((double)ekse2 + PRM);
+    p.e = ese1;
+    p1.e = ese2;
+    const float h1 = (float)(((double)eip + PRM) / (double)ekin);
+    float dcosth = (float)fmin(1.0, (double)h1 * ((double)ese1 - PRM) / ((double)ese1 + PRM));
+    float sinthe = sqrtf(1.0f - dcosth);
+    float costhe = sqrtf(dcosth);
+    float sinphi, cosphi;
+    uphi21(s, costhe, sinthe, cosphi, sinphi, p);
+    if (p.e < rd.ecut) {
+        edep += to_kinetic(p.e);
+        if (p.charge > 0) annih_at_rest(env, s, p, rd, sink, t);
+    } else {
+        create(sink, s, p, E_ELECTRON_STEP);
+    }
+    dcosth = (float)((double)h1 * ((double)ese2 - PRM) / ((double)ese2 + PRM));
+    sinthe = -sqrtf(1.0f - dcosth);
+    costhe = sqrtf(dcosth);
+    turn(costhe, sinthe, cosphi, sinphi, p1);
+    if (p1.e < rd.ecut) {
+        edep += to_kinetic(p1.e);
+        if (p1.charge > 0) annih_at_rest(env, s, p1, rd, sink, t);
+    } else {
+        create(sink, s, p1, E_ELECTRON_STEP);
+    }
+    t.score(p.region, edep * p.wt);
+}
+
+// ------------------------------------------------------------------ bremsstrahlung (Bethe-Heitler + Koch-Motz 2BS)
+template <class Sink>
+__device__ void brems(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+    const gegs_media_tables *mt = env.mt;
+    const gegs_region rd = env.reg[p.region];
+    const int m = rd.med;
+    const int m8 = 8 * m;
+    const int nsplit = mt->nbr_split;
+    float edep = 0.0f;
+    Part g = p;
+    if (nsplit < 1) return;
+    const float eie = p.e;
+    const int l = eie < 50.0f ? 1 : 3;
+    const float ekin = to_kinetic(eie);
+    const float elke = glog(ekin);
+    const float brmin = mt->ap[m] / ekin;
+    const float waux = elke - mt->log_ap[m];
+    const float a = p.u, b = p.v, cc = p.w;
+    float sinpsi = a * a + b * b, sindel = 0.0f, cosdel = 0.0f;
+    if (sinpsi > 1e-20f) {
+        sinpsi = sqrtf(sinpsi);
+        sindel = b / sinpsi;
+        cosdel = a / sinpsi;
+    }
+    const float ztarg = mt->zbrang[m];
+    const float tteie = eie / RM;
+    const float beta = sqrtf((tteie - 1.0f) * (tteie + 1.0f)) / tteie;
+    const float y2max = 2.0f * beta * (1.0f + beta) * tteie * tteie;
+    const float y2maxi = 1.0f / y2max;
+    const float z2maxi = sqrtf(y2max + 1.0f);
+    float ese = eie;
+    for (int ib = 1; ib <= nsplit; ++ib) {
+        float r, rejf, esg;
+        do {
+            r = s.next();
+            const float br = brmin * gexp(r * waux);
+            esg = ekin * br;
+            ese = eie - esg;
+            const float delta = esg / eie / ese * mt->delcm[m];
+            const float aux = ese / eie;
+            float phi1, phi2;
+            if (delta < 1.0f) {
+                phi1 = mt->dl1[m8 + l - 1] + delta * (mt->dl2[m8 + l - 1] + delta * mt->dl3[m8 + l - 1]);
+                phi2 = mt->dl1[m8 + l] + delta * (mt->dl2[m8 + l] + delta * mt->dl3[m8 + l]);
+            } else {
+                phi1 = mt->dl4[m8 + l - 1] + mt->dl5[m8 + l - 1] * glog(delta + mt->dl6[m8 + l - 1]);
+                phi2 = phi1;
+            }
+            rejf = (1.0f + aux * aux) * phi1 - 2.0f * aux * phi2 / 3.0f;
+            r = s.next();
+        } while (r >= rejf);
+        g.e = esg;
+        g.charge = 0;
+        g.wt = p.wt / (float)nsplit;
+        float y2tst;
+        {
+            const float esedei = ese / RM / tteie;
+            const float rj1 = 1.0f + esedei * esedei;
+            const float rj2 = rj1 + 2.0f * esedei;
+            float aux = 2.0f * ese * tteie / esg;
+            aux = aux * aux;
+            const float aux1 = aux * ztarg;
+            const float rj3 = aux1 > 10.0f ? mt->lzbrang[m] + (1.0f - aux1) / (aux1 * aux1) : glog(aux / (1.0f + aux1));
+            const float rejmax = rj1 * rj3 - rj2;
+            float rtest, rejtst;
+            do {
+                y2tst = s.next();
+                rtest = s.next();
+                const float aux3 = z2maxi / (y2tst + (1.0f - y2tst) * z2maxi);
+                rtest = rtest * aux3 * rejmax;
+                y2tst = aux3 * aux3 - 1.0f;
+                const float a4 = aux3 * aux3 * aux3 * aux3;
+                const float y2tst1 = esedei * y2tst / a4;
+                const float aux4 = 16.0f * y2tst1 - rj2;
+                const float aux5 = rj1 - 4.0f * y2tst1;
+                if (rtest < aux4 + aux5 * rj3) break;
+                const float aux2 = glog(aux / (1.0f + aux1 / a4));
+                rejtst = aux4 + aux5 * aux2;
+            } while (rtest >= rejtst);
+        }
+        const float costhe = 1.0f - 2.0f * y2tst * y2maxi;
+        const float sinthe = sqrtf(fmaxf((1.0f - costhe) * (1.0f + costhe), 0.0f));
+        float cphi, sphi;
+        azimuth_box(s, cphi, sphi);
+        if (sinpsi >= 1e-10f) {
+            const float us = sinthe * cphi, vs = sinthe * sphi;
+            g.u = cc * cosdel * us - sindel * vs + a * costhe;
+            g.v = cc * sindel * us + cosdel * vs + b * costhe;
+            g.w = cc * costhe - sinpsi * us;
+        } else {
+            g.u = sinthe * cphi;
+            g.v = sinthe * sphi;
+            g.w = cc * costhe;
+        }
+        if (g.e < rd.pcut) edep += g.e; else create(sink, s, g, P_PHOTON_STEP);
+    }
+    p.e = ese;
+    if (p.e < rd.ecut) edep += to_kinetic(p.e); else create(sink, s, p, E_ELECTRON_STEP);
+    t.score(p.region, edep * p.wt);
+}
+
+// ------------------------------------------------------------------ in-flight annihilation (Heitler)
+template <class Sink>
+__device__ void annih(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+    const gegs_media_tables *mt = env.mt;
+    const gegs_region rd = env.reg[p.region];
+    const int nsplit = mt->nbr_split;
+    float edep = 0.0f;
+    if (nsplit <= 0) return;
+    const float avip = (float)((double)p.e + PRM);
+    const float a = avip / RM;
+    const float g = a - 1.0f;
+    const float tt = g - 1.0f;
+    const float pp = sqrtf(a * tt);
+    const float pot = pp / tt;
+    const float ep0 = 1.0f / (a + pp);
+    const float wsamp = glog((1.0f - ep0) / ep0);
+    const float aa = p.u, bb = p.v, cc = p.w;
+    float sinpsi = aa * aa + bb * bb, sindel = 0.0f, cosdel = 0.0f;
+    if (sinpsi > 1e-20f) {
+        sinpsi = sqrtf(sinpsi);
+        sindel = bb / sinpsi;
+        cosdel = aa / sinpsi;
+    }
+    if (nsplit > 1) p.wt = p.wt / (float)nsplit;
+    for (int ib = 1; ib <= nsplit; ++ib) {
+        float r, rejf, ep;
+        do {
+            r = s.next();
+            ep = ep0 * gexp(r * wsamp);
+            r = s.next();
+            rejf = 1.0f - (ep * a - 1.0f) * (ep * a - 1.0f) / (ep * (a * a - 2.0f));
+        } while (r > rejf);
+        const float esg1 = avip * ep;
+        p.e = esg1;
+        p.charge = 0;
+        float costhe = fminf(1.0f, (esg1 - RM) * pot / esg1);
+        float sinthe = sqrtf(1.0f - costhe * costhe);
+        float cphi, sphi;
+        azimuth_box(s, cphi, sphi);
+        for (int k = 0; k < 2; ++k) {
+            if (k == 1) {
+                const float esg2 = avip - esg1;
+                p.e = esg2;
+                costhe = fminf(1.0f, (esg2 - RM) * pot / esg2);
+                sinthe = -sqrtf(1.0f - costhe * costhe);
+            }
+            if (sinpsi >= 1e-10f) {
+                const float us = sinthe * cphi, vs = sinthe * sphi;
+                p.u = cc * cosdel * us - sindel * vs + aa * costhe;
+                p.v = cc * sindel * us + cosdel * vs + bb * costhe;
+                p.w = cc * costhe - sinpsi * us;
+            } else {
+                p.u = sinthe * cphi;
+                p.v = sinthe * sphi;
+                p.w = cc * costhe;
+            }
+            if (p.e < rd.pcut) edep += p.e; else create(sink, s, p, P_PHOTON_STEP);
+        }
+    }
+    t.score(p.region, edep * p.wt);
+}
+
+// dispatch one particle to its event handler
+template <class Sink>
+__device__ inline void handle(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+    switch (p.status) {
+    case P_PHOTON_STEP: photon_step(env, s, p, sink, t); break;
+    case P_COMPTON: compton(env, s, p, sink, t); break;
+    case P_PHOTO: photo(env, s, p, sink, t); break;
+    case P_RAYLEIGH: rayleigh(env, s, p, sink, t); break;
+    case P_PAIR: pair(env, s, p, sink, t); break;
+    case E_ELECTRON_STEP: electron_step(env, s, p, sink, t); break;
+    case E_MOLLER: moller(env, s, p, sink, t); break;
+    case E_BREMS: brems(env, s, p, sink, t); break;
+    case E_BHABHA: bhabha(env, s, p, sink, t); break;
+    case E_ANNIH: annih(env, s, p, sink, t); break;
+    default: break;
+    }
+}
+
+}  // namespace gegs

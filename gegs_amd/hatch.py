@@ -1,0 +1,673 @@
+"""HATCH: build the device physics tables from PEGS4-format inputs.
+
+A restatement of the reference's host-side table builders:
+
+* ``read_pegs_file``      media.cu:332-830  (PEGS4 reader, dunit scaling, zbrang)
+* ``init_user_photon``    media.cu:832-1058 (ge/gmfp/gbr1/gbr2/cohe from element data)
+* ``kn_sigma0``           media.cu:316-330
+* ``get_data``            media.cu:260-311
+* ``init_rayleigh_data``  media.cu:1060-1292
+* ``egs_set_defaults``    PhysicsInit.cu:187-315
+* ``init_ms_sr``          PhysicsInit.cu:860-906
+* ``mscati``              PhysicsInit.cu:908-1182 (skin depth, sig_e, range_ep, tmxs)
+* ``fix_brems``           PhysicsInit.cu:1552-1641
+
+The builders run in float64 and round to float32 when stored, where the
+reference mixes float and double host arithmetic; the result is the same
+table to within float32 rounding of the inputs.  The output is a
+:class:`MediaTables` holding numpy arrays named and laid out exactly as the
+``gegs_media_tables`` block of ``include/gegs_media.h``.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+
+import numpy as np
+
+MXMED = 6
+MXGE = 200
+MXEKE = 150
+MXEL = 10
+MXRAYFF = 100
+RAYCDFSIZE = 100
+MAXL_MS, MAXQ_MS, MAXU_MS = 63, 7, 31
+MS_SIZE = (MAXU_MS + 1) * (MAXQ_MS + 1) * (MAXL_MS + 1)
+LAMBMIN_MS, LAMBMAX_MS, QMAX_MS = 1.0, 1e5, 0.5
+RM_F = float(np.float32(0.5110034))       # ELECTRON_REST_MASS_FLOAT promoted to double
+E_MAX_RR = 0.8
+MAX_SMAX = 1e10
+
+DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+
+
+class F2(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_float), ("y", ctypes.c_float)]
+
+
+class Ray(ctypes.Structure):
+    _fields_ = [("xgrid", ctypes.c_float), ("fcum", ctypes.c_float),
+                ("b_array", ctypes.c_float), ("c_array", ctypes.c_float)]
+
+
+def _arr(t, n):
+    return t * n
+
+
+class CMediaTables(ctypes.Structure):
+    """ctypes mirror of ``gegs_media_tables`` (include/gegs_media.h)."""
+    _fields_ = [
+        ("nmed", ctypes.c_int32), ("eii_flag", ctypes.c_int32), ("nbr_split", ctypes.c_int32),
+        ("ibrdst", ctypes.c_int32), ("iprdst", ctypes.c_int32), ("exact_bca", ctypes.c_int32),
+        ("spin_effects", ctypes.c_int32), ("transport_algorithm", ctypes.c_int32),
+        ("skindepth_for_bca", ctypes.c_float), ("dllambi", ctypes.c_float),
+        ("dqmsi", ctypes.c_float), ("rm", ctypes.c_float),
+        ("rho", _arr(ctypes.c_float, MXMED)), ("te", _arr(ctypes.c_float, MXMED)),
+        ("thmoll", _arr(ctypes.c_float, MXMED)), ("ae", _arr(ctypes.c_float, MXMED)),
+        ("ap", _arr(ctypes.c_float, MXMED)), ("log_ap", _arr(ctypes.c_float, MXMED)),
+        ("smaxir", _arr(ctypes.c_float, MXMED)), ("e_max_rr", _arr(ctypes.c_float, MXMED)),
+        ("zbrang", _arr(ctypes.c_float, MXMED)), ("lzbrang", _arr(ctypes.c_float, MXMED)),
+        ("delcm", _arr(ctypes.c_float, MXMED)), ("blcc", _arr(ctypes.c_float, MXMED)),
+        ("xcc", _arr(ctypes.c_float, MXMED)),
+        ("i_do_rr", _arr(ctypes.c_int32, MXMED)), ("meke", _arr(ctypes.c_int32, MXMED)),
+        ("nne", _arr(ctypes.c_int32, MXMED)),
+        ("zelem", _arr(ctypes.c_float, MXEL * MXMED)), ("pz", _arr(ctypes.c_float, MXEL * MXMED)),
+        ("dl1", _arr(ctypes.c_float, 8 * MXMED)), ("dl2", _arr(ctypes.c_float, 8 * MXMED)),
+        ("dl3", _arr(ctypes.c_float, 8 * MXMED)), ("dl4", _arr(ctypes.c_float, 8 * MXMED)),
+        ("dl5", _arr(ctypes.c_float, 8 * MXMED)), ("dl6", _arr(ctypes.c_float, 8 * MXMED)),
+        ("bpar", _arr(ctypes.c_float, 2 * MXMED)), ("sig_e", _arr(ctypes.c_float, 2 * MXMED)),
+        ("sig_ismonotone", _arr(ctypes.c_int32, 2 * MXMED)),
+        ("eke01", _arr(F2, MXMED)),
+        ("sig", _arr(F2, 2 * MXMED * MXEKE)), ("dedx", _arr(F2, 2 * MXMED * MXEKE)),
+        ("ebr1", _arr(F2, MXMED * MXEKE)), ("pbr1", _arr(F2, MXMED * MXEKE)),
+        ("pbr2", _arr(F2, MXMED * MXEKE)), ("tmxs", _arr(F2, MXMED * MXEKE)),
+        ("range_ep", _arr(ctypes.c_float, 2 * MXMED * MXEKE)),
+        ("e_array", _arr(ctypes.c_float, MXMED * MXEKE)),
+        ("ge", _arr(F2, MXMED)), ("gmfp", _arr(F2, MXMED * MXGE)), ("gbr1", _arr(F2, MXMED * MXGE)),
+        ("gbr2", _arr(F2, MXMED * MXGE)), ("cohe", _arr(F2, MXMED * MXGE)),
+        ("pmax", _arr(F2, MXMED * MXGE)),
+        ("rayleigh_data", _arr(Ray, MXMED * MXRAYFF)),
+        ("i_array", _arr(ctypes.c_uint32, MXMED * RAYCDFSIZE)),
+        ("ums", _arr(ctypes.c_float, MS_SIZE)), ("fms", _arr(ctypes.c_float, MS_SIZE)),
+        ("wms", _arr(ctypes.c_float, MS_SIZE)), ("ims", _arr(ctypes.c_int16, MS_SIZE)),
+    ]
+
+
+# ------------------------------------------------------------------ PEGS4 reader
+class PegsMedium:
+    pass
+
+
+def _kv_line(line):
+    out = {}
+    for entry in line.strip().split(","):
+        if "=" not in entry:
+            continue
+        k, v = entry.split("=", 1)
+        out[k.replace(" ", "")] = v.strip()
+    return out
+
+
+def read_pegs_file(path, names):
+    """PEGS4 reader following media.cu:332-608: media are matched by name,
+    in the order requested; a missing medium raises (reference code 10029)."""
+    with open(path) as f:
+        lines = f.read().split("\n")
+    found = {}
+    i = 0
+    while i < len(lines) and len(found) < len(names):
+        line = lines[i]
+        i += 1
+        if not line.startswith(" MEDIUM="):
+            continue
+        name = line[8:32].split(" ")[0].split(",")[0]
+        if name not in names:
+            continue
+        m = PegsMedium()
+        m.name = name
+        kv = _kv_line(lines[i]); i += 1
+        m.rho = float(kv["RHO"])
+        m.ne = int(kv["NE"])
+        m.z, m.wa, m.pz, m.rhoz = [], [], [], []
+        for _ in range(m.ne):
+            kv = _kv_line(lines[i]); i += 1
+            m.z.append(float(kv["Z"])); m.wa.append(float(kv["A"]))
+            m.pz.append(float(kv["PZ"])); m.rhoz.append(float(kv["RHOZ"]))
+        # the rest is whitespace-separated numbers in a fixed order
+        tok = []
+        j = i
+
+        def take(n):
+            nonlocal j
+            while len(tok) < n:
+                tok.extend(lines[j].split())
+                j += 1
+            vals = tok[:n]
+            del tok[:n]
+            return vals
+        m.rlc, m.ae, m.ap, m.ue, m.up = map(float, take(5))
+        m.te = m.ae - RM_F
+        m.thmoll = m.te * 2 + RM_F
+        ints = list(map(int, take(9)))
+        m.msge, m.mge, m.mseke, m.meke, m.mleke, m.mcmfp, m.mrange, m.irayl = ints[:8]
+        take(36)                                   # dl1..dl6 (fix_brems recomputes)
+        m.delcm = float(take(1)[0])
+        take(6)                                    # alphi, bpar, delpos
+        m.xr0, m.teff0, m.blcc, m.xcc = map(float, take(4))
+        m.eke0, m.eke1 = map(float, take(2))
+        rows = np.array(list(map(float, take(16 * m.meke)))).reshape(m.meke, 16)
+        m.esig, m.psig = rows[:, 0:2].copy(), rows[:, 2:4].copy()
+        m.ededx, m.pdedx = rows[:, 4:6].copy(), rows[:, 6:8].copy()
+        m.ebr1, m.pbr1, m.pbr2 = rows[:, 8:10].copy(), rows[:, 10:12].copy(), rows[:, 12:14].copy()
+        m.tmxs = rows[:, 14:16].copy()
+        take(3)                                    # ebinda, ge0, ge1
+        take(6 * m.mge)
+        if m.irayl == 1:
+            ngr = int(take(1)[0])
+            take(2 + 2 * ngr + 2 * m.mge)
+        i = j
+        found[name] = m
+    missing = [n for n in names if n not in found]
+    if missing:
+        raise RuntimeError("ERROR (10029): media not found in %s: %s" % (path, " ".join(missing)))
+    media = [found[n] for n in names]
+    # distance-unit scaling, dunit = 1 cm (media.cu:697-770)
+    for m in media:
+        dfacti = 1.0 / m.rlc
+        for arr in (m.esig, m.psig, m.ededx, m.pdedx, m.tmxs):
+            arr *= dfacti
+        m.blcc *= dfacti
+        m.xcc *= math.sqrt(dfacti)
+        m.smaxir = MAX_SMAX
+        zz = sum(p * z * (z + 1.0) for p, z in zip(m.pz, m.z))
+        pznorm = sum(m.pz)
+        m.zbrang = 8.116224e-05 * (zz / pznorm) ** (1.0 / 3.0)
+        m.lzbrang = -math.log(m.zbrang)
+    return media
+
+
+# ------------------------------------------------------------------ photons
+def read_xsec_data(path):
+    """Element cross sections (media.cu:114-142): per Z, n pairs (lnE, ln sigma)."""
+    with open(path) as f:
+        tok = f.read().split()
+    out = []
+    p = 0
+    for _ in range(100):
+        n = int(tok[p]); p += 1
+        d = np.array(tok[p:p + 2 * n], dtype=np.float64).reshape(n, 2)
+        p += 2 * n
+        out.append(d)
+    return out
+
+
+def kn_sigma0(e):
+    """Klein-Nishina total cross section per electron in barns (media.cu:316-330)."""
+    con = float(np.float32(0.1274783851))
+    ko = e / RM_F
+    if ko < 0.01:
+        return 8.0 * con / 3.0 * (1.0 - ko * (2.0 - ko * (5.2 - 13.3 * ko))) / RM_F
+    c1 = 1.0 / (ko * ko)
+    c2 = 1.0 - 2.0 * (1.0 + ko) * c1
+    c3 = (1.0 + 2.0 * ko) * c1
+    eps2 = 1.0
+    eps1 = 1.0 / (1.0 + 2.0 * ko)
+    return (c1 * (1.0 / eps1 - 1.0 / eps2) + c2 * math.log(eps2 / eps1)
+            + eps2 * (c3 + 0.5 * eps2) - eps1 * (c3 + 0.5 * eps1)) / e * con
+
+
+def get_data(flag, zs, pzs, data, ge):
+    """Mixture cross section on the MXGE grid (media.cu:260-311)."""
+    res = np.zeros(MXGE)
+    for z, pz in zip(zs, pzs):
+        d = data[int(z + 0.5) - 1].copy()
+        if flag != 0:
+            eth = 2.0 * RM_F if flag == 1 else 4.0 * RM_F
+            d[:, 1] -= 3.0 * np.log(1.0 - eth / np.exp(d[:, 0]))
+            d = np.vstack([[math.log(eth), d[0, 1]], d])
+        n = len(d)
+        for j in range(MXGE):
+            gle = (j - ge[0]) / ge[1]
+            e = math.exp(gle)
+            if gle < d[0, 0] or gle >= d[n - 1, 0]:
+                if flag == 0:
+                    raise RuntimeError("ERROR (10003): energy %g outside data range" % e)
+                sig = 0.0 if gle < d[0, 0] else math.exp(d[n - 1, 1])
+            else:
+                k = int(np.searchsorted(d[:, 0], gle, side="right") - 1)
+                p = (gle - d[k, 0]) / (d[k + 1, 0] - d[k, 0])
+                sig = math.exp(p * d[k + 1, 1] + (1.0 - p) * d[k, 1])
+            if flag != 0 and e > eth:
+                sig *= (1.0 - eth / e) ** 3
+            res[j] += pz * sig
+    return res
+
+
+def init_user_photon(media, prefix):
+    """ge, gmfp, gbr1, gbr2, cohe (media.cu:832-1058)."""
+    photo = read_xsec_data(prefix + "_photo.data")
+    rayl = read_xsec_data(prefix + "_rayleigh.data")
+    pair = read_xsec_data(prefix + "_pair.data")
+    trip = read_xsec_data(prefix + "_triplet.data")
+    nmed = len(media)
+    ge = np.zeros((nmed, 2))
+    tabs = {k: np.zeros((nmed, MXGE, 2)) for k in ("gmfp", "gbr1", "gbr2", "cohe")}
+    for i, m in enumerate(media):
+        ge[i, 1] = (MXGE - 1) / math.log(m.up / m.ap)
+        ge[i, 0] = -ge[i, 1] * math.log(m.ap)
+        order = np.argsort(np.array(m.z), kind="stable")
+        zs = [m.z[k] for k in order]
+        pzs = [m.pz[k] for k in order]
+        sum_a = sum(p * a for p, a in zip(m.pz, m.wa))
+        sum_z = sum(p * z for p, z in zip(m.pz, m.z))
+        con2 = m.rho / (sum_a * float(np.float32(1.6605655)))
+        s_ph = get_data(0, zs, pzs, photo, ge[i])
+        s_ra = get_data(0, zs, pzs, rayl, ge[i])
+        s_pa = get_data(1, zs, pzs, pair, ge[i])
+        s_tr = get_data(2, zs, pzs, trip, ge[i])
+        vals = {k: np.zeros(MXGE) for k in tabs}
+        gles = np.zeros(MXGE)
+        for j in range(MXGE):
+            gle = (j - ge[i, 0]) / ge[i, 1]
+            gles[j] = gle
+            e = math.exp(gle)
+            sig_kn = sum_z * kn_sigma0(e)
+            sig_p = s_pa[j] + s_tr[j]
+            sigma = sig_kn + sig_p + s_ph[j]
+            vals["gmfp"][j] = 1.0 / (sigma * con2)
+            vals["gbr1"][j] = sig_p / sigma
+            vals["gbr2"][j] = vals["gbr1"][j] + sig_kn / sigma
+            vals["cohe"][j] = sigma / (s_ra[j] + sigma)
+        for k, v in vals.items():
+            t = tabs[k][i]
+            for j in range(1, MXGE):
+                t[j - 1, 1] = (v[j] - v[j - 1]) * ge[i, 1]
+                t[j - 1, 0] = v[j] - t[j - 1, 1] * gles[j]
+            t[MXGE - 1, 1] = t[MXGE - 2, 1]
+            t[MXGE - 1, 0] = v[MXGE - 1] - t[MXGE - 1, 1] * gles[MXGE - 1]
+    return ge, tabs
+
+
+def read_ff_data(path):
+    with open(path) as f:
+        tok = np.array(f.read().split(), dtype=np.float64)
+    xval = tok[:MXRAYFF]
+    aff = tok[MXRAYFF:MXRAYFF + 100 * MXRAYFF].reshape(100, MXRAYFF)
+    return xval, aff
+
+
+def init_rayleigh_data(media, ff_path):
+    """Rayleigh sampling tables (media.cu:1060-1292)."""
+    xval, aff = read_ff_data(ff_path)
+    nmed = len(media)
+    ray = np.zeros((nmed, MXRAYFF, 4))
+    i_arr = np.zeros((nmed, RAYCDFSIZE), dtype=np.uint32)
+    pmax = np.zeros((nmed, MXGE, 2))
+    for i, m in enumerate(media):
+        ff = np.zeros(MXRAYFF)
+        xgrid = xval.copy()
+        for j in range(MXRAYFF):
+            v = 0.0
+            for z, pz in zip(m.z, m.pz):
+                v += pz * aff[int(z) - 1, j] ** 2
+            ff[j] = float(np.sqrt(np.float32(v)))   # sqrtf in the reference
+        if xgrid[0] < 1e-6:
+            xgrid[0] = float(np.float32(0.0001))
+        ge1 = (MXGE - 1) / math.log(m.up / m.ap)
+        ge0 = -ge1 * math.log(m.ap)
+        emin = math.exp(-ge0 / ge1)
+        emax = math.exp((MXGE - 1.0 - ge0) / ge1)
+        ff[ff == 0.0] = 5e-324
+        fcum = np.zeros(MXRAYFF)
+        b_arr = np.zeros(MXRAYFF)
+        s0 = 0.0
+        for j in range(MXRAYFF - 1):
+            b = math.log(ff[j + 1] / ff[j]) / math.log(xgrid[j + 1] / xgrid[j])
+            b_arr[j] = b
+            x1, x2 = xgrid[j], xgrid[j + 1]
+            p1, p2 = x1 ** (2 * b), x2 ** (2 * b)
+            s0 += ff[j] ** 2 * (x2 * x2 * p2 - x1 * x1 * p1) / ((1.0 + b) * p1)
+            fcum[j + 1] = s0
+        b_arr[MXRAYFF - 1] = 0.0
+        pe = np.zeros(MXGE)
+        dle = math.log(emax / emin) / (MXGE - 1.0)
+        idx = 1
+        for j in range(1, MXGE + 1):
+            e = emin * math.exp(dle * (j - 1.0))
+            xmax = 20.607544 * 2.0 * e / 0.5110034
+            k = 1
+            while k <= MXRAYFF - 1:
+                if xgrid[k - 1] <= xmax < xgrid[k]:
+                    break
+                k += 1
+            idx = k
+            b = b_arr[idx - 1]
+            x1, x2 = xgrid[idx - 1], xmax
+            p1, p2 = x1 ** (2 * b), x2 ** (2 * b)
+            pe[j - 1] = fcum[idx - 1] + ff[idx - 1] ** 2 * (x2 * x2 * p2 - x1 * x1 * p1) / ((1.0 + b) * p1)
+        i_arr[i, RAYCDFSIZE - 1] = idx
+        anorm = 1.0 / float(np.sqrt(np.float32(pe[MXGE - 1])))
+        anorm1 = 1.005 / pe[MXGE - 1]
+        anorm2 = 1.0 / pe[MXGE - 1]
+        pe = np.minimum(pe * anorm1, 1.0)
+        ff = ff * anorm
+        fcum = fcum * anorm2
+        c_arr = (1.0 + b_arr) / ((xgrid * ff) ** 2)
+        # uniform cumulative bins
+        dw = 1.0 / (RAYCDFSIZE - 1.0)
+        xold = xgrid[0]
+        ibin = 1
+        b = b_arr[0]
+        p1 = xgrid[0] ** (2 * b)
+        i_arr[i, 0] = 1
+        for j in range(2, RAYCDFSIZE):
+            w = dw
+            while True:
+                x1 = xold
+                x2 = xgrid[ibin]
+                t = x1 * x1 * x1 ** (2 * b)
+                p2 = x2 ** (2 * b)
+                aux = ff[ibin - 1] ** 2 * (x2 * x2 * p2 - t) / ((1.0 + b) * p1)
+                if aux > w:
+                    xold = math.exp(math.log(t + w * (1.0 + b) * p1 / ff[ibin - 1] ** 2) / (2.0 + 2.0 * b))
+                    i_arr[i, j - 1] = ibin
+                    break
+                w -= aux
+                xold = x2
+                ibin += 1
+                b = b_arr[ibin - 1]
+                p1 = xold ** (2 * b)
+        b_arr = 0.5 / (1.0 + b_arr)
+        for j in range(MXGE - 1):
+            gle = (j - ge0) / ge1
+            pmax[i, j, 1] = (pe[j + 1] - pe[j]) * ge1
+            pmax[i, j, 0] = pe[j] - pmax[i, j, 1] * gle
+        pmax[i, MXGE - 1] = pmax[i, MXGE - 2]
+        ray[i, :, 0] = xgrid
+        ray[i, :, 1] = fcum
+        ray[i, :, 2] = b_arr
+        ray[i, :, 3] = c_arr
+    return ray, i_arr, pmax
+
+
+# ------------------------------------------------------------------ electrons
+def init_ms_sr(path):
+    """q2+ surface reader (PhysicsInit.cu:860-906)."""
+    with open(path) as f:
+        tok = f.read().split()
+    ums = np.zeros(MS_SIZE)
+    fms = np.zeros(MS_SIZE)
+    wms = np.zeros(MS_SIZE)
+    ims = np.zeros(MS_SIZE, dtype=np.int64)
+    p = 0
+    L1, Q1 = MAXL_MS + 1, MAXQ_MS + 1
+
+    def ix(k, j, i):
+        return k * L1 * Q1 + j * L1 + i
+    for i in range(MAXL_MS + 1):
+        for j in range(MAXQ_MS + 1):
+            for k in range(MAXU_MS + 1):
+                ums[ix(k, j, i)] = float(tok[p]); p += 1
+            for k in range(MAXU_MS + 1):
+                fms[ix(k, j, i)] = float(tok[p]); p += 1
+            for k in range(MAXU_MS):
+                wms[ix(k, j, i)] = float(tok[p]); p += 1
+            for k in range(MAXU_MS):
+                ims[ix(k, j, i)] = int(tok[p]); p += 1
+            for k in range(MAXU_MS):
+                fms[ix(k, j, i)] = fms[ix(k + 1, j, i)] / fms[ix(k, j, i)] - 1.0
+                ims[ix(k, j, i)] -= 1
+            fms[ix(MAXU_MS, j, i)] = fms[ix(MAXU_MS - 1, j, i)]
+    llammin = math.log(LAMBMIN_MS)
+    llammax = math.log(LAMBMAX_MS)
+    dllamb = (llammax - llammin) / MAXL_MS
+    dqms = QMAX_MS / MAXQ_MS
+    return ums, fms, wms, ims, 1.0 / dllamb, 1.0 / dqms
+
+
+def mscati(media, region_ecuts, ximax=0.5, estepe=0.25):
+    """PhysicsInit.cu:908-1182 for PRESTA-I with inexact BCA (the reference default)."""
+    # skin depth for the BCA from the minimum ECUT over regions
+    ecutmn = min(region_ecuts) if region_ecuts else 1e30
+    tstbmn = 1e30
+    for m in media:
+        tstbm = (ecutmn - 0.5110034) * (ecutmn + 0.5110034) / (ecutmn * ecutmn)
+        r = ecutmn / m.xcc
+        tstbm = m.blcc * tstbm * r * r
+        aux = math.log(tstbm)
+        tstbm = math.log(tstbm / aux)
+        tstbmn = min(tstbmn, tstbm)
+    skindepth = math.exp(tstbmn)
+    for m in media:
+        m.blcc *= 1.16699413758864573
+        m.xcc = m.xcc * m.xcc
+    rm = RM_F
+    for m in media:
+        neke = m.meke
+        e0, e1 = m.eke0, m.eke1
+        sige, sigp = 1e-15, 1e-15
+        mono_e = mono_p = True
+        old_e = old_p = -1.0
+        for i in range(1, neke + 1):
+            ei = math.exp((i - e0) / e1)
+            eil = math.log(ei)
+            s = (m.esig[i - 1, 1] * eil + m.esig[i - 1, 0]) / (m.ededx[i - 1, 1] * eil + m.ededx[i - 1, 0])
+            sige = max(sige, s)
+            if s < old_e:
+                mono_e = False
+            old_e = s
+            s = (m.psig[i - 1, 1] * eil + m.psig[i - 1, 0]) / (m.pdedx[i - 1, 1] * eil + m.pdedx[i - 1, 0])
+            sigp = max(sigp, s)
+            if s < old_p:
+                mono_p = False
+            old_p = s
+        m.sig_e = (sige, sigp)
+        m.sig_ismonotone = (mono_e, mono_p)
+        # e_array, range_ep
+        e_array = np.zeros(MXEKE)
+        rng = np.zeros((MXEKE, 2))
+        ei = math.exp((1 - e0) / e1)
+        e_array[0] = ei
+        for i in range(1, neke):
+            eip1 = math.exp((i + 1 - e0) / e1)
+            e_array[i] = eip1
+            eke = (eip1 + ei) * 0.5
+            elke = math.log(eke)
+            lelke = int(e1 * elke + e0)
+            for q, tab in ((1, m.pdedx), (0, m.ededx)):
+                ededx = tab[lelke - 1, 1] * elke + tab[lelke - 1, 0]
+                aux = tab[i - 1, 1] / ededx
+                r1 = (eip1 - ei) / eke
+                rng[i, q] = rng[i - 1, q] + (eip1 - ei) / ededx * (1 + aux * (aux * 2 + 1) * r1 * r1 / 24)
+            ei = eip1
+        m.e_array = e_array
+        m.range_ep = rng
+        # tmxs
+        tm = np.zeros((MXEKE, 2))
+        eil = (1 - e0) / e1
+        ei = math.exp(eil)
+        p2 = ei * (ei + rm * 2)
+        beta2 = p2 / (p2 + rm * rm)
+        chi_a2 = m.xcc / (p2 * 4 * m.blcc)
+        dedx0 = m.ededx[0, 1] * eil + m.ededx[0, 0]
+        estepx = p2 * 2 * beta2 * dedx0 / ei / m.xcc / (math.log(1.0 / chi_a2 + 1) * (chi_a2 + 1) - 1)
+        estepx = min(estepx * ximax, estepe)
+        si = estepx * ei / dedx0
+        for i in range(1, neke):
+            elke = (i + 1 - e0) / e1
+            eke = math.exp(elke)
+            lelke = i + 1
+            p2 = eke * (eke + rm * 2)
+            beta2 = p2 / (p2 + rm * rm)
+            chi_a2 = m.xcc / (p2 * 4 * m.blcc)
+            ededx = m.ededx[lelke - 1, 1] * elke + m.ededx[lelke - 1, 0]
+            estepx = p2 * 2 * beta2 * ededx / eke / m.xcc / (math.log(1.0 / chi_a2 + 1) * (chi_a2 + 1) - 1)
+            estepx = min(estepx * ximax, estepe)
+            ekef = (1 - estepx) * eke
+            if ekef <= e_array[0]:
+                sip1 = (e_array[0] - ekef) / dedx0
+                ekef = e_array[0]
+                elkef = (1 - e0) / e1
+                lelkef = 1
+            else:
+                elkef = math.log(ekef)
+                lelkef = int(e1 * elkef + e0)
+                leip1l = lelkef + 1
+                eip1l = (leip1l - e0) / e1
+                eip1 = e_array[leip1l - 1]
+                aux = (eip1 - ekef) / eip1
+                elktmp = 0.5 * (elkef + eip1l + 0.25 * aux * aux * (1 + aux * (1 + 0.875 * aux)))
+                ektmp = (ekef + eip1) * 0.5
+                ededx = m.ededx[lelkef - 1, 1] * elktmp + m.ededx[lelkef - 1, 0]
+                aux = m.ededx[lelkef - 1, 1] / ededx
+                r1 = (eip1 - ekef) / ektmp
+                sip1 = (eip1 - ekef) / ededx * (1 + aux * (aux * 2 + 1) * r1 * r1 / 24)
+            sip1 = sip1 + rng[lelke - 1, 0] - rng[lelkef, 0]
+            tm[i - 1, 1] = (sip1 - si) * e1
+            tm[i - 1, 0] = sip1 - tm[i - 1, 1] * elke
+            si = sip1
+        tm[neke - 1] = tm[neke - 2]
+        m.tmxs_ms = tm
+    return skindepth
+
+
+def fix_brems(m):
+    """Screening-function coefficients dl1..dl6 and bpar (PhysicsInit.cu:1552-1641)."""
+    from .pegs_lite import fcoulc, xsif
+    m.log_ap = math.log(m.ap)
+    zt = zb = zf = 0.0
+    for z, p in zip(m.z, m.pz):
+        fc = fcoulc(z)
+        xi = xsif(z)
+        aux = p * z * (z + xi)
+        zt += aux
+        zb -= aux * math.log(z) / 3
+        zf += aux * fc
+    zv = (zb - zf) / zt
+    zg = zb / zt
+    f1 = 2 * (20.863 + 4 * zg) - 2 * (20.029 + 4 * zg) / 3
+    f2 = 2 * (20.863 + 4 * zv) - 2 * (20.029 + 4 * zv) / 3
+    dl = np.zeros((6, 8))
+    dl[:, 0] = [(20.863 + 4 * zg) / f1, -3.242 / f1, 0.625 / f1, (21.12 + 4 * zg) / f1, -4.184 / f1, 0.952]
+    dl[:, 1] = [(20.029 + 4 * zg) / f1, -1.93 / f1, -0.086 / f1, (21.12 + 4 * zg) / f1, -4.184 / f1, 0.952]
+    dl[:, 2] = [(20.863 + 4 * zv) / f2, -3.242 / f2, 0.625 / f2, (21.12 + 4 * zv) / f2, -4.184 / f2, 0.952]
+    dl[:, 3] = [(20.029 + 4 * zv) / f2, -1.93 / f2, -0.086 / f2, (21.12 + 4 * zv) / f2, -4.184 / f2, 0.952]
+    dl[:, 4] = [3 * (20.863 + 4 * zg) - (20.029 + 4 * zg), 3 * (-3.242) - (-1.930), 3 * 0.625 - (-0.086),
+                2 * 21.12 + 8 * zg, 2 * (-4.184), 0.952]
+    dl[:, 5] = [3 * (20.863 + 4 * zg) + (20.029 + 4 * zg), 3 * (-3.242) + (-1.930), 3 * 0.625 + (-0.086),
+                4 * 21.12 + 16 * zg, 4 * (-4.184), 0.952]
+    dl[:, 6] = [3 * (20.863 + 4 * zv) - (20.029 + 4 * zv), 3 * (-3.242) - (-1.930), 3 * 0.625 - (-0.086),
+                2 * 21.12 + 8 * zv, 2 * (-4.184), 0.952]
+    dl[:, 7] = [3 * (20.863 + 4 * zv) + (20.029 + 4 * zv), 3 * (-3.242) + (-1.930), 3 * 0.625 + (-0.086),
+                4 * 21.12 + 16 * zv, 4 * (-4.184), 0.952]
+    m.dl = dl
+    m.bpar = (12 * dl[0, 7] / (3 * dl[0, 7] + dl[0, 6]), dl[0, 6] / (3 * dl[0, 7] + dl[0, 6]))
+
+
+# ------------------------------------------------------------------ assembly
+class MediaTables:
+    """Device physics tables for an ordered list of media."""
+
+    def __init__(self, names, data_dir=DATA_DIR, pegs="pegslite700.pegs4dat", prefix="pegslite",
+                 mlc_medium=None, eii_flag=1):
+        if len(names) > MXMED:
+            raise ValueError("at most %d media (MXMED)" % MXMED)
+        self.names = list(names)
+        media = read_pegs_file(os.path.join(data_dir, pegs), self.names)
+        ge, ptabs = init_user_photon(media, os.path.join(data_dir, prefix))
+        ray, i_arr, pmax = init_rayleigh_data(media, os.path.join(data_dir, prefix + "_ff.data"))
+        ums, fms, wms, ims, dllambi, dqmsi = init_ms_sr(os.path.join(data_dir, prefix + "_msnew.data"))
+        # every region cut equals its medium's AE (init.cu:1005-1006)
+        skin = mscati(media, [m.ae for m in media])
+        for m in media:
+            fix_brems(m)
+        c = CMediaTables()
+        c.nmed = len(media)
+        c.eii_flag = eii_flag
+        c.nbr_split = 1
+        c.ibrdst = 1
+        c.iprdst = 1
+        c.exact_bca = 0
+        c.spin_effects = 0
+        c.transport_algorithm = 1
+        c.skindepth_for_bca = skin
+        c.dllambi = dllambi
+        c.dqmsi = dqmsi
+        c.rm = RM_F
+        for im, m in enumerate(media):
+            c.rho[im] = m.rho
+            c.te[im] = m.te
+            c.thmoll[im] = m.thmoll
+            c.ae[im] = m.ae
+            c.ap[im] = m.ap
+            c.log_ap[im] = m.log_ap
+            c.smaxir[im] = m.smaxir
+            c.e_max_rr[im] = E_MAX_RR
+            c.zbrang[im] = m.zbrang
+            c.lzbrang[im] = m.lzbrang
+            c.delcm[im] = m.delcm
+            c.blcc[im] = m.blcc
+            c.xcc[im] = m.xcc
+            c.i_do_rr[im] = 1 if (mlc_medium is not None and m.name == mlc_medium) else 0
+            c.meke[im] = m.meke
+            c.nne[im] = m.ne
+            for ie in range(m.ne):
+                c.zelem[MXMED * ie + im] = m.z[ie]
+                c.pz[MXMED * ie + im] = m.pz[ie]
+            for l in range(8):
+                c.dl1[8 * im + l] = m.dl[0, l]
+                c.dl2[8 * im + l] = m.dl[1, l]
+                c.dl3[8 * im + l] = m.dl[2, l]
+                c.dl4[8 * im + l] = m.dl[3, l]
+                c.dl5[8 * im + l] = m.dl[4, l]
+                c.dl6[8 * im + l] = m.dl[5, l]
+            c.bpar[2 * im] = m.bpar[0]
+            c.bpar[2 * im + 1] = m.bpar[1]
+            c.sig_e[im] = m.sig_e[0]
+            c.sig_e[MXMED + im] = m.sig_e[1]
+            c.sig_ismonotone[2 * im] = int(m.sig_ismonotone[0])
+            c.sig_ismonotone[2 * im + 1] = int(m.sig_ismonotone[1])
+            c.eke01[im].x = m.eke0
+            c.eke01[im].y = m.eke1
+            for i in range(m.meke):
+                for q, (st, dt) in enumerate(((m.esig, m.ededx), (m.psig, m.pdedx))):
+                    k = q * MXEKE * MXMED + im * MXEKE + i
+                    c.sig[k].x, c.sig[k].y = st[i, 0], st[i, 1]
+                    c.dedx[k].x, c.dedx[k].y = dt[i, 0], dt[i, 1]
+                k = im * MXEKE + i
+                c.ebr1[k].x, c.ebr1[k].y = m.ebr1[i]
+                c.pbr1[k].x, c.pbr1[k].y = m.pbr1[i]
+                c.pbr2[k].x, c.pbr2[k].y = m.pbr2[i]
+                c.tmxs[k].x, c.tmxs[k].y = m.tmxs_ms[i]
+                c.range_ep[(im * MXEKE + i) * 2] = m.range_ep[i, 0]
+                c.range_ep[(im * MXEKE + i) * 2 + 1] = m.range_ep[i, 1]
+                c.e_array[k] = m.e_array[i]
+            c.ge[im].x, c.ge[im].y = ge[im]
+            for j in range(MXGE):
+                k = im * MXGE + j
+                for name in ("gmfp", "gbr1", "gbr2", "cohe"):
+                    getattr(c, name)[k].x = ptabs[name][im, j, 0]
+                    getattr(c, name)[k].y = ptabs[name][im, j, 1]
+                c.pmax[k].x, c.pmax[k].y = pmax[im, j]
+            for j in range(MXRAYFF):
+                r = c.rayleigh_data[im * MXRAYFF + j]
+                r.xgrid, r.fcum, r.b_array, r.c_array = ray[im, j]
+            for j in range(RAYCDFSIZE):
+                c.i_array[im * RAYCDFSIZE + j] = int(i_arr[im, j])
+        for name, arr, dt in (("ums", ums, np.float32), ("fms", fms, np.float32),
+                              ("wms", wms, np.float32), ("ims", ims, np.int16)):
+            buf = np.ascontiguousarray(arr, dtype=dt)     # keep alive across the copy
+            ctypes.memmove(getattr(c, name), buf.ctypes.data, buf.nbytes)
+        self.c = c
+        self.media = media
+
+    def raw(self):
+        """Bytes of the ``gegs_media_tables`` block."""
+        return ctypes.string_at(ctypes.addressof(self.c), ctypes.sizeof(self.c))
+
+    def ptr(self):
+        return ctypes.addressof(self.c)
+
+    def index(self, name):
+        return self.names.index(name)

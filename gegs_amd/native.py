@@ -1,0 +1,155 @@
+"""ctypes binding of the C ABI (include/gegs.h) in gegs_amd/libgegs_hip.so.
+
+The product path has no CPU fallback: if the HIP library is missing or no
+GPU is visible, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import hatch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgegs_hip.so")
+
+# exported symbols declared in include/gegs.h
+SYMBOLS = [
+    "gegs_create", "gegs_destroy", "gegs_last_error", "gegs_sizeof_media_tables",
+    "gegs_load_media", "gegs_load_phantom", "gegs_set_field", "gegs_set_source",
+    "gegs_run_plain", "gegs_run_event", "gegs_set_history_tally", "gegs_history_tally",
+    "gegs_get_counters", "gegs_last_kernel_ms",
+]
+
+
+class RunOptions(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("history_offset", ctypes.c_uint64),
+                ("n_batches", ctypes.c_uint32), ("block_size", ctypes.c_uint32),
+                ("grid_size", ctypes.c_uint32), ("stack_depth", ctypes.c_uint32)]
+
+
+class Counters(ctypes.Structure):
+    _fields_ = [("photon_steps", ctypes.c_uint64), ("electron_steps", ctypes.c_uint64),
+                ("region_gathers", ctypes.c_uint64), ("deposits", ctypes.c_uint64),
+                ("interactions", ctypes.c_uint64 * 10), ("histories", ctypes.c_uint64),
+                ("overflow", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return dict(photon_steps=self.photon_steps, electron_steps=self.electron_steps,
+                    region_gathers=self.region_gathers, deposits=self.deposits,
+                    interactions=list(self.interactions), histories=self.histories,
+                    overflow=self.overflow)
+
+
+_lib = None
+
+
+def load():
+    """Load the in-tree HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("gegs_amd/libgegs_hip.so is missing: run `python -m gegs_amd.build` "
+                           "(the transport has no CPU fallback)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    L.gegs_create.restype = vp
+    L.gegs_create.argtypes = [ctypes.c_int]
+    L.gegs_destroy.argtypes = [vp]
+    L.gegs_last_error.restype = ctypes.c_char_p
+    L.gegs_last_error.argtypes = [vp]
+    L.gegs_sizeof_media_tables.restype = ctypes.c_size_t
+    L.gegs_load_media.argtypes = [vp, vp]
+    L.gegs_load_phantom.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp, vp]
+    L.gegs_set_field.argtypes = [vp, vp]
+    L.gegs_set_source.argtypes = [vp, vp, ctypes.c_uint64]
+    L.gegs_run_plain.argtypes = [vp, vp, vp, vp]
+    L.gegs_run_event.argtypes = [vp, vp, vp, vp]
+    L.gegs_set_history_tally.argtypes = [vp, ctypes.c_int]
+    L.gegs_history_tally.argtypes = [vp, vp, vp]
+    L.gegs_get_counters.argtypes = [vp, vp]
+    L.gegs_last_kernel_ms.argtypes = [vp, vp]
+    for name in SYMBOLS:
+        if getattr(L, name, None) is None:
+            raise RuntimeError("symbol %s missing from %s" % (name, LIB_PATH))
+    if L.gegs_sizeof_media_tables() != ctypes.sizeof(hatch.CMediaTables):
+        raise RuntimeError("gegs_media_tables layout mismatch between include/gegs_media.h and hatch.py")
+    _lib = L
+    return L
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class GegsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("gegs error %d: %s" % (code, msg))
+        self.code = code
+
+
+class Context:
+    """One device context (gegs_ctx) with a workload loaded."""
+
+    def __init__(self, device=0):
+        self.L = load()
+        self.h = self.L.gegs_create(device)
+        if not self.h:
+            raise RuntimeError("gegs_create failed: no usable HIP device %d" % device)
+        self.nvox = 0
+        self.nsrc = 0
+
+    def _check(self, rc):
+        if rc != 0:
+            raise GegsError(rc, self.L.gegs_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            self.L.gegs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load_workload(self, wl):
+        t = wl.tables
+        self._check(self.L.gegs_load_media(self.h, ctypes.c_void_p(t.ptr())))
+        ph = wl.phantom
+        reg = np.ascontiguousarray(wl.regions)
+        self._check(self.L.gegs_load_phantom(self.h, ph.nx, ph.ny, ph.nz, _p(ph.xb), _p(ph.yb), _p(ph.zb), _p(reg)))
+        self._check(self.L.gegs_set_field(self.h, ctypes.c_void_p(ctypes.addressof(wl.field))))
+        self.nvox = ph.nvox
+
+    def set_source(self, src):
+        src = np.ascontiguousarray(src)
+        self._check(self.L.gegs_set_source(self.h, _p(src), len(src)))
+        self.nsrc = len(src)
+
+    def run(self, seed, n_batches=10, history_offset=0, method="plain", per_history=False,
+            grid=0, block=0, depth=0):
+        self._check(self.L.gegs_set_history_tally(self.h, 1 if per_history else 0))
+        opt = RunOptions(seed, history_offset, n_batches, block, grid, depth)
+        dose = np.zeros(self.nvox, dtype=np.float64)
+        dose2 = np.zeros(self.nvox, dtype=np.float64)
+        fn = self.L.gegs_run_plain if method == "plain" else self.L.gegs_run_event
+        self._check(fn(self.h, ctypes.c_void_p(ctypes.addressof(opt)), _p(dose), _p(dose2)))
+        out = dict(dose=dose, dose2=dose2, n_batches=n_batches)
+        ms = ctypes.c_float()
+        self._check(self.L.gegs_last_kernel_ms(self.h, ctypes.byref(ms)))
+        out["ms"] = ms.value
+        cnt = Counters()
+        self._check(self.L.gegs_get_counters(self.h, ctypes.byref(cnt)))
+        out["counters"] = cnt.as_dict()
+        if per_history:
+            edep = np.zeros(self.nsrc, dtype=np.float64)
+            steps = np.zeros(self.nsrc, dtype=np.uint32)
+            self._check(self.L.gegs_history_tally(self.h, _p(edep), _p(steps)))
+            out["edep"] = edep
+            out["steps"] = steps
+        return out

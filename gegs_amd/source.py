@@ -1,0 +1,126 @@
+"""Source particles: egsphsp MODE0 phase-space I/O and the seeded synthetic
+sources of the benchmark workloads.
+
+* MODE0 layout (read_egsphsp.h:8-28, read_egsphsp.cpp:169-232): a 28-byte
+  header (``MODE_RW[5]`` padded to 8, NPPHSP, NPHOTPHSP, EKMAXPHSP, EKMINPHSP,
+  NINCPHSP) followed by 28-byte records LATCH, E, X, Y, U, V, WT.
+* decoding (CUDA_EGS.cu:56-67): LATCH bit 30 -> electron, bit 29 ->
+  positron, else photon; W = +sqrt(1-U^2-V^2) with the sign of WT; |E| is the
+  total energy; z is the phase-space plane ``z_phsp``.
+"""
+from __future__ import annotations
+
+import math
+import struct
+
+import numpy as np
+
+SOURCE_DTYPE = np.dtype([("e", "<f4"), ("x", "<f4"), ("y", "<f4"), ("z", "<f4"),
+                         ("u", "<f4"), ("v", "<f4"), ("w", "<f4"), ("wt", "<f4"),
+                         ("charge", "<i4")])
+PHSP_DTYPE = np.dtype([("latch", "<u4"), ("e", "<f4"), ("x", "<f4"), ("y", "<f4"),
+                       ("u", "<f4"), ("v", "<f4"), ("wt", "<f4")])
+RM = 0.5110034
+
+
+def write_egsphsp(path, rec, nincphsp=None):
+    """Write MODE0 records (structured array of PHSP_DTYPE)."""
+    rec = np.ascontiguousarray(rec, dtype=PHSP_DTYPE)
+    q = ((rec["latch"] >> 30) & 1) | ((rec["latch"] >> 29) & 1)
+    nphot = int(np.sum(q == 0))
+    ek = np.abs(rec["e"]) - np.where(q != 0, RM, 0.0)
+    hdr = struct.pack("<5s3xIIffI", b"MODE0", len(rec), nphot, float(ek.max()) if len(rec) else 0.0,
+                      float(ek.min()) if len(rec) else 0.0, nincphsp or len(rec))
+    assert len(hdr) == 28
+    with open(path, "wb") as f:
+        f.write(hdr)
+        f.write(rec.tobytes())
+
+
+def read_egsphsp(path, start=0, count=None):
+    """Read MODE0 records [start, start+count); returns (header dict, records)."""
+    with open(path, "rb") as f:
+        hdr = f.read(28)
+        mode, npp, nphot, ekmax, ekmin, ninc = struct.unpack("<5s3xIIffI", hdr)
+        if mode != b"MODE0":
+            raise ValueError("ERROR (2020): only MODE0 phase-space files are supported")
+        if count is None:
+            count = npp - start
+        if start + count > npp:
+            raise ValueError("ERROR (2019): fewer than %d particles remain" % count)
+        f.seek(28 + 28 * start)
+        rec = np.frombuffer(f.read(28 * count), dtype=PHSP_DTYPE).copy()
+    return dict(npphsp=npp, nphotphsp=nphot, ekmax=ekmax, ekmin=ekmin, ninc=ninc), rec
+
+
+def decode_phsp(rec, z_phsp):
+    """MODE0 records -> source particles (CUDA_EGS.cu:56-106)."""
+    out = np.zeros(len(rec), dtype=SOURCE_DTYPE)
+    latch = rec["latch"]
+    q = np.where(latch & 0x40000000, -1, np.where(latch & 0x20000000, 1, 0))
+    u = rec["u"].astype(np.float32)
+    v = rec["v"].astype(np.float32)
+    w = np.sqrt(np.maximum(np.float32(0.0), np.float32(1.0) - u * u - v * v)).astype(np.float32)
+    wt = rec["wt"].astype(np.float32)
+    w = np.where(wt < 0, -w, w)
+    out["e"] = np.abs(rec["e"])
+    out["x"] = rec["x"]
+    out["y"] = rec["y"]
+    out["z"] = np.float32(z_phsp)
+    out["u"] = u
+    out["v"] = v
+    out["w"] = w
+    out["wt"] = np.abs(wt)
+    out["charge"] = q
+    return out
+
+
+def parallel_beam(n, energy, half_width, z0, seed):
+    """Parallel square field of monoenergetic photons travelling along +z."""
+    rng = np.random.default_rng(seed)
+    s = np.zeros(n, dtype=SOURCE_DTYPE)
+    s["e"] = np.float32(energy)
+    s["x"] = rng.uniform(-half_width, half_width, n).astype(np.float32)
+    s["y"] = rng.uniform(-half_width, half_width, n).astype(np.float32)
+    s["z"] = np.float32(z0)
+    s["w"] = np.float32(1.0)
+    s["wt"] = np.float32(1.0)
+    return s
+
+
+def mv6_spectrum(nbins=24, emin=0.25, emax=6.0):
+    """Synthetic 6 MV-like bremsstrahlung spectrum: a thin-target (Kramers)
+    shape k^-1 (E0 - k) filtered by 1.5 cm of tungsten-equivalent attenuation,
+    normalised; 24 bins over 0.25-6 MeV.  Returns (bin edges, probabilities)."""
+    edges = np.linspace(emin, emax, nbins + 1)
+    mid = 0.5 * (edges[1:] + edges[:-1])
+    thin = (emax + 0.2 - mid) / mid
+    mu = 0.04 + 0.9 * mid ** -1.9 * 0.02      # crude flattening-filter hardening
+    w = thin * np.exp(-mu * 8.0)
+    w /= w.sum()
+    return edges, w
+
+
+def point_source_spectrum(n, ssd, field, seed, z_source=0.0, edges=None, probs=None):
+    """Point source at the origin of the head frame; directions towards points
+    uniform over a field x field square at distance ``ssd``; energies from the
+    binned spectrum (uniform within a bin)."""
+    if edges is None:
+        edges, probs = mv6_spectrum()
+    rng = np.random.default_rng(seed)
+    half = field / 2.0
+    px = rng.uniform(-half, half, n)
+    py = rng.uniform(-half, half, n)
+    norm = np.sqrt(px * px + py * py + ssd * ssd)
+    s = np.zeros(n, dtype=SOURCE_DTYPE)
+    b = rng.choice(len(probs), size=n, p=probs)
+    e = edges[b] + rng.uniform(0.0, 1.0, n) * (edges[b + 1] - edges[b])
+    s["e"] = e.astype(np.float32)
+    s["x"] = np.float32(0.0)
+    s["y"] = np.float32(0.0)
+    s["z"] = np.float32(z_source)
+    s["u"] = (px / norm).astype(np.float32)
+    s["v"] = (py / norm).astype(np.float32)
+    s["w"] = (ssd / norm).astype(np.float32)
+    s["wt"] = np.float32(1.0)
+    return s

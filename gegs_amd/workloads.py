@@ -1,0 +1,84 @@
+"""Seeded synthetic workloads of the benchmark (phantom + media + field + source).
+
+``water``      60^3 water at 0.5 cm, parallel 10x10 cm^2 field of 1.25 MeV
+               photons on the +z face, ECUT 0.7 MeV total / PCUT 0.01 MeV.
+``ct``         128x128x100 synthetic CT at 0.25 cm through the ctcreate ramp
+               (air, lung, ICRU tissue, ICRP bone); point source at SSD 100 cm,
+               10x10 cm^2 field, 24-bin 6 MV-like spectrum; CT axis convention
+               (the beam travels along phantom +y).
+"""
+from __future__ import annotations
+
+import ctypes
+import functools
+import math
+
+import numpy as np
+
+from . import hatch, phantom as ph, source as src
+
+M_PHANTOM = 6
+
+
+class Field(ctypes.Structure):
+    """ctypes mirror of ``gegs_field`` (include/gegs.h)."""
+    _fields_ = [("change_xyz", ctypes.c_int32), ("module_first", ctypes.c_int32),
+                ("source_to_isocenter", ctypes.c_float), ("cos_angle", ctypes.c_float),
+                ("sin_angle", ctypes.c_float), ("iso_x", ctypes.c_float),
+                ("iso_y", ctypes.c_float), ("iso_z", ctypes.c_float)]
+
+
+def make_field(change_xyz, sid, gantry_deg=0.0, iso=(0.0, 0.0, 0.0)):
+    a = np.float32(gantry_deg * 3.1415926 / 180.0)     # CUDA_EGS.cu:424
+    return Field(int(change_xyz), M_PHANTOM, float(sid), float(np.cos(a, dtype=np.float32)),
+                 float(np.sin(a, dtype=np.float32)), *map(float, iso))
+
+
+class Workload:
+    def __init__(self, name, phantom, media_names, field, other_media, source_fn):
+        self.name = name
+        self.phantom = phantom
+        self.media_names = media_names
+        self.field = field
+        self.other_media = other_media
+        self._source_fn = source_fn
+        self._tables = None
+        self._regions = None
+
+    @property
+    def tables(self):
+        if self._tables is None:
+            self._tables = _tables(tuple(self.media_names))
+        return self._tables
+
+    @property
+    def regions(self):
+        if self._regions is None:
+            self._regions = ph.region_table(self.phantom, self.tables, self.other_media)
+        return self._regions
+
+    def source(self, n, seed):
+        return self._source_fn(n, seed)
+
+
+@functools.lru_cache(maxsize=8)
+def _tables(names):
+    return hatch.MediaTables(list(names))
+
+
+def water(n=60, voxel=0.5, energy=1.25, field_half=5.0):
+    phan = ph.water_phantom(n, n * voxel)
+    fld = make_field(False, 100.0)
+    return Workload("water_60cube_1.25MeV", phan, ["H2O700ICRU", "AIR700ICRU"], fld,
+                    ["AIR700ICRU"] * 5,
+                    lambda k, seed: src.parallel_beam(k, energy, field_half, 90.0, seed))
+
+
+def ct(nx=128, ny=128, nz=100, voxel=0.25, seed=2):
+    phan = ph.ct_phantom(nx, ny, nz, voxel, seed)
+    body_half_ap = 0.30 * ny * voxel
+    sid = 100.0 + body_half_ap
+    fld = make_field(True, sid)
+    names = [r[0] for r in ph.DEFAULT_RAMP]
+    return Workload("ct_128x128x100_6MV", phan, names, fld, ["AIR700ICRU"] * 5,
+                    lambda k, s: src.point_source_spectrum(k, 100.0, 10.0, s))

@@ -1,0 +1,125 @@
+/*
+ * gegs.h -- C ABI of the MI355X transport library (libgegs_hip.so).
+ *
+ * The reference has no plugin API: its hot path sits behind the host <->
+ * device contract of launch_kernels()/runSimulation() (CUDA_EGS.cu:226,
+ * CUDA_EGS.cu:367), with state published through cudaMemcpyToSymbol and
+ * errors reported through exit().  This header replaces that contract with a
+ * context object, plain pointers and status codes.
+ *
+ * Ownership: the caller owns every host buffer; the context owns the device
+ * buffers it allocates.  One context per host thread; a context is not
+ * re-entrant.  No function calls exit().
+ */
+#ifndef GEGS_H
+#define GEGS_H
+#include <stddef.h>
+#include <stdint.h>
+#include "gegs_media.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes: reference error families (2xxx input, 8xxx phantom, 10xxx
+ * media, output.c:51-89) plus codes the reference does not have. */
+#define GEGS_OK 0
+#define GEGS_ERR_ARG 2001            /* bad argument / missing input */
+#define GEGS_ERR_PHANTOM 8001        /* inconsistent phantom definition */
+#define GEGS_ERR_MEDIA 10029         /* media tables missing or invalid */
+#define GEGS_ERR_DEVICE 58001        /* HIP runtime failure */
+#define GEGS_ERR_QUEUE_OVERFLOW 90001 /* a particle bank overflowed (the reference drops, utility.cu:270) */
+#define GEGS_ERR_STATE 90002         /* call out of order (e.g. run before load) */
+
+/* Module ids (CUDA_EGS.h:396-404). */
+#define GEGS_M_PHANTOM 6
+
+/* One source particle in the head frame, as decoded from an egsphsp MODE0
+ * record (CUDA_EGS.cu:40-158): total energy, position with z at the source
+ * plane, direction cosines, weight, charge (-1, 0, +1). */
+typedef struct gegs_source {
+    float e, x, y, z, u, v, w, wt;
+    int32_t charge;
+} gegs_source;
+
+/* Beam / field geometry (CUDA_EGS.h:103-104, :278-282; CUDA_EGS.cu:430-432). */
+typedef struct gegs_field {
+    int32_t change_xyz;          /* CT axis convention (Change_xyz) */
+    int32_t module_first;        /* module of source particles (module_order[m_FirstMd]) */
+    float source_to_isocenter;   /* SID, cm */
+    float cos_angle, sin_angle;  /* gantry rotation */
+    float iso_x, iso_y, iso_z;   /* isocentre in phantom coordinates */
+} gegs_field;
+
+/* Options of gegs_run_plain. */
+typedef struct gegs_run_options {
+    uint64_t seed;               /* Philox-4x32-10 key */
+    uint64_t history_offset;     /* global index of the first history */
+    uint32_t n_batches;          /* time-sliced statistical batches (>= 2) */
+    uint32_t block_size;         /* threads per block, 0 = default */
+    uint32_t grid_size;          /* blocks, 0 = default */
+    uint32_t stack_depth;        /* per-history particle stack depth, 0 = default */
+} gegs_run_options;
+
+/* Event counters (diagnostics; replaces DO_STEP_COUNT / PRINT_PART_NUM). */
+typedef struct gegs_counters {
+    uint64_t photon_steps;       /* photon free-flight segments */
+    uint64_t electron_steps;     /* condensed-history steps */
+    uint64_t region_gathers;     /* region records read on voxel entry */
+    uint64_t deposits;           /* escore calls with region >= 6 */
+    uint64_t interactions[10];   /* pops per event type (particle_status 0..9) */
+    uint64_t histories;
+    uint64_t overflow;           /* particles lost to a full stack */
+} gegs_counters;
+
+typedef struct gegs_ctx gegs_ctx;
+
+/* Lifecycle.  Replaces init()/free_all() (init.cu:1682, init.cu:1713). */
+gegs_ctx *gegs_create(int device);
+void gegs_destroy(gegs_ctx *ctx);
+const char *gegs_last_error(gegs_ctx *ctx);
+size_t gegs_sizeof_media_tables(void);
+
+/* Physics tables (copyToDevice, copy.cu:3-277; init_user_photon upload,
+ * media.cu:1030-1048). */
+int gegs_load_media(gegs_ctx *ctx, const gegs_media_tables *tables);
+
+/* Voxel phantom and region table (init_regions, init.cu:909-1051; phantom
+ * bounds, init.cu:1199-1215).  regions has 6 + nx*ny*nz records: 0 vacuum,
+ * 1..5 the head modules' media, then voxels x-fastest. */
+int gegs_load_phantom(gegs_ctx *ctx, uint32_t nx, uint32_t ny, uint32_t nz,
+                      const float *x_bounds, const float *y_bounds, const float *z_bounds,
+                      const gegs_region *regions);
+
+/* Field geometry (gantry symbols, CUDA_EGS.cu:430-432). */
+int gegs_set_field(gegs_ctx *ctx, const gegs_field *field);
+
+/* Source particles (load_stack_from_phspfile, CUDA_EGS.cu:40-158): copied to
+ * device memory; histories map 1:1 onto records. */
+int gegs_set_source(gegs_ctx *ctx, const gegs_source *src, uint64_t n);
+
+/* Plain per-history transport of all loaded source records: the baseline
+ * kernel.  dose and dose2 receive, per voxel, the energy deposited (MeV, FP64)
+ * and the sum over batches of the squared batch energy, from which the
+ * caller forms the batch variance. */
+int gegs_run_plain(gegs_ctx *ctx, const gegs_run_options *opt, double *dose, double *dose2);
+
+/* Same transport through the event-queued path: SoA particle banks in HBM,
+ * one kernel per event type, wavefront-compacted queues. */
+int gegs_run_event(gegs_ctx *ctx, const gegs_run_options *opt, double *dose, double *dose2);
+
+/* Per-history diagnostics of the last run (total energy deposited in the
+ * phantom per history, FP64; number of transport steps per history), sized
+ * by the number of source records; NULL pointers are skipped.  Available
+ * only when the run was made with gegs_set_history_tally(ctx, 1). */
+int gegs_set_history_tally(gegs_ctx *ctx, int enable);
+int gegs_history_tally(gegs_ctx *ctx, double *edep, uint32_t *steps);
+
+/* Counters and timing of the last run. */
+int gegs_get_counters(gegs_ctx *ctx, gegs_counters *out);
+int gegs_last_kernel_ms(gegs_ctx *ctx, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1739 @@
+/*
+ * egs_oracle.c -- CPU restatement of the reference's coupled e-/photon
+ * transport, used as the correctness oracle and as the CPU baseline.
+ *
+ * TEST INFRASTRUCTURE.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load this code, and only as the checker; the product
+ * path (gegs_amd/) never calls it.
+ *
+ * One history at a time with its own particle stack (the "plain" method of
+ * the small water case), LIFO, processed particle by particle.  Each function
+ * follows the reference file cited in its comment; the reference's ten global
+ * stacks (CUDA_EGS.h:362-367) become one per-history stack whose records keep
+ * the event type in `status`, so a push "to the compton stack" is a push with
+ * status = compton.
+ *
+ * Random numbers: Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) instead
+ * of cuRAND XORWOW (random.cu).  Every particle owns a counter-based stream
+ * keyed by (seed, lineage); a stream is buffered four uniforms at a time only
+ * while its particle is being processed, and a particle stored on the stack
+ * keeps just its next block index.  Particles created by an interaction get a
+ * lineage derived from the parent's lineage and stream position, so the
+ * random numbers every particle sees do not depend on the order in which
+ * particles are processed.  Uniforms are in (0, 1] like curand_uniform.
+ *
+ * Arithmetic: binary32 with explicit float literals; quantities the
+ * reference keeps in double (prm, the FP64 tallies) stay double.
+ * Transcendentals come from oracle_math.h.  Compile with -ffp-contract=off
+ * and without -ffast-math.
+ *
+ * Physics switches follow the reference defaults (CUDA_EGS.h:34-51): PRESTA-I,
+ * inexact BCA, no spin, KM bremsstrahlung angles, IPRDST=1, EII on,
+ * Rayleigh on, photo-electron angles on.  Bound Compton and atomic
+ * relaxation are controlled per region by the region flags.
+ *
+ * Reference quirks kept ("reference-compat"): Q1 hownear_phantom uses
+ * region-5 (geom.cu:1487-1490); its read past the last bound is defined here
+ * as one extrapolated bound (the bounds arrays carry N+2 entries).  Q3 the
+ * Compton br loop cap never fires (compton.cu:134-141).  Q5 nbr_split <= 0
+ * ends annih.  Q6 photo's low-energy test against edge_energies[1].  Q7 range
+ * rejection compares total energy with e_max_rr (electron.cu:1523).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/gegs.h"
+#include "oracle_math.h"
+
+#define RM 0.5110034f
+static const double PRM = 0.5110034;
+#define RMT2 (2.0f * RM)
+#define RMSQ (RM * RM)
+#define VACUUM_STEP 1e8f
+#define EPSGMFP 1e-5
+#define EPSEMFP 1e-5
+#define EPSTEP 0.0005f
+#define INFI_DIST 1e5f
+#define VAC_AND_OTH 6
+#define MXMED GEGS_MXMED
+#define MXGE GEGS_MXGE
+#define MXEKE GEGS_MXEKE
+#define MXRAYFF GEGS_MXRAYFF
+#define RAYCDFSIZE GEGS_RAYCDFSIZE
+#define MAXL_MS GEGS_MAXL_MS
+#define MAXQ_MS GEGS_MAXQ_MS
+#define MAXU_MS GEGS_MAXU_MS
+#define LAMBMAX_MS 1e5f
+#define QMIN_MS 1e-3f
+#define QMAX_MS 0.5f
+#define HC_INVERSE 80.65506856998f
+#define TWICE_HC2 0.000307444456f
+#define STACK_MAX 64
+
+enum {
+    P_PHOTON_STEP = 0, P_COMPTON = 1, P_PHOTO = 2, P_RAYLEIGH = 3, P_PAIR = 4,
+    E_ELECTRON_STEP = 5, E_MOLLER = 6, E_BREMS = 7, E_BHABHA = 8, E_ANNIH = 9
+};
+
+typedef struct opart {
+    float e, x, y, z, u, v, w, wt, dnear;
+    uint32_t region;
+    int32_t charge;
+    int32_t status;
+    int32_t module;
+    uint64_t lin;
+    uint32_t blk;
+} opart;
+
+/* ------------------------------------------------------------ Philox */
+static void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c[1] ^ k0;
+        uint32_t n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+static uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+typedef struct ostream {
+    uint64_t lin;
+    uint32_t blk;
+    int pos;
+    int nchild;
+    float buf[4];
+    uint32_t k0, k1;
+} ostream;
+
+static void st_begin(ostream *s, const opart *p, uint64_t seed) {
+    s->lin = p->lin;
+    s->blk = p->blk;
+    s->pos = 4;
+    s->nchild = 0;
+    s->k0 = (uint32_t)seed;
+    s->k1 = (uint32_t)(seed >> 32);
+}
+
+static float rnd(ostream *s) {
+    if (s->pos == 4) {
+        uint32_t c[4] = {s->blk, (uint32_t)s->lin, (uint32_t)(s->lin >> 32), 0x47454753u};
+        philox(c, s->k0, s->k1);
+        for (int i = 0; i < 4; ++i) s->buf[i] = (float)((c[i] >> 8) + 1u) * 5.96046448e-08f;
+        s->blk += 1;
+        s->pos = 0;
+    }
+    return s->buf[s->pos++];
+}
+
+static uint64_t child_lineage(ostream *s) {
+    uint64_t pos = (uint64_t)s->blk * 4u + (uint64_t)s->pos;
+    uint64_t k = (uint64_t)(s->nchild++);
+    return mix64(s->lin ^ mix64((pos << 8) | k));
+}
+
+/* ------------------------------------------------------------ context */
+typedef struct octx {
+    const gegs_media_tables *mt;
+    const gegs_region *reg;
+    const float *xb, *yb, *zb;      /* N+2 entries each */
+    uint32_t nx, ny, nz;
+    gegs_field f;
+    uint64_t seed;
+    double *dose, *dose2, *hist_edep;
+    uint32_t *hist_steps;
+    double *tmp;                     /* per-voxel deposit of the current history */
+    uint64_t *last;                  /* history tag of tmp */
+    uint64_t hist;
+    double cur_edep;
+    uint32_t cur_steps;
+    opart stack[STACK_MAX];
+    int sp;
+    gegs_counters cnt;
+} octx;
+
+static void push(octx *c, const opart *p) {
+    if (c->sp >= STACK_MAX) { c->cnt.overflow++; return; }
+    c->stack[c->sp++] = *p;
+}
+
+/* push the particle under processing to the next event (same stream) */
+static void push_cont(octx *c, ostream *s, opart *p, int status) {
+    p->status = status;
+    p->lin = s->lin;
+    p->blk = s->blk;
+    push(c, p);
+}
+
+/* push a particle created by the current interaction */
+static void push_child(octx *c, ostream *s, opart *p, int status) {
+    p->status = status;
+    p->lin = child_lineage(s);
+    p->blk = 0;
+    push(c, p);
+}
+
+/* escore, scoring.cu:14-24: FP64 tally, regions below 6 are not scored.
+ * History-by-history variance: per-voxel tag of the last history. */
+static void escore(octx *c, uint32_t region, float energy) {
+    if (region < VAC_AND_OTH) return;
+    double tmp = energy;
+    uint32_t v = region - VAC_AND_OTH;
+    c->cnt.deposits++;
+    c->dose[v] += tmp;
+    if (c->last[v] != c->hist + 1) {
+        if (c->last[v] != 0) c->dose2[v] += c->tmp[v] * c->tmp[v];
+        c->tmp[v] = 0.0;
+        c->last[v] = c->hist + 1;
+    }
+    c->tmp[v] += tmp;
+    c->cur_edep += tmp;
+}
+
+/* ------------------------------------------------------------ geometry */
+static void to_phantom(const octx *c, const opart *p0, float *q) {
+    const gegs_field *f = &c->f;
+    float x, y, z, u, v, w;
+    if (f->change_xyz) {
+        x = p0->y; y = p0->z - f->source_to_isocenter; z = p0->x;
+        u = p0->v; v = p0->w; w = p0->u;
+    } else {
+        x = p0->x; y = p0->y; z = p0->z - f->source_to_isocenter;
+        u = p0->u; v = p0->v; w = p0->w;
+    }
+    float tx = x * f->cos_angle - y * f->sin_angle;
+    float ty = x * f->sin_angle + y * f->cos_angle;
+    float tu = u * f->cos_angle - v * f->sin_angle;
+    float tv = u * f->sin_angle + v * f->cos_angle;
+    q[0] = tx + f->iso_x; q[1] = ty + f->iso_y; q[2] = z + f->iso_z;
+    q[3] = tu; q[4] = tv; q[5] = w;
+}
+
+/* isWhere, geom.cu:20-36 */
+static int is_where(float p, uint32_t nreg, const float *b) {
+    if (p < b[0] || p > b[nreg]) return -1;
+    if (nreg == 1) return 0;
+    int ml = 0, mu = (int)nreg;
+    while (mu - ml > 1) {
+        int mav = (ml + mu) / 2;
+        if (p <= b[mav]) mu = mav; else ml = mav;
+    }
+    return mu - 1;
+}
+
+/* howfar_phantom, geom.cu:54-307 */
+static uint32_t howfar_phantom(const octx *c, const opart *p0, float *t) {
+    float q[6];
+    to_phantom(c, p0, q);
+    const float px = q[0], py = q[1], pz = q[2], pu = q[3], pv = q[4], pw = q[5];
+    const uint32_t nx = c->nx, ny = c->ny, nz = c->nz;
+    const float *xb = c->xb, *yb = c->yb, *zb = c->zb;
+    if (p0->region >= VAC_AND_OTH) {
+        uint32_t ir = p0->region - VAC_AND_OTH;
+        int iz = (int)(ir / (nx * ny));
+        ir -= (uint32_t)iz * nx * ny;
+        int iy = (int)(ir / nx);
+        int ix = (int)(ir - (uint32_t)iy * nx);
+        uint32_t inew = p0->region;
+        if (pu > 0.0f) {
+            float d = (xb[ix + 1] - px) / pu;
+            if (d <= *t) { *t = d; if (++ix < (int)nx) inew = p0->region + 1; else inew = 0; }
+        } else if (pu < 0.0f) {
+            float d = (xb[ix] - px) / pu;
+            if (d <= *t) { *t = d; if (--ix >= 0) inew = p0->region - 1; else inew = 0; }
+        }
+        if (pv > 0.0f) {
+            float d = (yb[iy + 1] - py) / pv;
+            if (d <= *t) { *t = d; if (++iy < (int)ny) inew = p0->region + nx; else inew = 0; }
+        } else if (pv < 0.0f) {
+            float d = (yb[iy] - py) / pv;
+            if (d <= *t) { *t = d; if (--iy >= 0) inew = p0->region - nx; else inew = 0; }
+        }
+        if (pw > 0.0f) {
+            float d = (zb[iz + 1] - pz) / pw;
+            if (d <= *t) { *t = d; if (++iz < (int)nz) inew = p0->region + nx * ny; else inew = 0; }
+        } else if (pw < 0.0f) {
+            float d = (zb[iz] - pz) / pw;
+            if (d <= *t) { *t = d; if (--iz >= 0) inew = p0->region - nx * ny; else inew = 0; }
+        }
+        if (*t < 0.0f) *t = EPSTEP;
+        return inew;
+    }
+    int ix, iy, iz;
+    float t1 = 0.0f;
+    ix = -1;
+    if (px <= xb[0] && pu > 0.0f) { t1 = (xb[0] - px) / pu; ix = 0; }
+    else if (px >= xb[nx] && pu < 0.0f) { t1 = (xb[nx] - px) / pu; ix = (int)nx - 1; }
+    if (ix >= 0 && t1 <= *t) {
+        float y1 = py + pv * t1;
+        iy = is_where(y1, ny, yb);
+        if (iy >= 0) {
+            float z1 = pz + pw * t1;
+            iz = is_where(z1, nz, zb);
+            if (iz >= 0) { *t = t1; return (uint32_t)(ix + iy * (int)nx + iz * (int)(nx * ny)) + VAC_AND_OTH; }
+        }
+    }
+    iy = -1;
+    if (py <= yb[0] && pv > 0.0f) { t1 = (yb[0] - py) / pv; iy = 0; }
+    else if (py >= yb[ny] && pv < 0.0f) { t1 = (yb[ny] - py) / pv; iy = (int)ny - 1; }
+    if (iy >= 0 && t1 <= *t) {
+        float x1 = px + pu * t1;
+        ix = is_where(x1, nx, xb);
+        if (ix >= 0) {
+            float z1 = pz + pw * t1;
+            iz = is_where(z1, nz, zb);
+            if (iz >= 0) { *t = t1; return (uint32_t)(ix + iy * (int)nx + iz * (int)(nx * ny)) + VAC_AND_OTH; }
+        }
+    }
+    iz = -1;
+    if (pz <= zb[0] && pw > 0.0f) { t1 = (zb[0] - pz) / pw; iz = 0; }
+    else if (pz >= zb[nz] && pw < 0.0f) { t1 = (zb[nz] - pz) / pw; iz = (int)nz - 1; }
+    if (iz >= 0 && t1 <= *t) {
+        float x1 = px + pu * t1;
+        ix = is_where(x1, nx, xb);
+        if (ix >= 0) {
+            float y1 = py + pv * t1;
+            iy = is_where(y1, ny, yb);
+            if (iy >= 0) { *t = t1; return (uint32_t)(ix + iy * (int)nx + iz * (int)(nx * ny)) + VAC_AND_OTH; }
+        }
+    }
+    return 0;
+}
+
+/* hownear_phantom, geom.cu:1422-1543, with quirk Q1 (region - 5) */
+static float hownear_phantom(const octx *c, const opart *p0) {
+    float q[6];
+    to_phantom(c, p0, q);
+    const float px = q[0], py = q[1], pz = q[2];
+    const uint32_t nx = c->nx, ny = c->ny, nz = c->nz;
+    const float *xb = c->xb, *yb = c->yb, *zb = c->zb;
+    float tperp = INFI_DIST;
+    if (p0->region > 4) {
+        uint32_t ir = p0->region - 1 - 2 - 2;
+        int iz = (int)(ir / (nx * ny));
+        ir -= (uint32_t)iz * nx * ny;
+        int iy = (int)(ir / nx);
+        int ix = (int)(ir - (uint32_t)iy * nx);
+        float a, tx, ty, tz;
+        a = xb[ix + 1] - px; tx = px - xb[ix]; if (a < tx) tx = a;
+        a = yb[iy + 1] - py; ty = py - yb[iy]; if (a < ty) ty = a;
+        a = zb[iz + 1] - pz; tz = pz - zb[iz]; if (a < tz) tz = a;
+        if (tx < ty) tperp = tx; else tperp = ty;
+        if (tperp > tz) tperp = tz;
+    } else {
+        float p1x = xb[0], p1y = yb[0], p1z = zb[0];
+        float p2x = xb[nx], p2y = yb[ny], p2z = zb[nz];
+        if (px - p1x > 0 && px - p2x > 0) p2x = p2x;
+        else if (px - p1x < 0 && px - p2x < 0) p2x = p1x;
+        else p2x = px;
+        if (py - p1y > 0 && py - p2y > 0) p2y = p2y;
+        else if (py - p1y < 0 && py - p2y < 0) p2y = p1y;
+        else p2y = py;
+        if (pz - p1z > 0 && pz - p2z > 0) p2z = p2z;
+        else if (pz - p1z < 0 && pz - p2z < 0) p2z = p1z;
+        else p2z = pz;
+        float dx = px - p2x, dy = py - p2y, dz = pz - p2z;
+        tperp = sqrtf(dx * dx + dy * dy + dz * dz);
+    }
+    return tperp;
+}
+
+static uint32_t howfar(octx *c, const opart *p, float *t) { return howfar_phantom(c, p, t); }
+static float hownear(octx *c, const opart *p) { return hownear_phantom(c, p); }
+
+/* ------------------------------------------------------------ uphi (utility.cu:94-218) */
+static void rotate(float costhe, float sinthe, float cosphi, float sinphi, opart *p) {
+    float A = p->u, B = p->v, C = p->w;
+    float sinps2 = A * A + B * B;
+    if (sinps2 < 1e-20f) {
+        p->u = sinthe * cosphi;
+        p->v = sinthe * sinphi;
+        p->w = C * costhe;
+    } else {
+        float sinpsi = sqrtf(sinps2);
+        float us = sinthe * cosphi;
+        float vs = sinthe * sinphi;
+        float sindel = B / sinpsi;
+        float cosdel = A / sinpsi;
+        p->u = C * cosdel * us - sindel * vs + A * costhe;
+        p->v = C * sindel * us + cosdel * vs + B * costhe;
+        p->w = -sinpsi * us + C * costhe;
+    }
+}
+
+static void uphi21(ostream *s, float costhe, float sinthe, float *cosphi, float *sinphi, opart *p) {
+    float r1 = rnd(s);
+    om_sincos2pi(r1, sinphi, cosphi);
+    rotate(costhe, sinthe, *cosphi, *sinphi, p);
+}
+
+static void uphi32(float costhe, float sinthe, float cosphi, float sinphi, opart *p) {
+    rotate(costhe, sinthe, cosphi, sinphi, p);
+}
+
+/* box method for the azimuth, e.g. electron.cu:793-806 */
+static void select_azimuth(ostream *s, float *cphi, float *sphi) {
+    float xphi, yphi, rhophi2;
+    do {
+        xphi = rnd(s);
+        xphi = 2.0f * xphi - 1.0f;
+        yphi = rnd(s);
+        rhophi2 = xphi * xphi + yphi * yphi;
+    } while (rhophi2 > 1.0f);
+    rhophi2 = 1.0f / rhophi2;
+    *cphi = (xphi * xphi - yphi * yphi) * rhophi2;
+    *sphi = 2.0f * xphi * yphi * rhophi2;
+}
+
+/* ------------------------------------------------------------ annih_at_rest (annihatrest.cu:10-68) */
+static void annih_at_rest(octx *c, ostream *s, opart *p, const gegs_region *rd) {
+    const gegs_media_tables *mt = c->mt;
+    float edep = 0.0f;
+    if (mt->nbr_split > 1) p->wt = p->wt / (float)mt->nbr_split;
+    p->e = (float)PRM;
+    p->charge = 0;
+    for (int ibr = 1; ibr <= mt->nbr_split; ++ibr) {
+        float costhe = rnd(s);
+        costhe = 2.0f * costhe - 1.0f;
+        float sinthe = sqrtf(fmaxf(0.0f, (1.0f - costhe) * (1.0f + costhe)));
+        float cphi, sphi;
+        select_azimuth(s, &cphi, &sphi);
+        p->u = sinthe * cphi;
+        p->v = sinthe * sphi;
+        p->w = costhe;
+        if (p->e < rd->pcut) edep += p->e;
+        else push_child(c, s, p, P_PHOTON_STEP);
+        p->u = -p->u; p->v = -p->v; p->w = -p->w;
+        if (p->e < rd->pcut) edep += p->e;
+        else push_child(c, s, p, P_PHOTON_STEP);
+    }
+    escore(c, p->region, edep * p->wt);
+}
+
+/* ------------------------------------------------------------ photon_step (photon.cu:6-292) */
+static void photon_step(octx *c, ostream *s, opart *p) {
+    const gegs_media_tables *mt = c->mt;
+    gegs_region rd = c->reg[p->region];
+    for (;;) {
+        c->cnt.photon_steps++;
+        c->cur_steps++;
+        rd = c->reg[p->region];
+        if (p->e <= rd.pcut) { escore(c, p->region, p->e * p->wt); return; }
+        if (p->wt <= 0.0f) return;
+        float r1 = rnd(s);
+        float dpmfp = -om_logf(r1);
+        float gle = om_logf(p->e);
+        int lgle = 0;
+        float tstep, gmfp_val = 0.0f, cohfac = 0.0f;
+        uint32_t old_medium = rd.med;
+        if (rd.med == GEGS_VACUUM) {
+            tstep = VACUUM_STEP;
+        } else {
+            gegs_float2 ge = mt->ge[rd.med];
+            lgle = (int)(ge.x + ge.y * gle);
+            gegs_float2 gm = mt->gmfp[rd.med * MXGE + lgle];
+            float gmfpr0 = gm.x + gm.y * gle;
+            gmfp_val = gmfpr0 / rd.rhof;
+            if (rd.flags & GEGS_F_RAYLEIGH) {
+                gegs_float2 ch = mt->cohe[rd.med * MXGE + lgle];
+                cohfac = ch.x + ch.y * gle;
+                gmfp_val *= cohfac;
+            }
+            tstep = gmfp_val * dpmfp;
+        }
+        uint32_t new_region = howfar(c, p, &tstep);
+        int idisc = 0;
+        if (new_region == 0) idisc = (rd.med == GEGS_VACUUM) ? 1 : -1;
+        if (idisc > 0) { p->region = 0; return; }
+        p->x += p->u * tstep;
+        p->y += p->v * tstep;
+        p->z += p->w * tstep;
+        if (rd.med != GEGS_VACUUM) dpmfp = fmaxf(0.0f, dpmfp - tstep / gmfp_val);
+        old_medium = rd.med;
+        if (new_region != p->region) {
+            p->region = new_region;
+            rd = c->reg[new_region];
+            c->cnt.region_gathers++;
+        }
+        if (p->e <= rd.pcut || idisc < 0) { escore(c, p->region, p->e * p->wt); return; }
+        if (rd.med != old_medium || rd.med == GEGS_VACUUM || (double)dpmfp >= EPSGMFP) continue;
+        if (rd.flags & GEGS_F_RAYLEIGH) {
+            r1 = rnd(s);
+            if (r1 < 1.0f - cohfac) { push_cont(c, s, p, P_RAYLEIGH); return; }
+        }
+        gegs_float2 b1 = mt->gbr1[rd.med * MXGE + lgle];
+        gegs_float2 b2 = mt->gbr2[rd.med * MXGE + lgle];
+        float gbr1 = b1.x + b1.y * gle;
+        float gbr2 = b2.x + b2.y * gle;
+        r1 = rnd(s);
+        if (r1 <= gbr1 && p->e > 2.0f * RM) push_cont(c, s, p, P_PAIR);
+        else if (r1 < gbr2) push_cont(c, s, p, P_COMPTON);
+        else push_cont(c, s, p, P_PHOTO);
+        return;
+    }
+}
+
+/* ------------------------------------------------------------ compton (compton.cu:17-492), Klein-Nishina branch */
+static void compton(octx *c, ostream *s, opart *p) {
+    gegs_region rd = c->reg[p->region];
+    float edep = 0.0f;
+    opart p1 = *p;
+    float peig = p->e;
+    float ko = peig / RM;
+    float broi = 1.0f + 2.0f * ko;
+    float bro = 0.0f, br = 0.0f, temp = 0.0f, sinthe = 0.0f, costhe;
+    float alph1 = 0.0f, alpha = 0.0f, rejmax = 0.0f;
+    int first_time = 1;
+    /* bound Compton (ibcmpl > 0) needs the incoh shell data, which the
+     * pegs-lite data set does not carry; regions request KN (flags 0). */
+    do {
+        if (ko > 2.0f) {
+            if (first_time) {
+                alph1 = om_logf(broi);
+                bro = 1.0f / broi;
+                float alph2 = ko * (broi + 1.0f) * bro * bro;
+                alpha = alph1 + alph2;
+            }
+            float aux, rejf3, rnno19;
+            int loops = 0;
+            do {
+                float rnno15 = rnd(s);
+                float rnno16 = rnd(s);
+                if (rnno15 * alpha < alph1) br = om_expf(alph1 * rnno16) * bro;
+                else br = sqrtf(rnno16 * broi * broi + (1.0f - rnno16)) * bro;
+                temp = (1.0f - br) / (ko * br);
+                sinthe = fmaxf(0.0f, temp * (2.0f - temp));
+                aux = 1.0f + br * br;
+                rejf3 = aux - br * sinthe;
+                rnno19 = rnd(s);
+                loops++;
+                if (loops > 100) break;
+            } while (rnno19 * aux > rejf3);
+        } else {
+            if (first_time) {
+                bro = 1.0f / broi;
+                rejmax = broi + bro;
+            }
+            float rnno16, rejf3;
+            int loops = 0;
+            do {
+                float rnno15 = rnd(s);
+                rnno16 = rnd(s);
+                br = bro + (1.0f - bro) * rnno15;
+                temp = (1.0f - br) / (ko * br);
+                sinthe = fmaxf(0.0f, temp * (2.0f - temp));
+                rejf3 = 1.0f + br * br - br * sinthe;
+                if (loops++ > 100) break;
+            } while (rnno16 * br * rejmax > rejf3);
+        }
+        first_time = 0;
+    } while (br < bro || br > 1.0f);
+    costhe = 1.0f - temp;
+    float pesg = br * peig;
+    float pese = (float)((double)(peig - pesg) + PRM);
+    sinthe = sqrtf(sinthe);
+    p->e = pesg;
+    float sinphi, cosphi;
+    uphi21(s, costhe, sinthe, &cosphi, &sinphi, p);
+    if (p->e < rd.pcut) edep += p->e;
+    else push_child(c, s, p, P_PHOTON_STEP);
+    float aux = 1.0f + br * br - 2.0f * br * costhe;
+    if (aux > 1e-8f) {
+        costhe = (1.0f - br * costhe) / sqrtf(aux);
+        sinthe = (1.0f - costhe) * (1.0f + costhe);
+        if (sinthe > 0.0f) sinthe = -sqrtf(sinthe); else sinthe = 0.0f;
+    } else {
+        costhe = 0.0f;
+        sinthe = -1.0f;
+    }
+    p1.e = pese;
+    p1.charge = -1;
+    uphi32(costhe, sinthe, cosphi, sinphi, &p1);
+    if (p1.e < rd.ecut) edep += (float)((double)p1.e - PRM);
+    else push_child(c, s, &p1, E_ELECTRON_STEP);
+    escore(c, p->region, edep * p->wt);
+}
+
+/* ------------------------------------------------------------ photo (photo.cu:1-279), iedgfl = 0 branch */
+static void photo(octx *c, ostream *s, opart *p) {
+    gegs_region rd = c->reg[p->region];
+    float edep = 0.0f;
+    float peig = p->e;
+    /* atomic relaxation (iedgfl != 0) needs the photo_relax/photo_cs data;
+     * regions request the no-relaxation branch */
+    p->e = (float)((double)peig + PRM);
+    p->charge = -1;
+    if (p->e > rd.ecut) {
+        if (rd.flags & GEGS_F_PHOTOELECTRON_ANGLE) {
+            float beta = sqrtf((p->e - RM) * (p->e + RM)) / p->e;
+            float gamma = p->e / RM;
+            float alpha = 0.5f * gamma - 0.5f + 1.0f / gamma;
+            float ratio = beta / alpha;
+            float xi, rnpht2, sinth2, costhe;
+            do {
+                float rnpht = rnd(s);
+                rnpht = 2.0f * rnpht - 1.0f;
+                if (ratio <= 0.2f) {
+                    float fkappa = rnpht + 0.5f * ratio * (1.0f - rnpht) * (1.0f + rnpht);
+                    if (gamma < 100.0f) {
+                        costhe = (beta + fkappa) / (1.0f + beta * fkappa);
+                    } else {
+                        if (fkappa > 0.0f)
+                            costhe = 1.0f - (1.0f - fkappa) * (gamma - 3.0f) /
+                                     (2.0f * (1.0f + fkappa) * (gamma - 1.0f) * (gamma - 1.0f) * (gamma - 1.0f));
+                        else
+                            costhe = (beta + fkappa) / (1.0f + beta * fkappa);
+                    }
+                    xi = (1.0f + beta * fkappa) * gamma * gamma;
+                } else {
+                    xi = gamma * gamma * (1.0f + alpha * (sqrtf(1.0f + ratio * (2.0f * rnpht + ratio)) - 1.0f));
+                    costhe = (1.0f - 1.0f / xi) / beta;
+                }
+                sinth2 = fmaxf(0.0f, (1.0f - costhe) * (1.0f + costhe));
+                rnpht2 = rnd(s);
+            } while (rnpht2 > 0.5f * (1.0f + gamma) * sinth2 * xi / gamma);
+            float sinthe = sqrtf(sinth2);
+            float sinphi, cosphi;
+            uphi21(s, costhe, sinthe, &cosphi, &sinphi, p);
+        }
+        push_child(c, s, p, E_ELECTRON_STEP);
+    } else {
+        edep += (float)((double)p->e - PRM);
+    }
+    escore(c, p->region, edep * p->wt);
+}
+
+/* ------------------------------------------------------------ rayleigh (rayleigh.cu:4-157) */
+static void rayleigh(octx *c, ostream *s, opart *p) {
+    const gegs_media_tables *mt = c->mt;
+    gegs_region rd = c->reg[p->region];
+    float gle = om_logf(p->e);
+    gegs_float2 ge = mt->ge[rd.med];
+    int lgle = (int)(ge.x + ge.y * gle);
+    gegs_float2 pm = mt->pmax[rd.med * MXGE + lgle];
+    float pmax_val = pm.x + pm.y * gle;
+    float xmax = HC_INVERSE * p->e;
+    float costhe = 0.0f, costhe2 = 0.0f;
+    for (;;) {
+        float xv;
+        for (;;) {
+            float r1 = rnd(s);
+            float temp = r1 * pmax_val;
+            int dwi = RAYCDFSIZE - 1;
+            int ibin = (int)(temp * (float)dwi);
+            int ib = (int)mt->i_array[rd.med * RAYCDFSIZE + ibin] - 1;
+            int next_ib = (int)mt->i_array[rd.med * RAYCDFSIZE + ibin + 1] - 1;
+            if (next_ib > ib) {
+                for (;;) {
+                    gegs_rayleigh rdat = mt->rayleigh_data[rd.med * MXRAYFF + ib + 1];
+                    if (temp < rdat.fcum || ib >= RAYCDFSIZE - 2) break;
+                    ib++;
+                }
+            }
+            gegs_rayleigh rdat = mt->rayleigh_data[rd.med * MXRAYFF + ib];
+            temp = (temp - rdat.fcum) * rdat.c_array;
+            xv = rdat.xgrid * om_expf(om_logf(1.0f + temp) * rdat.b_array);
+            if (xv < xmax) break;
+        }
+        float r2 = rnd(s);
+        xv = xv / p->e;
+        costhe = 1.0f - TWICE_HC2 * xv * xv;
+        costhe2 = costhe * costhe;
+        if (2.0f * r2 < 1.0f + costhe2) break;
+    }
+    float sinthe = sqrtf(1.0f - costhe2);
+    float cosphi, sinphi;
+    uphi21(s, costhe, sinthe, &cosphi, &sinphi, p);
+    if (p->e < rd.pcut) escore(c, p->region, p->e * p->wt);
+    else push_child(c, s, p, P_PHOTON_STEP);
+}
+
+/* ------------------------------------------------------------ pair (pair.cu:14-431), iprdst = 1 */
+static void pair(octx *c, ostream *s, opart *p) {
+    const gegs_media_tables *mt = c->mt;
+    gegs_region rd = c->reg[p->region];
+    int medium = rd.med;
+    float edep = 0.0f;
+    opart p1 = *p;
+    float eig = p->e;
+    float ese1, ese2;
+    int iq1, iq2;
+    if (eig <= 2.1f) {
+        float rnno30 = rnd(s);
+        ese2 = (float)(PRM + (double)(0.5f * rnno30 * (eig - 2.0f * RM)));
+        rnno30 = rnd(s);
+        ese1 = eig - ese2;
+        if (rnno30 < 0.5f) { iq1 = -1; iq2 = 1; } else { iq1 = 1; iq2 = -1; }
+    } else {
+        float amax, bmax, aux1;
+        int l, l1;
+        if (eig < 50.0f) {
+            l = 5;
+            l1 = l + 1;
+            float delta = 4.0f * mt->delcm[medium] / eig;
+            if (delta < 1.0f) {
+                amax = mt->dl1[8 * medium + l - 1] + delta * (mt->dl2[8 * medium + l - 1] + delta * mt->dl3[8 * medium + l - 1]);
+                bmax = mt->dl1[8 * medium + l1 - 1] + delta * (mt->dl2[8 * medium + l1 - 1] + delta * mt->dl3[8 * medium + l1 - 1]);
+            } else {
+                float aux2 = om_logf(delta + mt->dl6[8 * medium + l - 1]);
+                amax = mt->dl4[8 * medium + l - 1] + mt->dl5[8 * medium + l - 1] * aux2;
+                bmax = mt->dl4[8 * medium + l1 - 1] + mt->dl5[8 * medium + l1 - 1] * aux2;
+            }
+            aux1 = 1.0f - RMT2 / eig;
+            aux1 = aux1 * aux1;
+            aux1 = aux1 * amax / 3.0f;
+            aux1 = aux1 / (bmax + aux1);
+        } else {
+            l = 7;
+            amax = mt->dl1[8 * medium + l - 1];
+            bmax = mt->dl1[8 * medium + l];
+            aux1 = mt->bpar[2 * medium + 1] * (1.0f - mt->bpar[8 * medium] * RM / eig);   /* Q2 */
+        }
+        float del0 = eig * mt->delcm[medium];
+        float eavail = eig - RMT2;
+        float rnno30, rejmax, rejf, eminus;
+        do {
+            rnno30 = rnd(s);
+            float rnno31 = rnd(s);
+            float br;
+            if (rnno30 > aux1) {
+                br = 0.5f * rnno31;
+                rejmax = bmax;
+                l1 = l + 1;
+            } else {
+                rnno30 = rnd(s);
+                rnno30 = fmaxf(rnno30, rnno31);
+                rnno31 = rnd(s);
+                float b1 = fmaxf(rnno30, rnno31);
+                br = 0.5f * (1.0f - b1);
+                rejmax = amax;
+                l1 = l;
+            }
+            eminus = br * eavail + RM;
+            float eplus = eig - eminus;
+            float delta = del0 / (eminus * eplus);
+            if (delta < 1.0f)
+                rejf = mt->dl1[8 * medium + l1 - 1] + delta * (mt->dl2[8 * medium + l1 - 1] + delta * mt->dl3[8 * medium + l1 - 1]);
+            else
+                rejf = mt->dl4[8 * medium + l1 - 1] + mt->dl5[8 * medium + l1 - 1] * om_logf(delta + mt->dl6[8 * medium + l1 - 1]);
+            rnno30 = rnd(s);
+        } while (rnno30 * rejmax > rejf);
+        ese2 = eminus;
+        ese1 = eig - ese2;
+        rnno30 = rnd(s);
+        if (rnno30 < 0.5f) { iq1 = -1; iq2 = 1; } else { iq1 = 1; iq2 = -1; }
+    }
+    p->e = ese1;
+    p1.e = ese2;
+    p->charge = iq1;
+    p1.charge = iq2;
+    float cosphi = 0.0f, sinphi = 0.0f;
+    for (int ichrg = 1; ichrg <= 2; ++ichrg) {
+        float ese = ichrg == 1 ? ese1 : ese2;
+        float pse = sqrtf(fmaxf(0.0f, (ese - RM) * (ese + RM)));
+        float costhe = rnd(s);
+        costhe = 1.0f - 2.0f * costhe;
+        float sinthe = RM * sqrtf((1.0f - costhe) * (1.0f + costhe)) / (pse * costhe + ese);
+        costhe = (ese * costhe + pse) / (pse * costhe + ese);
+        if (ichrg == 1) {
+            uphi21(s, costhe, sinthe, &cosphi, &sinphi, p);
+            if (p->e < rd.ecut) {
+                edep += (float)((double)p->e - PRM);
+                if (p->charge > 0) annih_at_rest(c, s, p, &rd);
+            } else {
+                push_child(c, s, p, E_ELECTRON_STEP);
+            }
+        } else {
+            sinthe = -sinthe;
+            uphi32(costhe, sinthe, cosphi, sinphi, &p1);
+            if (p1.e < rd.ecut) {
+                edep += (float)((double)p1.e - PRM);
+                if (p1.charge > 0) annih_at_rest(c, s, &p1, &rd);
+            } else {
+                push_child(c, s, &p1, E_ELECTRON_STEP);
+            }
+        }
+    }
+    escore(c, p->region, edep * p->wt);
+}
+
+/* ------------------------------------------------------------ mscat (electron.cu:77-242), spin off */
+static void mscat(const gegs_media_tables *mt, ostream *s, float lambda, float chia2, float q1,
+                  float *cost, float *sint) {
+    if (lambda <= 13.8f) {
+        float sprob = rnd(s);
+        float explambda = om_expf(-lambda);
+        if (sprob < explambda) { *cost = 1.0f; *sint = 0.0f; return; }
+        float wsum = (1.0f + lambda) * explambda;
+        if (sprob < wsum) {
+            float xi = rnd(s);
+            xi = 2.0f * chia2 * xi / (1.0f - xi + chia2);
+            *cost = 1.0f - xi;
+            *sint = sqrtf(xi * (2.0f - xi));
+            return;
+        }
+        if (lambda <= 1.0f) {
+            float wprob = explambda;
+            wsum = explambda;
+            *cost = 1.0f;
+            *sint = 0.0f;
+            int icount = 0;
+            do {
+                icount = icount + 1;
+                if (icount > 20) break;
+                wprob = wprob * lambda / (float)icount;
+                wsum = wsum + wprob;
+                float xi = rnd(s);
+                xi = 2.0f * chia2 * xi / (1.0f - xi + chia2);
+                float cosz = 1.0f - xi;
+                float sinz = xi * (2.0f - xi);
+                if (sinz > 1.0e-20f) {
+                    sinz = sqrtf(sinz);
+                    xi = rnd(s);
+                    float sphi, cphi;
+                    om_sincos2pi(xi, &sphi, &cphi);
+                    *cost = *cost * cosz - *sint * sinz * cphi;
+                    *sint = sqrtf(fmaxf(0.0f, (1.0f - *cost) * (1.0f + *cost)));
+                }
+            } while (wsum <= sprob);
+            return;
+        }
+    }
+    if (lambda <= LAMBMAX_MS) {
+        float llmbda = om_logf(lambda);
+        float ai = llmbda * mt->dllambi;
+        int msi = (int)ai;
+        ai = ai - (float)msi;
+        float xi = rnd(s);
+        if (xi < ai) msi = msi + 1;
+        int msj;
+        if (q1 < QMIN_MS) msj = 0;
+        else if (q1 < QMAX_MS) {
+            float aj = q1 * mt->dqmsi;
+            msj = (int)aj;
+            aj = aj - (float)msj;
+            xi = rnd(s);
+            if (xi < aj) msj = msj + 1;
+        } else msj = MAXQ_MS;
+        float omega2;
+        if (llmbda < 2.2299f)
+            omega2 = chia2 * (lambda + 4.0f) * (1.347006f + llmbda * (0.209364f - llmbda * (0.45525f - llmbda * (0.50142f - 0.081234f * llmbda))));
+        else
+            omega2 = chia2 * (lambda + 4.0f) * (-2.77164f + llmbda * (2.94874f - llmbda * (0.1535754f - llmbda * 0.00552888f)));
+        xi = rnd(s);
+        float ak = xi * (float)MAXU_MS;
+        int msk = (int)ak;
+        ak = ak - (float)msk;
+        int base = (msk * (MAXQ_MS + 1) + msj) * (MAXL_MS + 1) + msi;
+        if (ak > mt->wms[base]) msk = mt->ims[base];
+        int itmp = (msk * (MAXQ_MS + 1) + msj) * (MAXL_MS + 1) + msi;
+        float a = mt->fms[itmp];
+        float u = mt->ums[itmp];
+        float du = mt->ums[((msk + 1) * (MAXQ_MS + 1) + msj) * (MAXL_MS + 1) + msi] - u;
+        xi = rnd(s);
+        if (fabsf(a) < 0.2f) {
+            float x1 = 0.5f * (1.0f - xi) * a;
+            u = u + xi * du * (1.0f + x1 * (1.0f - xi * a));
+        } else {
+            u = u - du / a * (1.0f - sqrtf(1.0f + xi * a * (2.0f + a)));
+        }
+        xi = omega2 * u / (1.0f + 0.5f * omega2 - u);
+        if (xi > 1.99999f) xi = 1.99999f;
+        *cost = 1.0f - xi;
+        *sint = sqrtf(xi * (2.0f - xi));
+        return;
+    }
+    /* lambda above the table: the reference leaves the angle unset; take no
+     * deflection (unreachable for the step sizes the skin depth allows) */
+    *cost = 1.0f;
+    *sint = 0.0f;
+}
+
+/* msdist_pI (electron.cu:618-882), spin off */
+static float msdist_pI(const gegs_media_tables *mt, ostream *s, float eloss, float tustep, opart *p0,
+                       const gegs_region *rd) {
+    int medium = rd->med;
+    float e0 = p0->e - RM;
+    float blccc = mt->blcc[medium];
+    float xcccc = mt->xcc[medium];
+    float e = e0 - 0.5f * eloss;
+    float p2 = e * (e + RMT2);
+    float beta2 = p2 / (p2 + RMSQ);
+    float lambda = tustep * rd->rhof * blccc / beta2;
+    float factor = 1.0f / (1.0f + 0.9784671f * e);
+    float epsilon = eloss / e0;
+    epsilon = epsilon / (1.0f - 0.5f * epsilon);
+    float temp = 0.25f * (1.0f - factor * (1.0f - 0.333333f * factor)) * epsilon * epsilon;
+    lambda = lambda * (1.0f + temp);
+    float chia2 = xcccc / p2 / (4.0f * blccc);
+    lambda = lambda / (1.0f + chia2);
+    float chilog = om_logf(1.0f + 1.0f / chia2);
+    float q1 = 2.0f * chia2 * (chilog * (1.0f + chia2) - 1.0f);
+    float xi = q1 * lambda;
+    float ws, sint;
+    mscat(mt, s, lambda, chia2, xi, &ws, &sint);
+    float cphi, sphi;
+    select_azimuth(s, &cphi, &sphi);
+    float us = sint * cphi;
+    float vs = sint * sphi;
+    float z, r, r2, z2, r2max;
+    if (xi < 0.1f) z = 1.0f - xi * (0.5f - xi * (0.166666667f - 0.041666667f * xi));
+    else z = (1.0f - om_expf(-xi)) / xi;
+    r = 0.5f * sint;
+    r2 = r * r;
+    z2 = z * z;
+    r2max = 1.0f - z2;
+    if (r2max < r2) { r2 = r2max; r = sqrtf(r2); }
+    float ut = r * cphi, vt = r * sphi, wt = z;
+    float ustep = sqrtf(z2 + r2) * tustep;
+    float sint02 = p0->u * p0->u + p0->v * p0->v;
+    if (sint02 > 1e-20f) {
+        float sint0 = sqrtf(sint02);
+        float cphi0 = p0->u / sint0;
+        float sphi0 = p0->v / sint0;
+        float u2p = p0->w * us + sint0 * ws;
+        ws = p0->w * ws - sint0 * us;
+        us = u2p * cphi0 - vs * sphi0;
+        vs = u2p * sphi0 + vs * cphi0;
+        u2p = p0->w * ut + sint0 * wt;
+        wt = p0->w * wt - sint0 * ut;
+        ut = u2p * cphi0 - vt * sphi0;
+        vt = u2p * sphi0 + vt * cphi0;
+    } else {
+        wt = p0->w * wt;
+        ws = p0->w * ws;
+    }
+    p0->x = p0->x + tustep * ut;
+    p0->y = p0->y + tustep * vt;
+    p0->z = p0->z + tustep * wt;
+    p0->u = us;
+    p0->v = vs;
+    p0->w = ws;
+    return ustep;
+}
+
+/* $COMPUTE-ELOSS-G: energy loss over path `step` (electron.cu:1649-1701) */
+static float compute_eloss(const gegs_media_tables *mt, int qel, int medium, float range, float rhof,
+                           float step, float eke, float elke, int lelke) {
+    float de;
+    const gegs_float2 *dedx = mt->dedx + qel * MXEKE * MXMED + medium * MXEKE;
+    const float *rep = mt->range_ep;
+    float tuss = range - rep[(medium * MXEKE + lelke - 1) * 2 + qel] / rhof;
+    if (tuss >= step) {
+        gegs_float2 dd = dedx[lelke - 1];
+        float dedxmid = dd.x + dd.y * elke;
+        float aux = dd.y / dedxmid;
+        de = dedxmid * step * rhof;
+        float fedep = de / eke;
+        de = de * (1.0f - 0.5f * fedep * aux * (1.0f - 0.333333f * fedep * (aux - 1.0f - 0.25f * fedep * (2.0f - aux * (4.0f - aux)))));
+    } else {
+        int lelktmp = lelke;
+        tuss = (range - step) * rhof;
+        if (tuss <= 0.0f) {
+            de = eke - mt->te[medium] * 0.99f;
+        } else {
+            while (tuss < rep[(medium * MXEKE + lelktmp - 1) * 2 + qel]) lelktmp = lelktmp - 1;
+            gegs_float2 ek = mt->eke01[medium];
+            float elktmp = ((float)(lelktmp + 1) - ek.x) / ek.y;
+            float eketmp = mt->e_array[medium * MXEKE + lelktmp];
+            tuss = (rep[(medium * MXEKE + lelktmp) * 2 + qel] - tuss) / rhof;
+            gegs_float2 dd = dedx[lelktmp - 1];
+            float dedxmid = dd.x + dd.y * elktmp;
+            float aux = dd.y / dedxmid;
+            de = dedxmid * tuss * rhof;
+            float fedep = de / eketmp;
+            de = de * (1.0f - 0.5f * fedep * aux * (1.0f - 0.333333f * fedep * (aux - 1.0f - 0.25f * fedep * (2.0f - aux * (4.0f - aux)))));
+            de = de + eke - eketmp;
+        }
+    }
+    return de;
+}
+
+/* $COMPUTE-DRANGE: path from energy eke down to bin edge / ekef (electron.cu:1285-1360) */
+static float drange(const gegs_float2 *dedx, float eke, float elke, float ekef_or_edge, float elkf, int lbin) {
+    float fedep = 1.0f - ekef_or_edge / eke;
+    float elktmp = 0.5f * (elke + elkf + 0.25f * fedep * fedep * (1.0f + fedep * (1.0f + 0.875f * fedep)));
+    gegs_float2 dd = dedx[lbin - 1];
+    float dedxmid = dd.x + dd.y * elktmp;
+    dedxmid = 1.0f / dedxmid;
+    float aux = dd.y * dedxmid;
+    float temp = fedep / (2.0f - fedep);
+    aux = aux * (1.0f + 2.0f * aux) * temp * temp / 6.0f;
+    return fedep * eke * dedxmid * (1.0f + aux);
+}
+
+/* ------------------------------------------------------------ electr_step (electron.cu:888-2622) */
+static void electron_step(octx *c, ostream *s, opart *p) {
+    const gegs_media_tables *mt = c->mt;
+    float elke = 0.0f, demfp = 0.0f, sig0 = 0.0f, total_tstep = 0.0f;
+    int lelke = 0;
+    int compute_tstep = 1;
+    int is_tstep = 1;
+    for (;;) {
+        uint32_t irold = p->region;
+        uint32_t irl = irold;
+        gegs_region rd = c->reg[irl];
+        int medium = rd.med;
+        float rhof = rd.rhof;
+        float edep1;
+        int lelec = p->charge;
+        int qel = (1 + lelec) / 2;
+        float eie = p->e;
+        c->cnt.electron_steps++;
+        c->cur_steps++;
+        if (eie <= rd.ecut) {
+            edep1 = (float)((double)p->e - PRM);
+            escore(c, p->region, edep1 * p->wt);
+            if (p->charge > 0) annih_at_rest(c, s, p, &rd);
+            return;
+        }
+        if (p->wt <= 0.0f) return;
+        float eke = eie - RM;
+        if (is_tstep) {
+            compute_tstep = 1;
+            if (medium != GEGS_VACUUM) {
+                float rnne1 = rnd(s);
+                if (rnne1 == 0.0f) rnne1 = 1.0e-30f;
+                demfp = fmaxf(-om_logf(rnne1), (float)EPSEMFP);
+                elke = om_logf(eke);
+                gegs_float2 ek = mt->eke01[medium];
+                lelke = (int)(ek.x + ek.y * elke);
+                if (mt->sig_ismonotone[medium * 2 + qel]) {
+                    gegs_float2 sd = mt->sig[qel * MXEKE * MXMED + medium * MXEKE + lelke - 1];
+                    gegs_float2 dd = mt->dedx[qel * MXEKE * MXMED + medium * MXEKE + lelke - 1];
+                    float sigf = sd.x + sd.y * elke;
+                    float dedx0 = dd.x + dd.y * elke;
+                    sigf /= dedx0;
+                    sig0 = sigf;
+                } else {
+                    sig0 = mt->sig_e[qel * MXMED + medium];
+                }
+            }
+        }
+        float sig_val = 0.0f, dedx_val = 0.0f, tstep = 0.0f, ustep = 0.0f, tustep = 0.0f;
+        float tvstep = 0.0f, vstep = 0.0f, range = 0.0f, de = 0.0f;
+        float blccl = 0.0f, xccl = 0.0f, beta2 = 0.0f, chia2 = 0.0f, xi = 0.0f;
+        float ekems, tperp = 0.0f;
+        int callhowfar = 0, domultiple = 0, dosingle = 0, callmsdist = 0;
+        const gegs_float2 *dedx = mt->dedx + qel * MXEKE * MXMED + (medium == GEGS_VACUUM ? 0 : medium) * MXEKE;
+        if (medium == GEGS_VACUUM) {
+            tstep = VACUUM_STEP;
+            ustep = tstep;
+            tustep = ustep;
+            callhowfar = 1;
+        } else {
+            gegs_float2 ek = mt->eke01[medium];
+            sig_val = sig0;
+            if (sig_val <= 0.0f) {
+                tstep = VACUUM_STEP;
+                sig0 = 1.0e-15f;
+            } else {
+                if (compute_tstep) {
+                    float total_de = demfp / sig_val;
+                    float ekef = eke - total_de;
+                    if (ekef <= mt->e_array[medium * MXEKE + 0]) {
+                        tstep = VACUUM_STEP;
+                    } else {
+                        float elkef = om_logf(ekef);
+                        int lelkef = (int)(ek.x + ek.y * elkef);
+                        if (lelkef == lelke) {
+                            tstep = drange(dedx, eke, elke, ekef, elkef, lelke);
+                        } else {
+                            float ekei = mt->e_array[medium * MXEKE + lelke - 1];
+                            float elkei = ((float)lelke - ek.x) / ek.y;
+                            float tuss = drange(dedx, eke, elke, ekei, elkei, lelke);
+                            ekei = mt->e_array[medium * MXEKE + lelkef];
+                            elkei = ((float)(lelkef + 1) - ek.x) / ek.y;
+                            tstep = drange(dedx, ekei, elkei, ekef, elkef, lelkef);
+                            tstep = tstep + tuss + mt->range_ep[(medium * MXEKE + lelke - 1) * 2 + qel]
+                                    - mt->range_ep[(medium * MXEKE + lelkef) * 2 + qel];
+                        }
+                    }
+                    total_tstep = tstep;
+                    compute_tstep = 0;
+                }
+                tstep = total_tstep / rhof;
+            }
+            gegs_float2 dd = dedx[lelke - 1];
+            float dedx0 = dd.x + dd.y * elke;
+            dedx_val = rhof * dedx0;
+            gegs_float2 tm = mt->tmxs[medium * MXEKE + lelke - 1];
+            float tmxs_val = tm.x + tm.y * elke;
+            tmxs_val /= rhof;
+            float ekei = mt->e_array[medium * MXEKE + lelke - 1];
+            float elkei = ((float)lelke - ek.x) / ek.y;
+            range = drange(dedx, eke, elke, ekei, elkei, lelke);
+            range = (range + mt->range_ep[(medium * MXEKE + lelke - 1) * 2 + qel]) / rhof;
+            tmxs_val = fminf(tmxs_val, mt->smaxir[medium]);
+            tustep = fminf(tstep, fminf(tmxs_val, range));
+            tperp = hownear(c, p);
+            p->dnear = tperp;
+            if (mt->i_do_rr[medium] == 1 && p->e < mt->e_max_rr[medium]) {
+                if (tperp >= range) {
+                    edep1 = (float)((double)p->e - PRM);
+                    escore(c, p->region, edep1 * p->wt);
+                    if (lelec > 0) annih_at_rest(c, s, p, &rd);
+                    return;
+                }
+            }
+            blccl = rhof * mt->blcc[medium];
+            xccl = rhof * mt->xcc[medium];
+            float p2 = eke * (eke + RMT2);
+            beta2 = p2 / (p2 + RMSQ);
+            float ssmfp = beta2 / blccl;
+            float skindepth = mt->skindepth_for_bca * ssmfp;
+            tustep = fminf(tustep, fmaxf(tperp, skindepth));
+            if (tustep <= tperp && (!mt->exact_bca || tustep > skindepth)) {
+                callhowfar = 0;
+                domultiple = 0;
+                dosingle = 0;
+                callmsdist = 1;
+                de = compute_eloss(mt, qel, medium, range, rhof, tustep, eke, elke, lelke);
+                tvstep = tustep;
+                ustep = msdist_pI(mt, s, de, tustep, p, &rd);
+            } else {
+                callmsdist = 0;
+                /* inexact BCA (exact_bca = 0 in the reference defaults) */
+                dosingle = 0;
+                domultiple = 1;
+                ekems = eke - 0.5f * tustep * dedx_val;
+                float p2m = ekems * (ekems + RMT2);
+                beta2 = p2m / (p2m + RMSQ);
+                chia2 = xccl / (4.0f * blccl * p2m);
+                xi = 0.5f * xccl / p2m / beta2 * tustep;
+                xi = xi * (om_logf(1.0f + 1.0f / chia2) - 1.0f / (1.0f + chia2));
+                if (xi < 0.1f) ustep = tustep * (1.0f - xi * (0.5f - xi * 0.166667f));
+                else ustep = tustep * (1.0f - om_expf(-xi)) / xi;
+                callhowfar = ustep < tperp ? 0 : 1;
+            }
+        }
+        uint32_t irnew = p->region;
+        int idisc = 0;
+        float ustep0 = ustep;
+        if (callhowfar || p->wt <= 0.0f) irnew = howfar(c, p, &ustep);
+        if (irnew == 0) idisc = (rd.med == GEGS_VACUUM) ? 1 : -1;
+        if (idisc > 0) {
+            p->region = 0;
+            return;
+        }
+        if (ustep <= 0.0f) ustep = 0.0f;
+        if (ustep == 0.0f || medium == GEGS_VACUUM) {
+            if (ustep != 0.0f) {
+                vstep = ustep;
+                tvstep = vstep;
+                p->x = p->x + p->u * vstep;
+                p->y = p->y + p->v * vstep;
+                p->z = p->z + p->w * vstep;
+                p->dnear = p->dnear - vstep;
+            }
+            if (p->region != irnew) c->cnt.region_gathers++;
+            p->region = irnew;
+            irl = irnew;
+            rd = c->reg[irl];
+            if (eie <= rd.ecut) {
+                edep1 = (float)((double)p->e - PRM);
+                escore(c, p->region, edep1 * p->wt);
+                if (p->charge > 0) annih_at_rest(c, s, p, &rd);
+                return;
+            }
+            if (ustep != 0.0f && idisc < 0) {
+                edep1 = (float)((double)p->e + (double)p->charge * PRM);
+                escore(c, p->region, edep1 * p->wt);
+                return;
+            }
+            is_tstep = 1;
+            continue;
+        }
+        vstep = ustep;
+        if (callhowfar) {
+            if (vstep < ustep0) {
+                ekems = eke - 0.5f * tustep * vstep / ustep0 * dedx_val;
+                float p2m = ekems * (ekems + RMT2);
+                beta2 = p2m / (p2m + RMSQ);
+                chia2 = xccl / (4.0f * blccl * p2m);
+                xi = 0.5f * xccl / p2m / beta2 * vstep;
+                xi = xi * (om_logf(1.0f + 1.0f / chia2) - 1.0f / (1.0f + chia2));
+                if (xi < 0.1f) {
+                    tvstep = vstep * (1.0f + xi * (0.5f + xi * 0.333333f));
+                } else {
+                    if (xi < 0.999999f) tvstep = -vstep * om_logf(1.0f - xi) / xi;
+                    else return;   /* electron.cu:2031-2035: particle dropped */
+                }
+            } else {
+                tvstep = tustep;
+            }
+            de = compute_eloss(mt, qel, medium, range, rhof, tvstep, eke, elke, lelke);
+        } else {
+            tvstep = tustep;
+            if (!callmsdist) de = compute_eloss(mt, qel, medium, range, rhof, tvstep, eke, elke, lelke);
+        }
+        float save_de = de;
+        float edep = de;
+        float ekef = eke - de;
+        if (!callmsdist) {
+            float sinthe = 0.0f, costhe = 1.0f;
+            if (domultiple) {
+                float lambda = blccl * tvstep / beta2 / (1.0f + chia2);
+                mscat(mt, s, lambda, chia2, xi, &costhe, &sinthe);
+            }
+            p->x = p->x + p->u * vstep;
+            p->y = p->y + p->v * vstep;
+            p->z = p->z + p->w * vstep;
+            if (domultiple || dosingle) {
+                float sinphi, cosphi;
+                uphi21(s, costhe, sinthe, &cosphi, &sinphi, p);
+            }
+        }
+        (void)ekef;
+        p->dnear = p->dnear - vstep;
+        irold = p->region;
+        eie = eie - edep;
+        p->e = eie;
+        escore(c, irold, edep * p->wt);
+        if (irnew == irl && eie <= rd.ecut) {
+            edep1 = (float)((double)p->e - PRM);
+            escore(c, p->region, edep1 * p->wt);
+            if (p->charge > 0) annih_at_rest(c, s, p, &rd);
+            return;
+        }
+        int medold = medium;
+        if (medium != GEGS_VACUUM) {
+            eke = eie - RM;
+            elke = om_logf(eke);
+            gegs_float2 ek = mt->eke01[medium];
+            lelke = (int)(ek.x + ek.y * elke);
+        }
+        if (irnew != irold) {
+            p->region = irnew;
+            irl = irnew;
+            rd = c->reg[irl];
+            medium = rd.med;
+            c->cnt.region_gathers++;
+        }
+        if (eie <= rd.ecut) {
+            edep1 = (float)((double)p->e - PRM);
+            escore(c, p->region, edep1 * p->wt);
+            if (p->charge > 0) annih_at_rest(c, s, p, &rd);
+            return;
+        }
+        if (idisc < 0) {
+            edep1 = (float)((double)p->e + (double)p->charge * PRM);
+            escore(c, p->region, edep1 * p->wt);
+            return;
+        }
+        if (medium != medold) { is_tstep = 1; continue; }
+        demfp = demfp - save_de * sig_val;
+        total_tstep = total_tstep - tvstep * rhof;
+        if (total_tstep < 1e-9f) demfp = 0.0f;
+        if ((double)demfp >= EPSEMFP) { is_tstep = 0; continue; }
+        gegs_float2 sd = mt->sig[qel * MXEKE * MXMED + medium * MXEKE + lelke - 1];
+        gegs_float2 dd = mt->dedx[qel * MXEKE * MXMED + medium * MXEKE + lelke - 1];
+        float sigf = sd.x + sd.y * elke;
+        float dedx0 = dd.x + dd.y * elke;
+        sigf /= dedx0;
+        float sigratio = sigf / sig0;
+        float rfict = rnd(s);
+        if (rfict > sigratio) { is_tstep = 1; continue; }
+        if (lelec < 0) {
+            gegs_float2 b1 = mt->ebr1[medium * MXEKE + lelke - 1];
+            float ebr1 = b1.x + b1.y * elke;
+            float rnno24 = rnd(s);
+            if (rnno24 <= ebr1) { push_cont(c, s, p, E_BREMS); return; }
+            if (p->e <= mt->thmoll[medium] && mt->eii_flag == 0) {
+                if (ebr1 <= 0.0f) { is_tstep = 1; continue; }
+                push_cont(c, s, p, E_BREMS);
+                return;
+            }
+            push_cont(c, s, p, E_MOLLER);
+            return;
+        }
+        gegs_float2 pb1 = mt->pbr1[medium * MXEKE + lelke - 1];
+        float pbr1 = pb1.x + pb1.y * elke;
+        float rnno25 = rnd(s);
+        if (rnno25 < pbr1) { push_cont(c, s, p, E_BREMS); return; }
+        gegs_float2 pb2 = mt->pbr2[medium * MXEKE + lelke - 1];
+        float pbr2 = pb2.x + pb2.y * elke;
+        if (rnno25 < pbr2) push_cont(c, s, p, E_BHABHA);
+        else push_cont(c, s, p, E_ANNIH);
+        return;
+    }
+}
+
+/* ------------------------------------------------------------ moller (moller.cu:16-210) */
+static void moller(octx *c, ostream *s, opart *p) {
+    const gegs_media_tables *mt = c->mt;
+    gegs_region rd = c->reg[p->region];
+    int medium = rd.med;
+    float edep = 0.0f;
+    opart p1 = *p;
+    float eie = p->e;
+    float ekin = (float)((double)eie - PRM);
+    /* EII branch: no shell participates in the pegs-lite data (eii_nsh = 0) */
+    if (ekin <= 2.0f * mt->te[medium]) {
+        edep = (float)((double)p->e - PRM);
+        escore(c, p->region, edep * p->wt);
+        return;
+    }
+    float t0 = ekin / RM;
+    float e0 = t0 + 1.0f;
+    float extrae = eie - mt->thmoll[medium];
+    float e02 = e0 * e0;
+    float g2 = t0 * t0 / e02;
+    float g3 = (2.0f * t0 + 1.0f) / e02;
+    float gmax = 1.0f + 1.25f * g2;
+    float rnno27, rejf4, br, r;
+    do {
+        rnno27 = rnd(s);
+        br = mt->te[medium] / (ekin - extrae * rnno27);
+        r = br / (1.0f - br);
+        rnno27 = rnd(s);
+        rejf4 = 1.0f + g2 * br * br + r * (r - g3);
+        rnno27 = gmax * rnno27;
+    } while (rnno27 > rejf4);
+    float ekse2 = br * ekin;
+    float ese1 = eie - ekse2;
+    float ese2 = (float)((double)ekse2 + PRM);
+    p->e = ese1;
+    p1.e = ese2;
+    float h1 = (float)(((double)eie + PRM) / (double)ekin);
+    float dcosth = (float)((double)h1 * ((double)ese1 - PRM) / ((double)ese1 + PRM));
+    float sinthe = sqrtf(1.0f - dcosth);
+    float costhe = sqrtf(dcosth);
+    float sinphi, cosphi;
+    uphi21(s, costhe, sinthe, &cosphi, &sinphi, p);
+    if (p->e < rd.ecut) edep += (float)((double)p->e - PRM);
+    else push_child(c, s, p, E_ELECTRON_STEP);
+    dcosth = (float)((double)h1 * ((double)ese2 - PRM) / ((double)ese2 + PRM));
+    sinthe = -sqrtf(1.0f - dcosth);
+    costhe = sqrtf(dcosth);
+    uphi32(costhe, sinthe, cosphi, sinphi, &p1);
+    if (p1.e < rd.ecut) edep += (float)((double)p1.e - PRM);
+    else push_child(c, s, &p1, E_ELECTRON_STEP);
+    escore(c, p->region, edep * p->wt);
+}
+
+/* ------------------------------------------------------------ bhabha (bhabha.cu:12-172) */
+static void bhabha(octx *c, ostream *s, opart *p) {
+    const gegs_media_tables *mt = c->mt;
+    gegs_region rd = c->reg[p->region];
+    int medium = rd.med;
+    float edep = 0.0f;
+    opart p1 = *p;
+    float eip = p->e;
+    float ekin = (float)((double)eip - PRM);
+    float t0 = ekin / RM;
+    float e0 = t0 + 1.0f;
+    float yy = 1.0f / (t0 + 2.0f);
+    float e02 = e0 * e0;
+    float beta2 = (e02 - 1.0f) / e02;
+    float ep0 = mt->te[medium] / ekin;
+    float ep0c = 1.0f - ep0;
+    float y2 = yy * yy;
+    float yp = 1.0f - 2.0f * yy;
+    float yp2 = yp * yp;
+    float b4 = yp2 * yp;
+    float b3 = b4 + yp2;
+    float b2 = yp * (3.0f + y2);
+    float b1 = 2.0f - y2;
+    float rnno03, rejf2, br;
+    do {
+        rnno03 = rnd(s);
+        br = ep0 / (1.0f - ep0c * rnno03);
+        rnno03 = rnd(s);
+        rejf2 = 1.0f - beta2 * br * (b1 - br * (b2 - br * (b3 - br * b4)));
+    } while (rnno03 > rejf2);
+    if (br < 0.5f) {
+        p1.charge = -1;
+    } else {
+        p->charge = -1;
+        p1.charge = 1;
+        br = 1.0f - br;
+    }
+    br = fmaxf(br, 0.0f);
+    float ekse2 = br * ekin;
+    float ese1 = eip - ekse2;
+    float ese2 = (float)((double)ekse2 + PRM);
+    p->e = ese1;
+    p1.e = ese2;
+    float h1 = (float)(((double)eip + PRM) / (double)ekin);
+    float dcosth = (float)fmin(1.0, (double)h1 * ((double)ese1 - PRM) / ((double)ese1 + PRM));
+    float sinthe = sqrtf(1.0f - dcosth);
+    float costhe = sqrtf(dcosth);
+    float sinphi, cosphi;
+    uphi21(s, costhe, sinthe, &cosphi, &sinphi, p);
+    if (p->e < rd.ecut) {
+        edep += (float)((double)p->e - PRM);
+        if (p->charge > 0) annih_at_rest(c, s, p, &rd);
+    } else {
+        push_child(c, s, p, E_ELECTRON_STEP);
+    }
+    dcosth = (float)((double)h1 * ((double)ese2 - PRM) / ((double)ese2 + PRM));
+    sinthe = -sqrtf(1.0f - dcosth);
+    costhe = sqrtf(dcosth);
+    uphi32(costhe, sinthe, cosphi, sinphi, &p1);
+    if (p1.e < rd.ecut) {
+        edep += (float)((double)p1.e - PRM);
+        if (p1.charge > 0) annih_at_rest(c, s, &p1, &rd);
+    } else {
+        push_child(c, s, &p1, E_ELECTRON_STEP);
+    }
+    escore(c, p->region, edep * p->wt);
+}
+
+/* ------------------------------------------------------------ brems (brems.cu:36-308), ibrdst = 1 (KM) */
+static void brems(octx *c, ostream *s, opart *p) {
+    const gegs_media_tables *mt = c->mt;
+    gegs_region rd = c->reg[p->region];
+    int medium = rd.med;
+    float edep = 0.0f;
+    opart p1 = *p;
+    if (mt->nbr_split < 1) return;
+    float eie = p->e;
+    int l = eie < 50.0f ? 1 : 3;
+    int l1 = l + 1;
+    float ekin = (float)((double)eie - PRM);
+    float elke = om_logf(ekin);
+    float brmin = mt->ap[medium] / ekin;
+    float waux = elke - mt->log_ap[medium];
+    float a = p->u, b = p->v, cc = p->w;
+    float sinpsi = a * a + b * b, sindel = 0.0f, cosdel = 0.0f;
+    if (sinpsi > 1e-20f) {
+        sinpsi = sqrtf(sinpsi);
+        sindel = b / sinpsi;
+        cosdel = a / sinpsi;
+    }
+    float ztarg = mt->zbrang[medium];
+    float tteie = eie / RM;
+    float beta = sqrtf((tteie - 1.0f) * (tteie + 1.0f)) / tteie;
+    float y2max = 2.0f * beta * (1.0f + beta) * tteie * tteie;
+    float y2maxi = 1.0f / y2max;
+    float z2max = y2max + 1.0f;
+    float z2maxi = sqrtf(z2max);
+    float ese = eie;
+    for (int ibr = 1; ibr <= mt->nbr_split; ++ibr) {
+        float rnno06, rejf, esg;
+        do {
+            rnno06 = rnd(s);
+            float br = brmin * om_expf(rnno06 * waux);
+            esg = ekin * br;
+            ese = eie - esg;
+            float delta = esg / eie / ese * mt->delcm[medium];
+            float aux = ese / eie;
+            float phi1, phi2;
+            if (delta < 1.0f) {
+                phi1 = mt->dl1[medium * 8 + l - 1] + delta * (mt->dl2[medium * 8 + l - 1] + delta * mt->dl3[medium * 8 + l - 1]);
+                phi2 = mt->dl1[medium * 8 + l1 - 1] + delta * (mt->dl2[medium * 8 + l1 - 1] + delta * mt->dl3[medium * 8 + l1 - 1]);
+            } else {
+                phi1 = mt->dl4[medium * 8 + l - 1] + mt->dl5[medium * 8 + l - 1] * om_logf(delta + mt->dl6[medium * 8 + l - 1]);
+                phi2 = phi1;
+            }
+            rejf = (1.0f + aux * aux) * phi1 - 2.0f * aux * phi2 / 3.0f;
+            rnno06 = rnd(s);
+        } while (rnno06 >= rejf);
+        p1.e = esg;
+        p1.charge = 0;
+        p1.wt = p->wt / (float)mt->nbr_split;
+        /* Koch-Motz 2BS angle (brems.cu:191-233) */
+        float y2tst;
+        {
+            float esedei = ese / RM / tteie;
+            float rjarg1 = 1.0f + esedei * esedei;
+            float rjarg2 = rjarg1 + 2.0f * esedei;
+            float aux = 2.0f * ese * tteie / esg;
+            aux = aux * aux;
+            float aux1 = aux * ztarg;
+            float rjarg3;
+            if (aux1 > 10.0f) rjarg3 = mt->lzbrang[medium] + (1.0f - aux1) / (aux1 * aux1);
+            else rjarg3 = om_logf(aux / (1.0f + aux1));
+            float rejmax = rjarg1 * rjarg3 - rjarg2;
+            float rtest, rejtst;
+            do {
+                y2tst = rnd(s);
+                rtest = rnd(s);
+                float aux3 = z2maxi / (y2tst + (1.0f - y2tst) * z2maxi);
+                rtest = rtest * aux3 * rejmax;
+                y2tst = aux3 * aux3 - 1.0f;
+                float y2tst1 = esedei * y2tst / (aux3 * aux3 * aux3 * aux3);
+                float aux4 = 16.0f * y2tst1 - rjarg2;
+                float aux5 = rjarg1 - 4.0f * y2tst1;
+                if (rtest < aux4 + aux5 * rjarg3) break;
+                float aux2 = om_logf(aux / (1.0f + aux1 / (aux3 * aux3 * aux3 * aux3)));
+                rejtst = aux4 + aux5 * aux2;
+            } while (rtest >= rejtst);
+        }
+        float costhe = 1.0f - 2.0f * y2tst * y2maxi;
+        float sinthe = sqrtf(fmaxf((1.0f - costhe) * (1.0f + costhe), 0.0f));
+        float cphi, sphi;
+        select_azimuth(s, &cphi, &sphi);
+        if (sinpsi >= 1e-10f) {
+            float us = sinthe * cphi;
+            float vs = sinthe * sphi;
+            p1.u = cc * cosdel * us - sindel * vs + a * costhe;
+            p1.v = cc * sindel * us + cosdel * vs + b * costhe;
+            p1.w = cc * costhe - sinpsi * us;
+        } else {
+            p1.u = sinthe * cphi;
+            p1.v = sinthe * sphi;
+            p1.w = cc * costhe;
+        }
+        if (p1.e < rd.pcut) edep += p1.e;
+        else push_child(c, s, &p1, P_PHOTON_STEP);
+    }
+    p->e = ese;
+    if (p->e < rd.ecut) edep += (float)((double)p->e - PRM);
+    else push_child(c, s, p, E_ELECTRON_STEP);
+    escore(c, p->region, edep * p->wt);
+}
+
+/* ------------------------------------------------------------ annih (annih.cu:16-206) */
+static void annih(octx *c, ostream *s, opart *p) {
+    const gegs_media_tables *mt = c->mt;
+    gegs_region rd = c->reg[p->region];
+    float edep = 0.0f;
+    if (mt->nbr_split <= 0) return;      /* Q5 */
+    float avip = (float)((double)p->e + PRM);
+    float a = avip / RM;
+    float g = a - 1.0f;
+    float t = g - 1.0f;
+    float pp = sqrtf(a * t);
+    float pot = pp / t;
+    float ep0 = 1.0f / (a + pp);
+    float wsamp = om_logf((1.0f - ep0) / ep0);
+    float aa = p->u, bb = p->v, cc = p->w;
+    float sinpsi = aa * aa + bb * bb, sindel = 0.0f, cosdel = 0.0f;
+    if (sinpsi > 1e-20f) {
+        sinpsi = sqrtf(sinpsi);
+        sindel = bb / sinpsi;
+        cosdel = aa / sinpsi;
+    }
+    if (mt->nbr_split > 1) p->wt = p->wt / (float)mt->nbr_split;
+    for (int ibr = 1; ibr <= mt->nbr_split; ++ibr) {
+        float rnno01, rejf, ep;
+        do {
+            rnno01 = rnd(s);
+            ep = ep0 * om_expf(rnno01 * wsamp);
+            rnno01 = rnd(s);
+            rejf = 1.0f - (ep * a - 1.0f) * (ep * a - 1.0f) / (ep * (a * a - 2.0f));
+        } while (rnno01 > rejf);
+        float esg1 = avip * ep;
+        p->e = esg1;
+        p->charge = 0;
+        float costhe = fminf(1.0f, (esg1 - RM) * pot / esg1);
+        float sinthe = sqrtf(1.0f - costhe * costhe);
+        float cphi, sphi;
+        select_azimuth(s, &cphi, &sphi);
+        if (sinpsi >= 1e-10f) {
+            float us = sinthe * cphi, vs = sinthe * sphi;
+            p->u = cc * cosdel * us - sindel * vs + aa * costhe;
+            p->v = cc * sindel * us + cosdel * vs + bb * costhe;
+            p->w = cc * costhe - sinpsi * us;
+        } else {
+            p->u = sinthe * cphi;
+            p->v = sinthe * sphi;
+            p->w = cc * costhe;
+        }
+        if (p->e < rd.pcut) edep += p->e;
+        else push_child(c, s, p, P_PHOTON_STEP);
+        float esg2 = avip - esg1;
+        p->e = esg2;
+        costhe = fminf(1.0f, (esg2 - RM) * pot / esg2);
+        sinthe = -sqrtf(1.0f - costhe * costhe);
+        if (sinpsi >= 1e-10f) {
+            float us = sinthe * cphi, vs = sinthe * sphi;
+            p->u = cc * cosdel * us - sindel * vs + aa * costhe;
+            p->v = cc * sindel * us + cosdel * vs + bb * costhe;
+            p->w = cc * costhe - sinpsi * us;
+        } else {
+            p->u = sinthe * cphi;
+            p->v = sinthe * sphi;
+            p->w = cc * costhe;
+        }
+        if (p->e < rd.pcut) edep += p->e;
+        else push_child(c, s, p, P_PHOTON_STEP);
+    }
+    escore(c, p->region, edep * p->wt);
+}
+
+/* ------------------------------------------------------------ driver */
+uint64_t oracle_primary_lineage(uint64_t hist) { return mix64(hist ^ 0x5851F42D4C957F2Dull); }
+
+size_t oracle_sizeof_media_tables(void) { return sizeof(gegs_media_tables); }
+
+/*
+ * Transport n source particles, one history each (history index
+ * hist_offset + i).  dose: FP64 energy per voxel; dose2: history-by-history
+ * sum of squared per-history voxel energies.  hist_edep / hist_steps
+ * (optional): per-history energy scored in the phantom and step count.
+ * Bounds arrays carry nx+2, ny+2, nz+2 entries (see Q1 above).
+ */
+int oracle_run(const gegs_media_tables *mt, const gegs_region *regions,
+               uint32_t nx, uint32_t ny, uint32_t nz,
+               const float *xb, const float *yb, const float *zb,
+               const gegs_field *field, const gegs_source *src, uint64_t n,
+               uint64_t hist_offset, uint64_t seed,
+               double *dose, double *dose2, double *hist_edep, uint32_t *hist_steps,
+               gegs_counters *counters) {
+    octx *c = (octx *)calloc(1, sizeof(octx));
+    if (!c) return GEGS_ERR_ARG;
+    size_t nvox = (size_t)nx * ny * nz;
+    c->mt = mt;
+    c->reg = regions;
+    c->xb = xb; c->yb = yb; c->zb = zb;
+    c->nx = nx; c->ny = ny; c->nz = nz;
+    c->f = *field;
+    c->seed = seed;
+    c->dose = dose;
+    c->dose2 = dose2;
+    c->tmp = (double *)calloc(nvox, sizeof(double));
+    c->last = (uint64_t *)calloc(nvox, sizeof(uint64_t));
+    if (!c->tmp || !c->last) { free(c->tmp); free(c->last); free(c); return GEGS_ERR_ARG; }
+    for (uint64_t i = 0; i < n; ++i) {
+        c->hist = hist_offset + i;
+        c->cur_edep = 0.0;
+        c->cur_steps = 0;
+        c->cnt.histories++;
+        opart p;
+        memset(&p, 0, sizeof(p));
+        p.e = src[i].e; p.x = src[i].x; p.y = src[i].y; p.z = src[i].z;
+        p.u = src[i].u; p.v = src[i].v; p.w = src[i].w; p.wt = src[i].wt;
+        p.charge = src[i].charge;
+        p.region = 0;
+        p.module = field->module_first;
+        p.dnear = INFI_DIST;
+        p.lin = oracle_primary_lineage(c->hist);
+        p.blk = 0;
+        p.status = p.charge == 0 ? P_PHOTON_STEP : E_ELECTRON_STEP;
+        c->sp = 0;
+        push(c, &p);
+        while (c->sp > 0) {
+            opart q = c->stack[--c->sp];
+            ostream s;
+            st_begin(&s, &q, seed);
+            c->cnt.interactions[q.status]++;
+            switch (q.status) {
+            case P_PHOTON_STEP: photon_step(c, &s, &q); break;
+            case P_COMPTON: compton(c, &s, &q); break;
+            case P_PHOTO: photo(c, &s, &q); break;
+            case P_RAYLEIGH: rayleigh(c, &s, &q); break;
+            case P_PAIR: pair(c, &s, &q); break;
+            case E_ELECTRON_STEP: electron_step(c, &s, &q); break;
+            case E_MOLLER: moller(c, &s, &q); break;
+            case E_BREMS: brems(c, &s, &q); break;
+            case E_BHABHA: bhabha(c, &s, &q); break;
+            case E_ANNIH: annih(c, &s, &q); break;
+            default: break;
+            }
+        }
+        if (hist_edep) hist_edep[i] = c->cur_edep;
+        if (hist_steps) hist_steps[i] = c->cur_steps;
+    }
+    for (size_t v = 0; v < nvox; ++v)
+        if (c->last[v]) dose2[v] += c->tmp[v] * c->tmp[v];
+    if (counters) *counters = c->cnt;
+    free(c->tmp);
+    free(c->last);
+    free(c);
+    return GEGS_OK;
+}
+
+/* Single-sampler entry points for uniform-for-uniform checks: a sampler is
+ * driven through the same per-particle stream a transport run would give it
+ * (lineage lin, block blk), and the particle records it pushes are returned. */
+int oracle_interaction(const gegs_media_tables *mt, const gegs_region *regions,
+                       uint32_t nx, uint32_t ny, uint32_t nz,
+                       const float *xb, const float *yb, const float *zb,
+                       const gegs_field *field, int status, const float *state9,
+                       uint32_t region, uint64_t lin, uint32_t blk, uint64_t seed,
+                       float *out, int max_out, double *edep_out) {
+    octx *c = (octx *)calloc(1, sizeof(octx));
+    size_t nvox = (size_t)nx * ny * nz;
+    double *dose = (double *)calloc(nvox, sizeof(double));
+    double *dose2 = (double *)calloc(nvox, sizeof(double));
+    c->mt = mt; c->reg = regions; c->xb = xb; c->yb = yb; c->zb = zb;
+    c->nx = nx; c->ny = ny; c->nz = nz; c->f = *field; c->seed = seed;
+    c->dose = dose; c->dose2 = dose2;
+    c->tmp = (double *)calloc(nvox, sizeof(double));
+    c->last = (uint64_t *)calloc(nvox, sizeof(uint64_t));
+    opart p;
+    memset(&p, 0, sizeof(p));
+    p.e = state9[0]; p.x = state9[1]; p.y = state9[2]; p.z = state9[3];
+    p.u = state9[4]; p.v = state9[5]; p.w = state9[6]; p.wt = state9[7];
+    p.charge = (int)state9[8];
+    p.region = region; p.module = field->module_first; p.dnear = INFI_DIST;
+    p.lin = lin; p.blk = blk; p.status = status;
+    ostream s;
+    st_begin(&s, &p, seed);
+    switch (status) {
+    case P_PHOTON_STEP: photon_step(c, &s, &p); break;
+    case P_COMPTON: compton(c, &s, &p); break;
+    case P_PHOTO: photo(c, &s, &p); break;
+    case P_RAYLEIGH: rayleigh(c, &s, &p); break;
+    case P_PAIR: pair(c, &s, &p); break;
+    case E_ELECTRON_STEP: electron_step(c, &s, &p); break;
+    case E_MOLLER: moller(c, &s, &p); break;
+    case E_BREMS: brems(c, &s, &p); break;
+    case E_BHABHA: bhabha(c, &s, &p); break;
+    case E_ANNIH: annih(c, &s, &p); break;
+    default: break;
+    }
+    int n = c->sp < max_out ? c->sp : max_out;
+    for (int i = 0; i < n; ++i) {
+        const opart *q = &c->stack[i];
+        float *o = out + 12 * i;
+        o[0] = q->e; o[1] = q->x; o[2] = q->y; o[3] = q->z; o[4] = q->u; o[5] = q->v; o[6] = q->w;
+        o[7] = q->wt; o[8] = (float)q->charge; o[9] = (float)q->status; o[10] = (float)q->region;
+        o[11] = (float)q->blk;
+    }
+    double tot = 0.0;
+    for (size_t v = 0; v < nvox; ++v) tot += dose[v];
+    *edep_out = tot;
+    free(c->tmp); free(c->last); free(dose); free(dose2); free(c);
+    return n;
+}
+
+/* elementary functions, exported for their accuracy tests */
+float oracle_logf(float x) { return om_logf(x); }
+float oracle_expf(float x) { return om_expf(x); }
+void oracle_sincos2pi(float t, float *s, float *c) { om_sincos2pi(t, s, c); }
+void oracle_philox(uint32_t *ctr, uint32_t k0, uint32_t k1) { philox(ctr, k0, k1); }
+
+/* geometry probes for the index-parity tests */
+uint32_t oracle_howfar_phantom(uint32_t nx, uint32_t ny, uint32_t nz, const float *xb, const float *yb,
+                               const float *zb, const gegs_field *field, const float *pos_dir,
+                               uint32_t region, float *t) {
+    octx c;
+    memset(&c, 0, sizeof(c));
+    c.xb = xb; c.yb = yb; c.zb = zb; c.nx = nx; c.ny = ny; c.nz = nz; c.f = *field;
+    opart p;
+    memset(&p, 0, sizeof(p));
+    p.x = pos_dir[0]; p.y = pos_dir[1]; p.z = pos_dir[2];
+    p.u = pos_dir[3]; p.v = pos_dir[4]; p.w = pos_dir[5];
+    p.region = region;
+    return howfar_phantom(&c, &p, t);
+}
+
+float oracle_hownear_phantom(uint32_t nx, uint32_t ny, uint32_t nz, const float *xb, const float *yb,
+                             const float *zb, const gegs_field *field, const float *pos,
+                             uint32_t region) {
+    octx c;
+    memset(&c, 0, sizeof(c));
+    c.xb = xb; c.yb = yb; c.zb = zb; c.nx = nx; c.ny = ny; c.nz = nz; c.f = *field;
+    opart p;
+    memset(&p, 0, sizeof(p));
+    p.x = pos[0]; p.y = pos[1]; p.z = pos[2];
+    p.region = region;
+    return hownear_phantom(&c, &p);
+}

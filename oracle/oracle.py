@@ -1,0 +1,106 @@
+"""ctypes front end of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, always as the checker or the timed CPU
+baseline, never as part of the product path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+
+def build(force=False):
+    if force or not os.path.exists(LIB):
+        subprocess.check_call(["make", "-s", "-C", HERE, "-B" if force else "all"])
+    return LIB
+
+
+class Counters(ctypes.Structure):
+    _fields_ = [("photon_steps", ctypes.c_uint64), ("electron_steps", ctypes.c_uint64),
+                ("region_gathers", ctypes.c_uint64), ("deposits", ctypes.c_uint64),
+                ("interactions", ctypes.c_uint64 * 10), ("histories", ctypes.c_uint64),
+                ("overflow", ctypes.c_uint64)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = ctypes.CDLL(LIB)
+        vp = ctypes.c_void_p
+        L.oracle_run.restype = ctypes.c_int
+        L.oracle_run.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                 vp, vp, vp, vp, vp, ctypes.c_uint64, ctypes.c_uint64,
+                                 ctypes.c_uint64, vp, vp, vp, vp, vp]
+        L.oracle_interaction.restype = ctypes.c_int
+        L.oracle_interaction.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                         vp, vp, vp, vp, ctypes.c_int, vp, ctypes.c_uint32,
+                                         ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64,
+                                         vp, ctypes.c_int, vp]
+        L.oracle_logf.restype = ctypes.c_float
+        L.oracle_logf.argtypes = [ctypes.c_float]
+        L.oracle_expf.restype = ctypes.c_float
+        L.oracle_expf.argtypes = [ctypes.c_float]
+        L.oracle_sincos2pi.argtypes = [ctypes.c_float, vp, vp]
+        L.oracle_philox.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32]
+        L.oracle_primary_lineage.restype = ctypes.c_uint64
+        L.oracle_primary_lineage.argtypes = [ctypes.c_uint64]
+        L.oracle_sizeof_media_tables.restype = ctypes.c_size_t
+        L.oracle_howfar_phantom.restype = ctypes.c_uint32
+        L.oracle_howfar_phantom.argtypes = [ctypes.c_uint32] * 3 + [vp, vp, vp, vp, vp, ctypes.c_uint32, vp]
+        L.oracle_hownear_phantom.restype = ctypes.c_float
+        L.oracle_hownear_phantom.argtypes = [ctypes.c_uint32] * 3 + [vp, vp, vp, vp, vp, ctypes.c_uint32]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def run(workload, source, seed, hist_offset=0, per_history=False):
+    """Transport ``source`` (SOURCE_DTYPE array) through the workload.
+    Returns dict(dose, dose2, edep, steps, counters)."""
+    L = lib()
+    phan = workload.phantom
+    xb, yb, zb = phan.padded_bounds()
+    n = len(source)
+    src = np.ascontiguousarray(source)
+    dose = np.zeros(phan.nvox, dtype=np.float64)
+    dose2 = np.zeros(phan.nvox, dtype=np.float64)
+    edep = np.zeros(n, dtype=np.float64) if per_history else None
+    steps = np.zeros(n, dtype=np.uint32) if per_history else None
+    cnt = Counters()
+    reg = np.ascontiguousarray(workload.regions)
+    rc = L.oracle_run(ctypes.c_void_p(workload.tables.ptr()), _p(reg), phan.nx, phan.ny, phan.nz,
+                      _p(xb), _p(yb), _p(zb), ctypes.c_void_p(ctypes.addressof(workload.field)),
+                      _p(src), n, hist_offset, seed, _p(dose), _p(dose2), _p(edep), _p(steps),
+                      ctypes.c_void_p(ctypes.addressof(cnt)))
+    if rc != 0:
+        raise RuntimeError("oracle_run failed: %d" % rc)
+    return dict(dose=dose, dose2=dose2, edep=edep, steps=steps, counters=cnt)
+
+
+def interaction(workload, status, state9, region, lin, blk, seed, max_out=64):
+    """Run one event handler on one particle; returns (pushed records (k,12), edep)."""
+    L = lib()
+    phan = workload.phantom
+    xb, yb, zb = phan.padded_bounds()
+    st = np.ascontiguousarray(state9, dtype=np.float32)
+    out = np.zeros((max_out, 12), dtype=np.float32)
+    ed = np.zeros(1, dtype=np.float64)
+    reg = np.ascontiguousarray(workload.regions)
+    k = L.oracle_interaction(ctypes.c_void_p(workload.tables.ptr()), _p(reg), phan.nx, phan.ny, phan.nz,
+                             _p(xb), _p(yb), _p(zb), ctypes.c_void_p(ctypes.addressof(workload.field)),
+                             status, _p(st), region, lin, blk, seed, _p(out), max_out, _p(ed))
+    return out[:k].copy(), float(ed[0])
