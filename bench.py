@@ -1,0 +1,191 @@
+#!/usr/bin/env python3
+"""Benchmark of the coupled electron-photon transport on one MI355X.
+
+Workload: the heterogeneous-CT case -- seeded synthetic CT of 128x128x100
+voxels at 2.5 mm (air / lung / ICRU tissue / ICRP bone through the ctcreate
+ramp), point source at SSD 100 cm, 10x10 cm^2 field, 24-bin 6 MV-like
+spectrum; 1e7 histories by default (10 steps of 1e6).
+
+A step transports one batch of source records that are already resident in
+HBM.  Reported: histories/s of the whole job, dose efficiency 1/(sigma^2 T)
+(sigma = mean relative uncertainty over voxels above 50% of Dmax, from the
+step-to-step variance), the HBM roofline of the transport kernel, and the CPU
+oracle timed on this host's cores on a bounded sample.
+
+  python bench.py [--gpus 1] [--steps 10] [--warmup 1] [--histories-per-step 1000000]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+SOURCE_BYTES = 36          # one gegs_source record
+REGION_BYTES = 16          # one region record gathered on voxel entry
+TABLE_ROW_BYTES = 8        # one float2 table row
+DEPOSIT_BYTES = 16         # FP64 read-modify-write
+# table rows read per transport step (photon: ge, gmfp, cohe; electron: eke01,
+# sig, dedx x2, tmxs, e_array/range rows)
+ROWS_PER_PHOTON_STEP = 3
+ROWS_PER_ELECTRON_STEP = 6
+
+
+def algorithmic_bytes(counters, n_hist):
+    return (SOURCE_BYTES * n_hist + REGION_BYTES * counters["region_gathers"]
+            + TABLE_ROW_BYTES * (ROWS_PER_PHOTON_STEP * counters["photon_steps"]
+                                 + ROWS_PER_ELECTRON_STEP * counters["electron_steps"])
+            + DEPOSIT_BYTES * counters["deposits"])
+
+
+def mean_rel_sigma_from_steps(step_doses):
+    d = np.stack(step_doses)                 # (K, nvox)
+    k = d.shape[0]
+    tot = d.sum(axis=0)
+    var_tot = d.var(axis=0, ddof=1) * k       # variance of the sum of K batches
+    sel = tot > 0.5 * tot.max()
+    return float(np.mean(np.sqrt(var_tot[sel]) / tot[sel])), int(sel.sum())
+
+
+def cpu_baseline(wl, budget_s, threads):
+    from oracle import oracle
+    probe = wl.source(50000 * threads, 991)
+    t0 = time.perf_counter()
+    oracle.run(wl, probe, seed=5, threads=threads)
+    rate0 = len(probe) / (time.perf_counter() - t0)
+    n = int(max(20000, min(50_000_000, rate0 * budget_s)))
+    sample = wl.source(n, 992)
+    t0 = time.perf_counter()
+    oracle.run(wl, sample, seed=6, threads=threads)
+    dt = time.perf_counter() - t0
+    return dict(value=n / dt, unit="histories/s", cores=threads, kind="port",
+                sample="%d histories of the same CT workload (seed 992), oracle/liboracle.so "
+                       "(C restatement, gcc -O2, OpenMP over histories), %.1f s" % (n, dt))
+
+
+def run_method(ctx, method, steps, warmup, b, seed):
+    for w in range(warmup):
+        ctx.run(seed=seed, n_batches=2, method=method, first=w * b, count=b, history_offset=0)
+    step_doses, kernel_ms, counters = [], [], None
+    t0 = time.perf_counter()
+    for k in range(steps):
+        r = ctx.run(seed=seed, n_batches=2, method=method, first=(warmup + k) * b, count=b, history_offset=0)
+        step_doses.append(r["dose"])
+        kernel_ms.append(r["ms"])
+        counters = r["counters"] if counters is None else {
+            key: (counters[key] + r["counters"][key] if not isinstance(r["counters"][key], list)
+                  else [a + c for a, c in zip(counters[key], r["counters"][key])])
+            for key in counters}
+    elapsed = time.perf_counter() - t0
+    return elapsed, step_doses, kernel_ms, counters
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--histories-per-step", type=int, default=1_000_000)
+    ap.add_argument("--methods", default="plain", help="comma list of plain,event")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=20240)
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run a device-to-device copy of this many bytes (counter calibration)")
+    args = ap.parse_args()
+    if args.copy_bytes:
+        import torch
+        a = torch.ones(args.copy_bytes // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": args.copy_bytes, "launches": 3}))
+        return
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(histories shard across GPUs with one FP64 dose reduction; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from gegs_amd import native, workloads
+    wl = workloads.ct()
+    b = args.histories_per_step
+    n_total = (args.steps + args.warmup) * b
+    src = wl.source(n_total, 4242)
+    ctx = native.Context(0)
+    ctx.load_workload(wl)
+    ctx.set_source(src)          # resident in HBM before any timed step
+
+    results = {}
+    for method in [m.strip() for m in args.methods.split(",") if m.strip()]:
+        elapsed, doses, kms, cnt = run_method(ctx, method, args.steps, args.warmup, b, args.seed)
+        hist = args.steps * b
+        sig, nsel = mean_rel_sigma_from_steps(doses)
+        eff = 1.0 / (sig * sig * elapsed)
+        launches = 2 * args.steps
+        kernel_s = sum(kms) / 1e3
+        abytes = algorithmic_bytes(cnt, hist)
+        achieved = abytes / launches / (kernel_s / launches) / 1e9
+        results[method] = dict(value=hist / elapsed, ms_per_step=elapsed / args.steps * 1e3,
+                               sigma=sig, voxels=nsel, efficiency=eff, kernel_ms=kernel_s * 1e3,
+                               counters=cnt, achieved=achieved, bytes=abytes)
+
+    best = max(results, key=lambda m: results[m]["efficiency"])
+    r = results[best]
+    cpu = None
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        cpu = cpu_baseline(wl, args.cpu_seconds, threads)
+    line = {
+        "metric": "histories/s + dose eff. 1/(sigma^2 T) at 1% unc., batched vs plain; adjoint grad/s",
+        "value": r["value"],
+        "unit": "histories/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": "ct_128x128x100_2.5mm_6MV_point_source_SSD100_10x10",
+            "histories_per_step": b,
+            "histories": args.steps * b,
+            "method": best,
+            "ecut_mev": 0.7, "pcut_mev": 0.01,
+            "tallies": "fp64",
+        },
+        "dose_efficiency": r["efficiency"],
+        "dose_efficiency_unit": "1/(sigma^2 s), sigma = mean relative uncertainty over D > 50% Dmax",
+        "mean_rel_sigma": r["sigma"],
+        "histories_for_1pct": args.steps * b * (r["sigma"] / 0.01) ** 2,
+        "methods": {m: dict(histories_per_s=v["value"], dose_efficiency=v["efficiency"],
+                            ms_per_step=v["ms_per_step"]) for m, v in results.items()},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": r["achieved"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": r["achieved"] / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "plain_kernel" if best == "plain" else "event kernels",
+            "algorithmic_bytes_per_history": r["bytes"] / (args.steps * b),
+        },
+        "cpu_baseline": cpu,
+    }
+    if "plain" in results and "event" in results:
+        line["batched_vs_plain_efficiency"] = results["event"]["efficiency"] / results["plain"]["efficiency"]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -88,7 +88,7 @@ gegs_ctx *gegs_create(int device) {
     gegs_ctx *c = new gegs_ctx();
     c->device = device;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc(&c->d_next, sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&c->d_next, 4096 * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&c->d_counters, 16 * sizeof(unsigned long long)) != hipSuccess) {
         delete c;
         return nullptr;
@@ -211,7 +211,8 @@ static int prepare_tallies(gegs_ctx *c) {
     hipLaunchKernelGGL(fill_zero, dim3(blocks), dim3(256), 0, c->stream, c->d_chunk, c->nvox);
     hipLaunchKernelGGL(fill_zero, dim3(blocks), dim3(256), 0, c->stream, c->d_dose, c->nvox);
     hipLaunchKernelGGL(fill_zero, dim3(blocks), dim3(256), 0, c->stream, c->d_dose2, c->nvox);
-    HIPCHECK(c, hipMemsetAsync(c->d_counters, 0, 16 * sizeof(unsigned long long), c->stream), "clear counters");
+    hipLaunchKernelGGL(fill_zero_u64, dim3(1), dim3(64), 0, c->stream, c->d_counters, (size_t)16);
+    hipLaunchKernelGGL(fill_zero_u32, dim3(16), dim3(256), 0, c->stream, c->d_next, (size_t)4096);
     if (c->hist_tally) {
         if (c->hist_n < c->nsrc) {
             hipFree(c->d_hist_edep);
@@ -266,6 +267,8 @@ static int check_ready(gegs_ctx *c, const gegs_run_options *o) {
     if (!c->have_media || !c->have_phantom || !c->have_field) return fail(c, GEGS_ERR_STATE, "run: load media, phantom and field first");
     if (c->nsrc == 0) return fail(c, GEGS_ERR_STATE, "run: no source particles");
     if (o->n_batches < 2 || o->n_batches > 4096) return fail(c, GEGS_ERR_ARG, "run: n_batches must be in [2, 4096]");
+    if (o->first >= c->nsrc || (o->count && o->first + o->count > c->nsrc))
+        return fail(c, GEGS_ERR_ARG, "run: record range outside the loaded source");
     return GEGS_OK;
 }
 
@@ -287,7 +290,6 @@ int gegs_run_plain(gegs_ctx *c, const gegs_run_options *o, double *dose, double 
     a.env = make_env(c, o->seed);
     a.src = c->d_src;
     a.hist_offset = o->history_offset;
-    a.next = c->d_next;
     a.stack = c->d_stack;
     a.depth = depth;
     a.dose = c->d_chunk;
@@ -299,10 +301,11 @@ int gegs_run_plain(gegs_ctx *c, const gegs_run_options *o, double *dose, double 
     hipEventCreate(&t1);
     hipEventRecord(t0, c->stream);
     const uint32_t nb = o->n_batches;
+    const uint64_t r0 = o->first, nr = o->count ? o->count : c->nsrc - o->first;
     for (uint32_t b = 0; b < nb; ++b) {
-        a.first = (uint32_t)((c->nsrc * b) / nb);
-        a.last = (uint32_t)((c->nsrc * (b + 1)) / nb);
-        HIPCHECK(c, hipMemcpyAsync(c->d_next, &a.first, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream), "set counter");
+        a.first = (uint32_t)(r0 + (nr * b) / nb);
+        a.last = (uint32_t)(r0 + (nr * (b + 1)) / nb);
+        a.next = c->d_next + b;
         hipLaunchKernelGGL(plain_kernel, dim3(grid), dim3(block), 0, c->stream, a);
         hipLaunchKernelGGL(fold_batch, dim3(2048), dim3(256), 0, c->stream, c->d_chunk, c->d_dose, c->d_dose2, c->nvox);
     }
@@ -328,9 +331,10 @@ int gegs_run_event(gegs_ctx *c, const gegs_run_options *o, double *dose, double 
     hipError_t e = c->banks.reserve(c->nsrc, nb, cfg);
     if (e != hipSuccess) return fail(c, GEGS_ERR_DEVICE, "event banks", e);
     hipEventRecord(t0, c->stream);
+    const uint64_t r0 = o->first, nr = o->count ? o->count : c->nsrc - o->first;
     for (uint32_t b = 0; b < nb; ++b) {
-        const uint32_t first = (uint32_t)((c->nsrc * b) / nb);
-        const uint32_t last = (uint32_t)((c->nsrc * (b + 1)) / nb);
+        const uint32_t first = (uint32_t)(r0 + (nr * b) / nb);
+        const uint32_t last = (uint32_t)(r0 + (nr * (b + 1)) / nb);
         e = run_event_batch(c->banks, env, c->d_src, first, last, o->history_offset, c->d_chunk,
                             c->hist_tally ? c->d_hist_edep : nullptr, c->hist_tally ? c->d_hist_steps : nullptr,
                             c->d_counters, cfg, c->stream);
@@ -340,6 +344,37 @@ int gegs_run_event(gegs_ctx *c, const gegs_run_options *o, double *dose, double 
     hipEventRecord(t1, c->stream);
     HIPCHECK(c, hipGetLastError(), "launch event kernels");
     return finish_run(c, dose, dose2, t0, t1);
+}
+
+int gegs_sample_events(gegs_ctx *c, const gegs_particle *in, uint32_t n, uint64_t seed, gegs_particle *out,
+                       uint32_t max_out, uint32_t *n_out, double *edep) {
+    if (!c || !in || !out || !n_out || !edep || n == 0 || max_out == 0)
+        return fail(c, GEGS_ERR_ARG, "gegs_sample_events: bad argument");
+    if (!c->have_media || !c->have_phantom || !c->have_field) return fail(c, GEGS_ERR_STATE, "gegs_sample_events: load first");
+    for (uint32_t i = 0; i < n; ++i) {
+        if (in[i].status < 0 || in[i].status >= N_EVENTS) return fail(c, GEGS_ERR_ARG, "gegs_sample_events: bad status");
+        if ((size_t)in[i].region >= c->nvox + VAC_AND_OTH) return fail(c, GEGS_ERR_ARG, "gegs_sample_events: bad region");
+    }
+    hipSetDevice(c->device);
+    gegs_particle *d_in = nullptr, *d_out = nullptr;
+    uint32_t *d_n = nullptr;
+    double *d_e = nullptr;
+    HIPCHECK(c, hipMalloc(&d_in, n * sizeof(gegs_particle)), "hipMalloc sample");
+    HIPCHECK(c, hipMalloc(&d_out, (size_t)n * max_out * sizeof(gegs_particle)), "hipMalloc sample");
+    HIPCHECK(c, hipMalloc(&d_n, n * sizeof(uint32_t)), "hipMalloc sample");
+    HIPCHECK(c, hipMalloc(&d_e, n * sizeof(double)), "hipMalloc sample");
+    HIPCHECK(c, hipMemcpy(d_in, in, n * sizeof(gegs_particle), hipMemcpyHostToDevice), "copy sample");
+    Env env = make_env(c, seed);
+    hipLaunchKernelGGL(sample_kernel, dim3((n + 127) / 128), dim3(128), 0, c->stream, env, d_in, n, d_out, max_out, d_n, d_e);
+    HIPCHECK(c, hipStreamSynchronize(c->stream), "sample kernel");
+    HIPCHECK(c, hipMemcpy(out, d_out, (size_t)n * max_out * sizeof(gegs_particle), hipMemcpyDeviceToHost), "read sample");
+    HIPCHECK(c, hipMemcpy(n_out, d_n, n * sizeof(uint32_t), hipMemcpyDeviceToHost), "read sample");
+    HIPCHECK(c, hipMemcpy(edep, d_e, n * sizeof(double), hipMemcpyDeviceToHost), "read sample");
+    (void)hipFree(d_in);
+    (void)hipFree(d_out);
+    (void)hipFree(d_n);
+    (void)hipFree(d_e);
+    return GEGS_OK;
 }
 
 int gegs_history_tally(gegs_ctx *c, double *edep, uint32_t *steps) {

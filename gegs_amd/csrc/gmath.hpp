@@ -17,11 +17,11 @@
 
 namespace gegs {
 
-__host__ __device__ inline float pow2i(int k) {
+__host__ __device__ __forceinline__ float pow2i(int k) {
     return k >= -126 ? __uint_as_float((uint32_t)(k + 127) << 23) : __uint_as_float(1u << (k + 149));
 }
 
-__host__ __device__ inline float glog(float x) {
+__host__ __device__ __forceinline__ float glog(float x) {
     uint32_t bits = __float_as_uint(x);
     int k = (int)((bits >> 23) & 0xffu) - 127;
     if (k == -127) {
@@ -46,7 +46,7 @@ __host__ __device__ inline float glog(float x) {
     return kf * 0.693145752f + (lnm + kf * 1.42860677e-06f);
 }
 
-__host__ __device__ inline float gexp(float x) {
+__host__ __device__ __forceinline__ float gexp(float x) {
     if (x > 88.7f) return __uint_as_float(0x7f800000u);
     if (x < -103.9f) return 0.0f;
     const float kf = floorf(x * 1.44269504f + 0.5f);
@@ -73,7 +73,7 @@ __host__ __device__ inline float gexp(float x) {
     return s * pow2i(k);
 }
 
-__host__ __device__ inline void gsincos2pi(float t, float &sn, float &cs) {
+__host__ __device__ __forceinline__ void gsincos2pi(float t, float &sn, float &cs) {
     float y = t - floorf(t);
     const float q = floorf(y * 8.0f + 0.5f);
     y = y - q * 0.125f;

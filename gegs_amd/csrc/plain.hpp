@@ -20,10 +20,10 @@ struct StackSink {
     int depth;
     uint32_t overflow;
 
-    __device__ float *at(int slot, int word) const {
+    __device__ __forceinline__ float *at(int slot, int word) const {
         return base + ((size_t)slot * PART_WORDS + word) * stride + tid;
     }
-    __device__ void push(const Part &p) {
+    __device__ __forceinline__ void push(const Part &p) {
         if (sp >= depth) {
             overflow++;
             return;
@@ -44,7 +44,7 @@ struct StackSink {
         *at(k, 12) = __uint_as_float((uint32_t)(p.lin >> 32));
         *at(k, 13) = __uint_as_float(p.blk);
     }
-    __device__ Part pop() {
+    __device__ __forceinline__ Part pop() {
         const int k = --sp;
         Part p;
         p.e = *at(k, 0);
@@ -72,7 +72,7 @@ struct PlainArgs {
     const gegs_source *src;
     uint32_t first, last;        // source records [first, last)
     uint64_t hist_offset;        // global index of record 0
-    uint32_t *next;              // work counter, starts at first
+    uint32_t *next;              // work counter of this launch, starts at 0
     float *stack;
     int depth;
     double *dose;
@@ -81,7 +81,7 @@ struct PlainArgs {
     unsigned long long *counters;   // gegs_counters layout
 };
 
-__device__ inline Part source_particle(const gegs_source &r, int module) {
+__device__ __forceinline__ Part source_particle(const gegs_source &r, int module) {
     Part p;
     p.e = r.e; p.x = r.x; p.y = r.y; p.z = r.z;
     p.u = r.u; p.v = r.v; p.w = r.w; p.wt = r.wt;
@@ -109,8 +109,9 @@ __global__ void __launch_bounds__(128) plain_kernel(PlainArgs a) {
     for (int k = 0; k < N_EVENTS; ++k) pops[k] = 0;
     uint32_t nhist = 0;
     for (;;) {
-        const uint32_t i = atomicAdd(a.next, 1u);
-        if (i >= a.last) break;
+        const uint32_t k = atomicAdd(a.next, 1u);
+        if (k >= a.last - a.first) break;
+        const uint32_t i = a.first + k;
         Part p = source_particle(a.src[i], a.env.f.module_first);
         p.lin = primary_lineage(a.hist_offset + i);
         t.hist_edep = 0.0;
@@ -140,6 +141,44 @@ __global__ void __launch_bounds__(128) plain_kernel(PlainArgs a) {
     atomicAdd(c + 15, (unsigned long long)sink.overflow);
 }
 
+// sampler check: one handler invocation per input particle, pushes listed
+struct ListSink {
+    gegs_particle *out;
+    uint32_t max_out;
+    uint32_t n;
+    __device__ __forceinline__ void push(const Part &p) {
+        if (n < max_out) {
+            gegs_particle &o = out[n];
+            o.e = p.e; o.x = p.x; o.y = p.y; o.z = p.z; o.u = p.u; o.v = p.v; o.w = p.w;
+            o.wt = p.wt; o.dnear = p.dnear; o.region = p.region; o.charge = p.charge;
+            o.status = p.status; o.lin = p.lin; o.blk = p.blk; o.pad = 0;
+        }
+        n++;
+    }
+};
+
+__global__ void sample_kernel(Env env, const gegs_particle *in, uint32_t n, gegs_particle *out, uint32_t max_out,
+                              uint32_t *n_out, double *edep) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const gegs_particle r = in[i];
+    Part p;
+    p.e = r.e; p.x = r.x; p.y = r.y; p.z = r.z; p.u = r.u; p.v = r.v; p.w = r.w; p.wt = r.wt;
+    p.dnear = r.dnear; p.region = r.region; p.charge = r.charge; p.status = r.status;
+    p.module = env.f.module_first; p.lin = r.lin; p.blk = r.blk;
+    ListSink sink{out + (size_t)i * max_out, max_out, 0};
+    Tally t;
+    t.dose = nullptr;
+    t.hist_edep = 0.0;
+    t.steps = 0;
+    t.photon_steps = t.electron_steps = t.gathers = t.deposits = 0;
+    Stream s;
+    s.begin(p, env.seed);
+    handle(env, s, p, sink, t);
+    n_out[i] = sink.n;
+    edep[i] = t.hist_edep;
+}
+
 // batch boundary: fold the batch tally into the run totals and clear it
 __global__ void fold_batch(double *chunk, double *dose, double *dose2, size_t n) {
     for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x) {
@@ -150,6 +189,16 @@ __global__ void fold_batch(double *chunk, double *dose, double *dose2, size_t n)
             chunk[v] = 0.0;
         }
     }
+}
+
+__global__ void fill_zero_u32(uint32_t *p, size_t n) {
+    for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x)
+        p[v] = 0u;
+}
+
+__global__ void fill_zero_u64(unsigned long long *p, size_t n) {
+    for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x)
+        p[v] = 0ull;
 }
 
 __global__ void fill_zero(double *p, size_t n) {

@@ -75,7 +75,7 @@ struct Part {
 };
 
 // ------------------------------------------------------------------ Philox
-__host__ __device__ inline void philox10(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3,
+__host__ __device__ __forceinline__ void philox10(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3,
                                          uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
@@ -92,14 +92,14 @@ __host__ __device__ inline void philox10(uint32_t &c0, uint32_t &c1, uint32_t &c
     }
 }
 
-__host__ __device__ inline uint64_t splitmix(uint64_t z) {
+__host__ __device__ __forceinline__ uint64_t splitmix(uint64_t z) {
     z += 0x9E3779B97F4A7C15ull;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return z ^ (z >> 31);
 }
 
-__host__ __device__ inline uint64_t primary_lineage(uint64_t hist) {
+__host__ __device__ __forceinline__ uint64_t primary_lineage(uint64_t hist) {
     return splitmix(hist ^ 0x5851F42D4C957F2Dull);
 }
 
@@ -111,7 +111,7 @@ struct Stream {
     float b0, b1, b2, b3;
     uint32_t k0, k1;
 
-    __device__ void begin(const Part &p, uint64_t seed) {
+    __device__ __forceinline__ void begin(const Part &p, uint64_t seed) {
         lin = p.lin;
         blk = p.blk;
         pos = 4;
@@ -119,7 +119,7 @@ struct Stream {
         k0 = (uint32_t)seed;
         k1 = (uint32_t)(seed >> 32);
     }
-    __device__ float next() {
+    __device__ __forceinline__ float next() {
         if (pos == 4) {
             uint32_t c0 = blk, c1 = (uint32_t)lin, c2 = (uint32_t)(lin >> 32), c3 = 0x47454753u;
             philox10(c0, c1, c2, c3, k0, k1);
@@ -134,14 +134,14 @@ struct Stream {
         const int k = pos++;
         return k == 0 ? b0 : (k == 1 ? b1 : (k == 2 ? b2 : b3));
     }
-    __device__ uint64_t child() {
+    __device__ __forceinline__ uint64_t child() {
         const uint64_t at = (uint64_t)blk * 4u + (uint64_t)pos;
         const uint64_t k = (uint64_t)(nchild++);
         return lin ^ splitmix((at << 8) | k);
     }
 };
 
-__device__ inline uint64_t child_lineage(Stream &s) { return splitmix(s.child()); }
+__device__ __forceinline__ uint64_t child_lineage(Stream &s) { return splitmix(s.child()); }
 
 // ------------------------------------------------------------------ environment
 struct Env {
@@ -159,17 +159,17 @@ struct Tally {
     uint32_t steps;
     uint32_t photon_steps, electron_steps, gathers, deposits;
 
-    __device__ void score(uint32_t region, float energy) {
+    __device__ __forceinline__ void score(uint32_t region, float energy) {
         if (region < VAC_AND_OTH) return;
         const double v = (double)energy;
-        unsafeAtomicAdd(dose + (region - VAC_AND_OTH), v);
+        if (dose) unsafeAtomicAdd(dose + (region - VAC_AND_OTH), v);
         hist_edep += v;
         deposits++;
     }
 };
 
 template <class Sink>
-__device__ inline void forward(Sink &sink, Stream &s, Part &p, int status) {
+__device__ __forceinline__ void forward(Sink &sink, Stream &s, Part &p, int status) {
     p.status = status;
     p.lin = s.lin;
     p.blk = s.blk;
@@ -177,7 +177,7 @@ __device__ inline void forward(Sink &sink, Stream &s, Part &p, int status) {
 }
 
 template <class Sink>
-__device__ inline void create(Sink &sink, Stream &s, Part &p, int status) {
+__device__ __forceinline__ void create(Sink &sink, Stream &s, Part &p, int status) {
     p.status = status;
     p.lin = child_lineage(s);
     p.blk = 0;
@@ -189,7 +189,7 @@ struct Local {
     float x, y, z, u, v, w;
 };
 
-__device__ inline Local phantom_frame(const gegs_field &f, const Part &p) {
+__device__ __forceinline__ Local phantom_frame(const gegs_field &f, const Part &p) {
     float x, y, z, u, v, w;
     if (f.change_xyz) {
         x = p.y; y = p.z - f.source_to_isocenter; z = p.x;
@@ -212,7 +212,7 @@ __device__ inline Local phantom_frame(const gegs_field &f, const Part &p) {
     return q;
 }
 
-__device__ inline int locate(float p, uint32_t n, const float *b) {
+__device__ __forceinline__ int locate(float p, uint32_t n, const float *b) {
     if (p < b[0] || p > b[n]) return -1;
     if (n == 1) return 0;
     int lo = 0, hi = (int)n;
@@ -225,7 +225,7 @@ __device__ inline int locate(float p, uint32_t n, const float *b) {
 
 // distance to the next voxel face along the flight direction; returns the
 // region entered (0 = outside) and shortens t (geom.cu:54-307)
-__device__ inline uint32_t howfar_phantom(const Env &env, const Part &p, float &t) {
+__device__ __forceinline__ uint32_t howfar_phantom(const Env &env, const Part &p, float &t) {
     const Local q = phantom_frame(env.f, p);
     const uint32_t nx = env.nx, ny = env.ny, nz = env.nz, nxy = nx * ny;
     const float *xb = env.xb, *yb = env.yb, *zb = env.zb;
@@ -297,7 +297,7 @@ __device__ inline uint32_t howfar_phantom(const Env &env, const Part &p, float &
 
 // perpendicular distance to the closest face; keeps the reference's
 // region-5 voxel offset (geom.cu:1487-1490), reading the neighbouring voxel
-__device__ inline float hownear_phantom(const Env &env, const Part &p) {
+__device__ __forceinline__ float hownear_phantom(const Env &env, const Part &p) {
     const Local q = phantom_frame(env.f, p);
     const uint32_t nx = env.nx, ny = env.ny, nz = env.nz;
     const float *xb = env.xb, *yb = env.yb, *zb = env.zb;
@@ -325,7 +325,7 @@ __device__ inline float hownear_phantom(const Env &env, const Part &p) {
 }
 
 // ------------------------------------------------------------------ rotations (utility.cu:94-218)
-__device__ inline void turn(float costhe, float sinthe, float cosphi, float sinphi, Part &p) {
+__device__ __forceinline__ void turn(float costhe, float sinthe, float cosphi, float sinphi, Part &p) {
     const float A = p.u, B = p.v, C = p.w;
     const float s2 = A * A + B * B;
     if (s2 < 1e-20f) {
@@ -342,13 +342,13 @@ __device__ inline void turn(float costhe, float sinthe, float cosphi, float sinp
     p.w = -sp * us + C * costhe;
 }
 
-__device__ inline void uphi21(Stream &s, float costhe, float sinthe, float &cosphi, float &sinphi, Part &p) {
+__device__ __forceinline__ void uphi21(Stream &s, float costhe, float sinthe, float &cosphi, float &sinphi, Part &p) {
     const float r = s.next();
     gsincos2pi(r, sinphi, cosphi);
     turn(costhe, sinthe, cosphi, sinphi, p);
 }
 
-__device__ inline void azimuth_box(Stream &s, float &cphi, float &sphi) {
+__device__ __forceinline__ void azimuth_box(Stream &s, float &cphi, float &sphi) {
     float xp, yp, r2;
     do {
         xp = s.next();
@@ -361,11 +361,11 @@ __device__ inline void azimuth_box(Stream &s, float &cphi, float &sphi) {
     sphi = 2.0f * xp * yp * r2;
 }
 
-__device__ inline float to_kinetic(float etot) { return (float)((double)etot - PRM); }
+__device__ __forceinline__ float to_kinetic(float etot) { return (float)((double)etot - PRM); }
 
 // ------------------------------------------------------------------ annihilation at rest
 template <class Sink>
-__device__ void annih_at_rest(const Env &env, Stream &s, Part &p, const gegs_region &rd, Sink &sink, Tally &t) {
+__device__ __forceinline__ void annih_at_rest(const Env &env, Stream &s, Part &p, const gegs_region &rd, Sink &sink, Tally &t) {
     const int nsplit = env.mt->nbr_split;
     float edep = 0.0f;
     if (nsplit > 1) p.wt = p.wt / (float)nsplit;
@@ -391,7 +391,7 @@ __device__ void annih_at_rest(const Env &env, Stream &s, Part &p, const gegs_reg
 
 // ------------------------------------------------------------------ photon flight
 template <class Sink>
-__device__ void photon_step(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+__device__ __forceinline__ void photon_step(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
     const gegs_media_tables *mt = env.mt;
     for (;;) {
         t.photon_steps++;
@@ -452,7 +452,7 @@ __device__ void photon_step(const Env &env, Stream &s, Part &p, Sink &sink, Tall
 
 // ------------------------------------------------------------------ Compton (Klein-Nishina)
 template <class Sink>
-__device__ void compton(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+__device__ __forceinline__ void compton(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
     const gegs_region rd = env.reg[p.region];
     float edep = 0.0f;
     Part e1 = p;
@@ -528,7 +528,7 @@ __device__ void compton(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t
 
 // ------------------------------------------------------------------ photo-electric
 template <class Sink>
-__device__ void photo(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+__device__ __forceinline__ void photo(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
     const gegs_region rd = env.reg[p.region];
     float edep = 0.0f;
     p.e = (float)((double)p.e + PRM);
@@ -574,7 +574,7 @@ __device__ void photo(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) 
 
 // ------------------------------------------------------------------ Rayleigh
 template <class Sink>
-__device__ void rayleigh(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+__device__ __forceinline__ void rayleigh(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
     const gegs_media_tables *mt = env.mt;
     const gegs_region rd = env.reg[p.region];
     const float gle = glog(p.e);
@@ -615,7 +615,7 @@ __device__ void rayleigh(const Env &env, Stream &s, Part &p, Sink &sink, Tally &
 
 // ------------------------------------------------------------------ pair production
 template <class Sink>
-__device__ void pair(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+__device__ __forceinline__ void pair(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
     const gegs_media_tables *mt = env.mt;
     const gegs_region rd = env.reg[p.region];
     const int m8 = 8 * (int)rd.med;
@@ -717,7 +717,7 @@ __device__ void pair(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
 }
 
 // ------------------------------------------------------------------ multiple scattering
-__device__ inline void mscat(const gegs_media_tables *mt, Stream &s, float lambda, float chia2, float q1,
+__device__ __forceinline__ void mscat(const gegs_media_tables *mt, Stream &s, float lambda, float chia2, float q1,
                              float &cost, float &sint) {
     if (lambda <= 13.8f) {
         const float sprob = s.next();
@@ -808,7 +808,7 @@ __device__ inline void mscat(const gegs_media_tables *mt, Stream &s, float lambd
     sint = 0.0f;
 }
 
-__device__ inline float msdist_pI(const gegs_media_tables *mt, Stream &s, float eloss, float tustep, Part &p,
+__device__ __forceinline__ float msdist_pI(const gegs_media_tables *mt, Stream &s, float eloss, float tustep, Part &p,
                                   const gegs_region &rd) {
     const int m = rd.med;
     const float e0 = p.e - RM;
@@ -872,7 +872,7 @@ __device__ inline float msdist_pI(const gegs_media_tables *mt, Stream &s, float 
 }
 
 // energy lost over a path (the reference's $COMPUTE-ELOSS-G)
-__device__ inline float eloss_over(const gegs_media_tables *mt, const gegs_float2 *dedx, int qel, int m,
+__device__ __forceinline__ float eloss_over(const gegs_media_tables *mt, const gegs_float2 *dedx, int qel, int m,
                                    float range, float rhof, float step, float eke, float elke, int lelke) {
     const float *rep = mt->range_ep;
     float tuss = range - rep[(m * MXEKE + lelke - 1) * 2 + qel] / rhof;
@@ -905,7 +905,7 @@ __device__ inline float eloss_over(const gegs_media_tables *mt, const gegs_float
 
 // path length from eke down to e1 within one interpolation bin (the
 // reference's $COMPUTE-DRANGE)
-__device__ inline float path_in_bin(const gegs_float2 *dedx, float eke, float elke, float e1, float elk1, int lbin) {
+__device__ __forceinline__ float path_in_bin(const gegs_float2 *dedx, float eke, float elke, float e1, float elk1, int lbin) {
     const float fe = 1.0f - e1 / eke;
     const float elmid = 0.5f * (elke + elk1 + 0.25f * fe * fe * (1.0f + fe * (1.0f + 0.875f * fe)));
     const gegs_float2 dd = dedx[lbin - 1];
@@ -919,7 +919,7 @@ __device__ inline float path_in_bin(const gegs_float2 *dedx, float eke, float el
 
 // ------------------------------------------------------------------ condensed-history electron step
 template <class Sink>
-__device__ void electron_step(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+__device__ __forceinline__ void electron_step(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
     const gegs_media_tables *mt = env.mt;
     float elke = 0.0f, demfp = 0.0f, sig0 = 0.0f, total_tstep = 0.0f;
     int lelke = 0;
@@ -1198,7 +1198,7 @@ __device__ void electron_step(const Env &env, Stream &s, Part &p, Sink &sink, Ta
 
 // ------------------------------------------------------------------ Moller
 template <class Sink>
-__device__ void moller(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+__device__ __forceinline__ void moller(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
     const gegs_media_tables *mt = env.mt;
     const gegs_region rd = env.reg[p.region];
     const int m = rd.med;
@@ -1249,7 +1249,7 @@ __device__ void moller(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t)
 
 // ------------------------------------------------------------------ Bhabha
 template <class Sink>
-__device__ void bhabha(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+__device__ __forceinline__ void bhabha(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
     const gegs_media_tables *mt = env.mt;
     const gegs_region rd = env.reg[p.region];
     float edep = 0.0f;
@@ -1317,7 +1317,7 @@ __device__ void bhabha(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t)
 
 // ------------------------------------------------------------------ bremsstrahlung (Bethe-Heitler + Koch-Motz 2BS)
 template <class Sink>
-__device__ void brems(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+__device__ __forceinline__ void brems(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
     const gegs_media_tables *mt = env.mt;
     const gegs_region rd = env.reg[p.region];
     const int m = rd.med;
@@ -1418,7 +1418,7 @@ __device__ void brems(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) 
 
 // ------------------------------------------------------------------ in-flight annihilation (Heitler)
 template <class Sink>
-__device__ void annih(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+__device__ __forceinline__ void annih(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
     const gegs_media_tables *mt = env.mt;
     const gegs_region rd = env.reg[p.region];
     const int nsplit = mt->nbr_split;
@@ -1480,7 +1480,7 @@ __device__ void annih(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) 
 
 // dispatch one particle to its event handler
 template <class Sink>
-__device__ inline void handle(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+__device__ __forceinline__ void handle(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
     switch (p.status) {
     case P_PHOTON_STEP: photon_step(env, s, p, sink, t); break;
     case P_COMPTON: compton(env, s, p, sink, t); break;

@@ -20,14 +20,20 @@ SYMBOLS = [
     "gegs_create", "gegs_destroy", "gegs_last_error", "gegs_sizeof_media_tables",
     "gegs_load_media", "gegs_load_phantom", "gegs_set_field", "gegs_set_source",
     "gegs_run_plain", "gegs_run_event", "gegs_set_history_tally", "gegs_history_tally",
-    "gegs_get_counters", "gegs_last_kernel_ms",
+    "gegs_get_counters", "gegs_last_kernel_ms", "gegs_sample_events",
 ]
+
+PARTICLE_DTYPE = np.dtype([("e", "<f4"), ("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("u", "<f4"),
+                           ("v", "<f4"), ("w", "<f4"), ("wt", "<f4"), ("dnear", "<f4"),
+                           ("region", "<u4"), ("charge", "<i4"), ("status", "<i4"),
+                           ("lin", "<u8"), ("blk", "<u4"), ("pad", "<u4")])
 
 
 class RunOptions(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint64), ("history_offset", ctypes.c_uint64),
                 ("n_batches", ctypes.c_uint32), ("block_size", ctypes.c_uint32),
-                ("grid_size", ctypes.c_uint32), ("stack_depth", ctypes.c_uint32)]
+                ("grid_size", ctypes.c_uint32), ("stack_depth", ctypes.c_uint32),
+                ("first", ctypes.c_uint64), ("count", ctypes.c_uint64)]
 
 
 class Counters(ctypes.Structure):
@@ -72,6 +78,7 @@ def load():
     L.gegs_history_tally.argtypes = [vp, vp, vp]
     L.gegs_get_counters.argtypes = [vp, vp]
     L.gegs_last_kernel_ms.argtypes = [vp, vp]
+    L.gegs_sample_events.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint64, vp, ctypes.c_uint32, vp, vp]
     for name in SYMBOLS:
         if getattr(L, name, None) is None:
             raise RuntimeError("symbol %s missing from %s" % (name, LIB_PATH))
@@ -131,10 +138,21 @@ class Context:
         self._check(self.L.gegs_set_source(self.h, _p(src), len(src)))
         self.nsrc = len(src)
 
+    def sample_events(self, particles, seed, max_out=16):
+        """Run one event handler per particle (PARTICLE_DTYPE records);
+        returns (out[n, max_out], n_out[n], edep[n])."""
+        inp = np.ascontiguousarray(particles, dtype=PARTICLE_DTYPE)
+        n = len(inp)
+        out = np.zeros(n * max_out, dtype=PARTICLE_DTYPE)
+        n_out = np.zeros(n, dtype=np.uint32)
+        edep = np.zeros(n, dtype=np.float64)
+        self._check(self.L.gegs_sample_events(self.h, _p(inp), n, seed, _p(out), max_out, _p(n_out), _p(edep)))
+        return out.reshape(n, max_out), n_out, edep
+
     def run(self, seed, n_batches=10, history_offset=0, method="plain", per_history=False,
-            grid=0, block=0, depth=0):
+            grid=0, block=0, depth=0, first=0, count=0):
         self._check(self.L.gegs_set_history_tally(self.h, 1 if per_history else 0))
-        opt = RunOptions(seed, history_offset, n_batches, block, grid, depth)
+        opt = RunOptions(seed, history_offset, n_batches, block, grid, depth, first, count)
         dose = np.zeros(self.nvox, dtype=np.float64)
         dose2 = np.zeros(self.nvox, dtype=np.float64)
         fn = self.L.gegs_run_plain if method == "plain" else self.L.gegs_run_event

@@ -42,6 +42,18 @@ typedef struct gegs_source {
     int32_t charge;
 } gegs_source;
 
+/* A particle record as the event handlers see it (CUDA_EGS.h:294-322
+ * particle_t, with the stream state of its Philox lineage). */
+typedef struct gegs_particle {
+    float e, x, y, z, u, v, w, wt, dnear;
+    uint32_t region;
+    int32_t charge;
+    int32_t status;          /* event type, particle_status 0..9 */
+    uint64_t lin;            /* random-stream lineage */
+    uint32_t blk;            /* next Philox block of the lineage */
+    uint32_t pad;
+} gegs_particle;
+
 /* Beam / field geometry (CUDA_EGS.h:103-104, :278-282; CUDA_EGS.cu:430-432). */
 typedef struct gegs_field {
     int32_t change_xyz;          /* CT axis convention (Change_xyz) */
@@ -59,6 +71,8 @@ typedef struct gegs_run_options {
     uint32_t block_size;         /* threads per block, 0 = default */
     uint32_t grid_size;          /* blocks, 0 = default */
     uint32_t stack_depth;        /* per-history particle stack depth, 0 = default */
+    uint64_t first;              /* first source record of the run */
+    uint64_t count;              /* records to transport, 0 = all from first */
 } gegs_run_options;
 
 /* Event counters (diagnostics; replaces DO_STEP_COUNT / PRINT_PART_NUM). */
@@ -98,8 +112,8 @@ int gegs_set_field(gegs_ctx *ctx, const gegs_field *field);
  * device memory; histories map 1:1 onto records. */
 int gegs_set_source(gegs_ctx *ctx, const gegs_source *src, uint64_t n);
 
-/* Plain per-history transport of all loaded source records: the baseline
- * kernel.  dose and dose2 receive, per voxel, the energy deposited (MeV, FP64)
+/* Plain per-history transport of the source records [first, first+count)
+ * (history index = history_offset + record index): the baseline kernel.  dose and dose2 receive, per voxel, the energy deposited (MeV, FP64)
  * and the sum over batches of the squared batch energy, from which the
  * caller forms the batch variance. */
 int gegs_run_plain(gegs_ctx *ctx, const gegs_run_options *opt, double *dose, double *dose2);
@@ -114,6 +128,16 @@ int gegs_run_event(gegs_ctx *ctx, const gegs_run_options *opt, double *dose, dou
  * only when the run was made with gegs_set_history_tally(ctx, 1). */
 int gegs_set_history_tally(gegs_ctx *ctx, int enable);
 int gegs_history_tally(gegs_ctx *ctx, double *edep, uint32_t *steps);
+
+/* Sampler check: run the event handler in[i].status once on each of n
+ * particles, with the random stream each particle's lineage defines, and
+ * return the particles it pushes (at most max_out per input, in push order;
+ * n_out[i] = count) and the energy it scored (edep[i]).  No counterpart in the
+ * reference, whose samplers are reachable only through whole-stack kernels
+ * (compton.cu:17, photo.cu:1, ...); used to check samplers uniform for
+ * uniform against the oracle. */
+int gegs_sample_events(gegs_ctx *ctx, const gegs_particle *in, uint32_t n, uint64_t seed,
+                       gegs_particle *out, uint32_t max_out, uint32_t *n_out, double *edep);
 
 /* Counters and timing of the last run. */
 int gegs_get_counters(gegs_ctx *ctx, gegs_counters *out);

@@ -160,13 +160,19 @@ typedef struct octx {
     uint64_t hist;
     double cur_edep;
     uint32_t cur_steps;
+    double escaped;                  /* energy leaving the phantom or scored outside it */
+    double dropped;                  /* energy of particles the transport abandons */
     opart stack[STACK_MAX];
     int sp;
     gegs_counters cnt;
 } octx;
 
 static void push(octx *c, const opart *p) {
-    if (c->sp >= STACK_MAX) { c->cnt.overflow++; return; }
+    if (c->sp >= STACK_MAX) {
+        c->cnt.overflow++;
+        c->dropped += ((double)p->e + (double)p->charge * PRM) * (double)p->wt;
+        return;
+    }
     c->stack[c->sp++] = *p;
 }
 
@@ -189,7 +195,7 @@ static void push_child(octx *c, ostream *s, opart *p, int status) {
 /* escore, scoring.cu:14-24: FP64 tally, regions below 6 are not scored.
  * History-by-history variance: per-voxel tag of the last history. */
 static void escore(octx *c, uint32_t region, float energy) {
-    if (region < VAC_AND_OTH) return;
+    if (region < VAC_AND_OTH) { c->escaped += (double)energy; return; }
     double tmp = energy;
     uint32_t v = region - VAC_AND_OTH;
     c->cnt.deposits++;
@@ -457,7 +463,7 @@ static void photon_step(octx *c, ostream *s, opart *p) {
         uint32_t new_region = howfar(c, p, &tstep);
         int idisc = 0;
         if (new_region == 0) idisc = (rd.med == GEGS_VACUUM) ? 1 : -1;
-        if (idisc > 0) { p->region = 0; return; }
+        if (idisc > 0) { c->escaped += (double)p->e * (double)p->wt; p->region = 0; return; }
         p->x += p->u * tstep;
         p->y += p->v * tstep;
         p->z += p->w * tstep;
@@ -1126,6 +1132,7 @@ static void electron_step(octx *c, ostream *s, opart *p) {
         if (callhowfar || p->wt <= 0.0f) irnew = howfar(c, p, &ustep);
         if (irnew == 0) idisc = (rd.med == GEGS_VACUUM) ? 1 : -1;
         if (idisc > 0) {
+            c->escaped += ((double)p->e + (double)p->charge * PRM) * (double)p->wt;
             p->region = 0;
             return;
         }
@@ -1170,7 +1177,10 @@ static void electron_step(octx *c, ostream *s, opart *p) {
                     tvstep = vstep * (1.0f + xi * (0.5f + xi * 0.333333f));
                 } else {
                     if (xi < 0.999999f) tvstep = -vstep * om_logf(1.0f - xi) / xi;
-                    else return;   /* electron.cu:2031-2035: particle dropped */
+                    else {         /* electron.cu:2031-2035: particle dropped */
+                        c->dropped += ((double)p->e + (double)p->charge * PRM) * (double)p->wt;
+                        return;
+                    }
                 }
             } else {
                 tvstep = tustep;
@@ -1572,35 +1582,10 @@ uint64_t oracle_primary_lineage(uint64_t hist) { return mix64(hist ^ 0x5851F42D4
 
 size_t oracle_sizeof_media_tables(void) { return sizeof(gegs_media_tables); }
 
-/*
- * Transport n source particles, one history each (history index
- * hist_offset + i).  dose: FP64 energy per voxel; dose2: history-by-history
- * sum of squared per-history voxel energies.  hist_edep / hist_steps
- * (optional): per-history energy scored in the phantom and step count.
- * Bounds arrays carry nx+2, ny+2, nz+2 entries (see Q1 above).
- */
-int oracle_run(const gegs_media_tables *mt, const gegs_region *regions,
-               uint32_t nx, uint32_t ny, uint32_t nz,
-               const float *xb, const float *yb, const float *zb,
-               const gegs_field *field, const gegs_source *src, uint64_t n,
-               uint64_t hist_offset, uint64_t seed,
-               double *dose, double *dose2, double *hist_edep, uint32_t *hist_steps,
-               gegs_counters *counters) {
-    octx *c = (octx *)calloc(1, sizeof(octx));
-    if (!c) return GEGS_ERR_ARG;
-    size_t nvox = (size_t)nx * ny * nz;
-    c->mt = mt;
-    c->reg = regions;
-    c->xb = xb; c->yb = yb; c->zb = zb;
-    c->nx = nx; c->ny = ny; c->nz = nz;
-    c->f = *field;
-    c->seed = seed;
-    c->dose = dose;
-    c->dose2 = dose2;
-    c->tmp = (double *)calloc(nvox, sizeof(double));
-    c->last = (uint64_t *)calloc(nvox, sizeof(uint64_t));
-    if (!c->tmp || !c->last) { free(c->tmp); free(c->last); free(c); return GEGS_ERR_ARG; }
-    for (uint64_t i = 0; i < n; ++i) {
+/* Transport histories [i0, i1) of the source with one context (one thread). */
+static void run_range(octx *c, const gegs_source *src, uint64_t i0, uint64_t i1, uint64_t hist_offset,
+                      double *hist_edep, uint32_t *hist_steps) {
+    for (uint64_t i = i0; i < i1; ++i) {
         c->hist = hist_offset + i;
         c->cur_edep = 0.0;
         c->cur_steps = 0;
@@ -1611,7 +1596,7 @@ int oracle_run(const gegs_media_tables *mt, const gegs_region *regions,
         p.u = src[i].u; p.v = src[i].v; p.w = src[i].w; p.wt = src[i].wt;
         p.charge = src[i].charge;
         p.region = 0;
-        p.module = field->module_first;
+        p.module = c->f.module_first;
         p.dnear = INFI_DIST;
         p.lin = oracle_primary_lineage(c->hist);
         p.blk = 0;
@@ -1621,7 +1606,7 @@ int oracle_run(const gegs_media_tables *mt, const gegs_region *regions,
         while (c->sp > 0) {
             opart q = c->stack[--c->sp];
             ostream s;
-            st_begin(&s, &q, seed);
+            st_begin(&s, &q, c->seed);
             c->cnt.interactions[q.status]++;
             switch (q.status) {
             case P_PHOTON_STEP: photon_step(c, &s, &q); break;
@@ -1640,24 +1625,103 @@ int oracle_run(const gegs_media_tables *mt, const gegs_region *regions,
         if (hist_edep) hist_edep[i] = c->cur_edep;
         if (hist_steps) hist_steps[i] = c->cur_steps;
     }
+    size_t nvox = (size_t)c->nx * c->ny * c->nz;
     for (size_t v = 0; v < nvox; ++v)
-        if (c->last[v]) dose2[v] += c->tmp[v] * c->tmp[v];
-    if (counters) *counters = c->cnt;
-    free(c->tmp);
-    free(c->last);
-    free(c);
+        if (c->last[v]) { c->dose2[v] += c->tmp[v] * c->tmp[v]; c->last[v] = 0; c->tmp[v] = 0.0; }
+}
+
+/*
+ * Transport n source particles, one history each (history index
+ * hist_offset + i).  dose: FP64 energy per voxel; dose2: history-by-history
+ * sum of squared per-history voxel energies.  hist_edep / hist_steps
+ * (optional): per-history energy scored in the phantom and step count.
+ * balance (optional): [escaped, dropped] energy.  Bounds arrays carry
+ * nx+2, ny+2, nz+2 entries (see Q1 above).  nthreads > 1 splits the
+ * histories into contiguous ranges, one per OpenMP thread, each with its
+ * own tallies, summed at the end; per-history results do not depend on it.
+ */
+int oracle_run_mt(const gegs_media_tables *mt, const gegs_region *regions,
+                  uint32_t nx, uint32_t ny, uint32_t nz,
+                  const float *xb, const float *yb, const float *zb,
+                  const gegs_field *field, const gegs_source *src, uint64_t n,
+                  uint64_t hist_offset, uint64_t seed,
+                  double *dose, double *dose2, double *hist_edep, uint32_t *hist_steps,
+                  gegs_counters *counters, double *balance, int nthreads) {
+    size_t nvox = (size_t)nx * ny * nz;
+    if (nthreads < 1) nthreads = 1;
+    if ((uint64_t)nthreads > n && n > 0) nthreads = (int)n;
+    octx **cs = (octx **)calloc((size_t)nthreads, sizeof(octx *));
+    int ok = 1;
+    for (int t = 0; t < nthreads; ++t) {
+        octx *c = (octx *)calloc(1, sizeof(octx));
+        cs[t] = c;
+        if (!c) { ok = 0; continue; }
+        c->mt = mt; c->reg = regions; c->xb = xb; c->yb = yb; c->zb = zb;
+        c->nx = nx; c->ny = ny; c->nz = nz; c->f = *field; c->seed = seed;
+        c->dose = t == 0 ? dose : (double *)calloc(nvox, sizeof(double));
+        c->dose2 = t == 0 ? dose2 : (double *)calloc(nvox, sizeof(double));
+        c->tmp = (double *)calloc(nvox, sizeof(double));
+        c->last = (uint64_t *)calloc(nvox, sizeof(uint64_t));
+        if (!c->dose || !c->dose2 || !c->tmp || !c->last) ok = 0;
+    }
+    if (ok) {
+#pragma omp parallel for num_threads(nthreads) schedule(static, 1)
+        for (int t = 0; t < nthreads; ++t) {
+            uint64_t i0 = n * (uint64_t)t / (uint64_t)nthreads;
+            uint64_t i1 = n * (uint64_t)(t + 1) / (uint64_t)nthreads;
+            run_range(cs[t], src, i0, i1, hist_offset, hist_edep, hist_steps);
+        }
+    }
+    gegs_counters tot;
+    memset(&tot, 0, sizeof(tot));
+    double bal0 = 0.0, bal1 = 0.0;
+    for (int t = 0; t < nthreads; ++t) {
+        octx *c = cs[t];
+        if (!c) continue;
+        if (ok && t > 0) {
+            for (size_t v = 0; v < nvox; ++v) { dose[v] += c->dose[v]; dose2[v] += c->dose2[v]; }
+        }
+        tot.photon_steps += c->cnt.photon_steps;
+        tot.electron_steps += c->cnt.electron_steps;
+        tot.region_gathers += c->cnt.region_gathers;
+        tot.deposits += c->cnt.deposits;
+        for (int k = 0; k < 10; ++k) tot.interactions[k] += c->cnt.interactions[k];
+        tot.histories += c->cnt.histories;
+        tot.overflow += c->cnt.overflow;
+        bal0 += c->escaped;
+        bal1 += c->dropped;
+        if (t > 0) { free(c->dose); free(c->dose2); }
+        free(c->tmp);
+        free(c->last);
+        free(c);
+    }
+    free(cs);
+    if (!ok) return GEGS_ERR_ARG;
+    if (counters) *counters = tot;
+    if (balance) { balance[0] = bal0; balance[1] = bal1; }
     return GEGS_OK;
 }
 
-/* Single-sampler entry points for uniform-for-uniform checks: a sampler is
- * driven through the same per-particle stream a transport run would give it
- * (lineage lin, block blk), and the particle records it pushes are returned. */
+int oracle_run(const gegs_media_tables *mt, const gegs_region *regions,
+               uint32_t nx, uint32_t ny, uint32_t nz,
+               const float *xb, const float *yb, const float *zb,
+               const gegs_field *field, const gegs_source *src, uint64_t n,
+               uint64_t hist_offset, uint64_t seed,
+               double *dose, double *dose2, double *hist_edep, uint32_t *hist_steps,
+               gegs_counters *counters, double *balance) {
+    return oracle_run_mt(mt, regions, nx, ny, nz, xb, yb, zb, field, src, n, hist_offset, seed,
+                         dose, dose2, hist_edep, hist_steps, counters, balance, 1);
+}
+
+/* Single-sampler entry point for uniform-for-uniform checks: the handler of
+ * in->status runs on one particle with the stream a transport run would give
+ * it (lineage in->lin, block in->blk); the particles it pushes are returned
+ * in push order, and *edep receives the energy it scored. */
 int oracle_interaction(const gegs_media_tables *mt, const gegs_region *regions,
                        uint32_t nx, uint32_t ny, uint32_t nz,
                        const float *xb, const float *yb, const float *zb,
-                       const gegs_field *field, int status, const float *state9,
-                       uint32_t region, uint64_t lin, uint32_t blk, uint64_t seed,
-                       float *out, int max_out, double *edep_out) {
+                       const gegs_field *field, const gegs_particle *in, uint64_t seed,
+                       gegs_particle *out, int max_out, double *edep_out) {
     octx *c = (octx *)calloc(1, sizeof(octx));
     size_t nvox = (size_t)nx * ny * nz;
     double *dose = (double *)calloc(nvox, sizeof(double));
@@ -1669,14 +1733,13 @@ int oracle_interaction(const gegs_media_tables *mt, const gegs_region *regions,
     c->last = (uint64_t *)calloc(nvox, sizeof(uint64_t));
     opart p;
     memset(&p, 0, sizeof(p));
-    p.e = state9[0]; p.x = state9[1]; p.y = state9[2]; p.z = state9[3];
-    p.u = state9[4]; p.v = state9[5]; p.w = state9[6]; p.wt = state9[7];
-    p.charge = (int)state9[8];
-    p.region = region; p.module = field->module_first; p.dnear = INFI_DIST;
-    p.lin = lin; p.blk = blk; p.status = status;
+    p.e = in->e; p.x = in->x; p.y = in->y; p.z = in->z;
+    p.u = in->u; p.v = in->v; p.w = in->w; p.wt = in->wt; p.dnear = in->dnear;
+    p.charge = in->charge; p.region = in->region; p.module = field->module_first;
+    p.lin = in->lin; p.blk = in->blk; p.status = in->status;
     ostream s;
     st_begin(&s, &p, seed);
-    switch (status) {
+    switch (p.status) {
     case P_PHOTON_STEP: photon_step(c, &s, &p); break;
     case P_COMPTON: compton(c, &s, &p); break;
     case P_PHOTO: photo(c, &s, &p); break;
@@ -1692,10 +1755,11 @@ int oracle_interaction(const gegs_media_tables *mt, const gegs_region *regions,
     int n = c->sp < max_out ? c->sp : max_out;
     for (int i = 0; i < n; ++i) {
         const opart *q = &c->stack[i];
-        float *o = out + 12 * i;
-        o[0] = q->e; o[1] = q->x; o[2] = q->y; o[3] = q->z; o[4] = q->u; o[5] = q->v; o[6] = q->w;
-        o[7] = q->wt; o[8] = (float)q->charge; o[9] = (float)q->status; o[10] = (float)q->region;
-        o[11] = (float)q->blk;
+        gegs_particle *o = out + i;
+        memset(o, 0, sizeof(*o));
+        o->e = q->e; o->x = q->x; o->y = q->y; o->z = q->z; o->u = q->u; o->v = q->v; o->w = q->w;
+        o->wt = q->wt; o->dnear = q->dnear; o->region = q->region; o->charge = q->charge;
+        o->status = q->status; o->lin = q->lin; o->blk = q->blk;
     }
     double tot = 0.0;
     for (size_t v = 0; v < nvox; ++v) tot += dose[v];

@@ -41,12 +41,12 @@ def lib():
         L.oracle_run.restype = ctypes.c_int
         L.oracle_run.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                  vp, vp, vp, vp, vp, ctypes.c_uint64, ctypes.c_uint64,
-                                 ctypes.c_uint64, vp, vp, vp, vp, vp]
+                                 ctypes.c_uint64, vp, vp, vp, vp, vp, vp]
+        L.oracle_run_mt.restype = ctypes.c_int
+        L.oracle_run_mt.argtypes = L.oracle_run.argtypes + [ctypes.c_int]
         L.oracle_interaction.restype = ctypes.c_int
         L.oracle_interaction.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
-                                         vp, vp, vp, vp, ctypes.c_int, vp, ctypes.c_uint32,
-                                         ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64,
-                                         vp, ctypes.c_int, vp]
+                                         vp, vp, vp, vp, vp, ctypes.c_uint64, vp, ctypes.c_int, vp]
         L.oracle_logf.restype = ctypes.c_float
         L.oracle_logf.argtypes = [ctypes.c_float]
         L.oracle_expf.restype = ctypes.c_float
@@ -68,7 +68,7 @@ def _p(a):
     return ctypes.c_void_p(a.ctypes.data) if a is not None else None
 
 
-def run(workload, source, seed, hist_offset=0, per_history=False):
+def run(workload, source, seed, hist_offset=0, per_history=False, threads=1):
     """Transport ``source`` (SOURCE_DTYPE array) through the workload.
     Returns dict(dose, dose2, edep, steps, counters)."""
     L = lib()
@@ -81,26 +81,30 @@ def run(workload, source, seed, hist_offset=0, per_history=False):
     edep = np.zeros(n, dtype=np.float64) if per_history else None
     steps = np.zeros(n, dtype=np.uint32) if per_history else None
     cnt = Counters()
+    bal = np.zeros(2, dtype=np.float64)
     reg = np.ascontiguousarray(workload.regions)
-    rc = L.oracle_run(ctypes.c_void_p(workload.tables.ptr()), _p(reg), phan.nx, phan.ny, phan.nz,
-                      _p(xb), _p(yb), _p(zb), ctypes.c_void_p(ctypes.addressof(workload.field)),
-                      _p(src), n, hist_offset, seed, _p(dose), _p(dose2), _p(edep), _p(steps),
-                      ctypes.c_void_p(ctypes.addressof(cnt)))
+    rc = L.oracle_run_mt(ctypes.c_void_p(workload.tables.ptr()), _p(reg), phan.nx, phan.ny, phan.nz,
+                         _p(xb), _p(yb), _p(zb), ctypes.c_void_p(ctypes.addressof(workload.field)),
+                         _p(src), n, hist_offset, seed, _p(dose), _p(dose2), _p(edep), _p(steps),
+                         ctypes.c_void_p(ctypes.addressof(cnt)), _p(bal), int(threads))
     if rc != 0:
         raise RuntimeError("oracle_run failed: %d" % rc)
-    return dict(dose=dose, dose2=dose2, edep=edep, steps=steps, counters=cnt)
+    return dict(dose=dose, dose2=dose2, edep=edep, steps=steps, counters=cnt,
+                escaped=float(bal[0]), dropped=float(bal[1]))
 
 
-def interaction(workload, status, state9, region, lin, blk, seed, max_out=64):
-    """Run one event handler on one particle; returns (pushed records (k,12), edep)."""
+def interaction(workload, particle, seed, max_out=64):
+    """Run the handler of particle['status'] once (PARTICLE_DTYPE record from
+    gegs_amd.native); returns (pushed records, scored energy)."""
+    from gegs_amd.native import PARTICLE_DTYPE
     L = lib()
     phan = workload.phantom
     xb, yb, zb = phan.padded_bounds()
-    st = np.ascontiguousarray(state9, dtype=np.float32)
-    out = np.zeros((max_out, 12), dtype=np.float32)
+    inp = np.ascontiguousarray(np.asarray(particle, dtype=PARTICLE_DTYPE).reshape(1))
+    out = np.zeros(max_out, dtype=PARTICLE_DTYPE)
     ed = np.zeros(1, dtype=np.float64)
     reg = np.ascontiguousarray(workload.regions)
     k = L.oracle_interaction(ctypes.c_void_p(workload.tables.ptr()), _p(reg), phan.nx, phan.ny, phan.nz,
                              _p(xb), _p(yb), _p(zb), ctypes.c_void_p(ctypes.addressof(workload.field)),
-                             status, _p(st), region, lin, blk, seed, _p(out), max_out, _p(ed))
+                             _p(inp), seed, _p(out), max_out, _p(ed))
     return out[:k].copy(), float(ed[0])

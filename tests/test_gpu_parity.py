@@ -1,0 +1,125 @@
+"""GPU transport against the oracle, through the C ABI.
+
+With the same seed both sides draw the same Philox streams and evaluate the
+same binary32 arithmetic (no FMA contraction, correctly rounded division and
+square root, the same elementary-function algorithms), so trajectories agree
+history by history: per-history energy and step counts must be identical, and
+per-voxel dose may differ only by the order of the FP64 tally additions."""
+import numpy as np
+import pytest
+
+from gegs_amd import native, stats, workloads
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def water():
+    return workloads.water()
+
+
+@pytest.fixture(scope="module")
+def ct_small():
+    return workloads.ct()
+
+
+def _run_gpu(ctx, wl, src, seed, method="plain", per_history=True, n_batches=4):
+    ctx.load_workload(wl)
+    ctx.set_source(src)
+    return ctx.run(seed=seed, n_batches=n_batches, method=method, per_history=per_history)
+
+
+def test_water_per_history_bit_exact(gpu_ctx, water):
+    src = water.source(4000, 1)
+    g = _run_gpu(gpu_ctx, water, src, 1234)
+    o = oracle.run(water, src, seed=1234, per_history=True)
+    assert np.array_equal(g["edep"], o["edep"])
+    assert np.array_equal(g["steps"], o["steps"])
+    c = o["counters"]
+    gc = g["counters"]
+    assert gc["photon_steps"] == c.photon_steps
+    assert gc["electron_steps"] == c.electron_steps
+    assert gc["deposits"] == c.deposits
+    assert gc["interactions"] == list(c.interactions)
+    assert np.allclose(g["dose"], o["dose"], rtol=1e-12, atol=1e-12)
+
+
+def test_ct_per_history_bit_exact(gpu_ctx, ct_small):
+    src = ct_small.source(3000, 2)
+    g = _run_gpu(gpu_ctx, ct_small, src, 99)
+    o = oracle.run(ct_small, src, seed=99, per_history=True)
+    assert np.array_equal(g["edep"], o["edep"])
+    assert np.array_equal(g["steps"], o["steps"])
+    assert np.allclose(g["dose"], o["dose"], rtol=1e-12, atol=1e-12)
+
+
+EVENTS = [(0, 0, (0.02, 6.0)), (1, 0, (0.02, 6.0)), (2, 0, (0.012, 0.5)), (3, 0, (0.011, 1.0)),
+          (4, 0, (1.03, 6.0)), (5, -1, (0.72, 6.5)), (5, 1, (0.72, 6.5)), (6, -1, (1.0, 6.5)),
+          (7, -1, (0.75, 6.5)), (8, 1, (0.75, 6.5)), (9, 1, (0.75, 6.5))]
+
+
+@pytest.mark.parametrize("status,charge,erange", EVENTS)
+def test_samplers_uniform_for_uniform(gpu_ctx, ct_small, status, charge, erange):
+    """Every event handler, fed the same stream, returns bit-identical
+    particles and scored energy on the GPU and in the oracle."""
+    gpu_ctx.load_workload(ct_small)
+    ph = ct_small.phantom
+    rng = np.random.default_rng(status * 10 + charge + 3)
+    n = 300
+    parts = np.zeros(n, dtype=native.PARTICLE_DTYPE)
+    ix = rng.integers(40, 88, n)
+    iy = rng.integers(40, 88, n)
+    iz = rng.integers(30, 70, n)
+    region = 6 + ix + iy * ph.nx + iz * ph.nx * ph.ny
+    med = ct_small.regions["med"][region]
+    keep = med != 0xFFFF
+    # head-frame coordinates of the voxel centres (CT axis convention, gantry 0)
+    xc = 0.5 * (ph.xb[ix] + ph.xb[ix + 1])
+    yc = 0.5 * (ph.yb[iy] + ph.yb[iy + 1])
+    zc = 0.5 * (ph.zb[iz] + ph.zb[iz + 1])
+    sid = ct_small.field.source_to_isocenter
+    parts["x"], parts["y"], parts["z"] = zc, xc, yc + np.float32(sid)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    parts["u"], parts["v"], parts["w"] = d[:, 0], d[:, 1], d[:, 2]
+    parts["e"] = rng.uniform(*erange, n)
+    parts["wt"] = 1.0
+    parts["dnear"] = 1e5
+    parts["region"] = region
+    parts["charge"] = charge
+    parts["status"] = status
+    parts["lin"] = rng.integers(1, 2 ** 62, n, dtype=np.uint64)
+    parts["blk"] = rng.integers(0, 50, n)
+    parts = parts[keep]
+    out, nout, edep = gpu_ctx.sample_events(parts, seed=321, max_out=24)
+    for i, p in enumerate(parts):
+        o, e = oracle.interaction(ct_small, p, 321)
+        assert nout[i] == len(o), i
+        assert e == edep[i], i
+        assert out[i, :len(o)].tobytes() == o.tobytes(), i
+
+
+def test_dose_statistically_equivalent_across_seeds(gpu_ctx, ct_small):
+    """Independent seeds: per-voxel z-scores of GPU vs oracle dose follow a
+    unit normal (chi^2 per voxel near 1) over voxels above 20% of Dmax."""
+    n = 200000
+    g = _run_gpu(gpu_ctx, ct_small, ct_small.source(n, 8), 1, per_history=False, n_batches=20)
+    o = oracle.run(ct_small, ct_small.source(n, 9), seed=2, threads=8)
+    sg = stats.batch_sigma(g["dose"], g["dose2"], 20)
+    so = stats.history_sigma(o["dose"], o["dose2"], n)
+    # select on the mean of both runs so that neither side's noise biases z
+    mean = 0.5 * (g["dose"] + o["dose"])
+    mask = mean > 0.3 * mean.max()
+    chi2, meanz, nv = stats.zscore_chi2(g["dose"], sg, o["dose"], so, mask)
+    assert nv > 200
+    assert 0.75 < chi2 < 1.35, (chi2, meanz, nv)
+    assert abs(meanz) < 0.2
+
+
+def test_errors_are_status_codes(gpu_ctx):
+    fresh = native.Context(0)
+    with pytest.raises(native.GegsError) as e:
+        fresh.run(seed=1)
+    assert e.value.code == 90002
+    fresh.close()

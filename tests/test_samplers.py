@@ -1,0 +1,235 @@
+"""Oracle samplers against the closed-form distributions they draw from.
+
+Each sampler is driven through oracle_interaction with many independent
+lineages (the streams a transport run would give the particles)."""
+import math
+
+import numpy as np
+import pytest
+from scipy import stats
+
+from gegs_amd import native, workloads
+from oracle import oracle
+
+RM = 0.5110034
+P_COMPTON, P_PHOTO, P_RAYLEIGH, P_PAIR = 1, 2, 3, 4
+E_STEP, E_MOLLER, E_BREMS, E_BHABHA, E_ANNIH = 5, 6, 7, 8, 9
+
+
+@pytest.fixture(scope="module")
+def wl():
+    return workloads.water(n=20, voxel=1.5)
+
+
+def particle(status, e, charge=0, region=None, lin=1, wl=None):
+    p = np.zeros(1, dtype=native.PARTICLE_DTYPE)[0]
+    p["e"] = e
+    p["x"], p["y"], p["z"] = 0.1, 0.2, 100.0 + 15.0       # centre of the phantom, head frame
+    p["u"], p["v"], p["w"] = 0.0, 0.0, 1.0
+    p["wt"] = 1.0
+    p["dnear"] = 1e5
+    p["charge"] = charge
+    p["status"] = status
+    p["region"] = region if region is not None else 6 + 10 + 10 * 20 + 10 * 400
+    p["lin"] = lin
+    p["blk"] = 0
+    return p
+
+
+def run_many(wl, status, e, n, charge=0, seed=7):
+    outs, edeps = [], []
+    for i in range(n):
+        o, ed = oracle.interaction(wl, particle(status, e, charge, lin=1000 + i), seed)
+        outs.append(o)
+        edeps.append(ed)
+    return outs, np.array(edeps)
+
+
+def chi2_pvalue(samples, cdf, bins):
+    """Chi-square goodness of fit of samples to a distribution given by its CDF."""
+    counts, _ = np.histogram(samples, bins=bins)
+    exp = np.diff(cdf(bins)) * len(samples)
+    keep = exp > 5
+    c2 = np.sum((counts[keep] - exp[keep]) ** 2 / exp[keep])
+    return stats.chi2.sf(c2, keep.sum() - 1)
+
+
+def test_compton_klein_nishina(wl):
+    e0 = 1.25
+    outs, ed = run_many(wl, P_COMPTON, e0, 6000)
+    frac = []
+    for o, e in zip(outs, ed):
+        ph = o[o["charge"] == 0]
+        el = o[o["charge"] == -1]
+        assert len(ph) == 1 and len(el) <= 1
+        # energy conservation: E = E' + T_e, the electron pushed above ECUT or scored
+        te = el["e"][0] - RM if len(el) else e
+        assert abs(ph["e"][0] + te - e0) < 2e-6
+        frac.append(ph["e"][0] / e0)
+    frac = np.array(frac)
+    k = e0 / RM
+    lo = 1.0 / (1.0 + 2.0 * k)
+
+    def cdf(x):
+        x = np.clip(x, lo, 1.0)
+        g = np.linspace(lo, 1.0, 20001)
+        # KN in eps = E'/E: (1/eps + eps)(1 - eps sin^2 / (1 + eps^2)) with cos from eps
+        c = 1.0 - (1.0 / g - 1.0) / k
+        f = (1.0 / g + g) * (1.0 - g * (1.0 - c * c) / (1.0 + g * g))
+        cum = np.concatenate([[0], np.cumsum(0.5 * (f[1:] + f[:-1]) * np.diff(g))])
+        return np.interp(x, g, cum / cum[-1])
+    assert chi2_pvalue(frac, cdf, np.linspace(lo, 1.0, 41)) > 1e-4
+
+
+def test_pair_uniform_below_2_1_MeV(wl):
+    e0 = 1.9
+    outs, ed = run_many(wl, P_PAIR, e0, 3000)
+    tk = []
+    for o in outs:
+        # both leptons above ECUT are pushed; below it they deposit (and annihilate)
+        for q in o[o["charge"] != 0]:
+            tk.append(q["e"] - RM)
+    tk = np.array(tk)
+    kmax = e0 - 2 * RM
+    # kinetic energy of each lepton ~ U(0, kmax); pushed ones are those with total > AE
+    sel = tk[tk > 0.7 - RM + 1e-6]
+    lo = 0.7 - RM
+    assert stats.kstest((sel - lo) / (kmax - lo), "uniform").pvalue > 1e-4
+
+
+def test_moller_distribution(wl):
+    t0 = 3.0
+    outs, ed = run_many(wl, E_MOLLER, t0 + RM, 4000, charge=-1)
+    te = 0.7 - RM
+    eps = []
+    for o in outs:
+        e = np.sort(o["e"] - RM) if len(o) == 2 else None
+        if e is None:
+            continue
+        assert abs(e.sum() - t0) < 1e-5
+        eps.append(e[0] / t0)
+    eps = np.array(eps)
+    tau = t0 / RM
+    g2 = tau * tau / (tau + 1) ** 2
+    g3 = (2 * tau + 1) / (tau + 1) ** 2
+
+    def cdf(x):
+        g = np.linspace(te / t0, 0.5, 20001)
+        r = g / (1 - g)
+        f = (1 + g2 * g * g + r * (r - g3)) / (g * g)
+        cum = np.concatenate([[0], np.cumsum(0.5 * (f[1:] + f[:-1]) * np.diff(g))])
+        return np.interp(np.clip(x, g[0], 0.5), g, cum / cum[-1])
+    assert chi2_pvalue(eps, cdf, np.linspace(te / t0, 0.5, 31)) > 1e-4
+
+
+def test_bremsstrahlung_energy_distribution(wl):
+    """Photon energies follow the screened Bethe-Heitler density the tables
+    encode: (1/k)[(1+a^2) phi1 - 2a phi2/3] with a = E'/E (brems.cu:155-181)."""
+    t0 = 4.0
+    c = wl.tables.c
+    outs, ed = run_many(wl, E_BREMS, t0 + RM, 5000, charge=-1)
+    ks = np.array([q["e"] for o in outs for q in o if q["charge"] == 0] +
+                  [t0 + RM - (q["e"]) for o in outs for q in o if q["charge"] == -1 and False])
+    # photons below PCUT are scored, not pushed; only test above 10 keV
+    eie = t0 + RM
+    ap = c.ap[0]
+    delcm = c.delcm[0]
+    dl = lambda name, l: getattr(c, name)[l]
+
+    def dens(k):
+        ese = eie - k
+        a = ese / eie
+        d = k / eie / ese * delcm
+        phi1 = np.where(d < 1, dl("dl1", 0) + d * (dl("dl2", 0) + d * dl("dl3", 0)),
+                        dl("dl4", 0) + dl("dl5", 0) * np.log(d + dl("dl6", 0)))
+        phi2 = np.where(d < 1, dl("dl1", 1) + d * (dl("dl2", 1) + d * dl("dl3", 1)), phi1)
+        return np.maximum((1 + a * a) * phi1 - 2 * a * phi2 / 3, 0) / k
+
+    def cdf(x):
+        g = np.exp(np.linspace(np.log(ap), np.log(t0), 40001))
+        f = dens(g)
+        cum = np.concatenate([[0], np.cumsum(0.5 * (f[1:] + f[:-1]) * np.diff(g))])
+        return np.interp(np.clip(x, ap, t0), g, cum / cum[-1])
+    bins = np.exp(np.linspace(np.log(ap), np.log(t0), 25))
+    assert chi2_pvalue(ks, cdf, bins) > 1e-4
+
+
+def test_rayleigh_angles_follow_form_factor(wl):
+    """cos(theta) ~ (1 + cos^2) F^2(x) with x = E sin(theta/2)/(hc)."""
+    e0 = 0.03
+    outs, ed = run_many(wl, P_RAYLEIGH, e0, 4000)
+    w = np.array([o["w"][0] for o in outs])      # incident along +z
+    assert np.all(np.abs(np.array([o["e"][0] for o in outs]) - e0) == 0)
+    from gegs_amd.pegs_lite import form_factor
+    comp = wl.tables.media[0]
+
+    def ff2(mu):
+        x = e0 / 0.012398418 * np.sqrt(0.5 * (1 - mu))
+        return sum(p * form_factor(z, x) ** 2 for p, z in zip(comp.pz, comp.z))
+
+    def cdf(xs):
+        g = np.linspace(-1, 1, 40001)
+        f = (1 + g * g) * ff2(g)
+        cum = np.concatenate([[0], np.cumsum(0.5 * (f[1:] + f[:-1]) * np.diff(g))])
+        return np.interp(np.clip(xs, -1, 1), g, cum / cum[-1])
+    # the sampler interpolates the tabulated form factor in log-log; the
+    # agreement is at the table's resolution
+    assert chi2_pvalue(w, cdf, np.linspace(-1, 1, 21)) > 1e-4
+
+
+def test_annihilation_at_rest_back_to_back(wl):
+    # a positron below ECUT in electron_step annihilates at rest
+    outs, ed = run_many(wl, E_STEP, 0.65, 2000, charge=1)
+    ws = []
+    for o in outs:
+        assert len(o) == 2
+        assert np.all(o["e"] == np.float32(RM)) and np.all(o["charge"] == 0)
+        d = np.stack([o["u"], o["v"], o["w"]], axis=1)
+        assert np.allclose(d[0], -d[1])
+        ws.append(o["w"][0])
+    # isotropic: cos theta uniform in [-1, 1]
+    assert stats.kstest((np.array(ws) + 1) / 2, "uniform").pvalue > 1e-4
+    # the positron's kinetic energy was scored
+    assert np.allclose(ed, 0.65 - RM, atol=1e-6)
+
+
+def test_inflight_annihilation_conserves_energy(wl):
+    t0 = 2.0
+    outs, ed = run_many(wl, E_ANNIH, t0 + RM, 1000, charge=1)
+    for o, e in zip(outs, ed):
+        assert abs(o["e"].sum() + e - (t0 + 2 * RM)) < 1e-5
+
+
+def test_photo_electron_energy(wl):
+    outs, ed = run_many(wl, P_PHOTO, 0.3, 500)
+    for o in outs:
+        assert len(o) == 1 and o["charge"][0] == -1
+        assert abs(o["e"][0] - (0.3 + RM)) < 1e-6
+
+
+def test_photon_free_path_is_exponential(wl):
+    """Distance to the first interaction in water ~ Exp(mu) (the reference
+    resamples the number of mean free paths at every voxel face; memoryless)."""
+    c = wl.tables.c
+    e0 = 1.25
+    gle = math.log(e0)
+    l = int(c.ge[0].x + c.ge[0].y * gle)
+    mu = 1.0 / ((c.gmfp[l].x + c.gmfp[l].y * gle) * (c.cohe[l].x + c.cohe[l].y * gle))
+    d = []
+    for i in range(4000):
+        p = particle(0, e0, 0, lin=5000 + i)
+        p["z"] = 100.0 + 0.75       # 0.75 cm deep, beam along +z: 29 cm of water ahead
+        p["x"], p["y"] = 0.3, 0.4
+        p["region"] = 6 + 10 + 10 * 20
+        o, ed = oracle.interaction(wl, p, 11)
+        if len(o) == 1 and o["status"][0] in (1, 2, 3, 4):
+            d.append(o["z"][0] - p["z"])
+        else:
+            d.append(np.inf)       # left the phantom without interacting
+    d = np.array(d)
+    depth = 30.0 - 0.75
+    fin = d[np.isfinite(d)]
+    # truncated exponential on [0, depth]
+    cdf = lambda x: (1 - np.exp(-mu * x)) / (1 - math.exp(-mu * depth))
+    assert stats.kstest(fin, cdf).pvalue > 1e-4
+    assert abs(np.mean(~np.isfinite(d)) - math.exp(-mu * depth)) < 0.02

@@ -1,0 +1,67 @@
+"""Whole-transport properties of the oracle: energy conservation, symmetry,
+linearity in the source weights."""
+import numpy as np
+
+from gegs_amd import workloads
+from oracle import oracle
+
+
+def test_energy_conservation_water():
+    w = workloads.water(n=24, voxel=1.25)
+    src = w.source(3000, 21)
+    r = oracle.run(w, src, seed=77)
+    e_in = float(np.sum(src["e"].astype(np.float64) * src["wt"]))
+    e_out = r["dose"].sum() + r["escaped"] + r["dropped"]
+    assert abs(e_out - e_in) / e_in < 1e-5
+    assert r["counters"].overflow == 0
+
+
+def test_energy_conservation_ct_spectrum():
+    w = workloads.ct(nx=32, ny=32, nz=25, voxel=1.0, seed=2)
+    src = w.source(3000, 5)
+    r = oracle.run(w, src, seed=3)
+    e_in = float(np.sum(src["e"].astype(np.float64)))
+    e_out = r["dose"].sum() + r["escaped"] + r["dropped"]
+    assert abs(e_out - e_in) / e_in < 1e-5
+
+
+def test_mirror_symmetry_water():
+    """The parallel field is symmetric under x -> -x; so is the dose."""
+    w = workloads.water(n=20, voxel=1.5)
+    src = w.source(20000, 4)
+    r = oracle.run(w, src, seed=8)
+    d = r["dose"].reshape(20, 20, 20)
+    left = d[:, :, :10].sum()
+    right = d[:, :, 10:].sum()
+    sig = np.sqrt(r["dose2"].sum())
+    assert abs(left - right) < 4 * sig
+
+
+def test_weights_scale_dose_linearly():
+    """Doubling every source weight doubles the dose exactly (same streams)."""
+    w = workloads.water(n=12, voxel=2.5)
+    src = w.source(500, 9)
+    r1 = oracle.run(w, src, seed=5)
+    src2 = src.copy()
+    src2["wt"] = 2.0
+    r2 = oracle.run(w, src2, seed=5)
+    assert np.allclose(r2["dose"], 2.0 * r1["dose"], rtol=1e-12, atol=0)
+
+
+def test_history_offset_changes_streams_deterministically():
+    w = workloads.water(n=12, voxel=2.5)
+    src = w.source(200, 9)
+    a = oracle.run(w, src, seed=5, hist_offset=0, per_history=True)
+    b = oracle.run(w, src, seed=5, hist_offset=0, per_history=True)
+    c = oracle.run(w, src, seed=5, hist_offset=1000, per_history=True)
+    assert np.array_equal(a["edep"], b["edep"])
+    assert not np.array_equal(a["edep"], c["edep"])
+
+
+def test_depth_dose_builds_up_then_attenuates():
+    w = workloads.water(n=30, voxel=1.0)
+    src = w.source(20000, 13)
+    r = oracle.run(w, src, seed=17)
+    d = r["dose"].reshape(30, 30, 30)[:, 12:18, 12:18].sum(axis=(1, 2))
+    # beyond the build-up region the central-axis dose falls off with depth
+    assert d[3:8].mean() > d[20:25].mean() * 1.3
