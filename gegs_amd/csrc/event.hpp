@@ -1,23 +1,545 @@
-// event.hpp -- event-queued transport (placeholder until the queue kernels land)
+// event.hpp -- event-queued transport: SoA particle banks in HBM, one kernel
+// per event type, wavefront-aggregated queue appends.
+//
+// The reference already splits transport by event type (ten global AoS
+// stacks, one kernel each, CUDA_EGS.h:362-367, launched with a host sync and
+// a counter readback after every kernel, CUDA_EGS.cu:226-365).  Here:
+//   * each event type has a bank of particles stored word-major
+//     ([event][word][slot]), so the 64 lanes of a wavefront load and store
+//     consecutive words;
+//   * a kernel drains its bank and appends every particle it forwards or
+//     creates to the bank of the next event type with one atomic per
+//     wavefront and destination (ballot, popcount, lane rank), instead of one
+//     contended atomic per particle (utility.cu:222-306);
+//   * the photon free flight and the electron condensed-history step run as
+//     persistent loops that refill idle lanes from the bank every iteration,
+//     so lanes whose particle interacted or stopped take new work instead of
+//     idling until the slowest lane of the wavefront finishes;
+//   * primaries are injected from the resident source records whenever the
+//     number of particles in flight drops below a refill threshold;
+//   * a group of sweeps (inject, ten event kernels, bookkeeping) is captured
+//     in a hipGraph and replayed until no particle and no source record is
+//     left; the host reads one flag per replay.
+// Particles keep their Philox streams across banks, so every history follows
+// exactly the trajectory the plain kernel and the oracle give it.
 #pragma once
+#include <vector>
+
 #include "transport.hpp"
 
 namespace gegs {
 
+constexpr int EVW = 16;   // words per banked particle
+
+struct EvState {
+    uint32_t count[N_EVENTS];        // particles in each bank
+    uint32_t taken[N_EVENTS];        // work handed out from each bank
+    uint32_t done_blocks[N_EVENTS];  // blocks finished, for the bank reset
+    uint32_t overflow;               // particles that did not fit
+    uint32_t active;                 // more work remains after the sweep
+    uint32_t src_next, src_last;     // source records still to inject
+    uint32_t inj_start, inj_n;       // current injection
+    uint32_t sweeps;
+    uint32_t pad[3];
+};
+
 struct EventConfig {
-    uint32_t block = 256;
-    size_t capacity = 0;
+    uint32_t block = 256;            // threads per block of the event kernels
+    uint32_t grid = 1024;            // blocks of the event kernels
+    size_t capacity = 0;             // slots per bank (0 = 4 Mi)
+    uint32_t inject = 1u << 20;      // primaries injected per refill
+    uint32_t threshold = 1u << 20;   // refill when fewer particles are in flight
+    uint32_t sweeps_per_graph = 4;
 };
 
+struct EvArgs {
+    Env env;
+    float *bank;                     // [event][word][cap]
+    uint32_t cap;
+    EvState *st;
+    const gegs_source *src;
+    uint64_t hist_offset;
+    double *dose;
+    double *hist_edep;               // per source record, optional
+    uint32_t *hist_steps;            // optional
+    unsigned long long *counters;
+};
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t lanes_below() {
+    const uint32_t l = lane_id();
+    return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+__device__ __forceinline__ float *bank_word(const EvArgs &a, int e, int w, uint32_t idx) {
+    return a.bank + ((size_t)e * EVW + w) * a.cap + idx;
+}
+
+__device__ __forceinline__ void bank_store(const EvArgs &a, int e, uint32_t idx, const Part &p, uint32_t hist) {
+    *bank_word(a, e, 0, idx) = p.e;
+    *bank_word(a, e, 1, idx) = p.x;
+    *bank_word(a, e, 2, idx) = p.y;
+    *bank_word(a, e, 3, idx) = p.z;
+    *bank_word(a, e, 4, idx) = p.u;
+    *bank_word(a, e, 5, idx) = p.v;
+    *bank_word(a, e, 6, idx) = p.w;
+    *bank_word(a, e, 7, idx) = p.wt;
+    *bank_word(a, e, 8, idx) = p.dnear;
+    *bank_word(a, e, 9, idx) = __uint_as_float(p.region);
+    *bank_word(a, e, 10, idx) = __uint_as_float((uint32_t)(p.charge + 1) | ((uint32_t)p.module << 16));
+    *bank_word(a, e, 11, idx) = __uint_as_float((uint32_t)p.lin);
+    *bank_word(a, e, 12, idx) = __uint_as_float((uint32_t)(p.lin >> 32));
+    *bank_word(a, e, 13, idx) = __uint_as_float(p.blk);
+    *bank_word(a, e, 14, idx) = __uint_as_float(hist);
+}
+
+__device__ __forceinline__ Part bank_load(const EvArgs &a, int e, uint32_t idx, uint32_t &hist) {
+    Part p;
+    p.e = *bank_word(a, e, 0, idx);
+    p.x = *bank_word(a, e, 1, idx);
+    p.y = *bank_word(a, e, 2, idx);
+    p.z = *bank_word(a, e, 3, idx);
+    p.u = *bank_word(a, e, 4, idx);
+    p.v = *bank_word(a, e, 5, idx);
+    p.w = *bank_word(a, e, 6, idx);
+    p.wt = *bank_word(a, e, 7, idx);
+    p.dnear = *bank_word(a, e, 8, idx);
+    p.region = __float_as_uint(*bank_word(a, e, 9, idx));
+    const uint32_t pk = __float_as_uint(*bank_word(a, e, 10, idx));
+    p.charge = (int32_t)(pk & 0xffu) - 1;
+    p.module = (int32_t)(pk >> 16);
+    p.status = e;
+    p.lin = (uint64_t)__float_as_uint(*bank_word(a, e, 11, idx)) |
+            ((uint64_t)__float_as_uint(*bank_word(a, e, 12, idx)) << 32);
+    p.blk = __float_as_uint(*bank_word(a, e, 13, idx));
+    hist = __float_as_uint(*bank_word(a, e, 14, idx));
+    return p;
+}
+
+// Appends to the bank named by p.status.  The lanes that reach a push
+// together are grouped by destination; each group reserves its slots with
+// one atomic issued by its first lane.
+struct BankSink {
+    const EvArgs *a;
+    uint32_t hist;
+    __device__ __forceinline__ void push(const Part &p) {
+        const int dest = p.status;
+        for (;;) {
+            const int d0 = __builtin_amdgcn_readfirstlane(dest);
+            if (dest == d0) {
+                const uint64_t m = __ballot(1);
+                const uint32_t rank = (uint32_t)__popcll(m & lanes_below());
+                uint32_t base = 0;
+                if (rank == 0) base = atomicAdd(&a->st->count[d0], (uint32_t)__popcll(m));
+                base = __builtin_amdgcn_readfirstlane(base);
+                const uint32_t idx = base + rank;
+                if (idx < a->cap) bank_store(*a, d0, idx, p, hist);
+                else atomicAdd(&a->st->overflow, 1u);
+                break;
+            }
+        }
+    }
+};
+
+__device__ __forceinline__ void flush_particle(const EvArgs &a, uint32_t hist, Tally &t) {
+    if (a.hist_edep && t.hist_edep != 0.0) unsafeAtomicAdd(a.hist_edep + hist, t.hist_edep);
+    if (a.hist_steps && t.steps) atomicAdd(a.hist_steps + hist, t.steps);
+    t.hist_edep = 0.0;
+    t.steps = 0;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// counters are reduced over the wavefront first: one atomic per wave and
+// counter instead of one per lane (all lanes of the wave must call this)
+__device__ __forceinline__ void flush_counters(const EvArgs &a, const Tally &t, int event, uint32_t pops) {
+    unsigned long long *c = a.counters;
+    const uint32_t v0 = wave_sum(t.photon_steps), v1 = wave_sum(t.electron_steps);
+    const uint32_t v2 = wave_sum(t.gathers), v3 = wave_sum(t.deposits), v4 = wave_sum(pops);
+    if (lane_id() == 0) {
+        if (v0) atomicAdd(c + 0, (unsigned long long)v0);
+        if (v1) atomicAdd(c + 1, (unsigned long long)v1);
+        if (v2) atomicAdd(c + 2, (unsigned long long)v2);
+        if (v3) atomicAdd(c + 3, (unsigned long long)v3);
+        if (v4) atomicAdd(c + 4 + event, (unsigned long long)v4);
+    }
+}
+
+// empties a bank once its kernel has drained it (a single thread; runs
+// between the kernel that drains the bank and the first kernel that refills it)
+__global__ void reset_bank(EvState *st, int event) {
+    st->count[event] = 0;
+    st->taken[event] = 0;
+}
+
+__device__ __forceinline__ Tally fresh_tally(const EvArgs &a) {
+    Tally t;
+    t.dose = a.dose;
+    t.hist_edep = 0.0;
+    t.steps = 0;
+    t.photon_steps = t.electron_steps = t.gathers = t.deposits = 0;
+    return t;
+}
+
+// single-pass events: compton, photo, rayleigh, pair, moller, brems, bhabha,
+// annih.  Wavefront w takes slots [64 w, 64 w + 64) and then strides by the
+// grid; an empty bank costs one load per wave.
+template <int E>
+__global__ void __launch_bounds__(256) interaction_kernel(EvArgs a) {
+    const uint32_t n = a.st->count[E];
+    if (n == 0) return;
+    Tally t = fresh_tally(a);
+    BankSink sink{&a, 0};
+    uint32_t pops = 0;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t base = wave * 64u; base < n; base += nwaves * 64u) {
+        const uint32_t idx = base + lane_id();
+        if (idx < n) {
+            uint32_t hist;
+            Part p = bank_load(a, E, idx, hist);
+            sink.hist = hist;
+            Stream s;
+            s.begin(p, a.env.seed);
+            if (E == P_COMPTON) compton(a.env, s, p, sink, t);
+            else if (E == P_PHOTO) photo(a.env, s, p, sink, t);
+            else if (E == P_RAYLEIGH) rayleigh(a.env, s, p, sink, t);
+            else if (E == P_PAIR) pair(a.env, s, p, sink, t);
+            else if (E == E_MOLLER) moller(a.env, s, p, sink, t);
+            else if (E == E_BREMS) brems(a.env, s, p, sink, t);
+            else if (E == E_BHABHA) bhabha(a.env, s, p, sink, t);
+            else annih(a.env, s, p, sink, t);
+            flush_particle(a, hist, t);
+            pops++;
+        }
+    }
+    flush_counters(a, t, E, pops);
+}
+
+// Lane refill for the looping events: a wavefront reserves 64 bank slots
+// with one atomic and hands them to its idle lanes in order, reserving again
+// when they run out.
+struct Refill {
+    uint32_t next, end;      // wave-uniform reservation
+    bool exhausted;
+    __device__ __forceinline__ void init() {
+        next = end = 0;
+        exhausted = false;
+    }
+    // returns the slot for this lane (>= n: none); call with all lanes active
+    __device__ __forceinline__ uint32_t take(uint32_t *taken, uint32_t n, bool want) {
+        const uint64_t idle = __ballot(want);
+        uint32_t need = (uint32_t)__popcll(idle);
+        uint32_t slot = 0xFFFFFFFFu;
+        const uint32_t rank = (uint32_t)__popcll(idle & lanes_below());
+        if (need == 0 || exhausted) return slot;
+        if (end - next < need) {
+            // top up the reservation: keep the remainder, add 64 more
+            uint32_t got = 0;
+            if (lane_id() == 0) got = atomicAdd(taken, 64u);
+            got = __builtin_amdgcn_readfirstlane(got);
+            if (got >= n) {
+                exhausted = true;
+            } else {
+                // remainder [next, end) is used first, then [got, got + 64)
+                const uint32_t rem = end - next;
+                if (want && rank < rem) slot = next + rank;
+                else if (want) slot = got + (rank - rem);
+                const uint32_t used_new = need > rem ? need - rem : 0u;
+                next = got + used_new;
+                end = got + 64u;
+                return slot < n ? slot : 0xFFFFFFFFu;
+            }
+        }
+        if (want && rank < end - next) slot = next + rank;
+        const uint32_t used = need < end - next ? need : end - next;
+        next += used;
+        return slot < n ? slot : 0xFFFFFFFFu;
+    }
+};
+
+// photon free flight with lane refill
+__global__ void __launch_bounds__(256) photon_kernel(EvArgs a) {
+    const uint32_t n = a.st->count[P_PHOTON_STEP];
+    if (n == 0) return;
+    Tally t = fresh_tally(a);
+    BankSink sink{&a, 0};
+    uint32_t pops = 0;
+    bool have = false;
+    Part p;
+    Stream s;
+    uint32_t hist = 0;
+    Refill rf;
+    rf.init();
+    for (;;) {
+        const uint32_t slot = rf.take(&a.st->taken[P_PHOTON_STEP], n, !have);
+        if (!have && slot != 0xFFFFFFFFu) {
+            p = bank_load(a, P_PHOTON_STEP, slot, hist);
+            s.begin(p, a.env.seed);
+            have = true;
+            pops++;
+        }
+        if (!__any(have)) break;
+        if (have) {
+            sink.hist = hist;
+            if (!photon_iter(a.env, s, p, sink, t)) {
+                have = false;
+                flush_particle(a, hist, t);
+            }
+        }
+    }
+    flush_counters(a, t, P_PHOTON_STEP, pops);
+}
+
+// electron / positron condensed-history steps with lane refill
+__global__ void __launch_bounds__(256) electron_kernel(EvArgs a) {
+    const uint32_t n = a.st->count[E_ELECTRON_STEP];
+    if (n == 0) return;
+    Tally t = fresh_tally(a);
+    BankSink sink{&a, 0};
+    uint32_t pops = 0;
+    bool have = false;
+    Part p;
+    Stream s;
+    EState es;
+    uint32_t hist = 0;
+    Refill rf;
+    rf.init();
+    for (;;) {
+        const uint32_t slot = rf.take(&a.st->taken[E_ELECTRON_STEP], n, !have);
+        if (!have && slot != 0xFFFFFFFFu) {
+            p = bank_load(a, E_ELECTRON_STEP, slot, hist);
+            s.begin(p, a.env.seed);
+            es.init();
+            have = true;
+            pops++;
+        }
+        if (!__any(have)) break;
+        if (have) {
+            sink.hist = hist;
+            if (!electron_iter(a.env, s, p, es, sink, t)) {
+                have = false;
+                flush_particle(a, hist, t);
+            }
+        }
+    }
+    flush_counters(a, t, E_ELECTRON_STEP, pops);
+}
+
+// plan the next injection: refill when few particles are in flight
+__global__ void inject_plan(EvState *st, uint32_t threshold, uint32_t max_inject) {
+    uint32_t inflight = 0;
+    for (int e = 0; e < N_EVENTS; ++e) inflight += st->count[e];
+    uint32_t n = 0;
+    if (inflight < threshold && st->src_next < st->src_last) {
+        n = st->src_last - st->src_next;
+        if (n > max_inject) n = max_inject;
+    }
+    st->inj_start = st->src_next;
+    st->inj_n = n;
+    st->src_next += n;
+}
+
+__global__ void __launch_bounds__(256) inject_fill(EvArgs a) {
+    const uint32_t n = a.st->inj_n;
+    const uint32_t start = a.st->inj_start;
+    BankSink sink{&a, 0};
+    uint32_t injected = 0;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // whole wavefronts iterate together so that the aggregated push sees
+    // the lanes that have a record in the same pass
+    const uint32_t wave0 = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u;
+    for (uint32_t k0 = wave0; k0 < n; k0 += stride) {
+        const uint32_t k = k0 + lane_id();
+        if (k < n) {
+            const uint32_t rec = start + k;
+            const gegs_source r = a.src[rec];
+            Part p;
+            p.e = r.e; p.x = r.x; p.y = r.y; p.z = r.z;
+            p.u = r.u; p.v = r.v; p.w = r.w; p.wt = r.wt;
+            p.charge = r.charge;
+            p.region = 0;
+            p.module = a.env.f.module_first;
+            p.dnear = INFI_DIST;
+            p.blk = 0;
+            p.lin = primary_lineage(a.hist_offset + rec);
+            p.status = r.charge == 0 ? P_PHOTON_STEP : E_ELECTRON_STEP;
+            sink.hist = rec;
+            sink.push(p);
+            injected++;
+        }
+    }
+    const uint32_t tot = wave_sum(injected);
+    if (lane_id() == 0 && tot) atomicAdd(a.counters + 14, (unsigned long long)tot);
+}
+
+__global__ void sweep_end(EvState *st) {
+    // banks 1-4 are filled only by the photon kernel and 6-9 only by the
+    // electron kernel, both of which come before their consumers: all four
+    // interaction banks of each family are empty for the next sweep
+    const int drained[8] = {P_COMPTON, P_PHOTO, P_RAYLEIGH, P_PAIR, E_MOLLER, E_BREMS, E_BHABHA, E_ANNIH};
+    for (int k = 0; k < 8; ++k) {
+        st->count[drained[k]] = 0;
+        st->taken[drained[k]] = 0;
+    }
+    uint32_t inflight = 0;
+    for (int e = 0; e < N_EVENTS; ++e) inflight += st->count[e];
+    st->active = (inflight > 0 || st->src_next < st->src_last) ? 1u : 0u;
+    st->sweeps += 1;
+}
+
+__global__ void batch_begin(EvState *st, uint32_t first, uint32_t last) {
+    for (int e = 0; e < N_EVENTS; ++e) {
+        st->count[e] = 0;
+        st->taken[e] = 0;
+        st->done_blocks[e] = 0;
+    }
+    st->src_next = first;
+    st->src_last = last;
+    st->inj_start = first;
+    st->inj_n = 0;
+    st->active = 1;
+    st->sweeps = 0;
+}
+
+__global__ void clear_overflow(EvState *st) { st->overflow = 0; }
+
+__global__ void report_overflow(const EvState *st, unsigned long long *counters) {
+    counters[15] += (unsigned long long)st->overflow;
+}
+
+// ------------------------------------------------------------------ host side
 struct EventBanks {
-    hipError_t reserve(uint64_t, uint32_t, const EventConfig &) { return hipErrorNotSupported; }
-    void release() {}
+    float *bank = nullptr;
+    uint32_t cap = 0;
+    EvState *st = nullptr;
+    uint32_t *h_flags = nullptr;     // pinned: active, sweeps, overflow
+    hipGraphExec_t graph = nullptr;
+    EvArgs graph_args{};
+    EventConfig graph_cfg{};
+
+    hipError_t reserve(uint64_t, uint32_t, const EventConfig &cfg) {
+        const size_t want = cfg.capacity ? cfg.capacity : ((size_t)1 << 22);
+        if (want > 0xFFFFFFF0ull) return hipErrorInvalidValue;
+        hipError_t e = hipSuccess;
+        if (!st) {
+            e = hipMalloc(&st, sizeof(EvState));
+            if (e != hipSuccess) return e;
+            e = hipHostMalloc(&h_flags, 4 * sizeof(uint32_t));
+            if (e != hipSuccess) return e;
+        }
+        if (cap < want) {
+            if (bank) (void)hipFree(bank);
+            bank = nullptr;
+            cap = 0;
+            e = hipMalloc(&bank, want * N_EVENTS * EVW * sizeof(float));
+            if (e != hipSuccess) return e;
+            cap = (uint32_t)want;
+            drop_graph();
+        }
+        return hipSuccess;
+    }
+    void drop_graph() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
+    }
+    void release() {
+        drop_graph();
+        if (bank) (void)hipFree(bank);
+        if (st) (void)hipFree(st);
+        if (h_flags) (void)hipHostFree(h_flags);
+        bank = nullptr;
+        st = nullptr;
+        h_flags = nullptr;
+        cap = 0;
+    }
 };
 
-inline hipError_t run_event_batch(EventBanks &, const Env &, const gegs_source *, uint32_t, uint32_t, uint64_t,
-                                  double *, double *, uint32_t *, unsigned long long *, const EventConfig &,
-                                  hipStream_t) {
-    return hipErrorNotSupported;
+inline void launch_sweep(const EvArgs &a, const EventConfig &cfg, hipStream_t stream) {
+    const dim3 g(cfg.grid), b(cfg.block);
+    hipLaunchKernelGGL(inject_plan, dim3(1), dim3(1), 0, stream, a.st, cfg.threshold, cfg.inject);
+    hipLaunchKernelGGL(inject_fill, g, b, 0, stream, a);
+    hipLaunchKernelGGL(photon_kernel, g, b, 0, stream, a);
+    hipLaunchKernelGGL(reset_bank, dim3(1), dim3(1), 0, stream, a.st, (int)P_PHOTON_STEP);
+    hipLaunchKernelGGL(interaction_kernel<P_RAYLEIGH>, g, b, 0, stream, a);
+    hipLaunchKernelGGL(interaction_kernel<P_COMPTON>, g, b, 0, stream, a);
+    hipLaunchKernelGGL(interaction_kernel<P_PHOTO>, g, b, 0, stream, a);
+    hipLaunchKernelGGL(interaction_kernel<P_PAIR>, g, b, 0, stream, a);
+    hipLaunchKernelGGL(electron_kernel, g, b, 0, stream, a);
+    hipLaunchKernelGGL(reset_bank, dim3(1), dim3(1), 0, stream, a.st, (int)E_ELECTRON_STEP);
+    hipLaunchKernelGGL(interaction_kernel<E_MOLLER>, g, b, 0, stream, a);
+    hipLaunchKernelGGL(interaction_kernel<E_BREMS>, g, b, 0, stream, a);
+    hipLaunchKernelGGL(interaction_kernel<E_BHABHA>, g, b, 0, stream, a);
+    hipLaunchKernelGGL(interaction_kernel<E_ANNIH>, g, b, 0, stream, a);
+    hipLaunchKernelGGL(sweep_end, dim3(1), dim3(1), 0, stream, a.st);
+}
+
+inline bool same_args(const EvArgs &x, const EvArgs &y) {
+    return x.bank == y.bank && x.cap == y.cap && x.st == y.st && x.src == y.src && x.hist_offset == y.hist_offset &&
+           x.dose == y.dose && x.hist_edep == y.hist_edep && x.hist_steps == y.hist_steps &&
+           x.counters == y.counters && x.env.mt == y.env.mt && x.env.reg == y.env.reg && x.env.seed == y.env.seed &&
+           x.env.xb == y.env.xb && x.env.nx == y.env.nx && x.env.ny == y.env.ny && x.env.nz == y.env.nz &&
+           x.env.f.cos_angle == y.env.f.cos_angle && x.env.f.sin_angle == y.env.f.sin_angle &&
+           x.env.f.change_xyz == y.env.f.change_xyz && x.env.f.source_to_isocenter == y.env.f.source_to_isocenter &&
+           x.env.f.iso_x == y.env.f.iso_x && x.env.f.iso_y == y.env.f.iso_y && x.env.f.iso_z == y.env.f.iso_z;
+}
+
+inline bool same_cfg(const EventConfig &x, const EventConfig &y) {
+    return x.block == y.block && x.grid == y.grid && x.inject == y.inject && x.threshold == y.threshold &&
+           x.sweeps_per_graph == y.sweeps_per_graph;
+}
+
+// Transport source records [first, last) through the banks.  The sweep group
+// is captured once per argument set and replayed; the host polls one flag.
+inline hipError_t run_event_batch(EventBanks &bk, const Env &env, const gegs_source *src, uint32_t first,
+                                  uint32_t last, uint64_t hist_offset, double *dose, double *hist_edep,
+                                  uint32_t *hist_steps, unsigned long long *counters, const EventConfig &cfg,
+                                  hipStream_t stream, uint32_t max_sweeps = 100000) {
+    EvArgs a;
+    a.env = env;
+    a.bank = bk.bank;
+    a.cap = bk.cap;
+    a.st = bk.st;
+    a.src = src;
+    a.hist_offset = hist_offset;
+    a.dose = dose;
+    a.hist_edep = hist_edep;
+    a.hist_steps = hist_steps;
+    a.counters = counters;
+    hipError_t e;
+    if (!bk.graph || !same_args(a, bk.graph_args) || !same_cfg(cfg, bk.graph_cfg)) {
+        bk.drop_graph();
+        hipGraph_t g;
+        e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return e;
+        for (uint32_t k = 0; k < cfg.sweeps_per_graph; ++k) launch_sweep(a, cfg, stream);
+        e = hipStreamEndCapture(stream, &g);
+        if (e != hipSuccess) return e;
+        e = hipGraphInstantiate(&bk.graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return e;
+        bk.graph_args = a;
+        bk.graph_cfg = cfg;
+    }
+    hipLaunchKernelGGL(batch_begin, dim3(1), dim3(1), 0, stream, bk.st, first, last);
+    uint32_t sweeps = 0;
+    for (;;) {
+        e = hipGraphLaunch(bk.graph, stream);
+        if (e != hipSuccess) return e;
+        e = hipMemcpyAsync(bk.h_flags, &bk.st->active, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
+        if (e != hipSuccess) return e;
+        e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) return e;
+        sweeps += cfg.sweeps_per_graph;
+        if (!bk.h_flags[0]) break;
+        if (sweeps >= max_sweeps) return hipErrorLaunchTimeOut;
+    }
+    return hipSuccess;
 }
 
 }  // namespace gegs

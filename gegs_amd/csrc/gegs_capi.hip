@@ -320,17 +320,26 @@ int gegs_run_event(gegs_ctx *c, const gegs_run_options *o, double *dose, double 
     hipSetDevice(c->device);
     rc = prepare_tallies(c);
     if (rc) return rc;
-    hipEvent_t t0, t1;
-    hipEventCreate(&t0);
-    hipEventCreate(&t1);
     EventConfig cfg;
-    cfg.block = o->block_size ? o->block_size : 256;
+    if (o->block_size) cfg.block = o->block_size;
+    if (o->grid_size) cfg.grid = o->grid_size;
+    if (cfg.block != 256 && cfg.block != 128 && cfg.block != 64)
+        return fail(c, GEGS_ERR_ARG, "run_event: block_size must be 64, 128 or 256");
     cfg.capacity = o->stack_depth ? (size_t)o->stack_depth * 65536 : 0;
+    if (!o->grid_size) cfg.grid = 2048;
     const uint32_t nb = o->n_batches;
     Env env = make_env(c, o->seed);
     hipError_t e = c->banks.reserve(c->nsrc, nb, cfg);
     if (e != hipSuccess) return fail(c, GEGS_ERR_DEVICE, "event banks", e);
-    hipEventRecord(t0, c->stream);
+    if (c->hist_tally) {
+        hipLaunchKernelGGL(fill_zero, dim3(1024), dim3(256), 0, c->stream, c->d_hist_edep, (size_t)c->nsrc);
+        hipLaunchKernelGGL(fill_zero_u32, dim3(1024), dim3(256), 0, c->stream, c->d_hist_steps, (size_t)c->nsrc);
+    }
+    hipLaunchKernelGGL(clear_overflow, dim3(1), dim3(1), 0, c->stream, c->banks.st);
+    hipEvent_t t0, t1;
+    (void)hipEventCreate(&t0);
+    (void)hipEventCreate(&t1);
+    (void)hipEventRecord(t0, c->stream);
     const uint64_t r0 = o->first, nr = o->count ? o->count : c->nsrc - o->first;
     for (uint32_t b = 0; b < nb; ++b) {
         const uint32_t first = (uint32_t)(r0 + (nr * b) / nb);
@@ -341,7 +350,8 @@ int gegs_run_event(gegs_ctx *c, const gegs_run_options *o, double *dose, double 
         if (e != hipSuccess) return fail(c, GEGS_ERR_DEVICE, "event batch", e);
         hipLaunchKernelGGL(fold_batch, dim3(2048), dim3(256), 0, c->stream, c->d_chunk, c->d_dose, c->d_dose2, c->nvox);
     }
-    hipEventRecord(t1, c->stream);
+    hipLaunchKernelGGL(report_overflow, dim3(1), dim3(1), 0, c->stream, c->banks.st, c->d_counters);
+    (void)hipEventRecord(t1, c->stream);
     HIPCHECK(c, hipGetLastError(), "launch event kernels");
     return finish_run(c, dose, dose2, t0, t1);
 }

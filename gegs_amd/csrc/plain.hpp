@@ -130,15 +130,24 @@ __global__ void __launch_bounds__(128) plain_kernel(PlainArgs a) {
         if (a.hist_steps) a.hist_steps[i] = t.steps;
         nhist++;
     }
+    // one atomic per wavefront and counter
     unsigned long long *c = a.counters;
-    atomicAdd(c + 0, (unsigned long long)t.photon_steps);
-    atomicAdd(c + 1, (unsigned long long)t.electron_steps);
-    atomicAdd(c + 2, (unsigned long long)t.gathers);
-    atomicAdd(c + 3, (unsigned long long)t.deposits);
+    uint32_t vals[16];
+    vals[0] = t.photon_steps;
+    vals[1] = t.electron_steps;
+    vals[2] = t.gathers;
+    vals[3] = t.deposits;
 #pragma unroll
-    for (int k = 0; k < N_EVENTS; ++k) atomicAdd(c + 4 + k, (unsigned long long)pops[k]);
-    atomicAdd(c + 14, (unsigned long long)nhist);
-    atomicAdd(c + 15, (unsigned long long)sink.overflow);
+    for (int k = 0; k < N_EVENTS; ++k) vals[4 + k] = pops[k];
+    vals[14] = nhist;
+    vals[15] = sink.overflow;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        uint32_t v = vals[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (__lane_id() == 0 && v) atomicAdd(c + k, (unsigned long long)v);
+    }
 }
 
 // sampler check: one handler invocation per input particle, pushes listed

@@ -391,62 +391,67 @@ __device__ __forceinline__ void annih_at_rest(const Env &env, Stream &s, Part &p
 
 // ------------------------------------------------------------------ photon flight
 template <class Sink>
-__device__ __forceinline__ void photon_step(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+__device__ __forceinline__ bool photon_iter(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+    // one free-flight segment; true: the photon crossed a boundary and flies on
     const gegs_media_tables *mt = env.mt;
-    for (;;) {
-        t.photon_steps++;
-        t.steps++;
-        gegs_region rd = env.reg[p.region];
-        if (p.e <= rd.pcut) { t.score(p.region, p.e * p.wt); return; }
-        if (p.wt <= 0.0f) return;
-        float r1 = s.next();
-        float dpmfp = -glog(r1);
-        const float gle = glog(p.e);
-        int lgle = 0;
-        float tstep, gmfp_val = 0.0f, cohfac = 0.0f;
-        if (rd.med == VACUUM) {
-            tstep = VACUUM_STEP;
-        } else {
-            const gegs_float2 ge = mt->ge[rd.med];
-            lgle = (int)(ge.x + ge.y * gle);
-            const gegs_float2 gm = mt->gmfp[rd.med * MXGE + lgle];
-            gmfp_val = (gm.x + gm.y * gle) / rd.rhof;
-            if (rd.flags & GEGS_F_RAYLEIGH) {
-                const gegs_float2 ch = mt->cohe[rd.med * MXGE + lgle];
-                cohfac = ch.x + ch.y * gle;
-                gmfp_val *= cohfac;
-            }
-            tstep = gmfp_val * dpmfp;
-        }
-        const uint32_t nreg = howfar_phantom(env, p, tstep);
-        int idisc = 0;
-        if (nreg == 0) idisc = rd.med == VACUUM ? 1 : -1;
-        if (idisc > 0) { p.region = 0; return; }
-        p.x += p.u * tstep;
-        p.y += p.v * tstep;
-        p.z += p.w * tstep;
-        if (rd.med != VACUUM) dpmfp = fmaxf(0.0f, dpmfp - tstep / gmfp_val);
-        const uint32_t old_med = rd.med;
-        if (nreg != p.region) {
-            p.region = nreg;
-            rd = env.reg[nreg];
-            t.gathers++;
-        }
-        if (p.e <= rd.pcut || idisc < 0) { t.score(p.region, p.e * p.wt); return; }
-        if (rd.med != old_med || rd.med == VACUUM || (double)dpmfp >= EPSGMFP) continue;
+    t.photon_steps++;
+    t.steps++;
+    gegs_region rd = env.reg[p.region];
+    if (p.e <= rd.pcut) { t.score(p.region, p.e * p.wt); return false; }
+    if (p.wt <= 0.0f) return false;
+    float r1 = s.next();
+    float dpmfp = -glog(r1);
+    const float gle = glog(p.e);
+    int lgle = 0;
+    float tstep, gmfp_val = 0.0f, cohfac = 0.0f;
+    if (rd.med == VACUUM) {
+        tstep = VACUUM_STEP;
+    } else {
+        const gegs_float2 ge = mt->ge[rd.med];
+        lgle = (int)(ge.x + ge.y * gle);
+        const gegs_float2 gm = mt->gmfp[rd.med * MXGE + lgle];
+        gmfp_val = (gm.x + gm.y * gle) / rd.rhof;
         if (rd.flags & GEGS_F_RAYLEIGH) {
-            r1 = s.next();
-            if (r1 < 1.0f - cohfac) { forward(sink, s, p, P_RAYLEIGH); return; }
+            const gegs_float2 ch = mt->cohe[rd.med * MXGE + lgle];
+            cohfac = ch.x + ch.y * gle;
+            gmfp_val *= cohfac;
         }
-        const gegs_float2 g1 = mt->gbr1[rd.med * MXGE + lgle];
-        const gegs_float2 g2 = mt->gbr2[rd.med * MXGE + lgle];
-        const float gbr1 = g1.x + g1.y * gle;
-        const float gbr2 = g2.x + g2.y * gle;
+        tstep = gmfp_val * dpmfp;
+    }
+    const uint32_t nreg = howfar_phantom(env, p, tstep);
+    int idisc = 0;
+    if (nreg == 0) idisc = rd.med == VACUUM ? 1 : -1;
+    if (idisc > 0) { p.region = 0; return false; }
+    p.x += p.u * tstep;
+    p.y += p.v * tstep;
+    p.z += p.w * tstep;
+    if (rd.med != VACUUM) dpmfp = fmaxf(0.0f, dpmfp - tstep / gmfp_val);
+    const uint32_t old_med = rd.med;
+    if (nreg != p.region) {
+        p.region = nreg;
+        rd = env.reg[nreg];
+        t.gathers++;
+    }
+    if (p.e <= rd.pcut || idisc < 0) { t.score(p.region, p.e * p.wt); return false; }
+    if (rd.med != old_med || rd.med == VACUUM || (double)dpmfp >= EPSGMFP) return true;
+    if (rd.flags & GEGS_F_RAYLEIGH) {
         r1 = s.next();
-        if (r1 <= gbr1 && p.e > 2.0f * RM) forward(sink, s, p, P_PAIR);
-        else if (r1 < gbr2) forward(sink, s, p, P_COMPTON);
-        else forward(sink, s, p, P_PHOTO);
-        return;
+        if (r1 < 1.0f - cohfac) { forward(sink, s, p, P_RAYLEIGH); return false; }
+    }
+    const gegs_float2 g1 = mt->gbr1[rd.med * MXGE + lgle];
+    const gegs_float2 g2 = mt->gbr2[rd.med * MXGE + lgle];
+    const float gbr1 = g1.x + g1.y * gle;
+    const float gbr2 = g2.x + g2.y * gle;
+    r1 = s.next();
+    if (r1 <= gbr1 && p.e > 2.0f * RM) forward(sink, s, p, P_PAIR);
+    else if (r1 < gbr2) forward(sink, s, p, P_COMPTON);
+    else forward(sink, s, p, P_PHOTO);
+    return false;
+}
+
+template <class Sink>
+__device__ __forceinline__ void photon_step(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+    while (photon_iter(env, s, p, sink, t)) {
     }
 }
 
@@ -918,281 +923,300 @@ __device__ __forceinline__ float path_in_bin(const gegs_float2 *dedx, float eke,
 }
 
 // ------------------------------------------------------------------ condensed-history electron step
+// state of an electron that persists between condensed-history steps
+struct EState {
+    float elke, demfp, sig0, total_tstep;
+    int lelke;
+    bool compute_tstep, is_tstep;
+    __device__ __forceinline__ void init() {
+        elke = demfp = sig0 = total_tstep = 0.0f;
+        lelke = 0;
+        compute_tstep = true;
+        is_tstep = true;
+    }
+};
+
 template <class Sink>
-__device__ __forceinline__ void electron_step(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+__device__ __forceinline__ bool electron_iter(const Env &env, Stream &s, Part &p, EState &st, Sink &sink, Tally &t) {
+    // one condensed-history step; true: the electron continues stepping
     const gegs_media_tables *mt = env.mt;
-    float elke = 0.0f, demfp = 0.0f, sig0 = 0.0f, total_tstep = 0.0f;
-    int lelke = 0;
-    bool compute_tstep = true;
-    bool is_tstep = true;
-    for (;;) {
-        uint32_t irl = p.region;
-        gegs_region rd = env.reg[irl];
-        int medium = rd.med;
-        const float rhof = rd.rhof;
-        const int lelec = p.charge;
-        const int qel = (1 + lelec) / 2;
-        float eie = p.e;
-        t.electron_steps++;
-        t.steps++;
-        if (eie <= rd.ecut) {
-            t.score(p.region, to_kinetic(p.e) * p.wt);
-            if (p.charge > 0) annih_at_rest(env, s, p, rd, sink, t);
-            return;
-        }
-        if (p.wt <= 0.0f) return;
-        float eke = eie - RM;
-        const int mq = medium == (int)VACUUM ? 0 : medium;
-        const gegs_float2 *dedx = mt->dedx + qel * MXEKE * MXMED + mq * MXEKE;
-        if (is_tstep) {
-            compute_tstep = true;
-            if (medium != (int)VACUUM) {
-                float rn = s.next();
-                if (rn == 0.0f) rn = 1.0e-30f;
-                demfp = fmaxf(-glog(rn), (float)EPSEMFP);
-                elke = glog(eke);
-                const gegs_float2 ek = mt->eke01[medium];
-                lelke = (int)(ek.x + ek.y * elke);
-                if (mt->sig_ismonotone[medium * 2 + qel]) {
-                    const gegs_float2 sd = mt->sig[qel * MXEKE * MXMED + medium * MXEKE + lelke - 1];
-                    const gegs_float2 dd = dedx[lelke - 1];
-                    float sigf = sd.x + sd.y * elke;
-                    sigf /= dd.x + dd.y * elke;
-                    sig0 = sigf;
-                } else {
-                    sig0 = mt->sig_e[qel * MXMED + medium];
-                }
-            }
-        }
-        float sig_val = 0.0f, dedx_val = 0.0f, tstep = 0.0f, ustep = 0.0f, tustep = 0.0f;
-        float tvstep = 0.0f, vstep = 0.0f, range = 0.0f, de = 0.0f;
-        float blccl = 0.0f, xccl = 0.0f, beta2 = 0.0f, chia2 = 0.0f, xi = 0.0f, tperp = 0.0f;
-        bool callhowfar = false, domultiple = false, callmsdist = false;
-        if (medium == (int)VACUUM) {
-            tstep = VACUUM_STEP;
-            ustep = tstep;
-            tustep = ustep;
-            callhowfar = true;
-        } else {
-            const gegs_float2 ek = mt->eke01[medium];
-            sig_val = sig0;
-            if (sig_val <= 0.0f) {
-                tstep = VACUUM_STEP;
-                sig0 = 1.0e-15f;
-            } else {
-                if (compute_tstep) {
-                    const float ekef = eke - demfp / sig_val;
-                    if (ekef <= mt->e_array[medium * MXEKE]) {
-                        tstep = VACUUM_STEP;
-                    } else {
-                        const float elkef = glog(ekef);
-                        const int lelkef = (int)(ek.x + ek.y * elkef);
-                        if (lelkef == lelke) {
-                            tstep = path_in_bin(dedx, eke, elke, ekef, elkef, lelke);
-                        } else {
-                            const float ekei = mt->e_array[medium * MXEKE + lelke - 1];
-                            const float elkei = ((float)lelke - ek.x) / ek.y;
-                            const float tuss = path_in_bin(dedx, eke, elke, ekei, elkei, lelke);
-                            const float ekej = mt->e_array[medium * MXEKE + lelkef];
-                            const float elkej = ((float)(lelkef + 1) - ek.x) / ek.y;
-                            tstep = path_in_bin(dedx, ekej, elkej, ekef, elkef, lelkef);
-                            tstep = tstep + tuss + mt->range_ep[(medium * MXEKE + lelke - 1) * 2 + qel] -
-                                    mt->range_ep[(medium * MXEKE + lelkef) * 2 + qel];
-                        }
-                    }
-                    total_tstep = tstep;
-                    compute_tstep = false;
-                }
-                tstep = total_tstep / rhof;
-            }
-            const gegs_float2 dd = dedx[lelke - 1];
-            dedx_val = rhof * (dd.x + dd.y * elke);
-            const gegs_float2 tm = mt->tmxs[medium * MXEKE + lelke - 1];
-            float tmxs_val = tm.x + tm.y * elke;
-            tmxs_val /= rhof;
-            const float ekei = mt->e_array[medium * MXEKE + lelke - 1];
-            const float elkei = ((float)lelke - ek.x) / ek.y;
-            range = path_in_bin(dedx, eke, elke, ekei, elkei, lelke);
-            range = (range + mt->range_ep[(medium * MXEKE + lelke - 1) * 2 + qel]) / rhof;
-            tmxs_val = fminf(tmxs_val, mt->smaxir[medium]);
-            tustep = fminf(tstep, fminf(tmxs_val, range));
-            tperp = hownear_phantom(env, p);
-            p.dnear = tperp;
-            if (mt->i_do_rr[medium] == 1 && p.e < mt->e_max_rr[medium] && tperp >= range) {
-                t.score(p.region, to_kinetic(p.e) * p.wt);
-                if (lelec > 0) annih_at_rest(env, s, p, rd, sink, t);
-                return;
-            }
-            blccl = rhof * mt->blcc[medium];
-            xccl = rhof * mt->xcc[medium];
-            const float p2 = eke * (eke + RMT2);
-            beta2 = p2 / (p2 + RMSQ);
-            const float ssmfp = beta2 / blccl;
-            const float skindepth = mt->skindepth_for_bca * ssmfp;
-            tustep = fminf(tustep, fmaxf(tperp, skindepth));
-            if (tustep <= tperp && (!mt->exact_bca || tustep > skindepth)) {
-                callmsdist = true;
-                de = eloss_over(mt, dedx, qel, medium, range, rhof, tustep, eke, elke, lelke);
-                tvstep = tustep;
-                ustep = msdist_pI(mt, s, de, tustep, p, rd);
-            } else {
-                // inexact boundary-crossing algorithm
-                domultiple = true;
-                const float ekems = eke - 0.5f * tustep * dedx_val;
-                const float p2m = ekems * (ekems + RMT2);
-                beta2 = p2m / (p2m + RMSQ);
-                chia2 = xccl / (4.0f * blccl * p2m);
-                xi = 0.5f * xccl / p2m / beta2 * tustep;
-                xi = xi * (glog(1.0f + 1.0f / chia2) - 1.0f / (1.0f + chia2));
-                if (xi < 0.1f) ustep = tustep * (1.0f - xi * (0.5f - xi * 0.166667f));
-                else ustep = tustep * (1.0f - gexp(-xi)) / xi;
-                callhowfar = !(ustep < tperp);
-            }
-        }
-        uint32_t irnew = p.region;
-        int idisc = 0;
-        const float ustep0 = ustep;
-        if (callhowfar || p.wt <= 0.0f) irnew = howfar_phantom(env, p, ustep);
-        if (irnew == 0) idisc = rd.med == VACUUM ? 1 : -1;
-        if (idisc > 0) {
-            p.region = 0;
-            return;
-        }
-        if (ustep <= 0.0f) ustep = 0.0f;
-        if (ustep == 0.0f || medium == (int)VACUUM) {
-            if (ustep != 0.0f) {
-                vstep = ustep;
-                p.x = p.x + p.u * vstep;
-                p.y = p.y + p.v * vstep;
-                p.z = p.z + p.w * vstep;
-                p.dnear = p.dnear - vstep;
-            }
-            if (p.region != irnew) t.gathers++;
-            p.region = irnew;
-            rd = env.reg[irnew];
-            if (eie <= rd.ecut) {
-                t.score(p.region, to_kinetic(p.e) * p.wt);
-                if (p.charge > 0) annih_at_rest(env, s, p, rd, sink, t);
-                return;
-            }
-            if (ustep != 0.0f && idisc < 0) {
-                t.score(p.region, (float)((double)p.e + (double)p.charge * PRM) * p.wt);
-                return;
-            }
-            is_tstep = true;
-            continue;
-        }
-        vstep = ustep;
-        if (callhowfar) {
-            if (vstep < ustep0) {
-                const float ekems = eke - 0.5f * tustep * vstep / ustep0 * dedx_val;
-                const float p2m = ekems * (ekems + RMT2);
-                beta2 = p2m / (p2m + RMSQ);
-                chia2 = xccl / (4.0f * blccl * p2m);
-                xi = 0.5f * xccl / p2m / beta2 * vstep;
-                xi = xi * (glog(1.0f + 1.0f / chia2) - 1.0f / (1.0f + chia2));
-                if (xi < 0.1f) {
-                    tvstep = vstep * (1.0f + xi * (0.5f + xi * 0.333333f));
-                } else if (xi < 0.999999f) {
-                    tvstep = -vstep * glog(1.0f - xi) / xi;
-                } else {
-                    return;
-                }
-            } else {
-                tvstep = tustep;
-            }
-            de = eloss_over(mt, dedx, qel, medium, range, rhof, tvstep, eke, elke, lelke);
-        } else {
-            tvstep = tustep;
-            if (!callmsdist) de = eloss_over(mt, dedx, qel, medium, range, rhof, tvstep, eke, elke, lelke);
-        }
-        const float save_de = de;
-        if (!callmsdist) {
-            float sinthe = 0.0f, costhe = 1.0f;
-            if (domultiple) {
-                const float lambda = blccl * tvstep / beta2 / (1.0f + chia2);
-                mscat(mt, s, lambda, chia2, xi, costhe, sinthe);
-            }
-            p.x = p.x + p.u * vstep;
-            p.y = p.y + p.v * vstep;
-            p.z = p.z + p.w * vstep;
-            if (domultiple) {
-                float sinphi, cosphi;
-                uphi21(s, costhe, sinthe, cosphi, sinphi, p);
-            }
-        }
-        p.dnear = p.dnear - vstep;
-        const uint32_t irold = p.region;
-        eie = eie - de;
-        p.e = eie;
-        t.score(irold, de * p.wt);
-        if (irnew == irl && eie <= rd.ecut) {
-            t.score(p.region, to_kinetic(p.e) * p.wt);
-            if (p.charge > 0) annih_at_rest(env, s, p, rd, sink, t);
-            return;
-        }
-        const int medold = medium;
+    float &elke = st.elke, &demfp = st.demfp, &sig0 = st.sig0, &total_tstep = st.total_tstep;
+    int &lelke = st.lelke;
+    bool &compute_tstep = st.compute_tstep, &is_tstep = st.is_tstep;
+    uint32_t irl = p.region;
+    gegs_region rd = env.reg[irl];
+    int medium = rd.med;
+    const float rhof = rd.rhof;
+    const int lelec = p.charge;
+    const int qel = (1 + lelec) / 2;
+    float eie = p.e;
+    t.electron_steps++;
+    t.steps++;
+    if (eie <= rd.ecut) {
+        t.score(p.region, to_kinetic(p.e) * p.wt);
+        if (p.charge > 0) annih_at_rest(env, s, p, rd, sink, t);
+        return false;
+    }
+    if (p.wt <= 0.0f) return false;
+    float eke = eie - RM;
+    const int mq = medium == (int)VACUUM ? 0 : medium;
+    const gegs_float2 *dedx = mt->dedx + qel * MXEKE * MXMED + mq * MXEKE;
+    if (is_tstep) {
+        compute_tstep = true;
         if (medium != (int)VACUUM) {
-            eke = eie - RM;
+            float rn = s.next();
+            if (rn == 0.0f) rn = 1.0e-30f;
+            demfp = fmaxf(-glog(rn), (float)EPSEMFP);
             elke = glog(eke);
             const gegs_float2 ek = mt->eke01[medium];
             lelke = (int)(ek.x + ek.y * elke);
+            if (mt->sig_ismonotone[medium * 2 + qel]) {
+                const gegs_float2 sd = mt->sig[qel * MXEKE * MXMED + medium * MXEKE + lelke - 1];
+                const gegs_float2 dd = dedx[lelke - 1];
+                float sigf = sd.x + sd.y * elke;
+                sigf /= dd.x + dd.y * elke;
+                sig0 = sigf;
+            } else {
+                sig0 = mt->sig_e[qel * MXMED + medium];
+            }
         }
-        if (irnew != irold) {
-            p.region = irnew;
-            rd = env.reg[irnew];
-            medium = rd.med;
-            t.gathers++;
+    }
+    float sig_val = 0.0f, dedx_val = 0.0f, tstep = 0.0f, ustep = 0.0f, tustep = 0.0f;
+    float tvstep = 0.0f, vstep = 0.0f, range = 0.0f, de = 0.0f;
+    float blccl = 0.0f, xccl = 0.0f, beta2 = 0.0f, chia2 = 0.0f, xi = 0.0f, tperp = 0.0f;
+    bool callhowfar = false, domultiple = false, callmsdist = false;
+    if (medium == (int)VACUUM) {
+        tstep = VACUUM_STEP;
+        ustep = tstep;
+        tustep = ustep;
+        callhowfar = true;
+    } else {
+        const gegs_float2 ek = mt->eke01[medium];
+        sig_val = sig0;
+        if (sig_val <= 0.0f) {
+            tstep = VACUUM_STEP;
+            sig0 = 1.0e-15f;
+        } else {
+            if (compute_tstep) {
+                const float ekef = eke - demfp / sig_val;
+                if (ekef <= mt->e_array[medium * MXEKE]) {
+                    tstep = VACUUM_STEP;
+                } else {
+                    const float elkef = glog(ekef);
+                    const int lelkef = (int)(ek.x + ek.y * elkef);
+                    if (lelkef == lelke) {
+                        tstep = path_in_bin(dedx, eke, elke, ekef, elkef, lelke);
+                    } else {
+                        const float ekei = mt->e_array[medium * MXEKE + lelke - 1];
+                        const float elkei = ((float)lelke - ek.x) / ek.y;
+                        const float tuss = path_in_bin(dedx, eke, elke, ekei, elkei, lelke);
+                        const float ekej = mt->e_array[medium * MXEKE + lelkef];
+                        const float elkej = ((float)(lelkef + 1) - ek.x) / ek.y;
+                        tstep = path_in_bin(dedx, ekej, elkej, ekef, elkef, lelkef);
+                        tstep = tstep + tuss + mt->range_ep[(medium * MXEKE + lelke - 1) * 2 + qel] -
+                                mt->range_ep[(medium * MXEKE + lelkef) * 2 + qel];
+                    }
+                }
+                total_tstep = tstep;
+                compute_tstep = false;
+            }
+            tstep = total_tstep / rhof;
         }
+        const gegs_float2 dd = dedx[lelke - 1];
+        dedx_val = rhof * (dd.x + dd.y * elke);
+        const gegs_float2 tm = mt->tmxs[medium * MXEKE + lelke - 1];
+        float tmxs_val = tm.x + tm.y * elke;
+        tmxs_val /= rhof;
+        const float ekei = mt->e_array[medium * MXEKE + lelke - 1];
+        const float elkei = ((float)lelke - ek.x) / ek.y;
+        range = path_in_bin(dedx, eke, elke, ekei, elkei, lelke);
+        range = (range + mt->range_ep[(medium * MXEKE + lelke - 1) * 2 + qel]) / rhof;
+        tmxs_val = fminf(tmxs_val, mt->smaxir[medium]);
+        tustep = fminf(tstep, fminf(tmxs_val, range));
+        tperp = hownear_phantom(env, p);
+        p.dnear = tperp;
+        if (mt->i_do_rr[medium] == 1 && p.e < mt->e_max_rr[medium] && tperp >= range) {
+            t.score(p.region, to_kinetic(p.e) * p.wt);
+            if (lelec > 0) annih_at_rest(env, s, p, rd, sink, t);
+            return false;
+        }
+        blccl = rhof * mt->blcc[medium];
+        xccl = rhof * mt->xcc[medium];
+        const float p2 = eke * (eke + RMT2);
+        beta2 = p2 / (p2 + RMSQ);
+        const float ssmfp = beta2 / blccl;
+        const float skindepth = mt->skindepth_for_bca * ssmfp;
+        tustep = fminf(tustep, fmaxf(tperp, skindepth));
+        if (tustep <= tperp && (!mt->exact_bca || tustep > skindepth)) {
+            callmsdist = true;
+            de = eloss_over(mt, dedx, qel, medium, range, rhof, tustep, eke, elke, lelke);
+            tvstep = tustep;
+            ustep = msdist_pI(mt, s, de, tustep, p, rd);
+        } else {
+            // inexact boundary-crossing algorithm
+            domultiple = true;
+            const float ekems = eke - 0.5f * tustep * dedx_val;
+            const float p2m = ekems * (ekems + RMT2);
+            beta2 = p2m / (p2m + RMSQ);
+            chia2 = xccl / (4.0f * blccl * p2m);
+            xi = 0.5f * xccl / p2m / beta2 * tustep;
+            xi = xi * (glog(1.0f + 1.0f / chia2) - 1.0f / (1.0f + chia2));
+            if (xi < 0.1f) ustep = tustep * (1.0f - xi * (0.5f - xi * 0.166667f));
+            else ustep = tustep * (1.0f - gexp(-xi)) / xi;
+            callhowfar = !(ustep < tperp);
+        }
+    }
+    uint32_t irnew = p.region;
+    int idisc = 0;
+    const float ustep0 = ustep;
+    if (callhowfar || p.wt <= 0.0f) irnew = howfar_phantom(env, p, ustep);
+    if (irnew == 0) idisc = rd.med == VACUUM ? 1 : -1;
+    if (idisc > 0) {
+        p.region = 0;
+        return false;
+    }
+    if (ustep <= 0.0f) ustep = 0.0f;
+    if (ustep == 0.0f || medium == (int)VACUUM) {
+        if (ustep != 0.0f) {
+            vstep = ustep;
+            p.x = p.x + p.u * vstep;
+            p.y = p.y + p.v * vstep;
+            p.z = p.z + p.w * vstep;
+            p.dnear = p.dnear - vstep;
+        }
+        if (p.region != irnew) t.gathers++;
+        p.region = irnew;
+        rd = env.reg[irnew];
         if (eie <= rd.ecut) {
             t.score(p.region, to_kinetic(p.e) * p.wt);
             if (p.charge > 0) annih_at_rest(env, s, p, rd, sink, t);
-            return;
+            return false;
         }
-        if (idisc < 0) {
+        if (ustep != 0.0f && idisc < 0) {
             t.score(p.region, (float)((double)p.e + (double)p.charge * PRM) * p.wt);
-            return;
+            return false;
         }
-        if (medium != medold) {
-            is_tstep = true;
-            continue;
-        }
-        demfp = demfp - save_de * sig_val;
-        total_tstep = total_tstep - tvstep * rhof;
-        if (total_tstep < 1e-9f) demfp = 0.0f;
-        if ((double)demfp >= EPSEMFP) {
-            is_tstep = false;
-            continue;
-        }
-        const gegs_float2 sd = mt->sig[qel * MXEKE * MXMED + medium * MXEKE + lelke - 1];
-        const gegs_float2 dd = mt->dedx[qel * MXEKE * MXMED + medium * MXEKE + lelke - 1];
-        float sigf = sd.x + sd.y * elke;
-        sigf /= dd.x + dd.y * elke;
-        const float sigratio = sigf / sig0;
-        if (s.next() > sigratio) {
-            is_tstep = true;
-            continue;
-        }
-        if (lelec < 0) {
-            const gegs_float2 b1 = mt->ebr1[medium * MXEKE + lelke - 1];
-            const float ebr1 = b1.x + b1.y * elke;
-            if (s.next() <= ebr1) { forward(sink, s, p, E_BREMS); return; }
-            if (p.e <= mt->thmoll[medium] && mt->eii_flag == 0) {
-                if (ebr1 <= 0.0f) { is_tstep = true; continue; }
-                forward(sink, s, p, E_BREMS);
-                return;
+        is_tstep = true;
+        return true;
+    }
+    vstep = ustep;
+    if (callhowfar) {
+        if (vstep < ustep0) {
+            const float ekems = eke - 0.5f * tustep * vstep / ustep0 * dedx_val;
+            const float p2m = ekems * (ekems + RMT2);
+            beta2 = p2m / (p2m + RMSQ);
+            chia2 = xccl / (4.0f * blccl * p2m);
+            xi = 0.5f * xccl / p2m / beta2 * vstep;
+            xi = xi * (glog(1.0f + 1.0f / chia2) - 1.0f / (1.0f + chia2));
+            if (xi < 0.1f) {
+                tvstep = vstep * (1.0f + xi * (0.5f + xi * 0.333333f));
+            } else if (xi < 0.999999f) {
+                tvstep = -vstep * glog(1.0f - xi) / xi;
+            } else {
+                return false;
             }
-            forward(sink, s, p, E_MOLLER);
-            return;
+        } else {
+            tvstep = tustep;
         }
-        const gegs_float2 pb1 = mt->pbr1[medium * MXEKE + lelke - 1];
-        const float pbr1 = pb1.x + pb1.y * elke;
-        const float r25 = s.next();
-        if (r25 < pbr1) { forward(sink, s, p, E_BREMS); return; }
-        const gegs_float2 pb2 = mt->pbr2[medium * MXEKE + lelke - 1];
-        const float pbr2 = pb2.x + pb2.y * elke;
-        forward(sink, s, p, r25 < pbr2 ? E_BHABHA : E_ANNIH);
-        return;
+        de = eloss_over(mt, dedx, qel, medium, range, rhof, tvstep, eke, elke, lelke);
+    } else {
+        tvstep = tustep;
+        if (!callmsdist) de = eloss_over(mt, dedx, qel, medium, range, rhof, tvstep, eke, elke, lelke);
+    }
+    const float save_de = de;
+    if (!callmsdist) {
+        float sinthe = 0.0f, costhe = 1.0f;
+        if (domultiple) {
+            const float lambda = blccl * tvstep / beta2 / (1.0f + chia2);
+            mscat(mt, s, lambda, chia2, xi, costhe, sinthe);
+        }
+        p.x = p.x + p.u * vstep;
+        p.y = p.y + p.v * vstep;
+        p.z = p.z + p.w * vstep;
+        if (domultiple) {
+            float sinphi, cosphi;
+            uphi21(s, costhe, sinthe, cosphi, sinphi, p);
+        }
+    }
+    p.dnear = p.dnear - vstep;
+    const uint32_t irold = p.region;
+    eie = eie - de;
+    p.e = eie;
+    t.score(irold, de * p.wt);
+    if (irnew == irl && eie <= rd.ecut) {
+        t.score(p.region, to_kinetic(p.e) * p.wt);
+        if (p.charge > 0) annih_at_rest(env, s, p, rd, sink, t);
+        return false;
+    }
+    const int medold = medium;
+    if (medium != (int)VACUUM) {
+        eke = eie - RM;
+        elke = glog(eke);
+        const gegs_float2 ek = mt->eke01[medium];
+        lelke = (int)(ek.x + ek.y * elke);
+    }
+    if (irnew != irold) {
+        p.region = irnew;
+        rd = env.reg[irnew];
+        medium = rd.med;
+        t.gathers++;
+    }
+    if (eie <= rd.ecut) {
+        t.score(p.region, to_kinetic(p.e) * p.wt);
+        if (p.charge > 0) annih_at_rest(env, s, p, rd, sink, t);
+        return false;
+    }
+    if (idisc < 0) {
+        t.score(p.region, (float)((double)p.e + (double)p.charge * PRM) * p.wt);
+        return false;
+    }
+    if (medium != medold) {
+        is_tstep = true;
+        return true;
+    }
+    demfp = demfp - save_de * sig_val;
+    total_tstep = total_tstep - tvstep * rhof;
+    if (total_tstep < 1e-9f) demfp = 0.0f;
+    if ((double)demfp >= EPSEMFP) {
+        is_tstep = false;
+        return true;
+    }
+    const gegs_float2 sd = mt->sig[qel * MXEKE * MXMED + medium * MXEKE + lelke - 1];
+    const gegs_float2 dd = mt->dedx[qel * MXEKE * MXMED + medium * MXEKE + lelke - 1];
+    float sigf = sd.x + sd.y * elke;
+    sigf /= dd.x + dd.y * elke;
+    const float sigratio = sigf / sig0;
+    if (s.next() > sigratio) {
+        is_tstep = true;
+        return true;
+    }
+    if (lelec < 0) {
+        const gegs_float2 b1 = mt->ebr1[medium * MXEKE + lelke - 1];
+        const float ebr1 = b1.x + b1.y * elke;
+        if (s.next() <= ebr1) { forward(sink, s, p, E_BREMS); return false; }
+        if (p.e <= mt->thmoll[medium] && mt->eii_flag == 0) {
+            if (ebr1 <= 0.0f) { is_tstep = true; return true; }
+            forward(sink, s, p, E_BREMS);
+            return false;
+        }
+        forward(sink, s, p, E_MOLLER);
+        return false;
+    }
+    const gegs_float2 pb1 = mt->pbr1[medium * MXEKE + lelke - 1];
+    const float pbr1 = pb1.x + pb1.y * elke;
+    const float r25 = s.next();
+    if (r25 < pbr1) { forward(sink, s, p, E_BREMS); return false; }
+    const gegs_float2 pb2 = mt->pbr2[medium * MXEKE + lelke - 1];
+    const float pbr2 = pb2.x + pb2.y * elke;
+    forward(sink, s, p, r25 < pbr2 ? E_BHABHA : E_ANNIH);
+    return false;
+}
+
+template <class Sink>
+__device__ __forceinline__ void electron_step(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+    EState st;
+    st.init();
+    while (electron_iter(env, s, p, st, sink, t)) {
     }
 }
 
