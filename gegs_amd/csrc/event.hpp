@@ -29,23 +29,31 @@
 
 namespace gegs {
 
-constexpr int EVW = 16;   // words per banked particle
+constexpr int EVW = 20;        // words per banked particle (15-19: state of a suspended electron step)
+constexpr int SLOT_P_ALT = N_EVENTS;         // second buffer of the photon-step bank
+constexpr int SLOT_E_ALT = N_EVENTS + 1;     // second buffer of the electron-step bank
+constexpr int N_SLOTS = N_EVENTS + 2;
 
 struct EvState {
-    uint32_t count[N_EVENTS];        // particles in each bank
-    uint32_t taken[N_EVENTS];        // work handed out from each bank
-    uint32_t done_blocks[N_EVENTS];  // blocks finished, for the bank reset
+    uint32_t count[N_SLOTS];         // particles in each bank
+    uint32_t taken[N_SLOTS];         // work handed out from each bank
     uint32_t overflow;               // particles that did not fit
     uint32_t active;                 // more work remains after the sweep
+    uint32_t sweeps;
+    uint32_t flip;                   // buffer of banks 0 and 5 drained this sweep
     uint32_t src_next, src_last;     // source records still to inject
     uint32_t inj_start, inj_n;       // current injection
-    uint32_t sweeps;
-    uint32_t pad[3];
 };
 
 struct EventConfig {
-    uint32_t block = 256;            // threads per block of the event kernels
-    uint32_t grid = 1024;            // blocks of the event kernels
+    uint32_t block = 256;            // threads per block of the single-pass event kernels (<= 256)
+    uint32_t flight_block = 256;     // threads per block of the photon and electron kernels (<= 512)
+    uint32_t grid = 2048;            // blocks of the single-pass interaction kernels
+    uint32_t photon_grid = 0;        // blocks of the photon kernel (0: fill the device once)
+    uint32_t electron_grid = 0;      // blocks of the electron kernel (0: fill the device once)
+    uint32_t photon_chunk = 32;      // voxel crossings per photon and sweep (0: whole flight)
+    uint32_t electron_chunk = 16;    // condensed-history steps per electron and sweep (0: no limit)
+    uint32_t refill_min = 16;        // idle lanes of a wavefront that trigger a refill
     size_t capacity = 0;             // slots per bank (0 = 4 Mi)
     uint32_t inject = 1u << 20;      // primaries injected per refill
     uint32_t threshold = 1u << 20;   // refill when fewer particles are in flight
@@ -54,7 +62,7 @@ struct EventConfig {
 
 struct EvArgs {
     Env env;
-    float *bank;                     // [event][word][cap]
+    float *bank;                     // [slot][word][cap]
     uint32_t cap;
     EvState *st;
     const gegs_source *src;
@@ -72,59 +80,74 @@ __device__ __forceinline__ uint64_t lanes_below() {
     return l == 0 ? 0ull : (~0ull >> (64 - l));
 }
 
-__device__ __forceinline__ float *bank_word(const EvArgs &a, int e, int w, uint32_t idx) {
-    return a.bank + ((size_t)e * EVW + w) * a.cap + idx;
+// bank buffer of event e drained in this sweep (now) or in the next one
+__device__ __forceinline__ int bank_slot(int e, uint32_t flip, bool now) {
+    const bool alt = (flip != 0) == now;
+    if (e == P_PHOTON_STEP) return alt ? SLOT_P_ALT : P_PHOTON_STEP;
+    if (e == E_ELECTRON_STEP) return alt ? SLOT_E_ALT : E_ELECTRON_STEP;
+    return e;
 }
 
-__device__ __forceinline__ void bank_store(const EvArgs &a, int e, uint32_t idx, const Part &p, uint32_t hist) {
-    *bank_word(a, e, 0, idx) = p.e;
-    *bank_word(a, e, 1, idx) = p.x;
-    *bank_word(a, e, 2, idx) = p.y;
-    *bank_word(a, e, 3, idx) = p.z;
-    *bank_word(a, e, 4, idx) = p.u;
-    *bank_word(a, e, 5, idx) = p.v;
-    *bank_word(a, e, 6, idx) = p.w;
-    *bank_word(a, e, 7, idx) = p.wt;
-    *bank_word(a, e, 8, idx) = p.dnear;
-    *bank_word(a, e, 9, idx) = __uint_as_float(p.region);
-    *bank_word(a, e, 10, idx) = __uint_as_float((uint32_t)(p.charge + 1) | ((uint32_t)p.module << 16));
-    *bank_word(a, e, 11, idx) = __uint_as_float((uint32_t)p.lin);
-    *bank_word(a, e, 12, idx) = __uint_as_float((uint32_t)(p.lin >> 32));
-    *bank_word(a, e, 13, idx) = __uint_as_float(p.blk);
-    *bank_word(a, e, 14, idx) = __uint_as_float(hist);
+__device__ __forceinline__ float *bank_word(const EvArgs &a, int slot, int w, uint32_t idx) {
+    return a.bank + ((size_t)slot * EVW + w) * a.cap + idx;
 }
 
-__device__ __forceinline__ Part bank_load(const EvArgs &a, int e, uint32_t idx, uint32_t &hist) {
+__device__ __forceinline__ void bank_store(const EvArgs &a, int slot, uint32_t idx, const Part &p, uint32_t hist) {
+    *bank_word(a, slot, 0, idx) = p.e;
+    *bank_word(a, slot, 1, idx) = p.x;
+    *bank_word(a, slot, 2, idx) = p.y;
+    *bank_word(a, slot, 3, idx) = p.z;
+    *bank_word(a, slot, 4, idx) = p.u;
+    *bank_word(a, slot, 5, idx) = p.v;
+    *bank_word(a, slot, 6, idx) = p.w;
+    *bank_word(a, slot, 7, idx) = p.wt;
+    *bank_word(a, slot, 8, idx) = p.dnear;
+    *bank_word(a, slot, 9, idx) = __uint_as_float(p.region);
+    *bank_word(a, slot, 10, idx) =
+        __uint_as_float((uint32_t)(p.charge + 1) | ((uint32_t)p.spos << 8) | ((uint32_t)p.module << 16));
+    *bank_word(a, slot, 11, idx) = __uint_as_float((uint32_t)p.lin);
+    *bank_word(a, slot, 12, idx) = __uint_as_float((uint32_t)(p.lin >> 32));
+    *bank_word(a, slot, 13, idx) = __uint_as_float(p.blk);
+    *bank_word(a, slot, 14, idx) = __uint_as_float(hist);
+}
+
+__device__ __forceinline__ Part bank_load(const EvArgs &a, int slot, int event, uint32_t idx, uint32_t &hist) {
     Part p;
-    p.e = *bank_word(a, e, 0, idx);
-    p.x = *bank_word(a, e, 1, idx);
-    p.y = *bank_word(a, e, 2, idx);
-    p.z = *bank_word(a, e, 3, idx);
-    p.u = *bank_word(a, e, 4, idx);
-    p.v = *bank_word(a, e, 5, idx);
-    p.w = *bank_word(a, e, 6, idx);
-    p.wt = *bank_word(a, e, 7, idx);
-    p.dnear = *bank_word(a, e, 8, idx);
-    p.region = __float_as_uint(*bank_word(a, e, 9, idx));
-    const uint32_t pk = __float_as_uint(*bank_word(a, e, 10, idx));
+    p.e = *bank_word(a, slot, 0, idx);
+    p.x = *bank_word(a, slot, 1, idx);
+    p.y = *bank_word(a, slot, 2, idx);
+    p.z = *bank_word(a, slot, 3, idx);
+    p.u = *bank_word(a, slot, 4, idx);
+    p.v = *bank_word(a, slot, 5, idx);
+    p.w = *bank_word(a, slot, 6, idx);
+    p.wt = *bank_word(a, slot, 7, idx);
+    p.dnear = *bank_word(a, slot, 8, idx);
+    p.region = __float_as_uint(*bank_word(a, slot, 9, idx));
+    const uint32_t pk = __float_as_uint(*bank_word(a, slot, 10, idx));
     p.charge = (int32_t)(pk & 0xffu) - 1;
+    p.spos = (int32_t)((pk >> 8) & 15u);
     p.module = (int32_t)(pk >> 16);
-    p.status = e;
-    p.lin = (uint64_t)__float_as_uint(*bank_word(a, e, 11, idx)) |
-            ((uint64_t)__float_as_uint(*bank_word(a, e, 12, idx)) << 32);
-    p.blk = __float_as_uint(*bank_word(a, e, 13, idx));
-    hist = __float_as_uint(*bank_word(a, e, 14, idx));
+    p.status = event;
+    p.lin = (uint64_t)__float_as_uint(*bank_word(a, slot, 11, idx)) |
+            ((uint64_t)__float_as_uint(*bank_word(a, slot, 12, idx)) << 32);
+    p.blk = __float_as_uint(*bank_word(a, slot, 13, idx));
+    hist = __float_as_uint(*bank_word(a, slot, 14, idx));
     return p;
 }
 
 // Appends to the bank named by p.status.  The lanes that reach a push
 // together are grouped by destination; each group reserves its slots with
-// one atomic issued by its first lane.
+// one atomic issued by its first lane.  Photon-step and electron-step pushes
+// go to the buffer drained in this sweep when their kernel has not run yet
+// in the sweep (photons_now / electrons_now), else to the next sweep's.
 struct BankSink {
     const EvArgs *a;
     uint32_t hist;
+    uint32_t flip;
+    bool photons_now, electrons_now;
     __device__ __forceinline__ void push(const Part &p) {
-        const int dest = p.status;
+        const int e = p.status;
+        const int dest = bank_slot(e, flip, e == P_PHOTON_STEP ? photons_now : electrons_now);
         for (;;) {
             const int d0 = __builtin_amdgcn_readfirstlane(dest);
             if (dest == d0) {
@@ -140,7 +163,40 @@ struct BankSink {
             }
         }
     }
+    // an electron suspended between two condensed-history steps carries the
+    // step state (electron_iter's EState) to the sweep that resumes it
+    __device__ __forceinline__ void push_suspended(const Part &p, const EState &es) {
+        const int dest = bank_slot(p.status, flip, electrons_now);
+        const uint64_t m = __ballot(1);
+        const uint32_t rank = (uint32_t)__popcll(m & lanes_below());
+        uint32_t base = 0;
+        if (rank == 0) base = atomicAdd(&a->st->count[dest], (uint32_t)__popcll(m));
+        base = __builtin_amdgcn_readfirstlane(base);
+        const uint32_t idx = base + rank;
+        if (idx < a->cap) {
+            bank_store(*a, dest, idx, p, hist);
+            *bank_word(*a, dest, 15, idx) = es.elke;
+            *bank_word(*a, dest, 16, idx) = es.demfp;
+            *bank_word(*a, dest, 17, idx) = es.sig0;
+            *bank_word(*a, dest, 18, idx) = es.total_tstep;
+            *bank_word(*a, dest, 19, idx) = __uint_as_float((uint32_t)es.lelke | ((uint32_t)es.compute_tstep << 16) |
+                                                            ((uint32_t)es.is_tstep << 17));
+        } else {
+            atomicAdd(&a->st->overflow, 1u);
+        }
+    }
 };
+
+__device__ __forceinline__ void load_estate(const EvArgs &a, int slot, uint32_t idx, EState &es) {
+    es.elke = *bank_word(a, slot, 15, idx);
+    es.demfp = *bank_word(a, slot, 16, idx);
+    es.sig0 = *bank_word(a, slot, 17, idx);
+    es.total_tstep = *bank_word(a, slot, 18, idx);
+    const uint32_t f = __float_as_uint(*bank_word(a, slot, 19, idx));
+    es.lelke = (int)(f & 0xffffu);
+    es.compute_tstep = (f >> 16) & 1u;
+    es.is_tstep = (f >> 17) & 1u;
+}
 
 __device__ __forceinline__ void flush_particle(const EvArgs &a, uint32_t hist, Tally &t) {
     if (a.hist_edep && t.hist_edep != 0.0) unsafeAtomicAdd(a.hist_edep + hist, t.hist_edep);
@@ -170,13 +226,6 @@ __device__ __forceinline__ void flush_counters(const EvArgs &a, const Tally &t, 
     }
 }
 
-// empties a bank once its kernel has drained it (a single thread; runs
-// between the kernel that drains the bank and the first kernel that refills it)
-__global__ void reset_bank(EvState *st, int event) {
-    st->count[event] = 0;
-    st->taken[event] = 0;
-}
-
 __device__ __forceinline__ Tally fresh_tally(const EvArgs &a) {
     Tally t;
     t.dose = a.dose;
@@ -194,7 +243,8 @@ __global__ void __launch_bounds__(256) interaction_kernel(EvArgs a) {
     const uint32_t n = a.st->count[E];
     if (n == 0) return;
     Tally t = fresh_tally(a);
-    BankSink sink{&a, 0};
+    // photon interactions run before the electron kernel of the sweep
+    BankSink sink{&a, 0, a.st->flip, false, E < E_ELECTRON_STEP};
     uint32_t pops = 0;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
@@ -202,7 +252,7 @@ __global__ void __launch_bounds__(256) interaction_kernel(EvArgs a) {
         const uint32_t idx = base + lane_id();
         if (idx < n) {
             uint32_t hist;
-            Part p = bank_load(a, E, idx, hist);
+            Part p = bank_load(a, E, E, idx, hist);
             sink.hist = hist;
             Stream s;
             s.begin(p, a.env.seed);
@@ -231,13 +281,16 @@ struct Refill {
         next = end = 0;
         exhausted = false;
     }
-    // returns the slot for this lane (>= n: none); call with all lanes active
-    __device__ __forceinline__ uint32_t take(uint32_t *taken, uint32_t n, bool want) {
+    // returns the slot for this lane (>= n: none); call with all lanes active.
+    // Idle lanes are refilled together once at least min_idle of them wait
+    // (or the whole wavefront is idle), so that the loads that start a
+    // particle are paid once for many lanes rather than on every iteration.
+    __device__ __forceinline__ uint32_t take(uint32_t *taken, uint32_t n, bool want, uint32_t min_idle) {
         const uint64_t idle = __ballot(want);
         uint32_t need = (uint32_t)__popcll(idle);
         uint32_t slot = 0xFFFFFFFFu;
         const uint32_t rank = (uint32_t)__popcll(idle & lanes_below());
-        if (need == 0 || exhausted) return slot;
+        if (need == 0 || exhausted || (need < min_idle && need < 64u)) return slot;
         if (end - next < need) {
             // top up the reservation: keep the remainder, add 64 more
             uint32_t got = 0;
@@ -263,69 +316,105 @@ struct Refill {
     }
 };
 
-// photon free flight with lane refill
-__global__ void __launch_bounds__(256) photon_kernel(EvArgs a) {
-    const uint32_t n = a.st->count[P_PHOTON_STEP];
+// photon free flight with lane refill; a flight still going after `chunk`
+// crossings is suspended into the next sweep's bank
+template <bool TAB>
+__global__ void __launch_bounds__(512) photon_kernel(EvArgs a, uint32_t chunk, uint32_t refill_min) {
+    extern __shared__ float lds_env[];
+    const uint32_t flip = a.st->flip;
+    const int slot_in = bank_slot(P_PHOTON_STEP, flip, true);
+    const uint32_t n = a.st->count[slot_in];
     if (n == 0) return;
+    Env env = a.env;
+    stage_env<TAB>(env, lds_env);
     Tally t = fresh_tally(a);
-    BankSink sink{&a, 0};
+    BankSink sink{&a, 0, flip, false, true};
     uint32_t pops = 0;
     bool have = false;
     Part p;
     Stream s;
-    uint32_t hist = 0;
+    PhotonWalk w;
+    uint32_t hist = 0, iters = 0;
     Refill rf;
     rf.init();
     for (;;) {
-        const uint32_t slot = rf.take(&a.st->taken[P_PHOTON_STEP], n, !have);
+        const uint32_t slot = rf.take(&a.st->taken[slot_in], n, !have, refill_min);
         if (!have && slot != 0xFFFFFFFFu) {
-            p = bank_load(a, P_PHOTON_STEP, slot, hist);
-            s.begin(p, a.env.seed);
+            p = bank_load(a, slot_in, P_PHOTON_STEP, slot, hist);
+            s.begin(p, env.seed);
+            photon_walk_begin(env, p, w);
             have = true;
-            pops++;
+            iters = 0;
+            pops += (p.spos & 8) ? 0u : 1u;    // flights, not resumptions
         }
         if (!__any(have)) break;
         if (have) {
             sink.hist = hist;
-            if (!photon_iter(a.env, s, p, sink, t)) {
+            if (chunk && iters == chunk) {
+                suspend(sink, s, p, P_PHOTON_STEP);
+                have = false;
+                flush_particle(a, hist, t);
+            } else if (!photon_walk_iter(env, s, p, w, sink, t)) {
                 have = false;
                 flush_particle(a, hist, t);
             }
+            iters++;
         }
     }
     flush_counters(a, t, P_PHOTON_STEP, pops);
 }
 
-// electron / positron condensed-history steps with lane refill
-__global__ void __launch_bounds__(256) electron_kernel(EvArgs a) {
-    const uint32_t n = a.st->count[E_ELECTRON_STEP];
+// electron / positron condensed-history steps with lane refill; an electron
+// still stepping after `chunk` steps is suspended into the next sweep's bank
+template <bool TAB>
+__global__ void __launch_bounds__(512) electron_kernel(EvArgs a, uint32_t chunk, uint32_t refill_min) {
+    extern __shared__ float lds_env[];
+    const uint32_t flip = a.st->flip;
+    const int slot_in = bank_slot(E_ELECTRON_STEP, flip, true);
+    const uint32_t n = a.st->count[slot_in];
     if (n == 0) return;
+    Env env = a.env;
+    stage_env<TAB>(env, lds_env);
     Tally t = fresh_tally(a);
-    BankSink sink{&a, 0};
+    BankSink sink{&a, 0, flip, false, false};
     uint32_t pops = 0;
     bool have = false;
     Part p;
     Stream s;
     EState es;
-    uint32_t hist = 0;
+    uint32_t hist = 0, iters = 0;
     Refill rf;
     rf.init();
     for (;;) {
-        const uint32_t slot = rf.take(&a.st->taken[E_ELECTRON_STEP], n, !have);
+        const uint32_t slot = rf.take(&a.st->taken[slot_in], n, !have, refill_min);
         if (!have && slot != 0xFFFFFFFFu) {
-            p = bank_load(a, E_ELECTRON_STEP, slot, hist);
-            s.begin(p, a.env.seed);
-            es.init();
+            p = bank_load(a, slot_in, E_ELECTRON_STEP, slot, hist);
+            s.begin(p, env.seed);
+            if (p.spos & 8) {
+                load_estate(a, slot_in, slot, es);
+            } else {
+                es.init();
+                pops++;
+            }
             have = true;
-            pops++;
+            iters = 0;
         }
         if (!__any(have)) break;
         if (have) {
             sink.hist = hist;
-            if (!electron_iter(a.env, s, p, es, sink, t)) {
+            if (chunk && iters == chunk) {
+                p.status = E_ELECTRON_STEP;
+                p.lin = s.lin;
+                p.blk = s.blk;
+                p.spos = s.pos | 8;
+                sink.push_suspended(p, es);
+                have = false;
+                flush_particle(a, hist, t);
+            } else if (!electron_iter(env, s, p, es, sink, t)) {
                 have = false;
                 flush_particle(a, hist, t);
             }
+            iters++;
         }
     }
     flush_counters(a, t, E_ELECTRON_STEP, pops);
@@ -334,7 +423,7 @@ __global__ void __launch_bounds__(256) electron_kernel(EvArgs a) {
 // plan the next injection: refill when few particles are in flight
 __global__ void inject_plan(EvState *st, uint32_t threshold, uint32_t max_inject) {
     uint32_t inflight = 0;
-    for (int e = 0; e < N_EVENTS; ++e) inflight += st->count[e];
+    for (int e = 0; e < N_SLOTS; ++e) inflight += st->count[e];
     uint32_t n = 0;
     if (inflight < threshold && st->src_next < st->src_last) {
         n = st->src_last - st->src_next;
@@ -348,7 +437,7 @@ __global__ void inject_plan(EvState *st, uint32_t threshold, uint32_t max_inject
 __global__ void __launch_bounds__(256) inject_fill(EvArgs a) {
     const uint32_t n = a.st->inj_n;
     const uint32_t start = a.st->inj_start;
-    BankSink sink{&a, 0};
+    BankSink sink{&a, 0, a.st->flip, true, true};
     uint32_t injected = 0;
     const uint32_t stride = gridDim.x * blockDim.x;
     // whole wavefronts iterate together so that the aggregated push sees
@@ -367,6 +456,7 @@ __global__ void __launch_bounds__(256) inject_fill(EvArgs a) {
             p.module = a.env.f.module_first;
             p.dnear = INFI_DIST;
             p.blk = 0;
+            p.spos = 4;
             p.lin = primary_lineage(a.hist_offset + rec);
             p.status = r.charge == 0 ? P_PHOTON_STEP : E_ELECTRON_STEP;
             sink.hist = rec;
@@ -379,26 +469,31 @@ __global__ void __launch_bounds__(256) inject_fill(EvArgs a) {
 }
 
 __global__ void sweep_end(EvState *st) {
-    // banks 1-4 are filled only by the photon kernel and 6-9 only by the
-    // electron kernel, both of which come before their consumers: all four
-    // interaction banks of each family are empty for the next sweep
+    // every bank drained in this sweep is empty: the eight single-pass banks
+    // and the current buffers of banks 0 and 5, which take the next sweep's
+    // pushes after the flip
     const int drained[8] = {P_COMPTON, P_PHOTO, P_RAYLEIGH, P_PAIR, E_MOLLER, E_BREMS, E_BHABHA, E_ANNIH};
     for (int k = 0; k < 8; ++k) {
         st->count[drained[k]] = 0;
         st->taken[drained[k]] = 0;
     }
+    const int p_in = st->flip ? SLOT_P_ALT : P_PHOTON_STEP;
+    const int e_in = st->flip ? SLOT_E_ALT : E_ELECTRON_STEP;
+    st->count[p_in] = st->taken[p_in] = 0;
+    st->count[e_in] = st->taken[e_in] = 0;
+    st->flip ^= 1u;
     uint32_t inflight = 0;
-    for (int e = 0; e < N_EVENTS; ++e) inflight += st->count[e];
+    for (int e = 0; e < N_SLOTS; ++e) inflight += st->count[e];
     st->active = (inflight > 0 || st->src_next < st->src_last) ? 1u : 0u;
     st->sweeps += 1;
 }
 
 __global__ void batch_begin(EvState *st, uint32_t first, uint32_t last) {
-    for (int e = 0; e < N_EVENTS; ++e) {
+    for (int e = 0; e < N_SLOTS; ++e) {
         st->count[e] = 0;
         st->taken[e] = 0;
-        st->done_blocks[e] = 0;
     }
+    st->flip = 0;
     st->src_next = first;
     st->src_last = last;
     st->inj_start = first;
@@ -437,7 +532,7 @@ struct EventBanks {
             if (bank) (void)hipFree(bank);
             bank = nullptr;
             cap = 0;
-            e = hipMalloc(&bank, want * N_EVENTS * EVW * sizeof(float));
+            e = hipMalloc(&bank, want * N_SLOTS * EVW * sizeof(float));
             if (e != hipSuccess) return e;
             cap = (uint32_t)want;
             drop_graph();
@@ -460,18 +555,37 @@ struct EventBanks {
     }
 };
 
+// blocks that keep every CU busy once, for the persistent flight kernels
+template <class K>
+inline uint32_t resident_grid(K kernel, uint32_t block, size_t lds) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 1024;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 1024;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, (int)block, lds) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    return (uint32_t)(per_cu * cus);
+}
+
+inline bool tab_in_lds(const Env &env) { return env.nclass <= LDS_MAX_CLASSES; }
+
+inline size_t flight_lds(const Env &env) {
+    return env_lds_bytes(env.nx, env.ny, env.nz, env.nclass, env.nmed, tab_in_lds(env));
+}
+
 inline void launch_sweep(const EvArgs &a, const EventConfig &cfg, hipStream_t stream) {
-    const dim3 g(cfg.grid), b(cfg.block);
+    const dim3 g(cfg.grid), b(cfg.block), fb(cfg.flight_block);
+    const size_t lds = flight_lds(a.env);
+    const bool tab = tab_in_lds(a.env);
     hipLaunchKernelGGL(inject_plan, dim3(1), dim3(1), 0, stream, a.st, cfg.threshold, cfg.inject);
     hipLaunchKernelGGL(inject_fill, g, b, 0, stream, a);
-    hipLaunchKernelGGL(photon_kernel, g, b, 0, stream, a);
-    hipLaunchKernelGGL(reset_bank, dim3(1), dim3(1), 0, stream, a.st, (int)P_PHOTON_STEP);
+    if (tab) hipLaunchKernelGGL(photon_kernel<true>, dim3(cfg.photon_grid), fb, lds, stream, a, cfg.photon_chunk, cfg.refill_min);
+    else hipLaunchKernelGGL(photon_kernel<false>, dim3(cfg.photon_grid), fb, lds, stream, a, cfg.photon_chunk, cfg.refill_min);
     hipLaunchKernelGGL(interaction_kernel<P_RAYLEIGH>, g, b, 0, stream, a);
     hipLaunchKernelGGL(interaction_kernel<P_COMPTON>, g, b, 0, stream, a);
     hipLaunchKernelGGL(interaction_kernel<P_PHOTO>, g, b, 0, stream, a);
     hipLaunchKernelGGL(interaction_kernel<P_PAIR>, g, b, 0, stream, a);
-    hipLaunchKernelGGL(electron_kernel, g, b, 0, stream, a);
-    hipLaunchKernelGGL(reset_bank, dim3(1), dim3(1), 0, stream, a.st, (int)E_ELECTRON_STEP);
+    if (tab) hipLaunchKernelGGL(electron_kernel<true>, dim3(cfg.electron_grid), fb, lds, stream, a, cfg.electron_chunk, cfg.refill_min);
+    else hipLaunchKernelGGL(electron_kernel<false>, dim3(cfg.electron_grid), fb, lds, stream, a, cfg.electron_chunk, cfg.refill_min);
     hipLaunchKernelGGL(interaction_kernel<E_MOLLER>, g, b, 0, stream, a);
     hipLaunchKernelGGL(interaction_kernel<E_BREMS>, g, b, 0, stream, a);
     hipLaunchKernelGGL(interaction_kernel<E_BHABHA>, g, b, 0, stream, a);
@@ -482,7 +596,7 @@ inline void launch_sweep(const EvArgs &a, const EventConfig &cfg, hipStream_t st
 inline bool same_args(const EvArgs &x, const EvArgs &y) {
     return x.bank == y.bank && x.cap == y.cap && x.st == y.st && x.src == y.src && x.hist_offset == y.hist_offset &&
            x.dose == y.dose && x.hist_edep == y.hist_edep && x.hist_steps == y.hist_steps &&
-           x.counters == y.counters && x.env.mt == y.env.mt && x.env.reg == y.env.reg && x.env.seed == y.env.seed &&
+           x.counters == y.counters && x.env.mt == y.env.mt && x.env.rcls == y.env.rcls && x.env.rtab == y.env.rtab && x.env.seed == y.env.seed &&
            x.env.xb == y.env.xb && x.env.nx == y.env.nx && x.env.ny == y.env.ny && x.env.nz == y.env.nz &&
            x.env.f.cos_angle == y.env.f.cos_angle && x.env.f.sin_angle == y.env.f.sin_angle &&
            x.env.f.change_xyz == y.env.f.change_xyz && x.env.f.source_to_isocenter == y.env.f.source_to_isocenter &&
@@ -490,16 +604,26 @@ inline bool same_args(const EvArgs &x, const EvArgs &y) {
 }
 
 inline bool same_cfg(const EventConfig &x, const EventConfig &y) {
-    return x.block == y.block && x.grid == y.grid && x.inject == y.inject && x.threshold == y.threshold &&
-           x.sweeps_per_graph == y.sweeps_per_graph;
+    return x.block == y.block && x.flight_block == y.flight_block && x.grid == y.grid && x.photon_grid == y.photon_grid &&
+           x.electron_grid == y.electron_grid && x.photon_chunk == y.photon_chunk &&
+           x.electron_chunk == y.electron_chunk && x.refill_min == y.refill_min && x.inject == y.inject &&
+           x.threshold == y.threshold && x.sweeps_per_graph == y.sweeps_per_graph;
 }
 
 // Transport source records [first, last) through the banks.  The sweep group
 // is captured once per argument set and replayed; the host polls one flag.
 inline hipError_t run_event_batch(EventBanks &bk, const Env &env, const gegs_source *src, uint32_t first,
                                   uint32_t last, uint64_t hist_offset, double *dose, double *hist_edep,
-                                  uint32_t *hist_steps, unsigned long long *counters, const EventConfig &cfg,
+                                  uint32_t *hist_steps, unsigned long long *counters, const EventConfig &cfg_in,
                                   hipStream_t stream, uint32_t max_sweeps = 100000) {
+    EventConfig cfg = cfg_in;
+    const size_t lds = flight_lds(env);
+    if (!cfg.photon_grid)
+        cfg.photon_grid = tab_in_lds(env) ? resident_grid(photon_kernel<true>, cfg.flight_block, lds)
+                                          : resident_grid(photon_kernel<false>, cfg.flight_block, lds);
+    if (!cfg.electron_grid)
+        cfg.electron_grid = tab_in_lds(env) ? resident_grid(electron_kernel<true>, cfg.flight_block, lds)
+                                            : resident_grid(electron_kernel<false>, cfg.flight_block, lds);
     EvArgs a;
     a.env = env;
     a.bank = bk.bank;

@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/gegs.h"
@@ -23,7 +24,11 @@ struct gegs_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     gegs_media_tables *d_media = nullptr;
-    gegs_region *d_regions = nullptr;
+    uint16_t *d_rcls = nullptr;          // class index per region
+    gegs_region *d_rtab = nullptr;       // distinct region records
+    uint32_t nclass = 0;
+    uint32_t max_med = 0;                // largest non-vacuum medium index of the phantom
+    uint32_t nmed = 0;                   // media in the loaded tables
     float *d_xb = nullptr, *d_yb = nullptr, *d_zb = nullptr;
     uint32_t nx = 0, ny = 0, nz = 0;
     size_t nvox = 0;
@@ -44,6 +49,7 @@ struct gegs_ctx {
     gegs_counters counters{};
     float last_ms = 0.0f;
     EventBanks banks;
+    EventConfig tuning;
 };
 
 static int fail(gegs_ctx *c, int code, const char *what, hipError_t e = hipSuccess) {
@@ -101,7 +107,8 @@ void gegs_destroy(gegs_ctx *c) {
     hipSetDevice(c->device);
     hipStreamSynchronize(c->stream);
     hipFree(c->d_media);
-    hipFree(c->d_regions);
+    hipFree(c->d_rcls);
+    hipFree(c->d_rtab);
     hipFree(c->d_xb);
     hipFree(c->d_yb);
     hipFree(c->d_zb);
@@ -128,6 +135,7 @@ int gegs_load_media(gegs_ctx *c, const gegs_media_tables *t) {
     if (!c->d_media) HIPCHECK(c, hipMalloc(&c->d_media, sizeof(gegs_media_tables)), "hipMalloc media");
     HIPCHECK(c, hipMemcpyAsync(c->d_media, t, sizeof(gegs_media_tables), hipMemcpyHostToDevice, c->stream), "copy media");
     HIPCHECK(c, hipStreamSynchronize(c->stream), "sync media");
+    c->nmed = (uint32_t)t->nmed;
     c->have_media = true;
     return GEGS_OK;
 }
@@ -136,6 +144,8 @@ int gegs_load_phantom(gegs_ctx *c, uint32_t nx, uint32_t ny, uint32_t nz, const 
                       const float *zb, const gegs_region *regions) {
     if (!c || !xb || !yb || !zb || !regions) return fail(c, GEGS_ERR_ARG, "gegs_load_phantom: null argument");
     if (nx < 2 || ny < 2 || nz < 2) return fail(c, GEGS_ERR_PHANTOM, "gegs_load_phantom: need at least 2 voxels per axis");
+    if (nx > GEGS_MAX_AXIS || ny > GEGS_MAX_AXIS || nz > GEGS_MAX_AXIS)
+        return fail(c, GEGS_ERR_PHANTOM, "gegs_load_phantom: more than 4096 voxels along an axis");
     const size_t nvox = (size_t)nx * ny * nz;
     if (nvox + VAC_AND_OTH > 0xFFFFFFFFull) return fail(c, GEGS_ERR_PHANTOM, "gegs_load_phantom: too many voxels");
     // voxel media must exist in the loaded tables
@@ -159,11 +169,38 @@ int gegs_load_phantom(gegs_ctx *c, uint32_t nx, uint32_t ny, uint32_t nz, const 
         HIPCHECK(c, hipMalloc(d[a], pad.size() * sizeof(float)), "hipMalloc bounds");
         HIPCHECK(c, hipMemcpy(*d[a], pad.data(), pad.size() * sizeof(float), hipMemcpyHostToDevice), "copy bounds");
     }
-    hipFree(c->d_regions);
-    c->d_regions = nullptr;
-    HIPCHECK(c, hipMalloc(&c->d_regions, (nvox + VAC_AND_OTH) * sizeof(gegs_region)), "hipMalloc regions");
-    HIPCHECK(c, hipMemcpy(c->d_regions, regions, (nvox + VAC_AND_OTH) * sizeof(gegs_region), hipMemcpyHostToDevice),
-             "copy regions");
+    // deduplicate the region records: class index per region + record table
+    const size_t nreg = nvox + VAC_AND_OTH;
+    std::unordered_map<std::string, uint32_t> seen;
+    std::vector<gegs_region> table;
+    std::vector<uint16_t> cls(nreg);
+    for (size_t i = 0; i < nreg; ++i) {
+        const std::string key(reinterpret_cast<const char *>(&regions[i]), sizeof(gegs_region));
+        auto it = seen.find(key);
+        uint32_t k;
+        if (it == seen.end()) {
+            k = (uint32_t)table.size();
+            if (k > 0xFFFFu) return fail(c, GEGS_ERR_PHANTOM, "gegs_load_phantom: more than 65536 distinct region records");
+            seen.emplace(key, k);
+            table.push_back(regions[i]);
+        } else {
+            k = it->second;
+        }
+        cls[i] = (uint16_t)k;
+    }
+    hipFree(c->d_rcls);
+    hipFree(c->d_rtab);
+    c->d_rcls = nullptr;
+    c->d_rtab = nullptr;
+    HIPCHECK(c, hipMalloc(&c->d_rcls, nreg * sizeof(uint16_t)), "hipMalloc region classes");
+    HIPCHECK(c, hipMalloc(&c->d_rtab, table.size() * sizeof(gegs_region)), "hipMalloc region table");
+    HIPCHECK(c, hipMemcpy(c->d_rcls, cls.data(), nreg * sizeof(uint16_t), hipMemcpyHostToDevice), "copy region classes");
+    HIPCHECK(c, hipMemcpy(c->d_rtab, table.data(), table.size() * sizeof(gegs_region), hipMemcpyHostToDevice),
+             "copy region table");
+    c->nclass = (uint32_t)table.size();
+    c->max_med = 0;
+    for (const gegs_region &r : table)
+        if (r.med != GEGS_VACUUM && r.med > c->max_med) c->max_med = r.med;
     c->nx = nx;
     c->ny = ny;
     c->nz = nz;
@@ -230,7 +267,13 @@ static int prepare_tallies(gegs_ctx *c) {
 static Env make_env(gegs_ctx *c, uint64_t seed) {
     Env env;
     env.mt = c->d_media;
-    env.reg = c->d_regions;
+    env.rcls = c->d_rcls;
+    env.rtab = c->d_rtab;
+    env.nclass = c->nclass;
+    env.nmed = c->nmed;
+    env.pge = c->d_media->ge;
+    env.pgmfp = c->d_media->gmfp;
+    env.pcohe = c->d_media->cohe;
     env.xb = c->d_xb;
     env.yb = c->d_yb;
     env.zb = c->d_zb;
@@ -266,6 +309,7 @@ static int check_ready(gegs_ctx *c, const gegs_run_options *o) {
     if (!c || !o) return fail(c, GEGS_ERR_ARG, "run: null argument");
     if (!c->have_media || !c->have_phantom || !c->have_field) return fail(c, GEGS_ERR_STATE, "run: load media, phantom and field first");
     if (c->nsrc == 0) return fail(c, GEGS_ERR_STATE, "run: no source particles");
+    if (c->max_med >= c->nmed) return fail(c, GEGS_ERR_MEDIA, "run: the phantom uses a medium the loaded tables lack");
     if (o->n_batches < 2 || o->n_batches > 4096) return fail(c, GEGS_ERR_ARG, "run: n_batches must be in [2, 4096]");
     if (o->first >= c->nsrc || (o->count && o->first + o->count > c->nsrc))
         return fail(c, GEGS_ERR_ARG, "run: record range outside the loaded source");
@@ -306,7 +350,10 @@ int gegs_run_plain(gegs_ctx *c, const gegs_run_options *o, double *dose, double 
         a.first = (uint32_t)(r0 + (nr * b) / nb);
         a.last = (uint32_t)(r0 + (nr * (b + 1)) / nb);
         a.next = c->d_next + b;
-        hipLaunchKernelGGL(plain_kernel, dim3(grid), dim3(block), 0, c->stream, a);
+        const bool tab = c->nclass <= LDS_MAX_CLASSES;
+        const size_t lds = env_lds_bytes(c->nx, c->ny, c->nz, c->nclass, c->nmed, tab);
+        if (tab) hipLaunchKernelGGL(plain_kernel<true>, dim3(grid), dim3(block), lds, c->stream, a);
+        else hipLaunchKernelGGL(plain_kernel<false>, dim3(grid), dim3(block), lds, c->stream, a);
         hipLaunchKernelGGL(fold_batch, dim3(2048), dim3(256), 0, c->stream, c->d_chunk, c->d_dose, c->d_dose2, c->nvox);
     }
     hipEventRecord(t1, c->stream);
@@ -320,13 +367,7 @@ int gegs_run_event(gegs_ctx *c, const gegs_run_options *o, double *dose, double 
     hipSetDevice(c->device);
     rc = prepare_tallies(c);
     if (rc) return rc;
-    EventConfig cfg;
-    if (o->block_size) cfg.block = o->block_size;
-    if (o->grid_size) cfg.grid = o->grid_size;
-    if (cfg.block != 256 && cfg.block != 128 && cfg.block != 64)
-        return fail(c, GEGS_ERR_ARG, "run_event: block_size must be 64, 128 or 256");
-    cfg.capacity = o->stack_depth ? (size_t)o->stack_depth * 65536 : 0;
-    if (!o->grid_size) cfg.grid = 2048;
+    const EventConfig cfg = c->tuning;
     const uint32_t nb = o->n_batches;
     Env env = make_env(c, o->seed);
     hipError_t e = c->banks.reserve(c->nsrc, nb, cfg);
@@ -356,11 +397,61 @@ int gegs_run_event(gegs_ctx *c, const gegs_run_options *o, double *dose, double 
     return finish_run(c, dose, dose2, t0, t1);
 }
 
+int gegs_set_tuning(gegs_ctx *c, const gegs_tuning *t) {
+    if (!c) return GEGS_ERR_ARG;
+    EventConfig cfg;
+    if (t) {
+        if (t->block) cfg.block = t->block;
+        if (t->flight_block) cfg.flight_block = t->flight_block;
+        if (t->refill_min) cfg.refill_min = t->refill_min;
+        if (t->grid) cfg.grid = t->grid;
+        cfg.photon_grid = t->photon_grid;
+        cfg.electron_grid = t->electron_grid;
+        if (t->photon_chunk) cfg.photon_chunk = t->photon_chunk;
+        if (t->electron_chunk) cfg.electron_chunk = t->electron_chunk;
+        if (t->inject) cfg.inject = t->inject;
+        if (t->threshold) cfg.threshold = t->threshold;
+        if (t->sweeps_per_graph) cfg.sweeps_per_graph = t->sweeps_per_graph;
+        if (t->capacity) cfg.capacity = (size_t)t->capacity;
+    }
+    if (cfg.block != 256 && cfg.block != 128 && cfg.block != 64)
+        return fail(c, GEGS_ERR_ARG, "gegs_set_tuning: block must be 64, 128 or 256");
+    if (cfg.flight_block != 512 && cfg.flight_block != 256 && cfg.flight_block != 128 && cfg.flight_block != 64)
+        return fail(c, GEGS_ERR_ARG, "gegs_set_tuning: flight_block must be 64, 128, 256 or 512");
+    if (cfg.grid > 65536 || cfg.photon_grid > 65536 || cfg.electron_grid > 65536)
+        return fail(c, GEGS_ERR_ARG, "gegs_set_tuning: grid above 65536 blocks");
+    if (cfg.refill_min > 64) return fail(c, GEGS_ERR_ARG, "gegs_set_tuning: refill_min above 64");
+    if (cfg.sweeps_per_graph > 64) return fail(c, GEGS_ERR_ARG, "gegs_set_tuning: sweeps_per_graph above 64");
+    if (cfg.capacity > 0xFFFFFFF0ull) return fail(c, GEGS_ERR_ARG, "gegs_set_tuning: capacity above 2^32 - 16");
+    c->tuning = cfg;
+    return GEGS_OK;
+}
+
+int gegs_get_tuning(gegs_ctx *c, gegs_tuning *t) {
+    if (!c || !t) return GEGS_ERR_ARG;
+    const EventConfig &cfg = c->tuning;
+    t->block = cfg.block;
+    t->grid = cfg.grid;
+    t->photon_grid = cfg.photon_grid;
+    t->electron_grid = cfg.electron_grid;
+    t->photon_chunk = cfg.photon_chunk;
+    t->electron_chunk = cfg.electron_chunk;
+    t->inject = cfg.inject;
+    t->threshold = cfg.threshold;
+    t->sweeps_per_graph = cfg.sweeps_per_graph;
+    t->flight_block = cfg.flight_block;
+    t->refill_min = cfg.refill_min;
+    t->reserved = 0;
+    t->capacity = cfg.capacity ? cfg.capacity : ((uint64_t)1 << 22);
+    return GEGS_OK;
+}
+
 int gegs_sample_events(gegs_ctx *c, const gegs_particle *in, uint32_t n, uint64_t seed, gegs_particle *out,
                        uint32_t max_out, uint32_t *n_out, double *edep) {
     if (!c || !in || !out || !n_out || !edep || n == 0 || max_out == 0)
         return fail(c, GEGS_ERR_ARG, "gegs_sample_events: bad argument");
     if (!c->have_media || !c->have_phantom || !c->have_field) return fail(c, GEGS_ERR_STATE, "gegs_sample_events: load first");
+    if (c->max_med >= c->nmed) return fail(c, GEGS_ERR_MEDIA, "gegs_sample_events: the phantom uses a medium the loaded tables lack");
     for (uint32_t i = 0; i < n; ++i) {
         if (in[i].status < 0 || in[i].status >= N_EVENTS) return fail(c, GEGS_ERR_ARG, "gegs_sample_events: bad status");
         if ((size_t)in[i].region >= c->nvox + VAC_AND_OTH) return fail(c, GEGS_ERR_ARG, "gegs_sample_events: bad region");

@@ -39,7 +39,8 @@ struct StackSink {
         *at(k, 7) = p.wt;
         *at(k, 8) = p.dnear;
         *at(k, 9) = __uint_as_float(p.region);
-        *at(k, 10) = __uint_as_float((uint32_t)(p.charge + 1) | ((uint32_t)p.status << 8) | ((uint32_t)p.module << 16));
+        *at(k, 10) = __uint_as_float((uint32_t)(p.charge + 1) | ((uint32_t)p.status << 8) | ((uint32_t)p.module << 16) |
+                                     ((uint32_t)p.spos << 24));
         *at(k, 11) = __uint_as_float((uint32_t)p.lin);
         *at(k, 12) = __uint_as_float((uint32_t)(p.lin >> 32));
         *at(k, 13) = __uint_as_float(p.blk);
@@ -61,6 +62,7 @@ struct StackSink {
         p.charge = (int32_t)(pk & 0xffu) - 1;
         p.status = (int32_t)((pk >> 8) & 0xffu);
         p.module = (int32_t)((pk >> 16) & 0xffu);
+        p.spos = (int32_t)(pk >> 24);
         p.lin = (uint64_t)__float_as_uint(*at(k, 11)) | ((uint64_t)__float_as_uint(*at(k, 12)) << 32);
         p.blk = __float_as_uint(*at(k, 13));
         return p;
@@ -90,11 +92,16 @@ __device__ __forceinline__ Part source_particle(const gegs_source &r, int module
     p.module = module;
     p.dnear = INFI_DIST;
     p.blk = 0;
+    p.spos = 4;
     p.status = r.charge == 0 ? P_PHOTON_STEP : E_ELECTRON_STEP;
     return p;
 }
 
+template <bool TAB>
 __global__ void __launch_bounds__(128) plain_kernel(PlainArgs a) {
+    extern __shared__ float lds_env[];
+    Env env = a.env;
+    stage_env<TAB>(env, lds_env);
     StackSink sink;
     sink.base = a.stack;
     sink.stride = gridDim.x * blockDim.x;
@@ -112,7 +119,7 @@ __global__ void __launch_bounds__(128) plain_kernel(PlainArgs a) {
         const uint32_t k = atomicAdd(a.next, 1u);
         if (k >= a.last - a.first) break;
         const uint32_t i = a.first + k;
-        Part p = source_particle(a.src[i], a.env.f.module_first);
+        Part p = source_particle(a.src[i], env.f.module_first);
         p.lin = primary_lineage(a.hist_offset + i);
         t.hist_edep = 0.0;
         t.steps = 0;
@@ -121,10 +128,10 @@ __global__ void __launch_bounds__(128) plain_kernel(PlainArgs a) {
         while (sink.sp > 0) {
             Part q = sink.pop();
             Stream s;
-            s.begin(q, a.env.seed);
+            s.begin(q, env.seed);
 #pragma unroll
             for (int k = 0; k < N_EVENTS; ++k) pops[k] += (q.status == k);
-            handle(a.env, s, q, sink, t);
+            handle(env, s, q, sink, t);
         }
         if (a.hist_edep) a.hist_edep[i] = t.hist_edep;
         if (a.hist_steps) a.hist_steps[i] = t.steps;
@@ -174,7 +181,7 @@ __global__ void sample_kernel(Env env, const gegs_particle *in, uint32_t n, gegs
     Part p;
     p.e = r.e; p.x = r.x; p.y = r.y; p.z = r.z; p.u = r.u; p.v = r.v; p.w = r.w; p.wt = r.wt;
     p.dnear = r.dnear; p.region = r.region; p.charge = r.charge; p.status = r.status;
-    p.module = env.f.module_first; p.lin = r.lin; p.blk = r.blk;
+    p.module = env.f.module_first; p.lin = r.lin; p.blk = r.blk; p.spos = 4;
     ListSink sink{out + (size_t)i * max_out, max_out, 0};
     Tally t;
     t.dose = nullptr;

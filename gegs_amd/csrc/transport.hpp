@@ -72,6 +72,8 @@ struct Part {
     int32_t module;
     uint64_t lin;
     uint32_t blk;
+    int32_t spos;        // bits 0-2: uniforms of block blk-1 already used (4: none
+                         // pending); bit 3: a suspended flight that resumes
 };
 
 // ------------------------------------------------------------------ Philox
@@ -111,6 +113,19 @@ struct Stream {
     float b0, b1, b2, b3;
     uint32_t k0, k1;
 
+    __device__ __forceinline__ void refill() {
+        uint32_t c0 = blk, c1 = (uint32_t)lin, c2 = (uint32_t)(lin >> 32), c3 = 0x47454753u;
+        philox10(c0, c1, c2, c3, k0, k1);
+        const float s = 5.96046448e-08f;
+        b0 = (float)((c0 >> 8) + 1u) * s;
+        b1 = (float)((c1 >> 8) + 1u) * s;
+        b2 = (float)((c2 >> 8) + 1u) * s;
+        b3 = (float)((c3 >> 8) + 1u) * s;
+        blk += 1;
+        pos = 0;
+    }
+    // a particle suspended in the middle of a block (spos < 4) resumes at the
+    // same uniform: the block is generated again
     __device__ __forceinline__ void begin(const Part &p, uint64_t seed) {
         lin = p.lin;
         blk = p.blk;
@@ -118,19 +133,14 @@ struct Stream {
         nchild = 0;
         k0 = (uint32_t)seed;
         k1 = (uint32_t)(seed >> 32);
+        if ((p.spos & 7) < 4) {
+            blk -= 1;
+            refill();
+            pos = p.spos & 7;
+        }
     }
     __device__ __forceinline__ float next() {
-        if (pos == 4) {
-            uint32_t c0 = blk, c1 = (uint32_t)lin, c2 = (uint32_t)(lin >> 32), c3 = 0x47454753u;
-            philox10(c0, c1, c2, c3, k0, k1);
-            const float s = 5.96046448e-08f;
-            b0 = (float)((c0 >> 8) + 1u) * s;
-            b1 = (float)((c1 >> 8) + 1u) * s;
-            b2 = (float)((c2 >> 8) + 1u) * s;
-            b3 = (float)((c3 >> 8) + 1u) * s;
-            blk += 1;
-            pos = 0;
-        }
+        if (pos == 4) refill();
         const int k = pos++;
         return k == 0 ? b0 : (k == 1 ? b1 : (k == 2 ? b2 : b3));
     }
@@ -144,14 +154,76 @@ struct Stream {
 __device__ __forceinline__ uint64_t child_lineage(Stream &s) { return splitmix(s.child()); }
 
 // ------------------------------------------------------------------ environment
+// Region records are stored as a class index per region (2 bytes) into a
+// table of the distinct records: a CT phantom has a few thousand distinct
+// (medium, density) records, so the index volume of a 128x128x100 phantom
+// (3.3 MB) stays resident in the L2 of every XCD, where 16-byte records
+// (26 MB) would miss to HBM at every voxel crossing.  Values are unchanged.
 struct Env {
     const gegs_media_tables *mt;
-    const gegs_region *reg;
+    const uint16_t *rcls;            // class of region r, 6 + nx*ny*nz entries
+    const gegs_region *rtab;         // distinct region records
     const float *xb, *yb, *zb;   // N+2 entries each
+    const gegs_float2 *pge, *pgmfp, *pcohe;   // photon rows read at every crossing (mt->ge, gmfp, cohe)
     uint32_t nx, ny, nz;
+    uint32_t nclass, nmed;
     gegs_field f;
     uint64_t seed;
 };
+
+// LDS staging of what a voxel crossing reads: the voxel bounds and, when the
+// phantom has at most LDS_MAX_CLASSES distinct region records, the record
+// table and the photon rows ge/gmfp/cohe of every medium.  Layout:
+// [records nclass x 16 B][ge nmed x 8 B][gmfp nmed x MXGE x 8 B][cohe ...][bounds].
+// All threads of the block call stage_env() before any of them tracks a
+// particle; the env it returns points at LDS.
+constexpr uint32_t LDS_MAX_CLASSES = 2048;
+
+__host__ __device__ inline size_t env_lds_bytes(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t nclass, uint32_t nmed,
+                                                bool tab) {
+    size_t b = ((size_t)nx + ny + nz + 6) * sizeof(float);
+    if (tab) b += (size_t)nclass * 16 + (size_t)nmed * 8 + 2 * (size_t)nmed * MXGE * 8;
+    return b;
+}
+
+template <bool TAB>
+__device__ __forceinline__ void stage_env(Env &env, float *lds) {
+    float *p = lds;
+    const gegs_region *rtab = env.rtab;
+    const gegs_float2 *pge = env.pge, *pgmfp = env.pgmfp, *pcohe = env.pcohe;
+    if (TAB) {
+        float4 *rt = reinterpret_cast<float4 *>(p);
+        const float4 *src = reinterpret_cast<const float4 *>(env.rtab);
+        for (uint32_t i = threadIdx.x; i < env.nclass; i += blockDim.x) rt[i] = src[i];
+        rtab = reinterpret_cast<const gegs_region *>(rt);
+        p += (size_t)env.nclass * 4;
+        gegs_float2 *g = reinterpret_cast<gegs_float2 *>(p);
+        const uint32_t nrow = env.nmed * MXGE;
+        for (uint32_t i = threadIdx.x; i < env.nmed; i += blockDim.x) g[i] = env.mt->ge[i];
+        for (uint32_t i = threadIdx.x; i < nrow; i += blockDim.x) {
+            g[env.nmed + i] = env.mt->gmfp[i];
+            g[env.nmed + nrow + i] = env.mt->cohe[i];
+        }
+        pge = g;
+        pgmfp = g + env.nmed;
+        pcohe = g + env.nmed + nrow;
+        p += 2 * ((size_t)env.nmed + 2 * nrow);
+    }
+    const uint32_t nxb = env.nx + 2, nyb = env.ny + 2, nzb = env.nz + 2;
+    for (uint32_t i = threadIdx.x; i < nxb; i += blockDim.x) p[i] = env.xb[i];
+    for (uint32_t i = threadIdx.x; i < nyb; i += blockDim.x) p[nxb + i] = env.yb[i];
+    for (uint32_t i = threadIdx.x; i < nzb; i += blockDim.x) p[nxb + nyb + i] = env.zb[i];
+    __syncthreads();
+    env.xb = p;
+    env.yb = p + nxb;
+    env.zb = p + nxb + nyb;
+    env.rtab = rtab;
+    env.pge = pge;
+    env.pgmfp = pgmfp;
+    env.pcohe = pcohe;
+}
+
+__device__ __forceinline__ gegs_region region_rec(const Env &env, uint32_t r) { return env.rtab[env.rcls[r]]; }
 
 struct Tally {
     double *dose;          // per voxel, FP64
@@ -173,6 +245,7 @@ __device__ __forceinline__ void forward(Sink &sink, Stream &s, Part &p, int stat
     p.status = status;
     p.lin = s.lin;
     p.blk = s.blk;
+    p.spos = 4;
     sink.push(p);
 }
 
@@ -181,6 +254,7 @@ __device__ __forceinline__ void create(Sink &sink, Stream &s, Part &p, int statu
     p.status = status;
     p.lin = child_lineage(s);
     p.blk = 0;
+    p.spos = 4;
     sink.push(p);
 }
 
@@ -389,36 +463,154 @@ __device__ __forceinline__ void annih_at_rest(const Env &env, Stream &s, Part &p
     t.score(p.region, edep * p.wt);
 }
 
-// ------------------------------------------------------------------ photon flight
+// ------------------------------------------------------------------ photon flight (photon.cu:6-292)
+// One call of photon_walk_iter() is one photon step of the reference: a new
+// number of mean free paths is drawn at every voxel crossing (photon.cu:102-103),
+// the step is cut at the next voxel face (geom.cu:143-215), and when the
+// remaining path falls below EPSGMFP in the same medium the interaction type
+// is chosen (photon.cu:229-271).  The quantities that do not change during a
+// flight are held in registers instead of being recomputed or reloaded at
+// every crossing:
+//   ln E and the medium's interpolated gmfp/cohe rows (reloaded only when the
+//   medium changes), the direction in the phantom frame, the voxel indices
+//   (stepped by one on a crossing instead of decoded by integer division),
+//   and the region record of the current voxel (one gather per crossing).
+struct PhotonWalk {
+    gegs_region rd;          // region record of p.region
+    float gle;               // ln E
+    int32_t cmed;            // medium of the cached rows, -1: none
+    int32_t lgle;
+    float gmfpr0, cohf;      // interpolated gmfp and cohe of medium cmed
+    float lu, lv, lw;        // direction in the phantom frame
+    int32_t ix, iy, iz;      // voxel indices, valid when p.region >= VAC_AND_OTH
+};
+
+__device__ __forceinline__ void local_position(const gegs_field &f, const Part &p, float &qx, float &qy, float &qz) {
+    // the position part of phantom_frame(), operation for operation
+    float x, y, z;
+    if (f.change_xyz) {
+        x = p.y; y = p.z - f.source_to_isocenter; z = p.x;
+    } else {
+        x = p.x; y = p.y; z = p.z - f.source_to_isocenter;
+    }
+    const float rx = x * f.cos_angle - y * f.sin_angle;
+    const float ry = x * f.sin_angle + y * f.cos_angle;
+    qx = rx + f.iso_x;
+    qy = ry + f.iso_y;
+    qz = z + f.iso_z;
+}
+
+__device__ __forceinline__ void decode_voxel(const Env &env, uint32_t region, int32_t &ix, int32_t &iy, int32_t &iz) {
+    uint32_t ir = region - VAC_AND_OTH;
+    const uint32_t nxy = env.nx * env.ny;
+    iz = (int32_t)(ir / nxy);
+    ir -= (uint32_t)iz * nxy;
+    iy = (int32_t)(ir / env.nx);
+    ix = (int32_t)(ir - (uint32_t)iy * env.nx);
+}
+
+__device__ __forceinline__ void photon_walk_begin(const Env &env, const Part &p, PhotonWalk &w) {
+    w.rd = region_rec(env, p.region);
+    w.gle = glog(p.e);
+    w.cmed = -1;
+    w.lgle = 0;
+    w.gmfpr0 = w.cohf = 0.0f;
+    const Local q = phantom_frame(env.f, p);
+    w.lu = q.u;
+    w.lv = q.v;
+    w.lw = q.w;
+    w.ix = w.iy = w.iz = 0;
+    if (p.region >= VAC_AND_OTH) decode_voxel(env, p.region, w.ix, w.iy, w.iz);
+}
+
+// one free-flight segment; true: the photon crossed a boundary and flies on
 template <class Sink>
-__device__ __forceinline__ bool photon_iter(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
-    // one free-flight segment; true: the photon crossed a boundary and flies on
+__device__ __forceinline__ bool photon_walk_iter(const Env &env, Stream &s, Part &p, PhotonWalk &w, Sink &sink, Tally &t) {
     const gegs_media_tables *mt = env.mt;
     t.photon_steps++;
     t.steps++;
-    gegs_region rd = env.reg[p.region];
+    gegs_region rd = w.rd;
     if (p.e <= rd.pcut) { t.score(p.region, p.e * p.wt); return false; }
     if (p.wt <= 0.0f) return false;
+    // Inside the phantom the face the ray leaves the voxel through does not
+    // depend on the sampled path, so it is found first and the class of the
+    // voxel behind it is requested before the uniform is drawn; the load is
+    // in flight while the step length is computed.  howfar_phantom() takes
+    // the faces in x, y, z order with `d <= t`: the voxel is left through the
+    // last face of minimal distance when that distance is <= the sampled
+    // step, which is what the comparison below does.
+    const bool inside = p.region >= VAC_AND_OTH;
+    float dmin = 0.0f;
+    uint32_t ncand = 0;
+    int axis = -1;
+    uint16_t cls_next = 0;
+    if (inside) {
+        float qx, qy, qz;
+        local_position(env.f, p, qx, qy, qz);
+        const uint32_t nx = env.nx, nxy = env.nx * env.ny;
+        if (w.lu > 0.0f) {
+            dmin = (env.xb[w.ix + 1] - qx) / w.lu;
+            ncand = (w.ix + 1 < (int)nx) ? p.region + 1 : 0u;
+            axis = 0;
+        } else if (w.lu < 0.0f) {
+            dmin = (env.xb[w.ix] - qx) / w.lu;
+            ncand = (w.ix - 1 >= 0) ? p.region - 1 : 0u;
+            axis = 1;
+        }
+        if (w.lv > 0.0f) {
+            const float d = (env.yb[w.iy + 1] - qy) / w.lv;
+            if (axis < 0 || d <= dmin) { dmin = d; ncand = (w.iy + 1 < (int)env.ny) ? p.region + nx : 0u; axis = 2; }
+        } else if (w.lv < 0.0f) {
+            const float d = (env.yb[w.iy] - qy) / w.lv;
+            if (axis < 0 || d <= dmin) { dmin = d; ncand = (w.iy - 1 >= 0) ? p.region - nx : 0u; axis = 3; }
+        }
+        if (w.lw > 0.0f) {
+            const float d = (env.zb[w.iz + 1] - qz) / w.lw;
+            if (axis < 0 || d <= dmin) { dmin = d; ncand = (w.iz + 1 < (int)env.nz) ? p.region + nxy : 0u; axis = 4; }
+        } else if (w.lw < 0.0f) {
+            const float d = (env.zb[w.iz] - qz) / w.lw;
+            if (axis < 0 || d <= dmin) { dmin = d; ncand = (w.iz - 1 >= 0) ? p.region - nxy : 0u; axis = 5; }
+        }
+        if (axis >= 0) cls_next = env.rcls[ncand];
+    }
     float r1 = s.next();
     float dpmfp = -glog(r1);
-    const float gle = glog(p.e);
+    const float gle = w.gle;
     int lgle = 0;
     float tstep, gmfp_val = 0.0f, cohfac = 0.0f;
     if (rd.med == VACUUM) {
         tstep = VACUUM_STEP;
     } else {
-        const gegs_float2 ge = mt->ge[rd.med];
-        lgle = (int)(ge.x + ge.y * gle);
-        const gegs_float2 gm = mt->gmfp[rd.med * MXGE + lgle];
-        gmfp_val = (gm.x + gm.y * gle) / rd.rhof;
+        if ((int32_t)rd.med != w.cmed) {
+            const gegs_float2 ge = env.pge[rd.med];
+            w.lgle = (int)(ge.x + ge.y * gle);
+            const gegs_float2 gm = env.pgmfp[rd.med * MXGE + w.lgle];
+            const gegs_float2 ch = env.pcohe[rd.med * MXGE + w.lgle];
+            w.gmfpr0 = gm.x + gm.y * gle;
+            w.cohf = ch.x + ch.y * gle;
+            w.cmed = rd.med;
+        }
+        lgle = w.lgle;
+        gmfp_val = w.gmfpr0 / rd.rhof;
         if (rd.flags & GEGS_F_RAYLEIGH) {
-            const gegs_float2 ch = mt->cohe[rd.med * MXGE + lgle];
-            cohfac = ch.x + ch.y * gle;
+            cohfac = w.cohf;
             gmfp_val *= cohfac;
         }
         tstep = gmfp_val * dpmfp;
     }
-    const uint32_t nreg = howfar_phantom(env, p, tstep);
+    uint32_t nreg;
+    bool crossed = false;
+    if (inside) {
+        nreg = p.region;
+        if (axis >= 0 && dmin <= tstep) {
+            tstep = dmin;
+            nreg = ncand;
+            crossed = true;
+        }
+        if (tstep < 0.0f) tstep = EPSTEP;
+    } else {
+        nreg = howfar_phantom(env, p, tstep);
+    }
     int idisc = 0;
     if (nreg == 0) idisc = rd.med == VACUUM ? 1 : -1;
     if (idisc > 0) { p.region = 0; return false; }
@@ -428,8 +620,17 @@ __device__ __forceinline__ bool photon_iter(const Env &env, Stream &s, Part &p, 
     if (rd.med != VACUUM) dpmfp = fmaxf(0.0f, dpmfp - tstep / gmfp_val);
     const uint32_t old_med = rd.med;
     if (nreg != p.region) {
+        if (crossed) {
+            w.ix += axis == 0 ? 1 : (axis == 1 ? -1 : 0);
+            w.iy += axis == 2 ? 1 : (axis == 3 ? -1 : 0);
+            w.iz += axis == 4 ? 1 : (axis == 5 ? -1 : 0);
+            rd = env.rtab[cls_next];
+        } else {
+            if (nreg >= VAC_AND_OTH) decode_voxel(env, nreg, w.ix, w.iy, w.iz);
+            rd = region_rec(env, nreg);
+        }
         p.region = nreg;
-        rd = env.reg[nreg];
+        w.rd = rd;
         t.gathers++;
     }
     if (p.e <= rd.pcut || idisc < 0) { t.score(p.region, p.e * p.wt); return false; }
@@ -449,16 +650,29 @@ __device__ __forceinline__ bool photon_iter(const Env &env, Stream &s, Part &p, 
     return false;
 }
 
+// a flight interrupted between two crossings: the particle keeps its stream
+// position so that it resumes with the next uniform
+template <class Sink>
+__device__ __forceinline__ void suspend(Sink &sink, Stream &s, Part &p, int status) {
+    p.status = status;
+    p.lin = s.lin;
+    p.blk = s.blk;
+    p.spos = s.pos | 8;
+    sink.push(p);
+}
+
 template <class Sink>
 __device__ __forceinline__ void photon_step(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
-    while (photon_iter(env, s, p, sink, t)) {
+    PhotonWalk w;
+    photon_walk_begin(env, p, w);
+    while (photon_walk_iter(env, s, p, w, sink, t)) {
     }
 }
 
 // ------------------------------------------------------------------ Compton (Klein-Nishina)
 template <class Sink>
 __device__ __forceinline__ void compton(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
-    const gegs_region rd = env.reg[p.region];
+    const gegs_region rd = region_rec(env, p.region);
     float edep = 0.0f;
     Part e1 = p;
     const float peig = p.e;
@@ -534,7 +748,7 @@ __device__ __forceinline__ void compton(const Env &env, Stream &s, Part &p, Sink
 // ------------------------------------------------------------------ photo-electric
 template <class Sink>
 __device__ __forceinline__ void photo(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
-    const gegs_region rd = env.reg[p.region];
+    const gegs_region rd = region_rec(env, p.region);
     float edep = 0.0f;
     p.e = (float)((double)p.e + PRM);
     p.charge = -1;
@@ -581,7 +795,7 @@ __device__ __forceinline__ void photo(const Env &env, Stream &s, Part &p, Sink &
 template <class Sink>
 __device__ __forceinline__ void rayleigh(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
     const gegs_media_tables *mt = env.mt;
-    const gegs_region rd = env.reg[p.region];
+    const gegs_region rd = region_rec(env, p.region);
     const float gle = glog(p.e);
     const gegs_float2 ge = mt->ge[rd.med];
     const int lgle = (int)(ge.x + ge.y * gle);
@@ -622,7 +836,7 @@ __device__ __forceinline__ void rayleigh(const Env &env, Stream &s, Part &p, Sin
 template <class Sink>
 __device__ __forceinline__ void pair(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
     const gegs_media_tables *mt = env.mt;
-    const gegs_region rd = env.reg[p.region];
+    const gegs_region rd = region_rec(env, p.region);
     const int m8 = 8 * (int)rd.med;
     float edep = 0.0f;
     Part p1 = p;
@@ -944,7 +1158,7 @@ __device__ __forceinline__ bool electron_iter(const Env &env, Stream &s, Part &p
     int &lelke = st.lelke;
     bool &compute_tstep = st.compute_tstep, &is_tstep = st.is_tstep;
     uint32_t irl = p.region;
-    gegs_region rd = env.reg[irl];
+    gegs_region rd = region_rec(env, irl);
     int medium = rd.med;
     const float rhof = rd.rhof;
     const int lelec = p.charge;
@@ -1086,7 +1300,7 @@ __device__ __forceinline__ bool electron_iter(const Env &env, Stream &s, Part &p
         }
         if (p.region != irnew) t.gathers++;
         p.region = irnew;
-        rd = env.reg[irnew];
+        rd = region_rec(env, irnew);
         if (eie <= rd.ecut) {
             t.score(p.region, to_kinetic(p.e) * p.wt);
             if (p.charge > 0) annih_at_rest(env, s, p, rd, sink, t);
@@ -1157,7 +1371,7 @@ __device__ __forceinline__ bool electron_iter(const Env &env, Stream &s, Part &p
     }
     if (irnew != irold) {
         p.region = irnew;
-        rd = env.reg[irnew];
+        rd = region_rec(env, irnew);
         medium = rd.med;
         t.gathers++;
     }
@@ -1224,7 +1438,7 @@ __device__ __forceinline__ void electron_step(const Env &env, Stream &s, Part &p
 template <class Sink>
 __device__ __forceinline__ void moller(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
     const gegs_media_tables *mt = env.mt;
-    const gegs_region rd = env.reg[p.region];
+    const gegs_region rd = region_rec(env, p.region);
     const int m = rd.med;
     float edep = 0.0f;
     Part p1 = p;
@@ -1275,7 +1489,7 @@ __device__ __forceinline__ void moller(const Env &env, Stream &s, Part &p, Sink 
 template <class Sink>
 __device__ __forceinline__ void bhabha(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
     const gegs_media_tables *mt = env.mt;
-    const gegs_region rd = env.reg[p.region];
+    const gegs_region rd = region_rec(env, p.region);
     float edep = 0.0f;
     Part p1 = p;
     const float eip = p.e;
@@ -1343,7 +1557,7 @@ __device__ __forceinline__ void bhabha(const Env &env, Stream &s, Part &p, Sink 
 template <class Sink>
 __device__ __forceinline__ void brems(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
     const gegs_media_tables *mt = env.mt;
-    const gegs_region rd = env.reg[p.region];
+    const gegs_region rd = region_rec(env, p.region);
     const int m = rd.med;
     const int m8 = 8 * m;
     const int nsplit = mt->nbr_split;
@@ -1444,7 +1658,7 @@ __device__ __forceinline__ void brems(const Env &env, Stream &s, Part &p, Sink &
 template <class Sink>
 __device__ __forceinline__ void annih(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
     const gegs_media_tables *mt = env.mt;
-    const gegs_region rd = env.reg[p.region];
+    const gegs_region rd = region_rec(env, p.region);
     const int nsplit = mt->nbr_split;
     float edep = 0.0f;
     if (nsplit <= 0) return;

@@ -21,6 +21,7 @@ SYMBOLS = [
     "gegs_load_media", "gegs_load_phantom", "gegs_set_field", "gegs_set_source",
     "gegs_run_plain", "gegs_run_event", "gegs_set_history_tally", "gegs_history_tally",
     "gegs_get_counters", "gegs_last_kernel_ms", "gegs_sample_events",
+    "gegs_set_tuning", "gegs_get_tuning",
 ]
 
 PARTICLE_DTYPE = np.dtype([("e", "<f4"), ("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("u", "<f4"),
@@ -34,6 +35,23 @@ class RunOptions(ctypes.Structure):
                 ("n_batches", ctypes.c_uint32), ("block_size", ctypes.c_uint32),
                 ("grid_size", ctypes.c_uint32), ("stack_depth", ctypes.c_uint32),
                 ("first", ctypes.c_uint64), ("count", ctypes.c_uint64)]
+
+
+class Tuning(ctypes.Structure):
+    """gegs_tuning: launch shape and queue policy of the event-queued method."""
+    _fields_ = [("block", ctypes.c_uint32), ("grid", ctypes.c_uint32),
+                ("photon_grid", ctypes.c_uint32), ("electron_grid", ctypes.c_uint32),
+                ("photon_chunk", ctypes.c_uint32), ("electron_chunk", ctypes.c_uint32),
+                ("inject", ctypes.c_uint32), ("threshold", ctypes.c_uint32),
+                ("sweeps_per_graph", ctypes.c_uint32), ("flight_block", ctypes.c_uint32),
+                ("refill_min", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("capacity", ctypes.c_uint64)]
+
+    FIELDS = ("block", "grid", "photon_grid", "electron_grid", "photon_chunk", "electron_chunk",
+              "inject", "threshold", "sweeps_per_graph", "flight_block", "refill_min", "capacity")
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k in self.FIELDS}
 
 
 class Counters(ctypes.Structure):
@@ -79,6 +97,8 @@ def load():
     L.gegs_get_counters.argtypes = [vp, vp]
     L.gegs_last_kernel_ms.argtypes = [vp, vp]
     L.gegs_sample_events.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint64, vp, ctypes.c_uint32, vp, vp]
+    L.gegs_set_tuning.argtypes = [vp, vp]
+    L.gegs_get_tuning.argtypes = [vp, vp]
     for name in SYMBOLS:
         if getattr(L, name, None) is None:
             raise RuntimeError("symbol %s missing from %s" % (name, LIB_PATH))
@@ -137,6 +157,25 @@ class Context:
         src = np.ascontiguousarray(src)
         self._check(self.L.gegs_set_source(self.h, _p(src), len(src)))
         self.nsrc = len(src)
+
+    def set_tuning(self, tuning=None, **fields):
+        """Set the event-method tuning from a dict and/or keywords (zero or
+        missing fields take the library defaults); None resets to defaults."""
+        merged = dict(tuning or {})
+        merged.update(fields)
+        if not merged and tuning is None:
+            self._check(self.L.gegs_set_tuning(self.h, None))
+            return
+        unknown = set(merged) - set(Tuning.FIELDS)
+        if unknown:
+            raise ValueError("unknown tuning fields: %s" % sorted(unknown))
+        t = Tuning(**{k: int(v) for k, v in merged.items()})
+        self._check(self.L.gegs_set_tuning(self.h, ctypes.byref(t)))
+
+    def get_tuning(self):
+        t = Tuning()
+        self._check(self.L.gegs_get_tuning(self.h, ctypes.byref(t)))
+        return t.as_dict()
 
     def sample_events(self, particles, seed, max_out=16):
         """Run one event handler per particle (PARTICLE_DTYPE records);

@@ -34,6 +34,10 @@ extern "C" {
 /* Module ids (CUDA_EGS.h:396-404). */
 #define GEGS_M_PHANTOM 6
 
+/* voxels per phantom axis accepted by gegs_load_phantom (the reference caps at 128,
+   EGSnrc_Parameters.h:140-142) */
+#define GEGS_MAX_AXIS 4096
+
 /* One source particle in the head frame, as decoded from an egsphsp MODE0
  * record (CUDA_EGS.cu:40-158): total energy, position with z at the source
  * plane, direction cosines, weight, charge (-1, 0, +1). */
@@ -68,12 +72,32 @@ typedef struct gegs_run_options {
     uint64_t seed;               /* Philox-4x32-10 key */
     uint64_t history_offset;     /* global index of the first history */
     uint32_t n_batches;          /* time-sliced statistical batches (>= 2) */
-    uint32_t block_size;         /* threads per block, 0 = default */
-    uint32_t grid_size;          /* blocks, 0 = default */
-    uint32_t stack_depth;        /* per-history particle stack depth, 0 = default */
+    uint32_t block_size;         /* plain method: threads per block, 0 = default */
+    uint32_t grid_size;          /* plain method: blocks, 0 = default */
+    uint32_t stack_depth;        /* plain method: per-history particle stack depth, 0 = default */
     uint64_t first;              /* first source record of the run */
     uint64_t count;              /* records to transport, 0 = all from first */
 } gegs_run_options;
+
+/* Launch shape and queue policy of the event-queued method; a zero field
+ * takes the default.  The reference fixes these at compile time (block and
+ * grid sizes and stack sizes in CUDA_EGS.h, init.cu:31-57); here an
+ * autotuner (gegs_amd/autotune.py) picks them per workload. */
+typedef struct gegs_tuning {
+    uint32_t block;              /* threads per block of the single-pass event kernels: 64, 128 or 256 (256) */
+    uint32_t grid;               /* blocks of the single-pass event kernels (2048) */
+    uint32_t photon_grid;        /* blocks of the photon kernel (enough to fill every CU once) */
+    uint32_t electron_grid;      /* blocks of the electron kernel (same) */
+    uint32_t photon_chunk;       /* voxel crossings per photon and sweep (32; 0xFFFFFFFF: no limit) */
+    uint32_t electron_chunk;     /* condensed-history steps per electron and sweep (16; 0xFFFFFFFF: no limit) */
+    uint32_t inject;             /* primaries injected per refill (1 Mi) */
+    uint32_t threshold;          /* refill when fewer particles are in flight (1 Mi) */
+    uint32_t sweeps_per_graph;   /* sweeps captured in one graph replay (4) */
+    uint32_t flight_block;       /* threads per block of the photon and electron kernels: 64 ... 512 (256) */
+    uint32_t refill_min;         /* idle lanes of a wavefront that trigger a refill, 1 ... 64 (16) */
+    uint32_t reserved;
+    uint64_t capacity;           /* particles per bank (4 Mi) */
+} gegs_tuning;
 
 /* Event counters (diagnostics; replaces DO_STEP_COUNT / PRINT_PART_NUM). */
 typedef struct gegs_counters {
@@ -121,6 +145,12 @@ int gegs_run_plain(gegs_ctx *ctx, const gegs_run_options *opt, double *dose, dou
 /* Same transport through the event-queued path: SoA particle banks in HBM,
  * one kernel per event type, wavefront-compacted queues. */
 int gegs_run_event(gegs_ctx *ctx, const gegs_run_options *opt, double *dose, double *dose2);
+
+/* Tuning of gegs_run_event; NULL restores the defaults.  gegs_get_tuning
+ * returns the values in force with defaults filled in (grid sizes that fill
+ * the device are resolved at run time and reported as 0). */
+int gegs_set_tuning(gegs_ctx *ctx, const gegs_tuning *tuning);
+int gegs_get_tuning(gegs_ctx *ctx, gegs_tuning *tuning);
 
 /* Per-history diagnostics of the last run (total energy deposited in the
  * phantom per history, FP64; number of transport steps per history), sized

@@ -125,17 +125,32 @@ def test_errors_are_status_codes(gpu_ctx):
     fresh.close()
 
 
-@pytest.mark.parametrize("wl_name", ["water", "ct"])
-def test_event_path_reproduces_histories(gpu_ctx, wl_name, water, ct_small):
+TUNINGS = {
+    "default": None,
+    # every photon crossing and electron step is suspended and resumed in the
+    # next sweep, and primaries enter a few thousand at a time
+    "suspend_every_step": dict(photon_chunk=1, electron_chunk=1, inject=4096, threshold=8192,
+                               block=128, sweeps_per_graph=16),
+}
+
+
+@pytest.mark.parametrize("wl_name,tuning", [("water", "default"), ("ct", "default"),
+                                            ("ct", "suspend_every_step")])
+def test_event_path_reproduces_histories(gpu_ctx, wl_name, tuning, water, ct_small):
     """The event-queued path transports every history along the same
     trajectory as the plain kernel and the oracle: identical step counts and
     event counters; per-history energy equal up to the order of the FP64
     additions (deposits are summed per particle, then per history)."""
     wl = water if wl_name == "water" else ct_small
-    src = wl.source(20000, 31)
+    n = 20000 if tuning == "default" else 4000
+    src = wl.source(n, 31)
     gpu_ctx.load_workload(wl)
     gpu_ctx.set_source(src)
-    ev = gpu_ctx.run(seed=5, n_batches=3, method="event", per_history=True)
+    gpu_ctx.set_tuning(TUNINGS[tuning])
+    try:
+        ev = gpu_ctx.run(seed=5, n_batches=3, method="event", per_history=True)
+    finally:
+        gpu_ctx.set_tuning(None)
     pl = gpu_ctx.run(seed=5, n_batches=3, method="plain", per_history=True)
     assert np.array_equal(ev["steps"], pl["steps"])
     assert np.allclose(ev["edep"], pl["edep"], rtol=1e-12, atol=1e-12)
@@ -146,3 +161,14 @@ def test_event_path_reproduces_histories(gpu_ctx, wl_name, water, ct_small):
     assert np.allclose(ev["dose"], pl["dose"], rtol=1e-12, atol=1e-12)
     o = oracle.run(wl, src[:2000], seed=5, per_history=True)
     assert np.array_equal(ev["steps"][:2000], o["steps"])
+
+
+def test_tuning_roundtrip_and_validation(gpu_ctx):
+    gpu_ctx.set_tuning(dict(photon_chunk=7, inject=1000))
+    t = gpu_ctx.get_tuning()
+    assert t["photon_chunk"] == 7 and t["inject"] == 1000 and t["block"] == 256
+    with pytest.raises(native.GegsError) as e:
+        gpu_ctx.set_tuning(dict(block=96))
+    assert e.value.code == 2001
+    gpu_ctx.set_tuning(None)
+    assert gpu_ctx.get_tuning()["photon_chunk"] == 32
