@@ -29,7 +29,7 @@
 
 namespace gegs {
 
-constexpr int EVW = 20;        // words per banked particle (15-19: state of a suspended electron step)
+constexpr int EVW = 21;        // words per banked particle (15-19: state of a suspended electron step, 20: job)
 constexpr int SLOT_P_ALT = N_EVENTS;         // second buffer of the photon-step bank
 constexpr int SLOT_E_ALT = N_EVENTS + 1;     // second buffer of the electron-step bank
 constexpr int N_SLOTS = N_EVENTS + 2;
@@ -60,17 +60,41 @@ struct EventConfig {
     uint32_t sweeps_per_graph = 4;
 };
 
+// Source records of one batch, as slices of every job (field) of the run:
+// cursor k of the batch belongs to job j with prefix[j] <= k < prefix[j+1];
+// its record is rec0[j] + (k - prefix[j]), its Philox history index
+// hist0[j] + (k - prefix[j]) and its per-history tally slot slot0[j] + (k - prefix[j]).
+constexpr uint32_t MAX_JOBS = 512;
+
+struct BatchPlan {
+    uint32_t njob;
+    uint32_t total;                  // histories in the batch
+    uint32_t prefix[MAX_JOBS + 1];
+    uint32_t rec0[MAX_JOBS];
+    uint32_t slot0[MAX_JOBS];
+    uint64_t hist0[MAX_JOBS];
+};
+
 struct EvArgs {
     Env env;
+    const BatchPlan *plan;
+    const uint32_t *job_plan;        // dose buffer (plan) of each job
+    size_t nvox;
     float *bank;                     // [slot][word][cap]
     uint32_t cap;
     EvState *st;
     const gegs_source *src;
-    uint64_t hist_offset;
     double *dose;
     double *hist_edep;               // per source record, optional
     uint32_t *hist_steps;            // optional
     unsigned long long *counters;
+    // beamlet weights: history slot k belongs to beamlet beamlet[k]; the
+    // forward multiplies its source weight by bweight[beamlet[k]]
+    const uint32_t *beamlet;         // optional
+    const float *bweight;            // optional
+    // adjoint replay: dL/dD per plan and voxel, gradient per beamlet
+    const double *adj;
+    double *grad;
 };
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -109,6 +133,7 @@ __device__ __forceinline__ void bank_store(const EvArgs &a, int slot, uint32_t i
     *bank_word(a, slot, 12, idx) = __uint_as_float((uint32_t)(p.lin >> 32));
     *bank_word(a, slot, 13, idx) = __uint_as_float(p.blk);
     *bank_word(a, slot, 14, idx) = __uint_as_float(hist);
+    *bank_word(a, slot, 20, idx) = __uint_as_float(p.job);
 }
 
 __device__ __forceinline__ Part bank_load(const EvArgs &a, int slot, int event, uint32_t idx, uint32_t &hist) {
@@ -132,7 +157,17 @@ __device__ __forceinline__ Part bank_load(const EvArgs &a, int slot, int event, 
             ((uint64_t)__float_as_uint(*bank_word(a, slot, 12, idx)) << 32);
     p.blk = __float_as_uint(*bank_word(a, slot, 13, idx));
     hist = __float_as_uint(*bank_word(a, slot, 14, idx));
+    p.job = __float_as_uint(*bank_word(a, slot, 20, idx));
     return p;
+}
+
+// deposits of a particle go to the dose buffer (forward) or are weighted by
+// the dL/dD (adjoint) of its job's plan
+template <bool ADJ>
+__device__ __forceinline__ void bind_tally(const EvArgs &a, const Part &p, TallyT<ADJ> &t) {
+    const size_t off = (size_t)a.job_plan[p.job] * a.nvox;
+    if (ADJ) t.adj = a.adj + off;
+    else t.dose = a.dose + off;
 }
 
 // Appends to the bank named by p.status.  The lanes that reach a push
@@ -198,7 +233,12 @@ __device__ __forceinline__ void load_estate(const EvArgs &a, int slot, uint32_t 
     es.is_tstep = (f >> 17) & 1u;
 }
 
-__device__ __forceinline__ void flush_particle(const EvArgs &a, uint32_t hist, Tally &t) {
+template <bool ADJ>
+__device__ __forceinline__ void flush_particle(const EvArgs &a, uint32_t hist, TallyT<ADJ> &t) {
+    if (ADJ) {
+        if (t.acc != 0.0) unsafeAtomicAdd(a.grad + a.beamlet[hist], t.acc);
+        t.acc = 0.0;
+    }
     if (a.hist_edep && t.hist_edep != 0.0) unsafeAtomicAdd(a.hist_edep + hist, t.hist_edep);
     if (a.hist_steps && t.steps) atomicAdd(a.hist_steps + hist, t.steps);
     t.hist_edep = 0.0;
@@ -213,7 +253,8 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 
 // counters are reduced over the wavefront first: one atomic per wave and
 // counter instead of one per lane (all lanes of the wave must call this)
-__device__ __forceinline__ void flush_counters(const EvArgs &a, const Tally &t, int event, uint32_t pops) {
+template <class T>
+__device__ __forceinline__ void flush_counters(const EvArgs &a, const T &t, int event, uint32_t pops) {
     unsigned long long *c = a.counters;
     const uint32_t v0 = wave_sum(t.photon_steps), v1 = wave_sum(t.electron_steps);
     const uint32_t v2 = wave_sum(t.gathers), v3 = wave_sum(t.deposits), v4 = wave_sum(pops);
@@ -226,9 +267,12 @@ __device__ __forceinline__ void flush_counters(const EvArgs &a, const Tally &t, 
     }
 }
 
-__device__ __forceinline__ Tally fresh_tally(const EvArgs &a) {
-    Tally t;
+template <bool ADJ>
+__device__ __forceinline__ TallyT<ADJ> fresh_tally(const EvArgs &a) {
+    TallyT<ADJ> t;
     t.dose = a.dose;
+    t.adj = a.adj;
+    t.acc = 0.0;
     t.hist_edep = 0.0;
     t.steps = 0;
     t.photon_steps = t.electron_steps = t.gathers = t.deposits = 0;
@@ -238,11 +282,11 @@ __device__ __forceinline__ Tally fresh_tally(const EvArgs &a) {
 // single-pass events: compton, photo, rayleigh, pair, moller, brems, bhabha,
 // annih.  Wavefront w takes slots [64 w, 64 w + 64) and then strides by the
 // grid; an empty bank costs one load per wave.
-template <int E>
+template <int E, bool ADJ>
 __global__ void __launch_bounds__(256) interaction_kernel(EvArgs a) {
     const uint32_t n = a.st->count[E];
     if (n == 0) return;
-    Tally t = fresh_tally(a);
+    TallyT<ADJ> t = fresh_tally<ADJ>(a);
     // photon interactions run before the electron kernel of the sweep
     BankSink sink{&a, 0, a.st->flip, false, E < E_ELECTRON_STEP};
     uint32_t pops = 0;
@@ -253,6 +297,7 @@ __global__ void __launch_bounds__(256) interaction_kernel(EvArgs a) {
         if (idx < n) {
             uint32_t hist;
             Part p = bank_load(a, E, E, idx, hist);
+            bind_tally(a, p, t);
             sink.hist = hist;
             Stream s;
             s.begin(p, a.env.seed);
@@ -318,7 +363,7 @@ struct Refill {
 
 // photon free flight with lane refill; a flight still going after `chunk`
 // crossings is suspended into the next sweep's bank
-template <bool TAB>
+template <bool TAB, bool ADJ>
 __global__ void __launch_bounds__(512) photon_kernel(EvArgs a, uint32_t chunk, uint32_t refill_min) {
     extern __shared__ float lds_env[];
     const uint32_t flip = a.st->flip;
@@ -327,7 +372,7 @@ __global__ void __launch_bounds__(512) photon_kernel(EvArgs a, uint32_t chunk, u
     if (n == 0) return;
     Env env = a.env;
     stage_env<TAB>(env, lds_env);
-    Tally t = fresh_tally(a);
+    TallyT<ADJ> t = fresh_tally<ADJ>(a);
     BankSink sink{&a, 0, flip, false, true};
     uint32_t pops = 0;
     bool have = false;
@@ -341,6 +386,7 @@ __global__ void __launch_bounds__(512) photon_kernel(EvArgs a, uint32_t chunk, u
         const uint32_t slot = rf.take(&a.st->taken[slot_in], n, !have, refill_min);
         if (!have && slot != 0xFFFFFFFFu) {
             p = bank_load(a, slot_in, P_PHOTON_STEP, slot, hist);
+            bind_tally(a, p, t);
             s.begin(p, env.seed);
             photon_walk_begin(env, p, w);
             have = true;
@@ -366,7 +412,7 @@ __global__ void __launch_bounds__(512) photon_kernel(EvArgs a, uint32_t chunk, u
 
 // electron / positron condensed-history steps with lane refill; an electron
 // still stepping after `chunk` steps is suspended into the next sweep's bank
-template <bool TAB>
+template <bool TAB, bool ADJ>
 __global__ void __launch_bounds__(512) electron_kernel(EvArgs a, uint32_t chunk, uint32_t refill_min) {
     extern __shared__ float lds_env[];
     const uint32_t flip = a.st->flip;
@@ -375,7 +421,7 @@ __global__ void __launch_bounds__(512) electron_kernel(EvArgs a, uint32_t chunk,
     if (n == 0) return;
     Env env = a.env;
     stage_env<TAB>(env, lds_env);
-    Tally t = fresh_tally(a);
+    TallyT<ADJ> t = fresh_tally<ADJ>(a);
     BankSink sink{&a, 0, flip, false, false};
     uint32_t pops = 0;
     bool have = false;
@@ -389,6 +435,7 @@ __global__ void __launch_bounds__(512) electron_kernel(EvArgs a, uint32_t chunk,
         const uint32_t slot = rf.take(&a.st->taken[slot_in], n, !have, refill_min);
         if (!have && slot != 0xFFFFFFFFu) {
             p = bank_load(a, slot_in, E_ELECTRON_STEP, slot, hist);
+            bind_tally(a, p, t);
             s.begin(p, env.seed);
             if (p.spos & 8) {
                 load_estate(a, slot_in, slot, es);
@@ -438,6 +485,8 @@ __global__ void __launch_bounds__(256) inject_fill(EvArgs a) {
     const uint32_t n = a.st->inj_n;
     const uint32_t start = a.st->inj_start;
     BankSink sink{&a, 0, a.st->flip, true, true};
+    const BatchPlan &bp = *a.plan;
+    const uint32_t njob = bp.njob;
     uint32_t injected = 0;
     const uint32_t stride = gridDim.x * blockDim.x;
     // whole wavefronts iterate together so that the aggregated push sees
@@ -446,20 +495,30 @@ __global__ void __launch_bounds__(256) inject_fill(EvArgs a) {
     for (uint32_t k0 = wave0; k0 < n; k0 += stride) {
         const uint32_t k = k0 + lane_id();
         if (k < n) {
-            const uint32_t rec = start + k;
+            const uint32_t cur = start + k;
+            // job of the cursor: last j with prefix[j] <= cur
+            uint32_t lo = 0, hi = njob;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (bp.prefix[mid] <= cur) lo = mid; else hi = mid;
+            }
+            const uint32_t off = cur - bp.prefix[lo];
+            const uint32_t rec = bp.rec0[lo] + off;
             const gegs_source r = a.src[rec];
             Part p;
             p.e = r.e; p.x = r.x; p.y = r.y; p.z = r.z;
             p.u = r.u; p.v = r.v; p.w = r.w; p.wt = r.wt;
             p.charge = r.charge;
             p.region = 0;
-            p.module = a.env.f.module_first;
+            p.job = lo;
+            p.module = a.env.fields[lo].module_first;
             p.dnear = INFI_DIST;
             p.blk = 0;
             p.spos = 4;
-            p.lin = primary_lineage(a.hist_offset + rec);
+            p.lin = primary_lineage(bp.hist0[lo] + off);
             p.status = r.charge == 0 ? P_PHOTON_STEP : E_ELECTRON_STEP;
-            sink.hist = rec;
+            sink.hist = bp.slot0[lo] + off;
+            if (a.bweight) p.wt = p.wt * a.bweight[a.beamlet[sink.hist]];
             sink.push(p);
             injected++;
         }
@@ -514,6 +573,9 @@ struct EventBanks {
     uint32_t cap = 0;
     EvState *st = nullptr;
     uint32_t *h_flags = nullptr;     // pinned: active, sweeps, overflow
+    hipEvent_t done[2] = {nullptr, nullptr};   // completion of the two replays in flight
+    BatchPlan *d_plan = nullptr;     // injection plan of the current batch
+    BatchPlan *h_plan = nullptr;     // pinned staging copy
     hipGraphExec_t graph = nullptr;
     EvArgs graph_args{};
     EventConfig graph_cfg{};
@@ -527,6 +589,14 @@ struct EventBanks {
             if (e != hipSuccess) return e;
             e = hipHostMalloc(&h_flags, 4 * sizeof(uint32_t));
             if (e != hipSuccess) return e;
+            e = hipMalloc(&d_plan, sizeof(BatchPlan));
+            if (e != hipSuccess) return e;
+            e = hipHostMalloc(&h_plan, sizeof(BatchPlan));
+            if (e != hipSuccess) return e;
+            for (int k = 0; k < 2; ++k) {
+                e = hipEventCreateWithFlags(&done[k], hipEventDisableTiming);
+                if (e != hipSuccess) return e;
+            }
         }
         if (cap < want) {
             if (bank) (void)hipFree(bank);
@@ -548,9 +618,16 @@ struct EventBanks {
         if (bank) (void)hipFree(bank);
         if (st) (void)hipFree(st);
         if (h_flags) (void)hipHostFree(h_flags);
+        if (d_plan) (void)hipFree(d_plan);
+        for (int k = 0; k < 2; ++k)
+            if (done[k]) (void)hipEventDestroy(done[k]);
+        done[0] = done[1] = nullptr;
+        if (h_plan) (void)hipHostFree(h_plan);
         bank = nullptr;
         st = nullptr;
         h_flags = nullptr;
+        d_plan = nullptr;
+        h_plan = nullptr;
         cap = 0;
     }
 };
@@ -569,38 +646,68 @@ inline uint32_t resident_grid(K kernel, uint32_t block, size_t lds) {
 inline bool tab_in_lds(const Env &env) { return env.nclass <= LDS_MAX_CLASSES; }
 
 inline size_t flight_lds(const Env &env) {
-    return env_lds_bytes(env.nx, env.ny, env.nz, env.nclass, env.nmed, tab_in_lds(env));
+    return env_lds_bytes(env.nx, env.ny, env.nz, env.nclass, env.nmed, env.njob, tab_in_lds(env));
 }
 
-inline void launch_sweep(const EvArgs &a, const EventConfig &cfg, hipStream_t stream) {
+template <bool TAB, bool ADJ>
+inline void launch_sweep_t(const EvArgs &a, const EventConfig &cfg, hipStream_t stream) {
     const dim3 g(cfg.grid), b(cfg.block), fb(cfg.flight_block);
     const size_t lds = flight_lds(a.env);
-    const bool tab = tab_in_lds(a.env);
     hipLaunchKernelGGL(inject_plan, dim3(1), dim3(1), 0, stream, a.st, cfg.threshold, cfg.inject);
     hipLaunchKernelGGL(inject_fill, g, b, 0, stream, a);
-    if (tab) hipLaunchKernelGGL(photon_kernel<true>, dim3(cfg.photon_grid), fb, lds, stream, a, cfg.photon_chunk, cfg.refill_min);
-    else hipLaunchKernelGGL(photon_kernel<false>, dim3(cfg.photon_grid), fb, lds, stream, a, cfg.photon_chunk, cfg.refill_min);
-    hipLaunchKernelGGL(interaction_kernel<P_RAYLEIGH>, g, b, 0, stream, a);
-    hipLaunchKernelGGL(interaction_kernel<P_COMPTON>, g, b, 0, stream, a);
-    hipLaunchKernelGGL(interaction_kernel<P_PHOTO>, g, b, 0, stream, a);
-    hipLaunchKernelGGL(interaction_kernel<P_PAIR>, g, b, 0, stream, a);
-    if (tab) hipLaunchKernelGGL(electron_kernel<true>, dim3(cfg.electron_grid), fb, lds, stream, a, cfg.electron_chunk, cfg.refill_min);
-    else hipLaunchKernelGGL(electron_kernel<false>, dim3(cfg.electron_grid), fb, lds, stream, a, cfg.electron_chunk, cfg.refill_min);
-    hipLaunchKernelGGL(interaction_kernel<E_MOLLER>, g, b, 0, stream, a);
-    hipLaunchKernelGGL(interaction_kernel<E_BREMS>, g, b, 0, stream, a);
-    hipLaunchKernelGGL(interaction_kernel<E_BHABHA>, g, b, 0, stream, a);
-    hipLaunchKernelGGL(interaction_kernel<E_ANNIH>, g, b, 0, stream, a);
+    hipLaunchKernelGGL((photon_kernel<TAB, ADJ>), dim3(cfg.photon_grid), fb, lds, stream, a, cfg.photon_chunk,
+                       cfg.refill_min);
+    hipLaunchKernelGGL((interaction_kernel<P_RAYLEIGH, ADJ>), g, b, 0, stream, a);
+    hipLaunchKernelGGL((interaction_kernel<P_COMPTON, ADJ>), g, b, 0, stream, a);
+    hipLaunchKernelGGL((interaction_kernel<P_PHOTO, ADJ>), g, b, 0, stream, a);
+    hipLaunchKernelGGL((interaction_kernel<P_PAIR, ADJ>), g, b, 0, stream, a);
+    hipLaunchKernelGGL((electron_kernel<TAB, ADJ>), dim3(cfg.electron_grid), fb, lds, stream, a, cfg.electron_chunk,
+                       cfg.refill_min);
+    hipLaunchKernelGGL((interaction_kernel<E_MOLLER, ADJ>), g, b, 0, stream, a);
+    hipLaunchKernelGGL((interaction_kernel<E_BREMS, ADJ>), g, b, 0, stream, a);
+    hipLaunchKernelGGL((interaction_kernel<E_BHABHA, ADJ>), g, b, 0, stream, a);
+    hipLaunchKernelGGL((interaction_kernel<E_ANNIH, ADJ>), g, b, 0, stream, a);
     hipLaunchKernelGGL(sweep_end, dim3(1), dim3(1), 0, stream, a.st);
 }
 
+inline void launch_sweep(const EvArgs &a, const EventConfig &cfg, hipStream_t stream) {
+    const bool adj = a.adj != nullptr;
+    if (tab_in_lds(a.env)) {
+        if (adj) launch_sweep_t<true, true>(a, cfg, stream);
+        else launch_sweep_t<true, false>(a, cfg, stream);
+    } else {
+        if (adj) launch_sweep_t<false, true>(a, cfg, stream);
+        else launch_sweep_t<false, false>(a, cfg, stream);
+    }
+}
+
+template <bool TAB, bool ADJ>
+inline void flight_grids_t(EventConfig &cfg, size_t lds) {
+    if (!cfg.photon_grid) cfg.photon_grid = resident_grid(photon_kernel<TAB, ADJ>, cfg.flight_block, lds);
+    if (!cfg.electron_grid) cfg.electron_grid = resident_grid(electron_kernel<TAB, ADJ>, cfg.flight_block, lds);
+}
+
+// grid sizes of the persistent flight kernels that fill the device once
+inline void flight_grids(EventConfig &cfg, const Env &env, bool adj) {
+    const size_t lds = flight_lds(env);
+    if (tab_in_lds(env)) {
+        if (adj) flight_grids_t<true, true>(cfg, lds);
+        else flight_grids_t<true, false>(cfg, lds);
+    } else {
+        if (adj) flight_grids_t<false, true>(cfg, lds);
+        else flight_grids_t<false, false>(cfg, lds);
+    }
+}
+
 inline bool same_args(const EvArgs &x, const EvArgs &y) {
-    return x.bank == y.bank && x.cap == y.cap && x.st == y.st && x.src == y.src && x.hist_offset == y.hist_offset &&
+    return x.bank == y.bank && x.cap == y.cap && x.st == y.st && x.src == y.src && x.plan == y.plan &&
+           x.job_plan == y.job_plan && x.nvox == y.nvox && x.env.fields == y.env.fields && x.env.njob == y.env.njob &&
+           x.env.nclass == y.env.nclass && x.env.nmed == y.env.nmed &&
            x.dose == y.dose && x.hist_edep == y.hist_edep && x.hist_steps == y.hist_steps &&
+           x.beamlet == y.beamlet && x.bweight == y.bweight && x.adj == y.adj && x.grad == y.grad &&
            x.counters == y.counters && x.env.mt == y.env.mt && x.env.rcls == y.env.rcls && x.env.rtab == y.env.rtab && x.env.seed == y.env.seed &&
-           x.env.xb == y.env.xb && x.env.nx == y.env.nx && x.env.ny == y.env.ny && x.env.nz == y.env.nz &&
-           x.env.f.cos_angle == y.env.f.cos_angle && x.env.f.sin_angle == y.env.f.sin_angle &&
-           x.env.f.change_xyz == y.env.f.change_xyz && x.env.f.source_to_isocenter == y.env.f.source_to_isocenter &&
-           x.env.f.iso_x == y.env.f.iso_x && x.env.f.iso_y == y.env.f.iso_y && x.env.f.iso_z == y.env.f.iso_z;
+           x.env.xb == y.env.xb && x.env.yb == y.env.yb && x.env.zb == y.env.zb && x.env.nx == y.env.nx &&
+           x.env.ny == y.env.ny && x.env.nz == y.env.nz && x.env.pge == y.env.pge;
 }
 
 inline bool same_cfg(const EventConfig &x, const EventConfig &y) {
@@ -610,31 +717,40 @@ inline bool same_cfg(const EventConfig &x, const EventConfig &y) {
            x.threshold == y.threshold && x.sweeps_per_graph == y.sweeps_per_graph;
 }
 
-// Transport source records [first, last) through the banks.  The sweep group
-// is captured once per argument set and replayed; the host polls one flag.
-inline hipError_t run_event_batch(EventBanks &bk, const Env &env, const gegs_source *src, uint32_t first,
-                                  uint32_t last, uint64_t hist_offset, double *dose, double *hist_edep,
-                                  uint32_t *hist_steps, unsigned long long *counters, const EventConfig &cfg_in,
+// Transport the histories of one batch (bk.h_plan, filled by the caller)
+// through the banks.  The sweep group is captured once per argument set and
+// replayed; the host polls one flag per replay.
+// beamlet weights and adjoint buffers of a run (all optional)
+struct BeamletArgs {
+    const uint32_t *beamlet = nullptr;
+    const float *bweight = nullptr;
+    const double *adj = nullptr;
+    double *grad = nullptr;
+};
+
+inline hipError_t run_event_batch(EventBanks &bk, const Env &env, const gegs_source *src, const uint32_t *job_plan,
+                                  size_t nvox, double *dose, double *hist_edep, uint32_t *hist_steps,
+                                  unsigned long long *counters, const BeamletArgs &bl, const EventConfig &cfg_in,
                                   hipStream_t stream, uint32_t max_sweeps = 100000) {
     EventConfig cfg = cfg_in;
-    const size_t lds = flight_lds(env);
-    if (!cfg.photon_grid)
-        cfg.photon_grid = tab_in_lds(env) ? resident_grid(photon_kernel<true>, cfg.flight_block, lds)
-                                          : resident_grid(photon_kernel<false>, cfg.flight_block, lds);
-    if (!cfg.electron_grid)
-        cfg.electron_grid = tab_in_lds(env) ? resident_grid(electron_kernel<true>, cfg.flight_block, lds)
-                                            : resident_grid(electron_kernel<false>, cfg.flight_block, lds);
+    flight_grids(cfg, env, bl.adj != nullptr);
     EvArgs a;
     a.env = env;
+    a.plan = bk.d_plan;
+    a.job_plan = job_plan;
+    a.nvox = nvox;
     a.bank = bk.bank;
     a.cap = bk.cap;
     a.st = bk.st;
     a.src = src;
-    a.hist_offset = hist_offset;
     a.dose = dose;
     a.hist_edep = hist_edep;
     a.hist_steps = hist_steps;
     a.counters = counters;
+    a.beamlet = bl.beamlet;
+    a.bweight = bl.bweight;
+    a.adj = bl.adj;
+    a.grad = bl.grad;
     hipError_t e;
     if (!bk.graph || !same_args(a, bk.graph_args) || !same_cfg(cfg, bk.graph_cfg)) {
         bk.drop_graph();
@@ -650,18 +766,34 @@ inline hipError_t run_event_batch(EventBanks &bk, const Env &env, const gegs_sou
         bk.graph_args = a;
         bk.graph_cfg = cfg;
     }
-    hipLaunchKernelGGL(batch_begin, dim3(1), dim3(1), 0, stream, bk.st, first, last);
+    e = hipMemcpyAsync(bk.d_plan, bk.h_plan, sizeof(BatchPlan), hipMemcpyHostToDevice, stream);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(batch_begin, dim3(1), dim3(1), 0, stream, bk.st, 0u, bk.h_plan->total);
+    // Two replays in flight: while the host waits for the flag of replay k,
+    // replay k+1 is already queued, so the GPU does not idle during the host
+    // round trip.  The replay queued after the last busy one finds every bank
+    // empty and no record left; its kernels return at once.
     uint32_t sweeps = 0;
-    for (;;) {
-        e = hipGraphLaunch(bk.graph, stream);
-        if (e != hipSuccess) return e;
-        e = hipMemcpyAsync(bk.h_flags, &bk.st->active, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
-        if (e != hipSuccess) return e;
-        e = hipStreamSynchronize(stream);
+    auto launch = [&](int slot) -> hipError_t {
+        hipError_t r = hipGraphLaunch(bk.graph, stream);
+        if (r != hipSuccess) return r;
+        r = hipMemcpyAsync(bk.h_flags + 2 * slot, &bk.st->active, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
+        if (r != hipSuccess) return r;
+        return hipEventRecord(bk.done[slot], stream);
+    };
+    if ((e = launch(0)) != hipSuccess) return e;
+    if ((e = launch(1)) != hipSuccess) return e;
+    for (int k = 0;; ++k) {
+        const int slot = k & 1;
+        e = hipEventSynchronize(bk.done[slot]);
         if (e != hipSuccess) return e;
         sweeps += cfg.sweeps_per_graph;
-        if (!bk.h_flags[0]) break;
-        if (sweeps >= max_sweeps) return hipErrorLaunchTimeOut;
+        if (!bk.h_flags[2 * slot]) break;
+        if (sweeps >= max_sweeps) {
+            (void)hipStreamSynchronize(stream);
+            return hipErrorLaunchTimeOut;
+        }
+        if ((e = launch(slot)) != hipSuccess) return e;
     }
     return hipSuccess;
 }

@@ -33,6 +33,16 @@ struct gegs_ctx {
     uint32_t nx = 0, ny = 0, nz = 0;
     size_t nvox = 0;
     gegs_field field{};
+    gegs_field *d_fields = nullptr;      // beam geometry per job of the current run
+    uint32_t *d_job_plan = nullptr;      // dose buffer per job
+    size_t fields_cap = 0, job_plan_cap = 0;
+    uint64_t hist_len = 0;               // per-history slots of the last run
+    uint32_t *d_beamlet = nullptr;       // beamlet of each history slot
+    float *d_bweight = nullptr;          // weight of each beamlet
+    double *d_adj = nullptr, *d_grad = nullptr;
+    size_t beamlet_cap = 0, bweight_cap = 0, adj_cap = 0, grad_cap = 0;
+    uint64_t n_slots_bl = 0;             // history slots covered by d_beamlet
+    uint32_t n_beamlets = 0;
     bool have_media = false, have_phantom = false, have_field = false;
     gegs_source *d_src = nullptr;
     uint64_t nsrc = 0;
@@ -236,35 +246,52 @@ int gegs_set_history_tally(gegs_ctx *c, int enable) {
     return GEGS_OK;
 }
 
-static int prepare_tallies(gegs_ctx *c) {
+// zeroed tallies for n_plans dose buffers and n_slots per-history slots
+static int prepare_tallies(gegs_ctx *c, uint32_t n_plans, uint64_t n_slots) {
+    const size_t n = c->nvox * n_plans;
     size_t have = c->tally_n;
-    HIPCHECK(c, realloc_dev(c->d_chunk, have, c->nvox), "hipMalloc tally");
+    HIPCHECK(c, realloc_dev(c->d_chunk, have, n), "hipMalloc tally");
     have = c->tally_n;
-    HIPCHECK(c, realloc_dev(c->d_dose, have, c->nvox), "hipMalloc tally");
+    HIPCHECK(c, realloc_dev(c->d_dose, have, n), "hipMalloc tally");
     have = c->tally_n;
-    HIPCHECK(c, realloc_dev(c->d_dose2, have, c->nvox), "hipMalloc tally");
-    c->tally_n = c->nvox;
+    HIPCHECK(c, realloc_dev(c->d_dose2, have, n), "hipMalloc tally");
+    if (c->tally_n < n) c->tally_n = n;
     const int blocks = 2048;
-    hipLaunchKernelGGL(fill_zero, dim3(blocks), dim3(256), 0, c->stream, c->d_chunk, c->nvox);
-    hipLaunchKernelGGL(fill_zero, dim3(blocks), dim3(256), 0, c->stream, c->d_dose, c->nvox);
-    hipLaunchKernelGGL(fill_zero, dim3(blocks), dim3(256), 0, c->stream, c->d_dose2, c->nvox);
+    hipLaunchKernelGGL(fill_zero, dim3(blocks), dim3(256), 0, c->stream, c->d_chunk, n);
+    hipLaunchKernelGGL(fill_zero, dim3(blocks), dim3(256), 0, c->stream, c->d_dose, n);
+    hipLaunchKernelGGL(fill_zero, dim3(blocks), dim3(256), 0, c->stream, c->d_dose2, n);
     hipLaunchKernelGGL(fill_zero_u64, dim3(1), dim3(64), 0, c->stream, c->d_counters, (size_t)16);
     hipLaunchKernelGGL(fill_zero_u32, dim3(16), dim3(256), 0, c->stream, c->d_next, (size_t)4096);
+    c->hist_len = c->hist_tally ? n_slots : 0;
     if (c->hist_tally) {
-        if (c->hist_n < c->nsrc) {
+        if (c->hist_n < n_slots) {
             hipFree(c->d_hist_edep);
             hipFree(c->d_hist_steps);
             c->d_hist_edep = nullptr;
             c->d_hist_steps = nullptr;
-            HIPCHECK(c, hipMalloc(&c->d_hist_edep, c->nsrc * sizeof(double)), "hipMalloc history tally");
-            HIPCHECK(c, hipMalloc(&c->d_hist_steps, c->nsrc * sizeof(uint32_t)), "hipMalloc history tally");
-            c->hist_n = c->nsrc;
+            HIPCHECK(c, hipMalloc(&c->d_hist_edep, n_slots * sizeof(double)), "hipMalloc history tally");
+            HIPCHECK(c, hipMalloc(&c->d_hist_steps, n_slots * sizeof(uint32_t)), "hipMalloc history tally");
+            c->hist_n = n_slots;
         }
+        hipLaunchKernelGGL(fill_zero, dim3(1024), dim3(256), 0, c->stream, c->d_hist_edep, (size_t)n_slots);
+        hipLaunchKernelGGL(fill_zero_u32, dim3(1024), dim3(256), 0, c->stream, c->d_hist_steps, (size_t)n_slots);
     }
     return GEGS_OK;
 }
 
-static Env make_env(gegs_ctx *c, uint64_t seed) {
+// beam geometry and dose buffer of each job, in device memory
+static int upload_jobs(gegs_ctx *c, const std::vector<gegs_field> &fields, const std::vector<uint32_t> &plans) {
+    HIPCHECK(c, realloc_dev(c->d_fields, c->fields_cap, fields.size()), "hipMalloc fields");
+    HIPCHECK(c, realloc_dev(c->d_job_plan, c->job_plan_cap, plans.size()), "hipMalloc job plans");
+    HIPCHECK(c, hipMemcpyAsync(c->d_fields, fields.data(), fields.size() * sizeof(gegs_field), hipMemcpyHostToDevice,
+                               c->stream), "copy fields");
+    HIPCHECK(c, hipMemcpyAsync(c->d_job_plan, plans.data(), plans.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                               c->stream), "copy job plans");
+    HIPCHECK(c, hipStreamSynchronize(c->stream), "copy jobs");
+    return GEGS_OK;
+}
+
+static Env make_env(gegs_ctx *c, uint64_t seed, uint32_t njob) {
     Env env;
     env.mt = c->d_media;
     env.rcls = c->d_rcls;
@@ -280,12 +307,13 @@ static Env make_env(gegs_ctx *c, uint64_t seed) {
     env.nx = c->nx;
     env.ny = c->ny;
     env.nz = c->nz;
-    env.f = c->field;
+    env.fields = c->d_fields;
+    env.njob = njob;
     env.seed = seed;
     return env;
 }
 
-static int finish_run(gegs_ctx *c, double *dose, double *dose2, hipEvent_t t0, hipEvent_t t1) {
+static int finish_run(gegs_ctx *c, double *dose, double *dose2, hipEvent_t t0, hipEvent_t t1, uint32_t n_plans = 1) {
     HIPCHECK(c, hipEventSynchronize(t1), "run");
     hipEventElapsedTime(&c->last_ms, t0, t1);
     hipEventDestroy(t0);
@@ -299,8 +327,9 @@ static int finish_run(gegs_ctx *c, double *dose, double *dose2, hipEvent_t t0, h
     for (int k = 0; k < 10; ++k) c->counters.interactions[k] = h[4 + k];
     c->counters.histories = h[14];
     c->counters.overflow = h[15];
-    if (dose) HIPCHECK(c, hipMemcpy(dose, c->d_dose, c->nvox * sizeof(double), hipMemcpyDeviceToHost), "read dose");
-    if (dose2) HIPCHECK(c, hipMemcpy(dose2, c->d_dose2, c->nvox * sizeof(double), hipMemcpyDeviceToHost), "read dose2");
+    const size_t n = c->nvox * n_plans;
+    if (dose) HIPCHECK(c, hipMemcpy(dose, c->d_dose, n * sizeof(double), hipMemcpyDeviceToHost), "read dose");
+    if (dose2) HIPCHECK(c, hipMemcpy(dose2, c->d_dose2, n * sizeof(double), hipMemcpyDeviceToHost), "read dose2");
     if (c->counters.overflow) return fail(c, GEGS_ERR_QUEUE_OVERFLOW, "particle stack overflow: particles were lost");
     return GEGS_OK;
 }
@@ -320,7 +349,9 @@ int gegs_run_plain(gegs_ctx *c, const gegs_run_options *o, double *dose, double 
     int rc = check_ready(c, o);
     if (rc) return rc;
     hipSetDevice(c->device);
-    rc = prepare_tallies(c);
+    rc = prepare_tallies(c, 1, c->nsrc);
+    if (rc) return rc;
+    rc = upload_jobs(c, {c->field}, {0u});
     if (rc) return rc;
     const uint32_t block = o->block_size ? o->block_size : 128;
     if (block != 128) return fail(c, GEGS_ERR_ARG, "run_plain: block_size must be 128");
@@ -331,7 +362,7 @@ int gegs_run_plain(gegs_ctx *c, const gegs_run_options *o, double *dose, double 
     HIPCHECK(c, realloc_dev(c->d_stack, have, nthreads * (size_t)depth * PART_WORDS), "hipMalloc stack");
     c->stack_bytes = have * sizeof(float);
     PlainArgs a;
-    a.env = make_env(c, o->seed);
+    a.env = make_env(c, o->seed, 1);
     a.src = c->d_src;
     a.hist_offset = o->history_offset;
     a.stack = c->d_stack;
@@ -340,6 +371,8 @@ int gegs_run_plain(gegs_ctx *c, const gegs_run_options *o, double *dose, double 
     a.hist_edep = c->hist_tally ? c->d_hist_edep : nullptr;
     a.hist_steps = c->hist_tally ? c->d_hist_steps : nullptr;
     a.counters = c->d_counters;
+    const bool tab = c->nclass <= LDS_MAX_CLASSES;
+    const size_t lds = env_lds_bytes(c->nx, c->ny, c->nz, c->nclass, c->nmed, 1, tab);
     hipEvent_t t0, t1;
     hipEventCreate(&t0);
     hipEventCreate(&t1);
@@ -350,8 +383,6 @@ int gegs_run_plain(gegs_ctx *c, const gegs_run_options *o, double *dose, double 
         a.first = (uint32_t)(r0 + (nr * b) / nb);
         a.last = (uint32_t)(r0 + (nr * (b + 1)) / nb);
         a.next = c->d_next + b;
-        const bool tab = c->nclass <= LDS_MAX_CLASSES;
-        const size_t lds = env_lds_bytes(c->nx, c->ny, c->nz, c->nclass, c->nmed, tab);
         if (tab) hipLaunchKernelGGL(plain_kernel<true>, dim3(grid), dim3(block), lds, c->stream, a);
         else hipLaunchKernelGGL(plain_kernel<false>, dim3(grid), dim3(block), lds, c->stream, a);
         hipLaunchKernelGGL(fold_batch, dim3(2048), dim3(256), 0, c->stream, c->d_chunk, c->d_dose, c->d_dose2, c->nvox);
@@ -361,40 +392,164 @@ int gegs_run_plain(gegs_ctx *c, const gegs_run_options *o, double *dose, double 
     return finish_run(c, dose, dose2, t0, t1);
 }
 
-int gegs_run_event(gegs_ctx *c, const gegs_run_options *o, double *dose, double *dose2) {
-    int rc = check_ready(c, o);
-    if (rc) return rc;
+// One job of an event-queued run: a field, its dose buffer and its records.
+struct JobSpec {
+    uint32_t plan;
+    uint64_t first, count, history_offset, slot0;
+};
+
+// Event-queued transport of every job in one loop: each batch holds the
+// b-th of n_batches equal slices of every job, so that every batch is an
+// independent sample of every plan's dose.
+static int run_event_jobs(gegs_ctx *c, const gegs_run_options *o, const std::vector<gegs_field> &fields,
+                          const std::vector<JobSpec> &jobs, uint32_t n_plans, uint64_t n_slots, double *dose,
+                          double *dose2, const BeamletArgs &bl = BeamletArgs()) {
     hipSetDevice(c->device);
-    rc = prepare_tallies(c);
+    int rc = prepare_tallies(c, n_plans, n_slots);
+    if (rc) return rc;
+    std::vector<uint32_t> plans(jobs.size());
+    for (size_t j = 0; j < jobs.size(); ++j) plans[j] = jobs[j].plan;
+    rc = upload_jobs(c, fields, plans);
     if (rc) return rc;
     const EventConfig cfg = c->tuning;
     const uint32_t nb = o->n_batches;
-    Env env = make_env(c, o->seed);
+    Env env = make_env(c, o->seed, (uint32_t)jobs.size());
     hipError_t e = c->banks.reserve(c->nsrc, nb, cfg);
     if (e != hipSuccess) return fail(c, GEGS_ERR_DEVICE, "event banks", e);
-    if (c->hist_tally) {
-        hipLaunchKernelGGL(fill_zero, dim3(1024), dim3(256), 0, c->stream, c->d_hist_edep, (size_t)c->nsrc);
-        hipLaunchKernelGGL(fill_zero_u32, dim3(1024), dim3(256), 0, c->stream, c->d_hist_steps, (size_t)c->nsrc);
-    }
     hipLaunchKernelGGL(clear_overflow, dim3(1), dim3(1), 0, c->stream, c->banks.st);
     hipEvent_t t0, t1;
     (void)hipEventCreate(&t0);
     (void)hipEventCreate(&t1);
     (void)hipEventRecord(t0, c->stream);
-    const uint64_t r0 = o->first, nr = o->count ? o->count : c->nsrc - o->first;
+    const size_t ntally = c->nvox * n_plans;
     for (uint32_t b = 0; b < nb; ++b) {
-        const uint32_t first = (uint32_t)(r0 + (nr * b) / nb);
-        const uint32_t last = (uint32_t)(r0 + (nr * (b + 1)) / nb);
-        e = run_event_batch(c->banks, env, c->d_src, first, last, o->history_offset, c->d_chunk,
+        BatchPlan &bp = *c->banks.h_plan;
+        bp.njob = (uint32_t)jobs.size();
+        uint64_t total = 0;
+        for (size_t j = 0; j < jobs.size(); ++j) {
+            const JobSpec &js = jobs[j];
+            const uint64_t lo = js.count * b / nb, hi = js.count * (b + 1) / nb;
+            bp.prefix[j] = (uint32_t)total;
+            bp.rec0[j] = (uint32_t)(js.first + lo);
+            bp.slot0[j] = (uint32_t)(js.slot0 + lo);
+            bp.hist0[j] = js.history_offset + lo;
+            total += hi - lo;
+        }
+        if (total > 0xFFFFFFF0ull) return fail(c, GEGS_ERR_ARG, "run: more than 2^32-16 histories in one batch");
+        bp.prefix[jobs.size()] = (uint32_t)total;
+        bp.total = (uint32_t)total;
+        e = run_event_batch(c->banks, env, c->d_src, c->d_job_plan, c->nvox, c->d_chunk,
                             c->hist_tally ? c->d_hist_edep : nullptr, c->hist_tally ? c->d_hist_steps : nullptr,
-                            c->d_counters, cfg, c->stream);
+                            c->d_counters, bl, cfg, c->stream);
         if (e != hipSuccess) return fail(c, GEGS_ERR_DEVICE, "event batch", e);
-        hipLaunchKernelGGL(fold_batch, dim3(2048), dim3(256), 0, c->stream, c->d_chunk, c->d_dose, c->d_dose2, c->nvox);
+        if (!bl.adj)
+            hipLaunchKernelGGL(fold_batch, dim3(2048), dim3(256), 0, c->stream, c->d_chunk, c->d_dose, c->d_dose2, ntally);
     }
     hipLaunchKernelGGL(report_overflow, dim3(1), dim3(1), 0, c->stream, c->banks.st, c->d_counters);
     (void)hipEventRecord(t1, c->stream);
     HIPCHECK(c, hipGetLastError(), "launch event kernels");
-    return finish_run(c, dose, dose2, t0, t1);
+    return finish_run(c, dose, dose2, t0, t1, n_plans);
+}
+
+int gegs_run_event(gegs_ctx *c, const gegs_run_options *o, double *dose, double *dose2) {
+    int rc = check_ready(c, o);
+    if (rc) return rc;
+    const uint64_t r0 = o->first, nr = o->count ? o->count : c->nsrc - o->first;
+    // one job; per-history slots are record indices
+    const JobSpec js{0u, r0, nr, o->history_offset + r0, r0};
+    return run_event_jobs(c, o, {c->field}, {js}, 1, c->nsrc, dose, dose2);
+}
+
+static int parse_jobs(gegs_ctx *c, const gegs_job *jobs, uint32_t n_jobs, uint32_t n_plans, const gegs_run_options *o,
+                      std::vector<gegs_field> &fields, std::vector<JobSpec> &specs, uint64_t &slots) {
+    if (!c || !jobs || !o) return fail(c, GEGS_ERR_ARG, "run: null argument");
+    if (n_jobs == 0 || n_jobs > MAX_JOBS) return fail(c, GEGS_ERR_ARG, "run: 1 to 512 jobs");
+    if (n_plans == 0 || n_plans > n_jobs) return fail(c, GEGS_ERR_ARG, "run: 1 to n_jobs plans");
+    if (!c->have_media || !c->have_phantom) return fail(c, GEGS_ERR_STATE, "run: load media and phantom first");
+    if (c->nsrc == 0) return fail(c, GEGS_ERR_STATE, "run: no source particles");
+    if (c->max_med >= c->nmed) return fail(c, GEGS_ERR_MEDIA, "run: the phantom uses a medium the loaded tables lack");
+    if (o->n_batches < 2 || o->n_batches > 4096) return fail(c, GEGS_ERR_ARG, "run: n_batches must be in [2, 4096]");
+    fields.resize(n_jobs);
+    specs.resize(n_jobs);
+    slots = 0;
+    for (uint32_t j = 0; j < n_jobs; ++j) {
+        const gegs_job &jb = jobs[j];
+        if (jb.plan >= n_plans) return fail(c, GEGS_ERR_ARG, "run: job plan out of range");
+        if (jb.field.module_first != GEGS_M_PHANTOM) return fail(c, GEGS_ERR_ARG, "run: only the phantom module is built");
+        if (jb.count == 0 || jb.first >= c->nsrc || jb.first + jb.count > c->nsrc)
+            return fail(c, GEGS_ERR_ARG, "run: job record range outside the loaded source");
+        fields[j] = jb.field;
+        specs[j] = JobSpec{jb.plan, jb.first, jb.count, jb.history_offset, slots};
+        slots += jb.count;
+    }
+    return GEGS_OK;
+}
+
+int gegs_run_batched(gegs_ctx *c, const gegs_job *jobs, uint32_t n_jobs, uint32_t n_plans, const gegs_run_options *o,
+                     double *dose, double *dose2) {
+    std::vector<gegs_field> fields;
+    std::vector<JobSpec> specs;
+    uint64_t slots = 0;
+    int rc = parse_jobs(c, jobs, n_jobs, n_plans, o, fields, specs, slots);
+    if (rc) return rc;
+    BeamletArgs bl;
+    if (c->d_beamlet) {
+        if (c->n_slots_bl != slots) return fail(c, GEGS_ERR_ARG, "run: beamlet map does not cover the jobs' histories");
+        bl.beamlet = c->d_beamlet;
+        bl.bweight = c->d_bweight;
+    }
+    return run_event_jobs(c, o, fields, specs, n_plans, slots, dose, dose2, bl);
+}
+
+int gegs_set_beamlets(gegs_ctx *c, const uint32_t *beamlet, uint64_t n_slots, const float *weight, uint32_t n_beamlets) {
+    if (!c) return GEGS_ERR_ARG;
+    hipSetDevice(c->device);
+    if (!beamlet) {
+        c->n_slots_bl = 0;
+        c->n_beamlets = 0;
+        hipFree(c->d_beamlet);
+        hipFree(c->d_bweight);
+        c->d_beamlet = nullptr;
+        c->d_bweight = nullptr;
+        c->beamlet_cap = c->bweight_cap = 0;
+        return GEGS_OK;
+    }
+    if (!weight || n_slots == 0 || n_beamlets == 0) return fail(c, GEGS_ERR_ARG, "gegs_set_beamlets: bad argument");
+    for (uint64_t k = 0; k < n_slots; ++k)
+        if (beamlet[k] >= n_beamlets) return fail(c, GEGS_ERR_ARG, "gegs_set_beamlets: beamlet index out of range");
+    HIPCHECK(c, realloc_dev(c->d_beamlet, c->beamlet_cap, n_slots), "hipMalloc beamlets");
+    HIPCHECK(c, realloc_dev(c->d_bweight, c->bweight_cap, n_beamlets), "hipMalloc beamlet weights");
+    HIPCHECK(c, hipMemcpy(c->d_beamlet, beamlet, n_slots * sizeof(uint32_t), hipMemcpyHostToDevice), "copy beamlets");
+    HIPCHECK(c, hipMemcpy(c->d_bweight, weight, n_beamlets * sizeof(float), hipMemcpyHostToDevice), "copy weights");
+    c->n_slots_bl = n_slots;
+    c->n_beamlets = n_beamlets;
+    return GEGS_OK;
+}
+
+int gegs_adjoint(gegs_ctx *c, const gegs_job *jobs, uint32_t n_jobs, uint32_t n_plans, const gegs_run_options *o,
+                 const double *dLdD, double *grad) {
+    if (!c || !dLdD || !grad) return fail(c, GEGS_ERR_ARG, "gegs_adjoint: null argument");
+    std::vector<gegs_field> fields;
+    std::vector<JobSpec> specs;
+    uint64_t slots = 0;
+    int rc = parse_jobs(c, jobs, n_jobs, n_plans, o, fields, specs, slots);
+    if (rc) return rc;
+    if (!c->d_beamlet || c->n_slots_bl != slots)
+        return fail(c, GEGS_ERR_STATE, "gegs_adjoint: set a beamlet map covering the jobs' histories first");
+    hipSetDevice(c->device);
+    const size_t n = c->nvox * n_plans;
+    HIPCHECK(c, realloc_dev(c->d_adj, c->adj_cap, n), "hipMalloc dL/dD");
+    HIPCHECK(c, realloc_dev(c->d_grad, c->grad_cap, c->n_beamlets), "hipMalloc gradient");
+    HIPCHECK(c, hipMemcpy(c->d_adj, dLdD, n * sizeof(double), hipMemcpyHostToDevice), "copy dL/dD");
+    hipLaunchKernelGGL(fill_zero, dim3(64), dim3(256), 0, c->stream, c->d_grad, (size_t)c->n_beamlets);
+    BeamletArgs bl;
+    bl.beamlet = c->d_beamlet;     // replay at unit beamlet weight: no bweight
+    bl.adj = c->d_adj;
+    bl.grad = c->d_grad;
+    rc = run_event_jobs(c, o, fields, specs, n_plans, slots, nullptr, nullptr, bl);
+    if (rc) return rc;
+    HIPCHECK(c, hipMemcpy(grad, c->d_grad, c->n_beamlets * sizeof(double), hipMemcpyDeviceToHost), "read gradient");
+    return GEGS_OK;
 }
 
 int gegs_set_tuning(gegs_ctx *c, const gegs_tuning *t) {
@@ -465,7 +620,9 @@ int gegs_sample_events(gegs_ctx *c, const gegs_particle *in, uint32_t n, uint64_
     HIPCHECK(c, hipMalloc(&d_n, n * sizeof(uint32_t)), "hipMalloc sample");
     HIPCHECK(c, hipMalloc(&d_e, n * sizeof(double)), "hipMalloc sample");
     HIPCHECK(c, hipMemcpy(d_in, in, n * sizeof(gegs_particle), hipMemcpyHostToDevice), "copy sample");
-    Env env = make_env(c, seed);
+    const int rc = upload_jobs(c, {c->field}, {0u});
+    if (rc) return rc;
+    Env env = make_env(c, seed, 1);
     hipLaunchKernelGGL(sample_kernel, dim3((n + 127) / 128), dim3(128), 0, c->stream, env, d_in, n, d_out, max_out, d_n, d_e);
     HIPCHECK(c, hipStreamSynchronize(c->stream), "sample kernel");
     HIPCHECK(c, hipMemcpy(out, d_out, (size_t)n * max_out * sizeof(gegs_particle), hipMemcpyDeviceToHost), "read sample");
@@ -482,8 +639,8 @@ int gegs_history_tally(gegs_ctx *c, double *edep, uint32_t *steps) {
     if (!c) return GEGS_ERR_ARG;
     if (!c->hist_tally || !c->d_hist_edep) return fail(c, GEGS_ERR_STATE, "history tally not enabled");
     hipSetDevice(c->device);
-    if (edep) HIPCHECK(c, hipMemcpy(edep, c->d_hist_edep, c->nsrc * sizeof(double), hipMemcpyDeviceToHost), "read edep");
-    if (steps) HIPCHECK(c, hipMemcpy(steps, c->d_hist_steps, c->nsrc * sizeof(uint32_t), hipMemcpyDeviceToHost), "read steps");
+    if (edep) HIPCHECK(c, hipMemcpy(edep, c->d_hist_edep, c->hist_len * sizeof(double), hipMemcpyDeviceToHost), "read edep");
+    if (steps) HIPCHECK(c, hipMemcpy(steps, c->d_hist_steps, c->hist_len * sizeof(uint32_t), hipMemcpyDeviceToHost), "read steps");
     return GEGS_OK;
 }
 

@@ -44,6 +44,7 @@ struct StackSink {
         *at(k, 11) = __uint_as_float((uint32_t)p.lin);
         *at(k, 12) = __uint_as_float((uint32_t)(p.lin >> 32));
         *at(k, 13) = __uint_as_float(p.blk);
+        *at(k, 14) = __uint_as_float(p.job);
     }
     __device__ __forceinline__ Part pop() {
         const int k = --sp;
@@ -65,6 +66,7 @@ struct StackSink {
         p.spos = (int32_t)(pk >> 24);
         p.lin = (uint64_t)__float_as_uint(*at(k, 11)) | ((uint64_t)__float_as_uint(*at(k, 12)) << 32);
         p.blk = __float_as_uint(*at(k, 13));
+        p.job = __float_as_uint(*at(k, 14));
         return p;
     }
 };
@@ -92,6 +94,7 @@ __device__ __forceinline__ Part source_particle(const gegs_source &r, int module
     p.module = module;
     p.dnear = INFI_DIST;
     p.blk = 0;
+    p.job = 0;
     p.spos = 4;
     p.status = r.charge == 0 ? P_PHOTON_STEP : E_ELECTRON_STEP;
     return p;
@@ -119,7 +122,7 @@ __global__ void __launch_bounds__(128) plain_kernel(PlainArgs a) {
         const uint32_t k = atomicAdd(a.next, 1u);
         if (k >= a.last - a.first) break;
         const uint32_t i = a.first + k;
-        Part p = source_particle(a.src[i], env.f.module_first);
+        Part p = source_particle(a.src[i], env.fields[0].module_first);
         p.lin = primary_lineage(a.hist_offset + i);
         t.hist_edep = 0.0;
         t.steps = 0;
@@ -181,7 +184,7 @@ __global__ void sample_kernel(Env env, const gegs_particle *in, uint32_t n, gegs
     Part p;
     p.e = r.e; p.x = r.x; p.y = r.y; p.z = r.z; p.u = r.u; p.v = r.v; p.w = r.w; p.wt = r.wt;
     p.dnear = r.dnear; p.region = r.region; p.charge = r.charge; p.status = r.status;
-    p.module = env.f.module_first; p.lin = r.lin; p.blk = r.blk; p.spos = 4;
+    p.module = env.fields[0].module_first; p.lin = r.lin; p.blk = r.blk; p.spos = 4; p.job = 0;
     ListSink sink{out + (size_t)i * max_out, max_out, 0};
     Tally t;
     t.dose = nullptr;

@@ -72,6 +72,7 @@ struct Part {
     int32_t module;
     uint64_t lin;
     uint32_t blk;
+    uint32_t job;        // field of the plan the history belongs to (beam geometry, dose buffer)
     int32_t spos;        // bits 0-2: uniforms of block blk-1 already used (4: none
                          // pending); bit 3: a suspended flight that resumes
 };
@@ -167,21 +168,23 @@ struct Env {
     const gegs_float2 *pge, *pgmfp, *pcohe;   // photon rows read at every crossing (mt->ge, gmfp, cohe)
     uint32_t nx, ny, nz;
     uint32_t nclass, nmed;
-    gegs_field f;
+    const gegs_field *fields;        // beam geometry per job
+    uint32_t njob;
     uint64_t seed;
 };
 
 // LDS staging of what a voxel crossing reads: the voxel bounds and, when the
 // phantom has at most LDS_MAX_CLASSES distinct region records, the record
 // table and the photon rows ge/gmfp/cohe of every medium.  Layout:
-// [records nclass x 16 B][ge nmed x 8 B][gmfp nmed x MXGE x 8 B][cohe ...][bounds].
+// [records nclass x 16 B][ge nmed x 8 B][gmfp nmed x MXGE x 8 B][cohe ...]
+// [fields njob x 32 B][bounds].
 // All threads of the block call stage_env() before any of them tracks a
 // particle; the env it returns points at LDS.
 constexpr uint32_t LDS_MAX_CLASSES = 2048;
 
 __host__ __device__ inline size_t env_lds_bytes(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t nclass, uint32_t nmed,
-                                                bool tab) {
-    size_t b = ((size_t)nx + ny + nz + 6) * sizeof(float);
+                                                uint32_t njob, bool tab) {
+    size_t b = (size_t)njob * sizeof(gegs_field) + ((size_t)nx + ny + nz + 6) * sizeof(float);
     if (tab) b += (size_t)nclass * 16 + (size_t)nmed * 8 + 2 * (size_t)nmed * MXGE * 8;
     return b;
 }
@@ -209,6 +212,14 @@ __device__ __forceinline__ void stage_env(Env &env, float *lds) {
         pcohe = g + env.nmed + nrow;
         p += 2 * ((size_t)env.nmed + 2 * nrow);
     }
+    {
+        const uint32_t nw = env.njob * (uint32_t)(sizeof(gegs_field) / 4);
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(env.fields);
+        uint32_t *dst = reinterpret_cast<uint32_t *>(p);
+        for (uint32_t i = threadIdx.x; i < nw; i += blockDim.x) dst[i] = src[i];
+        env.fields = reinterpret_cast<const gegs_field *>(p);
+        p += nw;
+    }
     const uint32_t nxb = env.nx + 2, nyb = env.ny + 2, nzb = env.nz + 2;
     for (uint32_t i = threadIdx.x; i < nxb; i += blockDim.x) p[i] = env.xb[i];
     for (uint32_t i = threadIdx.x; i < nyb; i += blockDim.x) p[nxb + i] = env.yb[i];
@@ -225,8 +236,17 @@ __device__ __forceinline__ void stage_env(Env &env, float *lds) {
 
 __device__ __forceinline__ gegs_region region_rec(const Env &env, uint32_t r) { return env.rtab[env.rcls[r]]; }
 
-struct Tally {
-    double *dose;          // per voxel, FP64
+__device__ __forceinline__ const gegs_field &field_of(const Env &env, const Part &p) { return env.fields[p.job]; }
+
+// Scoring policy.  Forward (ADJ = false): every deposit E in voxel v is added
+// to the dose buffer in FP64 (escore, scoring.cu:14-24).  Adjoint replay
+// (ADJ = true): the deposit is weighted by dL/dD of that voxel and summed per
+// particle (acc); the sum goes to the gradient of the history's beamlet.
+template <bool ADJ>
+struct TallyT {
+    double *dose;          // per voxel, FP64 (forward)
+    const double *adj;     // dL/dD per voxel (adjoint)
+    double acc;            // sum of dL/dD * E over the particle's deposits (adjoint)
     double hist_edep;
     uint32_t steps;
     uint32_t photon_steps, electron_steps, gathers, deposits;
@@ -234,11 +254,14 @@ struct Tally {
     __device__ __forceinline__ void score(uint32_t region, float energy) {
         if (region < VAC_AND_OTH) return;
         const double v = (double)energy;
-        if (dose) unsafeAtomicAdd(dose + (region - VAC_AND_OTH), v);
+        if (ADJ) acc += adj[region - VAC_AND_OTH] * v;
+        else if (dose) unsafeAtomicAdd(dose + (region - VAC_AND_OTH), v);
         hist_edep += v;
         deposits++;
     }
 };
+using Tally = TallyT<false>;
+using AdjTally = TallyT<true>;
 
 template <class Sink>
 __device__ __forceinline__ void forward(Sink &sink, Stream &s, Part &p, int status) {
@@ -300,7 +323,7 @@ __device__ __forceinline__ int locate(float p, uint32_t n, const float *b) {
 // distance to the next voxel face along the flight direction; returns the
 // region entered (0 = outside) and shortens t (geom.cu:54-307)
 __device__ __forceinline__ uint32_t howfar_phantom(const Env &env, const Part &p, float &t) {
-    const Local q = phantom_frame(env.f, p);
+    const Local q = phantom_frame(field_of(env, p), p);
     const uint32_t nx = env.nx, ny = env.ny, nz = env.nz, nxy = nx * ny;
     const float *xb = env.xb, *yb = env.yb, *zb = env.zb;
     if (p.region >= VAC_AND_OTH) {
@@ -372,7 +395,7 @@ __device__ __forceinline__ uint32_t howfar_phantom(const Env &env, const Part &p
 // perpendicular distance to the closest face; keeps the reference's
 // region-5 voxel offset (geom.cu:1487-1490), reading the neighbouring voxel
 __device__ __forceinline__ float hownear_phantom(const Env &env, const Part &p) {
-    const Local q = phantom_frame(env.f, p);
+    const Local q = phantom_frame(field_of(env, p), p);
     const uint32_t nx = env.nx, ny = env.ny, nz = env.nz;
     const float *xb = env.xb, *yb = env.yb, *zb = env.zb;
     if (p.region > 4) {
@@ -438,8 +461,8 @@ __device__ __forceinline__ void azimuth_box(Stream &s, float &cphi, float &sphi)
 __device__ __forceinline__ float to_kinetic(float etot) { return (float)((double)etot - PRM); }
 
 // ------------------------------------------------------------------ annihilation at rest
-template <class Sink>
-__device__ __forceinline__ void annih_at_rest(const Env &env, Stream &s, Part &p, const gegs_region &rd, Sink &sink, Tally &t) {
+template <class Sink, class T>
+__device__ __forceinline__ void annih_at_rest(const Env &env, Stream &s, Part &p, const gegs_region &rd, Sink &sink, T &t) {
     const int nsplit = env.mt->nbr_split;
     float edep = 0.0f;
     if (nsplit > 1) p.wt = p.wt / (float)nsplit;
@@ -515,7 +538,7 @@ __device__ __forceinline__ void photon_walk_begin(const Env &env, const Part &p,
     w.cmed = -1;
     w.lgle = 0;
     w.gmfpr0 = w.cohf = 0.0f;
-    const Local q = phantom_frame(env.f, p);
+    const Local q = phantom_frame(field_of(env, p), p);
     w.lu = q.u;
     w.lv = q.v;
     w.lw = q.w;
@@ -524,8 +547,8 @@ __device__ __forceinline__ void photon_walk_begin(const Env &env, const Part &p,
 }
 
 // one free-flight segment; true: the photon crossed a boundary and flies on
-template <class Sink>
-__device__ __forceinline__ bool photon_walk_iter(const Env &env, Stream &s, Part &p, PhotonWalk &w, Sink &sink, Tally &t) {
+template <class Sink, class T>
+__device__ __forceinline__ bool photon_walk_iter(const Env &env, Stream &s, Part &p, PhotonWalk &w, Sink &sink, T &t) {
     const gegs_media_tables *mt = env.mt;
     t.photon_steps++;
     t.steps++;
@@ -546,7 +569,7 @@ __device__ __forceinline__ bool photon_walk_iter(const Env &env, Stream &s, Part
     uint16_t cls_next = 0;
     if (inside) {
         float qx, qy, qz;
-        local_position(env.f, p, qx, qy, qz);
+        local_position(field_of(env, p), p, qx, qy, qz);
         const uint32_t nx = env.nx, nxy = env.nx * env.ny;
         if (w.lu > 0.0f) {
             dmin = (env.xb[w.ix + 1] - qx) / w.lu;
@@ -661,8 +684,8 @@ __device__ __forceinline__ void suspend(Sink &sink, Stream &s, Part &p, int stat
     sink.push(p);
 }
 
-template <class Sink>
-__device__ __forceinline__ void photon_step(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+template <class Sink, class T>
+__device__ __forceinline__ void photon_step(const Env &env, Stream &s, Part &p, Sink &sink, T &t) {
     PhotonWalk w;
     photon_walk_begin(env, p, w);
     while (photon_walk_iter(env, s, p, w, sink, t)) {
@@ -670,8 +693,8 @@ __device__ __forceinline__ void photon_step(const Env &env, Stream &s, Part &p, 
 }
 
 // ------------------------------------------------------------------ Compton (Klein-Nishina)
-template <class Sink>
-__device__ __forceinline__ void compton(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+template <class Sink, class T>
+__device__ __forceinline__ void compton(const Env &env, Stream &s, Part &p, Sink &sink, T &t) {
     const gegs_region rd = region_rec(env, p.region);
     float edep = 0.0f;
     Part e1 = p;
@@ -746,8 +769,8 @@ __device__ __forceinline__ void compton(const Env &env, Stream &s, Part &p, Sink
 }
 
 // ------------------------------------------------------------------ photo-electric
-template <class Sink>
-__device__ __forceinline__ void photo(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+template <class Sink, class T>
+__device__ __forceinline__ void photo(const Env &env, Stream &s, Part &p, Sink &sink, T &t) {
     const gegs_region rd = region_rec(env, p.region);
     float edep = 0.0f;
     p.e = (float)((double)p.e + PRM);
@@ -792,8 +815,8 @@ __device__ __forceinline__ void photo(const Env &env, Stream &s, Part &p, Sink &
 }
 
 // ------------------------------------------------------------------ Rayleigh
-template <class Sink>
-__device__ __forceinline__ void rayleigh(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+template <class Sink, class T>
+__device__ __forceinline__ void rayleigh(const Env &env, Stream &s, Part &p, Sink &sink, T &t) {
     const gegs_media_tables *mt = env.mt;
     const gegs_region rd = region_rec(env, p.region);
     const float gle = glog(p.e);
@@ -833,8 +856,8 @@ __device__ __forceinline__ void rayleigh(const Env &env, Stream &s, Part &p, Sin
 }
 
 // ------------------------------------------------------------------ pair production
-template <class Sink>
-__device__ __forceinline__ void pair(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+template <class Sink, class T>
+__device__ __forceinline__ void pair(const Env &env, Stream &s, Part &p, Sink &sink, T &t) {
     const gegs_media_tables *mt = env.mt;
     const gegs_region rd = region_rec(env, p.region);
     const int m8 = 8 * (int)rd.med;
@@ -1150,8 +1173,8 @@ struct EState {
     }
 };
 
-template <class Sink>
-__device__ __forceinline__ bool electron_iter(const Env &env, Stream &s, Part &p, EState &st, Sink &sink, Tally &t) {
+template <class Sink, class T>
+__device__ __forceinline__ bool electron_iter(const Env &env, Stream &s, Part &p, EState &st, Sink &sink, T &t) {
     // one condensed-history step; true: the electron continues stepping
     const gegs_media_tables *mt = env.mt;
     float &elke = st.elke, &demfp = st.demfp, &sig0 = st.sig0, &total_tstep = st.total_tstep;
@@ -1426,8 +1449,8 @@ __device__ __forceinline__ bool electron_iter(const Env &env, Stream &s, Part &p
     return false;
 }
 
-template <class Sink>
-__device__ __forceinline__ void electron_step(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+template <class Sink, class T>
+__device__ __forceinline__ void electron_step(const Env &env, Stream &s, Part &p, Sink &sink, T &t) {
     EState st;
     st.init();
     while (electron_iter(env, s, p, st, sink, t)) {
@@ -1435,8 +1458,8 @@ __device__ __forceinline__ void electron_step(const Env &env, Stream &s, Part &p
 }
 
 // ------------------------------------------------------------------ Moller
-template <class Sink>
-__device__ __forceinline__ void moller(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+template <class Sink, class T>
+__device__ __forceinline__ void moller(const Env &env, Stream &s, Part &p, Sink &sink, T &t) {
     const gegs_media_tables *mt = env.mt;
     const gegs_region rd = region_rec(env, p.region);
     const int m = rd.med;
@@ -1486,8 +1509,8 @@ __device__ __forceinline__ void moller(const Env &env, Stream &s, Part &p, Sink 
 }
 
 // ------------------------------------------------------------------ Bhabha
-template <class Sink>
-__device__ __forceinline__ void bhabha(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+template <class Sink, class T>
+__device__ __forceinline__ void bhabha(const Env &env, Stream &s, Part &p, Sink &sink, T &t) {
     const gegs_media_tables *mt = env.mt;
     const gegs_region rd = region_rec(env, p.region);
     float edep = 0.0f;
@@ -1554,8 +1577,8 @@ __device__ __forceinline__ void bhabha(const Env &env, Stream &s, Part &p, Sink 
 }
 
 // ------------------------------------------------------------------ bremsstrahlung (Bethe-Heitler + Koch-Motz 2BS)
-template <class Sink>
-__device__ __forceinline__ void brems(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+template <class Sink, class T>
+__device__ __forceinline__ void brems(const Env &env, Stream &s, Part &p, Sink &sink, T &t) {
     const gegs_media_tables *mt = env.mt;
     const gegs_region rd = region_rec(env, p.region);
     const int m = rd.med;
@@ -1655,8 +1678,8 @@ __device__ __forceinline__ void brems(const Env &env, Stream &s, Part &p, Sink &
 }
 
 // ------------------------------------------------------------------ in-flight annihilation (Heitler)
-template <class Sink>
-__device__ __forceinline__ void annih(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+template <class Sink, class T>
+__device__ __forceinline__ void annih(const Env &env, Stream &s, Part &p, Sink &sink, T &t) {
     const gegs_media_tables *mt = env.mt;
     const gegs_region rd = region_rec(env, p.region);
     const int nsplit = mt->nbr_split;
@@ -1717,8 +1740,8 @@ __device__ __forceinline__ void annih(const Env &env, Stream &s, Part &p, Sink &
 }
 
 // dispatch one particle to its event handler
-template <class Sink>
-__device__ __forceinline__ void handle(const Env &env, Stream &s, Part &p, Sink &sink, Tally &t) {
+template <class Sink, class T>
+__device__ __forceinline__ void handle(const Env &env, Stream &s, Part &p, Sink &sink, T &t) {
     switch (p.status) {
     case P_PHOTON_STEP: photon_step(env, s, p, sink, t); break;
     case P_COMPTON: compton(env, s, p, sink, t); break;

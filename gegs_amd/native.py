@@ -21,7 +21,7 @@ SYMBOLS = [
     "gegs_load_media", "gegs_load_phantom", "gegs_set_field", "gegs_set_source",
     "gegs_run_plain", "gegs_run_event", "gegs_set_history_tally", "gegs_history_tally",
     "gegs_get_counters", "gegs_last_kernel_ms", "gegs_sample_events",
-    "gegs_set_tuning", "gegs_get_tuning",
+    "gegs_set_tuning", "gegs_get_tuning", "gegs_run_batched", "gegs_set_beamlets", "gegs_adjoint",
 ]
 
 PARTICLE_DTYPE = np.dtype([("e", "<f4"), ("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("u", "<f4"),
@@ -52,6 +52,13 @@ class Tuning(ctypes.Structure):
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k in self.FIELDS}
+
+
+class Job(ctypes.Structure):
+    """gegs_job: one field of a plan for gegs_run_batched."""
+    _fields_ = [("field", ctypes.c_byte * 32), ("plan", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("first", ctypes.c_uint64), ("count", ctypes.c_uint64),
+                ("history_offset", ctypes.c_uint64)]
 
 
 class Counters(ctypes.Structure):
@@ -98,6 +105,9 @@ def load():
     L.gegs_last_kernel_ms.argtypes = [vp, vp]
     L.gegs_sample_events.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint64, vp, ctypes.c_uint32, vp, vp]
     L.gegs_set_tuning.argtypes = [vp, vp]
+    L.gegs_run_batched.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp]
+    L.gegs_set_beamlets.argtypes = [vp, vp, ctypes.c_uint64, vp, ctypes.c_uint32]
+    L.gegs_adjoint.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp]
     L.gegs_get_tuning.argtypes = [vp, vp]
     for name in SYMBOLS:
         if getattr(L, name, None) is None:
@@ -187,6 +197,70 @@ class Context:
         edep = np.zeros(n, dtype=np.float64)
         self._check(self.L.gegs_sample_events(self.h, _p(inp), n, seed, _p(out), max_out, _p(n_out), _p(edep)))
         return out.reshape(n, max_out), n_out, edep
+
+    @staticmethod
+    def _jobs(jobs):
+        arr = (Job * len(jobs))()
+        for j, jb in enumerate(jobs):
+            ctypes.memmove(ctypes.addressof(arr[j].field), ctypes.addressof(jb["field"]), 32)
+            arr[j].plan = int(jb["plan"])
+            arr[j].first = int(jb["first"])
+            arr[j].count = int(jb["count"])
+            arr[j].history_offset = int(jb.get("history_offset", jb["first"]))
+        return arr
+
+    def set_beamlets(self, beamlet=None, weight=None):
+        """Beamlet index per history slot (job after job) and weight per
+        beamlet; None removes the map."""
+        if beamlet is None:
+            self._check(self.L.gegs_set_beamlets(self.h, None, 0, None, 0))
+            return
+        b = np.ascontiguousarray(beamlet, dtype=np.uint32)
+        w = np.ascontiguousarray(weight, dtype=np.float32)
+        self._check(self.L.gegs_set_beamlets(self.h, _p(b), len(b), _p(w), len(w)))
+
+    def adjoint(self, jobs, n_plans, seed, dLdD, n_beamlets, n_batches=10):
+        """Gradient of L with respect to the beamlet weights, given dL/dD of
+        shape (n_plans, nvox) (replay of run_batched with the same seed)."""
+        g = np.ascontiguousarray(dLdD, dtype=np.float64).reshape(-1)
+        if len(g) != n_plans * self.nvox:
+            raise ValueError("dLdD must have n_plans * nvox values")
+        grad = np.zeros(n_beamlets, dtype=np.float64)
+        opt = RunOptions(seed, 0, n_batches, 0, 0, 0, 0, 0)
+        self._check(self.L.gegs_adjoint(self.h, self._jobs(jobs), len(jobs), n_plans, ctypes.byref(opt),
+                                        _p(g), _p(grad)))
+        return grad
+
+    def run_batched(self, jobs, n_plans, seed, n_batches=10, per_history=False):
+        """Event-queued transport of several fields/plans in one loop.
+
+        jobs: sequence of dicts with keys field (workloads.Field), plan,
+        first, count and optionally history_offset (default: first).
+        Returns dose and dose2 of shape (n_plans, nvox)."""
+        arr = self._jobs(jobs)
+        self._check(self.L.gegs_set_history_tally(self.h, 1 if per_history else 0))
+        opt = RunOptions(seed, 0, n_batches, 0, 0, 0, 0, 0)
+        dose = np.zeros(n_plans * self.nvox, dtype=np.float64)
+        dose2 = np.zeros(n_plans * self.nvox, dtype=np.float64)
+        self._check(self.L.gegs_run_batched(self.h, arr, len(jobs), n_plans, ctypes.byref(opt), _p(dose), _p(dose2)))
+        out = dict(dose=dose.reshape(n_plans, self.nvox), dose2=dose2.reshape(n_plans, self.nvox), n_batches=n_batches)
+        ms = ctypes.c_float()
+        self._check(self.L.gegs_last_kernel_ms(self.h, ctypes.byref(ms)))
+        out["ms"] = ms.value
+        cnt = Counters()
+        self._check(self.L.gegs_get_counters(self.h, ctypes.byref(cnt)))
+        out["counters"] = cnt.as_dict()
+        if per_history:
+            n = sum(int(jb["count"]) for jb in jobs)
+            edep = np.zeros(n, dtype=np.float64)
+            steps = np.zeros(n, dtype=np.uint32)
+            self._check(self.L.gegs_history_tally(self.h, _p(edep), _p(steps)))
+            out["edep"] = edep
+            out["steps"] = steps
+        return out
+
+    def set_field(self, field):
+        self._check(self.L.gegs_set_field(self.h, ctypes.c_void_p(ctypes.addressof(field))))
 
     def run(self, seed, n_batches=10, history_offset=0, method="plain", per_history=False,
             grid=0, block=0, depth=0, first=0, count=0):

@@ -67,6 +67,20 @@ typedef struct gegs_field {
     float iso_x, iso_y, iso_z;   /* isocentre in phantom coordinates */
 } gegs_field;
 
+/* One field of a plan for gegs_run_batched: its beam geometry, the plan whose
+ * dose buffer receives its deposits, and the source records it transports.
+ * The reference runs one field per runSimulation call (CUDA_EGS.cu:367-900)
+ * with the gantry published per field (CUDA_EGS.cu:430-432); here all fields
+ * of all plans share one event loop. */
+typedef struct gegs_job {
+    gegs_field field;            /* beam geometry of the field */
+    uint32_t plan;               /* dose buffer: dose[plan * nvox + voxel] */
+    uint32_t reserved;
+    uint64_t first;              /* first source record */
+    uint64_t count;              /* records = histories of the field */
+    uint64_t history_offset;     /* Philox history index of record `first` */
+} gegs_job;
+
 /* Options of gegs_run_plain. */
 typedef struct gegs_run_options {
     uint64_t seed;               /* Philox-4x32-10 key */
@@ -146,6 +160,33 @@ int gegs_run_plain(gegs_ctx *ctx, const gegs_run_options *opt, double *dose, dou
  * one kernel per event type, wavefront-compacted queues. */
 int gegs_run_event(gegs_ctx *ctx, const gegs_run_options *opt, double *dose, double *dose2);
 
+/* Event-queued transport of several fields and plans in one loop (n_jobs <=
+ * 512).  Every batch holds the b-th of opt->n_batches equal slices of every
+ * job, so batch statistics hold per plan.  dose and dose2 receive n_plans *
+ * nvox values (plan-major).  Of opt only seed and n_batches are used.
+ * Per-history diagnostics, if enabled, are indexed job after job. */
+int gegs_run_batched(gegs_ctx *ctx, const gegs_job *jobs, uint32_t n_jobs, uint32_t n_plans,
+                     const gegs_run_options *opt, double *dose, double *dose2);
+
+/* Beamlet map for gegs_run_batched and gegs_adjoint: history slot k (job
+ * after job, as in gegs_run_batched) belongs to beamlet beamlet[k] <
+ * n_beamlets, and gegs_run_batched multiplies its source weight by
+ * weight[beamlet[k]].  beamlet == NULL removes the map.  No counterpart in
+ * the reference, which has no differentiable path. */
+int gegs_set_beamlets(gegs_ctx *ctx, const uint32_t *beamlet, uint64_t n_slots, const float *weight,
+                      uint32_t n_beamlets);
+
+/* Gradient of an objective L(D) with respect to the beamlet weights.  The
+ * histories of gegs_run_batched (same jobs, same opt->seed) are replayed from
+ * their counter-based Philox streams at unit beamlet weight; every deposit E
+ * in voxel v of plan p adds dLdD[p * nvox + v] * E to grad[beamlet].  The
+ * transport is linear in the weights (they multiply deposits and do not
+ * enter any sampling decision), so this is the exact gradient of the
+ * estimator for those random numbers.  dLdD: n_plans * nvox values; grad:
+ * n_beamlets values (FP64). */
+int gegs_adjoint(gegs_ctx *ctx, const gegs_job *jobs, uint32_t n_jobs, uint32_t n_plans,
+                 const gegs_run_options *opt, const double *dLdD, double *grad);
+
 /* Tuning of gegs_run_event; NULL restores the defaults.  gegs_get_tuning
  * returns the values in force with defaults filled in (grid sizes that fill
  * the device are resolved at run time and reported as 0). */
@@ -154,7 +195,8 @@ int gegs_get_tuning(gegs_ctx *ctx, gegs_tuning *tuning);
 
 /* Per-history diagnostics of the last run (total energy deposited in the
  * phantom per history, FP64; number of transport steps per history), sized
- * by the number of source records; NULL pointers are skipped.  Available
+ * by the number of source records (gegs_run_plain, gegs_run_event) or the
+ * total histories of the jobs (gegs_run_batched); NULL pointers are skipped.  Available
  * only when the run was made with gegs_set_history_tally(ctx, 1). */
 int gegs_set_history_tally(gegs_ctx *ctx, int enable);
 int gegs_history_tally(gegs_ctx *ctx, double *edep, uint32_t *steps);

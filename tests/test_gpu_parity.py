@@ -172,3 +172,42 @@ def test_tuning_roundtrip_and_validation(gpu_ctx):
     assert e.value.code == 2001
     gpu_ctx.set_tuning(None)
     assert gpu_ctx.get_tuning()["photon_chunk"] == 32
+
+
+def test_batched_fields_and_plans_match_per_field_runs(gpu_ctx, ct_small):
+    """Three fields of two plans in one event loop give, history by history,
+    the trajectories of separate per-field runs of the plain kernel, and each
+    plan's dose is the sum of its fields' doses."""
+    import copy
+    from gegs_amd import workloads
+    wl = ct_small
+    src = wl.source(6000, 41)
+    gpu_ctx.load_workload(wl)
+    gpu_ctx.set_source(src)
+    sid = wl.field.source_to_isocenter
+    fields = [workloads.make_field(True, sid, g) for g in (0.0, 90.0, 200.0)]
+    jobs = [dict(field=fields[0], plan=0, first=0, count=3000),
+            dict(field=fields[1], plan=1, first=3000, count=2000),
+            dict(field=fields[2], plan=0, first=5000, count=1000)]
+    bt = gpu_ctx.run_batched(jobs, 2, seed=11, n_batches=3, per_history=True)
+    assert bt["counters"]["overflow"] == 0
+    ref_dose = np.zeros((2, gpu_ctx.nvox))
+    slot = 0
+    try:
+        for jb in jobs:
+            gpu_ctx.set_field(jb["field"])
+            pl = gpu_ctx.run(seed=11, n_batches=2, method="plain", per_history=True,
+                             first=jb["first"], count=jb["count"])
+            sl = slice(jb["first"], jb["first"] + jb["count"])
+            assert np.array_equal(bt["steps"][slot:slot + jb["count"]], pl["steps"][sl])
+            assert np.allclose(bt["edep"][slot:slot + jb["count"]], pl["edep"][sl], rtol=1e-12, atol=1e-12)
+            ref_dose[jb["plan"]] += pl["dose"]
+            slot += jb["count"]
+    finally:
+        gpu_ctx.set_field(wl.field)
+    assert np.allclose(bt["dose"], ref_dose, rtol=1e-11, atol=1e-12)
+    # the rotated field against the oracle
+    wl90 = copy.copy(wl)
+    wl90.field = fields[1]
+    o = oracle.run(wl90, src[3000:3500], seed=11, hist_offset=3000, per_history=True)
+    assert np.array_equal(bt["steps"][3000:3500], o["steps"])
