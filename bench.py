@@ -6,11 +6,15 @@ voxels at 2.5 mm (air / lung / ICRU tissue / ICRP bone through the ctcreate
 ramp), point source at SSD 100 cm, 10x10 cm^2 field, 24-bin 6 MV-like
 spectrum; 1e7 histories by default (10 steps of 1e6).
 
-A step transports one batch of source records that are already resident in
-HBM.  Reported: histories/s of the whole job, dose efficiency 1/(sigma^2 T)
-(sigma = mean relative uncertainty over voxels above 50% of Dmax, from the
-step-to-step variance), the HBM roofline of the transport kernel, and the CPU
-oracle timed on this host's cores on a bounded sample.
+A step transports one batch of 1e6 source records already resident in HBM,
+once with the plain per-history kernel (the baseline method) and once with
+the batched event-queued loop (the headline).  Reported: histories/s of the
+whole job, dose efficiency 1/(sigma^2 T) (sigma = mean relative uncertainty
+over voxels above 50% of Dmax, from the step-to-step variance), the HBM
+roofline of the event loop's dominant kernel (the photon flight kernel,
+timed live with HIP events), the adjoint gradient rate of a 5-field,
+2000-beamlet plan, and the CPU oracle timed on this host's cores on a
+bounded sample.
 
   python bench.py [--gpus 1] [--steps 10] [--warmup 1] [--histories-per-step 1000000]
 """
@@ -36,6 +40,9 @@ DEPOSIT_BYTES = 16         # FP64 read-modify-write
 # sig, dedx x2, tmxs, e_array/range rows)
 ROWS_PER_PHOTON_STEP = 3
 ROWS_PER_ELECTRON_STEP = 6
+# photon flight kernel: the region record of the voxel and the three photon
+# rows, per photon step
+PHOTON_STEP_BYTES = REGION_BYTES + ROWS_PER_PHOTON_STEP * TABLE_ROW_BYTES
 
 
 def algorithmic_bytes(counters, n_hist):
@@ -70,21 +77,68 @@ def cpu_baseline(wl, budget_s, threads):
                        "(C restatement, gcc -O2, OpenMP over histories), %.1f s" % (n, dt))
 
 
+def _sync():
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except ImportError:
+        pass
+
+
+def add_counters(acc, c):
+    if acc is None:
+        return dict(c)
+    return {k: (acc[k] + c[k] if not isinstance(c[k], list) else [a + b for a, b in zip(acc[k], c[k])])
+            for k in acc}
+
+
 def run_method(ctx, method, steps, warmup, b, seed):
     for w in range(warmup):
-        ctx.run(seed=seed, n_batches=2, method=method, first=w * b, count=b, history_offset=0)
-    step_doses, kernel_ms, counters = [], [], None
+        ctx.run(seed=seed, n_batches=1, method=method, first=w * b, count=b)
+    if method == "event":
+        ctx.set_kernel_timing(True)
+    step_doses, counters = [], None
+    _sync()
     t0 = time.perf_counter()
     for k in range(steps):
-        r = ctx.run(seed=seed, n_batches=2, method=method, first=(warmup + k) * b, count=b, history_offset=0)
+        r = ctx.run(seed=seed, n_batches=1, method=method, first=(warmup + k) * b, count=b)
         step_doses.append(r["dose"])
-        kernel_ms.append(r["ms"])
-        counters = r["counters"] if counters is None else {
-            key: (counters[key] + r["counters"][key] if not isinstance(r["counters"][key], list)
-                  else [a + c for a, c in zip(counters[key], r["counters"][key])])
-            for key in counters}
+        counters = add_counters(counters, r["counters"])
+    _sync()
     elapsed = time.perf_counter() - t0
-    return elapsed, step_doses, kernel_ms, counters
+    timing = None
+    if method == "event":
+        timing = ctx.kernel_timing()
+        ctx.set_kernel_timing(False)
+    return elapsed, step_doses, counters, timing
+
+
+def adjoint_rate(wl, n_hist, seed, repeat=2):
+    """Gradients/s of the beamlet adjoint: 5 fields (gantry 0/72/144/216/288)
+    x 20x20 beamlets of 5 mm, one gradient = one replay of n_hist histories."""
+    from gegs_amd import beamlets, native, workloads
+    fields = [workloads.make_field(True, wl.field.source_to_isocenter, g) for g in (0, 72, 144, 216, 288)]
+    per = n_hist // len(fields)
+    src = wl.source(per * len(fields), 777)
+    ctx = native.Context(0)
+    ctx.load_workload(wl)
+    ctx.set_source(src)
+    plan = beamlets.BeamletPlan(ctx, fields, per, n_side=20, pitch=0.5, seed=seed, n_batches=2, src=src)
+    w = np.random.default_rng(4).uniform(0.5, 1.5, plan.n_beamlets)
+    d = plan.forward(w)
+    dLdD = 2.0 * (d - d.mean()) * (d > 0.5 * d.max())
+    plan.adjoint(dLdD)                      # graph capture / warm caches
+    best = None
+    for _ in range(repeat):
+        _sync()
+        t0 = time.perf_counter()
+        g = plan.adjoint(dLdD)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    ctx.close()
+    return dict(gradients_per_s=1.0 / best, ms_per_gradient=best * 1e3, histories_per_gradient=per * len(fields),
+                beamlets=plan.n_beamlets, fields=len(fields), nonzero=int(np.count_nonzero(g)))
 
 
 def main():
@@ -93,9 +147,12 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--histories-per-step", type=int, default=1_000_000)
-    ap.add_argument("--methods", default="plain", help="comma list of plain,event")
+    ap.add_argument("--methods", default="plain,event", help="comma list of plain,event")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--adjoint-histories", type=int, default=10_000_000,
+                    help="histories per adjoint gradient (0: skip)")
     ap.add_argument("--seed", type=int, default=20240)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (counter calibration)")
@@ -114,7 +171,7 @@ def main():
               "(histories shard across GPUs with one FP64 dose reduction; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
-    from gegs_amd import native, workloads
+    from gegs_amd import autotune, native, workloads
     wl = workloads.ct()
     b = args.histories_per_step
     n_total = (args.steps + args.warmup) * b
@@ -123,26 +180,47 @@ def main():
     ctx.load_workload(wl)
     ctx.set_source(src)          # resident in HBM before any timed step
 
+    methods = [m.strip() for m in args.methods.split(",") if m.strip()]
+    tuning = None
+    if "event" in methods and not args.no_autotune:
+        # tuned on the warmup records, before the timed region
+        tuning, _, _ = autotune.autotune(ctx, 0, min(b, 500_000), seed=args.seed)
     results = {}
-    for method in [m.strip() for m in args.methods.split(",") if m.strip()]:
-        elapsed, doses, kms, cnt = run_method(ctx, method, args.steps, args.warmup, b, args.seed)
+    for method in methods:
+        elapsed, doses, cnt, timing = run_method(ctx, method, args.steps, args.warmup, b, args.seed)
         hist = args.steps * b
         sig, nsel = mean_rel_sigma_from_steps(doses)
         eff = 1.0 / (sig * sig * elapsed)
-        launches = 2 * args.steps
-        kernel_s = sum(kms) / 1e3
-        abytes = algorithmic_bytes(cnt, hist)
-        achieved = abytes / launches / (kernel_s / launches) / 1e9
-        results[method] = dict(value=hist / elapsed, ms_per_step=elapsed / args.steps * 1e3,
-                               sigma=sig, voxels=nsel, efficiency=eff, kernel_ms=kernel_s * 1e3,
-                               counters=cnt, achieved=achieved, bytes=abytes)
+        res = dict(value=hist / elapsed, ms_per_step=elapsed / args.steps * 1e3, sigma=sig, voxels=nsel,
+                   efficiency=eff, counters=cnt, bytes=algorithmic_bytes(cnt, hist))
+        if method == "event" and timing and timing[1]:
+            photon_ms, launches = timing
+            pbytes = PHOTON_STEP_BYTES * cnt["photon_steps"]
+            res["roofline"] = dict(kernel="photon_kernel", launches=launches,
+                                   avg_launch_us=photon_ms / launches * 1e3,
+                                   algorithmic_bytes_per_launch=pbytes / launches,
+                                   achieved=pbytes / (photon_ms * 1e-3) / 1e9)
+        results[method] = res
 
-    best = max(results, key=lambda m: results[m]["efficiency"])
-    r = results[best]
+    head = "event" if "event" in results else methods[0]
+    r = results[head]
     cpu = None
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         cpu = cpu_baseline(wl, args.cpu_seconds, threads)
+    adj = None
+    if args.adjoint_histories:
+        ctx.close()
+        adj = adjoint_rate(wl, args.adjoint_histories, args.seed)
+    if "roofline" in r:
+        rf = r["roofline"]
+        roof = {"bound": "hbm", "achieved": rf["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": rf["achieved"] / HBM_PEAK_GBS, "traffic": None, "kernel": rf["kernel"],
+                "avg_launch_us": rf["avg_launch_us"], "launches": rf["launches"],
+                "algorithmic_bytes_per_launch": rf["algorithmic_bytes_per_launch"],
+                "algorithmic_bytes_per_photon_step": PHOTON_STEP_BYTES}
+    else:
+        roof = None
     line = {
         "metric": "histories/s + dose eff. 1/(sigma^2 T) at 1% unc., batched vs plain; adjoint grad/s",
         "value": r["value"],
@@ -160,7 +238,7 @@ def main():
             "workload": "ct_128x128x100_2.5mm_6MV_point_source_SSD100_10x10",
             "histories_per_step": b,
             "histories": args.steps * b,
-            "method": best,
+            "method": "event-queued, batched" if head == "event" else head,
             "ecut_mev": 0.7, "pcut_mev": 0.01,
             "tallies": "fp64",
         },
@@ -169,17 +247,12 @@ def main():
         "mean_rel_sigma": r["sigma"],
         "histories_for_1pct": args.steps * b * (r["sigma"] / 0.01) ** 2,
         "methods": {m: dict(histories_per_s=v["value"], dose_efficiency=v["efficiency"],
-                            ms_per_step=v["ms_per_step"]) for m, v in results.items()},
-        "roofline": {
-            "bound": "hbm",
-            "achieved": r["achieved"],
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": r["achieved"] / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": "plain_kernel" if best == "plain" else "event kernels",
-            "algorithmic_bytes_per_history": r["bytes"] / (args.steps * b),
-        },
+                            ms_per_step=v["ms_per_step"],
+                            algorithmic_bytes_per_history=v["bytes"] / (args.steps * b))
+                    for m, v in results.items()},
+        "tuning": tuning,
+        "roofline": roof,
+        "adjoint": adj,
         "cpu_baseline": cpu,
     }
     if "plain" in results and "event" in results:

@@ -116,48 +116,59 @@ __device__ __forceinline__ float *bank_word(const EvArgs &a, int slot, int w, ui
     return a.bank + ((size_t)slot * EVW + w) * a.cap + idx;
 }
 
+// Bank records stream through once per event hop: non-temporal loads and
+// stores keep them from evicting the voxel class volume and the tables that
+// every crossing reads from L2.
+__device__ __forceinline__ void bank_put(const EvArgs &a, int slot, int w, uint32_t idx, float v) {
+    __builtin_nontemporal_store(v, bank_word(a, slot, w, idx));
+}
+
+__device__ __forceinline__ float bank_get(const EvArgs &a, int slot, int w, uint32_t idx) {
+    return __builtin_nontemporal_load(bank_word(a, slot, w, idx));
+}
+
 __device__ __forceinline__ void bank_store(const EvArgs &a, int slot, uint32_t idx, const Part &p, uint32_t hist) {
-    *bank_word(a, slot, 0, idx) = p.e;
-    *bank_word(a, slot, 1, idx) = p.x;
-    *bank_word(a, slot, 2, idx) = p.y;
-    *bank_word(a, slot, 3, idx) = p.z;
-    *bank_word(a, slot, 4, idx) = p.u;
-    *bank_word(a, slot, 5, idx) = p.v;
-    *bank_word(a, slot, 6, idx) = p.w;
-    *bank_word(a, slot, 7, idx) = p.wt;
-    *bank_word(a, slot, 8, idx) = p.dnear;
-    *bank_word(a, slot, 9, idx) = __uint_as_float(p.region);
-    *bank_word(a, slot, 10, idx) =
-        __uint_as_float((uint32_t)(p.charge + 1) | ((uint32_t)p.spos << 8) | ((uint32_t)p.module << 16));
-    *bank_word(a, slot, 11, idx) = __uint_as_float((uint32_t)p.lin);
-    *bank_word(a, slot, 12, idx) = __uint_as_float((uint32_t)(p.lin >> 32));
-    *bank_word(a, slot, 13, idx) = __uint_as_float(p.blk);
-    *bank_word(a, slot, 14, idx) = __uint_as_float(hist);
-    *bank_word(a, slot, 20, idx) = __uint_as_float(p.job);
+    bank_put(a, slot, 0, idx, p.e);
+    bank_put(a, slot, 1, idx, p.x);
+    bank_put(a, slot, 2, idx, p.y);
+    bank_put(a, slot, 3, idx, p.z);
+    bank_put(a, slot, 4, idx, p.u);
+    bank_put(a, slot, 5, idx, p.v);
+    bank_put(a, slot, 6, idx, p.w);
+    bank_put(a, slot, 7, idx, p.wt);
+    bank_put(a, slot, 8, idx, p.dnear);
+    bank_put(a, slot, 9, idx, __uint_as_float(p.region));
+    bank_put(a, slot, 10, idx,
+             __uint_as_float((uint32_t)(p.charge + 1) | ((uint32_t)p.spos << 8) | ((uint32_t)p.module << 16)));
+    bank_put(a, slot, 11, idx, __uint_as_float((uint32_t)p.lin));
+    bank_put(a, slot, 12, idx, __uint_as_float((uint32_t)(p.lin >> 32)));
+    bank_put(a, slot, 13, idx, __uint_as_float(p.blk));
+    bank_put(a, slot, 14, idx, __uint_as_float(hist));
+    bank_put(a, slot, 20, idx, __uint_as_float(p.job));
 }
 
 __device__ __forceinline__ Part bank_load(const EvArgs &a, int slot, int event, uint32_t idx, uint32_t &hist) {
     Part p;
-    p.e = *bank_word(a, slot, 0, idx);
-    p.x = *bank_word(a, slot, 1, idx);
-    p.y = *bank_word(a, slot, 2, idx);
-    p.z = *bank_word(a, slot, 3, idx);
-    p.u = *bank_word(a, slot, 4, idx);
-    p.v = *bank_word(a, slot, 5, idx);
-    p.w = *bank_word(a, slot, 6, idx);
-    p.wt = *bank_word(a, slot, 7, idx);
-    p.dnear = *bank_word(a, slot, 8, idx);
-    p.region = __float_as_uint(*bank_word(a, slot, 9, idx));
-    const uint32_t pk = __float_as_uint(*bank_word(a, slot, 10, idx));
+    p.e = bank_get(a, slot, 0, idx);
+    p.x = bank_get(a, slot, 1, idx);
+    p.y = bank_get(a, slot, 2, idx);
+    p.z = bank_get(a, slot, 3, idx);
+    p.u = bank_get(a, slot, 4, idx);
+    p.v = bank_get(a, slot, 5, idx);
+    p.w = bank_get(a, slot, 6, idx);
+    p.wt = bank_get(a, slot, 7, idx);
+    p.dnear = bank_get(a, slot, 8, idx);
+    p.region = __float_as_uint(bank_get(a, slot, 9, idx));
+    const uint32_t pk = __float_as_uint(bank_get(a, slot, 10, idx));
     p.charge = (int32_t)(pk & 0xffu) - 1;
     p.spos = (int32_t)((pk >> 8) & 15u);
     p.module = (int32_t)(pk >> 16);
     p.status = event;
-    p.lin = (uint64_t)__float_as_uint(*bank_word(a, slot, 11, idx)) |
-            ((uint64_t)__float_as_uint(*bank_word(a, slot, 12, idx)) << 32);
-    p.blk = __float_as_uint(*bank_word(a, slot, 13, idx));
-    hist = __float_as_uint(*bank_word(a, slot, 14, idx));
-    p.job = __float_as_uint(*bank_word(a, slot, 20, idx));
+    p.lin = (uint64_t)__float_as_uint(bank_get(a, slot, 11, idx)) |
+            ((uint64_t)__float_as_uint(bank_get(a, slot, 12, idx)) << 32);
+    p.blk = __float_as_uint(bank_get(a, slot, 13, idx));
+    hist = __float_as_uint(bank_get(a, slot, 14, idx));
+    p.job = __float_as_uint(bank_get(a, slot, 20, idx));
     return p;
 }
 
@@ -210,12 +221,13 @@ struct BankSink {
         const uint32_t idx = base + rank;
         if (idx < a->cap) {
             bank_store(*a, dest, idx, p, hist);
-            *bank_word(*a, dest, 15, idx) = es.elke;
-            *bank_word(*a, dest, 16, idx) = es.demfp;
-            *bank_word(*a, dest, 17, idx) = es.sig0;
-            *bank_word(*a, dest, 18, idx) = es.total_tstep;
-            *bank_word(*a, dest, 19, idx) = __uint_as_float((uint32_t)es.lelke | ((uint32_t)es.compute_tstep << 16) |
-                                                            ((uint32_t)es.is_tstep << 17));
+            bank_put(*a, dest, 15, idx, es.elke);
+            bank_put(*a, dest, 16, idx, es.demfp);
+            bank_put(*a, dest, 17, idx, es.sig0);
+            bank_put(*a, dest, 18, idx, es.total_tstep);
+            bank_put(*a, dest, 19, idx,
+                     __uint_as_float((uint32_t)es.lelke | ((uint32_t)es.compute_tstep << 16) |
+                                     ((uint32_t)es.is_tstep << 17)));
         } else {
             atomicAdd(&a->st->overflow, 1u);
         }
@@ -223,11 +235,11 @@ struct BankSink {
 };
 
 __device__ __forceinline__ void load_estate(const EvArgs &a, int slot, uint32_t idx, EState &es) {
-    es.elke = *bank_word(a, slot, 15, idx);
-    es.demfp = *bank_word(a, slot, 16, idx);
-    es.sig0 = *bank_word(a, slot, 17, idx);
-    es.total_tstep = *bank_word(a, slot, 18, idx);
-    const uint32_t f = __float_as_uint(*bank_word(a, slot, 19, idx));
+    es.elke = bank_get(a, slot, 15, idx);
+    es.demfp = bank_get(a, slot, 16, idx);
+    es.sig0 = bank_get(a, slot, 17, idx);
+    es.total_tstep = bank_get(a, slot, 18, idx);
+    const uint32_t f = __float_as_uint(bank_get(a, slot, 19, idx));
     es.lelke = (int)(f & 0xffffu);
     es.compute_tstep = (f >> 16) & 1u;
     es.is_tstep = (f >> 17) & 1u;
@@ -394,6 +406,7 @@ __global__ void __launch_bounds__(512) photon_kernel(EvArgs a, uint32_t chunk, u
             pops += (p.spos & 8) ? 0u : 1u;    // flights, not resumptions
         }
         if (!__any(have)) break;
+        wave_lookahead(s, have);
         if (have) {
             sink.hist = hist;
             if (chunk && iters == chunk) {
@@ -447,6 +460,7 @@ __global__ void __launch_bounds__(512) electron_kernel(EvArgs a, uint32_t chunk,
             iters = 0;
         }
         if (!__any(have)) break;
+        wave_lookahead(s, have);
         if (have) {
             sink.hist = hist;
             if (chunk && iters == chunk) {
@@ -576,9 +590,17 @@ struct EventBanks {
     hipEvent_t done[2] = {nullptr, nullptr};   // completion of the two replays in flight
     BatchPlan *d_plan = nullptr;     // injection plan of the current batch
     BatchPlan *h_plan = nullptr;     // pinned staging copy
-    hipGraphExec_t graph = nullptr;
+    // one graph exec per replay slot; with kernel timing each holds its own
+    // event pairs around the photon kernel launches
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
     EvArgs graph_args{};
     EventConfig graph_cfg{};
+    bool graph_timed = false;
+    static constexpr int MAX_TIMED = 64;
+    hipEvent_t tev[2][2 * MAX_TIMED] = {};
+    bool timing = false;             // measure the photon kernel with HIP events
+    double photon_ms = 0.0;          // accumulated over the runs since the last reset
+    uint64_t photon_launches = 0;
 
     hipError_t reserve(uint64_t, uint32_t, const EventConfig &cfg) {
         const size_t want = cfg.capacity ? cfg.capacity : ((size_t)1 << 22);
@@ -610,11 +632,25 @@ struct EventBanks {
         return hipSuccess;
     }
     void drop_graph() {
-        if (graph) (void)hipGraphExecDestroy(graph);
-        graph = nullptr;
+        for (int k = 0; k < 2; ++k) {
+            if (graph[k]) (void)hipGraphExecDestroy(graph[k]);
+            graph[k] = nullptr;
+        }
+    }
+    hipError_t make_timing_events() {
+        for (int k = 0; k < 2; ++k)
+            for (int i = 0; i < 2 * MAX_TIMED; ++i)
+                if (!tev[k][i]) {
+                    const hipError_t e = hipEventCreate(&tev[k][i]);
+                    if (e != hipSuccess) return e;
+                }
+        return hipSuccess;
     }
     void release() {
         drop_graph();
+        for (int k = 0; k < 2; ++k)
+            for (int i = 0; i < 2 * MAX_TIMED; ++i)
+                if (tev[k][i]) (void)hipEventDestroy(tev[k][i]);
         if (bank) (void)hipFree(bank);
         if (st) (void)hipFree(st);
         if (h_flags) (void)hipHostFree(h_flags);
@@ -650,13 +686,16 @@ inline size_t flight_lds(const Env &env) {
 }
 
 template <bool TAB, bool ADJ>
-inline void launch_sweep_t(const EvArgs &a, const EventConfig &cfg, hipStream_t stream) {
+inline void launch_sweep_t(const EvArgs &a, const EventConfig &cfg, hipStream_t stream, hipEvent_t t0,
+                           hipEvent_t t1) {
     const dim3 g(cfg.grid), b(cfg.block), fb(cfg.flight_block);
     const size_t lds = flight_lds(a.env);
     hipLaunchKernelGGL(inject_plan, dim3(1), dim3(1), 0, stream, a.st, cfg.threshold, cfg.inject);
     hipLaunchKernelGGL(inject_fill, g, b, 0, stream, a);
+    if (t0) (void)hipEventRecord(t0, stream);
     hipLaunchKernelGGL((photon_kernel<TAB, ADJ>), dim3(cfg.photon_grid), fb, lds, stream, a, cfg.photon_chunk,
                        cfg.refill_min);
+    if (t1) (void)hipEventRecord(t1, stream);
     hipLaunchKernelGGL((interaction_kernel<P_RAYLEIGH, ADJ>), g, b, 0, stream, a);
     hipLaunchKernelGGL((interaction_kernel<P_COMPTON, ADJ>), g, b, 0, stream, a);
     hipLaunchKernelGGL((interaction_kernel<P_PHOTO, ADJ>), g, b, 0, stream, a);
@@ -670,14 +709,15 @@ inline void launch_sweep_t(const EvArgs &a, const EventConfig &cfg, hipStream_t 
     hipLaunchKernelGGL(sweep_end, dim3(1), dim3(1), 0, stream, a.st);
 }
 
-inline void launch_sweep(const EvArgs &a, const EventConfig &cfg, hipStream_t stream) {
+inline void launch_sweep(const EvArgs &a, const EventConfig &cfg, hipStream_t stream, hipEvent_t t0 = nullptr,
+                         hipEvent_t t1 = nullptr) {
     const bool adj = a.adj != nullptr;
     if (tab_in_lds(a.env)) {
-        if (adj) launch_sweep_t<true, true>(a, cfg, stream);
-        else launch_sweep_t<true, false>(a, cfg, stream);
+        if (adj) launch_sweep_t<true, true>(a, cfg, stream, t0, t1);
+        else launch_sweep_t<true, false>(a, cfg, stream, t0, t1);
     } else {
-        if (adj) launch_sweep_t<false, true>(a, cfg, stream);
-        else launch_sweep_t<false, false>(a, cfg, stream);
+        if (adj) launch_sweep_t<false, true>(a, cfg, stream, t0, t1);
+        else launch_sweep_t<false, false>(a, cfg, stream, t0, t1);
     }
 }
 
@@ -752,19 +792,30 @@ inline hipError_t run_event_batch(EventBanks &bk, const Env &env, const gegs_sou
     a.adj = bl.adj;
     a.grad = bl.grad;
     hipError_t e;
-    if (!bk.graph || !same_args(a, bk.graph_args) || !same_cfg(cfg, bk.graph_cfg)) {
+    const bool timed = bk.timing && cfg.sweeps_per_graph <= (uint32_t)EventBanks::MAX_TIMED;
+    if (!bk.graph[0] || !same_args(a, bk.graph_args) || !same_cfg(cfg, bk.graph_cfg) || timed != bk.graph_timed) {
         bk.drop_graph();
-        hipGraph_t g;
-        e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
-        if (e != hipSuccess) return e;
-        for (uint32_t k = 0; k < cfg.sweeps_per_graph; ++k) launch_sweep(a, cfg, stream);
-        e = hipStreamEndCapture(stream, &g);
-        if (e != hipSuccess) return e;
-        e = hipGraphInstantiate(&bk.graph, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess) return e;
+        if (timed) {
+            e = bk.make_timing_events();
+            if (e != hipSuccess) return e;
+        }
+        // untimed: one exec serves both slots; timed: each slot records its own events
+        for (int slot = 0; slot < (timed ? 2 : 1); ++slot) {
+            hipGraph_t g;
+            e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+            if (e != hipSuccess) return e;
+            for (uint32_t k = 0; k < cfg.sweeps_per_graph; ++k)
+                launch_sweep(a, cfg, stream, timed ? bk.tev[slot][2 * k] : nullptr,
+                             timed ? bk.tev[slot][2 * k + 1] : nullptr);
+            e = hipStreamEndCapture(stream, &g);
+            if (e != hipSuccess) return e;
+            e = hipGraphInstantiate(&bk.graph[slot], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return e;
+        }
         bk.graph_args = a;
         bk.graph_cfg = cfg;
+        bk.graph_timed = timed;
     }
     e = hipMemcpyAsync(bk.d_plan, bk.h_plan, sizeof(BatchPlan), hipMemcpyHostToDevice, stream);
     if (e != hipSuccess) return e;
@@ -775,7 +826,7 @@ inline hipError_t run_event_batch(EventBanks &bk, const Env &env, const gegs_sou
     // empty and no record left; its kernels return at once.
     uint32_t sweeps = 0;
     auto launch = [&](int slot) -> hipError_t {
-        hipError_t r = hipGraphLaunch(bk.graph, stream);
+        hipError_t r = hipGraphLaunch(bk.graph[timed ? slot : 0], stream);
         if (r != hipSuccess) return r;
         r = hipMemcpyAsync(bk.h_flags + 2 * slot, &bk.st->active, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
         if (r != hipSuccess) return r;
@@ -787,8 +838,34 @@ inline hipError_t run_event_batch(EventBanks &bk, const Env &env, const gegs_sou
         const int slot = k & 1;
         e = hipEventSynchronize(bk.done[slot]);
         if (e != hipSuccess) return e;
+        if (timed) {
+            // photon kernel launches of this replay, timed by HIP events on this stream
+            for (uint32_t k = 0; k < cfg.sweeps_per_graph; ++k) {
+                float ms = 0.0f;
+                if (hipEventElapsedTime(&ms, bk.tev[slot][2 * k], bk.tev[slot][2 * k + 1]) == hipSuccess) {
+                    bk.photon_ms += ms;
+                    bk.photon_launches += 1;
+                }
+            }
+        }
         sweeps += cfg.sweeps_per_graph;
-        if (!bk.h_flags[2 * slot]) break;
+        if (!bk.h_flags[2 * slot]) {
+            // the replay queued behind this one finds nothing left to do;
+            // wait for it so that its launches are counted too
+            const int other = slot ^ 1;
+            e = hipEventSynchronize(bk.done[other]);
+            if (e != hipSuccess) return e;
+            if (timed) {
+                for (uint32_t k = 0; k < cfg.sweeps_per_graph; ++k) {
+                    float ms = 0.0f;
+                    if (hipEventElapsedTime(&ms, bk.tev[other][2 * k], bk.tev[other][2 * k + 1]) == hipSuccess) {
+                        bk.photon_ms += ms;
+                        bk.photon_launches += 1;
+                    }
+                }
+            }
+            break;
+        }
         if (sweeps >= max_sweeps) {
             (void)hipStreamSynchronize(stream);
             return hipErrorLaunchTimeOut;

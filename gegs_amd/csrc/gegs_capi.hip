@@ -339,7 +339,7 @@ static int check_ready(gegs_ctx *c, const gegs_run_options *o) {
     if (!c->have_media || !c->have_phantom || !c->have_field) return fail(c, GEGS_ERR_STATE, "run: load media, phantom and field first");
     if (c->nsrc == 0) return fail(c, GEGS_ERR_STATE, "run: no source particles");
     if (c->max_med >= c->nmed) return fail(c, GEGS_ERR_MEDIA, "run: the phantom uses a medium the loaded tables lack");
-    if (o->n_batches < 2 || o->n_batches > 4096) return fail(c, GEGS_ERR_ARG, "run: n_batches must be in [2, 4096]");
+    if (o->n_batches < 1 || o->n_batches > 4096) return fail(c, GEGS_ERR_ARG, "run: n_batches must be in [1, 4096]");
     if (o->first >= c->nsrc || (o->count && o->first + o->count > c->nsrc))
         return fail(c, GEGS_ERR_ARG, "run: record range outside the loaded source");
     return GEGS_OK;
@@ -468,7 +468,7 @@ static int parse_jobs(gegs_ctx *c, const gegs_job *jobs, uint32_t n_jobs, uint32
     if (!c->have_media || !c->have_phantom) return fail(c, GEGS_ERR_STATE, "run: load media and phantom first");
     if (c->nsrc == 0) return fail(c, GEGS_ERR_STATE, "run: no source particles");
     if (c->max_med >= c->nmed) return fail(c, GEGS_ERR_MEDIA, "run: the phantom uses a medium the loaded tables lack");
-    if (o->n_batches < 2 || o->n_batches > 4096) return fail(c, GEGS_ERR_ARG, "run: n_batches must be in [2, 4096]");
+    if (o->n_batches < 1 || o->n_batches > 4096) return fail(c, GEGS_ERR_ARG, "run: n_batches must be in [1, 4096]");
     fields.resize(n_jobs);
     specs.resize(n_jobs);
     slots = 0;
@@ -641,6 +641,21 @@ int gegs_history_tally(gegs_ctx *c, double *edep, uint32_t *steps) {
     hipSetDevice(c->device);
     if (edep) HIPCHECK(c, hipMemcpy(edep, c->d_hist_edep, c->hist_len * sizeof(double), hipMemcpyDeviceToHost), "read edep");
     if (steps) HIPCHECK(c, hipMemcpy(steps, c->d_hist_steps, c->hist_len * sizeof(uint32_t), hipMemcpyDeviceToHost), "read steps");
+    return GEGS_OK;
+}
+
+int gegs_set_kernel_timing(gegs_ctx *c, int enable) {
+    if (!c) return GEGS_ERR_ARG;
+    c->banks.timing = enable != 0;
+    c->banks.photon_ms = 0.0;
+    c->banks.photon_launches = 0;
+    return GEGS_OK;
+}
+
+int gegs_kernel_timing(gegs_ctx *c, double *photon_ms, uint64_t *photon_launches) {
+    if (!c || !photon_ms || !photon_launches) return GEGS_ERR_ARG;
+    *photon_ms = c->banks.photon_ms;
+    *photon_launches = c->banks.photon_launches;
     return GEGS_OK;
 }
 

@@ -108,23 +108,45 @@ __host__ __device__ __forceinline__ uint64_t primary_lineage(uint64_t hist) {
 
 struct Stream {
     uint64_t lin;
-    uint32_t blk;
-    int pos;
+    uint32_t blk;            // block after the one being consumed
+    int pos;                 // uniforms of the current block already used
     int nchild;
-    float b0, b1, b2, b3;
+    float b0, b1, b2, b3;    // current block
+    float n0, n1, n2, n3;    // block blk, generated ahead of use when `ahead`
+    bool ahead;
     uint32_t k0, k1;
 
-    __device__ __forceinline__ void refill() {
-        uint32_t c0 = blk, c1 = (uint32_t)lin, c2 = (uint32_t)(lin >> 32), c3 = 0x47454753u;
+    __device__ __forceinline__ void gen(uint32_t block, float &o0, float &o1, float &o2, float &o3) const {
+        uint32_t c0 = block, c1 = (uint32_t)lin, c2 = (uint32_t)(lin >> 32), c3 = 0x47454753u;
         philox10(c0, c1, c2, c3, k0, k1);
         const float s = 5.96046448e-08f;
-        b0 = (float)((c0 >> 8) + 1u) * s;
-        b1 = (float)((c1 >> 8) + 1u) * s;
-        b2 = (float)((c2 >> 8) + 1u) * s;
-        b3 = (float)((c3 >> 8) + 1u) * s;
+        o0 = (float)((c0 >> 8) + 1u) * s;
+        o1 = (float)((c1 >> 8) + 1u) * s;
+        o2 = (float)((c2 >> 8) + 1u) * s;
+        o3 = (float)((c3 >> 8) + 1u) * s;
+    }
+    __device__ __forceinline__ void refill() {
+        if (ahead) {
+            b0 = n0; b1 = n1; b2 = n2; b3 = n3;
+            ahead = false;
+        } else {
+            gen(blk, b0, b1, b2, b3);
+        }
         blk += 1;
         pos = 0;
     }
+    // Generates the next block before it is needed.  Called by all lanes of
+    // a wavefront together once one of them has used up its current block,
+    // so that one full-width Philox evaluation serves most lanes, instead of
+    // the divergent per-lane refills inside next() running on nearly every
+    // iteration with a few lanes each.  The uniforms drawn are unchanged.
+    __device__ __forceinline__ void lookahead() {
+        if (!ahead) {
+            gen(blk, n0, n1, n2, n3);
+            ahead = true;
+        }
+    }
+    __device__ __forceinline__ bool starved() const { return pos == 4 && !ahead; }
     // a particle suspended in the middle of a block (spos < 4) resumes at the
     // same uniform: the block is generated again
     __device__ __forceinline__ void begin(const Part &p, uint64_t seed) {
@@ -132,6 +154,7 @@ struct Stream {
         blk = p.blk;
         pos = 4;
         nchild = 0;
+        ahead = false;
         k0 = (uint32_t)seed;
         k1 = (uint32_t)(seed >> 32);
         if ((p.spos & 7) < 4) {
@@ -151,6 +174,13 @@ struct Stream {
         return lin ^ splitmix((at << 8) | k);
     }
 };
+
+// all lanes of the wavefront must reach this together
+__device__ __forceinline__ void wave_lookahead(Stream &s, bool have) {
+    if (__any(have && s.starved())) {
+        if (have) s.lookahead();
+    }
+}
 
 __device__ __forceinline__ uint64_t child_lineage(Stream &s) { return splitmix(s.child()); }
 
@@ -688,7 +718,9 @@ template <class Sink, class T>
 __device__ __forceinline__ void photon_step(const Env &env, Stream &s, Part &p, Sink &sink, T &t) {
     PhotonWalk w;
     photon_walk_begin(env, p, w);
-    while (photon_walk_iter(env, s, p, w, sink, t)) {
+    for (;;) {
+        wave_lookahead(s, true);
+        if (!photon_walk_iter(env, s, p, w, sink, t)) break;
     }
 }
 
@@ -1453,7 +1485,9 @@ template <class Sink, class T>
 __device__ __forceinline__ void electron_step(const Env &env, Stream &s, Part &p, Sink &sink, T &t) {
     EState st;
     st.init();
-    while (electron_iter(env, s, p, st, sink, t)) {
+    for (;;) {
+        wave_lookahead(s, true);
+        if (!electron_iter(env, s, p, st, sink, t)) break;
     }
 }
 

@@ -22,6 +22,7 @@ SYMBOLS = [
     "gegs_run_plain", "gegs_run_event", "gegs_set_history_tally", "gegs_history_tally",
     "gegs_get_counters", "gegs_last_kernel_ms", "gegs_sample_events",
     "gegs_set_tuning", "gegs_get_tuning", "gegs_run_batched", "gegs_set_beamlets", "gegs_adjoint",
+    "gegs_set_kernel_timing", "gegs_kernel_timing",
 ]
 
 PARTICLE_DTYPE = np.dtype([("e", "<f4"), ("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("u", "<f4"),
@@ -85,6 +86,14 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError("gegs_amd/libgegs_hip.so is missing: run `python -m gegs_amd.build` "
                            "(the transport has no CPU fallback)")
+    # PyTorch ships its own HIP runtime under the same soname
+    # (libamdhip64.so.7).  Loading torch first makes the library bind to that
+    # runtime, so that the process has one HIP runtime and torch tensors and
+    # this library share the device (two runtimes do not both see the GPU).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     vp = ctypes.c_void_p
     L.gegs_create.restype = vp
@@ -106,6 +115,8 @@ def load():
     L.gegs_sample_events.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint64, vp, ctypes.c_uint32, vp, vp]
     L.gegs_set_tuning.argtypes = [vp, vp]
     L.gegs_run_batched.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp]
+    L.gegs_set_kernel_timing.argtypes = [vp, ctypes.c_int]
+    L.gegs_kernel_timing.argtypes = [vp, vp, vp]
     L.gegs_set_beamlets.argtypes = [vp, vp, ctypes.c_uint64, vp, ctypes.c_uint32]
     L.gegs_adjoint.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp]
     L.gegs_get_tuning.argtypes = [vp, vp]
@@ -258,6 +269,19 @@ class Context:
             out["edep"] = edep
             out["steps"] = steps
         return out
+
+    def set_kernel_timing(self, enable=True):
+        """Time every photon kernel launch of the event path with HIP events
+        (resets the accumulators)."""
+        self._check(self.L.gegs_set_kernel_timing(self.h, 1 if enable else 0))
+
+    def kernel_timing(self):
+        """(summed photon kernel launch durations in ms, launches) since the
+        last set_kernel_timing."""
+        ms = ctypes.c_double()
+        n = ctypes.c_uint64()
+        self._check(self.L.gegs_kernel_timing(self.h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
 
     def set_field(self, field):
         self._check(self.L.gegs_set_field(self.h, ctypes.c_void_p(ctypes.addressof(field))))

@@ -85,7 +85,7 @@ typedef struct gegs_job {
 typedef struct gegs_run_options {
     uint64_t seed;               /* Philox-4x32-10 key */
     uint64_t history_offset;     /* global index of the first history */
-    uint32_t n_batches;          /* time-sliced statistical batches (>= 2) */
+    uint32_t n_batches;          /* time-sliced statistical batches (1 ... 4096; the batch variance needs >= 2) */
     uint32_t block_size;         /* plain method: threads per block, 0 = default */
     uint32_t grid_size;          /* plain method: blocks, 0 = default */
     uint32_t stack_depth;        /* plain method: per-history particle stack depth, 0 = default */
@@ -210,6 +210,14 @@ int gegs_history_tally(gegs_ctx *ctx, double *edep, uint32_t *steps);
  * uniform against the oracle. */
 int gegs_sample_events(gegs_ctx *ctx, const gegs_particle *in, uint32_t n, uint64_t seed,
                        gegs_particle *out, uint32_t max_out, uint32_t *n_out, double *edep);
+
+/* Timing of the event path's photon kernel with HIP events on the context's
+ * stream (event-record nodes around every photon kernel launch in the
+ * captured sweep graphs).  Enabling resets the accumulators; gegs_kernel_timing
+ * returns the summed launch durations (ms) and the number of launches since.
+ * Replaces the reference's per-launch host timing (CUDA_EGS.cu:226-365). */
+int gegs_set_kernel_timing(gegs_ctx *ctx, int enable);
+int gegs_kernel_timing(gegs_ctx *ctx, double *photon_ms, uint64_t *photon_launches);
 
 /* Counters and timing of the last run. */
 int gegs_get_counters(gegs_ctx *ctx, gegs_counters *out);

@@ -1769,6 +1769,22 @@ int oracle_interaction(const gegs_media_tables *mt, const gegs_region *regions,
 }
 
 /* elementary functions, exported for their accuracy tests */
+/* mscat probe: n draws of the multiple-scattering angle for fixed (lambda,
+ * chia2, q1), stream i keyed by lineage lin0 + i; returns cos(theta) */
+void oracle_mscat(const gegs_media_tables *mt, float lambda, float chia2, float q1, uint64_t seed, uint64_t lin0,
+                  uint32_t n, float *cost) {
+    for (uint32_t i = 0; i < n; ++i) {
+        opart p;
+        memset(&p, 0, sizeof(p));
+        p.lin = lin0 + i;
+        ostream s;
+        st_begin(&s, &p, seed);
+        float c, sn;
+        mscat(mt, &s, lambda, chia2, q1, &c, &sn);
+        cost[i] = c;
+    }
+}
+
 float oracle_logf(float x) { return om_logf(x); }
 float oracle_expf(float x) { return om_expf(x); }
 void oracle_sincos2pi(float t, float *s, float *c) { om_sincos2pi(t, s, c); }

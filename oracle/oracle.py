@@ -47,6 +47,8 @@ def lib():
         L.oracle_interaction.restype = ctypes.c_int
         L.oracle_interaction.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                          vp, vp, vp, vp, vp, ctypes.c_uint64, vp, ctypes.c_int, vp]
+        L.oracle_mscat.argtypes = [vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_uint64,
+                                   ctypes.c_uint64, ctypes.c_uint32, vp]
         L.oracle_logf.restype = ctypes.c_float
         L.oracle_logf.argtypes = [ctypes.c_float]
         L.oracle_expf.restype = ctypes.c_float
@@ -91,6 +93,13 @@ def run(workload, source, seed, hist_offset=0, per_history=False, threads=1):
         raise RuntimeError("oracle_run failed: %d" % rc)
     return dict(dose=dose, dose2=dose2, edep=edep, steps=steps, counters=cnt,
                 escaped=float(bal[0]), dropped=float(bal[1]))
+
+
+def mscat(tables, lam, chia2, q1, n, seed=1, lin0=1):
+    """n multiple-scattering polar cosines from the oracle's mscat."""
+    out = np.zeros(n, dtype=np.float32)
+    lib().oracle_mscat(ctypes.c_void_p(tables.ptr()), lam, chia2, q1, seed, lin0, n, _p(out))
+    return out
 
 
 def interaction(workload, particle, seed, max_out=64):

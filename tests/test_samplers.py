@@ -233,3 +233,86 @@ def test_photon_free_path_is_exponential(wl):
     cdf = lambda x: (1 - np.exp(-mu * x)) / (1 - math.exp(-mu * depth))
     assert stats.kstest(fin, cdf).pvalue > 1e-4
     assert abs(np.mean(~np.isfinite(d)) - math.exp(-mu * depth)) < 0.02
+
+
+def test_bhabha_energy_distribution(wl):
+    """Electron kinetic-energy fraction eps in [T_e/T, 1] follows the Bhabha
+    cross section 1/eps^2 - beta^2 (B1/eps - B2 + B3 eps - B4 eps^2) with
+    y = 1/(gamma + 1), B1 = 2 - y^2, B2 = 3 - 6y + y^2 - 2y^3,
+    B3 = 2 - 10y + 16y^2 - 8y^3, B4 = 1 - 6y + 12y^2 - 8y^3 (Bhabha 1936)."""
+    t0 = 2.5
+    outs, ed = run_many(wl, E_BHABHA, t0 + RM, 5000, charge=1)
+    eps = []
+    for o in outs:
+        el = o[o["charge"] == -1]
+        assert len(el) == 1
+        eps.append((el["e"][0] - RM) / t0)
+    eps = np.array(eps)
+    gam = t0 / RM + 1.0
+    y = 1.0 / (gam + 1.0)
+    beta2 = 1.0 - 1.0 / gam ** 2
+    b1, b2 = 2 - y * y, 3 - 6 * y + y * y - 2 * y ** 3
+    b3, b4 = 2 - 10 * y + 16 * y * y - 8 * y ** 3, 1 - 6 * y + 12 * y * y - 8 * y ** 3
+    lo = (0.7 - RM) / t0
+
+    def cdf(x):
+        g = np.linspace(lo, 1.0, 40001)
+        f = 1 / g ** 2 - beta2 * (b1 / g - b2 + b3 * g - b4 * g * g)
+        cum = np.concatenate([[0], np.cumsum(0.5 * (f[1:] + f[:-1]) * np.diff(g))])
+        return np.interp(np.clip(x, lo, 1.0), g, cum / cum[-1])
+    bins = lo + (1 - lo) * np.linspace(0, 1, 31) ** 2
+    assert chi2_pvalue(eps, cdf, bins) > 1e-4
+
+
+def test_pair_bethe_heitler_above_2_1_MeV(wl):
+    """Above 2.1 MeV the pair sampler draws the lower-energy lepton's share
+    b = (E_lower - m)/(k - 2m) in [0, 1/2] from a two-branch mixture: with the
+    weight (1 - 2m/k)^2 A_max/3 : C_max, b ~ 6(1-2b)^2 accepted with A(delta)/A_max,
+    else b ~ U(0, 1/2) accepted with C(delta)/C_max (pair.cu:111-181).  The net
+    density is (1 - 2m/k)^2 (1 - 2b)^2 A(delta) + C(delta), delta = k delcm /
+    (E- E+), with A and C the pair screening fits (dl index 4 and 5)."""
+    k = 6.0
+    outs, ed = run_many(wl, P_PAIR, k, 6000)
+    c = wl.tables.c
+    delcm = c.delcm[0]
+    eav = k - 2 * RM
+    bs = []
+    for o in outs:
+        lep = o[o["charge"] != 0]
+        if len(lep) == 2:
+            bs.append((min(lep["e"]) - RM) / eav)
+    bs = np.array(bs)
+    lo = (0.7 - RM) / eav + 1e-6          # the lower lepton must be above ECUT to be pushed
+
+    def screen(l, d):
+        return np.where(d < 1, c.dl1[l] + d * (c.dl2[l] + d * c.dl3[l]), c.dl4[l] + c.dl5[l] * np.log(d + c.dl6[l]))
+
+    def dens(b):
+        em = b * eav + RM
+        d = k * delcm / (em * (k - em))
+        return (1 - 2 * RM / k) ** 2 * (1 - 2 * b) ** 2 * screen(4, d) + screen(5, d)
+
+    def cdf(x):
+        g = np.linspace(lo, 0.5, 40001)
+        f = dens(g)
+        cum = np.concatenate([[0], np.cumsum(0.5 * (f[1:] + f[:-1]) * np.diff(g))])
+        return np.interp(np.clip(x, lo, 0.5), g, cum / cum[-1])
+    sel = bs[bs >= lo]
+    assert chi2_pvalue(sel, cdf, np.linspace(lo, 0.5, 26)) > 1e-4
+
+
+@pytest.mark.parametrize("lam,chia2", [(0.8, 0.05), (5.0, 5e-3), (20.0, 1e-3), (200.0, 1e-4), (3000.0, 5e-6)])
+def test_multiple_scattering_mean_cosine(wl, lam, chia2):
+    """Goudsmit-Saunderson for screened Rutherford scattering: after lambda
+    elastic mean free paths with screening parameter chia2 the mean polar
+    cosine is exp(-G1), G1 = 2 lambda chia2 [(1 + chia2) ln(1 + 1/chia2) - 1]
+    (the first moment the electron step passes to mscat as q1).  The cases
+    cover the lambda <= 1, 1 < lambda <= 13.8 and table branches, inside the
+    q^(2+) tables' domain (G1 below about 0.5)."""
+    assert lam > 0
+    g1 = 2 * lam * chia2 * ((1 + chia2) * math.log(1 + 1 / chia2) - 1)
+    n = 40000
+    cost = oracle.mscat(wl.tables, lam, chia2, g1, n, seed=3).astype(np.float64)
+    expect = math.exp(-g1)
+    err = cost.std() / math.sqrt(n)
+    assert abs(cost.mean() - expect) < 4 * err + 3e-3 * (1 - expect), (cost.mean(), expect, err)
