@@ -93,10 +93,14 @@ def add_counters(acc, c):
             for k in acc}
 
 
-def run_method(ctx, method, steps, warmup, b, seed):
+def run_method(ctx, method, steps, warmup, b, seed, kernel_timing=False):
+    """Times `steps` steps of one batch of b records each.  With
+    kernel_timing the event path launches its sweeps directly on the stream
+    with a HIP event pair around every photon kernel launch (graph replays
+    cannot carry timed events); returns (elapsed, doses, counters, timing)."""
     for w in range(warmup):
         ctx.run(seed=seed, n_batches=1, method=method, first=w * b, count=b)
-    if method == "event":
+    if kernel_timing:
         ctx.set_kernel_timing(True)
     step_doses, counters = [], None
     _sync()
@@ -108,7 +112,7 @@ def run_method(ctx, method, steps, warmup, b, seed):
     _sync()
     elapsed = time.perf_counter() - t0
     timing = None
-    if method == "event":
+    if kernel_timing:
         timing = ctx.kernel_timing()
         ctx.set_kernel_timing(False)
     return elapsed, step_doses, counters, timing
@@ -187,19 +191,25 @@ def main():
         tuning, _, _ = autotune.autotune(ctx, 0, min(b, 500_000), seed=args.seed)
     results = {}
     for method in methods:
-        elapsed, doses, cnt, timing = run_method(ctx, method, args.steps, args.warmup, b, args.seed)
+        elapsed, doses, cnt, _ = run_method(ctx, method, args.steps, args.warmup, b, args.seed)
         hist = args.steps * b
         sig, nsel = mean_rel_sigma_from_steps(doses)
         eff = 1.0 / (sig * sig * elapsed)
         res = dict(value=hist / elapsed, ms_per_step=elapsed / args.steps * 1e3, sigma=sig, voxels=nsel,
                    efficiency=eff, counters=cnt, bytes=algorithmic_bytes(cnt, hist))
-        if method == "event" and timing and timing[1]:
-            photon_ms, launches = timing
-            pbytes = PHOTON_STEP_BYTES * cnt["photon_steps"]
-            res["roofline"] = dict(kernel="photon_kernel", launches=launches,
-                                   avg_launch_us=photon_ms / launches * 1e3,
-                                   algorithmic_bytes_per_launch=pbytes / launches,
-                                   achieved=pbytes / (photon_ms * 1e-3) / 1e9)
+        if method == "event":
+            # the same steps again with every photon kernel launch timed by
+            # HIP events on the library's stream
+            el2, _, cnt2, timing = run_method(ctx, method, args.steps, 0, b, args.seed, kernel_timing=True)
+            if timing and timing[1]:
+                photon_ms, launches = timing
+                pbytes = PHOTON_STEP_BYTES * cnt2["photon_steps"]
+                res["roofline"] = dict(kernel="photon_kernel", launches=launches,
+                                       avg_launch_us=photon_ms / launches * 1e3,
+                                       algorithmic_bytes_per_launch=pbytes / launches,
+                                       achieved=pbytes / (photon_ms * 1e-3) / 1e9,
+                                       photon_kernel_share=photon_ms * 1e-3 / el2,
+                                       timed_pass_histories_per_s=hist / el2)
         results[method] = res
 
     head = "event" if "event" in results else methods[0]
@@ -218,7 +228,11 @@ def main():
                 "frac": rf["achieved"] / HBM_PEAK_GBS, "traffic": None, "kernel": rf["kernel"],
                 "avg_launch_us": rf["avg_launch_us"], "launches": rf["launches"],
                 "algorithmic_bytes_per_launch": rf["algorithmic_bytes_per_launch"],
-                "algorithmic_bytes_per_photon_step": PHOTON_STEP_BYTES}
+                "algorithmic_bytes_per_photon_step": PHOTON_STEP_BYTES,
+                "photon_kernel_share": rf["photon_kernel_share"],
+                "timed_pass_histories_per_s": rf["timed_pass_histories_per_s"],
+                "timing": "HIP events around every photon kernel launch over a second pass of the same steps "
+                          "(sweeps launched on the stream; the value pass replays graphs)"}
     else:
         roof = None
     line = {

@@ -104,6 +104,10 @@ __device__ __forceinline__ uint64_t lanes_below() {
     return l == 0 ? 0ull : (~0ull >> (64 - l));
 }
 
+// the lowest active lane issues a wavefront's shared atomics (lane 0 where
+// the whole wavefront is active)
+__device__ __forceinline__ bool first_lane() { return (__ballot(1) & lanes_below()) == 0; }
+
 // bank buffer of event e drained in this sweep (now) or in the next one
 __device__ __forceinline__ int bank_slot(int e, uint32_t flip, bool now) {
     const bool alt = (flip != 0) == now;
@@ -139,7 +143,8 @@ __device__ __forceinline__ void bank_store(const EvArgs &a, int slot, uint32_t i
     bank_put(a, slot, 8, idx, p.dnear);
     bank_put(a, slot, 9, idx, __uint_as_float(p.region));
     bank_put(a, slot, 10, idx,
-             __uint_as_float((uint32_t)(p.charge + 1) | ((uint32_t)p.spos << 8) | ((uint32_t)p.module << 16)));
+             __uint_as_float((uint32_t)(p.charge + 1) | ((uint32_t)p.spos << 8) | (((uint32_t)p.module & 7u) << 16) |
+                             (((uint32_t)(p.bank + 1) & 7u) << 19) | ((uint32_t)(p.leaf + 128) << 24)));
     bank_put(a, slot, 11, idx, __uint_as_float((uint32_t)p.lin));
     bank_put(a, slot, 12, idx, __uint_as_float((uint32_t)(p.lin >> 32)));
     bank_put(a, slot, 13, idx, __uint_as_float(p.blk));
@@ -162,7 +167,9 @@ __device__ __forceinline__ Part bank_load(const EvArgs &a, int slot, int event, 
     const uint32_t pk = __float_as_uint(bank_get(a, slot, 10, idx));
     p.charge = (int32_t)(pk & 0xffu) - 1;
     p.spos = (int32_t)((pk >> 8) & 15u);
-    p.module = (int32_t)(pk >> 16);
+    p.module = (int32_t)((pk >> 16) & 7u);
+    p.bank = (int32_t)((pk >> 19) & 7u) - 1;
+    p.leaf = (int32_t)(pk >> 24) - 128;
     p.status = event;
     p.lin = (uint64_t)__float_as_uint(bank_get(a, slot, 11, idx)) |
             ((uint64_t)__float_as_uint(bank_get(a, slot, 12, idx)) << 32);
@@ -191,22 +198,31 @@ struct BankSink {
     uint32_t hist;
     uint32_t flip;
     bool photons_now, electrons_now;
+    // The groups are served one per iteration by every lane of the call:
+    // `todo` (a wavefront-uniform mask) loses the served group each time and
+    // the loop ends when it is empty.  A loop whose only exit is a divergent
+    // `dest == readfirstlane(dest)` test must not be used here: its state does
+    // not change between iterations, so under C++ forward-progress rules the
+    // compiler may assume the first iteration exits and drop the test, which
+    // sends every lane's particle to the first lane's destination count.
     __device__ __forceinline__ void push(const Part &p) {
         const int e = p.status;
         const int dest = bank_slot(e, flip, e == P_PHOTON_STEP ? photons_now : electrons_now);
-        for (;;) {
-            const int d0 = __builtin_amdgcn_readfirstlane(dest);
-            if (dest == d0) {
-                const uint64_t m = __ballot(1);
-                const uint32_t rank = (uint32_t)__popcll(m & lanes_below());
-                uint32_t base = 0;
-                if (rank == 0) base = atomicAdd(&a->st->count[d0], (uint32_t)__popcll(m));
-                base = __builtin_amdgcn_readfirstlane(base);
-                const uint32_t idx = base + rank;
+        uint64_t todo = __ballot(1);
+        while (todo) {
+            const int leader = __builtin_ctzll(todo);
+            const int d0 = __builtin_amdgcn_readlane(dest, leader);
+            const bool mine = dest == d0;
+            const uint64_t m = __ballot(mine);
+            uint32_t base = 0;
+            if ((int)lane_id() == leader) base = atomicAdd(&a->st->count[d0], (uint32_t)__popcll(m));
+            base = __builtin_amdgcn_readlane(base, leader);
+            if (mine) {
+                const uint32_t idx = base + (uint32_t)__popcll(m & lanes_below());
                 if (idx < a->cap) bank_store(*a, d0, idx, p, hist);
                 else atomicAdd(&a->st->overflow, 1u);
-                break;
             }
+            todo &= ~m;
         }
     }
     // an electron suspended between two condensed-history steps carries the
@@ -270,7 +286,7 @@ __device__ __forceinline__ void flush_counters(const EvArgs &a, const T &t, int 
     unsigned long long *c = a.counters;
     const uint32_t v0 = wave_sum(t.photon_steps), v1 = wave_sum(t.electron_steps);
     const uint32_t v2 = wave_sum(t.gathers), v3 = wave_sum(t.deposits), v4 = wave_sum(pops);
-    if (lane_id() == 0) {
+    if (first_lane()) {
         if (v0) atomicAdd(c + 0, (unsigned long long)v0);
         if (v1) atomicAdd(c + 1, (unsigned long long)v1);
         if (v2) atomicAdd(c + 2, (unsigned long long)v2);
@@ -351,7 +367,7 @@ struct Refill {
         if (end - next < need) {
             // top up the reservation: keep the remainder, add 64 more
             uint32_t got = 0;
-            if (lane_id() == 0) got = atomicAdd(taken, 64u);
+            if (first_lane()) got = atomicAdd(taken, 64u);
             got = __builtin_amdgcn_readfirstlane(got);
             if (got >= n) {
                 exhausted = true;
@@ -526,6 +542,8 @@ __global__ void __launch_bounds__(256) inject_fill(EvArgs a) {
             p.region = 0;
             p.job = lo;
             p.module = a.env.fields[lo].module_first;
+            p.leaf = OUTMLC;
+            p.bank = -1;
             p.dnear = INFI_DIST;
             p.blk = 0;
             p.spos = 4;
@@ -538,7 +556,7 @@ __global__ void __launch_bounds__(256) inject_fill(EvArgs a) {
         }
     }
     const uint32_t tot = wave_sum(injected);
-    if (lane_id() == 0 && tot) atomicAdd(a.counters + 14, (unsigned long long)tot);
+    if (first_lane() && tot) atomicAdd(a.counters + 14, (unsigned long long)tot);
 }
 
 __global__ void sweep_end(EvState *st) {
@@ -590,12 +608,11 @@ struct EventBanks {
     hipEvent_t done[2] = {nullptr, nullptr};   // completion of the two replays in flight
     BatchPlan *d_plan = nullptr;     // injection plan of the current batch
     BatchPlan *h_plan = nullptr;     // pinned staging copy
-    // one graph exec per replay slot; with kernel timing each holds its own
-    // event pairs around the photon kernel launches
+    // the captured sweep group (graph[0]; graph[1] is unused) and, for
+    // kernel timing, one set of event pairs per replay slot
     hipGraphExec_t graph[2] = {nullptr, nullptr};
     EvArgs graph_args{};
     EventConfig graph_cfg{};
-    bool graph_timed = false;
     static constexpr int MAX_TIMED = 64;
     hipEvent_t tev[2][2 * MAX_TIMED] = {};
     bool timing = false;             // measure the photon kernel with HIP events
@@ -644,6 +661,18 @@ struct EventBanks {
                     const hipError_t e = hipEventCreate(&tev[k][i]);
                     if (e != hipSuccess) return e;
                 }
+        return hipSuccess;
+    }
+    // photon kernel launches of the replay in `slot`, timed by the event
+    // pairs its graph records on the stream
+    hipError_t add_timing(int slot, uint32_t sweeps) {
+        for (uint32_t k = 0; k < sweeps; ++k) {
+            float ms = 0.0f;
+            const hipError_t e = hipEventElapsedTime(&ms, tev[slot][2 * k], tev[slot][2 * k + 1]);
+            if (e != hipSuccess) return e;
+            photon_ms += ms;
+            photon_launches += 1;
+        }
         return hipSuccess;
     }
     void release() {
@@ -747,7 +776,9 @@ inline bool same_args(const EvArgs &x, const EvArgs &y) {
            x.beamlet == y.beamlet && x.bweight == y.bweight && x.adj == y.adj && x.grad == y.grad &&
            x.counters == y.counters && x.env.mt == y.env.mt && x.env.rcls == y.env.rcls && x.env.rtab == y.env.rtab && x.env.seed == y.env.seed &&
            x.env.xb == y.env.xb && x.env.yb == y.env.yb && x.env.zb == y.env.zb && x.env.nx == y.env.nx &&
-           x.env.ny == y.env.ny && x.env.nz == y.env.nz && x.env.pge == y.env.pge;
+           x.env.ny == y.env.ny && x.env.nz == y.env.nz && x.env.pge == y.env.pge && x.env.pgmfp == y.env.pgmfp &&
+           x.env.pcohe == y.env.pcohe && x.env.mlc == y.env.mlc && x.env.endnodes == y.env.endnodes &&
+           x.env.mlc_next == y.env.mlc_next;
 }
 
 inline bool same_cfg(const EventConfig &x, const EventConfig &y) {
@@ -792,30 +823,27 @@ inline hipError_t run_event_batch(EventBanks &bk, const Env &env, const gegs_sou
     a.adj = bl.adj;
     a.grad = bl.grad;
     hipError_t e;
+    // With kernel timing the sweeps are launched directly on the stream,
+    // with an event pair around every photon kernel launch (HIP reports no
+    // elapsed time for events recorded by graph nodes); otherwise the sweep
+    // group is captured once per argument set and replayed.
     const bool timed = bk.timing && cfg.sweeps_per_graph <= (uint32_t)EventBanks::MAX_TIMED;
-    if (!bk.graph[0] || !same_args(a, bk.graph_args) || !same_cfg(cfg, bk.graph_cfg) || timed != bk.graph_timed) {
+    if (timed) {
+        e = bk.make_timing_events();
+        if (e != hipSuccess) return e;
+    } else if (!bk.graph[0] || !same_args(a, bk.graph_args) || !same_cfg(cfg, bk.graph_cfg)) {
         bk.drop_graph();
-        if (timed) {
-            e = bk.make_timing_events();
-            if (e != hipSuccess) return e;
-        }
-        // untimed: one exec serves both slots; timed: each slot records its own events
-        for (int slot = 0; slot < (timed ? 2 : 1); ++slot) {
-            hipGraph_t g;
-            e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
-            if (e != hipSuccess) return e;
-            for (uint32_t k = 0; k < cfg.sweeps_per_graph; ++k)
-                launch_sweep(a, cfg, stream, timed ? bk.tev[slot][2 * k] : nullptr,
-                             timed ? bk.tev[slot][2 * k + 1] : nullptr);
-            e = hipStreamEndCapture(stream, &g);
-            if (e != hipSuccess) return e;
-            e = hipGraphInstantiate(&bk.graph[slot], g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (e != hipSuccess) return e;
-        }
+        hipGraph_t g;
+        e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return e;
+        for (uint32_t k = 0; k < cfg.sweeps_per_graph; ++k) launch_sweep(a, cfg, stream);
+        e = hipStreamEndCapture(stream, &g);
+        if (e != hipSuccess) return e;
+        e = hipGraphInstantiate(&bk.graph[0], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return e;
         bk.graph_args = a;
         bk.graph_cfg = cfg;
-        bk.graph_timed = timed;
     }
     e = hipMemcpyAsync(bk.d_plan, bk.h_plan, sizeof(BatchPlan), hipMemcpyHostToDevice, stream);
     if (e != hipSuccess) return e;
@@ -826,7 +854,14 @@ inline hipError_t run_event_batch(EventBanks &bk, const Env &env, const gegs_sou
     // empty and no record left; its kernels return at once.
     uint32_t sweeps = 0;
     auto launch = [&](int slot) -> hipError_t {
-        hipError_t r = hipGraphLaunch(bk.graph[timed ? slot : 0], stream);
+        hipError_t r = hipSuccess;
+        if (timed) {
+            for (uint32_t k = 0; k < cfg.sweeps_per_graph; ++k)
+                launch_sweep(a, cfg, stream, bk.tev[slot][2 * k], bk.tev[slot][2 * k + 1]);
+            r = hipGetLastError();
+        } else {
+            r = hipGraphLaunch(bk.graph[0], stream);
+        }
         if (r != hipSuccess) return r;
         r = hipMemcpyAsync(bk.h_flags + 2 * slot, &bk.st->active, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
         if (r != hipSuccess) return r;
@@ -838,16 +873,7 @@ inline hipError_t run_event_batch(EventBanks &bk, const Env &env, const gegs_sou
         const int slot = k & 1;
         e = hipEventSynchronize(bk.done[slot]);
         if (e != hipSuccess) return e;
-        if (timed) {
-            // photon kernel launches of this replay, timed by HIP events on this stream
-            for (uint32_t k = 0; k < cfg.sweeps_per_graph; ++k) {
-                float ms = 0.0f;
-                if (hipEventElapsedTime(&ms, bk.tev[slot][2 * k], bk.tev[slot][2 * k + 1]) == hipSuccess) {
-                    bk.photon_ms += ms;
-                    bk.photon_launches += 1;
-                }
-            }
-        }
+        if (timed && (e = bk.add_timing(slot, cfg.sweeps_per_graph)) != hipSuccess) return e;
         sweeps += cfg.sweeps_per_graph;
         if (!bk.h_flags[2 * slot]) {
             // the replay queued behind this one finds nothing left to do;
@@ -855,15 +881,7 @@ inline hipError_t run_event_batch(EventBanks &bk, const Env &env, const gegs_sou
             const int other = slot ^ 1;
             e = hipEventSynchronize(bk.done[other]);
             if (e != hipSuccess) return e;
-            if (timed) {
-                for (uint32_t k = 0; k < cfg.sweeps_per_graph; ++k) {
-                    float ms = 0.0f;
-                    if (hipEventElapsedTime(&ms, bk.tev[other][2 * k], bk.tev[other][2 * k + 1]) == hipSuccess) {
-                        bk.photon_ms += ms;
-                        bk.photon_launches += 1;
-                    }
-                }
-            }
+            if (timed && (e = bk.add_timing(other, cfg.sweeps_per_graph)) != hipSuccess) return e;
             break;
         }
         if (sweeps >= max_sweeps) {

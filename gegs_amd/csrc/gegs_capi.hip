@@ -37,6 +37,12 @@ struct gegs_ctx {
     uint32_t *d_job_plan = nullptr;      // dose buffer per job
     size_t fields_cap = 0, job_plan_cap = 0;
     uint64_t hist_len = 0;               // per-history slots of the last run
+    gegs_mlc *d_mlc = nullptr;           // MLC geometry
+    gegs_mlc h_mlc{};
+    bool have_mlc = false;
+    std::vector<float> leaf_pos;         // 120 per leaf set
+    gegs_float2 *d_endnodes = nullptr;   // leaf ends per job
+    size_t endnodes_cap = 0;
     uint32_t *d_beamlet = nullptr;       // beamlet of each history slot
     float *d_bweight = nullptr;          // weight of each beamlet
     double *d_adj = nullptr, *d_grad = nullptr;
@@ -221,9 +227,29 @@ int gegs_load_phantom(gegs_ctx *c, uint32_t nx, uint32_t ny, uint32_t nz, const 
 
 int gegs_set_field(gegs_ctx *c, const gegs_field *f) {
     if (!c || !f) return fail(c, GEGS_ERR_ARG, "gegs_set_field: null argument");
-    if (f->module_first != GEGS_M_PHANTOM) return fail(c, GEGS_ERR_ARG, "gegs_set_field: only the phantom module is built");
+    if (f->module_first != GEGS_M_PHANTOM && f->module_first != GEGS_M_MLC)
+        return fail(c, GEGS_ERR_ARG, "gegs_set_field: fields start in the MLC or the phantom module");
     c->field = *f;
     c->have_field = true;
+    return GEGS_OK;
+}
+
+int gegs_load_mlc(gegs_ctx *c, const gegs_mlc *m) {
+    if (!c || !m) return fail(c, GEGS_ERR_ARG, "gegs_load_mlc: null argument");
+    if (!(m->zmin > 0.0f) || !(m->zthick > 0.0f) || !(m->rmax > 0.0f) || !(m->x_end > m->x_start) ||
+        !(m->wl_full > 0.0f) || !(m->wl_target > 0.0f) || !(m->wl_isocenter > 0.0f))
+        return fail(c, GEGS_ERR_ARG, "gegs_load_mlc: inconsistent MLC geometry");
+    hipSetDevice(c->device);
+    if (!c->d_mlc) HIPCHECK(c, hipMalloc(&c->d_mlc, sizeof(gegs_mlc)), "hipMalloc MLC");
+    HIPCHECK(c, hipMemcpy(c->d_mlc, m, sizeof(gegs_mlc), hipMemcpyHostToDevice), "copy MLC");
+    c->h_mlc = *m;
+    c->have_mlc = true;
+    return GEGS_OK;
+}
+
+int gegs_set_leaf_positions(gegs_ctx *c, const float *pos, uint32_t n_sets) {
+    if (!c || (!pos && n_sets)) return fail(c, GEGS_ERR_ARG, "gegs_set_leaf_positions: null argument");
+    c->leaf_pos.assign(pos, pos + (size_t)n_sets * 2 * GEGS_MLC_LEAVES);
     return GEGS_OK;
 }
 
@@ -279,14 +305,51 @@ static int prepare_tallies(gegs_ctx *c, uint32_t n_plans, uint64_t n_slots) {
     return GEGS_OK;
 }
 
-// beam geometry and dose buffer of each job, in device memory
-static int upload_jobs(gegs_ctx *c, const std::vector<gegs_field> &fields, const std::vector<uint32_t> &plans) {
+// leaf ends of one leaf set for a field (init_leaf_pos, init.cu:1594-1662):
+// an open pair's bank A end moves to -pos[j+60], its bank B end to pos[j]
+// (the banks are read inverted), scaled from the isocentre to the leaf
+// centre plane; a closed pair keeps the closed profile
+static void leaf_ends(const gegs_mlc &m, const float *pos, float sid, gegs_float2 *out) {
+    const float zcenter = m.zmin + m.zthick / 2.0f;
+    for (int j = 0; j < GEGS_MLC_LEAVES; ++j) {
+        gegs_float2 *en = out + (size_t)j * 34;
+        for (int i = 0; i < 34; ++i) en[i] = m.cross_yz[i];
+        if (pos[j] + pos[j + GEGS_MLC_LEAVES] > 0.0f) {
+            const float POS = pos[j], NEG = -pos[j + GEGS_MLC_LEAVES];
+            for (int i = 2; i < 17; ++i) en[i].x = m.cross_yz[i].x + NEG * zcenter / sid;
+            for (int i = 17; i < 32; ++i) en[i].x = m.cross_yz[i].x + POS * zcenter / sid;
+        }
+    }
+}
+
+// beam geometry, dose buffer and leaf ends of each job, in device memory
+static int upload_jobs(gegs_ctx *c, const std::vector<gegs_field> &fields, const std::vector<uint32_t> &plans,
+                       const std::vector<uint32_t> &leaf_sets = {0u}) {
     HIPCHECK(c, realloc_dev(c->d_fields, c->fields_cap, fields.size()), "hipMalloc fields");
     HIPCHECK(c, realloc_dev(c->d_job_plan, c->job_plan_cap, plans.size()), "hipMalloc job plans");
     HIPCHECK(c, hipMemcpyAsync(c->d_fields, fields.data(), fields.size() * sizeof(gegs_field), hipMemcpyHostToDevice,
                                c->stream), "copy fields");
     HIPCHECK(c, hipMemcpyAsync(c->d_job_plan, plans.data(), plans.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
                                c->stream), "copy job plans");
+    bool any_mlc = false;
+    for (const gegs_field &f : fields) any_mlc |= f.module_first == GEGS_M_MLC;
+    std::vector<gegs_float2> ends;
+    if (any_mlc) {
+        if (!c->have_mlc) return fail(c, GEGS_ERR_STATE, "run: a field starts in the MLC but no MLC is loaded");
+        const size_t nsets = c->leaf_pos.size() / (2 * GEGS_MLC_LEAVES);
+        ends.resize(fields.size() * GEGS_MLC_LEAVES * 34);
+        for (size_t j = 0; j < fields.size(); ++j) {
+            if (fields[j].module_first != GEGS_M_MLC) continue;
+            const uint32_t set = j < leaf_sets.size() ? leaf_sets[j] : 0u;
+            if (set >= nsets) return fail(c, GEGS_ERR_ARG, "run: a field's leaf set is not loaded");
+            if (!(fields[j].source_to_isocenter > 0.0f)) return fail(c, GEGS_ERR_ARG, "run: source_to_isocenter must be > 0");
+            leaf_ends(c->h_mlc, c->leaf_pos.data() + set * 2 * GEGS_MLC_LEAVES, fields[j].source_to_isocenter,
+                      ends.data() + j * GEGS_MLC_LEAVES * 34);
+        }
+        HIPCHECK(c, realloc_dev(c->d_endnodes, c->endnodes_cap, ends.size()), "hipMalloc leaf ends");
+        HIPCHECK(c, hipMemcpyAsync(c->d_endnodes, ends.data(), ends.size() * sizeof(gegs_float2),
+                                   hipMemcpyHostToDevice, c->stream), "copy leaf ends");
+    }
     HIPCHECK(c, hipStreamSynchronize(c->stream), "copy jobs");
     return GEGS_OK;
 }
@@ -309,6 +372,9 @@ static Env make_env(gegs_ctx *c, uint64_t seed, uint32_t njob) {
     env.nz = c->nz;
     env.fields = c->d_fields;
     env.njob = njob;
+    env.mlc = c->have_mlc ? c->d_mlc : nullptr;
+    env.endnodes = c->d_endnodes;
+    env.mlc_next = GEGS_M_PHANTOM;
     env.seed = seed;
     return env;
 }
@@ -396,6 +462,7 @@ int gegs_run_plain(gegs_ctx *c, const gegs_run_options *o, double *dose, double 
 struct JobSpec {
     uint32_t plan;
     uint64_t first, count, history_offset, slot0;
+    uint32_t leaf_set;
 };
 
 // Event-queued transport of every job in one loop: each batch holds the
@@ -407,9 +474,12 @@ static int run_event_jobs(gegs_ctx *c, const gegs_run_options *o, const std::vec
     hipSetDevice(c->device);
     int rc = prepare_tallies(c, n_plans, n_slots);
     if (rc) return rc;
-    std::vector<uint32_t> plans(jobs.size());
-    for (size_t j = 0; j < jobs.size(); ++j) plans[j] = jobs[j].plan;
-    rc = upload_jobs(c, fields, plans);
+    std::vector<uint32_t> plans(jobs.size()), sets(jobs.size());
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        plans[j] = jobs[j].plan;
+        sets[j] = jobs[j].leaf_set;
+    }
+    rc = upload_jobs(c, fields, plans, sets);
     if (rc) return rc;
     const EventConfig cfg = c->tuning;
     const uint32_t nb = o->n_batches;
@@ -456,7 +526,7 @@ int gegs_run_event(gegs_ctx *c, const gegs_run_options *o, double *dose, double 
     if (rc) return rc;
     const uint64_t r0 = o->first, nr = o->count ? o->count : c->nsrc - o->first;
     // one job; per-history slots are record indices
-    const JobSpec js{0u, r0, nr, o->history_offset + r0, r0};
+    const JobSpec js{0u, r0, nr, o->history_offset + r0, r0, 0u};
     return run_event_jobs(c, o, {c->field}, {js}, 1, c->nsrc, dose, dose2);
 }
 
@@ -475,11 +545,12 @@ static int parse_jobs(gegs_ctx *c, const gegs_job *jobs, uint32_t n_jobs, uint32
     for (uint32_t j = 0; j < n_jobs; ++j) {
         const gegs_job &jb = jobs[j];
         if (jb.plan >= n_plans) return fail(c, GEGS_ERR_ARG, "run: job plan out of range");
-        if (jb.field.module_first != GEGS_M_PHANTOM) return fail(c, GEGS_ERR_ARG, "run: only the phantom module is built");
+        if (jb.field.module_first != GEGS_M_PHANTOM && jb.field.module_first != GEGS_M_MLC)
+            return fail(c, GEGS_ERR_ARG, "run: fields start in the MLC or the phantom module");
         if (jb.count == 0 || jb.first >= c->nsrc || jb.first + jb.count > c->nsrc)
             return fail(c, GEGS_ERR_ARG, "run: job record range outside the loaded source");
         fields[j] = jb.field;
-        specs[j] = JobSpec{jb.plan, jb.first, jb.count, jb.history_offset, slots};
+        specs[j] = JobSpec{jb.plan, jb.first, jb.count, jb.history_offset, slots, jb.leaf_set};
         slots += jb.count;
     }
     return GEGS_OK;

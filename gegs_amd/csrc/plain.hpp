@@ -45,6 +45,7 @@ struct StackSink {
         *at(k, 12) = __uint_as_float((uint32_t)(p.lin >> 32));
         *at(k, 13) = __uint_as_float(p.blk);
         *at(k, 14) = __uint_as_float(p.job);
+        *at(k, 15) = __uint_as_float((uint32_t)(p.leaf + 128) | ((uint32_t)(p.bank + 1) << 8));
     }
     __device__ __forceinline__ Part pop() {
         const int k = --sp;
@@ -67,6 +68,9 @@ struct StackSink {
         p.lin = (uint64_t)__float_as_uint(*at(k, 11)) | ((uint64_t)__float_as_uint(*at(k, 12)) << 32);
         p.blk = __float_as_uint(*at(k, 13));
         p.job = __float_as_uint(*at(k, 14));
+        const uint32_t lb = __float_as_uint(*at(k, 15));
+        p.leaf = (int32_t)(lb & 0xffu) - 128;
+        p.bank = (int32_t)(lb >> 8) - 1;
         return p;
     }
 };
@@ -95,6 +99,8 @@ __device__ __forceinline__ Part source_particle(const gegs_source &r, int module
     p.dnear = INFI_DIST;
     p.blk = 0;
     p.job = 0;
+    p.leaf = OUTMLC;
+    p.bank = -1;
     p.spos = 4;
     p.status = r.charge == 0 ? P_PHOTON_STEP : E_ELECTRON_STEP;
     return p;
@@ -185,6 +191,7 @@ __global__ void sample_kernel(Env env, const gegs_particle *in, uint32_t n, gegs
     p.e = r.e; p.x = r.x; p.y = r.y; p.z = r.z; p.u = r.u; p.v = r.v; p.w = r.w; p.wt = r.wt;
     p.dnear = r.dnear; p.region = r.region; p.charge = r.charge; p.status = r.status;
     p.module = env.fields[0].module_first; p.lin = r.lin; p.blk = r.blk; p.spos = 4; p.job = 0;
+    p.leaf = OUTMLC; p.bank = -1;
     ListSink sink{out + (size_t)i * max_out, max_out, 0};
     Tally t;
     t.dose = nullptr;

@@ -73,6 +73,8 @@ struct Part {
     uint64_t lin;
     uint32_t blk;
     uint32_t job;        // field of the plan the history belongs to (beam geometry, dose buffer)
+    int32_t leaf;        // MLC leaf index (OUTMLC / PASSMLC outside the leaf rows)
+    int32_t bank;        // MLC bank: 0 opening, 1 bank A, 2 bank B, -1 not yet located
     int32_t spos;        // bits 0-2: uniforms of block blk-1 already used (4: none
                          // pending); bit 3: a suspended flight that resumes
 };
@@ -200,6 +202,9 @@ struct Env {
     uint32_t nclass, nmed;
     const gegs_field *fields;        // beam geometry per job
     uint32_t njob;
+    const gegs_mlc *mlc;             // MLC geometry (null: no MLC)
+    const gegs_float2 *endnodes;     // leaf ends per job: [job][60][34]
+    int32_t mlc_next;                // module after the MLC
     uint64_t seed;
 };
 
@@ -451,6 +456,21 @@ __device__ __forceinline__ float hownear_phantom(const Env &env, const Part &p) 
     return sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);
 }
 
+}  // namespace gegs
+
+#include "mlc.hpp"
+
+namespace gegs {
+
+// module dispatch of the geometry queries (photon.cu:142-158, electron.cu:1491-1505, :1836-1850)
+__device__ __forceinline__ uint32_t howfar_module(const Env &env, Part &p, float &t) {
+    return p.module == GEGS_M_MLC ? howfar_mlc(env, p, t) : howfar_phantom(env, p, t);
+}
+
+__device__ __forceinline__ float hownear_module(const Env &env, Part &p) {
+    return p.module == GEGS_M_MLC ? hownear_mlc(env, p) : hownear_phantom(env, p);
+}
+
 // ------------------------------------------------------------------ rotations (utility.cu:94-218)
 __device__ __forceinline__ void turn(float costhe, float sinthe, float cosphi, float sinphi, Part &p) {
     const float A = p.u, B = p.v, C = p.w;
@@ -662,10 +682,10 @@ __device__ __forceinline__ bool photon_walk_iter(const Env &env, Stream &s, Part
         }
         if (tstep < 0.0f) tstep = EPSTEP;
     } else {
-        nreg = howfar_phantom(env, p, tstep);
+        nreg = howfar_module(env, p, tstep);
     }
     int idisc = 0;
-    if (nreg == 0) idisc = rd.med == VACUUM ? 1 : -1;
+    if (nreg == 0) idisc = (rd.med == VACUUM || p.leaf == PASSMLC) ? 1 : -1;
     if (idisc > 0) { p.region = 0; return false; }
     p.x += p.u * tstep;
     p.y += p.v * tstep;
@@ -1302,7 +1322,7 @@ __device__ __forceinline__ bool electron_iter(const Env &env, Stream &s, Part &p
         range = (range + mt->range_ep[(medium * MXEKE + lelke - 1) * 2 + qel]) / rhof;
         tmxs_val = fminf(tmxs_val, mt->smaxir[medium]);
         tustep = fminf(tstep, fminf(tmxs_val, range));
-        tperp = hownear_phantom(env, p);
+        tperp = hownear_module(env, p);
         p.dnear = tperp;
         if (mt->i_do_rr[medium] == 1 && p.e < mt->e_max_rr[medium] && tperp >= range) {
             t.score(p.region, to_kinetic(p.e) * p.wt);
@@ -1338,8 +1358,8 @@ __device__ __forceinline__ bool electron_iter(const Env &env, Stream &s, Part &p
     uint32_t irnew = p.region;
     int idisc = 0;
     const float ustep0 = ustep;
-    if (callhowfar || p.wt <= 0.0f) irnew = howfar_phantom(env, p, ustep);
-    if (irnew == 0) idisc = rd.med == VACUUM ? 1 : -1;
+    if (callhowfar || p.wt <= 0.0f) irnew = howfar_module(env, p, ustep);
+    if (irnew == 0) idisc = (rd.med == VACUUM || p.leaf == PASSMLC) ? 1 : -1;
     if (idisc > 0) {
         p.region = 0;
         return false;

@@ -1,0 +1,379 @@
+"""Treatment-head inputs: the DYNVMLC multileaf collimator and its fields.
+
+The reference reads the MLC from the DYNVMLC block of a BEAMnrc .egsinp file
+(read_egsinp, init.cu:345-611): leaf dimensions of the three leaf types
+(full, target, isocentre), the number of leaves, the leaf row start and air
+gap, the leaf-end radius and the default openings.  It builds the leaf
+cross-sections across the leaves (Leaf::VerCal_*, read_egsinp.h:37-86), with
+sides focused on the source, the closed leaf-end profile along the leaf
+travel (leaf_end, init.cu:256-343) and, per field, the leaf ends shifted to
+the field's leaf positions (init_leaf_pos, init.cu:1594-1662).  Fields come
+from a .mlcinfo list (read_mlcinp, init.cu:1377-1500): gantry angle, jaws,
+MU and a Varian leaf file of 120 positions.
+
+The reference's VarianMLC.egsinp and .mlcinfo files are not available, so
+``MILLENNIUM_LIKE`` is a synthetic 120-leaf geometry of this build (40 inner
+pairs of 5 mm and 20 outer pairs of 10 mm projected at the isocentre; leaf
+height, tongue-and-groove and rail dimensions chosen to be physically
+plausible, not taken from a vendor drawing).  The leaf-end profile is a
+circular arc of the leaf-end radius sampled at equal angles (the reference
+hard-codes the 15 points of its arc).
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import math
+
+import numpy as np
+
+NLEAF = 60
+M_MLC = 3
+
+
+class Float2(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_float), ("y", ctypes.c_float)]
+
+
+class Mlc(ctypes.Structure):
+    """ctypes mirror of ``gegs_mlc`` (include/gegs.h)."""
+    _fields_ = [("zmin", ctypes.c_float), ("zthick", ctypes.c_float), ("rmax", ctypes.c_float),
+                ("x_start", ctypes.c_float), ("x_end", ctypes.c_float), ("x_mid_1", ctypes.c_float),
+                ("x_mid_2", ctypes.c_float), ("air_gap", ctypes.c_float), ("wl_full", ctypes.c_float),
+                ("wl_target", ctypes.c_float), ("wl_isocenter", ctypes.c_float), ("wt", ctypes.c_float),
+                ("cross_xz", Float2 * (NLEAF * 16)), ("boundary_xz", Float2 * 4), ("boundary_yz", Float2 * 4),
+                ("cross_yz", Float2 * 34)]
+
+    def cross(self):
+        return np.ctypeslib.as_array(self.cross_xz).view(np.float32).reshape(NLEAF, 16, 2)
+
+
+@dataclasses.dataclass
+class Leaf:
+    """DYNVMLC leaf type: widths (wl leaf, wt tongue, wg groove, wtip tip,
+    wts/wbs top/bottom rail) and z planes (absolute, from the source)."""
+    wl: float
+    wt: float
+    wg: float
+    wtip: float
+    wts: float
+    wbs: float
+    ztip: float
+    zl: float
+    zt: float
+    zg: float
+    zth: float
+    zbh: float
+    holepos: float
+    zts: float
+    zbs: float
+
+
+@dataclasses.dataclass
+class Dynvmlc:
+    rmax: float
+    zmin: float
+    zthick: float
+    full: Leaf
+    target: Leaf
+    isocenter: Leaf
+    n_full: int            # full leaves on each side
+    n_half: int            # half leaves in the middle (target + isocentre)
+    x_start: float
+    air_gap: float
+    rad_end: float
+    openings: list         # [(neg, pos, count), ...] covering 60 leaves
+    medium: str = "W700ICRU"
+
+
+def _vertices(kind, lf, zmin, zthick):
+    """The 14 cross-section vertices (x, z) of a leaf (read_egsinp.h:37-86)."""
+    zmax = zmin + zthick
+    if kind == "full":
+        v = [(0, lf.ztip), (lf.wtip, lf.ztip), (lf.wtip, lf.zl), (lf.wt + lf.wl - lf.wg, lf.zl),
+             (lf.wt + lf.wl - lf.wg, lf.zg), (lf.wt + lf.wl, lf.zg), (lf.wt + lf.wl, zmax),
+             (lf.wt + lf.wts - lf.wbs, zmax), (lf.wt + lf.wts - lf.wbs, lf.zbs), (lf.wt + lf.wts, lf.zbs),
+             (lf.wt + lf.wts, lf.zts), (lf.wt, lf.zts), (lf.wt, lf.zt), (0, lf.zt)]
+    elif kind == "target":
+        v = [(0, zmin), (lf.wt + lf.wl - lf.wbs + lf.wts, zmin), (lf.wt + lf.wl - lf.wbs + lf.wts, lf.zts),
+             (lf.wt + lf.wl - lf.wbs, lf.zts), (lf.wt + lf.wl - lf.wbs, lf.zbs), (lf.wt + lf.wl, lf.zbs),
+             (lf.wt + lf.wl, lf.zg), (lf.wt + lf.wl - lf.wg, lf.zg), (lf.wt + lf.wl - lf.wg, lf.ztip),
+             (lf.wt + lf.wl - lf.wg - lf.wtip, lf.ztip), (lf.wt + lf.wl - lf.wg - lf.wtip, lf.zl),
+             (lf.wt, lf.zl), (lf.wt, lf.zt), (0, lf.zt)]
+    else:
+        v = [(0, lf.zt), (lf.wt, lf.zt), (lf.wt, lf.ztip), (lf.wt + lf.wtip, lf.ztip), (lf.wt + lf.wtip, lf.zl),
+             (lf.wt + lf.wl - lf.wg, lf.zl), (lf.wt + lf.wl - lf.wg, lf.zg), (lf.wt + lf.wl, lf.zg),
+             (lf.wt + lf.wl, zmax), (lf.wts - lf.wbs, zmax), (lf.wts - lf.wbs, lf.zbs), (lf.wts, lf.zbs),
+             (lf.wts, lf.zts), (0, lf.zts)]
+    return [(np.float32(x), np.float32(z)) for x, z in v]
+
+
+def leaf_end_profile(zmin, zthick, rmax, rad_end):
+    """The closed leaf pair along the leaf travel, 34 (y, z) vertices: the
+    -rmax edge, bank A's rounded end (tip at y = 0 at mid height), bank B's
+    end, the +rmax edge."""
+    zc = zthick / 2.0
+    th = math.asin(min(1.0, zc / rad_end))
+    ang = np.linspace(-th, th, 15)
+    ys = rad_end * np.sin(ang)                   # along z, relative to the mid plane
+    xs = rad_end * (1.0 - np.cos(ang))           # retreat of the end from the tip
+    out = np.zeros((34, 2), dtype=np.float32)
+    out[0] = (-rmax, zmin + zthick)
+    out[1] = (-rmax, zmin)
+    for i in range(15):                          # bank A, top to bottom, y <= 0
+        out[2 + i] = (-xs[i], ys[i] + zc + zmin)
+    for i in range(15):                          # bank B, bottom to top, y >= 0
+        out[17 + i] = (xs[14 - i], ys[14 - i] + zc + zmin)
+    out[32] = (rmax, zmin)
+    out[33] = (rmax, zmin + zthick)
+    return out
+
+
+def build_mlc(d: Dynvmlc) -> Mlc:
+    """gegs_mlc of a DYNVMLC description (read_egsinp, init.cu:441-611)."""
+    m = Mlc()
+    f32 = np.float32
+    zmin, zthick = f32(d.zmin), f32(d.zthick)
+    m.zmin, m.zthick, m.rmax = float(zmin), float(zthick), float(d.rmax)
+    shapes = {"full": _vertices("full", d.full, zmin, zthick), "target": _vertices("target", d.target, zmin, zthick),
+              "isocenter": _vertices("isocenter", d.isocenter, zmin, zthick)}
+    order = ["full"] * d.n_full + ["target", "isocenter"] * (d.n_half // 2) + ["full"] * d.n_full
+    if len(order) != NLEAF:
+        raise ValueError("DYNVMLC: %d leaves, expected %d" % (len(order), NLEAF))
+    width = {"full": f32(d.full.wl), "target": f32(d.target.wl), "isocenter": f32(d.isocenter.wl)}
+    X = f32(d.x_start)
+    gap = f32(d.air_gap)
+    x_mid_1 = x_mid_2 = None
+    for i, kind in enumerate(order):
+        if i == d.n_full:
+            x_mid_1 = X
+        if i == d.n_full + d.n_half:
+            x_mid_2 = X
+        for j, (vx, vz) in enumerate(shapes[kind]):
+            m.cross_xz[i * 16 + j].x = float(f32((X + vx) / zmin * vz))
+            m.cross_xz[i * 16 + j].y = float(vz)
+        X = f32(X + width[kind] + gap)
+    x_end = f32(X - gap + f32(d.full.wg))
+    m.x_start, m.x_end = float(d.x_start), float(x_end)
+    m.x_mid_1, m.x_mid_2 = float(x_mid_1), float(x_mid_2)
+    m.air_gap = float(gap)
+    m.wl_full, m.wl_target, m.wl_isocenter = float(d.full.wl), float(d.target.wl), float(d.isocenter.wl)
+    m.wt = float(d.full.wt)
+    # default openings in the 15th slot of every leaf (init.cu:497-515)
+    i = 0
+    for neg, pos, num in d.openings:
+        for _ in range(num):
+            m.cross_xz[14 + i * 16].x = neg
+            m.cross_xz[14 + i * 16].y = pos
+            i += 1
+    zt = f32(zmin + zthick)
+    bxz = [(f32(d.x_start) / zmin * zt, zt), (f32(d.x_start), zmin), (x_end, zmin), (x_end / zmin * zt, zt)]
+    byz = [(-d.rmax, zt), (-d.rmax, zmin), (d.rmax, zmin), (d.rmax, zt)]
+    for k in range(4):
+        m.boundary_xz[k].x, m.boundary_xz[k].y = float(bxz[k][0]), float(bxz[k][1])
+        m.boundary_yz[k].x, m.boundary_yz[k].y = float(byz[k][0]), float(byz[k][1])
+    prof = leaf_end_profile(float(zmin), float(zthick), d.rmax, d.rad_end)
+    for k in range(34):
+        m.cross_yz[k].x, m.cross_yz[k].y = float(prof[k, 0]), float(prof[k, 1])
+    return m
+
+
+def _leaf(wl, wt, wg, wtip, wts, wbs, ztip, zl, zt, zg, zts, zbs):
+    return Leaf(wl, wt, wg, wtip, wts, wbs, ztip, zl, zt, zg, zth=zg, zbh=zg, holepos=0.0, zts=zts, zbs=zbs)
+
+
+# synthetic 120-leaf MLC: leaf pitch 0.48 cm (outer) and 0.24 cm (inner) at
+# zmin = 48 cm, i.e. 1.0 cm and 0.5 cm at a 100 cm isocentre; leaves 6.9 cm
+# high, a 0.006 cm air gap between neighbours.  The leaves are plain focused
+# slabs (no tongue, groove, tip or protruding rail: those widths are 0 and
+# the step heights coincide with the leaf faces), so the 14-vertex
+# cross-sections degenerate to focused quadrilaterals.  The reference's leaf
+# tracking assumes air between neighbouring leaves at every height and a leaf
+# lookup from each leaf's nominal span; tongue-and-groove dimensions that
+# keep both true need the vendor drawing, which is not available.
+MILLENNIUM_LIKE = Dynvmlc(
+    rmax=20.0, zmin=48.0, zthick=6.9,
+    full=_leaf(0.474, 0.0, 0.0, 0.0, 0.0, 0.0, 48.0, 48.0, 48.0, 54.9, 54.9, 54.9),
+    target=_leaf(0.234, 0.0, 0.0, 0.0, 0.0, 0.0, 54.9, 54.9, 48.0, 54.9, 48.0, 54.9),
+    isocenter=_leaf(0.234, 0.0, 0.0, 0.0, 0.0, 0.0, 48.0, 48.0, 48.0, 54.9, 54.9, 54.9),
+    n_full=10, n_half=40, x_start=0.0,
+    air_gap=0.006, rad_end=7.0, openings=[(0.0, 0.0, 60)])
+
+
+def millennium_like():
+    """The synthetic MLC with its leaf rows centred on the beam axis."""
+    d = dataclasses.replace(MILLENNIUM_LIKE)
+    # centre the leaf rows on the axis: total width at zmin
+    total = 2 * d.n_full * (d.full.wl + d.air_gap) + (d.n_half // 2) * (d.target.wl + d.isocenter.wl + 2 * d.air_gap)
+    d.x_start = -0.5 * total
+    return d
+
+
+# ------------------------------------------------------------------ egsinp DYNVMLC block
+CM_TITLE = "*********** start of CM DYNVMLC"
+
+
+def write_dynvmlc(path, d: Dynvmlc):
+    """The DYNVMLC lines read_egsinp consumes (init.cu:389-465), in order."""
+    def leaf_line_a(lf):
+        return "%g, %g, %g, %g, %g, %g," % (lf.wl, lf.wt, lf.wg, lf.wtip, lf.wts, lf.wbs)
+    lines = [CM_TITLE + " with identifier MLC *********",
+             "%g, RMAX" % d.rmax, "Varian-like MLC (synthetic)", "0, ZFOCUS mode",
+             "%g, ZMIN" % d.zmin, "%g, ZTHICK" % d.zthick,
+             leaf_line_a(d.full),
+             "%g, %g, %g, %g, %g, %g," % (d.full.ztip, d.full.zl, d.full.zt, d.full.zg, d.full.zth, d.full.zbh),
+             "%g," % d.full.holepos, "%g, %g," % (d.full.zts, d.full.zbs),
+             leaf_line_a(d.target),
+             "%g, %g, %g, %g," % (d.target.zts, d.target.zbs, d.target.zth, d.target.zbh),
+             "%g," % d.target.holepos,
+             "%g, %g, %g, %g," % (d.target.zt, d.target.zg, d.target.zl, d.target.ztip),
+             leaf_line_a(d.isocenter),
+             "%g, %g, %g, %g, %g, %g," % (d.isocenter.ztip, d.isocenter.zl, d.isocenter.zt, d.isocenter.zg,
+                                          d.isocenter.zth, d.isocenter.zbh),
+             "%g," % d.isocenter.holepos, "%g, %g," % (d.isocenter.zts, d.isocenter.zbs),
+             "%d, NUM FULL LEAVES LEFT" % d.n_full, "%d, NUM HALF LEAVES" % d.n_half,
+             "%d, NUM FULL LEAVES RIGHT" % d.n_full, "%g, START" % d.x_start, "%g, LEAFGAP" % d.air_gap,
+             "0, ENDTYPE", "%g, LEAFRADIUS" % d.rad_end, "0, ZFOCUS of leaf sides"]
+    lines += ["%g, %g, %d" % o for o in d.openings]
+    lines += ["0.7, 0.01, 0, 0", "0, ", "", d.medium]
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def _nums(line, n):
+    vals = [t for t in line.replace(",", " ").split()]
+    out = []
+    for t in vals:
+        try:
+            out.append(float(t))
+        except ValueError:
+            break
+    if len(out) < n:
+        raise ValueError("DYNVMLC: expected %d numbers in %r" % (n, line))
+    return out[:n]
+
+
+def read_dynvmlc(path) -> Dynvmlc:
+    """Parse the DYNVMLC block the way read_egsinp does (line by line after
+    the CM title; comments after the numbers are ignored)."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    try:
+        k = next(i for i, l in enumerate(lines) if l.startswith(CM_TITLE[:31]))
+    except StopIteration:
+        raise ValueError("no DYNVMLC block in %s" % path)
+    it = iter(lines[k + 1:])
+    rmax = _nums(next(it), 1)[0]
+    next(it)
+    next(it)
+    zmin = _nums(next(it), 1)[0]
+    zthick = _nums(next(it), 1)[0]
+    fa = _nums(next(it), 6)
+    fb = _nums(next(it), 6)
+    fh = _nums(next(it), 1)[0]
+    fr = _nums(next(it), 2)
+    full = Leaf(*fa, ztip=fb[0], zl=fb[1], zt=fb[2], zg=fb[3], zth=fb[4], zbh=fb[5], holepos=fh, zts=fr[0], zbs=fr[1])
+    ta = _nums(next(it), 6)
+    tb = _nums(next(it), 4)
+    th = _nums(next(it), 1)[0]
+    tc = _nums(next(it), 4)
+    target = Leaf(*ta, ztip=tc[3], zl=tc[2], zt=tc[0], zg=tc[1], zth=tb[2], zbh=tb[3], holepos=th, zts=tb[0], zbs=tb[1])
+    ia = _nums(next(it), 6)
+    ib = _nums(next(it), 6)
+    ih = _nums(next(it), 1)[0]
+    ir = _nums(next(it), 2)
+    iso = Leaf(*ia, ztip=ib[0], zl=ib[1], zt=ib[2], zg=ib[3], zth=ib[4], zbh=ib[5], holepos=ih, zts=ir[0], zbs=ir[1])
+    n_full = int(_nums(next(it), 1)[0])
+    n_half = int(_nums(next(it), 1)[0])
+    next(it)
+    x_start = _nums(next(it), 1)[0]
+    gap = _nums(next(it), 1)[0]
+    next(it)
+    rad = _nums(next(it), 1)[0]
+    next(it)
+    openings, n = [], 0
+    while n < NLEAF:
+        a, b, c = _nums(next(it), 3)
+        openings.append((a, b, int(c)))
+        n += int(c)
+    next(it)
+    next(it)
+    next(it)
+    medium = next(it).strip()
+    return Dynvmlc(rmax, zmin, zthick, full, target, iso, n_full, n_half, x_start, gap, rad, openings, medium)
+
+
+# ------------------------------------------------------------------ fields (.mlcinfo)
+@dataclasses.dataclass
+class MlcField:
+    angle: float
+    x1: float
+    x2: float
+    y1: float
+    y2: float
+    mu: float
+    leaf_file: str
+    positions: np.ndarray | None = None     # 120 leaf positions (cm at the isocentre)
+
+
+def write_leaf_file(path, positions):
+    """A Varian leaf file: 14 header lines, then 120 lines whose value starts
+    at column 11 (read_mlcinp reads after skipping 10 characters)."""
+    pos = np.asarray(positions, dtype=np.float64)
+    head = ["File Rev = H", "Treatment = Dynamic Dose", "Last Name = synthetic", "First Name = ",
+            "Patient ID = 0", "Number of Fields = 1", "Number of Leaves = 120", "Tolerance = 0.50",
+            "Field = 1", "Index = 0.0000", "Carriage Group = 1", "Operator = ", "Collimator = 0.0", ""]
+    rows = []
+    for j in range(120):
+        bank = "A" if j < 60 else "B"
+        rows.append("Leaf%3d%s=%8.4f" % (j % 60 + 1, bank, pos[j]))
+    with open(path, "w") as f:
+        f.write("\n".join(head + rows) + "\n")
+
+
+def read_leaf_file(path):
+    with open(path) as f:
+        lines = f.read().splitlines()
+    vals = [float(lines[14 + j][10:].split()[0].strip("=")) for j in range(120)]
+    return np.array(vals, dtype=np.float32)
+
+
+def write_mlcinfo(path, fields):
+    rows = ["MLC fields (synthetic)", "%d" % len(fields)]
+    for i, f in enumerate(fields):
+        rows.append("%d, %g, %g, %g, %g, %g, %g, %s" % (i + 1, f.angle, f.x1, f.x2, f.y1, f.y2, f.mu, f.leaf_file))
+    with open(path, "w") as fo:
+        fo.write("\n".join(rows) + "\n")
+
+
+def read_mlcinfo(path, data_dir=""):
+    """Fields of a .mlcinfo list (read_mlcinp, init.cu:1414-1470)."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    n = int(lines[1].split(",")[0])
+    out = []
+    for line in lines[2:2 + n]:
+        parts = [t.strip() for t in line.split(",")]
+        vals = [float(t) for t in parts[1:7]]
+        out.append(MlcField(*vals, leaf_file=data_dir + ",".join(parts[7:]).strip()))
+    return out
+
+
+def random_aperture(seed, half=5.0, smooth=3):
+    """120 leaf positions of an irregular aperture inside a 2 half x 2 half
+    field: pairs whose projected width lies outside the field stay closed,
+    the others open to smoothly varying seeded positions: the +x leaf end at
+    +pos[j], the -x end at -pos[j+60], both within `half` (the leaf-file
+    order of init.cu:1607-1613)."""
+    rng = np.random.default_rng(seed)
+    centers = np.concatenate([np.arange(10) * 1.0 - 19.5, np.arange(40) * 0.5 - 9.75, np.arange(10) * 1.0 + 10.5])
+    a = rng.uniform(0.3 * half, half, NLEAF)
+    b = rng.uniform(0.3 * half, half, NLEAF)
+    k = np.ones(smooth) / smooth
+    a = np.convolve(a, k, mode="same")
+    b = np.convolve(b, k, mode="same")
+    pos = np.zeros(2 * NLEAF, dtype=np.float32)
+    inside = np.abs(centers) < half
+    pos[:NLEAF] = np.where(inside, b, 0.0)            # +x leaf end at +pos[j]
+    pos[NLEAF:] = np.where(inside, a, 0.0)            # -x leaf end at -pos[j+60]
+    return pos

@@ -22,7 +22,7 @@ SYMBOLS = [
     "gegs_run_plain", "gegs_run_event", "gegs_set_history_tally", "gegs_history_tally",
     "gegs_get_counters", "gegs_last_kernel_ms", "gegs_sample_events",
     "gegs_set_tuning", "gegs_get_tuning", "gegs_run_batched", "gegs_set_beamlets", "gegs_adjoint",
-    "gegs_set_kernel_timing", "gegs_kernel_timing",
+    "gegs_set_kernel_timing", "gegs_kernel_timing", "gegs_load_mlc", "gegs_set_leaf_positions",
 ]
 
 PARTICLE_DTYPE = np.dtype([("e", "<f4"), ("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("u", "<f4"),
@@ -58,7 +58,7 @@ class Tuning(ctypes.Structure):
 class Job(ctypes.Structure):
     """gegs_job: one field of a plan for gegs_run_batched."""
     _fields_ = [("field", ctypes.c_byte * 32), ("plan", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32), ("first", ctypes.c_uint64), ("count", ctypes.c_uint64),
+                ("leaf_set", ctypes.c_uint32), ("first", ctypes.c_uint64), ("count", ctypes.c_uint64),
                 ("history_offset", ctypes.c_uint64)]
 
 
@@ -115,6 +115,8 @@ def load():
     L.gegs_sample_events.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint64, vp, ctypes.c_uint32, vp, vp]
     L.gegs_set_tuning.argtypes = [vp, vp]
     L.gegs_run_batched.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp]
+    L.gegs_load_mlc.argtypes = [vp, vp]
+    L.gegs_set_leaf_positions.argtypes = [vp, vp, ctypes.c_uint32]
     L.gegs_set_kernel_timing.argtypes = [vp, ctypes.c_int]
     L.gegs_kernel_timing.argtypes = [vp, vp, vp]
     L.gegs_set_beamlets.argtypes = [vp, vp, ctypes.c_uint64, vp, ctypes.c_uint32]
@@ -171,6 +173,9 @@ class Context:
         ph = wl.phantom
         reg = np.ascontiguousarray(wl.regions)
         self._check(self.L.gegs_load_phantom(self.h, ph.nx, ph.ny, ph.nz, _p(ph.xb), _p(ph.yb), _p(ph.zb), _p(reg)))
+        if getattr(wl, "mlc", None) is not None:
+            self._check(self.L.gegs_load_mlc(self.h, ctypes.c_void_p(ctypes.addressof(wl.mlc))))
+            self.set_leaf_positions(wl.leaf_positions)
         self._check(self.L.gegs_set_field(self.h, ctypes.c_void_p(ctypes.addressof(wl.field))))
         self.nvox = ph.nvox
 
@@ -215,10 +220,19 @@ class Context:
         for j, jb in enumerate(jobs):
             ctypes.memmove(ctypes.addressof(arr[j].field), ctypes.addressof(jb["field"]), 32)
             arr[j].plan = int(jb["plan"])
+            arr[j].leaf_set = int(jb.get("leaf_set", 0))
             arr[j].first = int(jb["first"])
             arr[j].count = int(jb["count"])
             arr[j].history_offset = int(jb.get("history_offset", jb["first"]))
         return arr
+
+    def set_leaf_positions(self, positions):
+        """Leaf positions of one or more leaf sets (120 floats each: bank B
+        +pos for leaves 0-59, then bank A); a job picks its set by index."""
+        pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1)
+        if len(pos) == 0 or len(pos) % 120:
+            raise ValueError("leaf positions come in sets of 120")
+        self._check(self.L.gegs_set_leaf_positions(self.h, _p(pos), len(pos) // 120))
 
     def set_beamlets(self, beamlet=None, weight=None):
         """Beamlet index per history slot (job after job) and weight per

@@ -124,3 +124,50 @@ def point_source_spectrum(n, ssd, field, seed, z_source=0.0, edges=None, probs=N
     s["w"] = (ssd / norm).astype(np.float32)
     s["wt"] = np.float32(1.0)
     return s
+
+
+def encode_phsp(src):
+    """Source particles -> MODE0 records (the inverse of decode_phsp): charge
+    in latch bits 29/30, total energy (EGSnrc stores it for charged particles,
+    the photon energy otherwise), w < 0 flagged by a negative weight."""
+    rec = np.zeros(len(src), dtype=PHSP_DTYPE)
+    q = src["charge"]
+    rec["latch"] = np.where(q < 0, 0x40000000, np.where(q > 0, 0x20000000, 0)).astype(np.uint32)
+    rec["e"] = src["e"]
+    rec["x"] = src["x"]
+    rec["y"] = src["y"]
+    rec["u"] = src["u"]
+    rec["v"] = src["v"]
+    rec["wt"] = np.where(src["w"] < 0, -src["wt"], src["wt"]).astype(np.float32)
+    return rec
+
+
+def head_phase_space(n, z_plane, ssd, field, seed, electron_fraction=0.03, edges=None, probs=None):
+    """Synthetic phase space scored on the plane z = z_plane above the
+    collimators: particles from a point source at the origin travelling
+    towards points uniform over a (field + 2 cm) square at distance ssd;
+    photons with the 6 MV-like spectrum, a fraction of contaminant electrons
+    (kinetic energy uniform in 0.2-4 MeV, total energy stored)."""
+    if edges is None:
+        edges, probs = mv6_spectrum()
+    rng = np.random.default_rng(seed)
+    half = field / 2.0 + 1.0
+    px = rng.uniform(-half, half, n)
+    py = rng.uniform(-half, half, n)
+    norm = np.sqrt(px * px + py * py + ssd * ssd)
+    u, v, w = px / norm, py / norm, ssd / norm
+    t = z_plane / w
+    s = np.zeros(n, dtype=SOURCE_DTYPE)
+    b = rng.choice(len(probs), size=n, p=probs)
+    e = edges[b] + rng.uniform(0.0, 1.0, n) * (edges[b + 1] - edges[b])
+    el = rng.uniform(0.0, 1.0, n) < electron_fraction
+    s["e"] = np.where(el, rng.uniform(0.2, 4.0, n) + RM, e).astype(np.float32)
+    s["charge"] = np.where(el, -1, 0)
+    s["x"] = (u * t).astype(np.float32)
+    s["y"] = (v * t).astype(np.float32)
+    s["z"] = np.float32(z_plane)
+    s["u"] = u.astype(np.float32)
+    s["v"] = v.astype(np.float32)
+    s["w"] = w.astype(np.float32)
+    s["wt"] = np.float32(1.0)
+    return s

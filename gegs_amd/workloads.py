@@ -28,15 +28,19 @@ class Field(ctypes.Structure):
                 ("iso_y", ctypes.c_float), ("iso_z", ctypes.c_float)]
 
 
-def make_field(change_xyz, sid, gantry_deg=0.0, iso=(0.0, 0.0, 0.0)):
+def make_field(change_xyz, sid, gantry_deg=0.0, iso=(0.0, 0.0, 0.0), module_first=M_PHANTOM):
     a = np.float32(gantry_deg * 3.1415926 / 180.0)     # CUDA_EGS.cu:424
-    return Field(int(change_xyz), M_PHANTOM, float(sid), float(np.cos(a, dtype=np.float32)),
+    return Field(int(change_xyz), int(module_first), float(sid), float(np.cos(a, dtype=np.float32)),
                  float(np.sin(a, dtype=np.float32)), *map(float, iso))
 
 
 class Workload:
-    def __init__(self, name, phantom, media_names, field, other_media, source_fn):
+    def __init__(self, name, phantom, media_names, field, other_media, source_fn, mlc=None, leaf_positions=None,
+                 mlc_medium=None):
         self.name = name
+        self.mlc = mlc                        # head.Mlc, fields starting in the MLC
+        self.leaf_positions = leaf_positions  # 120 leaf positions of the field
+        self.mlc_medium = mlc_medium
         self.phantom = phantom
         self.media_names = media_names
         self.field = field
@@ -48,7 +52,7 @@ class Workload:
     @property
     def tables(self):
         if self._tables is None:
-            self._tables = _tables(tuple(self.media_names))
+            self._tables = _tables(tuple(self.media_names), self.mlc_medium)
         return self._tables
 
     @property
@@ -62,8 +66,8 @@ class Workload:
 
 
 @functools.lru_cache(maxsize=8)
-def _tables(names):
-    return hatch.MediaTables(list(names))
+def _tables(names, mlc_medium=None):
+    return hatch.MediaTables(list(names), mlc_medium=mlc_medium)
 
 
 def water(n=60, voxel=0.5, energy=1.25, field_half=5.0):
@@ -82,3 +86,24 @@ def ct(nx=128, ny=128, nz=100, voxel=0.25, seed=2):
     names = [r[0] for r in ph.DEFAULT_RAMP]
     return Workload("ct_128x128x100_6MV", phan, names, fld, ["AIR700ICRU"] * 5,
                     lambda k, s: src.point_source_spectrum(k, 100.0, 10.0, s))
+
+
+PHSP_Z = 16.44          # phase-space plane of the reference's run inputs (Test*.ini, [Phase space])
+
+
+def ct_mlc(gantry_deg=0.0, aperture_seed=3, nx=128, ny=128, nz=100, voxel=0.25, seed=2):
+    """The CT workload behind a 60-pair MLC: a synthetic head phase space at
+    z = 16.44 cm (3% electrons) enters the MLC (synthetic Millennium-like
+    geometry, seeded irregular aperture inside 10 x 10 cm^2), then the
+    phantom.  Regions 1-5: air, jaw W, MLC W, air, air."""
+    from . import head
+    phan = ph.ct_phantom(nx, ny, nz, voxel, seed)
+    sid = 100.0 + 0.30 * ny * voxel
+    fld = make_field(True, sid, gantry_deg, module_first=head.M_MLC)
+    names = [r[0] for r in ph.DEFAULT_RAMP] + ["W700ICRU"]
+    mlc = head.build_mlc(head.millennium_like())
+    pos = head.random_aperture(aperture_seed)
+    return Workload("ct_128x128x100_6MV_mlc", phan, names, fld,
+                    ["AIR700ICRU", "W700ICRU", "W700ICRU", "AIR700ICRU", "AIR700ICRU"],
+                    lambda k, s: src.head_phase_space(k, PHSP_Z, 100.0, 10.0, s),
+                    mlc=mlc, leaf_positions=pos, mlc_medium="W700ICRU")

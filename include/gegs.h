@@ -32,7 +32,27 @@ extern "C" {
 #define GEGS_ERR_STATE 90002         /* call out of order (e.g. run before load) */
 
 /* Module ids (CUDA_EGS.h:396-404). */
+#define GEGS_M_MLC 3
 #define GEGS_M_PHANTOM 6
+
+/* Multileaf collimator (DYNVMLC-type, read_egsinp, init.cu:345-611).  Head
+ * frame: z from the source; the MLC occupies z in [zmin, zmin + zthick].
+ * cross_xz: leaf i's cross-section across the leaves, 14 vertices (x, z) at
+ * [16 i, 16 i + 14), sides focused on the source.  cross_yz: the closed leaf
+ * pair along the leaf travel (y, z): [0,1] the -rmax edge, [2,16] bank A's
+ * rounded end, [17,31] bank B's, [32,33] the +rmax edge (leaf_end,
+ * init.cu:256-343).  The scalars drive the leaf index lookup (indexOfLeaf,
+ * geom.cu:346-410).  Regions in the module: 0 vacuum, 1 air, 3 MLC tungsten. */
+#define GEGS_MLC_LEAVES 60
+typedef struct gegs_mlc {
+    float zmin, zthick, rmax;
+    float x_start, x_end, x_mid_1, x_mid_2;
+    float air_gap, wl_full, wl_target, wl_isocenter, wt;
+    gegs_float2 cross_xz[GEGS_MLC_LEAVES * 16];
+    gegs_float2 boundary_xz[4];
+    gegs_float2 boundary_yz[4];
+    gegs_float2 cross_yz[34];
+} gegs_mlc;
 
 /* voxels per phantom axis accepted by gegs_load_phantom (the reference caps at 128,
    EGSnrc_Parameters.h:140-142) */
@@ -75,7 +95,7 @@ typedef struct gegs_field {
 typedef struct gegs_job {
     gegs_field field;            /* beam geometry of the field */
     uint32_t plan;               /* dose buffer: dose[plan * nvox + voxel] */
-    uint32_t reserved;
+    uint32_t leaf_set;           /* row of gegs_set_leaf_positions (fields starting in the MLC) */
     uint64_t first;              /* first source record */
     uint64_t count;              /* records = histories of the field */
     uint64_t history_offset;     /* Philox history index of record `first` */
@@ -143,8 +163,21 @@ int gegs_load_phantom(gegs_ctx *ctx, uint32_t nx, uint32_t ny, uint32_t nz,
                       const float *x_bounds, const float *y_bounds, const float *z_bounds,
                       const gegs_region *regions);
 
-/* Field geometry (gantry symbols, CUDA_EGS.cu:430-432). */
+/* Field geometry (gantry symbols, CUDA_EGS.cu:430-432).  A field whose
+ * module_first is GEGS_M_MLC starts its particles in the MLC, which needs
+ * gegs_load_mlc and a leaf-position row (row 0 for gegs_run_plain/_event). */
 int gegs_set_field(gegs_ctx *ctx, const gegs_field *field);
+
+/* MLC geometry (init.cu:345-611). */
+int gegs_load_mlc(gegs_ctx *ctx, const gegs_mlc *mlc);
+
+/* Leaf positions of n_sets fields, 120 values per set, in the order of the
+ * .mlcinfo leaf files (init.cu:1480-1500), cm at the isocentre plane: leaf
+ * pair j has its +x end at +pos[j] and its -x end at -pos[j+60] (the
+ * reference's INVERSE_MLC_Bank_AB order, CUDA_EGS.h:27, init.cu:1607-1613);
+ * a pair with pos[j] + pos[j+60] <= 0 stays closed.  The leaf ends follow
+ * init_leaf_pos (init.cu:1594-1662). */
+int gegs_set_leaf_positions(gegs_ctx *ctx, const float *positions, uint32_t n_sets);
 
 /* Source particles (load_stack_from_phspfile, CUDA_EGS.cu:40-158): copied to
  * device memory; histories map 1:1 onto records. */

@@ -86,7 +86,13 @@ typedef struct opart {
     int32_t module;
     uint64_t lin;
     uint32_t blk;
+    int32_t leaf;      /* MLC leaf index, OUTMLC / PASSMLC outside the leaf rows */
+    int32_t bank;      /* MLC bank: 0 opening, 1 A, 2 B, -1 not located */
 } opart;
+
+#define OUTMLC (-99)
+#define PASSMLC (-98)
+#define NLEAF GEGS_MLC_LEAVES
 
 /* ------------------------------------------------------------ Philox */
 static void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
@@ -152,6 +158,8 @@ typedef struct octx {
     const float *xb, *yb, *zb;      /* N+2 entries each */
     uint32_t nx, ny, nz;
     gegs_field f;
+    const gegs_mlc *mlc;            /* NULL: no MLC */
+    const gegs_float2 *endnodes;    /* [60][34] leaf ends of the field */
     uint64_t seed;
     double *dose, *dose2, *hist_edep;
     uint32_t *hist_steps;
@@ -357,8 +365,237 @@ static float hownear_phantom(const octx *c, const opart *p0) {
     return tperp;
 }
 
-static uint32_t howfar(octx *c, const opart *p, float *t) { return howfar_phantom(c, p, t); }
-static float hownear(octx *c, const opart *p) { return hownear_phantom(c, p); }
+
+/* ------------------------------------------------------------ MLC (geom.cu:310-790, :1152-1286) */
+typedef struct { float x, y; } pt2;
+
+static pt2 mk2(float x, float y) { pt2 q; q.x = x; q.y = y; return q; }
+
+/* crossing number of a ray towards -x (geom.cu:310-327) */
+static int in_poly(pt2 p, const gegs_float2 *v, int n) {
+    int odd = 0, j = n - 1;
+    for (int i = 0; i < n; ++i) {
+        if (((v[i].y < p.y && v[j].y >= p.y) || (v[j].y < p.y && v[i].y >= p.y)) && (v[i].x <= p.x || v[j].x <= p.x)) {
+            if (v[i].x + (p.y - v[i].y) / (v[j].y - v[i].y) * (v[j].x - v[i].x) < p.x) odd = !odd;
+        }
+        j = i;
+    }
+    return odd;
+}
+
+static int leaf_bank(pt2 p, const gegs_float2 *en) {
+    if (in_poly(p, en, 17)) return 1;
+    if (in_poly(p, en + 17, 17)) return 2;
+    return 0;
+}
+
+/* leaf index of a point (x across the leaves, y = z), geom.cu:346-410 */
+static int leaf_index(const gegs_mlc *m, pt2 p) {
+    float x = p.x / p.y * m->zmin;
+    float z = p.y - m->zmin;
+    if (z < 0.0f || z > m->zthick) return OUTMLC;
+    if (x < m->x_start || x > m->x_end) return OUTMLC;
+    int i = OUTMLC;
+    if (x < m->x_start + m->wt) i = 0;
+    else if (x < m->x_mid_1 + m->wt) i = (int)((x - m->x_start - m->wt) / (m->wl_full + m->air_gap));
+    else if (x < m->x_mid_2 + m->wt) {
+        float pr = m->wl_target + m->wl_isocenter + m->air_gap * 2.0f;
+        i = (int)((x - m->x_mid_1 - m->wt) / pr);
+        i = (x - m->x_mid_1 - m->wt - pr * (float)i < m->wl_target + m->air_gap) ? 10 + i * 2 : 11 + i * 2;
+    } else if (x < m->x_end - m->wl_full) i = (int)((x - m->x_mid_2 - m->wt) / (m->wl_full + m->air_gap)) + 50;
+    else if (x <= m->x_end) i = 58;
+    if (i < 0) return OUTMLC;
+    if (in_poly(p, m->cross_xz + i * 16, 14)) return i;
+    if (i < NLEAF - 1) return in_poly(p, m->cross_xz + (i + 1) * 16, 14) ? i + 1 : -(i + 1);
+    return OUTMLC;
+}
+
+/* nearest edge along v (in units of |v|) and inside flag (geom.cu:442-484) */
+static float ray_poly(pt2 o, pt2 v, const gegs_float2 *q, int n, int *inside) {
+    float best = INFI_DIST + 1.0f;
+    int hits = 0;
+    for (int i = 0; i < n; ++i) {
+        int j = i + 1 < n ? i + 1 : 0;
+        float den = v.x * (q[j].y - q[i].y) - v.y * (q[j].x - q[i].x);
+        if (den != 0.0f) {
+            float d = ((q[j].x - q[i].x) * (o.y - q[i].y) - (q[j].y - q[i].y) * (o.x - q[i].x)) / den;
+            float u = ((q[i].x - o.x) * v.y - (q[i].y - o.y) * v.x) / den;
+            if (d >= 0.0f && u >= 0.0f && u < 1.0f) {
+                hits++;
+                if (d < best) best = d;
+            }
+        }
+    }
+    *inside = hits & 1;
+    return best;
+}
+
+static float edge_dist(pt2 p, gegs_float2 v, gegs_float2 w) {
+    float l2 = (v.x - w.x) * (v.x - w.x) + (v.y - w.y) * (v.y - w.y);
+    if (l2 == 0.0f) return sqrtf((p.x - v.x) * (p.x - v.x) + (p.y - v.y) * (p.y - v.y));
+    float t = ((p.x - v.x) * (w.x - v.x) + (p.y - v.y) * (w.y - v.y)) / l2;
+    if (t < 0.0f) return sqrtf((p.x - v.x) * (p.x - v.x) + (p.y - v.y) * (p.y - v.y));
+    if (t > 1.0f) return sqrtf((p.x - w.x) * (p.x - w.x) + (p.y - w.y) * (p.y - w.y));
+    float ax = v.x + t * (w.x - v.x), ay = v.y + t * (w.y - v.y);
+    return sqrtf((p.x - ax) * (p.x - ax) + (p.y - ay) * (p.y - ay));
+}
+
+static float poly_near(pt2 p, const gegs_float2 *q, int n) {
+    float best = 1e9f;
+    for (int i = 0; i < n; ++i) {
+        float d = edge_dist(p, q[i], q[i + 1 < n ? i + 1 : 0]);
+        if (d < best) best = d;
+    }
+    return best;
+}
+
+static const gegs_float2 *bank_polygon(const gegs_float2 *en, int bank, int *n) {
+    if (bank == 0) { *n = 30; return en + 2; }
+    *n = 17;
+    return bank == 2 ? en + 17 : en;
+}
+
+/* region of a point: 3 (MLC W) inside a leaf, 1 (air) otherwise (geom.cu:486-516) */
+static uint32_t mlc_where(const gegs_mlc *m, const gegs_float2 *ends, float qx, float qy, float qz, int *index, int *bank) {
+    uint32_t r = 1u;
+    if (*index == OUTMLC) *index = leaf_index(m, mk2(qx, qz));
+    if (*index >= 0) {
+        *bank = leaf_bank(mk2(qy, qz), ends + *index * 34);
+        r = *bank > 0 ? 3u : 1u;
+    }
+    return r;
+}
+
+/* howfar_MLC, geom.cu:518-790 */
+static uint32_t howfar_mlc(const octx *c, opart *p, float *t) {
+    const gegs_mlc *m = c->mlc;
+    const gegs_float2 *ends = c->endnodes;
+    uint32_t inew = p->region;
+    int bank = p->bank, index = p->leaf;
+    int lo = 0, hi = NLEAF, in1, in2;
+    if (p->y < -m->rmax || p->y > m->rmax) return 0u;
+    pt2 a = mk2(p->x, p->z), b = mk2(p->y, p->z), va = mk2(p->u, p->w), vb = mk2(p->v, p->w);
+    float dxz = ray_poly(a, va, m->boundary_xz, 4, &in1);
+    float dyz = ray_poly(b, vb, m->boundary_yz, 4, &in2);
+    if (dxz >= INFI_DIST || dyz >= INFI_DIST) return 0u;
+    float bound = dxz < dyz ? dxz : dyz;
+    int jump = 0;
+    if (in1 && in2) {
+        pt2 e = mk2(a.x + dxz * va.x, a.y + (dxz - 0.001f) * va.y);
+        lo = leaf_index(m, e);
+        if (lo < 0) lo = -lo;
+    } else {
+        inew = 0u;
+        if (bound >= INFI_DIST) return inew;
+        if (*t > bound) {
+            *t = bound + EPSTEP;
+            jump = 1;
+        } else {
+            return inew;
+        }
+    }
+    if (!jump) {
+        index = p->leaf > -NLEAF ? p->leaf : leaf_index(m, a);
+        if (index >= 0) {
+            int n;
+            dxz = ray_poly(a, va, m->cross_xz + index * 16, 14, &in1);
+            const gegs_float2 *yz = bank_polygon(ends + index * 34, p->bank, &n);
+            dyz = ray_poly(b, vb, yz, n, &in2);
+            if (dyz > dxz) {
+                if (va.x < 0.0f && index > 0) index = -index;
+                else if (index < NLEAF - 1) index = -(index + 1);
+            }
+        } else if (index > -NLEAF) {
+            int sx = 1, i;
+            if (va.x < 0.0f) {
+                sx = -1;
+                if (lo > 0) lo -= 1;
+                hi = -index + 1;
+            } else {
+                hi = lo;
+                lo = -index - 1;
+            }
+            i = -index - sx;
+            if (lo < 0) lo = 0;
+            if (hi > NLEAF - 1) hi = NLEAF - 1;
+            if (i < 0) i = 0;
+            if (i > NLEAF - 1) i = NLEAF - 1;
+            float hit;
+            do {
+                hit = ray_poly(a, va, m->cross_xz + i * 16, 14, &in1);
+                i += sx;
+            } while (i >= lo && i <= hi && hit >= INFI_DIST);
+            if (hit < dxz) {
+                dxz = hit;
+                index = i - sx;
+            } else {
+                index = PASSMLC;
+                p->module = GEGS_M_PHANTOM;
+            }
+        }
+        float dm = dxz < dyz ? dxz : dyz;
+        int out = 0;
+        if (dm + EPSTEP >= bound) {
+            dm = bound;
+            out = 1;
+        }
+        if (*t > dm) {
+            *t = dm + EPSTEP;
+            if (out) {
+                p->leaf = PASSMLC;
+                p->module = GEGS_M_PHANTOM;
+                return 1u;
+            }
+        } else {
+            return inew;
+        }
+    }
+    inew = mlc_where(m, ends, p->x + *t * p->u, p->y + *t * p->v, p->z + *t * p->w, &index, &bank);
+    p->leaf = index;
+    p->bank = bank;
+    return inew;
+}
+
+/* hownear_MLC, geom.cu:1193-1286 */
+static float hownear_mlc(const octx *c, opart *p) {
+    const gegs_mlc *m = c->mlc;
+    const gegs_float2 *ends = c->endnodes;
+    pt2 a = mk2(p->x, p->z), b = mk2(p->y, p->z);
+    float txz = INFI_DIST + 1.0f, tyz = INFI_DIST + 1.0f;
+    int in1 = 1, in2 = 1;
+    if (p->leaf < -NLEAF) {
+        in1 = in_poly(a, m->boundary_xz, 4);
+        in2 = in_poly(b, m->boundary_yz, 4);
+    }
+    float bxz = poly_near(a, m->boundary_xz, 4), byz = poly_near(b, m->boundary_yz, 4);
+    float bound = bxz < byz ? bxz : byz;
+    if (!(in1 && in2)) return bound;
+    int index;
+    if (p->leaf > PASSMLC) index = p->leaf;
+    else {
+        index = leaf_index(m, a);
+        p->leaf = index;
+    }
+    if (index >= 0) {
+        int n;
+        txz = poly_near(a, m->cross_xz + index * 16, 14);
+        if (p->bank < 0) p->bank = leaf_bank(b, ends + index * 34);
+        const gegs_float2 *yz = bank_polygon(ends + index * 34, p->bank, &n);
+        tyz = poly_near(b, yz, n);
+    } else if (index > -NLEAF) {
+        int i = -index;
+        float u = poly_near(a, m->cross_xz + i * 16, 14);
+        txz = poly_near(a, m->cross_xz + (i - 1) * 16, 14);
+        if (u < txz) txz = u;
+    }
+    return txz < tyz ? txz : tyz;
+}
+
+static uint32_t howfar(octx *c, opart *p, float *t) {
+    return p->module == GEGS_M_MLC ? howfar_mlc(c, p, t) : howfar_phantom(c, p, t);
+}
+static float hownear(octx *c, opart *p) { return p->module == GEGS_M_MLC ? hownear_mlc(c, p) : hownear_phantom(c, p); }
+
 
 /* ------------------------------------------------------------ uphi (utility.cu:94-218) */
 static void rotate(float costhe, float sinthe, float cosphi, float sinphi, opart *p) {
@@ -462,7 +699,7 @@ static void photon_step(octx *c, ostream *s, opart *p) {
         }
         uint32_t new_region = howfar(c, p, &tstep);
         int idisc = 0;
-        if (new_region == 0) idisc = (rd.med == GEGS_VACUUM) ? 1 : -1;
+        if (new_region == 0) idisc = (rd.med == GEGS_VACUUM || p->leaf == PASSMLC) ? 1 : -1;
         if (idisc > 0) { c->escaped += (double)p->e * (double)p->wt; p->region = 0; return; }
         p->x += p->u * tstep;
         p->y += p->v * tstep;
@@ -1130,7 +1367,7 @@ static void electron_step(octx *c, ostream *s, opart *p) {
         int idisc = 0;
         float ustep0 = ustep;
         if (callhowfar || p->wt <= 0.0f) irnew = howfar(c, p, &ustep);
-        if (irnew == 0) idisc = (rd.med == GEGS_VACUUM) ? 1 : -1;
+        if (irnew == 0) idisc = (rd.med == GEGS_VACUUM || p->leaf == PASSMLC) ? 1 : -1;
         if (idisc > 0) {
             c->escaped += ((double)p->e + (double)p->charge * PRM) * (double)p->wt;
             p->region = 0;
@@ -1597,6 +1834,8 @@ static void run_range(octx *c, const gegs_source *src, uint64_t i0, uint64_t i1,
         p.charge = src[i].charge;
         p.region = 0;
         p.module = c->f.module_first;
+        p.leaf = OUTMLC;
+        p.bank = -1;
         p.dnear = INFI_DIST;
         p.lin = oracle_primary_lineage(c->hist);
         p.blk = 0;
@@ -1640,13 +1879,51 @@ static void run_range(octx *c, const gegs_source *src, uint64_t i0, uint64_t i1,
  * histories into contiguous ranges, one per OpenMP thread, each with its
  * own tallies, summed at the end; per-history results do not depend on it.
  */
+/* leaf ends of a field (init_leaf_pos, init.cu:1594-1662): an open pair's
+ * bank A end moves to -pos[j+60], bank B's to pos[j], scaled from the
+ * isocentre plane to the leaf centre plane */
+void oracle_leaf_ends(const gegs_mlc *m, const float *pos, float sid, gegs_float2 *out) {
+    float zc = m->zmin + m->zthick / 2.0f;
+    for (int j = 0; j < NLEAF; ++j) {
+        gegs_float2 *en = out + j * 34;
+        for (int i = 0; i < 34; ++i) en[i] = m->cross_yz[i];
+        if (pos[j] + pos[j + NLEAF] > 0.0f) {
+            float a = -pos[j + NLEAF], b = pos[j];
+            for (int i = 2; i < 17; ++i) en[i].x = m->cross_yz[i].x + a * zc / sid;
+            for (int i = 17; i < 32; ++i) en[i].x = m->cross_yz[i].x + b * zc / sid;
+        }
+    }
+}
+
+/* MLC probes for the geometry tests */
+int oracle_leaf_index(const gegs_mlc *m, float x, float z) { return leaf_index(m, mk2(x, z)); }
+uint32_t oracle_howfar_mlc(const gegs_mlc *m, const gegs_float2 *ends, const gegs_particle *in, int32_t *leaf,
+                           int32_t *bank, int32_t *module, float *t) {
+    octx c;
+    memset(&c, 0, sizeof(c));
+    c.mlc = m;
+    c.endnodes = ends;
+    opart p;
+    memset(&p, 0, sizeof(p));
+    p.x = in->x; p.y = in->y; p.z = in->z; p.u = in->u; p.v = in->v; p.w = in->w;
+    p.region = in->region; p.leaf = *leaf; p.bank = *bank; p.module = GEGS_M_MLC;
+    uint32_t r = howfar_mlc(&c, &p, t);
+    *leaf = p.leaf; *bank = p.bank; *module = p.module;
+    return r;
+}
+uint32_t oracle_where_mlc(const gegs_mlc *m, const gegs_float2 *ends, float x, float y, float z) {
+    int index = OUTMLC, bank = -1;
+    return mlc_where(m, ends, x, y, z, &index, &bank);
+}
+
 int oracle_run_mt(const gegs_media_tables *mt, const gegs_region *regions,
                   uint32_t nx, uint32_t ny, uint32_t nz,
                   const float *xb, const float *yb, const float *zb,
                   const gegs_field *field, const gegs_source *src, uint64_t n,
                   uint64_t hist_offset, uint64_t seed,
                   double *dose, double *dose2, double *hist_edep, uint32_t *hist_steps,
-                  gegs_counters *counters, double *balance, int nthreads) {
+                  gegs_counters *counters, double *balance, int nthreads,
+                  const gegs_mlc *mlc, const gegs_float2 *endnodes) {
     size_t nvox = (size_t)nx * ny * nz;
     if (nthreads < 1) nthreads = 1;
     if ((uint64_t)nthreads > n && n > 0) nthreads = (int)n;
@@ -1658,6 +1935,7 @@ int oracle_run_mt(const gegs_media_tables *mt, const gegs_region *regions,
         if (!c) { ok = 0; continue; }
         c->mt = mt; c->reg = regions; c->xb = xb; c->yb = yb; c->zb = zb;
         c->nx = nx; c->ny = ny; c->nz = nz; c->f = *field; c->seed = seed;
+        c->mlc = mlc; c->endnodes = endnodes;
         c->dose = t == 0 ? dose : (double *)calloc(nvox, sizeof(double));
         c->dose2 = t == 0 ? dose2 : (double *)calloc(nvox, sizeof(double));
         c->tmp = (double *)calloc(nvox, sizeof(double));
@@ -1710,7 +1988,7 @@ int oracle_run(const gegs_media_tables *mt, const gegs_region *regions,
                double *dose, double *dose2, double *hist_edep, uint32_t *hist_steps,
                gegs_counters *counters, double *balance) {
     return oracle_run_mt(mt, regions, nx, ny, nz, xb, yb, zb, field, src, n, hist_offset, seed,
-                         dose, dose2, hist_edep, hist_steps, counters, balance, 1);
+                         dose, dose2, hist_edep, hist_steps, counters, balance, 1, NULL, NULL);
 }
 
 /* Single-sampler entry point for uniform-for-uniform checks: the handler of
@@ -1736,6 +2014,7 @@ int oracle_interaction(const gegs_media_tables *mt, const gegs_region *regions,
     p.e = in->e; p.x = in->x; p.y = in->y; p.z = in->z;
     p.u = in->u; p.v = in->v; p.w = in->w; p.wt = in->wt; p.dnear = in->dnear;
     p.charge = in->charge; p.region = in->region; p.module = field->module_first;
+    p.leaf = OUTMLC; p.bank = -1;
     p.lin = in->lin; p.blk = in->blk; p.status = in->status;
     ostream s;
     st_begin(&s, &p, seed);

@@ -43,7 +43,14 @@ def lib():
                                  vp, vp, vp, vp, vp, ctypes.c_uint64, ctypes.c_uint64,
                                  ctypes.c_uint64, vp, vp, vp, vp, vp, vp]
         L.oracle_run_mt.restype = ctypes.c_int
-        L.oracle_run_mt.argtypes = L.oracle_run.argtypes + [ctypes.c_int]
+        L.oracle_run_mt.argtypes = L.oracle_run.argtypes + [ctypes.c_int, vp, vp]
+        L.oracle_leaf_ends.argtypes = [vp, vp, ctypes.c_float, vp]
+        L.oracle_leaf_index.restype = ctypes.c_int
+        L.oracle_leaf_index.argtypes = [vp, ctypes.c_float, ctypes.c_float]
+        L.oracle_howfar_mlc.restype = ctypes.c_uint32
+        L.oracle_howfar_mlc.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.oracle_where_mlc.restype = ctypes.c_uint32
+        L.oracle_where_mlc.argtypes = [vp, vp, ctypes.c_float, ctypes.c_float, ctypes.c_float]
         L.oracle_interaction.restype = ctypes.c_int
         L.oracle_interaction.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                          vp, vp, vp, vp, vp, ctypes.c_uint64, vp, ctypes.c_int, vp]
@@ -85,14 +92,47 @@ def run(workload, source, seed, hist_offset=0, per_history=False, threads=1):
     cnt = Counters()
     bal = np.zeros(2, dtype=np.float64)
     reg = np.ascontiguousarray(workload.regions)
+    mlc_ptr, ends_ptr, ends = None, None, None
+    if workload.field.module_first == 3:          # the field starts in the MLC
+        mlc_ptr = ctypes.c_void_p(ctypes.addressof(workload.mlc))
+        ends = leaf_ends(workload.mlc, workload.leaf_positions, workload.field.source_to_isocenter)
+        ends_ptr = _p(ends)
     rc = L.oracle_run_mt(ctypes.c_void_p(workload.tables.ptr()), _p(reg), phan.nx, phan.ny, phan.nz,
                          _p(xb), _p(yb), _p(zb), ctypes.c_void_p(ctypes.addressof(workload.field)),
                          _p(src), n, hist_offset, seed, _p(dose), _p(dose2), _p(edep), _p(steps),
-                         ctypes.c_void_p(ctypes.addressof(cnt)), _p(bal), int(threads))
+                         ctypes.c_void_p(ctypes.addressof(cnt)), _p(bal), int(threads), mlc_ptr, ends_ptr)
     if rc != 0:
         raise RuntimeError("oracle_run failed: %d" % rc)
     return dict(dose=dose, dose2=dose2, edep=edep, steps=steps, counters=cnt,
                 escaped=float(bal[0]), dropped=float(bal[1]))
+
+
+def leaf_ends(mlc, positions, sid):
+    """[60, 34, 2] float32 leaf ends of one leaf set (120 positions)."""
+    pos = np.ascontiguousarray(positions, dtype=np.float32)
+    out = np.zeros((60, 34, 2), dtype=np.float32)
+    lib().oracle_leaf_ends(ctypes.c_void_p(ctypes.addressof(mlc)), _p(pos), float(sid), _p(out))
+    return out
+
+
+def leaf_index(mlc, x, z):
+    return lib().oracle_leaf_index(ctypes.c_void_p(ctypes.addressof(mlc)), float(x), float(z))
+
+
+def where_mlc(mlc, ends, x, y, z):
+    return lib().oracle_where_mlc(ctypes.c_void_p(ctypes.addressof(mlc)), _p(ends), float(x), float(y), float(z))
+
+
+def howfar_mlc(mlc, ends, particle, leaf, bank, t):
+    """(region, t, leaf, bank, module) of the oracle's howfar_MLC."""
+    part = np.ascontiguousarray(np.array([particle]).astype(particle.dtype))
+    lf = ctypes.c_int32(leaf)
+    bk = ctypes.c_int32(bank)
+    md = ctypes.c_int32(0)
+    tt = ctypes.c_float(t)
+    r = lib().oracle_howfar_mlc(ctypes.c_void_p(ctypes.addressof(mlc)), _p(ends), _p(part), ctypes.byref(lf),
+                                ctypes.byref(bk), ctypes.byref(md), ctypes.byref(tt))
+    return r, tt.value, lf.value, bk.value, md.value
 
 
 def mscat(tables, lam, chia2, q1, n, seed=1, lin0=1):

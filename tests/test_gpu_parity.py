@@ -211,3 +211,24 @@ def test_batched_fields_and_plans_match_per_field_runs(gpu_ctx, ct_small):
     wl90.field = fields[1]
     o = oracle.run(wl90, src[3000:3500], seed=11, hist_offset=3000, per_history=True)
     assert np.array_equal(bt["steps"][3000:3500], o["steps"])
+
+
+def test_kernel_timing_leaves_results_unchanged(gpu_ctx, ct_small):
+    """Timing the photon kernel with event nodes inside the replayed graphs
+    changes nothing in the transport and reports one duration per launch."""
+    src = ct_small.source(20000, 8)
+    gpu_ctx.load_workload(ct_small)
+    gpu_ctx.set_source(src)
+    ref = gpu_ctx.run(seed=3, n_batches=1, method="event", per_history=True)
+    gpu_ctx.set_kernel_timing(True)
+    try:
+        t = gpu_ctx.run(seed=3, n_batches=1, method="event", per_history=True)
+        ms, launches = gpu_ctx.kernel_timing()
+    finally:
+        gpu_ctx.set_kernel_timing(False)
+    assert np.array_equal(t["steps"], ref["steps"])
+    assert np.array_equal(t["edep"], ref["edep"])
+    sweeps_per_graph = gpu_ctx.get_tuning()["sweeps_per_graph"]
+    assert launches > 0 and launches % sweeps_per_graph == 0
+    # the photon kernel is part of the run bracketed by the run's own events
+    assert 0.0 < ms <= 1.02 * t["ms"] + 0.5

@@ -1,0 +1,138 @@
+"""MLC geometry of the oracle (restatement of geom.cu:310-790, :1152-1286)
+against a brute-force model of the same solid: a point is tungsten when its
+(x, z) lies in a leaf's cross-section and its (y, z) in that leaf pair's bank
+A or bank B end polygon; in the slab otherwise it is air."""
+import os
+
+import numpy as np
+import pytest
+
+from gegs_amd import head, native
+from oracle import oracle
+
+
+def inside(poly, px, py):
+    """Even-odd rule, vectorised over points (numpy restatement)."""
+    x, y = poly[:, 0], poly[:, 1]
+    xj, yj = np.roll(x, 1), np.roll(y, 1)
+    px = np.asarray(px, dtype=np.float64)[..., None]
+    py = np.asarray(py, dtype=np.float64)[..., None]
+    cond = (y > py) != (yj > py)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        xc = (xj - x) * (py - y) / (yj - y) + x
+    return np.sum(cond & (px < xc), axis=-1) % 2 == 1
+
+
+@pytest.fixture(scope="module")
+def mlc():
+    return head.build_mlc(head.millennium_like())
+
+
+def brute_region(m, ends, x, y, z):
+    cx = m.cross()
+    if z < m.zmin or z > m.zmin + m.zthick:
+        return 1
+    for i in range(60):
+        if inside(cx[i, :14].astype(np.float64), x, z):
+            en = ends[i].astype(np.float64)
+            if inside(en[:17], y, z) or inside(en[17:], y, z):
+                return 3
+            return 1
+    return 1
+
+
+def test_leaf_index_matches_polygon_membership(mlc):
+    rng = np.random.default_rng(1)
+    cx = mlc.cross()
+    n_checked = 0
+    for _ in range(3000):
+        z = rng.uniform(mlc.zmin, mlc.zmin + mlc.zthick)
+        x = rng.uniform(mlc.x_start, mlc.x_end) * z / mlc.zmin
+        k = oracle.leaf_index(mlc, x, z)
+        members = [i for i in range(60) if inside(cx[i, :14].astype(np.float64), x, z)]
+        if k >= 0:
+            assert k in members, (x, z, k, members)
+            n_checked += 1
+        elif k > -60:
+            # in the gap next to leaf -k-1 / -k: neither candidate holds it
+            assert (-k - 1) not in members and (-k) not in members, (x, z, k, members)
+    assert n_checked > 1000
+
+
+def test_region_lookup_matches_brute_force(mlc):
+    pos = head.random_aperture(7)
+    ends = oracle.leaf_ends(mlc, pos, 100.0)
+    rng = np.random.default_rng(2)
+    for _ in range(1500):
+        z = rng.uniform(mlc.zmin, mlc.zmin + mlc.zthick)
+        x = rng.uniform(-0.25, 0.25) * z
+        y = rng.uniform(-0.1, 0.1) * z
+        assert oracle.where_mlc(mlc, ends, x, y, z) == brute_region(mlc, ends, x, y, z), (x, y, z)
+
+
+def trace(mlc, ends, x, y, z, u, v, w, max_steps=200):
+    """Follow a ray through the MLC with repeated howfar calls (no
+    interactions) as the transport does; returns [(s0, s1, region), ...]."""
+    p = np.zeros(1, dtype=native.PARTICLE_DTYPE)[0]
+    p["x"], p["y"], p["z"], p["u"], p["v"], p["w"] = x, y, z, u, v, w
+    p["region"] = 1
+    leaf, bank, s, segs = -99, -1, 0.0, []
+    for _ in range(max_steps):
+        r, t, leaf, bank, module = oracle.howfar_mlc(mlc, ends, p, leaf, bank, 1e8)
+        if t >= 1e7 or r == 0:
+            break
+        segs.append((s, s + t, int(p["region"])))
+        s += t
+        p["x"] += np.float32(u * t)
+        p["y"] += np.float32(v * t)
+        p["z"] += np.float32(w * t)
+        p["region"] = r
+        if module != 3:
+            break
+    return segs
+
+
+def test_howfar_boundaries_match_brute_force(mlc):
+    """Every segment between two howfar boundaries is homogeneous in the
+    brute-force model (no boundary missed), and tungsten segments are
+    reported as tungsten."""
+    pos = head.random_aperture(11)
+    ends = oracle.leaf_ends(mlc, pos, 100.0)
+    rng = np.random.default_rng(3)
+    n_w = 0
+    for _ in range(150):
+        # rays from the source towards the 10x10 cm field at 100 cm, some beyond it
+        ax, ay = rng.uniform(-7.0, 7.0, 2)
+        d = np.array([ax, ay, 100.0])
+        d /= np.linalg.norm(d)
+        segs = trace(mlc, ends, 0.0, 0.0, 10.0, *d)
+        for s0, s1, reg in segs:
+            if s1 - s0 < 4e-3:
+                continue
+            for f in (0.25, 0.5, 0.75):
+                s = s0 + 1e-3 + f * (s1 - s0 - 2e-3)
+                q = np.array([0.0, 0.0, 10.0]) + s * d
+                if q[2] < mlc.zmin or q[2] > mlc.zmin + mlc.zthick:
+                    continue
+                assert brute_region(mlc, ends, *q) == reg, (d, s0, s1, reg, q)
+                n_w += reg == 3
+    assert n_w > 50
+
+
+def test_mlc_text_inputs_roundtrip(tmp_path, mlc):
+    d = head.millennium_like()
+    path = os.path.join(tmp_path, "mlc.egsinp")
+    head.write_dynvmlc(path, d)
+    d2 = head.read_dynvmlc(path)
+    m2 = head.build_mlc(d2)
+    assert np.array_equal(m2.cross(), mlc.cross())
+    assert d2.medium == d.medium and d2.rad_end == d.rad_end
+    pos = head.random_aperture(5)
+    lf = os.path.join(tmp_path, "f1.mlc")
+    head.write_leaf_file(lf, pos)
+    assert np.allclose(head.read_leaf_file(lf), pos, atol=1e-4)
+    info = os.path.join(tmp_path, "plan.mlcinfo")
+    fields = [head.MlcField(a, -5, 5, -5, 5, 100.0 + a, "f%d.mlc" % i) for i, a in enumerate((0, 72, 144))]
+    head.write_mlcinfo(info, fields)
+    back = head.read_mlcinfo(info)
+    assert [f.angle for f in back] == [0, 72, 144] and back[2].leaf_file == "f2.mlc"
