@@ -4,7 +4,10 @@
 Workload: the heterogeneous-CT case -- seeded synthetic CT of 128x128x100
 voxels at 2.5 mm (air / lung / ICRU tissue / ICRP bone through the ctcreate
 ramp), point source at SSD 100 cm, 10x10 cm^2 field, 24-bin 6 MV-like
-spectrum; 1e7 histories by default (10 steps of 1e6).
+spectrum; 1e7 histories by default (10 steps of 1e6).  Beside it, the
+5-field MLC plan (gantry 0/72/144/216/288 degrees, seeded apertures, uneven
+field weights, 2e7 phase-space records with 3% electrons) as five plain
+launches per step against one batched event loop per step.
 
 A step transports one batch of 1e6 source records already resident in HBM,
 once with the plain per-history kernel (the baseline method) and once with
@@ -17,6 +20,7 @@ timed live with HIP events), the adjoint gradient rate of a 5-field,
 bounded sample.
 
   python bench.py [--gpus 1] [--steps 10] [--warmup 1] [--histories-per-step 1000000]
+                  [--mlc-plan-histories 20000000] [--adjoint-histories 10000000] [--tuning JSON]
 """
 from __future__ import annotations
 
@@ -43,6 +47,19 @@ ROWS_PER_ELECTRON_STEP = 6
 # photon flight kernel: the region record of the voxel and the three photon
 # rows, per photon step
 PHOTON_STEP_BYTES = REGION_BYTES + ROWS_PER_PHOTON_STEP * TABLE_ROW_BYTES
+
+
+def measured_traffic_per_photon_step():
+    """HBM bytes per photon step of the photon kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", "photon_kernel_ct128_event.md")
+    try:
+        with open(path) as f:
+            m = re.search(r"^hbm_bytes_per_photon_step: ([0-9.]+)$", f.read(), re.M)
+    except OSError:
+        return None
+    return float(m.group(1)) if m else None
 
 
 def algorithmic_bytes(counters, n_hist):
@@ -118,6 +135,80 @@ def run_method(ctx, method, steps, warmup, b, seed, kernel_timing=False):
     return elapsed, step_doses, counters, timing
 
 
+def plan_sigma(dose, dose2, n_batches):
+    """Mean relative uncertainty over voxels above 50% of Dmax of a dose
+    summed over n_batches batches (dose2 = sum of squared batch doses)."""
+    var = np.maximum(dose2 - dose * dose / n_batches, 0.0) * n_batches / (n_batches - 1)
+    sel = dose > 0.5 * dose.max()
+    return float(np.mean(np.sqrt(var[sel]) / dose[sel])), int(sel.sum()), var
+
+
+def mlc_plan_bench(n_total, n_batches, seed, autotune_first=True, fixed_tuning=None):
+    """The 5-field MLC plan, each method through its own interface: the
+    plain kernel runs field after field (its leaf set and gantry loaded, one
+    run of n_batches batches per field); the batched path runs the five
+    fields in one event loop whose every batch holds 1/n_batches of every
+    field.  Both transport the same histories with the same random streams,
+    so the plan dose is the same; the uncertainty comes from each run's
+    batch variance (summed over the fields for the plain runs)."""
+    from gegs_amd import autotune, native, workloads
+    plan = workloads.mlc_plan()
+    src = plan.source(n_total, 3)
+    ctx = native.Context(0)
+    ctx.load_workload(plan.workload)
+    ctx.set_source(src)
+    tuning = fixed_tuning
+    if fixed_tuning:
+        ctx.set_tuning(fixed_tuning)
+    elif autotune_first:
+        ctx.set_leaf_positions(plan.leaf_positions[0])
+        ctx.set_field(plan.fields[0])
+        tuning, _, _ = autotune.autotune(ctx, 0, min(int(plan.counts(n_total)[0]), 500_000), seed=seed)
+    jobs = plan.jobs(n_total)
+    out = {}
+    # warm-up: graph capture and first launches on a small slice
+    ctx.set_leaf_positions(plan.leaf_positions)
+    ctx.run_batched(plan.jobs(n_total, 0, 100), 1, seed=seed, n_batches=1)
+    ctx.set_leaf_positions(plan.leaf_positions[0])
+    ctx.set_field(plan.fields[0])
+    ctx.run(seed=seed, n_batches=1, method="plain", first=0, count=min(100_000, n_total))
+    for method in ("plain", "batched"):
+        _sync()
+        t0 = time.perf_counter()
+        if method == "batched":
+            ctx.set_leaf_positions(plan.leaf_positions)
+            r = ctx.run_batched(jobs, 1, seed=seed, n_batches=n_batches)
+            dose, var = r["dose"][0], None
+            sig, nsel, _ = plan_sigma(dose, r["dose2"][0], n_batches)
+            counters = r["counters"]
+        else:
+            dose, var, counters = None, None, None
+            for jb in jobs:
+                ctx.set_leaf_positions(plan.leaf_positions[jb["leaf_set"]])
+                ctx.set_field(jb["field"])
+                r = ctx.run(seed=seed, n_batches=n_batches, method="plain", first=jb["first"], count=jb["count"])
+                _, _, v = plan_sigma(r["dose"], r["dose2"], n_batches)
+                dose = r["dose"] if dose is None else dose + r["dose"]
+                var = v if var is None else var + v
+                counters = add_counters(counters, r["counters"])
+        _sync()
+        el = time.perf_counter() - t0
+        if method == "plain":
+            sel = dose > 0.5 * dose.max()
+            sig, nsel = float(np.mean(np.sqrt(var[sel]) / dose[sel])), int(sel.sum())
+        out[method] = dict(histories_per_s=n_total / el, seconds=el, mean_rel_sigma=sig, voxels=nsel,
+                           dose_efficiency=1.0 / (sig * sig * el), photon_steps=counters["photon_steps"],
+                           electron_steps=counters["electron_steps"], dose_sum=float(dose.sum()))
+    ctx.close()
+    out["batched_vs_plain_efficiency"] = out["batched"]["dose_efficiency"] / out["plain"]["dose_efficiency"]
+    out["batched_vs_plain_time"] = out["plain"]["seconds"] / out["batched"]["seconds"]
+    out.update(workload="ct_128x128x100_mlc_5field_gantry_0_72_144_216_288", histories=n_total,
+               n_batches=n_batches, field_weights=[float(x) for x in plan.weights], tuning=tuning,
+               source="egsphsp MODE0 list decoded at z = 16.44 cm, %.1f%% electrons"
+                      % (100.0 * float(np.mean(src["charge"] != 0))))
+    return out
+
+
 def adjoint_rate(wl, n_hist, seed, repeat=2):
     """Gradients/s of the beamlet adjoint: 5 fields (gantry 0/72/144/216/288)
     x 20x20 beamlets of 5 mm, one gradient = one replay of n_hist histories."""
@@ -155,8 +246,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tuning", default=None,
+                    help="JSON dict of event-path tuning fields to use instead of autotuning")
     ap.add_argument("--adjoint-histories", type=int, default=10_000_000,
                     help="histories per adjoint gradient (0: skip)")
+    ap.add_argument("--mlc-plan-histories", type=int, default=20_000_000,
+                    help="histories of the 5-field MLC plan comparison (0: skip)")
     ap.add_argument("--seed", type=int, default=20240)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (counter calibration)")
@@ -186,7 +281,10 @@ def main():
 
     methods = [m.strip() for m in args.methods.split(",") if m.strip()]
     tuning = None
-    if "event" in methods and not args.no_autotune:
+    if args.tuning:
+        tuning = json.loads(args.tuning)
+        ctx.set_tuning(tuning)
+    elif "event" in methods and not args.no_autotune:
         # tuned on the warmup records, before the timed region
         tuning, _, _ = autotune.autotune(ctx, 0, min(b, 500_000), seed=args.seed)
     results = {}
@@ -204,7 +302,9 @@ def main():
             if timing and timing[1]:
                 photon_ms, launches = timing
                 pbytes = PHOTON_STEP_BYTES * cnt2["photon_steps"]
+                per_step = measured_traffic_per_photon_step()
                 res["roofline"] = dict(kernel="photon_kernel", launches=launches,
+                                       traffic=(per_step * cnt2["photon_steps"] / launches) if per_step else None,
                                        avg_launch_us=photon_ms / launches * 1e3,
                                        algorithmic_bytes_per_launch=pbytes / launches,
                                        achieved=pbytes / (photon_ms * 1e-3) / 1e9,
@@ -218,14 +318,20 @@ def main():
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         cpu = cpu_baseline(wl, args.cpu_seconds, threads)
+    ctx.close()
     adj = None
     if args.adjoint_histories:
-        ctx.close()
         adj = adjoint_rate(wl, args.adjoint_histories, args.seed)
+    mlc_plan = None
+    if args.mlc_plan_histories:
+        mlc_plan = mlc_plan_bench(args.mlc_plan_histories, args.steps, args.seed, not args.no_autotune,
+                                  json.loads(args.tuning) if args.tuning else None)
     if "roofline" in r:
         rf = r["roofline"]
         roof = {"bound": "hbm", "achieved": rf["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": rf["achieved"] / HBM_PEAK_GBS, "traffic": None, "kernel": rf["kernel"],
+                "frac": rf["achieved"] / HBM_PEAK_GBS, "traffic": rf["traffic"], "kernel": rf["kernel"],
+                "traffic_source": "profiles/photon_kernel_ct128_event.md: rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1 "
+                                  "(1 GiB copy calibration) per photon step, times this run's photon steps per launch",
                 "avg_launch_us": rf["avg_launch_us"], "launches": rf["launches"],
                 "algorithmic_bytes_per_launch": rf["algorithmic_bytes_per_launch"],
                 "algorithmic_bytes_per_photon_step": PHOTON_STEP_BYTES,
@@ -262,11 +368,14 @@ def main():
         "histories_for_1pct": args.steps * b * (r["sigma"] / 0.01) ** 2,
         "methods": {m: dict(histories_per_s=v["value"], dose_efficiency=v["efficiency"],
                             ms_per_step=v["ms_per_step"],
-                            algorithmic_bytes_per_history=v["bytes"] / (args.steps * b))
+                            algorithmic_bytes_per_history=v["bytes"] / (args.steps * b),
+                            photon_steps_per_history=v["counters"]["photon_steps"] / (args.steps * b),
+                            electron_steps_per_history=v["counters"]["electron_steps"] / (args.steps * b))
                     for m, v in results.items()},
         "tuning": tuning,
         "roofline": roof,
         "adjoint": adj,
+        "mlc_plan": mlc_plan,
         "cpu_baseline": cpu,
     }
     if "plain" in results and "event" in results:

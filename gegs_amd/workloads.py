@@ -107,3 +107,72 @@ def ct_mlc(gantry_deg=0.0, aperture_seed=3, nx=128, ny=128, nz=100, voxel=0.25, 
                     ["AIR700ICRU", "W700ICRU", "W700ICRU", "AIR700ICRU", "AIR700ICRU"],
                     lambda k, s: src.head_phase_space(k, PHSP_Z, 100.0, 10.0, s),
                     mlc=mlc, leaf_positions=pos, mlc_medium="W700ICRU")
+
+
+C3_GANTRY = (0.0, 72.0, 144.0, 216.0, 288.0)
+
+
+def field_weights(n, lo, hi, seed):
+    """n field weights in [lo, hi] that sum to 1 (seeded; redrawn until the
+    normalised draw stays inside the interval)."""
+    rng = np.random.default_rng(seed)
+    for _ in range(100000):
+        w = rng.uniform(lo, hi, n)
+        w = w / w.sum()
+        if w.min() >= lo and w.max() <= hi:
+            return w
+    raise ValueError("no weights in [%g, %g] sum to 1 for n = %d" % (lo, hi, n))
+
+
+class Plan:
+    """Fields of one plan sharing a phantom, an MLC and one phase-space list:
+    field j uses leaf set j and the records [offset_j, offset_j + count_j)."""
+
+    def __init__(self, name, workload, fields, leaf_positions, weights, source_fn):
+        self.name = name
+        self.workload = workload
+        self.fields = fields
+        self.leaf_positions = np.asarray(leaf_positions, dtype=np.float32)
+        self.weights = np.asarray(weights, dtype=np.float64)
+        self._source = source_fn
+
+    def source(self, n, seed):
+        return self._source(n, seed)
+
+    def counts(self, n_total):
+        """Histories per field: the weights times n_total, rounded, summing to n_total."""
+        c = np.floor(self.weights * n_total).astype(np.int64)
+        c[np.argmax(self.weights)] += n_total - int(c.sum())
+        return c
+
+    def jobs(self, n_total, part=0, n_parts=1, plan=0):
+        """Jobs of part `part` of n_parts equal slices of every field (for
+        gegs_run_batched with the plan's leaf sets loaded in field order)."""
+        c = self.counts(n_total)
+        off = np.concatenate([[0], np.cumsum(c)[:-1]])
+        out = []
+        for j, f in enumerate(self.fields):
+            lo = int(c[j] * part // n_parts)
+            hi = int(c[j] * (part + 1) // n_parts)
+            if hi > lo:
+                out.append(dict(field=f, plan=plan, first=int(off[j]) + lo, count=hi - lo, leaf_set=j))
+        return out
+
+
+def mlc_plan(gantry=C3_GANTRY, seed=3, weight_range=(0.1, 0.35)):
+    """Five MLC fields on the CT phantom: gantry 0/72/144/216/288 degrees,
+    seeded irregular apertures inside 10 x 10 cm^2, uneven field weights in
+    [0.1, 0.35], one egsphsp MODE0 phase-space list (decoded) shared by the
+    fields in disjoint slices."""
+    from . import head
+    wl = ct_mlc()
+    sid = wl.field.source_to_isocenter
+    fields = [make_field(True, sid, g, module_first=head.M_MLC) for g in gantry]
+    pos = np.stack([head.random_aperture(seed * 1000 + j) for j in range(len(fields))])
+    w = field_weights(len(fields), weight_range[0], weight_range[1], seed)
+
+    def phsp(n, s):
+        rec = src.encode_phsp(src.head_phase_space(n, PHSP_Z, 100.0, 10.0, s))
+        return src.decode_phsp(rec, PHSP_Z)
+
+    return Plan("ct_mlc_5field", wl, fields, pos, w, phsp)

@@ -136,3 +136,23 @@ def test_mlc_text_inputs_roundtrip(tmp_path, mlc):
     head.write_mlcinfo(info, fields)
     back = head.read_mlcinfo(info)
     assert [f.angle for f in back] == [0, 72, 144] and back[2].leaf_file == "f2.mlc"
+
+
+def test_mlc_plan_weights_and_job_slices():
+    """The 5-field plan: weights in [0.1, 0.35] summing to 1, per-field
+    counts summing to the total, and the n_parts slices of every field
+    partitioning its records without gaps or overlaps."""
+    from gegs_amd import workloads
+    plan = workloads.mlc_plan()
+    w = plan.weights
+    assert len(w) == 5 and abs(w.sum() - 1.0) < 1e-12 and w.min() >= 0.1 and w.max() <= 0.35
+    c = plan.counts(1_000_003)
+    assert c.sum() == 1_000_003
+    seen = np.zeros(1_000_003, dtype=np.int32)
+    for part in range(7):
+        for jb in plan.jobs(1_000_003, part, 7):
+            seen[jb["first"]:jb["first"] + jb["count"]] += 1
+            assert 0 <= jb["leaf_set"] < 5
+    assert np.all(seen == 1)
+    assert plan.leaf_positions.shape == (5, 120)
+    assert len({tuple(p) for p in plan.leaf_positions}) == 5
