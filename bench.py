@@ -378,8 +378,12 @@ def main():
         "mlc_plan": mlc_plan,
         "cpu_baseline": cpu,
     }
+    if mlc_plan:
+        # the batching target is stated on the 5-field MLC plan
+        line["batched_vs_plain_efficiency"] = mlc_plan["batched_vs_plain_efficiency"]
+        line["batched_vs_plain_workload"] = mlc_plan["workload"]
     if "plain" in results and "event" in results:
-        line["batched_vs_plain_efficiency"] = results["event"]["efficiency"] / results["plain"]["efficiency"]
+        line["event_vs_plain_efficiency_single_field_ct"] = results["event"]["efficiency"] / results["plain"]["efficiency"]
     print(json.dumps(line), flush=True)
 
 
