@@ -778,7 +778,7 @@ inline bool same_args(const EvArgs &x, const EvArgs &y) {
            x.env.xb == y.env.xb && x.env.yb == y.env.yb && x.env.zb == y.env.zb && x.env.nx == y.env.nx &&
            x.env.ny == y.env.ny && x.env.nz == y.env.nz && x.env.pge == y.env.pge && x.env.pgmfp == y.env.pgmfp &&
            x.env.pcohe == y.env.pcohe && x.env.mlc == y.env.mlc && x.env.endnodes == y.env.endnodes &&
-           x.env.mlc_next == y.env.mlc_next;
+           x.env.mlc_next == y.env.mlc_next && x.env.jaws == y.env.jaws && x.env.jawx_next == y.env.jawx_next;
 }
 
 inline bool same_cfg(const EventConfig &x, const EventConfig &y) {

@@ -43,6 +43,11 @@ struct gegs_ctx {
     std::vector<float> leaf_pos;         // 120 per leaf set
     gegs_float2 *d_endnodes = nullptr;   // leaf ends per job
     size_t endnodes_cap = 0;
+    gegs_jaws h_jaws{};                  // jaw geometry
+    bool have_jaws = false;
+    std::vector<float> jaw_pos;          // x1, x2, y1, y2 per set
+    gegs_float2 *d_jaws = nullptr;       // jaw polygons per job
+    size_t jaws_cap = 0;
     uint32_t *d_beamlet = nullptr;       // beamlet of each history slot
     float *d_bweight = nullptr;          // weight of each beamlet
     double *d_adj = nullptr, *d_grad = nullptr;
@@ -137,6 +142,15 @@ void gegs_destroy(gegs_ctx *c) {
     hipFree(c->d_counters);
     hipFree(c->d_hist_edep);
     hipFree(c->d_hist_steps);
+    hipFree(c->d_fields);
+    hipFree(c->d_job_plan);
+    hipFree(c->d_mlc);
+    hipFree(c->d_endnodes);
+    hipFree(c->d_jaws);
+    hipFree(c->d_beamlet);
+    hipFree(c->d_bweight);
+    hipFree(c->d_adj);
+    hipFree(c->d_grad);
     c->banks.release();
     hipStreamDestroy(c->stream);
     delete c;
@@ -227,8 +241,8 @@ int gegs_load_phantom(gegs_ctx *c, uint32_t nx, uint32_t ny, uint32_t nz, const 
 
 int gegs_set_field(gegs_ctx *c, const gegs_field *f) {
     if (!c || !f) return fail(c, GEGS_ERR_ARG, "gegs_set_field: null argument");
-    if (f->module_first != GEGS_M_PHANTOM && f->module_first != GEGS_M_MLC)
-        return fail(c, GEGS_ERR_ARG, "gegs_set_field: fields start in the MLC or the phantom module");
+    if (f->module_first != GEGS_M_PHANTOM && f->module_first != GEGS_M_MLC && f->module_first != GEGS_M_SECJAWY)
+        return fail(c, GEGS_ERR_ARG, "gegs_set_field: fields start in the jaws, the MLC or the phantom module");
     c->field = *f;
     c->have_field = true;
     return GEGS_OK;
@@ -244,6 +258,25 @@ int gegs_load_mlc(gegs_ctx *c, const gegs_mlc *m) {
     HIPCHECK(c, hipMemcpy(c->d_mlc, m, sizeof(gegs_mlc), hipMemcpyHostToDevice), "copy MLC");
     c->h_mlc = *m;
     c->have_mlc = true;
+    return GEGS_OK;
+}
+
+int gegs_load_jaws(gegs_ctx *c, const gegs_jaws *j) {
+    if (!c || !j) return fail(c, GEGS_ERR_ARG, "gegs_load_jaws: null argument");
+    if (!(j->rmax > 0.0f) || !(j->zmaxy > j->zminy) || !(j->zmaxx > j->zminx) || !(j->zminy > 0.0f) ||
+        !(j->zminx > 0.0f))
+        return fail(c, GEGS_ERR_ARG, "gegs_load_jaws: inconsistent jaw geometry");
+    c->h_jaws = *j;
+    c->have_jaws = true;
+    return GEGS_OK;
+}
+
+int gegs_set_jaw_positions(gegs_ctx *c, const float *xy, uint32_t n_sets) {
+    if (!c || (!xy && n_sets)) return fail(c, GEGS_ERR_ARG, "gegs_set_jaw_positions: null argument");
+    for (uint32_t k = 0; k < n_sets; ++k)
+        if (!(xy[4 * k + 1] > xy[4 * k]) || !(xy[4 * k + 3] > xy[4 * k + 2]))
+            return fail(c, GEGS_ERR_ARG, "gegs_set_jaw_positions: need x1 < x2 and y1 < y2");
+    c->jaw_pos.assign(xy, xy + (size_t)n_sets * 4);
     return GEGS_OK;
 }
 
@@ -322,6 +355,41 @@ static void leaf_ends(const gegs_mlc &m, const float *pos, float sid, gegs_float
     }
 }
 
+// jaw polygons of one field (setup_secjaws, init.cu:1504-1592): the field
+// rectangle at each slab's top (+-0.5 cm), the two slabs, and per pair the
+// -side jaw, the opening and the +side jaw; layout as in jaws.hpp.  The
+// +-0.5 offsets are double literals in the reference and are applied in
+// double, the scalings in float.
+static void jaw_polys(const gegs_jaws &g, const float *xy, float sid, gegs_float2 *out) {
+    const float x1 = xy[0], x2 = xy[1], y1 = xy[2], y2 = xy[3];
+    const float zmin[2] = {g.zminy, g.zminx}, zmax[2] = {g.zmaxy, g.zmaxx};
+    for (int k = 0; k < 2; ++k) {
+        gegs_float2 *b = out + 4 * k;
+        const float a1 = x1 * zmin[k] / sid, a2 = x2 * zmin[k] / sid;
+        const float c1 = y1 * zmin[k] / sid, c2 = y2 * zmin[k] / sid;
+        b[0] = {(float)((double)a1 - 0.5), (float)((double)c1 - 0.5)};
+        b[1] = {(float)((double)a2 + 0.5), (float)((double)c1 - 0.5)};
+        b[2] = {(float)((double)a2 + 0.5), (float)((double)c2 + 0.5)};
+        b[3] = {(float)((double)a1 - 0.5), (float)((double)c2 + 0.5)};
+        gegs_float2 *m = out + 8 + 4 * k;
+        m[0] = {-g.rmax, zmin[k]};
+        m[1] = {g.rmax, zmin[k]};
+        m[2] = {g.rmax, zmax[k]};
+        m[3] = {-g.rmax, zmax[k]};
+        // the Y pair's polygon uses the y openings, the X pair's the x openings
+        const float lo = k == 0 ? y1 : x1, hi = k == 0 ? y2 : x2;
+        gegs_float2 *e = out + 16 + 8 * k;
+        e[0] = {-g.rmax, zmax[k]};
+        e[1] = {-g.rmax, zmin[k]};
+        e[2] = {lo * zmin[k] / sid, zmin[k]};
+        e[3] = {lo * zmax[k] / sid, zmax[k]};
+        e[4] = {hi * zmax[k] / sid, zmax[k]};
+        e[5] = {hi * zmin[k] / sid, zmin[k]};
+        e[6] = {g.rmax, zmin[k]};
+        e[7] = {g.rmax, zmax[k]};
+    }
+}
+
 // beam geometry, dose buffer and leaf ends of each job, in device memory
 static int upload_jobs(gegs_ctx *c, const std::vector<gegs_field> &fields, const std::vector<uint32_t> &plans,
                        const std::vector<uint32_t> &leaf_sets = {0u}) {
@@ -331,15 +399,38 @@ static int upload_jobs(gegs_ctx *c, const std::vector<gegs_field> &fields, const
                                c->stream), "copy fields");
     HIPCHECK(c, hipMemcpyAsync(c->d_job_plan, plans.data(), plans.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
                                c->stream), "copy job plans");
-    bool any_mlc = false;
-    for (const gegs_field &f : fields) any_mlc |= f.module_first == GEGS_M_MLC;
+    // a field starting in the jaws passes the MLC too when one is loaded
+    auto uses_mlc = [&](const gegs_field &f) {
+        return f.module_first == GEGS_M_MLC || (f.module_first == GEGS_M_SECJAWY && c->have_mlc);
+    };
+    bool any_mlc = false, any_jaws = false;
+    for (const gegs_field &f : fields) {
+        any_mlc |= uses_mlc(f);
+        any_jaws |= f.module_first == GEGS_M_SECJAWY;
+    }
+    if (any_jaws) {
+        if (!c->have_jaws) return fail(c, GEGS_ERR_STATE, "run: a field starts in the jaws but no jaws are loaded");
+        const size_t nsets = c->jaw_pos.size() / 4;
+        std::vector<gegs_float2> polys(fields.size() * 32);
+        for (size_t j = 0; j < fields.size(); ++j) {
+            if (fields[j].module_first != GEGS_M_SECJAWY) continue;
+            const uint32_t set = j < leaf_sets.size() ? leaf_sets[j] : 0u;
+            if (set >= nsets) return fail(c, GEGS_ERR_ARG, "run: a field's jaw set is not loaded");
+            if (!(fields[j].source_to_isocenter > 0.0f)) return fail(c, GEGS_ERR_ARG, "run: source_to_isocenter must be > 0");
+            jaw_polys(c->h_jaws, c->jaw_pos.data() + set * 4, fields[j].source_to_isocenter, polys.data() + j * 32);
+        }
+        HIPCHECK(c, realloc_dev(c->d_jaws, c->jaws_cap, polys.size()), "hipMalloc jaw polygons");
+        HIPCHECK(c, hipMemcpyAsync(c->d_jaws, polys.data(), polys.size() * sizeof(gegs_float2), hipMemcpyHostToDevice,
+                                   c->stream), "copy jaw polygons");
+        HIPCHECK(c, hipStreamSynchronize(c->stream), "copy jaw polygons");
+    }
     std::vector<gegs_float2> ends;
     if (any_mlc) {
         if (!c->have_mlc) return fail(c, GEGS_ERR_STATE, "run: a field starts in the MLC but no MLC is loaded");
         const size_t nsets = c->leaf_pos.size() / (2 * GEGS_MLC_LEAVES);
         ends.resize(fields.size() * GEGS_MLC_LEAVES * 34);
         for (size_t j = 0; j < fields.size(); ++j) {
-            if (fields[j].module_first != GEGS_M_MLC) continue;
+            if (!uses_mlc(fields[j])) continue;
             const uint32_t set = j < leaf_sets.size() ? leaf_sets[j] : 0u;
             if (set >= nsets) return fail(c, GEGS_ERR_ARG, "run: a field's leaf set is not loaded");
             if (!(fields[j].source_to_isocenter > 0.0f)) return fail(c, GEGS_ERR_ARG, "run: source_to_isocenter must be > 0");
@@ -375,6 +466,8 @@ static Env make_env(gegs_ctx *c, uint64_t seed, uint32_t njob) {
     env.mlc = c->have_mlc ? c->d_mlc : nullptr;
     env.endnodes = c->d_endnodes;
     env.mlc_next = GEGS_M_PHANTOM;
+    env.jaws = c->d_jaws;
+    env.jawx_next = c->have_mlc ? GEGS_M_MLC : GEGS_M_PHANTOM;
     env.seed = seed;
     return env;
 }
@@ -545,8 +638,9 @@ static int parse_jobs(gegs_ctx *c, const gegs_job *jobs, uint32_t n_jobs, uint32
     for (uint32_t j = 0; j < n_jobs; ++j) {
         const gegs_job &jb = jobs[j];
         if (jb.plan >= n_plans) return fail(c, GEGS_ERR_ARG, "run: job plan out of range");
-        if (jb.field.module_first != GEGS_M_PHANTOM && jb.field.module_first != GEGS_M_MLC)
-            return fail(c, GEGS_ERR_ARG, "run: fields start in the MLC or the phantom module");
+        if (jb.field.module_first != GEGS_M_PHANTOM && jb.field.module_first != GEGS_M_MLC &&
+            jb.field.module_first != GEGS_M_SECJAWY)
+            return fail(c, GEGS_ERR_ARG, "run: fields start in the jaws, the MLC or the phantom module");
         if (jb.count == 0 || jb.first >= c->nsrc || jb.first + jb.count > c->nsrc)
             return fail(c, GEGS_ERR_ARG, "run: job record range outside the loaded source");
         fields[j] = jb.field;

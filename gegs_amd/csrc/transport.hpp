@@ -205,6 +205,8 @@ struct Env {
     const gegs_mlc *mlc;             // MLC geometry (null: no MLC)
     const gegs_float2 *endnodes;     // leaf ends per job: [job][60][34]
     int32_t mlc_next;                // module after the MLC
+    int32_t jawx_next;               // module after the X jaws (the MLC when loaded)
+    const gegs_float2 *jaws;         // jaw polygons per job: [job][32]
     uint64_t seed;
 };
 
@@ -459,16 +461,23 @@ __device__ __forceinline__ float hownear_phantom(const Env &env, const Part &p) 
 }  // namespace gegs
 
 #include "mlc.hpp"
+#include "jaws.hpp"
 
 namespace gegs {
 
 // module dispatch of the geometry queries (photon.cu:142-158, electron.cu:1491-1505, :1836-1850)
 __device__ __forceinline__ uint32_t howfar_module(const Env &env, Part &p, float &t) {
-    return p.module == GEGS_M_MLC ? howfar_mlc(env, p, t) : howfar_phantom(env, p, t);
+    if (p.module == GEGS_M_PHANTOM) return howfar_phantom(env, p, t);
+    if (p.module == GEGS_M_MLC) return howfar_mlc(env, p, t);
+    if (p.module == GEGS_M_SECJAWY) return howfar_jaws<true>(env, p, t);
+    return howfar_jaws<false>(env, p, t);
 }
 
 __device__ __forceinline__ float hownear_module(const Env &env, Part &p) {
-    return p.module == GEGS_M_MLC ? hownear_mlc(env, p) : hownear_phantom(env, p);
+    if (p.module == GEGS_M_PHANTOM) return hownear_phantom(env, p);
+    if (p.module == GEGS_M_MLC) return hownear_mlc(env, p);
+    if (p.module == GEGS_M_SECJAWY) return hownear_jaws<true>(env, p);
+    return hownear_jaws<false>(env, p);
 }
 
 // ------------------------------------------------------------------ rotations (utility.cu:94-218)

@@ -28,11 +28,47 @@ import math
 import numpy as np
 
 NLEAF = 60
+M_SECJAWY = 1
+M_SECJAWX = 2
 M_MLC = 3
 
 
 class Float2(ctypes.Structure):
     _fields_ = [("x", ctypes.c_float), ("y", ctypes.c_float)]
+
+
+class Jaws(ctypes.Structure):
+    """ctypes mirror of ``gegs_jaws`` (include/gegs.h): the JAWS component of
+    the head (init.cu:612-688), z in cm from the source."""
+    _fields_ = [("rmax", ctypes.c_float), ("zminy", ctypes.c_float), ("zmaxy", ctypes.c_float),
+                ("zminx", ctypes.c_float), ("zmaxx", ctypes.c_float)]
+
+
+# jaw slabs of a Clinac-type head (z of the Y and X pairs, cm from the source)
+CLINAC_LIKE_JAWS = dict(rmax=20.0, zminy=28.0, zmaxy=35.8, zminx=36.7, zmaxx=44.5)
+
+
+def build_jaws(rmax, zminy, zmaxy, zminx, zmaxx):
+    return Jaws(rmax, zminy, zmaxy, zminx, zmaxx)
+
+
+def jaw_positions_for(leaf_positions, margin=0.5, half=5.0):
+    """Jaw openings x1, x2, y1, y2 (cm at the isocentre) that frame an MLC
+    aperture: across the leaves the open pairs' extent, along the leaves the
+    leaf ends' extent, each widened by `margin` and kept within +-(half +
+    margin)."""
+    pos = np.asarray(leaf_positions, dtype=np.float64).reshape(2 * NLEAF)
+    open_ = pos[:NLEAF] + pos[NLEAF:] > 0
+    centers = np.concatenate([np.arange(10) * 1.0 - 19.5, np.arange(40) * 0.5 - 9.75, np.arange(10) * 1.0 + 10.5])
+    width = np.concatenate([np.full(10, 1.0), np.full(40, 0.5), np.full(10, 1.0)])
+    lim = half + margin
+    if not open_.any():
+        return np.array([-margin, margin, -margin, margin], dtype=np.float32)
+    y1 = max(-lim, float((centers - width / 2)[open_].min()) - margin)
+    y2 = min(lim, float((centers + width / 2)[open_].max()) + margin)
+    x1 = max(-lim, float(-pos[NLEAF:][open_].max()) - margin)
+    x2 = min(lim, float(pos[:NLEAF][open_].max()) + margin)
+    return np.array([x1, x2, y1, y2], dtype=np.float32)
 
 
 class Mlc(ctypes.Structure):

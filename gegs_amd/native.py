@@ -23,6 +23,7 @@ SYMBOLS = [
     "gegs_get_counters", "gegs_last_kernel_ms", "gegs_sample_events",
     "gegs_set_tuning", "gegs_get_tuning", "gegs_run_batched", "gegs_set_beamlets", "gegs_adjoint",
     "gegs_set_kernel_timing", "gegs_kernel_timing", "gegs_load_mlc", "gegs_set_leaf_positions",
+    "gegs_load_jaws", "gegs_set_jaw_positions",
 ]
 
 PARTICLE_DTYPE = np.dtype([("e", "<f4"), ("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("u", "<f4"),
@@ -116,6 +117,8 @@ def load():
     L.gegs_set_tuning.argtypes = [vp, vp]
     L.gegs_run_batched.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp]
     L.gegs_load_mlc.argtypes = [vp, vp]
+    L.gegs_load_jaws.argtypes = [vp, vp]
+    L.gegs_set_jaw_positions.argtypes = [vp, vp, ctypes.c_uint32]
     L.gegs_set_leaf_positions.argtypes = [vp, vp, ctypes.c_uint32]
     L.gegs_set_kernel_timing.argtypes = [vp, ctypes.c_int]
     L.gegs_kernel_timing.argtypes = [vp, vp, vp]
@@ -176,6 +179,9 @@ class Context:
         if getattr(wl, "mlc", None) is not None:
             self._check(self.L.gegs_load_mlc(self.h, ctypes.c_void_p(ctypes.addressof(wl.mlc))))
             self.set_leaf_positions(wl.leaf_positions)
+        if getattr(wl, "jaws", None) is not None:
+            self._check(self.L.gegs_load_jaws(self.h, ctypes.c_void_p(ctypes.addressof(wl.jaws))))
+            self.set_jaw_positions(wl.jaw_positions)
         self._check(self.L.gegs_set_field(self.h, ctypes.c_void_p(ctypes.addressof(wl.field))))
         self.nvox = ph.nvox
 
@@ -233,6 +239,14 @@ class Context:
         if len(pos) == 0 or len(pos) % 120:
             raise ValueError("leaf positions come in sets of 120")
         self._check(self.L.gegs_set_leaf_positions(self.h, _p(pos), len(pos) // 120))
+
+    def set_jaw_positions(self, xy):
+        """Jaw openings of one or more sets, 4 floats each (x1, x2, y1, y2 in
+        cm at the isocentre); set k goes with leaf set k."""
+        a = np.ascontiguousarray(xy, dtype=np.float32).reshape(-1)
+        if len(a) == 0 or len(a) % 4:
+            raise ValueError("jaw positions come in sets of 4")
+        self._check(self.L.gegs_set_jaw_positions(self.h, _p(a), len(a) // 4))
 
     def set_beamlets(self, beamlet=None, weight=None):
         """Beamlet index per history slot (job after job) and weight per

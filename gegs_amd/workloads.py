@@ -36,8 +36,10 @@ def make_field(change_xyz, sid, gantry_deg=0.0, iso=(0.0, 0.0, 0.0), module_firs
 
 class Workload:
     def __init__(self, name, phantom, media_names, field, other_media, source_fn, mlc=None, leaf_positions=None,
-                 mlc_medium=None):
+                 mlc_medium=None, jaws=None, jaw_positions=None):
         self.name = name
+        self.jaws = jaws                      # head.Jaws, fields starting in the jaws
+        self.jaw_positions = jaw_positions    # x1, x2, y1, y2 of the field
         self.mlc = mlc                        # head.Mlc, fields starting in the MLC
         self.leaf_positions = leaf_positions  # 120 leaf positions of the field
         self.mlc_medium = mlc_medium
@@ -107,6 +109,20 @@ def ct_mlc(gantry_deg=0.0, aperture_seed=3, nx=128, ny=128, nz=100, voxel=0.25, 
                     ["AIR700ICRU", "W700ICRU", "W700ICRU", "AIR700ICRU", "AIR700ICRU"],
                     lambda k, s: src.head_phase_space(k, PHSP_Z, 100.0, 10.0, s),
                     mlc=mlc, leaf_positions=pos, mlc_medium="W700ICRU")
+
+
+def ct_jaws_mlc(gantry_deg=0.0, aperture_seed=3, seed=2):
+    """The MLC workload with the secondary jaws above the MLC (Clinac-like
+    slabs at z 28-35.8 cm and 36.7-44.5 cm) framing the aperture: fields
+    start in the Y jaws, pass the X jaws and the MLC, then the phantom.
+    Regions 1-5: air, jaw W, MLC W, air, air."""
+    from . import head
+    wl = ct_mlc(gantry_deg, aperture_seed, seed=seed)
+    wl.name = "ct_128x128x100_6MV_jaws_mlc"
+    wl.field = make_field(True, wl.field.source_to_isocenter, gantry_deg, module_first=head.M_SECJAWY)
+    wl.jaws = head.build_jaws(**head.CLINAC_LIKE_JAWS)
+    wl.jaw_positions = head.jaw_positions_for(wl.leaf_positions)
+    return wl
 
 
 C3_GANTRY = (0.0, 72.0, 144.0, 216.0, 288.0)

@@ -32,8 +32,21 @@ extern "C" {
 #define GEGS_ERR_STATE 90002         /* call out of order (e.g. run before load) */
 
 /* Module ids (CUDA_EGS.h:396-404). */
+#define GEGS_M_SECJAWY 1
+#define GEGS_M_SECJAWX 2
 #define GEGS_M_MLC 3
 #define GEGS_M_PHANTOM 6
+
+/* Secondary collimator jaws (SecJaws, read from the BEAMnrc JAWS input,
+ * init.cu:612-688): two tungsten slabs, the Y pair at z in [zminy, zmaxy]
+ * and the X pair at z in [zminx, zmaxx] (head frame), inside |r| < rmax.
+ * The jaw faces are focused on the source; their openings come per field
+ * from gegs_set_jaw_positions.  Regions in the module: 0 vacuum, 1 air,
+ * 2 jaw tungsten.  A field starting in the jaws passes Y, then X, then the
+ * MLC when one is loaded, then the phantom (CUDA_EGS.cu:1280-1298). */
+typedef struct gegs_jaws {
+    float rmax, zminy, zmaxy, zminx, zmaxx;
+} gegs_jaws;
 
 /* Multileaf collimator (DYNVMLC-type, read_egsinp, init.cu:345-611).  Head
  * frame: z from the source; the MLC occupies z in [zmin, zmin + zthick].
@@ -165,11 +178,21 @@ int gegs_load_phantom(gegs_ctx *ctx, uint32_t nx, uint32_t ny, uint32_t nz,
 
 /* Field geometry (gantry symbols, CUDA_EGS.cu:430-432).  A field whose
  * module_first is GEGS_M_MLC starts its particles in the MLC, which needs
- * gegs_load_mlc and a leaf-position row (row 0 for gegs_run_plain/_event). */
+ * gegs_load_mlc and a leaf-position row (row 0 for gegs_run_plain/_event);
+ * one with GEGS_M_SECJAWY starts them in the jaws (gegs_load_jaws and a
+ * jaw-position row), then the MLC if one is loaded. */
 int gegs_set_field(gegs_ctx *ctx, const gegs_field *field);
 
 /* MLC geometry (init.cu:345-611). */
 int gegs_load_mlc(gegs_ctx *ctx, const gegs_mlc *mlc);
+
+/* Jaw geometry (SecJaws, init.cu:612-688) and the jaw openings of n_sets
+ * fields, 4 values per set: x1, x2, y1, y2 in cm at the isocentre plane
+ * (the field line of the .mlcinfo list, init.cu:1431).  Set k goes with
+ * leaf set k: a job's leaf_set selects both.  The jaw polygons of a field
+ * follow setup_secjaws (init.cu:1504-1592). */
+int gegs_load_jaws(gegs_ctx *ctx, const gegs_jaws *jaws);
+int gegs_set_jaw_positions(gegs_ctx *ctx, const float *x1x2y1y2, uint32_t n_sets);
 
 /* Leaf positions of n_sets fields, 120 values per set, in the order of the
  * .mlcinfo leaf files (init.cu:1480-1500), cm at the isocentre plane: leaf

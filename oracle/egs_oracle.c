@@ -160,6 +160,8 @@ typedef struct octx {
     gegs_field f;
     const gegs_mlc *mlc;            /* NULL: no MLC */
     const gegs_float2 *endnodes;    /* [60][34] leaf ends of the field */
+    const gegs_float2 *jaws;        /* [32] jaw polygons of the field (NULL: no jaws) */
+    int jawx_next;                  /* module after the X jaws */
     uint64_t seed;
     double *dose, *dose2, *hist_edep;
     uint32_t *hist_steps;
@@ -591,10 +593,91 @@ static float hownear_mlc(const octx *c, opart *p) {
     return txz < tyz ? txz : tyz;
 }
 
-static uint32_t howfar(octx *c, opart *p, float *t) {
-    return p->module == GEGS_M_MLC ? howfar_mlc(c, p, t) : howfar_phantom(c, p, t);
+/* ------------------------------------------------------------ secondary jaws
+ * howfar_SecJawsY / _SecJawsX (geom.cu:793-1035) and hownear (geom.cu:1288-1374)
+ * as built with ROTATE_SecJaws_XY (CUDA_EGS.h:25): across-jaw points are
+ * (y, z) and (x, z); the Y pair's polygons are tested in (x, z), the X
+ * pair's in (y, z); the entry test against the field rectangle uses (y, x).
+ * Polygons of the field: [0,4) minY, [4,8) minX, [8,12) maxY, [12,16) maxX,
+ * [16,24) geometry Y, [24,32) geometry X. */
+static uint32_t howfar_jaws(const octx *c, opart *p, float *t, int is_y) {
+    const gegs_float2 *bmin = c->jaws + (is_y ? 0 : 4);
+    const gegs_float2 *bmax = c->jaws + (is_y ? 8 : 12);
+    const gegs_float2 *g = c->jaws + (is_y ? 16 : 24);
+    pt2 a = mk2(p->y, p->z), b = mk2(p->x, p->z), va = mk2(p->v, p->w), vb = mk2(p->u, p->w);
+    int ina, inb, dummy;
+    float dxz = ray_poly(a, va, bmax, 4, &ina);
+    float dyz = ray_poly(b, vb, bmax, 4, &inb);
+    if (dxz >= INFI_DIST || dyz >= INFI_DIST) return 0u;
+    float bound = dxz < dyz ? dxz : dyz;
+    pt2 pp = is_y ? b : a, vv = is_y ? vb : va;
+    uint32_t r;
+    if (ina && inb) {
+        if (in_poly(pp, g + 2, 4)) {
+            r = 1u;
+            float d1 = ray_poly(pp, vv, g, 4, &dummy);
+            float d2 = ray_poly(pp, vv, g + 4, 4, &dummy);
+            if (d1 < INFI_DIST) {
+                if (*t > d1) { r = 2u; *t = d1 + EPSTEP; }
+            } else if (d2 < INFI_DIST) {
+                if (*t > d2) { r = 2u; *t = d2 + EPSTEP; }
+            } else if (*t > bound) {
+                *t = bound + EPSTEP;
+                p->module = is_y ? GEGS_M_SECJAWX : c->jawx_next;
+            }
+        } else {
+            r = 2u;
+            float d0 = ray_poly(pp, vv, g + 2, 4, &dummy);
+            if (d0 < INFI_DIST) {
+                /* geom.cu:992: the X pair steps by dis_yz here */
+                if (*t > d0) { r = 1u; *t = (is_y ? d0 : dyz) + EPSTEP; }
+            } else if (*t > bound) {
+                r = 0u;
+                *t = bound + EPSTEP;
+            }
+        }
+    } else {
+        r = 1u;
+        if (*t > bound) {
+            *t = bound + EPSTEP;
+            pt2 q = mk2(p->y + *t * p->v, p->x + *t * p->u);
+            if (!in_poly(q, bmin, 4)) {
+                r = 0u;
+            } else {
+                if (is_y) q.x = q.y;
+                q.y = p->z + *t * p->w;
+                r = in_poly(q, g + 2, 4) ? 1u : 2u;
+            }
+        }
+    }
+    return r;
 }
-static float hownear(octx *c, opart *p) { return p->module == GEGS_M_MLC ? hownear_mlc(c, p) : hownear_phantom(c, p); }
+
+static float hownear_jaws(const octx *c, const opart *p, int is_y) {
+    const gegs_float2 *bmax = c->jaws + (is_y ? 8 : 12);
+    const gegs_float2 *g = c->jaws + (is_y ? 16 : 24);
+    pt2 a = mk2(p->y, p->z), b = mk2(p->x, p->z);
+    int ina = in_poly(a, bmax, 4), inb = in_poly(b, bmax, 4);
+    float ba = poly_near(a, bmax, 4), bb = poly_near(b, bmax, 4);
+    if (!(ina && inb)) return ba < bb ? ba : bb;
+    /* Y: polygons in (x, z), the other bound is that of (y, z); X: the reverse */
+    pt2 pp = is_y ? b : a;
+    float own = is_y ? bb : ba, other = is_y ? ba : bb;
+    float d = poly_near(pp, g + 2, 4);
+    if (!in_poly(pp, g + 2, 4) && d > own) d = own;
+    return d < other ? d : other;
+}
+
+static uint32_t howfar(octx *c, opart *p, float *t) {
+    if (p->module == GEGS_M_PHANTOM) return howfar_phantom(c, p, t);
+    if (p->module == GEGS_M_MLC) return howfar_mlc(c, p, t);
+    return howfar_jaws(c, p, t, p->module == GEGS_M_SECJAWY);
+}
+static float hownear(octx *c, opart *p) {
+    if (p->module == GEGS_M_PHANTOM) return hownear_phantom(c, p);
+    if (p->module == GEGS_M_MLC) return hownear_mlc(c, p);
+    return hownear_jaws(c, p, p->module == GEGS_M_SECJAWY);
+}
 
 
 /* ------------------------------------------------------------ uphi (utility.cu:94-218) */
@@ -1911,6 +1994,51 @@ uint32_t oracle_howfar_mlc(const gegs_mlc *m, const gegs_float2 *ends, const geg
     *leaf = p.leaf; *bank = p.bank; *module = p.module;
     return r;
 }
+/* jaw polygons of a field (setup_secjaws, init.cu:1504-1592); the +-0.5 cm
+ * margins are double literals in the reference */
+void oracle_jaw_polys(const gegs_jaws *g, const float *xy, float sid, gegs_float2 *out) {
+    float zmin[2] = {g->zminy, g->zminx}, zmax[2] = {g->zmaxy, g->zmaxx};
+    for (int k = 0; k < 2; ++k) {
+        float a1 = xy[0] * zmin[k] / sid, a2 = xy[1] * zmin[k] / sid;
+        float c1 = xy[2] * zmin[k] / sid, c2 = xy[3] * zmin[k] / sid;
+        gegs_float2 *bm = out + 4 * k, *bx = out + 8 + 4 * k, *e = out + 16 + 8 * k;
+        bm[0].x = (float)(a1 - 0.5); bm[0].y = (float)(c1 - 0.5);
+        bm[1].x = (float)(a2 + 0.5); bm[1].y = (float)(c1 - 0.5);
+        bm[2].x = (float)(a2 + 0.5); bm[2].y = (float)(c2 + 0.5);
+        bm[3].x = (float)(a1 - 0.5); bm[3].y = (float)(c2 + 0.5);
+        bx[0].x = -g->rmax; bx[0].y = zmin[k];
+        bx[1].x = g->rmax;  bx[1].y = zmin[k];
+        bx[2].x = g->rmax;  bx[2].y = zmax[k];
+        bx[3].x = -g->rmax; bx[3].y = zmax[k];
+        float lo = k == 0 ? xy[2] : xy[0], hi = k == 0 ? xy[3] : xy[1];
+        e[0].x = -g->rmax; e[0].y = zmax[k];
+        e[1].x = -g->rmax; e[1].y = zmin[k];
+        e[2].x = lo * zmin[k] / sid; e[2].y = zmin[k];
+        e[3].x = lo * zmax[k] / sid; e[3].y = zmax[k];
+        e[4].x = hi * zmax[k] / sid; e[4].y = zmax[k];
+        e[5].x = hi * zmin[k] / sid; e[5].y = zmin[k];
+        e[6].x = g->rmax; e[6].y = zmin[k];
+        e[7].x = g->rmax; e[7].y = zmax[k];
+    }
+}
+
+/* jaw probe for the geometry tests: howfar of one particle in the Y (is_y)
+ * or X pair; returns the region, updates t and module */
+uint32_t oracle_howfar_jaws(const gegs_float2 *polys, int is_y, int jawx_next, const gegs_particle *in,
+                            int32_t *module, float *t) {
+    octx c;
+    memset(&c, 0, sizeof(c));
+    c.jaws = polys;
+    c.jawx_next = jawx_next;
+    opart p;
+    memset(&p, 0, sizeof(p));
+    p.x = in->x; p.y = in->y; p.z = in->z; p.u = in->u; p.v = in->v; p.w = in->w;
+    p.region = in->region; p.module = is_y ? GEGS_M_SECJAWY : GEGS_M_SECJAWX;
+    uint32_t r = howfar_jaws(&c, &p, t, is_y);
+    *module = p.module;
+    return r;
+}
+
 uint32_t oracle_where_mlc(const gegs_mlc *m, const gegs_float2 *ends, float x, float y, float z) {
     int index = OUTMLC, bank = -1;
     return mlc_where(m, ends, x, y, z, &index, &bank);
@@ -1923,7 +2051,7 @@ int oracle_run_mt(const gegs_media_tables *mt, const gegs_region *regions,
                   uint64_t hist_offset, uint64_t seed,
                   double *dose, double *dose2, double *hist_edep, uint32_t *hist_steps,
                   gegs_counters *counters, double *balance, int nthreads,
-                  const gegs_mlc *mlc, const gegs_float2 *endnodes) {
+                  const gegs_mlc *mlc, const gegs_float2 *endnodes, const gegs_float2 *jaws) {
     size_t nvox = (size_t)nx * ny * nz;
     if (nthreads < 1) nthreads = 1;
     if ((uint64_t)nthreads > n && n > 0) nthreads = (int)n;
@@ -1936,6 +2064,7 @@ int oracle_run_mt(const gegs_media_tables *mt, const gegs_region *regions,
         c->mt = mt; c->reg = regions; c->xb = xb; c->yb = yb; c->zb = zb;
         c->nx = nx; c->ny = ny; c->nz = nz; c->f = *field; c->seed = seed;
         c->mlc = mlc; c->endnodes = endnodes;
+        c->jaws = jaws; c->jawx_next = mlc ? GEGS_M_MLC : GEGS_M_PHANTOM;
         c->dose = t == 0 ? dose : (double *)calloc(nvox, sizeof(double));
         c->dose2 = t == 0 ? dose2 : (double *)calloc(nvox, sizeof(double));
         c->tmp = (double *)calloc(nvox, sizeof(double));
@@ -1988,7 +2117,7 @@ int oracle_run(const gegs_media_tables *mt, const gegs_region *regions,
                double *dose, double *dose2, double *hist_edep, uint32_t *hist_steps,
                gegs_counters *counters, double *balance) {
     return oracle_run_mt(mt, regions, nx, ny, nz, xb, yb, zb, field, src, n, hist_offset, seed,
-                         dose, dose2, hist_edep, hist_steps, counters, balance, 1, NULL, NULL);
+                         dose, dose2, hist_edep, hist_steps, counters, balance, 1, NULL, NULL, NULL);
 }
 
 /* Single-sampler entry point for uniform-for-uniform checks: the handler of

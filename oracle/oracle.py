@@ -43,7 +43,10 @@ def lib():
                                  vp, vp, vp, vp, vp, ctypes.c_uint64, ctypes.c_uint64,
                                  ctypes.c_uint64, vp, vp, vp, vp, vp, vp]
         L.oracle_run_mt.restype = ctypes.c_int
-        L.oracle_run_mt.argtypes = L.oracle_run.argtypes + [ctypes.c_int, vp, vp]
+        L.oracle_run_mt.argtypes = L.oracle_run.argtypes + [ctypes.c_int, vp, vp, vp]
+        L.oracle_jaw_polys.argtypes = [vp, vp, ctypes.c_float, vp]
+        L.oracle_howfar_jaws.restype = ctypes.c_uint32
+        L.oracle_howfar_jaws.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, vp]
         L.oracle_leaf_ends.argtypes = [vp, vp, ctypes.c_float, vp]
         L.oracle_leaf_index.restype = ctypes.c_int
         L.oracle_leaf_index.argtypes = [vp, ctypes.c_float, ctypes.c_float]
@@ -92,19 +95,43 @@ def run(workload, source, seed, hist_offset=0, per_history=False, threads=1):
     cnt = Counters()
     bal = np.zeros(2, dtype=np.float64)
     reg = np.ascontiguousarray(workload.regions)
-    mlc_ptr, ends_ptr, ends = None, None, None
-    if workload.field.module_first == 3:          # the field starts in the MLC
-        mlc_ptr = ctypes.c_void_p(ctypes.addressof(workload.mlc))
-        ends = leaf_ends(workload.mlc, workload.leaf_positions, workload.field.source_to_isocenter)
+    mlc_ptr, ends_ptr, ends, jaws = None, None, None, None
+    first = workload.field.module_first
+    sid = workload.field.source_to_isocenter
+    mlc = getattr(workload, "mlc", None)
+    if first == 3 or (first == 1 and mlc is not None):    # the field passes the MLC
+        mlc_ptr = ctypes.c_void_p(ctypes.addressof(mlc))
+        ends = leaf_ends(mlc, workload.leaf_positions, sid)
         ends_ptr = _p(ends)
+    if first == 1:                                        # the field starts in the jaws
+        jaws = jaw_polys(workload.jaws, workload.jaw_positions, sid)
     rc = L.oracle_run_mt(ctypes.c_void_p(workload.tables.ptr()), _p(reg), phan.nx, phan.ny, phan.nz,
                          _p(xb), _p(yb), _p(zb), ctypes.c_void_p(ctypes.addressof(workload.field)),
                          _p(src), n, hist_offset, seed, _p(dose), _p(dose2), _p(edep), _p(steps),
-                         ctypes.c_void_p(ctypes.addressof(cnt)), _p(bal), int(threads), mlc_ptr, ends_ptr)
+                         ctypes.c_void_p(ctypes.addressof(cnt)), _p(bal), int(threads), mlc_ptr, ends_ptr,
+                         _p(jaws))
     if rc != 0:
         raise RuntimeError("oracle_run failed: %d" % rc)
     return dict(dose=dose, dose2=dose2, edep=edep, steps=steps, counters=cnt,
                 escaped=float(bal[0]), dropped=float(bal[1]))
+
+
+def jaw_polys(jaws, xy, sid):
+    """[32, 2] float32 jaw polygons of one field (x1, x2, y1, y2 at the isocentre)."""
+    xy = np.ascontiguousarray(xy, dtype=np.float32).reshape(4)
+    out = np.zeros((32, 2), dtype=np.float32)
+    lib().oracle_jaw_polys(ctypes.c_void_p(ctypes.addressof(jaws)), _p(xy), float(sid), _p(out))
+    return out
+
+
+def howfar_jaws(polys, is_y, jawx_next, particle, t):
+    """(region, t, module) of the oracle's howfar_SecJawsY / X."""
+    part = np.ascontiguousarray(particle)
+    tt = ctypes.c_float(t)
+    mod = ctypes.c_int32(0)
+    r = lib().oracle_howfar_jaws(_p(np.ascontiguousarray(polys, dtype=np.float32)), int(is_y), int(jawx_next),
+                                 _p(part), ctypes.byref(mod), ctypes.byref(tt))
+    return int(r), tt.value, mod.value
 
 
 def leaf_ends(mlc, positions, sid):

@@ -79,3 +79,23 @@ def test_mlc_leaf_sets_per_job(gpu_ctx, ct_mlc):
         o = oracle.run(wl, src[sl], seed=23, hist_offset=jb["first"], per_history=True)
         assert np.array_equal(bt["steps"][sl], o["steps"])
         assert np.array_equal(bt["edep"][sl], o["edep"])
+
+
+def test_jaws_then_mlc_bit_exact(gpu_ctx):
+    """Fields starting in the secondary jaws pass Y, X, the MLC and the
+    phantom: the plain kernel reproduces the oracle history by history and
+    the event path reproduces the plain kernel."""
+    wl = workloads.ct_jaws_mlc()
+    src = wl.source(4000, 9)
+    _load(gpu_ctx, wl, src)
+    try:
+        g = gpu_ctx.run(seed=31, n_batches=2, method="plain", per_history=True)
+        e = gpu_ctx.run(seed=31, n_batches=2, method="event", per_history=True)
+    finally:
+        gpu_ctx.set_field(workloads.ct_mlc().field)
+    o = oracle.run(wl, src, seed=31, per_history=True)
+    assert np.array_equal(g["steps"], o["steps"])
+    assert np.array_equal(g["edep"], o["edep"])
+    assert np.array_equal(e["steps"], g["steps"])
+    assert np.allclose(e["edep"], g["edep"], rtol=1e-12, atol=1e-12)
+    assert e["counters"]["overflow"] == 0

@@ -1,0 +1,127 @@
+"""Secondary-jaw geometry of the oracle (restatement of geom.cu:793-1035,
+:1288-1374 with ROTATE_SecJaws_XY) against a closed-form model of the same
+solid: inside a pair's slab (zmin <= z <= zmax, |x|, |y| <= rmax) a point is
+in the opening when the coordinate that pair collimates lies between the
+field edges projected along rays from the source (x for the Y pair, y for
+the X pair: lo * z / SID <= c <= hi * z / SID), and in tungsten otherwise."""
+import numpy as np
+import pytest
+
+from gegs_amd import head, workloads
+from gegs_amd.native import PARTICLE_DTYPE
+from oracle import oracle
+
+SID = 130.0
+
+
+@pytest.fixture(scope="module")
+def geom():
+    j = head.build_jaws(**head.CLINAC_LIKE_JAWS)
+    xy = np.array([-4.0, 5.0, -3.5, 4.5], dtype=np.float32)
+    return j, xy, oracle.jaw_polys(j, xy, SID)
+
+
+def test_jaw_polygons_closed_form(geom):
+    j, xy, poly = geom
+    x1, x2, y1, y2 = (float(v) for v in xy)
+    f32 = np.float32
+    # field rectangles at the slab tops, +-0.5 cm
+    for k, zmin in ((0, j.zminy), (1, j.zminx)):
+        b = poly[4 * k:4 * k + 4]
+        a1 = f32(f32(f32(x1) * f32(zmin)) / f32(SID))
+        c2 = f32(f32(f32(y2) * f32(zmin)) / f32(SID))
+        assert b[0, 0] == f32(np.float64(a1) - 0.5) and b[2, 1] == f32(np.float64(c2) + 0.5)
+    # the Y pair's opening is framed by y1, y2; the X pair's by x1, x2
+    gy, gx = poly[16:24], poly[24:32]
+    assert gy[2, 0] == f32(f32(f32(y1) * f32(j.zminy)) / f32(SID))
+    assert gy[4, 0] == f32(f32(f32(y2) * f32(j.zmaxy)) / f32(SID))
+    assert gx[3, 0] == f32(f32(f32(x1) * f32(j.zmaxx)) / f32(SID))
+    assert gx[5, 0] == f32(f32(f32(x2) * f32(j.zminx)) / f32(SID))
+    assert np.all(poly[8:12, 1] == np.float32([j.zminy, j.zminy, j.zmaxy, j.zmaxy]))
+
+
+def classify(j, xy, is_y, x, y, z):
+    """0 outside the slab, 1 opening, 2 tungsten (closed form)."""
+    zmin, zmax = (j.zminy, j.zmaxy) if is_y else (j.zminx, j.zmaxx)
+    if z < zmin or z > zmax or abs(x) > j.rmax or abs(y) > j.rmax:
+        return 0
+    lo, hi = (xy[2], xy[3]) if is_y else (xy[0], xy[1])
+    c = x if is_y else y
+    return 1 if lo * z / SID <= c <= hi * z / SID else 2
+
+
+def test_howfar_walk_matches_closed_form(geom):
+    """Rays from above the Y slab walked through Y then X: every step ends
+    in the region the closed form gives at its end point, the region at the
+    step's midpoint is the region the particle was in, and leaving a slab's
+    bottom through the opening hands the particle to the next module."""
+    j, xy, poly = geom
+    rng = np.random.default_rng(7)
+    checked = handed = 0
+    for _ in range(600):
+        # a point source at the origin aiming inside +-8 cm at the isocentre
+        tx, ty = rng.uniform(-8, 8, 2)
+        d = np.array([tx, ty, SID], dtype=np.float64)
+        d /= np.linalg.norm(d)
+        z0 = 20.0
+        part = np.zeros(1, dtype=PARTICLE_DTYPE)
+        pos = np.array([d[0] / d[2] * z0, d[1] / d[2] * z0, z0], dtype=np.float32)
+        part["u"], part["v"], part["w"] = np.float32(d[0]), np.float32(d[1]), np.float32(d[2])
+        for is_y in (1, 0):
+            region = 0
+            for _step in range(12):
+                part["x"], part["y"], part["z"] = pos
+                part["region"] = region
+                r, t, module = oracle.howfar_jaws(poly, is_y, 3, part, 1e8)
+                if t >= 1e7:
+                    break
+                new = pos + np.float32(t) * np.array([part["u"][0], part["v"][0], part["w"][0]], dtype=np.float32)
+                mid = pos + 0.5 * (new - pos)
+                exp_mid = classify(j, xy, is_y, *mid)
+                if region in (1, 2) and exp_mid in (1, 2):
+                    assert exp_mid == region, (is_y, pos, new, region, exp_mid)
+                exp_end = classify(j, xy, is_y, *new)
+                if exp_end != 0:
+                    if r == 0:
+                        # entry outside the +-0.5 cm field rectangle: discarded
+                        assert region == 0
+                        break
+                    assert r == exp_end, (is_y, pos, new, r, exp_end)
+                    checked += 1
+                pos = new.astype(np.float32)
+                region = r
+                if module != (1 if is_y else 2):
+                    # left the slab through the opening
+                    assert region == 1 and exp_end == 0
+                    assert module == (2 if is_y else 3)
+                    handed += 1
+                    break
+                if r == 0 and exp_end == 0 and region == 0:
+                    break
+    assert checked > 500 and handed > 200, (checked, handed)
+
+
+def test_ct_jaws_mlc_transport_conserves_and_collimates():
+    """Whole histories through jaws, MLC and phantom in the oracle: energy
+    balances (deposited + escaped + dropped = incident) up to the reference's
+    brems quirk -- a positron left below ECUT by a bremsstrahlung event
+    scores its kinetic energy and is not annihilated (brems.cu:281-282), so
+    2 m c^2 leave the balance each time -- and closing the jaws onto a
+    narrower field lowers the phantom dose."""
+    wl = workloads.ct_jaws_mlc()
+    src = wl.source(3000, 5)
+    o = oracle.run(wl, src, seed=3, per_history=True, threads=4)
+    assert o["counters"].photon_steps > 0
+    rm = 0.5110034
+    e = src["e"].astype(np.float64)
+    incident = float(np.sum(np.where(src["charge"] != 0, e - rm, e) * src["wt"]))
+    out = o["edep"].sum() + o["escaped"] + o["dropped"]
+    missing = incident - out
+    assert missing > -1e-6 * incident
+    # whole multiples of 2 m c^2
+    k = round(missing / (2 * rm))
+    assert abs(missing - k * 2 * rm) < 1e-5 * incident, (missing, k)
+    narrow = workloads.ct_jaws_mlc()
+    narrow.jaw_positions = np.array([-1.0, 1.0, -1.0, 1.0], dtype=np.float32)
+    o2 = oracle.run(narrow, src, seed=3, per_history=True, threads=4)
+    assert o2["dose"].sum() < 0.5 * o["dose"].sum()
