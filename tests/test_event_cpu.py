@@ -1,0 +1,107 @@
+"""The event-queued path's own kernels, compiled for the CPU, against the
+oracle: per-history energy and step counts must be identical.
+
+tests/cpu_event/event_cpu.cpp includes gegs_amd/csrc/event.hpp unchanged and
+compiles it as host C++ with AddressSanitizer (host code only) against
+tests/cpu_event/include/hip/hip_runtime.h, where every thread runs as a
+wavefront with one active lane.  This checks, without a GPU, the bank
+layout and packing, injection, suspension and resumption of flights
+(chunked photon walks, electron step state), queue routing between the ten
+event banks, the module chain jaws -> MLC -> phantom and the Philox stream
+bookkeeping; the LDS array is sized exactly to the staged bytes so that an
+out-of-range read of the staged tables fails the run.  What it cannot check
+is anything between lanes of one wavefront (the GPU tests do)."""
+import ctypes
+import os
+import shutil
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from gegs_amd import workloads
+from oracle import oracle
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CLANG = next((c for c in ("/opt/rocm/llvm/bin/clang++", shutil.which("clang++")) if c and os.path.exists(c)), None)
+
+pytestmark = pytest.mark.skipif(CLANG is None, reason="no clang++ for the CPU build of the event path")
+
+
+def lds_floats(wl):
+    """Bytes the flight kernels stage (env_lds_bytes with the record table
+    in LDS, one job) divided by 4."""
+    reg = np.ascontiguousarray(wl.regions)
+    nclass = len(np.unique(reg.view(np.dtype((np.void, reg.dtype.itemsize)))))
+    ph = wl.phantom
+    nmed = wl.tables.c.nmed
+    assert nclass <= 2048
+    b = 32 + (ph.nx + ph.ny + ph.nz + 6) * 4 + nclass * 16 + nmed * 8 + 2 * nmed * 200 * 8
+    return b // 4
+
+
+def build(tmp, floats):
+    exe = os.path.join(tmp, "event_cpu_%d" % floats)
+    if not os.path.exists(exe):
+        subprocess.check_call([CLANG, "-std=c++17", "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer",
+                               "-ffp-contract=off", "-I", os.path.join(HERE, "cpu_event", "include"),
+                               "-DLDS_FLOATS=%d" % floats, "-o", exe,
+                               os.path.join(HERE, "cpu_event", "event_cpu.cpp")])
+    return exe
+
+
+def dump(wl, src, seed, d, nbatch, pchunk, echunk, cap=1 << 16):
+    os.makedirs(d, exist_ok=True)
+    ph = wl.phantom
+    first = wl.field.module_first
+    mlc = getattr(wl, "mlc", None)
+    have_mlc = 1 if (first == 3 or (first == 1 and mlc is not None)) else 0
+    have_jaws = 1 if first == 1 else 0
+
+    def put(name, data):
+        with open(os.path.join(d, name), "wb") as f:
+            f.write(data)
+
+    put("meta.bin", struct.pack("<10I2Q", ph.nx, ph.ny, ph.nz, len(src), nbatch, pchunk, echunk, have_mlc,
+                                have_jaws, cap, seed, 0))
+    put("tables.bin", ctypes.string_at(ctypes.addressof(wl.tables.c), ctypes.sizeof(wl.tables.c)))
+    put("regions.bin", np.ascontiguousarray(wl.regions).tobytes())
+    for n, b in (("xb", ph.xb), ("yb", ph.yb), ("zb", ph.zb)):
+        put(n + ".bin", np.ascontiguousarray(b, dtype=np.float32).tobytes())
+    put("field.bin", ctypes.string_at(ctypes.addressof(wl.field), ctypes.sizeof(wl.field)))
+    put("src.bin", np.ascontiguousarray(src).tobytes())
+    sid = wl.field.source_to_isocenter
+    if have_mlc:
+        put("mlc.bin", ctypes.string_at(ctypes.addressof(mlc), ctypes.sizeof(mlc)))
+        put("endnodes.bin", oracle.leaf_ends(mlc, wl.leaf_positions, sid).tobytes())
+    if have_jaws:
+        put("jaws.bin", oracle.jaw_polys(wl.jaws, wl.jaw_positions, sid).tobytes())
+
+
+CASES = {
+    "water": (lambda: workloads.water(), 1500, 32, 16),
+    "ct": (lambda: workloads.ct(), 400, 32, 16),
+    "ct_suspend_every_step": (lambda: workloads.ct(), 150, 1, 1),
+    "ct_mlc": (lambda: workloads.ct_mlc(), 2500, 32, 16),
+    "ct_jaws_mlc": (lambda: workloads.ct_jaws_mlc(), 2500, 8, 4),
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_event_path_cpu_build_matches_oracle(case, tmp_path):
+    make, n, pchunk, echunk = CASES[case]
+    wl = make()
+    src = wl.source(n, 11)
+    d = str(tmp_path / case)
+    dump(wl, src, 77, d, 2, pchunk, echunk)
+    exe = build(str(tmp_path), lds_floats(wl))
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=0")
+    r = subprocess.run([exe, d], capture_output=True, text=True, env=env, timeout=600)
+    assert r.returncode == 0, r.stderr[-3000:]
+    edep = np.fromfile(os.path.join(d, "out_edep.bin"), dtype=np.float64)
+    steps = np.fromfile(os.path.join(d, "out_steps.bin"), dtype=np.uint32)
+    o = oracle.run(wl, src, seed=77, per_history=True, threads=4)
+    assert np.array_equal(steps, o["steps"])
+    assert np.allclose(edep, o["edep"], rtol=1e-12, atol=1e-12)
+    assert np.any(edep > 0)
