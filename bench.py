@@ -9,11 +9,13 @@ spectrum; 1e7 histories by default (10 steps of 1e6).  Beside it, the
 field weights, 2e7 phase-space records with 3% electrons) as five plain
 launches per step against one batched event loop per step.
 
-A step transports one batch of 1e6 source records already resident in HBM,
-once with the plain per-history kernel (the baseline method) and once with
-the batched event-queued loop (the headline).  Reported: histories/s of the
-whole job, dose efficiency 1/(sigma^2 T) (sigma = mean relative uncertainty
-over voxels above 50% of Dmax, from the step-to-step variance), the HBM
+A step is one statistical batch of 1e6 source records already resident in
+HBM; the timed call transports all steps, once with the plain per-history
+kernel (the baseline method, one launch per batch) and once with the batched
+event-queued loop (the headline, all batches in one loop).  Reported:
+histories/s of the whole job, dose efficiency 1/(sigma^2 T) (sigma = mean
+relative uncertainty over voxels above 50% of Dmax, from the batch
+variance), the HBM
 roofline of the event loop's dominant kernel (the photon flight kernel,
 timed live with HIP events), the adjoint gradient rate of a 5-field,
 2000-beamlet plan, and the CPU oracle timed on this host's cores on a
@@ -69,15 +71,6 @@ def algorithmic_bytes(counters, n_hist):
             + DEPOSIT_BYTES * counters["deposits"])
 
 
-def mean_rel_sigma_from_steps(step_doses):
-    d = np.stack(step_doses)                 # (K, nvox)
-    k = d.shape[0]
-    tot = d.sum(axis=0)
-    var_tot = d.var(axis=0, ddof=1) * k       # variance of the sum of K batches
-    sel = tot > 0.5 * tot.max()
-    return float(np.mean(np.sqrt(var_tot[sel]) / tot[sel])), int(sel.sum())
-
-
 def cpu_baseline(wl, budget_s, threads):
     from oracle import oracle
     probe = wl.source(50000 * threads, 991)
@@ -111,29 +104,28 @@ def add_counters(acc, c):
 
 
 def run_method(ctx, method, steps, warmup, b, seed, kernel_timing=False):
-    """Times `steps` steps of one batch of b records each.  With
-    kernel_timing the event path launches its sweeps directly on the stream
-    with a HIP event pair around every photon kernel launch (graph replays
-    cannot carry timed events); returns (elapsed, doses, counters, timing)."""
-    for w in range(warmup):
-        ctx.run(seed=seed, n_batches=1, method=method, first=w * b, count=b)
+    """Times `steps` steps of b records each, as one call of the method with
+    `steps` statistical batches (the plain kernel launches batch after
+    batch; the event path carries all batches in one loop and tallies each
+    in its own dose row).  With kernel_timing the event path launches its
+    sweeps directly on the stream with a HIP event pair around every photon
+    kernel launch (graph replays cannot carry timed events).  Returns
+    (elapsed, dose, dose2, counters, timing)."""
+    if warmup:
+        # same batch count as the timed call, so that its graph is reused
+        ctx.run(seed=seed, n_batches=steps, method=method, first=0, count=warmup * b)
     if kernel_timing:
         ctx.set_kernel_timing(True)
-    step_doses, counters = [], None
     _sync()
     t0 = time.perf_counter()
-    for k in range(steps):
-        r = ctx.run(seed=seed, n_batches=1, method=method, first=(warmup + k) * b, count=b)
-        step_doses.append(r["dose"])
-        counters = add_counters(counters, r["counters"])
+    r = ctx.run(seed=seed, n_batches=steps, method=method, first=warmup * b, count=steps * b)
     _sync()
     elapsed = time.perf_counter() - t0
     timing = None
     if kernel_timing:
         timing = ctx.kernel_timing()
         ctx.set_kernel_timing(False)
-    return elapsed, step_doses, counters, timing
-
+    return elapsed, r["dose"], r["dose2"], r["counters"], timing
 
 def plan_sigma(dose, dose2, n_batches):
     """Mean relative uncertainty over voxels above 50% of Dmax of a dose
@@ -289,16 +281,16 @@ def main():
         tuning, _, _ = autotune.autotune(ctx, 0, min(b, 500_000), seed=args.seed)
     results = {}
     for method in methods:
-        elapsed, doses, cnt, _ = run_method(ctx, method, args.steps, args.warmup, b, args.seed)
+        elapsed, dose, dose2, cnt, _ = run_method(ctx, method, args.steps, args.warmup, b, args.seed)
         hist = args.steps * b
-        sig, nsel = mean_rel_sigma_from_steps(doses)
+        sig, nsel, _ = plan_sigma(dose, dose2, args.steps)
         eff = 1.0 / (sig * sig * elapsed)
         res = dict(value=hist / elapsed, ms_per_step=elapsed / args.steps * 1e3, sigma=sig, voxels=nsel,
                    efficiency=eff, counters=cnt, bytes=algorithmic_bytes(cnt, hist))
         if method == "event":
             # the same steps again with every photon kernel launch timed by
             # HIP events on the library's stream
-            el2, _, cnt2, timing = run_method(ctx, method, args.steps, 0, b, args.seed, kernel_timing=True)
+            el2, _, _, cnt2, timing = run_method(ctx, method, args.steps, 0, b, args.seed, kernel_timing=True)
             if timing and timing[1]:
                 photon_ms, launches = timing
                 pbytes = PHOTON_STEP_BYTES * cnt2["photon_steps"]

@@ -20,12 +20,13 @@ SPACE = {
     "photon_chunk": [16, 32, 64, 128],
     "electron_chunk": [8, 16, 32, 64],
     "refill_min": [8, 16, 32, 48],
+    "refill_slots": [64, 256, 1024],
     "flight_block": [256, 512],
     "inject": [1 << 18, 1 << 19, 1 << 20],
     "sweeps_per_graph": [2, 4, 8],
 }
 
-START = dict(photon_chunk=64, electron_chunk=16, refill_min=32, flight_block=256,
+START = dict(photon_chunk=64, electron_chunk=16, refill_min=32, refill_slots=64, flight_block=256,
              inject=1 << 20, threshold=1 << 20, sweeps_per_graph=4)
 
 

@@ -54,6 +54,7 @@ struct EventConfig {
     uint32_t photon_chunk = 32;      // voxel crossings per photon and sweep (0: whole flight)
     uint32_t electron_chunk = 16;    // condensed-history steps per electron and sweep (0: no limit)
     uint32_t refill_min = 16;        // idle lanes of a wavefront that trigger a refill
+    uint32_t refill_slots = 64;      // bank slots a wavefront reserves per refill atomic
     size_t capacity = 0;             // slots per bank (0 = 4 Mi)
     uint32_t inject = 1u << 20;      // primaries injected per refill
     uint32_t threshold = 1u << 20;   // refill when fewer particles are in flight
@@ -95,6 +96,10 @@ struct EvArgs {
     // adjoint replay: dL/dD per plan and voxel, gradient per beamlet
     const double *adj;
     double *grad;
+    // statistical batches tallied in one pass: a deposit of history slot h
+    // of job j goes to dose row (plan(j) * tally_batches + batch(h)); 1: one
+    // row per plan
+    uint32_t tally_batches;
 };
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -181,9 +186,19 @@ __device__ __forceinline__ Part bank_load(const EvArgs &a, int slot, int event, 
 
 // deposits of a particle go to the dose buffer (forward) or are weighted by
 // the dL/dD (adjoint) of its job's plan
+// The batch of slot `hist` is the b-th of tally_batches equal slices of its
+// job, the slices of sequential batches: [floor(count b / nb), floor(count (b+1) / nb)).
 template <bool ADJ>
-__device__ __forceinline__ void bind_tally(const EvArgs &a, const Part &p, TallyT<ADJ> &t) {
-    const size_t off = (size_t)a.job_plan[p.job] * a.nvox;
+__device__ __forceinline__ void bind_tally(const EvArgs &a, const Part &p, uint32_t hist, TallyT<ADJ> &t) {
+    size_t row = a.job_plan[p.job];
+    if (!ADJ && a.tally_batches > 1) {
+        const BatchPlan &bp = *a.plan;
+        const uint64_t count = bp.prefix[p.job + 1] - bp.prefix[p.job];
+        const uint64_t off = hist - bp.slot0[p.job];
+        const uint64_t nb = a.tally_batches;
+        row = row * nb + ((off + 1) * nb - 1) / count;
+    }
+    const size_t off = row * a.nvox;
     if (ADJ) t.adj = a.adj + off;
     else t.dose = a.dose + off;
 }
@@ -325,7 +340,7 @@ __global__ void __launch_bounds__(256) interaction_kernel(EvArgs a) {
         if (idx < n) {
             uint32_t hist;
             Part p = bank_load(a, E, E, idx, hist);
-            bind_tally(a, p, t);
+            bind_tally(a, p, hist, t);
             sink.hist = hist;
             Stream s;
             s.begin(p, a.env.seed);
@@ -344,9 +359,11 @@ __global__ void __launch_bounds__(256) interaction_kernel(EvArgs a) {
     flush_counters(a, t, E, pops);
 }
 
-// Lane refill for the looping events: a wavefront reserves 64 bank slots
-// with one atomic and hands them to its idle lanes in order, reserving again
-// when they run out.
+// Lane refill for the looping events: a wavefront reserves `slots` (>= 64)
+// bank slots with one atomic and hands them to its idle lanes in order,
+// reserving again when they run out.  Every reserved slot is transported by
+// the wavefront that holds it: the loop ends only when the reservation is
+// used up and the bank is exhausted.
 struct Refill {
     uint32_t next, end;      // wave-uniform reservation
     bool exhausted;
@@ -358,27 +375,28 @@ struct Refill {
     // Idle lanes are refilled together once at least min_idle of them wait
     // (or the whole wavefront is idle), so that the loads that start a
     // particle are paid once for many lanes rather than on every iteration.
-    __device__ __forceinline__ uint32_t take(uint32_t *taken, uint32_t n, bool want, uint32_t min_idle) {
+    __device__ __forceinline__ uint32_t take(uint32_t *taken, uint32_t n, bool want, uint32_t min_idle,
+                                             uint32_t slots) {
         const uint64_t idle = __ballot(want);
         uint32_t need = (uint32_t)__popcll(idle);
         uint32_t slot = 0xFFFFFFFFu;
         const uint32_t rank = (uint32_t)__popcll(idle & lanes_below());
         if (need == 0 || exhausted || (need < min_idle && need < 64u)) return slot;
         if (end - next < need) {
-            // top up the reservation: keep the remainder, add 64 more
+            // top up the reservation: keep the remainder, add `slots` more
             uint32_t got = 0;
-            if (first_lane()) got = atomicAdd(taken, 64u);
+            if (first_lane()) got = atomicAdd(taken, slots);
             got = __builtin_amdgcn_readfirstlane(got);
             if (got >= n) {
                 exhausted = true;
             } else {
-                // remainder [next, end) is used first, then [got, got + 64)
+                // remainder [next, end) is used first, then [got, got + slots)
                 const uint32_t rem = end - next;
                 if (want && rank < rem) slot = next + rank;
                 else if (want) slot = got + (rank - rem);
                 const uint32_t used_new = need > rem ? need - rem : 0u;
                 next = got + used_new;
-                end = got + 64u;
+                end = got + slots;
                 return slot < n ? slot : 0xFFFFFFFFu;
             }
         }
@@ -392,7 +410,8 @@ struct Refill {
 // photon free flight with lane refill; a flight still going after `chunk`
 // crossings is suspended into the next sweep's bank
 template <bool TAB, bool ADJ>
-__global__ void __launch_bounds__(512) photon_kernel(EvArgs a, uint32_t chunk, uint32_t refill_min) {
+__global__ void __launch_bounds__(512) photon_kernel(EvArgs a, uint32_t chunk, uint32_t refill_min,
+                                                     uint32_t refill_slots) {
     extern __shared__ float lds_env[];
     const uint32_t flip = a.st->flip;
     const int slot_in = bank_slot(P_PHOTON_STEP, flip, true);
@@ -411,10 +430,10 @@ __global__ void __launch_bounds__(512) photon_kernel(EvArgs a, uint32_t chunk, u
     Refill rf;
     rf.init();
     for (;;) {
-        const uint32_t slot = rf.take(&a.st->taken[slot_in], n, !have, refill_min);
+        const uint32_t slot = rf.take(&a.st->taken[slot_in], n, !have, refill_min, refill_slots);
         if (!have && slot != 0xFFFFFFFFu) {
             p = bank_load(a, slot_in, P_PHOTON_STEP, slot, hist);
-            bind_tally(a, p, t);
+            bind_tally(a, p, hist, t);
             s.begin(p, env.seed);
             photon_walk_begin(env, p, w);
             have = true;
@@ -442,7 +461,8 @@ __global__ void __launch_bounds__(512) photon_kernel(EvArgs a, uint32_t chunk, u
 // electron / positron condensed-history steps with lane refill; an electron
 // still stepping after `chunk` steps is suspended into the next sweep's bank
 template <bool TAB, bool ADJ>
-__global__ void __launch_bounds__(512) electron_kernel(EvArgs a, uint32_t chunk, uint32_t refill_min) {
+__global__ void __launch_bounds__(512) electron_kernel(EvArgs a, uint32_t chunk, uint32_t refill_min,
+                                                       uint32_t refill_slots) {
     extern __shared__ float lds_env[];
     const uint32_t flip = a.st->flip;
     const int slot_in = bank_slot(E_ELECTRON_STEP, flip, true);
@@ -461,10 +481,10 @@ __global__ void __launch_bounds__(512) electron_kernel(EvArgs a, uint32_t chunk,
     Refill rf;
     rf.init();
     for (;;) {
-        const uint32_t slot = rf.take(&a.st->taken[slot_in], n, !have, refill_min);
+        const uint32_t slot = rf.take(&a.st->taken[slot_in], n, !have, refill_min, refill_slots);
         if (!have && slot != 0xFFFFFFFFu) {
             p = bank_load(a, slot_in, E_ELECTRON_STEP, slot, hist);
-            bind_tally(a, p, t);
+            bind_tally(a, p, hist, t);
             s.begin(p, env.seed);
             if (p.spos & 8) {
                 load_estate(a, slot_in, slot, es);
@@ -723,14 +743,14 @@ inline void launch_sweep_t(const EvArgs &a, const EventConfig &cfg, hipStream_t 
     hipLaunchKernelGGL(inject_fill, g, b, 0, stream, a);
     if (t0) (void)hipEventRecord(t0, stream);
     hipLaunchKernelGGL((photon_kernel<TAB, ADJ>), dim3(cfg.photon_grid), fb, lds, stream, a, cfg.photon_chunk,
-                       cfg.refill_min);
+                       cfg.refill_min, cfg.refill_slots);
     if (t1) (void)hipEventRecord(t1, stream);
     hipLaunchKernelGGL((interaction_kernel<P_RAYLEIGH, ADJ>), g, b, 0, stream, a);
     hipLaunchKernelGGL((interaction_kernel<P_COMPTON, ADJ>), g, b, 0, stream, a);
     hipLaunchKernelGGL((interaction_kernel<P_PHOTO, ADJ>), g, b, 0, stream, a);
     hipLaunchKernelGGL((interaction_kernel<P_PAIR, ADJ>), g, b, 0, stream, a);
     hipLaunchKernelGGL((electron_kernel<TAB, ADJ>), dim3(cfg.electron_grid), fb, lds, stream, a, cfg.electron_chunk,
-                       cfg.refill_min);
+                       cfg.refill_min, cfg.refill_slots);
     hipLaunchKernelGGL((interaction_kernel<E_MOLLER, ADJ>), g, b, 0, stream, a);
     hipLaunchKernelGGL((interaction_kernel<E_BREMS, ADJ>), g, b, 0, stream, a);
     hipLaunchKernelGGL((interaction_kernel<E_BHABHA, ADJ>), g, b, 0, stream, a);
@@ -778,14 +798,15 @@ inline bool same_args(const EvArgs &x, const EvArgs &y) {
            x.env.xb == y.env.xb && x.env.yb == y.env.yb && x.env.zb == y.env.zb && x.env.nx == y.env.nx &&
            x.env.ny == y.env.ny && x.env.nz == y.env.nz && x.env.pge == y.env.pge && x.env.pgmfp == y.env.pgmfp &&
            x.env.pcohe == y.env.pcohe && x.env.mlc == y.env.mlc && x.env.endnodes == y.env.endnodes &&
-           x.env.mlc_next == y.env.mlc_next && x.env.jaws == y.env.jaws && x.env.jawx_next == y.env.jawx_next;
+           x.env.mlc_next == y.env.mlc_next && x.env.jaws == y.env.jaws && x.env.jawx_next == y.env.jawx_next &&
+           x.tally_batches == y.tally_batches;
 }
 
 inline bool same_cfg(const EventConfig &x, const EventConfig &y) {
     return x.block == y.block && x.flight_block == y.flight_block && x.grid == y.grid && x.photon_grid == y.photon_grid &&
            x.electron_grid == y.electron_grid && x.photon_chunk == y.photon_chunk &&
            x.electron_chunk == y.electron_chunk && x.refill_min == y.refill_min && x.inject == y.inject &&
-           x.threshold == y.threshold && x.sweeps_per_graph == y.sweeps_per_graph;
+           x.threshold == y.threshold && x.sweeps_per_graph == y.sweeps_per_graph && x.refill_slots == y.refill_slots;
 }
 
 // Transport the histories of one batch (bk.h_plan, filled by the caller)
@@ -802,9 +823,14 @@ struct BeamletArgs {
 inline hipError_t run_event_batch(EventBanks &bk, const Env &env, const gegs_source *src, const uint32_t *job_plan,
                                   size_t nvox, double *dose, double *hist_edep, uint32_t *hist_steps,
                                   unsigned long long *counters, const BeamletArgs &bl, const EventConfig &cfg_in,
-                                  hipStream_t stream, uint32_t max_sweeps = 100000) {
+                                  hipStream_t stream, uint32_t tally_batches = 1, uint32_t max_sweeps = 1000000) {
     EventConfig cfg = cfg_in;
     flight_grids(cfg, env, bl.adj != nullptr);
+    // the work counters `taken` run past the bank size by at most one
+    // reservation per wavefront; keep them inside 32 bits
+    const uint64_t waves = (uint64_t)(cfg.photon_grid > cfg.electron_grid ? cfg.photon_grid : cfg.electron_grid) *
+                           (cfg.flight_block / 64);
+    if ((uint64_t)bk.cap + waves * cfg.refill_slots >= 0xFFFFFFFFull) return hipErrorInvalidValue;
     EvArgs a;
     a.env = env;
     a.plan = bk.d_plan;
@@ -822,6 +848,7 @@ inline hipError_t run_event_batch(EventBanks &bk, const Env &env, const gegs_sou
     a.bweight = bl.bweight;
     a.adj = bl.adj;
     a.grad = bl.grad;
+    a.tally_batches = tally_batches;
     hipError_t e;
     // With kernel timing the sweeps are launched directly on the stream,
     // with an event pair around every photon kernel launch (HIP reports no

@@ -58,7 +58,8 @@ struct gegs_ctx {
     gegs_source *d_src = nullptr;
     uint64_t nsrc = 0;
     double *d_chunk = nullptr, *d_dose = nullptr, *d_dose2 = nullptr;
-    size_t tally_n = 0;
+    size_t tally_n = 0;                  // capacity of d_dose / d_dose2
+    size_t chunk_n = 0;                  // capacity of d_chunk (batch rows)
     float *d_stack = nullptr;
     size_t stack_bytes = 0;
     uint32_t *d_next = nullptr;
@@ -306,17 +307,16 @@ int gegs_set_history_tally(gegs_ctx *c, int enable) {
 }
 
 // zeroed tallies for n_plans dose buffers and n_slots per-history slots
-static int prepare_tallies(gegs_ctx *c, uint32_t n_plans, uint64_t n_slots) {
+static int prepare_tallies(gegs_ctx *c, uint32_t n_plans, uint64_t n_slots, uint32_t batch_rows = 1) {
     const size_t n = c->nvox * n_plans;
+    HIPCHECK(c, realloc_dev(c->d_chunk, c->chunk_n, n * batch_rows), "hipMalloc tally");
     size_t have = c->tally_n;
-    HIPCHECK(c, realloc_dev(c->d_chunk, have, n), "hipMalloc tally");
-    have = c->tally_n;
     HIPCHECK(c, realloc_dev(c->d_dose, have, n), "hipMalloc tally");
     have = c->tally_n;
     HIPCHECK(c, realloc_dev(c->d_dose2, have, n), "hipMalloc tally");
     if (c->tally_n < n) c->tally_n = n;
     const int blocks = 2048;
-    hipLaunchKernelGGL(fill_zero, dim3(blocks), dim3(256), 0, c->stream, c->d_chunk, n);
+    hipLaunchKernelGGL(fill_zero, dim3(blocks), dim3(256), 0, c->stream, c->d_chunk, n * batch_rows);
     hipLaunchKernelGGL(fill_zero, dim3(blocks), dim3(256), 0, c->stream, c->d_dose, n);
     hipLaunchKernelGGL(fill_zero, dim3(blocks), dim3(256), 0, c->stream, c->d_dose2, n);
     hipLaunchKernelGGL(fill_zero_u64, dim3(1), dim3(64), 0, c->stream, c->d_counters, (size_t)16);
@@ -561,11 +561,26 @@ struct JobSpec {
 // Event-queued transport of every job in one loop: each batch holds the
 // b-th of n_batches equal slices of every job, so that every batch is an
 // independent sample of every plan's dose.
+// Batch rows held at once by a continuous run (bytes of per-batch dose rows).
+static constexpr size_t CONTINUOUS_TALLY_BYTES = (size_t)16 << 30;
+
 static int run_event_jobs(gegs_ctx *c, const gegs_run_options *o, const std::vector<gegs_field> &fields,
                           const std::vector<JobSpec> &jobs, uint32_t n_plans, uint64_t n_slots, double *dose,
                           double *dose2, const BeamletArgs &bl = BeamletArgs()) {
     hipSetDevice(c->device);
-    int rc = prepare_tallies(c, n_plans, n_slots);
+    // Continuous batching: all statistical batches go through one event loop
+    // (one drain at the end instead of one per batch), each deposit tallied
+    // in the dose row of its history's batch; the rows are folded into dose
+    // and dose2 afterwards.  The adjoint has no batch statistics and always
+    // runs in one pass.  Falls back to one loop per batch when the rows do
+    // not fit, or when a pass would exceed 2^32 - 16 histories.
+    const uint32_t nb = o->n_batches;
+    uint64_t total_all = 0;
+    for (const JobSpec &js : jobs) total_all += js.count;
+    const bool one_pass = total_all <= 0xFFFFFFF0ull &&
+                          (bl.adj || (size_t)nb * n_plans * c->nvox * sizeof(double) <= CONTINUOUS_TALLY_BYTES);
+    const uint32_t rows = (one_pass && !bl.adj) ? nb : 1u;
+    int rc = prepare_tallies(c, n_plans, n_slots, rows);
     if (rc) return rc;
     std::vector<uint32_t> plans(jobs.size()), sets(jobs.size());
     for (size_t j = 0; j < jobs.size(); ++j) {
@@ -575,7 +590,6 @@ static int run_event_jobs(gegs_ctx *c, const gegs_run_options *o, const std::vec
     rc = upload_jobs(c, fields, plans, sets);
     if (rc) return rc;
     const EventConfig cfg = c->tuning;
-    const uint32_t nb = o->n_batches;
     Env env = make_env(c, o->seed, (uint32_t)jobs.size());
     hipError_t e = c->banks.reserve(c->nsrc, nb, cfg);
     if (e != hipSuccess) return fail(c, GEGS_ERR_DEVICE, "event banks", e);
@@ -585,13 +599,14 @@ static int run_event_jobs(gegs_ctx *c, const gegs_run_options *o, const std::vec
     (void)hipEventCreate(&t1);
     (void)hipEventRecord(t0, c->stream);
     const size_t ntally = c->nvox * n_plans;
-    for (uint32_t b = 0; b < nb; ++b) {
+    const uint32_t passes = one_pass ? 1u : nb;
+    for (uint32_t b = 0; b < passes; ++b) {
         BatchPlan &bp = *c->banks.h_plan;
         bp.njob = (uint32_t)jobs.size();
         uint64_t total = 0;
         for (size_t j = 0; j < jobs.size(); ++j) {
             const JobSpec &js = jobs[j];
-            const uint64_t lo = js.count * b / nb, hi = js.count * (b + 1) / nb;
+            const uint64_t lo = js.count * b / passes, hi = js.count * (b + 1) / passes;
             bp.prefix[j] = (uint32_t)total;
             bp.rec0[j] = (uint32_t)(js.first + lo);
             bp.slot0[j] = (uint32_t)(js.slot0 + lo);
@@ -603,10 +618,11 @@ static int run_event_jobs(gegs_ctx *c, const gegs_run_options *o, const std::vec
         bp.total = (uint32_t)total;
         e = run_event_batch(c->banks, env, c->d_src, c->d_job_plan, c->nvox, c->d_chunk,
                             c->hist_tally ? c->d_hist_edep : nullptr, c->hist_tally ? c->d_hist_steps : nullptr,
-                            c->d_counters, bl, cfg, c->stream);
+                            c->d_counters, bl, cfg, c->stream, rows);
         if (e != hipSuccess) return fail(c, GEGS_ERR_DEVICE, "event batch", e);
         if (!bl.adj)
-            hipLaunchKernelGGL(fold_batch, dim3(2048), dim3(256), 0, c->stream, c->d_chunk, c->d_dose, c->d_dose2, ntally);
+            hipLaunchKernelGGL(fold_batches, dim3(2048), dim3(256), 0, c->stream, c->d_chunk, c->d_dose, c->d_dose2,
+                               c->nvox, (size_t)n_plans, rows);
     }
     hipLaunchKernelGGL(report_overflow, dim3(1), dim3(1), 0, c->stream, c->banks.st, c->d_counters);
     (void)hipEventRecord(t1, c->stream);
@@ -733,7 +749,10 @@ int gegs_set_tuning(gegs_ctx *c, const gegs_tuning *t) {
         if (t->threshold) cfg.threshold = t->threshold;
         if (t->sweeps_per_graph) cfg.sweeps_per_graph = t->sweeps_per_graph;
         if (t->capacity) cfg.capacity = (size_t)t->capacity;
+        if (t->refill_slots) cfg.refill_slots = t->refill_slots;
     }
+    if (cfg.refill_slots < 64 || cfg.refill_slots > 4096 || cfg.refill_slots % 64)
+        return fail(c, GEGS_ERR_ARG, "gegs_set_tuning: refill_slots must be a multiple of 64 in [64, 4096]");
     if (cfg.block != 256 && cfg.block != 128 && cfg.block != 64)
         return fail(c, GEGS_ERR_ARG, "gegs_set_tuning: block must be 64, 128 or 256");
     if (cfg.flight_block != 512 && cfg.flight_block != 256 && cfg.flight_block != 128 && cfg.flight_block != 64)
@@ -761,7 +780,7 @@ int gegs_get_tuning(gegs_ctx *c, gegs_tuning *t) {
     t->sweeps_per_graph = cfg.sweeps_per_graph;
     t->flight_block = cfg.flight_block;
     t->refill_min = cfg.refill_min;
-    t->reserved = 0;
+    t->refill_slots = c->tuning.refill_slots;
     t->capacity = cfg.capacity ? cfg.capacity : ((uint64_t)1 << 22);
     return GEGS_OK;
 }

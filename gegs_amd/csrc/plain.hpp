@@ -217,6 +217,27 @@ __global__ void fold_batch(double *chunk, double *dose, double *dose2, size_t n)
     }
 }
 
+// dose and dose2 of n_rows x nvox from nb per-batch rows each ([row][batch][voxel]),
+// clearing the batch rows
+__global__ void fold_batches(double *chunk, double *dose, double *dose2, size_t nvox, size_t n_rows, uint32_t nb) {
+    const size_t n = nvox * n_rows;
+    for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x) {
+        const size_t row = v / nvox, vox = v - row * nvox;
+        double s = 0.0, s2 = 0.0;
+        for (uint32_t b = 0; b < nb; ++b) {
+            double *c = chunk + (row * nb + b) * nvox + vox;
+            const double d = *c;
+            if (d != 0.0) {
+                s += d;
+                s2 += d * d;
+                *c = 0.0;
+            }
+        }
+        dose[v] += s;
+        dose2[v] += s2;
+    }
+}
+
 __global__ void fill_zero_u32(uint32_t *p, size_t n) {
     for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x)
         p[v] = 0u;

@@ -142,7 +142,7 @@ typedef struct gegs_tuning {
     uint32_t sweeps_per_graph;   /* sweeps captured in one graph replay (4) */
     uint32_t flight_block;       /* threads per block of the photon and electron kernels: 64 ... 512 (256) */
     uint32_t refill_min;         /* idle lanes of a wavefront that trigger a refill, 1 ... 64 (16) */
-    uint32_t reserved;
+    uint32_t refill_slots;       /* bank slots a wavefront of the photon/electron kernels reserves per atomic: 64 ... 4096, multiple of 64 (64) */
     uint64_t capacity;           /* particles per bank (4 Mi) */
 } gegs_tuning;
 

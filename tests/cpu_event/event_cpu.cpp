@@ -131,7 +131,8 @@ int main(int argc, char **argv) {
     EvState *st = (EvState *)calloc(1, sizeof(EvState));
     BatchPlan *bp = (BatchPlan *)calloc(1, sizeof(BatchPlan));
     uint32_t *job_plan = (uint32_t *)calloc(1, sizeof(uint32_t));
-    double *dose = (double *)calloc(nvox, sizeof(double));
+    // one dose row per statistical batch, all batches in one pass
+    double *dose = (double *)calloc(nvox * m.nbatch, sizeof(double));
     double *hist_edep = (double *)calloc(m.nsrc, sizeof(double));
     uint32_t *hist_steps = (uint32_t *)calloc(m.nsrc, sizeof(uint32_t));
     unsigned long long *counters = (unsigned long long *)calloc(16, sizeof(unsigned long long));
@@ -153,6 +154,7 @@ int main(int argc, char **argv) {
     a.bweight = nullptr;
     a.adj = nullptr;
     a.grad = nullptr;
+    a.tally_batches = m.nbatch;
     // one-lane wavefronts: a refill needs only one idle lane; the flight
     // kernels run one thread per block so that each stages all of LDS
     EventConfig cfg;
@@ -165,27 +167,27 @@ int main(int argc, char **argv) {
     cfg.electron_chunk = m.echunk;
     cfg.refill_min = 1;
     uint64_t sweeps = 0;
-    for (uint32_t b = 0; b < m.nbatch; ++b) {
-        const uint64_t lo = (uint64_t)m.nsrc * b / m.nbatch, hi = (uint64_t)m.nsrc * (b + 1) / m.nbatch;
-        bp->njob = 1;
-        bp->total = (uint32_t)(hi - lo);
-        bp->prefix[0] = 0;
-        bp->prefix[1] = (uint32_t)(hi - lo);
-        bp->rec0[0] = (uint32_t)lo;
-        bp->slot0[0] = (uint32_t)lo;
-        bp->hist0[0] = m.hist_offset + lo;
-        cpu_launch(batch_begin, dim3(1), dim3(1), st, 0u, (uint32_t)(hi - lo));
-        for (;;) {
-            launch_sweep(a, cfg, nullptr);
-            ++sweeps;
-            if (st->overflow) return 5;
-            if (!st->active) break;
-            if (sweeps > 1000000) return 6;
-        }
+    bp->njob = 1;
+    bp->total = m.nsrc;
+    bp->prefix[0] = 0;
+    bp->prefix[1] = m.nsrc;
+    bp->rec0[0] = 0;
+    bp->slot0[0] = 0;
+    bp->hist0[0] = m.hist_offset;
+    // a small injection size keeps several generations of particles in flight
+    cfg.inject = 256;
+    cfg.threshold = 512;
+    cpu_launch(batch_begin, dim3(1), dim3(1), st, 0u, m.nsrc);
+    for (;;) {
+        launch_sweep(a, cfg, nullptr);
+        ++sweeps;
+        if (st->overflow) return 5;
+        if (!st->active) break;
+        if (sweeps > 1000000) return 6;
     }
     spill(d + "/out_edep.bin", hist_edep, m.nsrc * sizeof(double));
     spill(d + "/out_steps.bin", hist_steps, m.nsrc * sizeof(uint32_t));
-    spill(d + "/out_dose.bin", dose, nvox * sizeof(double));
+    spill(d + "/out_dose.bin", dose, nvox * m.nbatch * sizeof(double));
     printf("sweeps %llu\n", (unsigned long long)sweeps);
     return 0;
 }

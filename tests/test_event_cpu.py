@@ -7,8 +7,9 @@ tests/cpu_event/include/hip/hip_runtime.h, where every thread runs as a
 wavefront with one active lane.  This checks, without a GPU, the bank
 layout and packing, injection, suspension and resumption of flights
 (chunked photon walks, electron step state), queue routing between the ten
-event banks, the module chain jaws -> MLC -> phantom and the Philox stream
-bookkeeping; the LDS array is sized exactly to the staged bytes so that an
+event banks, the module chain jaws -> MLC -> phantom, the Philox stream
+bookkeeping and continuous batching (two statistical batches in one pass,
+each deposit in its history's batch row); the LDS array is sized exactly to the staged bytes so that an
 out-of-range read of the staged tables fails the run.  What it cannot check
 is anything between lanes of one wavefront (the GPU tests do)."""
 import ctypes
@@ -101,7 +102,13 @@ def test_event_path_cpu_build_matches_oracle(case, tmp_path):
     assert r.returncode == 0, r.stderr[-3000:]
     edep = np.fromfile(os.path.join(d, "out_edep.bin"), dtype=np.float64)
     steps = np.fromfile(os.path.join(d, "out_steps.bin"), dtype=np.uint32)
+    rows = np.fromfile(os.path.join(d, "out_dose.bin"), dtype=np.float64).reshape(2, -1)
     o = oracle.run(wl, src, seed=77, per_history=True, threads=4)
     assert np.array_equal(steps, o["steps"])
     assert np.allclose(edep, o["edep"], rtol=1e-12, atol=1e-12)
     assert np.any(edep > 0)
+    # both batches ran in one pass; each deposit is in its history's batch row
+    half = n // 2
+    for b, (lo, hi) in enumerate(((0, half), (half, n))):
+        ob = oracle.run(wl, src[lo:hi], seed=77, hist_offset=lo, threads=4)
+        assert np.allclose(rows[b], ob["dose"], rtol=1e-11, atol=1e-12), b
