@@ -34,15 +34,25 @@ constexpr int SLOT_P_ALT = N_EVENTS;         // second buffer of the photon-step
 constexpr int SLOT_E_ALT = N_EVENTS + 1;     // second buffer of the electron-step bank
 constexpr int N_SLOTS = N_EVENTS + 2;
 
+// A queue counter alone in a 256-byte block.  Device-scope atomics are
+// performed beyond the L2 of the XCD that issues them; while one is out, its
+// L2 channel holds the line, and the channel's later requests wait behind
+// it.  Counters that share a line serialize each other's atomics, and a
+// wavefront load with any lane mapped to that channel (most gathers of 64
+// scattered voxel classes have one) waits with them.
+struct alignas(256) QCount {
+    uint32_t v;
+};
+
 struct EvState {
-    uint32_t count[N_SLOTS];         // particles in each bank
-    uint32_t taken[N_SLOTS];         // work handed out from each bank
     uint32_t overflow;               // particles that did not fit
-    uint32_t active;                 // more work remains after the sweep
+    uint32_t active;                 // more work remains after the sweep (read by the host with `sweeps`)
     uint32_t sweeps;
     uint32_t flip;                   // buffer of banks 0 and 5 drained this sweep
     uint32_t src_next, src_last;     // source records still to inject
     uint32_t inj_start, inj_n;       // current injection
+    QCount count[N_SLOTS];           // particles in each bank
+    QCount taken[N_SLOTS];           // work handed out from each bank
 };
 
 struct EventConfig {
@@ -230,7 +240,7 @@ struct BankSink {
             const bool mine = dest == d0;
             const uint64_t m = __ballot(mine);
             uint32_t base = 0;
-            if ((int)lane_id() == leader) base = atomicAdd(&a->st->count[d0], (uint32_t)__popcll(m));
+            if ((int)lane_id() == leader) base = atomicAdd(&a->st->count[d0].v, (uint32_t)__popcll(m));
             base = __builtin_amdgcn_readlane(base, leader);
             if (mine) {
                 const uint32_t idx = base + (uint32_t)__popcll(m & lanes_below());
@@ -247,7 +257,7 @@ struct BankSink {
         const uint64_t m = __ballot(1);
         const uint32_t rank = (uint32_t)__popcll(m & lanes_below());
         uint32_t base = 0;
-        if (rank == 0) base = atomicAdd(&a->st->count[dest], (uint32_t)__popcll(m));
+        if (rank == 0) base = atomicAdd(&a->st->count[dest].v, (uint32_t)__popcll(m));
         base = __builtin_amdgcn_readfirstlane(base);
         const uint32_t idx = base + rank;
         if (idx < a->cap) {
@@ -327,7 +337,7 @@ __device__ __forceinline__ TallyT<ADJ> fresh_tally(const EvArgs &a) {
 // grid; an empty bank costs one load per wave.
 template <int E, bool ADJ>
 __global__ void __launch_bounds__(256) interaction_kernel(EvArgs a) {
-    const uint32_t n = a.st->count[E];
+    const uint32_t n = a.st->count[E].v;
     if (n == 0) return;
     TallyT<ADJ> t = fresh_tally<ADJ>(a);
     // photon interactions run before the electron kernel of the sweep
@@ -407,20 +417,31 @@ struct Refill {
     }
 };
 
+// A flight ends with at most one push, of the photon itself (to the bank of
+// its interaction, or suspended): the walk marks it here and the particle
+// stays in the lane's registers until the wavefront next refills.
+struct HoldSink {
+    bool held;
+    __device__ __forceinline__ void push(const Part &) { held = true; }
+};
+
 // photon free flight with lane refill; a flight still going after `chunk`
-// crossings is suspended into the next sweep's bank
+// crossings is suspended into the next sweep's bank.  The photons of ended
+// flights are appended together when the wavefront refills (or runs dry),
+// so that one atomic per destination serves many lanes.
 template <bool TAB, bool ADJ>
 __global__ void __launch_bounds__(512) photon_kernel(EvArgs a, uint32_t chunk, uint32_t refill_min,
                                                      uint32_t refill_slots) {
     extern __shared__ float lds_env[];
     const uint32_t flip = a.st->flip;
     const int slot_in = bank_slot(P_PHOTON_STEP, flip, true);
-    const uint32_t n = a.st->count[slot_in];
+    const uint32_t n = a.st->count[slot_in].v;
     if (n == 0) return;
     Env env = a.env;
     stage_env<TAB>(env, lds_env);
     TallyT<ADJ> t = fresh_tally<ADJ>(a);
     BankSink sink{&a, 0, flip, false, true};
+    HoldSink hold{false};
     uint32_t pops = 0;
     bool have = false;
     Part p;
@@ -430,7 +451,17 @@ __global__ void __launch_bounds__(512) photon_kernel(EvArgs a, uint32_t chunk, u
     Refill rf;
     rf.init();
     for (;;) {
-        const uint32_t slot = rf.take(&a.st->taken[slot_in], n, !have, refill_min, refill_slots);
+        // the held photons go out before a refill can overwrite their lanes:
+        // the same condition as in Refill::take, or no refill is coming
+        const uint32_t nidle = (uint32_t)__popcll(__ballot(!have));
+        if (nidle && (nidle >= refill_min || nidle == 64u || rf.exhausted) && __any(hold.held)) {
+            if (hold.held) {
+                sink.hist = hist;
+                sink.push(p);
+                hold.held = false;
+            }
+        }
+        const uint32_t slot = rf.take(&a.st->taken[slot_in].v, n, !have, refill_min, refill_slots);
         if (!have && slot != 0xFFFFFFFFu) {
             p = bank_load(a, slot_in, P_PHOTON_STEP, slot, hist);
             bind_tally(a, p, hist, t);
@@ -443,12 +474,11 @@ __global__ void __launch_bounds__(512) photon_kernel(EvArgs a, uint32_t chunk, u
         if (!__any(have)) break;
         wave_lookahead(s, have);
         if (have) {
-            sink.hist = hist;
             if (chunk && iters == chunk) {
-                suspend(sink, s, p, P_PHOTON_STEP);
+                suspend(hold, s, p, P_PHOTON_STEP);
                 have = false;
                 flush_particle(a, hist, t);
-            } else if (!photon_walk_iter(env, s, p, w, sink, t)) {
+            } else if (!photon_walk_iter(env, s, p, w, hold, t)) {
                 have = false;
                 flush_particle(a, hist, t);
             }
@@ -466,7 +496,7 @@ __global__ void __launch_bounds__(512) electron_kernel(EvArgs a, uint32_t chunk,
     extern __shared__ float lds_env[];
     const uint32_t flip = a.st->flip;
     const int slot_in = bank_slot(E_ELECTRON_STEP, flip, true);
-    const uint32_t n = a.st->count[slot_in];
+    const uint32_t n = a.st->count[slot_in].v;
     if (n == 0) return;
     Env env = a.env;
     stage_env<TAB>(env, lds_env);
@@ -481,7 +511,7 @@ __global__ void __launch_bounds__(512) electron_kernel(EvArgs a, uint32_t chunk,
     Refill rf;
     rf.init();
     for (;;) {
-        const uint32_t slot = rf.take(&a.st->taken[slot_in], n, !have, refill_min, refill_slots);
+        const uint32_t slot = rf.take(&a.st->taken[slot_in].v, n, !have, refill_min, refill_slots);
         if (!have && slot != 0xFFFFFFFFu) {
             p = bank_load(a, slot_in, E_ELECTRON_STEP, slot, hist);
             bind_tally(a, p, hist, t);
@@ -520,7 +550,7 @@ __global__ void __launch_bounds__(512) electron_kernel(EvArgs a, uint32_t chunk,
 // plan the next injection: refill when few particles are in flight
 __global__ void inject_plan(EvState *st, uint32_t threshold, uint32_t max_inject) {
     uint32_t inflight = 0;
-    for (int e = 0; e < N_SLOTS; ++e) inflight += st->count[e];
+    for (int e = 0; e < N_SLOTS; ++e) inflight += st->count[e].v;
     uint32_t n = 0;
     if (inflight < threshold && st->src_next < st->src_last) {
         n = st->src_last - st->src_next;
@@ -585,24 +615,24 @@ __global__ void sweep_end(EvState *st) {
     // pushes after the flip
     const int drained[8] = {P_COMPTON, P_PHOTO, P_RAYLEIGH, P_PAIR, E_MOLLER, E_BREMS, E_BHABHA, E_ANNIH};
     for (int k = 0; k < 8; ++k) {
-        st->count[drained[k]] = 0;
-        st->taken[drained[k]] = 0;
+        st->count[drained[k]].v = 0;
+        st->taken[drained[k]].v = 0;
     }
     const int p_in = st->flip ? SLOT_P_ALT : P_PHOTON_STEP;
     const int e_in = st->flip ? SLOT_E_ALT : E_ELECTRON_STEP;
-    st->count[p_in] = st->taken[p_in] = 0;
-    st->count[e_in] = st->taken[e_in] = 0;
+    st->count[p_in].v = st->taken[p_in].v = 0;
+    st->count[e_in].v = st->taken[e_in].v = 0;
     st->flip ^= 1u;
     uint32_t inflight = 0;
-    for (int e = 0; e < N_SLOTS; ++e) inflight += st->count[e];
+    for (int e = 0; e < N_SLOTS; ++e) inflight += st->count[e].v;
     st->active = (inflight > 0 || st->src_next < st->src_last) ? 1u : 0u;
     st->sweeps += 1;
 }
 
 __global__ void batch_begin(EvState *st, uint32_t first, uint32_t last) {
     for (int e = 0; e < N_SLOTS; ++e) {
-        st->count[e] = 0;
-        st->taken[e] = 0;
+        st->count[e].v = 0;
+        st->taken[e].v = 0;
     }
     st->flip = 0;
     st->src_next = first;

@@ -128,7 +128,9 @@ int main(int argc, char **argv) {
         return 7;
     }
 
-    EvState *st = (EvState *)calloc(1, sizeof(EvState));
+    // the queue counters are 256-byte aligned blocks
+    EvState *st = (EvState *)aligned_alloc(alignof(EvState), sizeof(EvState));
+    memset(st, 0, sizeof(EvState));
     BatchPlan *bp = (BatchPlan *)calloc(1, sizeof(BatchPlan));
     uint32_t *job_plan = (uint32_t *)calloc(1, sizeof(uint32_t));
     // one dose row per statistical batch, all batches in one pass
