@@ -44,6 +44,13 @@ struct alignas(256) QCount {
     uint32_t v;
 };
 
+// Every bank is split into QSEG equal append segments, each with its own
+// counter: block b appends to segment b % QSEG.  Blocks are dealt to the 8
+// XCDs in turn, so a segment's counter is used by one XCD, and each XCD
+// spreads its appends to one bank over QSEG / 8 lines instead of queueing
+// them on one.  Consumers see a bank as one range through BankView.
+constexpr int QSEG = 32;
+
 struct EvState {
     uint32_t overflow;               // particles that did not fit
     uint32_t active;                 // more work remains after the sweep (read by the host with `sweeps`)
@@ -51,8 +58,9 @@ struct EvState {
     uint32_t flip;                   // buffer of banks 0 and 5 drained this sweep
     uint32_t src_next, src_last;     // source records still to inject
     uint32_t inj_start, inj_n;       // current injection
-    QCount count[N_SLOTS];           // particles in each bank
-    QCount taken[N_SLOTS];           // work handed out from each bank
+    uint32_t inflight;               // particles in the banks when the last sweep ended
+    QCount count[N_SLOTS][QSEG];     // particles appended to each segment of each bank
+    QCount taken[N_SLOTS];           // work handed out from each bank (over its whole range)
 };
 
 struct EventConfig {
@@ -93,6 +101,7 @@ struct EvArgs {
     size_t nvox;
     float *bank;                     // [slot][word][cap]
     uint32_t cap;
+    uint32_t seg_cap;                // cap / QSEG: slots per append segment
     EvState *st;
     const gegs_source *src;
     double *dose;
@@ -122,6 +131,35 @@ __device__ __forceinline__ uint64_t lanes_below() {
 // the lowest active lane issues a wavefront's shared atomics (lane 0 where
 // the whole wavefront is active)
 __device__ __forceinline__ bool first_lane() { return (__ballot(1) & lanes_below()) == 0; }
+
+// A bank's particles as one range [0, total()): segment g contributes
+// [pre[g], pre[g+1]), stored at bank slots g * seg_cap + [0, pre[g+1] - pre[g]).
+// Counts past seg_cap are appends that overflowed (counted in `overflow`).
+struct BankView {
+    uint32_t pre[QSEG + 1];
+    __device__ __forceinline__ void load(const EvArgs &a, int slot) {
+        pre[0] = 0;
+#pragma unroll
+        for (int g = 0; g < QSEG; ++g) {
+            const uint32_t c = a.st->count[slot][g].v;
+            pre[g + 1] = pre[g] + (c < a.seg_cap ? c : a.seg_cap);
+        }
+    }
+    __device__ __forceinline__ uint32_t total() const { return pre[QSEG]; }
+    // bank slot of range position k < total(): the last segment that starts
+    // at or before k holds it (empty segments start where the next one does)
+    __device__ __forceinline__ uint32_t slot_of(const EvArgs &a, uint32_t k) const {
+        uint32_t base = 0, off = k;
+#pragma unroll
+        for (int g = 1; g < QSEG; ++g) {
+            if (k >= pre[g]) {
+                base = (uint32_t)g * a.seg_cap;
+                off = k - pre[g];
+            }
+        }
+        return base + off;
+    }
+};
 
 // bank buffer of event e drained in this sweep (now) or in the next one
 __device__ __forceinline__ int bank_slot(int e, uint32_t flip, bool now) {
@@ -231,6 +269,7 @@ struct BankSink {
     // compiler may assume the first iteration exits and drop the test, which
     // sends every lane's particle to the first lane's destination count.
     __device__ __forceinline__ void push(const Part &p) {
+        const uint32_t seg = blockIdx.x % QSEG;
         const int e = p.status;
         const int dest = bank_slot(e, flip, e == P_PHOTON_STEP ? photons_now : electrons_now);
         uint64_t todo = __ballot(1);
@@ -240,11 +279,11 @@ struct BankSink {
             const bool mine = dest == d0;
             const uint64_t m = __ballot(mine);
             uint32_t base = 0;
-            if ((int)lane_id() == leader) base = atomicAdd(&a->st->count[d0].v, (uint32_t)__popcll(m));
+            if ((int)lane_id() == leader) base = atomicAdd(&a->st->count[d0][seg].v, (uint32_t)__popcll(m));
             base = __builtin_amdgcn_readlane(base, leader);
             if (mine) {
                 const uint32_t idx = base + (uint32_t)__popcll(m & lanes_below());
-                if (idx < a->cap) bank_store(*a, d0, idx, p, hist);
+                if (idx < a->seg_cap) bank_store(*a, d0, seg * a->seg_cap + idx, p, hist);
                 else atomicAdd(&a->st->overflow, 1u);
             }
             todo &= ~m;
@@ -253,14 +292,16 @@ struct BankSink {
     // an electron suspended between two condensed-history steps carries the
     // step state (electron_iter's EState) to the sweep that resumes it
     __device__ __forceinline__ void push_suspended(const Part &p, const EState &es) {
+        const uint32_t seg = blockIdx.x % QSEG;
         const int dest = bank_slot(p.status, flip, electrons_now);
         const uint64_t m = __ballot(1);
         const uint32_t rank = (uint32_t)__popcll(m & lanes_below());
         uint32_t base = 0;
-        if (rank == 0) base = atomicAdd(&a->st->count[dest].v, (uint32_t)__popcll(m));
+        if (rank == 0) base = atomicAdd(&a->st->count[dest][seg].v, (uint32_t)__popcll(m));
         base = __builtin_amdgcn_readfirstlane(base);
-        const uint32_t idx = base + rank;
-        if (idx < a->cap) {
+        const uint32_t at = base + rank;
+        const uint32_t idx = seg * a->seg_cap + at;
+        if (at < a->seg_cap) {
             bank_store(*a, dest, idx, p, hist);
             bank_put(*a, dest, 15, idx, es.elke);
             bank_put(*a, dest, 16, idx, es.demfp);
@@ -337,7 +378,9 @@ __device__ __forceinline__ TallyT<ADJ> fresh_tally(const EvArgs &a) {
 // grid; an empty bank costs one load per wave.
 template <int E, bool ADJ>
 __global__ void __launch_bounds__(256) interaction_kernel(EvArgs a) {
-    const uint32_t n = a.st->count[E].v;
+    BankView view;
+    view.load(a, E);
+    const uint32_t n = view.total();
     if (n == 0) return;
     TallyT<ADJ> t = fresh_tally<ADJ>(a);
     // photon interactions run before the electron kernel of the sweep
@@ -349,7 +392,7 @@ __global__ void __launch_bounds__(256) interaction_kernel(EvArgs a) {
         const uint32_t idx = base + lane_id();
         if (idx < n) {
             uint32_t hist;
-            Part p = bank_load(a, E, E, idx, hist);
+            Part p = bank_load(a, E, E, view.slot_of(a, idx), hist);
             bind_tally(a, p, hist, t);
             sink.hist = hist;
             Stream s;
@@ -435,7 +478,9 @@ __global__ void __launch_bounds__(512) photon_kernel(EvArgs a, uint32_t chunk, u
     extern __shared__ float lds_env[];
     const uint32_t flip = a.st->flip;
     const int slot_in = bank_slot(P_PHOTON_STEP, flip, true);
-    const uint32_t n = a.st->count[slot_in].v;
+    BankView view;
+    view.load(a, slot_in);
+    const uint32_t n = view.total();
     if (n == 0) return;
     Env env = a.env;
     stage_env<TAB>(env, lds_env);
@@ -463,7 +508,7 @@ __global__ void __launch_bounds__(512) photon_kernel(EvArgs a, uint32_t chunk, u
         }
         const uint32_t slot = rf.take(&a.st->taken[slot_in].v, n, !have, refill_min, refill_slots);
         if (!have && slot != 0xFFFFFFFFu) {
-            p = bank_load(a, slot_in, P_PHOTON_STEP, slot, hist);
+            p = bank_load(a, slot_in, P_PHOTON_STEP, view.slot_of(a, slot), hist);
             bind_tally(a, p, hist, t);
             s.begin(p, env.seed);
             photon_walk_begin(env, p, w);
@@ -496,7 +541,9 @@ __global__ void __launch_bounds__(512) electron_kernel(EvArgs a, uint32_t chunk,
     extern __shared__ float lds_env[];
     const uint32_t flip = a.st->flip;
     const int slot_in = bank_slot(E_ELECTRON_STEP, flip, true);
-    const uint32_t n = a.st->count[slot_in].v;
+    BankView view;
+    view.load(a, slot_in);
+    const uint32_t n = view.total();
     if (n == 0) return;
     Env env = a.env;
     stage_env<TAB>(env, lds_env);
@@ -513,11 +560,12 @@ __global__ void __launch_bounds__(512) electron_kernel(EvArgs a, uint32_t chunk,
     for (;;) {
         const uint32_t slot = rf.take(&a.st->taken[slot_in].v, n, !have, refill_min, refill_slots);
         if (!have && slot != 0xFFFFFFFFu) {
-            p = bank_load(a, slot_in, E_ELECTRON_STEP, slot, hist);
+            const uint32_t at = view.slot_of(a, slot);
+            p = bank_load(a, slot_in, E_ELECTRON_STEP, at, hist);
             bind_tally(a, p, hist, t);
             s.begin(p, env.seed);
             if (p.spos & 8) {
-                load_estate(a, slot_in, slot, es);
+                load_estate(a, slot_in, at, es);
             } else {
                 es.init();
                 pops++;
@@ -549,10 +597,8 @@ __global__ void __launch_bounds__(512) electron_kernel(EvArgs a, uint32_t chunk,
 
 // plan the next injection: refill when few particles are in flight
 __global__ void inject_plan(EvState *st, uint32_t threshold, uint32_t max_inject) {
-    uint32_t inflight = 0;
-    for (int e = 0; e < N_SLOTS; ++e) inflight += st->count[e].v;
     uint32_t n = 0;
-    if (inflight < threshold && st->src_next < st->src_last) {
+    if (st->inflight < threshold && st->src_next < st->src_last) {
         n = st->src_last - st->src_next;
         if (n > max_inject) n = max_inject;
     }
@@ -613,27 +659,30 @@ __global__ void sweep_end(EvState *st) {
     // every bank drained in this sweep is empty: the eight single-pass banks
     // and the current buffers of banks 0 and 5, which take the next sweep's
     // pushes after the flip
-    const int drained[8] = {P_COMPTON, P_PHOTO, P_RAYLEIGH, P_PAIR, E_MOLLER, E_BREMS, E_BHABHA, E_ANNIH};
-    for (int k = 0; k < 8; ++k) {
-        st->count[drained[k]].v = 0;
-        st->taken[drained[k]].v = 0;
-    }
     const int p_in = st->flip ? SLOT_P_ALT : P_PHOTON_STEP;
     const int e_in = st->flip ? SLOT_E_ALT : E_ELECTRON_STEP;
-    st->count[p_in].v = st->taken[p_in].v = 0;
-    st->count[e_in].v = st->taken[e_in].v = 0;
-    st->flip ^= 1u;
+    const int drained[10] = {P_COMPTON, P_PHOTO, P_RAYLEIGH, P_PAIR, E_MOLLER, E_BREMS, E_BHABHA, E_ANNIH, p_in, e_in};
+    for (int k = 0; k < 10; ++k) {
+        for (int g = 0; g < QSEG; ++g) st->count[drained[k]][g].v = 0;
+        st->taken[drained[k]].v = 0;
+    }
+    // what is left in flight is in the other buffers of banks 0 and 5
+    const int p_next = st->flip ? P_PHOTON_STEP : SLOT_P_ALT;
+    const int e_next = st->flip ? E_ELECTRON_STEP : SLOT_E_ALT;
     uint32_t inflight = 0;
-    for (int e = 0; e < N_SLOTS; ++e) inflight += st->count[e].v;
+    for (int g = 0; g < QSEG; ++g) inflight += st->count[p_next][g].v + st->count[e_next][g].v;
+    st->flip ^= 1u;
+    st->inflight = inflight;
     st->active = (inflight > 0 || st->src_next < st->src_last) ? 1u : 0u;
     st->sweeps += 1;
 }
 
 __global__ void batch_begin(EvState *st, uint32_t first, uint32_t last) {
     for (int e = 0; e < N_SLOTS; ++e) {
-        st->count[e].v = 0;
+        for (int g = 0; g < QSEG; ++g) st->count[e][g].v = 0;
         st->taken[e].v = 0;
     }
+    st->inflight = 0;
     st->flip = 0;
     st->src_next = first;
     st->src_last = last;
@@ -670,8 +719,9 @@ struct EventBanks {
     uint64_t photon_launches = 0;
 
     hipError_t reserve(uint64_t, uint32_t, const EventConfig &cfg) {
-        const size_t want = cfg.capacity ? cfg.capacity : ((size_t)1 << 22);
-        if (want > 0xFFFFFFF0ull) return hipErrorInvalidValue;
+        // a whole number of append segments per bank
+        const size_t want = (cfg.capacity ? cfg.capacity : ((size_t)1 << 22)) / QSEG * QSEG;
+        if (want > 0xFFFFFFF0ull || want < (size_t)QSEG) return hipErrorInvalidValue;
         hipError_t e = hipSuccess;
         if (!st) {
             e = hipMalloc(&st, sizeof(EvState));
@@ -868,6 +918,7 @@ inline hipError_t run_event_batch(EventBanks &bk, const Env &env, const gegs_sou
     a.nvox = nvox;
     a.bank = bk.bank;
     a.cap = bk.cap;
+    a.seg_cap = bk.cap / QSEG;
     a.st = bk.st;
     a.src = src;
     a.dose = dose;

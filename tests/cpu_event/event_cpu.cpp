@@ -146,6 +146,7 @@ int main(int argc, char **argv) {
     a.nvox = nvox;
     a.bank = bank;
     a.cap = m.cap;
+    a.seg_cap = m.cap / QSEG;
     a.st = st;
     a.src = src;
     a.dose = dose;
