@@ -192,3 +192,89 @@ def mlc_plan(gantry=C3_GANTRY, seed=3, weight_range=(0.1, 0.35)):
         return src.decode_phsp(rec, PHSP_Z)
 
     return Plan("ct_mlc_5field", wl, fields, pos, w, phsp)
+
+
+def ct_cube(n=128, voxel=0.25, seed=4):
+    """The dose-optimisation phantom: the synthetic CT at n^3 voxels of
+    `voxel` cm (128^3 x 2.5 mm), point source at SSD 100 with a 10 x 10 field."""
+    wl = ct(nx=n, ny=n, nz=n, voxel=voxel, seed=seed)
+    wl.name = "ct_%dcube_6MV" % n
+    return wl
+
+
+def target_and_oar(phantom, r_target=3.0, oar=(4.0, 5.0)):
+    """Voxel masks of the optimisation objective: a target sphere of radius
+    r_target cm at the isocentre (the phantom centre) and an organ-at-risk
+    shell oar[0] <= r < oar[1] cm around it."""
+    xc = 0.5 * (phantom.xb[:-1] + phantom.xb[1:]).astype(np.float64)
+    yc = 0.5 * (phantom.yb[:-1] + phantom.yb[1:]).astype(np.float64)
+    zc = 0.5 * (phantom.zb[:-1] + phantom.zb[1:]).astype(np.float64)
+    Z, Y, X = np.meshgrid(zc, yc, xc, indexing="ij")
+    r = np.sqrt(X ** 2 + Y ** 2 + Z ** 2).reshape(-1)
+    return r < r_target, (r >= oar[0]) & (r < oar[1])
+
+
+def objective(phantom, dose, boost=1.2):
+    """L = sum_v m_v (D_v - P_v)^2 with m = 1 on the target and the OAR shell,
+    P = boost x the mean target dose of `dose` on the target and 0 on the OAR.
+    Returns (m, P); dL/dD = 2 m (D - P)."""
+    target, oar = target_and_oar(phantom)
+    m = (target | oar).astype(np.float64)
+    p = np.where(target, boost * float(dose[target].mean()), 0.0)
+    return m, p
+
+
+class MultiPlan:
+    """Several plans on one phantom, each with its own fields and history
+    budget, all fed by one point-source record list: field j of plan k uses
+    records [first, first + count) of that list, in plan-then-field order."""
+
+    def __init__(self, name, workload, plans, budgets):
+        self.name = name
+        self.workload = workload
+        self.plans = plans              # list of lists of Field
+        self.budgets = np.asarray(budgets, dtype=np.int64)
+
+    @property
+    def n_histories(self):
+        return int(self.budgets.sum())
+
+    def source(self, seed):
+        return self.workload.source(self.n_histories, seed)
+
+    def jobs(self):
+        out, first = [], 0
+        for k, fields in enumerate(self.plans):
+            b = int(self.budgets[k])
+            for j, f in enumerate(fields):
+                c = b * (j + 1) // len(fields) - b * j // len(fields)
+                out.append(dict(field=f, plan=k, first=first, count=c))
+                first += c
+        return out
+
+
+def multi_plan(n_plans=16, total=100_000_000, n=(256, 256, 180), voxel=0.2, seed=5,
+               fields=(1, 9), budget=(1_000_000, 20_000_000)):
+    """The plan-queue workload: the synthetic CT at 256 x 256 x 180 voxels of
+    2 mm, n_plans plans of 1-9 coplanar fields at seeded gantry angles, history
+    budgets in [1e6, 2e7] summing to `total` (1e8); 6 MV-like point source,
+    SSD 100, 10 x 10 cm^2 fields."""
+    rng = np.random.default_rng(seed)
+    wl = ct(nx=n[0], ny=n[1], nz=n[2], voxel=voxel, seed=seed)
+    wl.name = "ct_%dx%dx%d_6MV" % n
+    sid = wl.field.source_to_isocenter
+    plans = []
+    for _ in range(n_plans):
+        k = int(rng.integers(fields[0], fields[1] + 1))
+        g0 = float(rng.uniform(0.0, 360.0))
+        plans.append([make_field(True, sid, (g0 + 360.0 * j / k) % 360.0) for j in range(k)])
+    lo, hi = budget
+    for _ in range(100000):
+        b = rng.uniform(lo, hi, n_plans)
+        b = np.floor(b / b.sum() * total).astype(np.int64)
+        b[np.argmax(b)] += total - int(b.sum())
+        if b.min() >= lo and b.max() <= hi:
+            break
+    else:
+        raise ValueError("no budgets in [%g, %g] sum to %d" % (lo, hi, total))
+    return MultiPlan("ct_%dx%dx%d_%dplans" % (n[0], n[1], n[2], n_plans), wl, plans, b)
