@@ -18,11 +18,13 @@ relative uncertainty over voxels above 50% of Dmax, from the batch
 variance), the HBM
 roofline of the event loop's dominant kernel (the photon flight kernel,
 timed live with HIP events), the adjoint gradient rate of a 5-field,
-2000-beamlet plan, and the CPU oracle timed on this host's cores on a
-bounded sample.
+2000-beamlet plan on the 128^3 CT, a queue of 16 plans (93 fields, 1e8
+histories) on a 256x256x180 CT batched vs field by field, and the CPU
+oracle timed on this host's cores on a bounded sample.
 
   python bench.py [--gpus 1] [--steps 10] [--warmup 1] [--histories-per-step 1000000]
-                  [--mlc-plan-histories 20000000] [--adjoint-histories 10000000] [--tuning JSON]
+                  [--mlc-plan-histories 20000000] [--adjoint-histories 10000000]
+                  [--plan-queue-histories 100000000] [--tuning JSON]
 """
 from __future__ import annotations
 
@@ -201,11 +203,15 @@ def mlc_plan_bench(n_total, n_batches, seed, autotune_first=True, fixed_tuning=N
     return out
 
 
-def adjoint_rate(wl, n_hist, seed, repeat=2):
-    """Gradients/s of the beamlet adjoint: 5 fields (gantry 0/72/144/216/288)
-    x 20x20 beamlets of 5 mm, one gradient = one replay of n_hist histories."""
+def adjoint_rate(n_hist, seed, repeat=2):
+    """Gradients/s of the beamlet adjoint on the dose-optimisation workload:
+    the 128^3 CT at 2.5 mm, 5 fields (gantry 0/72/144/216/288) x 20x20
+    beamlets of 5 mm, w ~ U(0.5, 1.5), L = sum m (D - P)^2 over a 3 cm target
+    sphere and a 4-5 cm OAR shell; one gradient = one replay of n_hist
+    histories."""
     from gegs_amd import beamlets, native, workloads
-    fields = [workloads.make_field(True, wl.field.source_to_isocenter, g) for g in (0, 72, 144, 216, 288)]
+    wl = workloads.ct_cube()
+    fields = [workloads.make_field(True, wl.field.source_to_isocenter, g) for g in workloads.C3_GANTRY]
     per = n_hist // len(fields)
     src = wl.source(per * len(fields), 777)
     ctx = native.Context(0)
@@ -214,7 +220,8 @@ def adjoint_rate(wl, n_hist, seed, repeat=2):
     plan = beamlets.BeamletPlan(ctx, fields, per, n_side=20, pitch=0.5, seed=seed, n_batches=2, src=src)
     w = np.random.default_rng(4).uniform(0.5, 1.5, plan.n_beamlets)
     d = plan.forward(w)
-    dLdD = 2.0 * (d - d.mean()) * (d > 0.5 * d.max())
+    m, p = workloads.objective(wl.phantom, d)
+    dLdD = 2.0 * m * (d - p)
     plan.adjoint(dLdD)                      # graph capture / warm caches
     best = None
     for _ in range(repeat):
@@ -225,7 +232,65 @@ def adjoint_rate(wl, n_hist, seed, repeat=2):
         best = dt if best is None else min(best, dt)
     ctx.close()
     return dict(gradients_per_s=1.0 / best, ms_per_gradient=best * 1e3, histories_per_gradient=per * len(fields),
-                beamlets=plan.n_beamlets, fields=len(fields), nonzero=int(np.count_nonzero(g)))
+                beamlets=plan.n_beamlets, fields=len(fields), nonzero=int(np.count_nonzero(g)),
+                workload="%s, target sphere r 3 cm + OAR shell 4-5 cm" % wl.name)
+
+
+def plan_queue_bench(total, n_batches, seed, tuning=None):
+    """16 plans of 1-9 fields (budgets in [1e6, 2e7] summing to `total`) on
+    the 256 x 256 x 180 CT at 2 mm: all 93 fields in one batched run (one
+    dose buffer per plan) against the plain kernel field after field.  Same
+    histories and streams on both sides; the per-plan uncertainty comes from
+    each method's batch variance."""
+    from gegs_amd import native, workloads
+    mp = workloads.multi_plan(total=total)
+    src = mp.source(5)
+    jobs = mp.jobs()
+    nplan = len(mp.plans)
+    ctx = native.Context(0)
+    ctx.load_workload(mp.workload)
+    ctx.set_source(src)
+    del src
+    if tuning:
+        ctx.set_tuning(tuning)
+    # warm-up: graph capture and first launches on small slices
+    small = [dict(jb, count=min(jb["count"], 2000)) for jb in jobs]
+    ctx.run_batched(small, nplan, seed=seed, n_batches=1)
+    ctx.set_field(jobs[0]["field"])
+    ctx.run(seed=seed, n_batches=1, method="plain", first=0, count=100_000)
+    _sync()
+    t0 = time.perf_counter()
+    b = ctx.run_batched(jobs, nplan, seed=seed, n_batches=n_batches)
+    _sync()
+    t_b = time.perf_counter() - t0
+    dose = np.zeros_like(b["dose"])
+    var = np.zeros_like(b["dose"])
+    _sync()
+    t0 = time.perf_counter()
+    for jb in jobs:
+        ctx.set_field(jb["field"])
+        r = ctx.run(seed=seed, n_batches=n_batches, method="plain", first=jb["first"], count=jb["count"])
+        _, _, v = plan_sigma(r["dose"], r["dose2"], n_batches)
+        dose[jb["plan"]] += r["dose"]
+        var[jb["plan"]] += v
+    _sync()
+    t_p = time.perf_counter() - t0
+    ctx.close()
+    ratio, sig_b, sig_p = [], [], []
+    for k in range(nplan):
+        sb, _, _ = plan_sigma(b["dose"][k], b["dose2"][k], n_batches)
+        sel = dose[k] > 0.5 * dose[k].max()
+        sp = float(np.mean(np.sqrt(var[k][sel]) / dose[k][sel]))
+        sig_b.append(sb)
+        sig_p.append(sp)
+        ratio.append((sp * sp * t_p) / (sb * sb * t_b))
+    rel = float(np.max(np.abs(dose - b["dose"])) / np.max(np.abs(dose)))
+    return dict(workload=mp.name, plans=nplan, fields=len(jobs), histories=mp.n_histories, n_batches=n_batches,
+                batched=dict(seconds=t_b, histories_per_s=mp.n_histories / t_b),
+                plain=dict(seconds=t_p, histories_per_s=mp.n_histories / t_p),
+                batched_vs_plain_time=t_p / t_b, batched_vs_plain_efficiency_mean_over_plans=float(np.mean(ratio)),
+                mean_rel_sigma_batched=float(np.mean(sig_b)), mean_rel_sigma_plain=float(np.mean(sig_p)),
+                max_rel_dose_difference=rel)
 
 
 def main():
@@ -244,6 +309,8 @@ def main():
                     help="histories per adjoint gradient (0: skip)")
     ap.add_argument("--mlc-plan-histories", type=int, default=20_000_000,
                     help="histories of the 5-field MLC plan comparison (0: skip)")
+    ap.add_argument("--plan-queue-histories", type=int, default=100_000_000,
+                    help="histories of the 16-plan queue comparison on the 256x256x180 CT (0: skip)")
     ap.add_argument("--seed", type=int, default=20240)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a device-to-device copy of this many bytes (counter calibration)")
@@ -313,11 +380,14 @@ def main():
     ctx.close()
     adj = None
     if args.adjoint_histories:
-        adj = adjoint_rate(wl, args.adjoint_histories, args.seed)
+        adj = adjoint_rate(args.adjoint_histories, args.seed)
     mlc_plan = None
     if args.mlc_plan_histories:
         mlc_plan = mlc_plan_bench(args.mlc_plan_histories, args.steps, args.seed, not args.no_autotune,
                                   json.loads(args.tuning) if args.tuning else None)
+    queue = None
+    if args.plan_queue_histories:
+        queue = plan_queue_bench(args.plan_queue_histories, args.steps, args.seed, tuning)
     if "roofline" in r:
         rf = r["roofline"]
         roof = {"bound": "hbm", "achieved": rf["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -368,6 +438,7 @@ def main():
         "roofline": roof,
         "adjoint": adj,
         "mlc_plan": mlc_plan,
+        "plan_queue": queue,
         "cpu_baseline": cpu,
     }
     if mlc_plan:
