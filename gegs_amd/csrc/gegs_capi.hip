@@ -762,6 +762,8 @@ int gegs_set_tuning(gegs_ctx *c, const gegs_tuning *t) {
     if (cfg.refill_min > 64) return fail(c, GEGS_ERR_ARG, "gegs_set_tuning: refill_min above 64");
     if (cfg.sweeps_per_graph > 64) return fail(c, GEGS_ERR_ARG, "gegs_set_tuning: sweeps_per_graph above 64");
     if (cfg.capacity > 0xFFFFFFF0ull) return fail(c, GEGS_ERR_ARG, "gegs_set_tuning: capacity above 2^32 - 16");
+    if (cfg.capacity && cfg.capacity < (size_t)gegs::QSEG)
+        return fail(c, GEGS_ERR_ARG, "gegs_set_tuning: capacity below one slot per append segment (32)");
     c->tuning = cfg;
     return GEGS_OK;
 }

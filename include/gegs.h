@@ -143,7 +143,8 @@ typedef struct gegs_tuning {
     uint32_t flight_block;       /* threads per block of the photon and electron kernels: 64 ... 512 (256) */
     uint32_t refill_min;         /* idle lanes of a wavefront that trigger a refill, 1 ... 64 (16) */
     uint32_t refill_slots;       /* bank slots a wavefront of the photon/electron kernels reserves per atomic: 64 ... 4096, multiple of 64 (64) */
-    uint64_t capacity;           /* particles per bank (4 Mi) */
+    uint64_t capacity;           /* particles per bank (4 Mi), rounded down to a multiple of 32: each bank is
+                                    32 equal append segments, and a full segment counts as overflow */
 } gegs_tuning;
 
 /* Event counters (diagnostics; replaces DO_STEP_COUNT / PRINT_PART_NUM). */

@@ -170,6 +170,10 @@ def test_tuning_roundtrip_and_validation(gpu_ctx):
     with pytest.raises(native.GegsError) as e:
         gpu_ctx.set_tuning(dict(block=96))
     assert e.value.code == 2001
+    # a bank is 32 append segments of at least one slot each
+    with pytest.raises(native.GegsError) as e:
+        gpu_ctx.set_tuning(dict(capacity=31))
+    assert e.value.code == 2001
     gpu_ctx.set_tuning(None)
     assert gpu_ctx.get_tuning()["photon_chunk"] == 32
 
