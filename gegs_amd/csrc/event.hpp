@@ -472,7 +472,7 @@ struct HoldSink {
 // crossings is suspended into the next sweep's bank.  The photons of ended
 // flights are appended together when the wavefront refills (or runs dry),
 // so that one atomic per destination serves many lanes.
-template <bool TAB, bool ADJ, bool HEAD>
+template <bool TAB, bool ADJ>
 __global__ void __launch_bounds__(512) photon_kernel(EvArgs a, uint32_t chunk, uint32_t refill_min,
                                                      uint32_t refill_slots) {
     extern __shared__ float lds_env[];
@@ -523,7 +523,7 @@ __global__ void __launch_bounds__(512) photon_kernel(EvArgs a, uint32_t chunk, u
                 suspend(hold, s, p, P_PHOTON_STEP);
                 have = false;
                 flush_particle(a, hist, t);
-            } else if (!photon_walk_iter<HEAD>(env, s, p, w, hold, t)) {
+            } else if (!photon_walk_iter(env, s, p, w, hold, t)) {
                 have = false;
                 flush_particle(a, hist, t);
             }
@@ -535,7 +535,7 @@ __global__ void __launch_bounds__(512) photon_kernel(EvArgs a, uint32_t chunk, u
 
 // electron / positron condensed-history steps with lane refill; an electron
 // still stepping after `chunk` steps is suspended into the next sweep's bank
-template <bool TAB, bool ADJ, bool HEAD>
+template <bool TAB, bool ADJ>
 __global__ void __launch_bounds__(512) electron_kernel(EvArgs a, uint32_t chunk, uint32_t refill_min,
                                                        uint32_t refill_slots) {
     extern __shared__ float lds_env[];
@@ -585,7 +585,7 @@ __global__ void __launch_bounds__(512) electron_kernel(EvArgs a, uint32_t chunk,
                 sink.push_suspended(p, es);
                 have = false;
                 flush_particle(a, hist, t);
-            } else if (!electron_iter<HEAD>(env, s, p, es, sink, t)) {
+            } else if (!electron_iter(env, s, p, es, sink, t)) {
                 have = false;
                 flush_particle(a, hist, t);
             }
@@ -814,7 +814,7 @@ inline size_t flight_lds(const Env &env) {
     return env_lds_bytes(env.nx, env.ny, env.nz, env.nclass, env.nmed, env.njob, tab_in_lds(env));
 }
 
-template <bool TAB, bool ADJ, bool HEAD>
+template <bool TAB, bool ADJ>
 inline void launch_sweep_t(const EvArgs &a, const EventConfig &cfg, hipStream_t stream, hipEvent_t t0,
                            hipEvent_t t1) {
     const dim3 g(cfg.grid), b(cfg.block), fb(cfg.flight_block);
@@ -822,14 +822,14 @@ inline void launch_sweep_t(const EvArgs &a, const EventConfig &cfg, hipStream_t 
     hipLaunchKernelGGL(inject_plan, dim3(1), dim3(1), 0, stream, a.st, cfg.threshold, cfg.inject);
     hipLaunchKernelGGL(inject_fill, g, b, 0, stream, a);
     if (t0) (void)hipEventRecord(t0, stream);
-    hipLaunchKernelGGL((photon_kernel<TAB, ADJ, HEAD>), dim3(cfg.photon_grid), fb, lds, stream, a, cfg.photon_chunk,
+    hipLaunchKernelGGL((photon_kernel<TAB, ADJ>), dim3(cfg.photon_grid), fb, lds, stream, a, cfg.photon_chunk,
                        cfg.refill_min, cfg.refill_slots);
     if (t1) (void)hipEventRecord(t1, stream);
     hipLaunchKernelGGL((interaction_kernel<P_RAYLEIGH, ADJ>), g, b, 0, stream, a);
     hipLaunchKernelGGL((interaction_kernel<P_COMPTON, ADJ>), g, b, 0, stream, a);
     hipLaunchKernelGGL((interaction_kernel<P_PHOTO, ADJ>), g, b, 0, stream, a);
     hipLaunchKernelGGL((interaction_kernel<P_PAIR, ADJ>), g, b, 0, stream, a);
-    hipLaunchKernelGGL((electron_kernel<TAB, ADJ, HEAD>), dim3(cfg.electron_grid), fb, lds, stream, a, cfg.electron_chunk,
+    hipLaunchKernelGGL((electron_kernel<TAB, ADJ>), dim3(cfg.electron_grid), fb, lds, stream, a, cfg.electron_chunk,
                        cfg.refill_min, cfg.refill_slots);
     hipLaunchKernelGGL((interaction_kernel<E_MOLLER, ADJ>), g, b, 0, stream, a);
     hipLaunchKernelGGL((interaction_kernel<E_BREMS, ADJ>), g, b, 0, stream, a);
@@ -838,44 +838,33 @@ inline void launch_sweep_t(const EvArgs &a, const EventConfig &cfg, hipStream_t 
     hipLaunchKernelGGL(sweep_end, dim3(1), dim3(1), 0, stream, a.st);
 }
 
-// Flight kernels without the head geometry serve runs whose particles all
-// stay in the phantom, when the record table is in LDS (the common case);
-// every other run takes the kernels with the jaws and the MLC.
-inline bool phantom_only(const Env &env) { return !env.head && tab_in_lds(env); }
-
 inline void launch_sweep(const EvArgs &a, const EventConfig &cfg, hipStream_t stream, hipEvent_t t0 = nullptr,
                          hipEvent_t t1 = nullptr) {
     const bool adj = a.adj != nullptr;
-    if (phantom_only(a.env)) {
-        if (adj) launch_sweep_t<true, true, false>(a, cfg, stream, t0, t1);
-        else launch_sweep_t<true, false, false>(a, cfg, stream, t0, t1);
-    } else if (tab_in_lds(a.env)) {
-        if (adj) launch_sweep_t<true, true, true>(a, cfg, stream, t0, t1);
-        else launch_sweep_t<true, false, true>(a, cfg, stream, t0, t1);
+    if (tab_in_lds(a.env)) {
+        if (adj) launch_sweep_t<true, true>(a, cfg, stream, t0, t1);
+        else launch_sweep_t<true, false>(a, cfg, stream, t0, t1);
     } else {
-        if (adj) launch_sweep_t<false, true, true>(a, cfg, stream, t0, t1);
-        else launch_sweep_t<false, false, true>(a, cfg, stream, t0, t1);
+        if (adj) launch_sweep_t<false, true>(a, cfg, stream, t0, t1);
+        else launch_sweep_t<false, false>(a, cfg, stream, t0, t1);
     }
 }
 
-template <bool TAB, bool ADJ, bool HEAD>
+template <bool TAB, bool ADJ>
 inline void flight_grids_t(EventConfig &cfg, size_t lds) {
-    if (!cfg.photon_grid) cfg.photon_grid = resident_grid(photon_kernel<TAB, ADJ, HEAD>, cfg.flight_block, lds);
-    if (!cfg.electron_grid) cfg.electron_grid = resident_grid(electron_kernel<TAB, ADJ, HEAD>, cfg.flight_block, lds);
+    if (!cfg.photon_grid) cfg.photon_grid = resident_grid(photon_kernel<TAB, ADJ>, cfg.flight_block, lds);
+    if (!cfg.electron_grid) cfg.electron_grid = resident_grid(electron_kernel<TAB, ADJ>, cfg.flight_block, lds);
 }
 
 // grid sizes of the persistent flight kernels that fill the device once
 inline void flight_grids(EventConfig &cfg, const Env &env, bool adj) {
     const size_t lds = flight_lds(env);
-    if (phantom_only(env)) {
-        if (adj) flight_grids_t<true, true, false>(cfg, lds);
-        else flight_grids_t<true, false, false>(cfg, lds);
-    } else if (tab_in_lds(env)) {
-        if (adj) flight_grids_t<true, true, true>(cfg, lds);
-        else flight_grids_t<true, false, true>(cfg, lds);
+    if (tab_in_lds(env)) {
+        if (adj) flight_grids_t<true, true>(cfg, lds);
+        else flight_grids_t<true, false>(cfg, lds);
     } else {
-        if (adj) flight_grids_t<false, true, true>(cfg, lds);
-        else flight_grids_t<false, false, true>(cfg, lds);
+        if (adj) flight_grids_t<false, true>(cfg, lds);
+        else flight_grids_t<false, false>(cfg, lds);
     }
 }
 
@@ -890,7 +879,6 @@ inline bool same_args(const EvArgs &x, const EvArgs &y) {
            x.env.ny == y.env.ny && x.env.nz == y.env.nz && x.env.pge == y.env.pge && x.env.pgmfp == y.env.pgmfp &&
            x.env.pcohe == y.env.pcohe && x.env.mlc == y.env.mlc && x.env.endnodes == y.env.endnodes &&
            x.env.mlc_next == y.env.mlc_next && x.env.jaws == y.env.jaws && x.env.jawx_next == y.env.jawx_next &&
-           x.env.head == y.env.head &&
            x.tally_batches == y.tally_batches;
 }
 

@@ -45,7 +45,6 @@ struct gegs_ctx {
     size_t endnodes_cap = 0;
     gegs_jaws h_jaws{};                  // jaw geometry
     bool have_jaws = false;
-    bool run_head = true;                // a job of the last uploaded set starts in the jaws or the MLC
     std::vector<float> jaw_pos;          // x1, x2, y1, y2 per set
     gegs_float2 *d_jaws = nullptr;       // jaw polygons per job
     size_t jaws_cap = 0;
@@ -409,7 +408,6 @@ static int upload_jobs(gegs_ctx *c, const std::vector<gegs_field> &fields, const
         any_mlc |= uses_mlc(f);
         any_jaws |= f.module_first == GEGS_M_SECJAWY;
     }
-    c->run_head = any_mlc || any_jaws;
     if (any_jaws) {
         if (!c->have_jaws) return fail(c, GEGS_ERR_STATE, "run: a field starts in the jaws but no jaws are loaded");
         const size_t nsets = c->jaw_pos.size() / 4;
@@ -471,7 +469,6 @@ static Env make_env(gegs_ctx *c, uint64_t seed, uint32_t njob) {
     env.jaws = c->d_jaws;
     env.jawx_next = c->have_mlc ? GEGS_M_MLC : GEGS_M_PHANTOM;
     env.seed = seed;
-    env.head = c->run_head ? 1u : 0u;
     return env;
 }
 

@@ -208,9 +208,6 @@ struct Env {
     int32_t jawx_next;               // module after the X jaws (the MLC when loaded)
     const gegs_float2 *jaws;         // jaw polygons per job: [job][32]
     uint64_t seed;
-    // some job of the run starts in the jaws or the MLC (host-side choice of
-    // the flight kernels; 0: every particle stays in the phantom module)
-    uint32_t head;
 };
 
 // LDS staging of what a voxel crossing reads: the voxel bounds and, when the
@@ -469,19 +466,15 @@ __device__ __forceinline__ float hownear_phantom(const Env &env, const Part &p) 
 namespace gegs {
 
 // module dispatch of the geometry queries (photon.cu:142-158, electron.cu:1491-1505, :1836-1850)
-// HEAD = false: a run whose fields all start in the phantom, where no
-// particle is ever in another module; the head geometry is left out.
-template <bool HEAD = true>
 __device__ __forceinline__ uint32_t howfar_module(const Env &env, Part &p, float &t) {
-    if (!HEAD || p.module == GEGS_M_PHANTOM) return howfar_phantom(env, p, t);
+    if (p.module == GEGS_M_PHANTOM) return howfar_phantom(env, p, t);
     if (p.module == GEGS_M_MLC) return howfar_mlc(env, p, t);
     if (p.module == GEGS_M_SECJAWY) return howfar_jaws<true>(env, p, t);
     return howfar_jaws<false>(env, p, t);
 }
 
-template <bool HEAD = true>
 __device__ __forceinline__ float hownear_module(const Env &env, Part &p) {
-    if (!HEAD || p.module == GEGS_M_PHANTOM) return hownear_phantom(env, p);
+    if (p.module == GEGS_M_PHANTOM) return hownear_phantom(env, p);
     if (p.module == GEGS_M_MLC) return hownear_mlc(env, p);
     if (p.module == GEGS_M_SECJAWY) return hownear_jaws<true>(env, p);
     return hownear_jaws<false>(env, p);
@@ -613,7 +606,7 @@ __device__ __forceinline__ void photon_walk_begin(const Env &env, const Part &p,
 }
 
 // one free-flight segment; true: the photon crossed a boundary and flies on
-template <bool HEAD = true, class Sink, class T>
+template <class Sink, class T>
 __device__ __forceinline__ bool photon_walk_iter(const Env &env, Stream &s, Part &p, PhotonWalk &w, Sink &sink, T &t) {
     const gegs_media_tables *mt = env.mt;
     t.photon_steps++;
@@ -698,7 +691,7 @@ __device__ __forceinline__ bool photon_walk_iter(const Env &env, Stream &s, Part
         }
         if (tstep < 0.0f) tstep = EPSTEP;
     } else {
-        nreg = howfar_module<HEAD>(env, p, tstep);
+        nreg = howfar_module(env, p, tstep);
     }
     int idisc = 0;
     if (nreg == 0) idisc = (rd.med == VACUUM || p.leaf == PASSMLC) ? 1 : -1;
@@ -1241,7 +1234,7 @@ struct EState {
     }
 };
 
-template <bool HEAD = true, class Sink, class T>
+template <class Sink, class T>
 __device__ __forceinline__ bool electron_iter(const Env &env, Stream &s, Part &p, EState &st, Sink &sink, T &t) {
     // one condensed-history step; true: the electron continues stepping
     const gegs_media_tables *mt = env.mt;
@@ -1338,7 +1331,7 @@ __device__ __forceinline__ bool electron_iter(const Env &env, Stream &s, Part &p
         range = (range + mt->range_ep[(medium * MXEKE + lelke - 1) * 2 + qel]) / rhof;
         tmxs_val = fminf(tmxs_val, mt->smaxir[medium]);
         tustep = fminf(tstep, fminf(tmxs_val, range));
-        tperp = hownear_module<HEAD>(env, p);
+        tperp = hownear_module(env, p);
         p.dnear = tperp;
         if (mt->i_do_rr[medium] == 1 && p.e < mt->e_max_rr[medium] && tperp >= range) {
             t.score(p.region, to_kinetic(p.e) * p.wt);
@@ -1374,7 +1367,7 @@ __device__ __forceinline__ bool electron_iter(const Env &env, Stream &s, Part &p
     uint32_t irnew = p.region;
     int idisc = 0;
     const float ustep0 = ustep;
-    if (callhowfar || p.wt <= 0.0f) irnew = howfar_module<HEAD>(env, p, ustep);
+    if (callhowfar || p.wt <= 0.0f) irnew = howfar_module(env, p, ustep);
     if (irnew == 0) idisc = (rd.med == VACUUM || p.leaf == PASSMLC) ? 1 : -1;
     if (idisc > 0) {
         p.region = 0;

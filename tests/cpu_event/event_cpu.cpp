@@ -122,7 +122,6 @@ int main(int argc, char **argv) {
     env.jaws = jaws;
     env.jawx_next = mlc ? GEGS_M_MLC : GEGS_M_PHANTOM;
     env.seed = m.seed;
-    env.head = (m.have_mlc || m.have_jaws) ? 1u : 0u;
     const size_t lds = flight_lds(env);
     if (lds != sizeof(lds_env)) {
         fprintf(stderr, "LDS %zu bytes, array %zu\n", lds, sizeof(lds_env));
