@@ -157,7 +157,7 @@ def mlc_plan_bench(n_total, n_batches, seed, autotune_first=True, fixed_tuning=N
     elif autotune_first:
         ctx.set_leaf_positions(plan.leaf_positions[0])
         ctx.set_field(plan.fields[0])
-        tuning, _, _ = autotune.autotune(ctx, 0, min(int(plan.counts(n_total)[0]), 500_000), seed=seed)
+        tuning, _, _ = autotune.autotune(ctx, 0, min(int(plan.counts(n_total)[0]), 1_000_000), seed=seed, repeat=3)
     jobs = plan.jobs(n_total)
     out = {}
     # warm-up: graph capture and first launches on a small slice
@@ -345,7 +345,7 @@ def main():
         ctx.set_tuning(tuning)
     elif "event" in methods and not args.no_autotune:
         # tuned on the warmup records, before the timed region
-        tuning, _, _ = autotune.autotune(ctx, 0, min(b, 500_000), seed=args.seed)
+        tuning, _, _ = autotune.autotune(ctx, 0, min(max(args.warmup, 1) * b, 2_000_000), seed=args.seed, repeat=3)
     results = {}
     for method in methods:
         elapsed, dose, dose2, cnt, _ = run_method(ctx, method, args.steps, args.warmup, b, args.seed)

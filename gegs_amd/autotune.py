@@ -26,7 +26,7 @@ SPACE = {
     "sweeps_per_graph": [2, 4, 8],
 }
 
-START = dict(photon_chunk=64, electron_chunk=16, refill_min=32, refill_slots=64, flight_block=256,
+START = dict(photon_chunk=64, electron_chunk=16, refill_min=16, refill_slots=64, flight_block=256,
              inject=1 << 20, threshold=1 << 20, sweeps_per_graph=4)
 
 
