@@ -89,6 +89,49 @@ def cpu_baseline(wl, budget_s, threads):
                        "(C restatement, gcc -O2, OpenMP over histories), %.1f s" % (n, dt))
 
 
+def cpu_detail(wl, threads, seconds=4.0):
+    """The oracle beside the headline baseline: one core on the same CT
+    workload, and all cores on the 60^3 water phantom (1.25 MeV parallel
+    beam), each on a bounded sample of about `seconds`."""
+    from oracle import oracle
+    out = {"nproc": os.cpu_count()}
+    for name, w, th in (("ct_1core", wl, 1), ("water60_%dcores" % threads, None, threads)):
+        if w is None:
+            from gegs_amd import workloads
+            w = workloads.water()
+        probe = w.source(20000 * th, 993)
+        t0 = time.perf_counter()
+        oracle.run(w, probe, seed=7, threads=th)
+        rate0 = len(probe) / (time.perf_counter() - t0)
+        n = int(max(10000, min(20_000_000, rate0 * seconds)))
+        sample = w.source(n, 994)
+        t0 = time.perf_counter()
+        oracle.run(w, sample, seed=8, threads=th)
+        dt = time.perf_counter() - t0
+        out[name] = dict(histories_per_s=n / dt, histories=n, seconds=dt, cores=th)
+    return out
+
+
+def atomic_peak(dose, deposits_per_s, reps=6):
+    """FP64 atomic adds per second that the device sustains on the dose
+    array when the target voxels follow the measured dose distribution (the
+    observed conflict pattern), with the tally's own hardware atomic
+    (gegs_atomic_probe), beside the event loop's deposits per second."""
+    from gegs_amd import native
+    p = np.clip(np.asarray(dose, dtype=np.float64), 0.0, None)
+    p = p / p.sum()
+    idx = np.random.default_rng(5).choice(len(p), size=1 << 24, p=p).astype(np.uint32)
+    ctx = native.Context(0)
+    try:
+        rate = ctx.atomic_probe(idx, len(p), reps=reps)
+    finally:
+        ctx.close()
+    return dict(peak_fp64_atomics_per_s=rate, event_loop_deposits_per_s=deposits_per_s,
+                frac=deposits_per_s / rate,
+                method="gegs_atomic_probe: %d FP64 atomic adds over the %d voxels, drawn from the dose "
+                       "distribution of the run" % (reps * len(idx), len(p)))
+
+
 def _sync():
     try:
         import torch
@@ -353,7 +396,8 @@ def main():
         sig, nsel, _ = plan_sigma(dose, dose2, args.steps)
         eff = 1.0 / (sig * sig * elapsed)
         res = dict(value=hist / elapsed, ms_per_step=elapsed / args.steps * 1e3, sigma=sig, voxels=nsel,
-                   efficiency=eff, counters=cnt, bytes=algorithmic_bytes(cnt, hist))
+                   efficiency=eff, counters=cnt, bytes=algorithmic_bytes(cnt, hist), dose=dose,
+                   deposits_per_s=cnt["deposits"] / elapsed)
         if method == "event":
             # the same steps again with every photon kernel launch timed by
             # HIP events on the library's stream
@@ -374,10 +418,13 @@ def main():
     head = "event" if "event" in results else methods[0]
     r = results[head]
     cpu = None
+    cpu_more = None
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         cpu = cpu_baseline(wl, args.cpu_seconds, threads)
+        cpu_more = cpu_detail(wl, threads)
     ctx.close()
+    atomics = atomic_peak(r["dose"], r["deposits_per_s"]) if head == "event" else None
     adj = None
     if args.adjoint_histories:
         adj = adjoint_rate(args.adjoint_histories, args.seed)
@@ -440,6 +487,8 @@ def main():
         "mlc_plan": mlc_plan,
         "plan_queue": queue,
         "cpu_baseline": cpu,
+        "cpu_baseline_detail": cpu_more,
+        "fp64_atomics": atomics,
     }
     if mlc_plan:
         # the batching target is stated on the 5-field MLC plan

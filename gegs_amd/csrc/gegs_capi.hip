@@ -845,6 +845,40 @@ int gegs_kernel_timing(gegs_ctx *c, double *photon_ms, uint64_t *photon_launches
     return GEGS_OK;
 }
 
+int gegs_atomic_probe(gegs_ctx *c, const uint32_t *idx, uint64_t n, uint32_t nvox, uint32_t reps, float *ms) {
+    if (!c || !idx || !ms || n == 0 || nvox == 0 || reps == 0) return fail(c, GEGS_ERR_ARG, "gegs_atomic_probe: bad argument");
+    for (uint64_t i = 0; i < n; ++i)
+        if (idx[i] >= nvox) return fail(c, GEGS_ERR_ARG, "gegs_atomic_probe: index beyond nvox");
+    hipSetDevice(c->device);
+    double *acc = nullptr;
+    uint32_t *d_idx = nullptr;
+    HIPCHECK(c, hipMalloc(&acc, (size_t)nvox * sizeof(double)), "hipMalloc probe");
+    hipError_t e = hipMalloc(&d_idx, n * sizeof(uint32_t));
+    if (e != hipSuccess) {
+        hipFree(acc);
+        return fail(c, GEGS_ERR_DEVICE, "hipMalloc probe", e);
+    }
+    e = hipMemcpy(d_idx, idx, n * sizeof(uint32_t), hipMemcpyHostToDevice);
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&t0);
+    if (e == hipSuccess) e = hipEventCreate(&t1);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(fill_zero, dim3(1024), dim3(256), 0, c->stream, acc, (size_t)nvox);
+        hipLaunchKernelGGL(atomic_probe, dim3(2048), dim3(256), 0, c->stream, acc, d_idx, n, 1u);
+        (void)hipEventRecord(t0, c->stream);
+        hipLaunchKernelGGL(atomic_probe, dim3(2048), dim3(256), 0, c->stream, acc, d_idx, n, reps);
+        (void)hipEventRecord(t1, c->stream);
+        e = hipEventSynchronize(t1);
+        if (e == hipSuccess) e = hipEventElapsedTime(ms, t0, t1);
+    }
+    if (t0) hipEventDestroy(t0);
+    if (t1) hipEventDestroy(t1);
+    hipFree(acc);
+    hipFree(d_idx);
+    if (e != hipSuccess) return fail(c, GEGS_ERR_DEVICE, "atomic probe", e);
+    return GEGS_OK;
+}
+
 int gegs_get_counters(gegs_ctx *c, gegs_counters *out) {
     if (!c || !out) return GEGS_ERR_ARG;
     *out = c->counters;

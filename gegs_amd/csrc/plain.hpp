@@ -253,4 +253,13 @@ __global__ void fill_zero(double *p, size_t n) {
         p[v] = 0.0;
 }
 
+// FP64 atomic adds at the given voxel indices, `reps` passes over them: the
+// same hardware atomic the dose tally uses, for a measured peak rate
+__global__ void atomic_probe(double *acc, const uint32_t *idx, uint64_t n, uint32_t reps) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint32_t r = 0; r < reps; ++r)
+        for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+            unsafeAtomicAdd(acc + idx[i], 1.0);
+}
+
 }  // namespace gegs

@@ -20,7 +20,7 @@ SYMBOLS = [
     "gegs_create", "gegs_destroy", "gegs_last_error", "gegs_sizeof_media_tables",
     "gegs_load_media", "gegs_load_phantom", "gegs_set_field", "gegs_set_source",
     "gegs_run_plain", "gegs_run_event", "gegs_set_history_tally", "gegs_history_tally",
-    "gegs_get_counters", "gegs_last_kernel_ms", "gegs_sample_events",
+    "gegs_get_counters", "gegs_last_kernel_ms", "gegs_sample_events", "gegs_atomic_probe",
     "gegs_set_tuning", "gegs_get_tuning", "gegs_run_batched", "gegs_set_beamlets", "gegs_adjoint",
     "gegs_set_kernel_timing", "gegs_kernel_timing", "gegs_load_mlc", "gegs_set_leaf_positions",
     "gegs_load_jaws", "gegs_set_jaw_positions",
@@ -115,6 +115,7 @@ def load():
     L.gegs_get_counters.argtypes = [vp, vp]
     L.gegs_last_kernel_ms.argtypes = [vp, vp]
     L.gegs_sample_events.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint64, vp, ctypes.c_uint32, vp, vp]
+    L.gegs_atomic_probe.argtypes = [vp, vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, vp]
     L.gegs_set_tuning.argtypes = [vp, vp]
     L.gegs_run_batched.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp]
     L.gegs_load_mlc.argtypes = [vp, vp]
@@ -209,6 +210,14 @@ class Context:
         t = Tuning()
         self._check(self.L.gegs_get_tuning(self.h, ctypes.byref(t)))
         return t.as_dict()
+
+    def atomic_probe(self, idx, nvox, reps=4):
+        """FP64 atomic adds per second at the voxel indices idx (uint32 <
+        nvox), `reps` timed passes (gegs_atomic_probe)."""
+        idx = np.ascontiguousarray(idx, dtype=np.uint32)
+        ms = ctypes.c_float()
+        self._check(self.L.gegs_atomic_probe(self.h, _p(idx), len(idx), int(nvox), int(reps), ctypes.byref(ms)))
+        return reps * len(idx) / (ms.value * 1e-3)
 
     def sample_events(self, particles, seed, max_out=16):
         """Run one event handler per particle (PARTICLE_DTYPE records);

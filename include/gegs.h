@@ -276,6 +276,14 @@ int gegs_sample_events(gegs_ctx *ctx, const gegs_particle *in, uint32_t n, uint6
 int gegs_set_kernel_timing(gegs_ctx *ctx, int enable);
 int gegs_kernel_timing(gegs_ctx *ctx, double *photon_ms, uint64_t *photon_launches);
 
+/* Measured FP64 atomic rate: `reps` passes of one hardware FP64 atomic add
+ * per entry of idx (n voxel indices < nvox) into a zeroed nvox array, the
+ * atomic the dose tally uses; *ms is the time of the timed passes (after one
+ * untimed pass).  No counterpart in the reference; reports the tally's
+ * deposits per second against the device's rate at the same conflict pattern
+ * (escore's FP64 atomicAdd, scoring.cu:14-24, utility.cu:52-61). */
+int gegs_atomic_probe(gegs_ctx *ctx, const uint32_t *idx, uint64_t n, uint32_t nvox, uint32_t reps, float *ms);
+
 /* Counters and timing of the last run. */
 int gegs_get_counters(gegs_ctx *ctx, gegs_counters *out);
 int gegs_last_kernel_ms(gegs_ctx *ctx, float *ms);
