@@ -236,3 +236,13 @@ def test_kernel_timing_leaves_results_unchanged(gpu_ctx, ct_small):
     assert launches > 0 and launches % sweeps_per_graph == 0
     # the photon kernel is part of the run bracketed by the run's own events
     assert 0.0 < ms <= 1.02 * t["ms"] + 0.5
+
+
+def test_atomic_probe_rate_and_bounds(gpu_ctx):
+    idx = np.random.default_rng(3).integers(0, 1000, 1 << 16).astype(np.uint32)
+    assert gpu_ctx.atomic_probe(idx, 1000, reps=2) > 0
+    bad = idx.copy()
+    bad[7] = 1000
+    with pytest.raises(native.GegsError) as e:
+        gpu_ctx.atomic_probe(bad, 1000)
+    assert e.value.code == 2001
