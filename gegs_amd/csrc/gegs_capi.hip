@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/gegs.h"
+#include "ctimport.hpp"
 #include "event.hpp"
 #include "plain.hpp"
 
@@ -876,6 +877,43 @@ int gegs_atomic_probe(gegs_ctx *c, const uint32_t *idx, uint64_t n, uint32_t nvo
     hipFree(acc);
     hipFree(d_idx);
     if (e != hipSuccess) return fail(c, GEGS_ERR_DEVICE, "atomic probe", e);
+    return GEGS_OK;
+}
+
+int gegs_ct_subsize(int32_t size[3], int16_t *hu, const float voxel[3], float offset[3], const float box_min[3],
+                    const float box_max[3]) {
+    if (!size || !hu || !voxel || !offset || !box_min || !box_max) return GEGS_ERR_ARG;
+    for (int a = 0; a < 3; ++a)
+        if (size[a] < 1 || size[a] > (int32_t)GEGS_MAX_CT_AXIS || !(voxel[a] > 0.0f)) return GEGS_ERR_ARG;
+    int n[3] = {size[0], size[1], size[2]};
+    gegs::ct::subsize(n, hu, voxel, offset, box_min, box_max);
+    for (int a = 0; a < 3; ++a) size[a] = n[a];
+    return GEGS_OK;
+}
+
+int gegs_ct_resample_grid(const int32_t size[3], const float voxel[3], float thickness[3], int32_t out_size[3]) {
+    if (!size || !voxel || !thickness || !out_size) return GEGS_ERR_ARG;
+    for (int a = 0; a < 3; ++a)
+        if (size[a] < 1 || size[a] > (int32_t)GEGS_MAX_CT_AXIS || !(voxel[a] > 0.0f)) return GEGS_ERR_ARG;
+    const int n[3] = {size[0], size[1], size[2]};
+    int out[3];
+    const int rc = gegs::ct::resample_grid(n, voxel, thickness, out, (int)GEGS_MAX_AXIS);
+    if (rc) return rc;
+    for (int a = 0; a < 3; ++a) out_size[a] = out[a];
+    return GEGS_OK;
+}
+
+int gegs_ct_resample(const int32_t size[3], const float voxel[3], const int16_t *hu, const float offset[3],
+                     const float thickness[3], const int32_t out_size[3], float *xb, float *yb, float *zb,
+                     int16_t *out) {
+    if (!size || !voxel || !hu || !offset || !thickness || !out_size || !xb || !yb || !zb || !out) return GEGS_ERR_ARG;
+    for (int a = 0; a < 3; ++a) {
+        if (size[a] < 1 || size[a] > (int32_t)GEGS_MAX_CT_AXIS || !(voxel[a] > 0.0f)) return GEGS_ERR_ARG;
+        if (out_size[a] < 1 || out_size[a] > (int32_t)GEGS_MAX_AXIS || !(thickness[a] > 0.0f)) return GEGS_ERR_ARG;
+    }
+    const int n[3] = {size[0], size[1], size[2]};
+    const int o[3] = {out_size[0], out_size[1], out_size[2]};
+    gegs::ct::resample(n, voxel, hu, offset, thickness, o, xb, yb, zb, out);
     return GEGS_OK;
 }
 

@@ -21,6 +21,7 @@ SYMBOLS = [
     "gegs_load_media", "gegs_load_phantom", "gegs_set_field", "gegs_set_source",
     "gegs_run_plain", "gegs_run_event", "gegs_set_history_tally", "gegs_history_tally",
     "gegs_get_counters", "gegs_last_kernel_ms", "gegs_sample_events", "gegs_atomic_probe",
+    "gegs_ct_subsize", "gegs_ct_resample_grid", "gegs_ct_resample",
     "gegs_set_tuning", "gegs_get_tuning", "gegs_run_batched", "gegs_set_beamlets", "gegs_adjoint",
     "gegs_set_kernel_timing", "gegs_kernel_timing", "gegs_load_mlc", "gegs_set_leaf_positions",
     "gegs_load_jaws", "gegs_set_jaw_positions",
@@ -116,6 +117,9 @@ def load():
     L.gegs_last_kernel_ms.argtypes = [vp, vp]
     L.gegs_sample_events.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint64, vp, ctypes.c_uint32, vp, vp]
     L.gegs_atomic_probe.argtypes = [vp, vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, vp]
+    L.gegs_ct_subsize.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.gegs_ct_resample_grid.argtypes = [vp, vp, vp, vp]
+    L.gegs_ct_resample.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.gegs_set_tuning.argtypes = [vp, vp]
     L.gegs_run_batched.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp]
     L.gegs_load_mlc.argtypes = [vp, vp]

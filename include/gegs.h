@@ -70,6 +70,7 @@ typedef struct gegs_mlc {
 /* voxels per phantom axis accepted by gegs_load_phantom (the reference caps at 128,
    EGSnrc_Parameters.h:140-142) */
 #define GEGS_MAX_AXIS 4096
+#define GEGS_MAX_CT_AXIS 4096       /* CT input voxels per axis (the reference: 512 x 512 x 270, EGSnrc_Parameters.h:137-139) */
 
 /* One source particle in the head frame, as decoded from an egsphsp MODE0
  * record (CUDA_EGS.cu:40-158): total energy, position with z at the source
@@ -283,6 +284,30 @@ int gegs_kernel_timing(gegs_ctx *ctx, double *photon_ms, uint64_t *photon_launch
  * deposits per second against the device's rate at the same conflict pattern
  * (escore's FP64 atomicAdd, scoring.cu:14-24, utility.cu:52-61). */
 int gegs_atomic_probe(gegs_ctx *ctx, const uint32_t *idx, uint64_t n, uint32_t nvox, uint32_t reps, float *ms);
+
+/* CT front end (host only, no device or context needed).  Volumes of CT
+ * numbers are x-fastest int16: hu[i + nx * (j + ny * k)]; lengths in cm.
+ *
+ * gegs_ct_subsize replaces SubsizeCT (ctcreate.c:40-274): crops the volume in
+ * place to the CT voxels that cover [box_min, box_max] (all six zero: the
+ * whole volume; a box outside or empty along an axis keeps that axis whole),
+ * updating size and offset (the lower corner).
+ *
+ * gegs_ct_resample_grid and gegs_ct_resample replace ResampleCT
+ * (ctcreate.c:277-823): the output grid has int(extent / thickness) voxels per
+ * axis starting at offset, thickness adjusted to fill the CT extent (returned
+ * in place), or the reference's input errors 5002 (thickness <= 0), 5003
+ * (thickness above the CT extent), 5004 (more than GEGS_MAX_AXIS voxels; the
+ * reference's limit is 128, EGSnrc_Parameters.h:140-142).  The resampled
+ * value of a voxel is the volume-weighted mean CT number, accumulated in
+ * binary32 in the reference's order and truncated to an integer; xb, yb, zb
+ * receive out_size + 1 bounds each. */
+int gegs_ct_subsize(int32_t size[3], int16_t *hu, const float voxel[3], float offset[3],
+                    const float box_min[3], const float box_max[3]);
+int gegs_ct_resample_grid(const int32_t size[3], const float voxel[3], float thickness[3], int32_t out_size[3]);
+int gegs_ct_resample(const int32_t size[3], const float voxel[3], const int16_t *hu, const float offset[3],
+                     const float thickness[3], const int32_t out_size[3], float *xb, float *yb, float *zb,
+                     int16_t *out);
 
 /* Counters and timing of the last run. */
 int gegs_get_counters(gegs_ctx *ctx, gegs_counters *out);
