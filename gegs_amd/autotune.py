@@ -22,12 +22,12 @@ SPACE = {
     "refill_min": [8, 16, 32, 48],
     "refill_slots": [64, 256, 1024],
     "flight_block": [256, 512],
-    "inject": [1 << 18, 1 << 19, 1 << 20],
+    "inject": [1 << 20, 1 << 21, 1 << 22, 6 << 20],
     "sweeps_per_graph": [2, 4, 8],
 }
 
 START = dict(photon_chunk=64, electron_chunk=16, refill_min=16, refill_slots=64, flight_block=256,
-             inject=1 << 20, threshold=1 << 20, sweeps_per_graph=4)
+             inject=1 << 22, threshold=1 << 22, sweeps_per_graph=4)
 
 
 def _rate(ctx, tuning, first, count, seed, repeat):

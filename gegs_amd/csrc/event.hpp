@@ -73,9 +73,9 @@ struct EventConfig {
     uint32_t electron_chunk = 16;    // condensed-history steps per electron and sweep (0: no limit)
     uint32_t refill_min = 16;        // idle lanes of a wavefront that trigger a refill
     uint32_t refill_slots = 64;      // bank slots a wavefront reserves per refill atomic
-    size_t capacity = 0;             // slots per bank (0 = 4 Mi)
-    uint32_t inject = 1u << 20;      // primaries injected per refill
-    uint32_t threshold = 1u << 20;   // refill when fewer particles are in flight
+    size_t capacity = 0;             // slots per bank (0: twice what can be in flight, at least 4 Mi)
+    uint32_t inject = 1u << 22;      // primaries injected per refill
+    uint32_t threshold = 1u << 22;   // refill when fewer particles are in flight
     uint32_t sweeps_per_graph = 4;
 };
 
@@ -718,9 +718,21 @@ struct EventBanks {
     double photon_ms = 0.0;          // accumulated over the runs since the last reset
     uint64_t photon_launches = 0;
 
-    hipError_t reserve(uint64_t, uint32_t, const EventConfig &cfg) {
+    // Banks for a run of n_hist histories.  Without an explicit capacity a
+    // bank holds twice what can be in flight at an injection (what is left
+    // below the threshold plus the injection, neither more than the run's
+    // histories), and at least 4 Mi particles.
+    static size_t auto_capacity(uint64_t n_hist, const EventConfig &cfg) {
+        const uint64_t inj = cfg.inject < n_hist ? cfg.inject : n_hist;
+        const uint64_t thr = cfg.threshold < n_hist ? cfg.threshold : n_hist;
+        const uint64_t want = 2 * (inj + thr);
+        const uint64_t floor = (uint64_t)1 << 22;
+        const uint64_t cap = want > floor ? want : floor;
+        return (size_t)(cap < 0xFFFFF000ull ? cap : 0xFFFFF000ull);
+    }
+    hipError_t reserve(uint64_t n_hist, uint32_t, const EventConfig &cfg) {
         // a whole number of append segments per bank
-        const size_t want = (cfg.capacity ? cfg.capacity : ((size_t)1 << 22)) / QSEG * QSEG;
+        const size_t want = (cfg.capacity ? cfg.capacity : auto_capacity(n_hist, cfg)) / QSEG * QSEG;
         if (want > 0xFFFFFFF0ull || want < (size_t)QSEG) return hipErrorInvalidValue;
         hipError_t e = hipSuccess;
         if (!st) {

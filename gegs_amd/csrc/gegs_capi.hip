@@ -592,7 +592,7 @@ static int run_event_jobs(gegs_ctx *c, const gegs_run_options *o, const std::vec
     if (rc) return rc;
     const EventConfig cfg = c->tuning;
     Env env = make_env(c, o->seed, (uint32_t)jobs.size());
-    hipError_t e = c->banks.reserve(c->nsrc, nb, cfg);
+    hipError_t e = c->banks.reserve(total_all, nb, cfg);
     if (e != hipSuccess) return fail(c, GEGS_ERR_DEVICE, "event banks", e);
     hipLaunchKernelGGL(clear_overflow, dim3(1), dim3(1), 0, c->stream, c->banks.st);
     hipEvent_t t0, t1;
@@ -784,7 +784,7 @@ int gegs_get_tuning(gegs_ctx *c, gegs_tuning *t) {
     t->flight_block = cfg.flight_block;
     t->refill_min = cfg.refill_min;
     t->refill_slots = c->tuning.refill_slots;
-    t->capacity = cfg.capacity ? cfg.capacity : ((uint64_t)1 << 22);
+    t->capacity = cfg.capacity;
     return GEGS_OK;
 }
 

@@ -138,14 +138,16 @@ typedef struct gegs_tuning {
     uint32_t electron_grid;      /* blocks of the electron kernel (same) */
     uint32_t photon_chunk;       /* voxel crossings per photon and sweep (32; 0xFFFFFFFF: no limit) */
     uint32_t electron_chunk;     /* condensed-history steps per electron and sweep (16; 0xFFFFFFFF: no limit) */
-    uint32_t inject;             /* primaries injected per refill (1 Mi) */
-    uint32_t threshold;          /* refill when fewer particles are in flight (1 Mi) */
+    uint32_t inject;             /* primaries injected per refill (4 Mi) */
+    uint32_t threshold;          /* refill when fewer particles are in flight (4 Mi) */
     uint32_t sweeps_per_graph;   /* sweeps captured in one graph replay (4) */
     uint32_t flight_block;       /* threads per block of the photon and electron kernels: 64 ... 512 (256) */
     uint32_t refill_min;         /* idle lanes of a wavefront that trigger a refill, 1 ... 64 (16) */
     uint32_t refill_slots;       /* bank slots a wavefront of the photon/electron kernels reserves per atomic: 64 ... 4096, multiple of 64 (64) */
-    uint64_t capacity;           /* particles per bank (4 Mi), rounded down to a multiple of 32: each bank is
-                                    32 equal append segments, and a full segment counts as overflow */
+    uint64_t capacity;           /* particles per bank, rounded down to a multiple of 32: each bank is 32 equal
+                                    append segments, and a full segment counts as overflow (0: twice the
+                                    injection plus the threshold, each at most the run's histories, and at
+                                    least 4 Mi) */
 } gegs_tuning;
 
 /* Event counters (diagnostics; replaces DO_STEP_COUNT / PRINT_PART_NUM). */
