@@ -6,9 +6,12 @@ on the workload: how long photon flights are (voxel count along the beam,
 density), how many electrons each history sets in motion, how many
 histories a batch holds.  ``autotune`` measures histories/s of short runs on
 a sample of the workload's own source records and walks the tuning space one
-coordinate at a time (photon chunk, electron chunk, refill threshold, flight
-block size, injection size), keeping a change only when it is faster by more
-than the run-to-run noise.  Every trial transports real histories with the
+coordinate at a time (photon chunk, electron chunk, refill threshold and
+reservation, flight block size, sweeps per graph), keeping a change only when
+it is faster by more than the run-to-run noise.  The injection size is not
+searched: its effect shows only in runs longer than one injection, which a
+short trial is not; it stays at 6 Mi primaries (the fastest of 1-8 Mi on
+1e7-history runs of the CT workload).  Every trial transports real histories with the
 production kernels; the tuning never changes results, only speed (the event
 path reproduces the plain kernel history by history for every tuning).
 """
@@ -22,12 +25,11 @@ SPACE = {
     "refill_min": [8, 16, 32, 48],
     "refill_slots": [64, 256, 1024],
     "flight_block": [256, 512],
-    "inject": [1 << 20, 1 << 21, 1 << 22, 6 << 20],
     "sweeps_per_graph": [2, 4, 8],
 }
 
 START = dict(photon_chunk=64, electron_chunk=16, refill_min=16, refill_slots=64, flight_block=256,
-             inject=1 << 22, threshold=1 << 22, sweeps_per_graph=4)
+             inject=6 << 20, threshold=6 << 20, sweeps_per_graph=4)
 
 
 def _rate(ctx, tuning, first, count, seed, repeat):

@@ -74,8 +74,8 @@ struct EventConfig {
     uint32_t refill_min = 16;        // idle lanes of a wavefront that trigger a refill
     uint32_t refill_slots = 64;      // bank slots a wavefront reserves per refill atomic
     size_t capacity = 0;             // slots per bank (0: twice what can be in flight, at least 4 Mi)
-    uint32_t inject = 1u << 22;      // primaries injected per refill
-    uint32_t threshold = 1u << 22;   // refill when fewer particles are in flight
+    uint32_t inject = 6u << 20;      // primaries injected per refill
+    uint32_t threshold = 6u << 20;   // refill when fewer particles are in flight
     uint32_t sweeps_per_graph = 4;
 };
 

@@ -138,8 +138,8 @@ typedef struct gegs_tuning {
     uint32_t electron_grid;      /* blocks of the electron kernel (same) */
     uint32_t photon_chunk;       /* voxel crossings per photon and sweep (32; 0xFFFFFFFF: no limit) */
     uint32_t electron_chunk;     /* condensed-history steps per electron and sweep (16; 0xFFFFFFFF: no limit) */
-    uint32_t inject;             /* primaries injected per refill (4 Mi) */
-    uint32_t threshold;          /* refill when fewer particles are in flight (4 Mi) */
+    uint32_t inject;             /* primaries injected per refill (6 Mi) */
+    uint32_t threshold;          /* refill when fewer particles are in flight (6 Mi) */
     uint32_t sweeps_per_graph;   /* sweeps captured in one graph replay (4) */
     uint32_t flight_block;       /* threads per block of the photon and electron kernels: 64 ... 512 (256) */
     uint32_t refill_min;         /* idle lanes of a wavefront that trigger a refill, 1 ... 64 (16) */
