@@ -1,0 +1,149 @@
+"""Dose post-processing of a plan: per-field output factors and MU weighting,
+interpolation of the dose onto a CT grid, absolute dose at a point.
+
+Restates the reference's host-side steps in binary32 where the reference
+computes in float:
+
+* ``field_output_factor`` -- init.cu:1444-1455: the equivalent square
+  sqrt((x2 - x1)(y2 - y1)) of the jaw opening, linearly interpolated in the
+  (field size, output factor) table between the first pair of sizes that
+  brackets it (lower size exclusive, upper inclusive); None when no pair does
+  (the reference then leaves the field's factor unset).
+* ``plan_dose`` -- CUDA_EGS.cu:1715-1735: the plan's energy deposit is the sum
+  over fields of each field's deposit times its output factor (when output
+  factors are on); each field runs MU x histories-per-MU histories
+  (CUDA_EGS.cu:377).
+* ``interp_dose`` -- output.c:221-315: separable linear interpolation along z,
+  then y, then x, onto points start + size * i; the dose of voxel k is placed
+  at its lower boundary, points at or outside the first or the last-but-one
+  boundary get 0.  (The reference keeps the three passes in one array with
+  offsets that keep each pass's inputs intact for grids at least as fine as
+  the phantom; separate arrays give the same values.)
+* ``abs_dose`` -- output.c:780-872: trilinear interpolation at a point with
+  the same boundary placement, the voxel searched from index 1 and clamped to
+  [0, n - 2], and the dose of the voxel nearest the point, whose z midpoint
+  the reference takes as (z_bounds[k] + x_bounds[k + 1]) / 2 (quirk Q9, kept).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+f32 = np.float32
+
+
+def read_output_factor_table(path):
+    """Output-factor file (init.cu:1389-1412): a header line, the number of
+    entries, then one "field_size output_factor" pair per line."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    n = int(lines[1].split()[0])
+    sizes, factors = [], []
+    for line in lines[2:2 + n]:
+        a, b = line.split()[:2]
+        sizes.append(float(a))
+        factors.append(float(b))
+    return np.array(sizes, dtype=np.float32), np.array(factors, dtype=np.float32)
+
+
+def field_output_factor(x1, x2, y1, y2, sizes, factors):
+    """Output factor of a field with jaw opening [x1, x2] x [y1, y2] (cm)."""
+    a = f32(np.sqrt(f32(f32(f32(x2) - f32(x1)) * f32(f32(y2) - f32(y1)))))
+    for j in range(len(sizes) - 1):
+        if a > sizes[j] and a <= sizes[j + 1]:
+            return f32(factors[j] + f32(f32(f32(a - sizes[j]) * f32(factors[j + 1] - factors[j]))
+                                        / f32(sizes[j + 1] - sizes[j])))
+    return None
+
+
+def plan_dose(field_deposits, output_factors=None):
+    """Sum of the fields' energy deposits (FP64), each times its output factor
+    when output factors are given."""
+    total = np.zeros_like(np.asarray(field_deposits[0], dtype=np.float64))
+    for i, d in enumerate(field_deposits):
+        d = np.asarray(d, dtype=np.float64)
+        total += d * float(output_factors[i]) if output_factors is not None else d
+    return total
+
+
+def histories_for_field(mu, histories_per_mu):
+    """Histories of a field: MU x histories per MU, truncated (CUDA_EGS.cu:377)."""
+    return int(mu * histories_per_mu)
+
+
+def _axis_interp(vals, bounds, coords):
+    """Linear interpolation of vals[..., k] placed at bounds[k] onto coords
+    along the last axis (one pass of interp_dose)."""
+    n = vals.shape[-1]
+    out = np.zeros(vals.shape[:-1] + (len(coords),), dtype=np.float32)
+    for c, t in enumerate(coords):
+        t = f32(t)
+        if t <= bounds[0] or t >= bounds[n - 1]:
+            continue
+        for k in range(n - 1):
+            if t > bounds[k] and t <= bounds[k + 1]:
+                lo, hi = vals[..., k], vals[..., k + 1]
+                out[..., c] = lo + f32(t - bounds[k]) * (hi - lo) / f32(bounds[k + 1] - bounds[k])
+                break
+    return out
+
+
+def interp_dose(dose, xb, yb, zb, start, sizes, dimen):
+    """dose[k, j, i] (float32, nz x ny x nx) on the phantom with bounds xb, yb,
+    zb, interpolated onto the points start + sizes * index (dimen points per
+    axis); returns a float32 array [dimen_z, dimen_y, dimen_x]."""
+    d = np.asarray(dose, dtype=np.float32)
+    xb, yb, zb = (np.asarray(b, dtype=np.float32) for b in (xb, yb, zb))
+    pts = [np.array([f32(f32(start[a]) + f32(f32(sizes[a]) * f32(i))) for i in range(dimen[a])], dtype=np.float32)
+           for a in range(3)]
+    # z pass: values along z for every (y, x) column
+    vz = _axis_interp(np.moveaxis(d, 0, -1), zb, pts[2])          # [ny, nx, dz]
+    vy = _axis_interp(np.moveaxis(vz, 0, -1), yb, pts[1])         # [nx, dz, dy]
+    vx = _axis_interp(np.moveaxis(vy, 0, -1), xb, pts[0])         # [dz, dy, dx]
+    return vx
+
+
+def write_interp_dose(path_prefix, grid):
+    """The interpolated dose file of output.c:222-312: int32 dimensions x, y,
+    z then the float32 values, x fastest; name <prefix>_Interpolate_XXXxYYYxZZZ.bin."""
+    nz, ny, nx = grid.shape
+    fname = "%s_Interpolate_%03dx%03dx%03d.bin" % (path_prefix, nx, ny, nz)
+    with open(fname, "wb") as f:
+        np.array([nx, ny, nz], dtype=np.int32).tofile(f)
+        np.ascontiguousarray(grid, dtype=np.float32).tofile(f)
+    return fname
+
+
+def _cell(b, n, p):
+    i = 1
+    while i < n - 2:
+        if p > b[i] and p <= b[i + 1]:
+            break
+        i += 1
+    if i == n - 2:
+        i = 0 if p <= b[1] else n - 2
+    return i
+
+
+def abs_dose(dose, xb, yb, zb, pos):
+    """(dose at pos, (i, j, k) of the voxel taken as nearest, its dose)."""
+    d = np.asarray(dose, dtype=np.float32)
+    nz, ny, nx = d.shape
+    xb, yb, zb = (np.asarray(b, dtype=np.float32) for b in (xb, yb, zb))
+    px, py, pz = (f32(v) for v in pos)
+    i, j, k = _cell(xb, nx, px), _cell(yb, ny, py), _cell(zb, nz, pz)
+
+    def zlin(ii, jj):
+        a, b = d[k, jj, ii], d[k + 1, jj, ii]
+        return f32(a + f32(f32(pz - zb[k]) * f32(b - a)) / f32(zb[k + 1] - zb[k]))
+
+    a0, a1, a2, a3 = zlin(i, j), zlin(i + 1, j), zlin(i, j + 1), zlin(i + 1, j + 1)
+    a0 = f32(a0 + f32(f32(py - yb[j]) * f32(a2 - a0)) / f32(yb[j + 1] - yb[j]))
+    a1 = f32(a1 + f32(f32(py - yb[j]) * f32(a3 - a1)) / f32(yb[j + 1] - yb[j]))
+    a0 = f32(a0 + f32(f32(px - xb[i]) * f32(a1 - a0)) / f32(xb[i + 1] - xb[i]))
+    if px > f32(f32(xb[i] + xb[i + 1]) / f32(2)):
+        i += 1
+    if py > f32(f32(yb[j] + yb[j + 1]) / f32(2)):
+        j += 1
+    if pz > f32(f32(zb[k] + xb[k + 1]) / f32(2)):      # the reference's z midpoint uses x_bounds (Q9)
+        k += 1
+    return float(a0), (i, j, k), float(d[k, j, i])
