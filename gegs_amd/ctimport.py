@@ -232,12 +232,54 @@ def resample(vol, thickness):
     return res, xb, yb, zb
 
 
-def ctcreate(list_path, thickness, box_min=(0, 0, 0), box_max=(0, 0, 0), ramp=ph.DEFAULT_RAMP):
+def read_ct_ramps(path, max_media=6):
+    """CT ramps file (ctcreate.c:886-1040): "n,lowest CT number" on the first
+    line, then per medium a name line (24 characters) and "CT upper bound,
+    density at the lower end, density at the upper end, unused".  n = 0
+    selects the built-in 4-media ramp.  Returns (ramp, lowest CT number);
+    error 5005 for n outside [0, max_media] (the reference's MXMED is 6,
+    EGSnrc_Parameters.h:5)."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    head = lines[0].replace(" ", "").split(",")
+    n, lower = int(head[0]), int(head[1])
+    if n < 0 or n > max_media:
+        raise ValueError("ERROR (5005): number of materials out of range")
+    if n == 0:
+        return list(ph.DEFAULT_RAMP), ph.DEFAULT_RAMP_LOWER
+    ramp = []
+    for m in range(n):
+        name = lines[1 + 2 * m][:24].strip()
+        vals = [v for v in lines[2 + 2 * m].replace(" ", "").split(",") if v]
+        ramp.append((name, int(vals[0]), float(np.float32(vals[1])), float(np.float32(vals[2]))))
+    return ramp, lower
+
+
+def ctcreate(list_path, thickness, box_min=(0, 0, 0), box_max=(0, 0, 0), ramp=ph.DEFAULT_RAMP,
+             lower=ph.DEFAULT_RAMP_LOWER):
     """DICOM series -> Phantom: read, crop, resample, CT ramp."""
     vol = subsize(read_dicom_series(list_path), box_min, box_max)
     hu, xb, yb, zb = resample(vol, thickness)
-    med, dens = ph.ct_to_medium(hu.reshape(-1), ramp=ramp)
+    med, dens = ph.ct_to_medium(hu.reshape(-1), ramp=ramp, lower=lower)
     return ph.Phantom([r[0] for r in ramp], xb, yb, zb, med, dens)
+
+
+def phantom_from_run_input(ri, base_dir=None):
+    """The phantom a run input asks for with "use ctcreate = Yes" ([CTcreate]:
+    CT ramps file, dicom file list, xyz boundaries x1 x2 y1 y2 z1 z2, 3d voxel
+    sizes), files relative to base_dir (default: the input's folder)."""
+    if not ri.use_ctcreate:
+        raise ValueError("the run input does not use ctcreate")
+    base = base_dir or os.path.dirname(os.path.abspath(ri.path))
+
+    def local(p):
+        p = p.replace("\\", "/")
+        return p if os.path.isabs(p) else os.path.join(base, p)
+
+    ramp, lower = read_ct_ramps(local(ri.ct_ramps_file))
+    b = ri.xyz_boundaries
+    return ctcreate(local(ri.dicom_file_list), ri.ct_voxel_sizes, (b[0], b[2], b[4]), (b[1], b[3], b[5]),
+                    ramp=ramp, lower=lower)
 
 
 # ------------------------------------------------------------------ writer

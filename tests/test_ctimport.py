@@ -238,3 +238,59 @@ def test_ctcreate_end_to_end(tmp_path):
     m = med.reshape(ph.nz, ph.ny, ph.nx)
     xc = 0.5 * (ph.xb[:-1] + ph.xb[1:])
     assert np.all(m[:, :, xc < 0] == 1) and np.all(m[:, :, xc > 0] == 3)
+
+
+def test_ct_ramps_file(tmp_path):
+    p = str(tmp_path / "ramps.inp")
+    with open(p, "w") as f:
+        f.write("2,-1000\n")
+        f.write("AIR700ICRU".ljust(24) + "\n")
+        f.write("-900,0.001,0.1,0\n")
+        f.write("H2O700ICRU".ljust(24) + "\n")
+        f.write("3000,0.9,2.5,0\n")
+    ramp, lower = ctimport.read_ct_ramps(p)
+    assert lower == -1000
+    assert ramp == [("AIR700ICRU", -900, float(np.float32(0.001)), float(np.float32(0.1))),
+                    ("H2O700ICRU", 3000, float(np.float32(0.9)), 2.5)]
+    med, dens = phantom.ct_to_medium(np.array([-1000, -950, -900, 0, 3000, 3001]), ramp=ramp, lower=lower)
+    assert list(med) == [1, 1, 1, 2, 2, 2]
+    with open(p, "w") as f:
+        f.write("0,0\n")
+    assert ctimport.read_ct_ramps(p)[0] == list(phantom.DEFAULT_RAMP)
+    with open(p, "w") as f:
+        f.write("9,0\n")
+    with pytest.raises(ValueError, match="5005"):
+        ctimport.read_ct_ramps(p)
+
+
+@needs_lib
+def test_phantom_from_run_input(tmp_path):
+    from gegs_amd import runinput
+    d = str(tmp_path)
+    os.makedirs(os.path.join(d, "dicom"))
+    names = []
+    for k in range(4):
+        pix = np.full((8, 8), 1024, dtype=np.int16)
+        nm = "dicom/c%d.dcm" % k
+        ctimport.write_ct_slice(os.path.join(d, nm), pix, (-6.0, -6.0, 3.0 * k), (1.5, 1.5))
+        names.append(os.path.basename(nm))
+    with open(os.path.join(d, "dicom", "list.dlt"), "w") as f:
+        f.write("\n".join(names))
+    with open(os.path.join(d, "ramps.inp"), "w") as f:
+        f.write("0,0\n")
+    text = open(os.path.join(os.path.dirname(__file__), "golden", "ini", "TestCalibration.ini")).read()
+    text = text.replace("use ctcreate = No", "use ctcreate = Yes")
+    text = text.replace('CT ramps file = "CT_ramps.inp"', 'CT ramps file = "ramps.inp"')
+    text = text.replace('dicom file list = "..\\data\\Dicom\\PELICAN\\LungAbs.dlt"', 'dicom file list = "dicom\\list.dlt"')
+    text = text.replace("xyz boundaries = -17.65 12.85 -26.95 -6.45 -19.65 -7.65", "xyz boundaries = -0.5 0.5 -0.4 0.4 0.0 0.8")
+    text = text.replace("3d voxel sizes = 0.5 0.5 0.5\n\n[Phantom]", "3d voxel sizes = 0.3 0.3 0.3\n\n[Phantom]")
+    ini = os.path.join(d, "run.ini")
+    with open(ini, "w") as f:
+        f.write(text)
+    ri = runinput.read_run_input(ini)
+    ph = ctimport.phantom_from_run_input(ri)
+    vol = ctimport.subsize(ctimport.read_dicom_series(os.path.join(d, "dicom", "list.dlt")),
+                           (-0.5, -0.4, 0.0), (0.5, 0.4, 0.8))
+    hu, xb, _, _ = ctimport.resample(vol, (0.3, 0.3, 0.3))
+    assert (ph.nx, ph.ny, ph.nz) == (hu.shape[2], hu.shape[1], hu.shape[0])
+    assert np.array_equal(ph.xb, xb) and np.all(ph.media_idx == 3)
