@@ -147,3 +147,33 @@ def abs_dose(dose, xb, yb, zb, pos):
     if pz > f32(f32(zb[k] + xb[k + 1]) / f32(2)):      # the reference's z midpoint uses x_bounds (Q9)
         k += 1
     return float(a0), (i, j, k), float(d[k, j, i])
+
+
+def gamma_pass_rate(ref, evl, spacing, dose_crit=0.01, dta=0.1, cutoff=0.1):
+    """Global gamma index of the evaluated dose against the reference dose on
+    the same voxel grid (arrays [z, y, x], spacing (dx, dy, dz) in cm): for each
+    reference voxel with dose above cutoff x max(ref), gamma = min over
+    evaluated voxels within 3 x dta of sqrt((dD / (dose_crit x max(ref)))^2 +
+    (r / dta)^2); returns the fraction of those voxels with gamma <= 1."""
+    ref = np.asarray(ref, dtype=np.float64)
+    evl = np.asarray(evl, dtype=np.float64)
+    dmax = ref.max()
+    norm = dose_crit * dmax
+    dx, dy, dz = spacing
+    reach = [int(np.ceil(3.0 * dta / h)) for h in (dz, dy, dx)]
+    best = np.full(ref.shape, np.inf)
+    pad = [(r, r) for r in reach]
+    ev = np.pad(evl, pad, mode="constant", constant_values=np.nan)
+    nz, ny, nx = ref.shape
+    for kz in range(-reach[0], reach[0] + 1):
+        for ky in range(-reach[1], reach[1] + 1):
+            for kx in range(-reach[2], reach[2] + 1):
+                r2 = (kz * dz) ** 2 + (ky * dy) ** 2 + (kx * dx) ** 2
+                if r2 > (3.0 * dta) ** 2:
+                    continue
+                sh = ev[reach[0] + kz:reach[0] + kz + nz, reach[1] + ky:reach[1] + ky + ny,
+                        reach[2] + kx:reach[2] + kx + nx]
+                g2 = ((sh - ref) / norm) ** 2 + r2 / dta ** 2
+                best = np.fmin(best, g2)
+    sel = ref > cutoff * dmax
+    return float(np.mean(best[sel] <= 1.0))

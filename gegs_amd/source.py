@@ -171,3 +171,23 @@ def head_phase_space(n, z_plane, ssd, field, seed, electron_fraction=0.03, edges
     s["w"] = w.astype(np.float32)
     s["wt"] = np.float32(1.0)
     return s
+
+
+def stratified_parallel_beam(n_side, energy, half_width, z0, seed):
+    """Parallel square field of monoenergetic photons along +z with one source
+    point in each cell of an n_side x n_side grid over the field (uniform
+    within its cell): the source positions add no sampling noise beyond the
+    cell size, so reusing the records for many histories (different history
+    offsets) leaves only the transport's own noise."""
+    rng = np.random.default_rng(seed)
+    n = n_side * n_side
+    cell = 2.0 * half_width / n_side
+    idx = np.arange(n, dtype=np.int64)
+    s = np.zeros(n, dtype=SOURCE_DTYPE)
+    s["e"] = np.float32(energy)
+    s["x"] = (-half_width + cell * ((idx % n_side) + rng.random(n))).astype(np.float32)
+    s["y"] = (-half_width + cell * ((idx // n_side) + rng.random(n))).astype(np.float32)
+    s["z"] = np.float32(z0)
+    s["w"] = np.float32(1.0)
+    s["wt"] = np.float32(1.0)
+    return s

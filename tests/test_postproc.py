@@ -84,3 +84,23 @@ def test_abs_dose_trilinear_and_nearest_voxel():
     _, (i, j, k), _ = postproc.abs_dose(dose, xb, yb, zb, (0.1, 0.1, 0.6))
     # z = 0.6 in voxel 1 ([0.5, 1.0]); quirk midpoint (0.5 + xb[2] = -1.0) / 2 = -0.25 < 0.6: voxel 2
     assert k == 2
+
+
+def test_gamma_pass_rate_closed_forms():
+    rng = np.random.default_rng(0)
+    ref = rng.uniform(50.0, 100.0, (6, 7, 8))
+    spacing = (0.5, 0.5, 0.5)
+    assert postproc.gamma_pass_rate(ref, ref, spacing) == 1.0
+    # a uniform 0.5% of Dmax shift passes, a 2% shift fails everywhere (neighbours are 5 DTA away)
+    dmax = ref.max()
+    assert postproc.gamma_pass_rate(ref, ref + 0.005 * dmax, spacing) == 1.0
+    assert postproc.gamma_pass_rate(ref, ref + 0.02 * dmax, spacing) == 0.0
+    # on a fine grid the distance criterion lets a shifted gradient pass: a ramp of 1% of Dmax per
+    # 0.5 mm, shifted by one 0.5 mm voxel, is 1% off in dose but 0.5 mm away
+    z = np.arange(40) * 0.05
+    ramp = (1.0 + 0.2 * z)[:, None, None] * np.ones((40, 3, 3))
+    shifted = np.concatenate([ramp[1:], ramp[-1:]], axis=0)
+    fine = (0.05, 0.05, 0.05)
+    # every voxel but the first, whose matching neighbour lies outside the grid
+    assert postproc.gamma_pass_rate(ramp, shifted, fine, dose_crit=0.005) == 39 / 40
+    assert postproc.gamma_pass_rate(ramp, shifted, fine, dose_crit=0.005, dta=0.01) < 0.5
