@@ -2225,3 +2225,16 @@ float oracle_hownear_phantom(uint32_t nx, uint32_t ny, uint32_t nz, const float 
     p.region = region;
     return hownear_phantom(&c, &p);
 }
+
+/* the head -> phantom frame of one point and direction (to_phantom):
+ * out = phantom x, y, z, u, v, w */
+void oracle_to_phantom(const gegs_field *field, const float *pos_dir, float *out) {
+    octx c;
+    memset(&c, 0, sizeof(c));
+    c.f = *field;
+    opart p;
+    memset(&p, 0, sizeof(p));
+    p.x = pos_dir[0]; p.y = pos_dir[1]; p.z = pos_dir[2];
+    p.u = pos_dir[3]; p.v = pos_dir[4]; p.w = pos_dir[5];
+    to_phantom(&c, &p, out);
+}

@@ -70,6 +70,7 @@ def lib():
         L.oracle_sizeof_media_tables.restype = ctypes.c_size_t
         L.oracle_howfar_phantom.restype = ctypes.c_uint32
         L.oracle_howfar_phantom.argtypes = [ctypes.c_uint32] * 3 + [vp, vp, vp, vp, vp, ctypes.c_uint32, vp]
+        L.oracle_to_phantom.argtypes = [vp, vp, vp]
         L.oracle_hownear_phantom.restype = ctypes.c_float
         L.oracle_hownear_phantom.argtypes = [ctypes.c_uint32] * 3 + [vp, vp, vp, vp, vp, ctypes.c_uint32]
         _lib = L
@@ -184,3 +185,13 @@ def interaction(workload, particle, seed, max_out=64):
                              _p(xb), _p(yb), _p(zb), ctypes.c_void_p(ctypes.addressof(workload.field)),
                              _p(inp), seed, _p(out), max_out, _p(ed))
     return out[:k].copy(), float(ed[0])
+
+
+def to_phantom(field, pos_dir):
+    """Head -> phantom frame of points and directions (N x 6 float32)."""
+    L = lib()
+    pd = np.ascontiguousarray(pos_dir, dtype=np.float32).reshape(-1, 6)
+    out = np.zeros_like(pd)
+    for i in range(len(pd)):
+        L.oracle_to_phantom(ctypes.byref(field), _p(pd[i]), _p(out[i]))
+    return out

@@ -195,3 +195,39 @@ def test_hownear_keeps_reference_offset():
         expect = min(pos[0] - ph.xb[ix + 1], ph.xb[ix + 2] - pos[0])
         assert t <= 0.0
         assert t <= expect + 1e-6
+
+
+@pytest.mark.parametrize("change_xyz", [0, 1])
+def test_phantom_frame_within_2ulp_of_fp64(change_xyz):
+    """The head -> phantom transform (geom.cu:59-109) runs in binary32 as in
+    the reference, so that voxel indexing matches it bit for bit; against the
+    same transform in float64 (with the same binary32 cos/sin of the gantry
+    angle) every coordinate is within 2 ULP of the terms' scale."""
+    from gegs_amd import workloads
+    rng = np.random.default_rng(11)
+    for gantry in (0.0, 33.0, 90.0, 217.5):
+        f = workloads.make_field(change_xyz, 100.0, gantry, iso=(1.25, -0.5, 2.0))
+        pd = np.zeros((500, 6), dtype=np.float32)
+        pd[:, :3] = rng.uniform(-30, 130, (500, 3))
+        d = rng.normal(size=(500, 3))
+        pd[:, 3:] = d / np.linalg.norm(d, axis=1)[:, None]
+        got = oracle.to_phantom(f, pd).astype(np.float64)
+        c, s = np.float64(np.float32(f.cos_angle)), np.float64(np.float32(f.sin_angle))
+        p = pd.astype(np.float64)
+        if change_xyz:
+            x, y, z = p[:, 1], p[:, 2] - np.float64(np.float32(f.source_to_isocenter)), p[:, 0]
+            u, v, w = p[:, 4], p[:, 5], p[:, 3]
+        else:
+            x, y, z = p[:, 0], p[:, 1], p[:, 2] - np.float64(np.float32(f.source_to_isocenter))
+            u, v, w = p[:, 3], p[:, 4], p[:, 5]
+        iso = [np.float64(np.float32(f.iso_x)), np.float64(np.float32(f.iso_y)), np.float64(np.float32(f.iso_z))]
+        exact = np.stack([x * c - y * s + iso[0], x * s + y * c + iso[1], z + iso[2],
+                          u * c - v * s, u * s + v * c, w], axis=1)
+        scale = np.stack([np.abs(x * c) + np.abs(y * s) + abs(iso[0]), np.abs(x * s) + np.abs(y * c) + abs(iso[1]),
+                          np.abs(z) + abs(iso[2]), np.ones_like(u), np.ones_like(u), np.ones_like(u)], axis=1)
+        ulp = np.spacing(scale.astype(np.float32)).astype(np.float64)
+        assert np.all(np.abs(got - exact) <= 2 * ulp), (gantry, np.max(np.abs(got - exact) / ulp))
+    # a quarter turn maps the frame's x onto y
+    f = workloads.make_field(0, 100.0, 90.0)
+    q = oracle.to_phantom(f, np.array([[3.0, 0.0, 100.0, 1.0, 0.0, 0.0]], dtype=np.float32))[0]
+    assert abs(q[0]) < 1e-5 and abs(q[1] - 3.0) < 1e-5 and abs(q[4] - 1.0) < 1e-6
