@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgegs_hip.so")
 SOURCES = ["gegs_capi.hip"]
-HEADERS = ["transport.hpp", "gmath.hpp", "plain.hpp", "event.hpp", "mlc.hpp", "jaws.hpp"]
+HEADERS = ["transport.hpp", "gmath.hpp", "plain.hpp", "event.hpp", "mlc.hpp", "jaws.hpp", "blockwedge.hpp", "ctimport.hpp"]
 
 HIPCC_FLAGS = [
     "-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-shared",

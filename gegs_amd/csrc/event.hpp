@@ -23,6 +23,12 @@
 // Particles keep their Philox streams across banks, so every history follows
 // exactly the trajectory the plain kernel and the oracle give it.
 #pragma once
+
+// Occupancy floor of a flight kernel: at most 512 / n VGPRs, so that n waves
+// fit on a SIMD (the electron kernel sits at the 3-wave limit of 168 VGPRs).
+#ifndef GEGS_WAVES_PER_EU
+#define GEGS_WAVES_PER_EU(n) __attribute__((amdgpu_waves_per_eu(n)))
+#endif
 #include <vector>
 
 #include "transport.hpp"
@@ -536,7 +542,7 @@ __global__ void __launch_bounds__(512) photon_kernel(EvArgs a, uint32_t chunk, u
 // electron / positron condensed-history steps with lane refill; an electron
 // still stepping after `chunk` steps is suspended into the next sweep's bank
 template <bool TAB, bool ADJ>
-__global__ void __launch_bounds__(512) electron_kernel(EvArgs a, uint32_t chunk, uint32_t refill_min,
+__global__ void __launch_bounds__(512) GEGS_WAVES_PER_EU(3) electron_kernel(EvArgs a, uint32_t chunk, uint32_t refill_min,
                                                        uint32_t refill_slots) {
     extern __shared__ float lds_env[];
     const uint32_t flip = a.st->flip;
@@ -891,6 +897,7 @@ inline bool same_args(const EvArgs &x, const EvArgs &y) {
            x.env.ny == y.env.ny && x.env.nz == y.env.nz && x.env.pge == y.env.pge && x.env.pgmfp == y.env.pgmfp &&
            x.env.pcohe == y.env.pcohe && x.env.mlc == y.env.mlc && x.env.endnodes == y.env.endnodes &&
            x.env.mlc_next == y.env.mlc_next && x.env.jaws == y.env.jaws && x.env.jawx_next == y.env.jawx_next &&
+           x.env.bw == y.env.bw && x.env.block_next == y.env.block_next && x.env.wedge_nver == y.env.wedge_nver &&
            x.tally_batches == y.tally_batches;
 }
 

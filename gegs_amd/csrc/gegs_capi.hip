@@ -48,6 +48,8 @@ struct gegs_ctx {
     bool have_jaws = false;
     std::vector<float> jaw_pos;          // x1, x2, y1, y2 per set
     gegs_float2 *d_jaws = nullptr;       // jaw polygons per job
+    gegs_block_wedge h_bw{};             // block and wedge geometry
+    gegs_float2 *d_bw = nullptr;         // their 18 vertices on the device
     size_t jaws_cap = 0;
     uint32_t *d_beamlet = nullptr;       // beamlet of each history slot
     float *d_bweight = nullptr;          // weight of each beamlet
@@ -149,6 +151,7 @@ void gegs_destroy(gegs_ctx *c) {
     hipFree(c->d_mlc);
     hipFree(c->d_endnodes);
     hipFree(c->d_jaws);
+    hipFree(c->d_bw);
     hipFree(c->d_beamlet);
     hipFree(c->d_bweight);
     hipFree(c->d_adj);
@@ -241,10 +244,15 @@ int gegs_load_phantom(gegs_ctx *c, uint32_t nx, uint32_t ny, uint32_t nz, const 
     return GEGS_OK;
 }
 
+// modules a field may start in: the jaws, the MLC, the block, the wedge or the phantom
+static bool valid_first(int32_t m) {
+    return m == GEGS_M_PHANTOM || m == GEGS_M_MLC || m == GEGS_M_SECJAWY || m == GEGS_M_BLOCK || m == GEGS_M_WEDGE;
+}
+
 int gegs_set_field(gegs_ctx *c, const gegs_field *f) {
     if (!c || !f) return fail(c, GEGS_ERR_ARG, "gegs_set_field: null argument");
-    if (f->module_first != GEGS_M_PHANTOM && f->module_first != GEGS_M_MLC && f->module_first != GEGS_M_SECJAWY)
-        return fail(c, GEGS_ERR_ARG, "gegs_set_field: fields start in the jaws, the MLC or the phantom module");
+    if (!valid_first(f->module_first))
+        return fail(c, GEGS_ERR_ARG, "gegs_set_field: fields start in the jaws, the MLC, the block, the wedge or the phantom");
     c->field = *f;
     c->have_field = true;
     return GEGS_OK;
@@ -270,6 +278,24 @@ int gegs_load_jaws(gegs_ctx *c, const gegs_jaws *j) {
         return fail(c, GEGS_ERR_ARG, "gegs_load_jaws: inconsistent jaw geometry");
     c->h_jaws = *j;
     c->have_jaws = true;
+    return GEGS_OK;
+}
+
+int gegs_load_block_wedge(gegs_ctx *c, const gegs_block_wedge *bw) {
+    if (!c || !bw) return fail(c, GEGS_ERR_ARG, "gegs_load_block_wedge: null argument");
+    if (bw->has_wedge && (bw->wedge_nver < 3 || bw->wedge_nver > 6))
+        return fail(c, GEGS_ERR_ARG, "gegs_load_block_wedge: the wedge cross-section needs 3 to 6 vertices");
+    gegs_float2 v[18];
+    for (int i = 0; i < 4; ++i) {
+        v[i] = bw->block_xz[i];
+        v[4 + i] = bw->block_yz[i];
+        v[14 + i] = bw->wedge_bound[i];
+    }
+    for (int i = 0; i < 6; ++i) v[8 + i] = bw->wedge[i];
+    hipSetDevice(c->device);
+    if (!c->d_bw) HIPCHECK(c, hipMalloc(&c->d_bw, sizeof(v)), "hipMalloc block and wedge");
+    HIPCHECK(c, hipMemcpy(c->d_bw, v, sizeof(v), hipMemcpyHostToDevice), "copy block and wedge");
+    c->h_bw = *bw;
     return GEGS_OK;
 }
 
@@ -408,6 +434,10 @@ static int upload_jobs(gegs_ctx *c, const std::vector<gegs_field> &fields, const
     for (const gegs_field &f : fields) {
         any_mlc |= uses_mlc(f);
         any_jaws |= f.module_first == GEGS_M_SECJAWY;
+        if (f.module_first == GEGS_M_BLOCK && !c->h_bw.has_block)
+            return fail(c, GEGS_ERR_STATE, "run: a field starts in the block but no block is loaded");
+        if (f.module_first == GEGS_M_WEDGE && !c->h_bw.has_wedge)
+            return fail(c, GEGS_ERR_STATE, "run: a field starts in the wedge but no wedge is loaded");
     }
     if (any_jaws) {
         if (!c->have_jaws) return fail(c, GEGS_ERR_STATE, "run: a field starts in the jaws but no jaws are loaded");
@@ -466,9 +496,13 @@ static Env make_env(gegs_ctx *c, uint64_t seed, uint32_t njob) {
     env.njob = njob;
     env.mlc = c->have_mlc ? c->d_mlc : nullptr;
     env.endnodes = c->d_endnodes;
-    env.mlc_next = GEGS_M_PHANTOM;
+    // module order jaws -> MLC -> block -> wedge -> phantom over the loaded modules (CUDA_EGS.cu:1279-1298)
+    env.block_next = c->h_bw.has_wedge ? GEGS_M_WEDGE : GEGS_M_PHANTOM;
+    env.mlc_next = c->h_bw.has_block ? GEGS_M_BLOCK : env.block_next;
     env.jaws = c->d_jaws;
-    env.jawx_next = c->have_mlc ? GEGS_M_MLC : GEGS_M_PHANTOM;
+    env.jawx_next = c->have_mlc ? GEGS_M_MLC : env.mlc_next;
+    env.bw = c->d_bw;
+    env.wedge_nver = c->h_bw.wedge_nver;
     env.seed = seed;
     return env;
 }
@@ -655,9 +689,8 @@ static int parse_jobs(gegs_ctx *c, const gegs_job *jobs, uint32_t n_jobs, uint32
     for (uint32_t j = 0; j < n_jobs; ++j) {
         const gegs_job &jb = jobs[j];
         if (jb.plan >= n_plans) return fail(c, GEGS_ERR_ARG, "run: job plan out of range");
-        if (jb.field.module_first != GEGS_M_PHANTOM && jb.field.module_first != GEGS_M_MLC &&
-            jb.field.module_first != GEGS_M_SECJAWY)
-            return fail(c, GEGS_ERR_ARG, "run: fields start in the jaws, the MLC or the phantom module");
+        if (!valid_first(jb.field.module_first))
+            return fail(c, GEGS_ERR_ARG, "run: fields start in the jaws, the MLC, the block, the wedge or the phantom");
         if (jb.count == 0 || jb.first >= c->nsrc || jb.first + jb.count > c->nsrc)
             return fail(c, GEGS_ERR_ARG, "run: job record range outside the loaded source");
         fields[j] = jb.field;

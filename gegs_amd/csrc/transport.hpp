@@ -207,6 +207,9 @@ struct Env {
     int32_t mlc_next;                // module after the MLC
     int32_t jawx_next;               // module after the X jaws (the MLC when loaded)
     const gegs_float2 *jaws;         // jaw polygons per job: [job][32]
+    const gegs_float2 *bw;           // block and wedge cross-sections (18 vertices; null: neither loaded)
+    int32_t block_next;              // module after the block
+    int32_t wedge_nver;              // vertices of the wedge cross-section
     uint64_t seed;
 };
 
@@ -462,6 +465,7 @@ __device__ __forceinline__ float hownear_phantom(const Env &env, const Part &p) 
 
 #include "mlc.hpp"
 #include "jaws.hpp"
+#include "blockwedge.hpp"
 
 namespace gegs {
 
@@ -470,6 +474,7 @@ __device__ __forceinline__ uint32_t howfar_module(const Env &env, Part &p, float
     if (p.module == GEGS_M_PHANTOM) return howfar_phantom(env, p, t);
     if (p.module == GEGS_M_MLC) return howfar_mlc(env, p, t);
     if (p.module == GEGS_M_SECJAWY) return howfar_jaws<true>(env, p, t);
+    if (p.module == GEGS_M_BLOCK || p.module == GEGS_M_WEDGE) return howfar_block_wedge(env, p, t);
     return howfar_jaws<false>(env, p, t);
 }
 
@@ -477,6 +482,7 @@ __device__ __forceinline__ float hownear_module(const Env &env, Part &p) {
     if (p.module == GEGS_M_PHANTOM) return hownear_phantom(env, p);
     if (p.module == GEGS_M_MLC) return hownear_mlc(env, p);
     if (p.module == GEGS_M_SECJAWY) return hownear_jaws<true>(env, p);
+    if (p.module == GEGS_M_BLOCK || p.module == GEGS_M_WEDGE) return hownear_block_wedge(env, p);
     return hownear_jaws<false>(env, p);
 }
 

@@ -413,3 +413,186 @@ def random_aperture(seed, half=5.0, smooth=3):
     pos[:NLEAF] = np.where(inside, b, 0.0)            # +x leaf end at +pos[j]
     pos[NLEAF:] = np.where(inside, a, 0.0)            # -x leaf end at -pos[j+60]
     return pos
+
+
+# ------------------------------------------------------------------ block and wedge
+M_WEDGE = 4
+M_BLOCK = 5
+
+_PYRAMIDS_TITLE = "*********** start of CM PYRAMIDS with identifier "
+
+
+class BlockWedge(ctypes.Structure):
+    """ctypes mirror of ``gegs_block_wedge`` (include/gegs.h): the block's
+    two quadrilaterals, the wedge's cross-section (wedge_nver vertices) and
+    its bounding rectangle, head frame, z in cm from the source."""
+    _fields_ = [("block_xz", Float2 * 4), ("block_yz", Float2 * 4), ("wedge", Float2 * 6),
+                ("wedge_bound", Float2 * 4), ("has_block", ctypes.c_int32), ("has_wedge", ctypes.c_int32),
+                ("wedge_nver", ctypes.c_int32), ("pad", ctypes.c_int32)]
+
+
+def _f2(arr, pts):
+    for i, (x, y) in enumerate(pts):
+        arr[i].x, arr[i].y = float(np.float32(x)), float(np.float32(y))
+
+
+def block_polygons(coord):
+    """The block's cross-sections from its 10 PYRAMIDS values zmin, zmax,
+    XFP, XBP, XFN, XBN, YFP, YBP, YFN, YBN (init.cu:828-844): (XFP, zmin),
+    (XBP, zmax), (XBN, zmax), (XFN, zmin) and the same with the Y values."""
+    c = [np.float32(v) for v in coord]
+    if len(c) != 10:
+        raise ValueError("a block needs 10 values")
+    xz = [(c[2], c[0]), (c[3], c[1]), (c[5], c[1]), (c[4], c[0])]
+    yz = [(c[6], c[0]), (c[7], c[1]), (c[9], c[1]), (c[8], c[0])]
+    return np.array(xz, dtype=np.float32), np.array(yz, dtype=np.float32)
+
+
+def wedge_polygon(rmax, layers):
+    """The wedge's cross-section and bounding rectangle from its layers, each
+    zmin, zmax, XFP, XBP, XFN, XBN (read_egsinp_wedge, init.cu:752-778).  The
+    reference traces the slanted side of the layer stack: when the + side of
+    the first layer is vertical (XFP == XBP) the - side's front values, else
+    (XFN == XBN) the + side's; in the direction in which the first layer's
+    front and back differ, from the first layer up or from the last layer
+    down, one vertex per layer, closed by a vertex on the last (first) layer's
+    far face and one that pairs the first traced x with the far z.  Returns
+    (vertices [layers + 2, 2], rectangle [4, 2])."""
+    c = np.asarray(layers, dtype=np.float32).reshape(-1)
+    n = len(c) // 6
+    if n < 1 or n > 4:
+        raise ValueError("The number of layers %d is larger than 4!!" % n)   # error 9005 of the reference
+    if c[2] - c[3] == 0:
+        start, step, offset = (4, 6, 4) if c[4] - c[5] > 0 else (6 * n - 1, -6, 4)
+    elif c[4] - c[5] == 0:
+        start, step, offset = (2, 6, 2) if c[2] - c[3] < 0 else (6 * n - 3, -6, 2)
+    else:
+        raise ValueError("neither side of the wedge's first layer is vertical")
+    first_x = c[start]
+    pts = [None]
+    while 0 <= start < 6 * n:
+        pts.append((c[start], c[start - offset]))
+        start += step
+    start = start - step + step // 6
+    pts.append((c[start], c[start - offset]))
+    pts[0] = (first_x, c[start - offset])
+    bound = [(-rmax, c[0]), (rmax, c[0]), (rmax, c[6 * n - 5]), (-rmax, c[6 * n - 5])]
+    return np.array(pts, dtype=np.float32), np.array(bound, dtype=np.float32)
+
+
+def build_block_wedge(block=None, wedge=None):
+    """gegs_block_wedge from block = 10 PYRAMIDS values (or None) and wedge =
+    (rmax, layers) (or None)."""
+    bw = BlockWedge()
+    if block is not None:
+        xz, yz = block_polygons(block)
+        _f2(bw.block_xz, xz)
+        _f2(bw.block_yz, yz)
+        bw.has_block = 1
+    if wedge is not None:
+        geom, bound = wedge_polygon(*wedge)
+        _f2(bw.wedge, geom)
+        _f2(bw.wedge_bound, bound)
+        bw.wedge_nver = len(geom)
+        bw.has_wedge = 1
+    return bw
+
+
+def _numbers(line, n):
+    """The first n comma-separated numbers of a line (the reference reads
+    each with ``>> value >> ch``, the separator going into ch)."""
+    toks = [t for t in line.replace("\t", " ").split(",")]
+    out = []
+    for t in toks:
+        t = t.strip()
+        if not t:
+            continue
+        out.append(float(t.split()[0]))
+        if len(out) == n:
+            return out
+    raise ValueError("expected %d numbers in %r" % (n, line))
+
+
+def _find_pyramids(lines, ident):
+    title = (_PYRAMIDS_TITLE + ident)[:54]
+    for i, line in enumerate(lines):
+        if line[:54] == title:
+            return i
+    return None
+
+
+def read_egsinp_block(path):
+    """(10 PYRAMIDS values, medium name) of the block of a BEAMnrc input, or
+    None when the file has no PYRAMIDS component named BLOCK
+    (read_egsinp_block, init.cu:793-905): after the title line, RMAX, two
+    lines skipped, the 10 values, two lines skipped, the medium line."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    i = _find_pyramids(lines, "BLOCK")
+    if i is None:
+        return None
+    coord = _numbers(lines[i + 4], 10)
+    return coord, lines[i + 7]
+
+
+def read_egsinp_wedge(path):
+    """(rmax, layers [n, 6], medium name) of the wedge of a BEAMnrc input, or
+    None without a PYRAMIDS component named WEDGE (read_egsinp_wedge,
+    init.cu:692-791): after the title line, RMAX, a line skipped, the number
+    of layers (at most 4), one line of 6 values per layer, two lines
+    skipped, the medium line."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    i = _find_pyramids(lines, "WEDGE")
+    if i is None:
+        return None
+    rmax = _numbers(lines[i + 1], 1)[0]
+    n = int(_numbers(lines[i + 3], 1)[0])
+    if n > 4:
+        raise ValueError("The number of layers %d is larger than 4!!" % n)
+    layers = [_numbers(lines[i + 4 + j], 6) for j in range(n)]
+    return rmax, np.array(layers, dtype=np.float32), lines[i + 4 + n + 2]
+
+
+def write_egsinp_pyramids(path, block=None, wedge=None, block_medium="W700ICRU", wedge_medium="W700ICRU"):
+    """A BEAMnrc-style input holding a PYRAMIDS component BLOCK (10 values)
+    and/or WEDGE ((rmax, layers)), in the line layout the readers above
+    expect."""
+    out = ["BEAMnrc-style input with PYRAMIDS components"]
+    if block is not None:
+        out += [_PYRAMIDS_TITLE + "BLOCK", "10.0, RMAX", "block", "1, # LAYERS",
+                ", ".join("%.6g" % v for v in block) + ",", "0, 0, 0, 0,", "0, 0, 0, 0,", block_medium]
+    if wedge is not None:
+        rmax, layers = wedge
+        layers = np.asarray(layers, dtype=np.float64).reshape(-1, 6)
+        out += [_PYRAMIDS_TITLE + "WEDGE", "%.6g, RMAX" % rmax, "wedge", "%d, # LAYERS" % len(layers)]
+        out += [", ".join("%.6g" % v for v in row) + "," for row in layers]
+        out += ["0, 0, 0, 0,", "0, 0, 0, 0,", wedge_medium]
+    with open(path, "w") as f:
+        f.write("\n".join(out) + "\n")
+
+
+def block_wedge_from_run_input(ri):
+    """(gegs_block_wedge, block medium, wedge medium) of a run input
+    (runinput.RunInput) whose [Block] and/or [Wedge] module is On, or None
+    when both are off.  As the reference does (CUDA_EGS.cu:1231-1275,
+    init.cu:1327-1329), each file is the data directory followed by the
+    file property, and the medium is the line the component names."""
+    from .runinput import get
+    if not (ri.block or ri.wedge):
+        return None
+    block = wedge = None
+    bmed = wmed = None
+    if ri.block:
+        path = ri.data_directory + get(ri.sections, "Block", "block file").strip().strip('"')
+        r = read_egsinp_block(path)
+        if r is None:
+            raise ValueError("no PYRAMIDS component BLOCK in %s" % path)
+        block, bmed = r
+    if ri.wedge:
+        path = ri.data_directory + get(ri.sections, "Wedge", "wedge file").strip().strip('"')
+        r = read_egsinp_wedge(path)
+        if r is None:
+            raise ValueError("no PYRAMIDS component WEDGE in %s" % path)
+        wedge, wmed = (r[0], r[1]), r[2]
+    return build_block_wedge(block, wedge), bmed, wmed

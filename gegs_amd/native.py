@@ -24,7 +24,7 @@ SYMBOLS = [
     "gegs_ct_subsize", "gegs_ct_resample_grid", "gegs_ct_resample",
     "gegs_set_tuning", "gegs_get_tuning", "gegs_run_batched", "gegs_set_beamlets", "gegs_adjoint",
     "gegs_set_kernel_timing", "gegs_kernel_timing", "gegs_load_mlc", "gegs_set_leaf_positions",
-    "gegs_load_jaws", "gegs_set_jaw_positions",
+    "gegs_load_jaws", "gegs_set_jaw_positions", "gegs_load_block_wedge",
 ]
 
 PARTICLE_DTYPE = np.dtype([("e", "<f4"), ("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("u", "<f4"),
@@ -124,6 +124,7 @@ def load():
     L.gegs_run_batched.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp]
     L.gegs_load_mlc.argtypes = [vp, vp]
     L.gegs_load_jaws.argtypes = [vp, vp]
+    L.gegs_load_block_wedge.argtypes = [vp, vp]
     L.gegs_set_jaw_positions.argtypes = [vp, vp, ctypes.c_uint32]
     L.gegs_set_leaf_positions.argtypes = [vp, vp, ctypes.c_uint32]
     L.gegs_set_kernel_timing.argtypes = [vp, ctypes.c_int]
@@ -188,6 +189,8 @@ class Context:
         if getattr(wl, "jaws", None) is not None:
             self._check(self.L.gegs_load_jaws(self.h, ctypes.c_void_p(ctypes.addressof(wl.jaws))))
             self.set_jaw_positions(wl.jaw_positions)
+        if getattr(wl, "block_wedge", None) is not None:
+            self.load_block_wedge(wl.block_wedge)
         self._check(self.L.gegs_set_field(self.h, ctypes.c_void_p(ctypes.addressof(wl.field))))
         self.nvox = ph.nvox
 
@@ -253,6 +256,11 @@ class Context:
         if len(pos) == 0 or len(pos) % 120:
             raise ValueError("leaf positions come in sets of 120")
         self._check(self.L.gegs_set_leaf_positions(self.h, _p(pos), len(pos) // 120))
+
+    def load_block_wedge(self, bw):
+        """Block and wedge geometry (head.BlockWedge); has_block / has_wedge
+        switch the modules on for this context."""
+        self._check(self.L.gegs_load_block_wedge(self.h, ctypes.c_void_p(ctypes.addressof(bw))))
 
     def set_jaw_positions(self, xy):
         """Jaw openings of one or more sets, 4 floats each (x1, x2, y1, y2 in

@@ -142,18 +142,19 @@ def encode_phsp(src):
     return rec
 
 
-def head_phase_space(n, z_plane, ssd, field, seed, electron_fraction=0.03, edges=None, probs=None):
+def head_phase_space(n, z_plane, ssd, field, seed, electron_fraction=0.03, edges=None, probs=None, center=(0.0, 0.0)):
     """Synthetic phase space scored on the plane z = z_plane above the
     collimators: particles from a point source at the origin travelling
     towards points uniform over a (field + 2 cm) square at distance ssd;
     photons with the 6 MV-like spectrum, a fraction of contaminant electrons
-    (kinetic energy uniform in 0.2-4 MeV, total energy stored)."""
+    (kinetic energy uniform in 0.2-4 MeV, total energy stored).  The square
+    is centred on ``center`` (x, y at distance ssd)."""
     if edges is None:
         edges, probs = mv6_spectrum()
     rng = np.random.default_rng(seed)
     half = field / 2.0 + 1.0
-    px = rng.uniform(-half, half, n)
-    py = rng.uniform(-half, half, n)
+    px = rng.uniform(-half, half, n) + center[0]
+    py = rng.uniform(-half, half, n) + center[1]
     norm = np.sqrt(px * px + py * py + ssd * ssd)
     u, v, w = px / norm, py / norm, ssd / norm
     t = z_plane / w

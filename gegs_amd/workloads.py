@@ -50,6 +50,7 @@ class Workload:
         self._source_fn = source_fn
         self._tables = None
         self._regions = None
+        self.block_wedge = None               # head.BlockWedge, fields passing a block or a wedge
 
     @property
     def tables(self):
@@ -122,6 +123,40 @@ def ct_jaws_mlc(gantry_deg=0.0, aperture_seed=3, seed=2):
     wl.field = make_field(True, wl.field.source_to_isocenter, gantry_deg, module_first=head.M_SECJAWY)
     wl.jaws = head.build_jaws(**head.CLINAC_LIKE_JAWS)
     wl.jaw_positions = head.jaw_positions_for(wl.leaf_positions)
+    return wl
+
+
+# A shielding block and a physical wedge below the jaws (head frame, cm from
+# the source), as BEAMnrc PYRAMIDS values.  The block: 3 cm of tungsten at
+# z 55-58 cm with source-focused faces; its "X" values (which, as the
+# reference builds, bound the head's y) cover y in [0.55, 2.2] cm at the
+# block's top, its "Y" values x in [-1.65, 1.65] cm.  The wedge: three
+# 0.5 cm layers at z 60-61.5 cm whose - side steps from 2.4 to -1.2 cm, a
+# tungsten wedge with a 1.5 cm heel, over |x| <= 10 cm.
+BLOCK_PYRAMIDS = [55.0, 58.0, 2.2, 2.32, 0.55, 0.58, 1.65, 1.74, -1.65, -1.74]
+WEDGE_PYRAMIDS = (10.0, [[60.0, 60.5, 2.4, 2.4, 2.4, 1.2],
+                         [60.5, 61.0, 2.4, 2.4, 1.2, 0.0],
+                         [61.0, 61.5, 2.4, 2.4, 0.0, -1.2]])
+
+
+def ct_block_wedge(module_first=None, block=True, wedge=True, gantry_deg=0.0, seed=2):
+    """The CT workload (128x128x100 at 2.5 mm) under a block and/or a wedge:
+    the synthetic head phase space at z = 16.44 cm, aimed at x in [-2.5, 2.5],
+    y in [0, 5] cm at 100 cm (over the block and the wedge, and partly beside
+    them), enters the first loaded of block, wedge, then the phantom.
+    Regions 1-5: air, jaw W, MLC W, block W, wedge W."""
+    from . import head
+    phan = ph.ct_phantom(128, 128, 100, 0.25, seed)
+    sid = 100.0 + 0.30 * 128 * 0.25
+    if module_first is None:
+        module_first = head.M_BLOCK if block else head.M_WEDGE
+    fld = make_field(True, sid, gantry_deg, module_first=module_first)
+    names = [r[0] for r in ph.DEFAULT_RAMP] + ["W700ICRU"]
+    bw = head.build_block_wedge(BLOCK_PYRAMIDS if block else None, WEDGE_PYRAMIDS if wedge else None)
+    wl = Workload("ct_128x128x100_6MV_" + "_".join(n for n, on in (("block", block), ("wedge", wedge)) if on),
+                  phan, names, fld, ["AIR700ICRU", "W700ICRU", "W700ICRU", "W700ICRU", "W700ICRU"],
+                  lambda k, s: src.head_phase_space(k, PHSP_Z, 100.0, 3.0, s, center=(0.0, 2.5)))
+    wl.block_wedge = bw
     return wl
 
 

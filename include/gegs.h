@@ -35,6 +35,8 @@ extern "C" {
 #define GEGS_M_SECJAWY 1
 #define GEGS_M_SECJAWX 2
 #define GEGS_M_MLC 3
+#define GEGS_M_WEDGE 4
+#define GEGS_M_BLOCK 5
 #define GEGS_M_PHANTOM 6
 
 /* Secondary collimator jaws (SecJaws, read from the BEAMnrc JAWS input,
@@ -47,6 +49,24 @@ extern "C" {
 typedef struct gegs_jaws {
     float rmax, zminy, zmaxy, zminx, zmaxx;
 } gegs_jaws;
+
+/* Block and wedge (two BEAMnrc PYRAMIDS modules, read_egsinp_block and
+ * read_egsinp_wedge, init.cu:692-905).  Each is the intersection of two
+ * prisms given by their cross-sections in the head frame: the block by two
+ * quadrilaterals, block_xz (x, z) and block_yz (y, z); the wedge by a
+ * polygon of wedge_nver (3..6) vertices across the wedge and the rectangle
+ * wedge_bound (|r| <= rmax, the wedge's z range).  As built by the reference
+ * (ROTATE_Block_XY and ROTATE_Wedge_XY, CUDA_EGS.h:28-29), the first polygon
+ * is tested with the point (y, z) and the second with (x, z).  Regions: 1 air,
+ * 4 block, 5 wedge.  Modules are passed in the order jaws, MLC, block, wedge,
+ * phantom, skipping those not loaded (CUDA_EGS.cu:1279-1298). */
+typedef struct gegs_block_wedge {
+    gegs_float2 block_xz[4];
+    gegs_float2 block_yz[4];
+    gegs_float2 wedge[6];
+    gegs_float2 wedge_bound[4];
+    int32_t has_block, has_wedge, wedge_nver, pad;
+} gegs_block_wedge;
 
 /* Multileaf collimator (DYNVMLC-type, read_egsinp, init.cu:345-611).  Head
  * frame: z from the source; the MLC occupies z in [zmin, zmin + zthick].
@@ -197,6 +217,12 @@ int gegs_load_mlc(gegs_ctx *ctx, const gegs_mlc *mlc);
  * follow setup_secjaws (init.cu:1504-1592). */
 int gegs_load_jaws(gegs_ctx *ctx, const gegs_jaws *jaws);
 int gegs_set_jaw_positions(gegs_ctx *ctx, const float *x1x2y1y2, uint32_t n_sets);
+
+/* Block and wedge geometry (init.cu:692-905, cudaMemcpyToSymbol of blockXZ,
+ * blockYZ, goem_Wed, boundWed, nVer_Wed); has_block / has_wedge switch the
+ * modules on (Block_on, Wedge_on, CUDA_EGS.cu:1231-1270).  Fields may then
+ * start in GEGS_M_BLOCK or GEGS_M_WEDGE when no earlier module is loaded. */
+int gegs_load_block_wedge(gegs_ctx *ctx, const gegs_block_wedge *bw);
 
 /* Leaf positions of n_sets fields, 120 values per set, in the order of the
  * .mlcinfo leaf files (init.cu:1480-1500), cm at the isocentre plane: leaf

@@ -162,6 +162,8 @@ typedef struct octx {
     const gegs_float2 *endnodes;    /* [60][34] leaf ends of the field */
     const gegs_float2 *jaws;        /* [32] jaw polygons of the field (NULL: no jaws) */
     int jawx_next;                  /* module after the X jaws */
+    const gegs_block_wedge *bw;     /* block and wedge (NULL: neither) */
+    int mlc_next, block_next;       /* modules after the MLC and after the block */
     uint64_t seed;
     double *dose, *dose2, *hist_edep;
     uint32_t *hist_steps;
@@ -532,7 +534,7 @@ static uint32_t howfar_mlc(const octx *c, opart *p, float *t) {
                 index = i - sx;
             } else {
                 index = PASSMLC;
-                p->module = GEGS_M_PHANTOM;
+                p->module = c->mlc_next;
             }
         }
         float dm = dxz < dyz ? dxz : dyz;
@@ -545,7 +547,7 @@ static uint32_t howfar_mlc(const octx *c, opart *p, float *t) {
             *t = dm + EPSTEP;
             if (out) {
                 p->leaf = PASSMLC;
-                p->module = GEGS_M_PHANTOM;
+                p->module = c->mlc_next;
                 return 1u;
             }
         } else {
@@ -668,15 +670,68 @@ static float hownear_jaws(const octx *c, const opart *p, int is_y) {
     return d < other ? d : other;
 }
 
+/* ------------------------------------------------------------ block and wedge
+ * howfar_block / howfar_wedge (geom.cu:1037-1149) and their hownear
+ * (geom.cu:1376-1412) as built with ROTATE_Block_XY and ROTATE_Wedge_XY
+ * (CUDA_EGS.h:28-29): the first cross-section (blockXZ, goem_Wed) is tested
+ * with (y, z) and (v, w), the second (blockYZ, boundWed) with (x, z), (u, w).
+ * Outside the two prisms a particle steps onto the nearer one and counts as
+ * inside; inside both it leaves at the nearer exit for the next module; a
+ * ray that misses either prism passes to the next module at once. */
+static uint32_t howfar_prisms(opart *p, float *t, const gegs_float2 *qa, int na, const gegs_float2 *qb,
+                              uint32_t reg, int next) {
+    int ina, inb;
+    float da = ray_poly(mk2(p->y, p->z), mk2(p->v, p->w), qa, na, &ina);
+    float db = ray_poly(mk2(p->x, p->z), mk2(p->u, p->w), qb, 4, &inb);
+    if (da >= INFI_DIST || db >= INFI_DIST) {
+        p->module = next;
+        return 1u;
+    }
+    float bound = da < db ? da : db;
+    if (ina && inb) {
+        if (*t > bound) {
+            *t = bound + EPSTEP;
+            p->module = next;
+            return 1u;
+        }
+        return reg;
+    }
+    if (*t > bound) {
+        *t = bound + EPSTEP;
+        return reg;
+    }
+    return 1u;
+}
+
+static float hownear_prisms(const opart *p, const gegs_float2 *qa, int na, const gegs_float2 *qb) {
+    float a = poly_near(mk2(p->y, p->z), qa, na);
+    float b = poly_near(mk2(p->x, p->z), qb, 4);
+    return a < b ? a : b;
+}
+
 static uint32_t howfar(octx *c, opart *p, float *t) {
     if (p->module == GEGS_M_PHANTOM) return howfar_phantom(c, p, t);
     if (p->module == GEGS_M_MLC) return howfar_mlc(c, p, t);
+    if (p->module == GEGS_M_BLOCK) return howfar_prisms(p, t, c->bw->block_xz, 4, c->bw->block_yz, 4u, c->block_next);
+    if (p->module == GEGS_M_WEDGE)
+        return howfar_prisms(p, t, c->bw->wedge, c->bw->wedge_nver, c->bw->wedge_bound, 5u, GEGS_M_PHANTOM);
     return howfar_jaws(c, p, t, p->module == GEGS_M_SECJAWY);
 }
 static float hownear(octx *c, opart *p) {
     if (p->module == GEGS_M_PHANTOM) return hownear_phantom(c, p);
     if (p->module == GEGS_M_MLC) return hownear_mlc(c, p);
+    if (p->module == GEGS_M_BLOCK) return hownear_prisms(p, c->bw->block_xz, 4, c->bw->block_yz);
+    if (p->module == GEGS_M_WEDGE) return hownear_prisms(p, c->bw->wedge, c->bw->wedge_nver, c->bw->wedge_bound);
     return hownear_jaws(c, p, p->module == GEGS_M_SECJAWY);
+}
+
+/* module order jaws -> MLC -> block -> wedge -> phantom over the loaded
+ * modules (CUDA_EGS.cu:1279-1298) */
+static void set_module_order(octx *c, const gegs_mlc *mlc, const gegs_block_wedge *bw) {
+    c->bw = bw;
+    c->block_next = bw && bw->has_wedge ? GEGS_M_WEDGE : GEGS_M_PHANTOM;
+    c->mlc_next = bw && bw->has_block ? GEGS_M_BLOCK : c->block_next;
+    c->jawx_next = mlc ? GEGS_M_MLC : c->mlc_next;
 }
 
 
@@ -1986,6 +2041,7 @@ uint32_t oracle_howfar_mlc(const gegs_mlc *m, const gegs_float2 *ends, const geg
     memset(&c, 0, sizeof(c));
     c.mlc = m;
     c.endnodes = ends;
+    set_module_order(&c, m, NULL);
     opart p;
     memset(&p, 0, sizeof(p));
     p.x = in->x; p.y = in->y; p.z = in->z; p.u = in->u; p.v = in->v; p.w = in->w;
@@ -2051,7 +2107,8 @@ int oracle_run_mt(const gegs_media_tables *mt, const gegs_region *regions,
                   uint64_t hist_offset, uint64_t seed,
                   double *dose, double *dose2, double *hist_edep, uint32_t *hist_steps,
                   gegs_counters *counters, double *balance, int nthreads,
-                  const gegs_mlc *mlc, const gegs_float2 *endnodes, const gegs_float2 *jaws) {
+                  const gegs_mlc *mlc, const gegs_float2 *endnodes, const gegs_float2 *jaws,
+                  const gegs_block_wedge *bw) {
     size_t nvox = (size_t)nx * ny * nz;
     if (nthreads < 1) nthreads = 1;
     if ((uint64_t)nthreads > n && n > 0) nthreads = (int)n;
@@ -2064,7 +2121,8 @@ int oracle_run_mt(const gegs_media_tables *mt, const gegs_region *regions,
         c->mt = mt; c->reg = regions; c->xb = xb; c->yb = yb; c->zb = zb;
         c->nx = nx; c->ny = ny; c->nz = nz; c->f = *field; c->seed = seed;
         c->mlc = mlc; c->endnodes = endnodes;
-        c->jaws = jaws; c->jawx_next = mlc ? GEGS_M_MLC : GEGS_M_PHANTOM;
+        c->jaws = jaws;
+        set_module_order(c, mlc, bw);
         c->dose = t == 0 ? dose : (double *)calloc(nvox, sizeof(double));
         c->dose2 = t == 0 ? dose2 : (double *)calloc(nvox, sizeof(double));
         c->tmp = (double *)calloc(nvox, sizeof(double));
@@ -2117,7 +2175,7 @@ int oracle_run(const gegs_media_tables *mt, const gegs_region *regions,
                double *dose, double *dose2, double *hist_edep, uint32_t *hist_steps,
                gegs_counters *counters, double *balance) {
     return oracle_run_mt(mt, regions, nx, ny, nz, xb, yb, zb, field, src, n, hist_offset, seed,
-                         dose, dose2, hist_edep, hist_steps, counters, balance, 1, NULL, NULL, NULL);
+                         dose, dose2, hist_edep, hist_steps, counters, balance, 1, NULL, NULL, NULL, NULL);
 }
 
 /* Single-sampler entry point for uniform-for-uniform checks: the handler of
@@ -2130,6 +2188,7 @@ int oracle_interaction(const gegs_media_tables *mt, const gegs_region *regions,
                        const gegs_field *field, const gegs_particle *in, uint64_t seed,
                        gegs_particle *out, int max_out, double *edep_out) {
     octx *c = (octx *)calloc(1, sizeof(octx));
+    set_module_order(c, NULL, NULL);
     size_t nvox = (size_t)nx * ny * nz;
     double *dose = (double *)calloc(nvox, sizeof(double));
     double *dose2 = (double *)calloc(nvox, sizeof(double));
@@ -2237,4 +2296,22 @@ void oracle_to_phantom(const gegs_field *field, const float *pos_dir, float *out
     p.x = pos_dir[0]; p.y = pos_dir[1]; p.z = pos_dir[2];
     p.u = pos_dir[3]; p.v = pos_dir[4]; p.w = pos_dir[5];
     to_phantom(&c, &p, out);
+}
+
+/* block / wedge probe for the geometry tests: howfar (returns the region,
+ * updates t and *module) and hownear of one particle in the block (is_wedge
+ * 0) or the wedge (1) of bw, with the module order of bw alone */
+uint32_t oracle_howfar_block_wedge(const gegs_block_wedge *bw, int is_wedge, const gegs_particle *in, int32_t *module,
+                                   float *t, float *tperp) {
+    octx c;
+    memset(&c, 0, sizeof(c));
+    set_module_order(&c, NULL, bw);
+    opart p;
+    memset(&p, 0, sizeof(p));
+    p.x = in->x; p.y = in->y; p.z = in->z; p.u = in->u; p.v = in->v; p.w = in->w;
+    p.region = in->region; p.module = is_wedge ? GEGS_M_WEDGE : GEGS_M_BLOCK;
+    *tperp = hownear(&c, &p);
+    uint32_t r = howfar(&c, &p, t);
+    *module = p.module;
+    return r;
 }

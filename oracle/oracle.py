@@ -43,10 +43,12 @@ def lib():
                                  vp, vp, vp, vp, vp, ctypes.c_uint64, ctypes.c_uint64,
                                  ctypes.c_uint64, vp, vp, vp, vp, vp, vp]
         L.oracle_run_mt.restype = ctypes.c_int
-        L.oracle_run_mt.argtypes = L.oracle_run.argtypes + [ctypes.c_int, vp, vp, vp]
+        L.oracle_run_mt.argtypes = L.oracle_run.argtypes + [ctypes.c_int, vp, vp, vp, vp]
         L.oracle_jaw_polys.argtypes = [vp, vp, ctypes.c_float, vp]
         L.oracle_howfar_jaws.restype = ctypes.c_uint32
         L.oracle_howfar_jaws.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, vp]
+        L.oracle_howfar_block_wedge.restype = ctypes.c_uint32
+        L.oracle_howfar_block_wedge.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
         L.oracle_leaf_ends.argtypes = [vp, vp, ctypes.c_float, vp]
         L.oracle_leaf_index.restype = ctypes.c_int
         L.oracle_leaf_index.argtypes = [vp, ctypes.c_float, ctypes.c_float]
@@ -106,11 +108,13 @@ def run(workload, source, seed, hist_offset=0, per_history=False, threads=1):
         ends_ptr = _p(ends)
     if first == 1:                                        # the field starts in the jaws
         jaws = jaw_polys(workload.jaws, workload.jaw_positions, sid)
+    bw = getattr(workload, "block_wedge", None)
+    bw_ptr = ctypes.c_void_p(ctypes.addressof(bw)) if bw is not None else None
     rc = L.oracle_run_mt(ctypes.c_void_p(workload.tables.ptr()), _p(reg), phan.nx, phan.ny, phan.nz,
                          _p(xb), _p(yb), _p(zb), ctypes.c_void_p(ctypes.addressof(workload.field)),
                          _p(src), n, hist_offset, seed, _p(dose), _p(dose2), _p(edep), _p(steps),
                          ctypes.c_void_p(ctypes.addressof(cnt)), _p(bal), int(threads), mlc_ptr, ends_ptr,
-                         _p(jaws))
+                         _p(jaws), bw_ptr)
     if rc != 0:
         raise RuntimeError("oracle_run failed: %d" % rc)
     return dict(dose=dose, dose2=dose2, edep=edep, steps=steps, counters=cnt,
@@ -133,6 +137,18 @@ def howfar_jaws(polys, is_y, jawx_next, particle, t):
     r = lib().oracle_howfar_jaws(_p(np.ascontiguousarray(polys, dtype=np.float32)), int(is_y), int(jawx_next),
                                  _p(part), ctypes.byref(mod), ctypes.byref(tt))
     return int(r), tt.value, mod.value
+
+
+def howfar_block_wedge(bw, is_wedge, particle, t):
+    """(region, t, module, tperp) of the oracle's howfar_block / howfar_wedge
+    and hownear for one particle."""
+    part = np.ascontiguousarray(particle)
+    tt = ctypes.c_float(t)
+    tp = ctypes.c_float(0.0)
+    mod = ctypes.c_int32(0)
+    r = lib().oracle_howfar_block_wedge(ctypes.c_void_p(ctypes.addressof(bw)), int(is_wedge), _p(part),
+                                        ctypes.byref(mod), ctypes.byref(tt), ctypes.byref(tp))
+    return int(r), tt.value, mod.value, tp.value
 
 
 def leaf_ends(mlc, positions, sid):

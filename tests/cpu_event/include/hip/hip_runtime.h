@@ -18,6 +18,7 @@
 #define __global__
 #define __forceinline__ inline
 #define __launch_bounds__(...)
+#define GEGS_WAVES_PER_EU(n)
 #define __shared__
 
 struct dim3 {
