@@ -898,6 +898,7 @@ inline bool same_args(const EvArgs &x, const EvArgs &y) {
            x.env.pcohe == y.env.pcohe && x.env.mlc == y.env.mlc && x.env.endnodes == y.env.endnodes &&
            x.env.mlc_next == y.env.mlc_next && x.env.jaws == y.env.jaws && x.env.jawx_next == y.env.jawx_next &&
            x.env.bw == y.env.bw && x.env.block_next == y.env.block_next && x.env.wedge_nver == y.env.wedge_nver &&
+           x.env.fixes == y.env.fixes &&
            x.tally_batches == y.tally_batches;
 }
 

@@ -50,6 +50,7 @@ struct gegs_ctx {
     gegs_float2 *d_jaws = nullptr;       // jaw polygons per job
     gegs_block_wedge h_bw{};             // block and wedge geometry
     gegs_float2 *d_bw = nullptr;         // their 18 vertices on the device
+    uint32_t fixes = 0;                  // GEGS_FIX_* mask
     size_t jaws_cap = 0;
     uint32_t *d_beamlet = nullptr;       // beamlet of each history slot
     float *d_bweight = nullptr;          // weight of each beamlet
@@ -299,6 +300,13 @@ int gegs_load_block_wedge(gegs_ctx *c, const gegs_block_wedge *bw) {
     return GEGS_OK;
 }
 
+int gegs_set_fixes(gegs_ctx *c, uint32_t mask) {
+    if (!c) return fail(c, GEGS_ERR_ARG, "gegs_set_fixes: null context");
+    if (mask & ~GEGS_FIX_ALL) return fail(c, GEGS_ERR_ARG, "gegs_set_fixes: unknown fix bits");
+    c->fixes = mask;
+    return GEGS_OK;
+}
+
 int gegs_set_jaw_positions(gegs_ctx *c, const float *xy, uint32_t n_sets) {
     if (!c || (!xy && n_sets)) return fail(c, GEGS_ERR_ARG, "gegs_set_jaw_positions: null argument");
     for (uint32_t k = 0; k < n_sets; ++k)
@@ -503,6 +511,7 @@ static Env make_env(gegs_ctx *c, uint64_t seed, uint32_t njob) {
     env.jawx_next = c->have_mlc ? GEGS_M_MLC : env.mlc_next;
     env.bw = c->d_bw;
     env.wedge_nver = c->h_bw.wedge_nver;
+    env.fixes = c->fixes;
     env.seed = seed;
     return env;
 }

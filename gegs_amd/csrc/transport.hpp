@@ -207,9 +207,10 @@ struct Env {
     int32_t mlc_next;                // module after the MLC
     int32_t jawx_next;               // module after the X jaws (the MLC when loaded)
     const gegs_float2 *jaws;         // jaw polygons per job: [job][32]
-    const gegs_float2 *bw;           // block and wedge cross-sections (18 vertices; null: neither loaded)
-    int32_t block_next;              // module after the block
-    int32_t wedge_nver;              // vertices of the wedge cross-section
+    const gegs_float2 *bw = nullptr; // block and wedge cross-sections (18 vertices; null: neither loaded)
+    int32_t block_next = GEGS_M_PHANTOM;  // module after the block
+    int32_t wedge_nver = 0;          // vertices of the wedge cross-section
+    uint32_t fixes = 0;              // GEGS_FIX_* corrections of reference quirks (0: as the reference)
     uint64_t seed;
 };
 
@@ -438,8 +439,11 @@ __device__ __forceinline__ float hownear_phantom(const Env &env, const Part &p) 
     const Local q = phantom_frame(field_of(env, p), p);
     const uint32_t nx = env.nx, ny = env.ny, nz = env.nz;
     const float *xb = env.xb, *yb = env.yb, *zb = env.zb;
-    if (p.region > 4) {
-        uint32_t ir = p.region - 5u;
+    // the reference takes the voxel of region r as r - 5 (quirk Q1, geom.cu:1487-1490): the
+    // neighbouring voxel; GEGS_FIX_Q1 uses the voxel itself, r - 6, as howfar does
+    const uint32_t first = (env.fixes & GEGS_FIX_Q1) ? 6u : 5u;
+    if (p.region >= first) {
+        uint32_t ir = p.region - first;
         const int iz = (int)(ir / (nx * ny));
         ir -= (uint32_t)iz * nx * ny;
         const int iy = (int)(ir / nx);

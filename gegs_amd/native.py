@@ -24,7 +24,7 @@ SYMBOLS = [
     "gegs_ct_subsize", "gegs_ct_resample_grid", "gegs_ct_resample",
     "gegs_set_tuning", "gegs_get_tuning", "gegs_run_batched", "gegs_set_beamlets", "gegs_adjoint",
     "gegs_set_kernel_timing", "gegs_kernel_timing", "gegs_load_mlc", "gegs_set_leaf_positions",
-    "gegs_load_jaws", "gegs_set_jaw_positions", "gegs_load_block_wedge",
+    "gegs_load_jaws", "gegs_set_jaw_positions", "gegs_load_block_wedge", "gegs_set_fixes",
 ]
 
 PARTICLE_DTYPE = np.dtype([("e", "<f4"), ("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("u", "<f4"),
@@ -125,6 +125,7 @@ def load():
     L.gegs_load_mlc.argtypes = [vp, vp]
     L.gegs_load_jaws.argtypes = [vp, vp]
     L.gegs_load_block_wedge.argtypes = [vp, vp]
+    L.gegs_set_fixes.argtypes = [vp, ctypes.c_uint32]
     L.gegs_set_jaw_positions.argtypes = [vp, vp, ctypes.c_uint32]
     L.gegs_set_leaf_positions.argtypes = [vp, vp, ctypes.c_uint32]
     L.gegs_set_kernel_timing.argtypes = [vp, ctypes.c_int]
@@ -143,6 +144,9 @@ def load():
 
 def _p(a):
     return ctypes.c_void_p(a.ctypes.data)
+
+
+FIX_Q1 = 1                 # GEGS_FIX_Q1: hownear_phantom from the particle's own voxel
 
 
 class GegsError(RuntimeError):
@@ -261,6 +265,11 @@ class Context:
         """Block and wedge geometry (head.BlockWedge); has_block / has_wedge
         switch the modules on for this context."""
         self._check(self.L.gegs_load_block_wedge(self.h, ctypes.c_void_p(ctypes.addressof(bw))))
+
+    def set_fixes(self, mask):
+        """GEGS_FIX_* corrections of reference quirks for later runs of this
+        context (0, the default: results as the reference's)."""
+        self._check(self.L.gegs_set_fixes(self.h, int(mask)))
 
     def set_jaw_positions(self, xy):
         """Jaw openings of one or more sets, 4 floats each (x1, x2, y1, y2 in

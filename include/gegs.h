@@ -224,6 +224,17 @@ int gegs_set_jaw_positions(gegs_ctx *ctx, const float *x1x2y1y2, uint32_t n_sets
  * start in GEGS_M_BLOCK or GEGS_M_WEDGE when no earlier module is loaded. */
 int gegs_load_block_wedge(gegs_ctx *ctx, const gegs_block_wedge *bw);
 
+/* Opt-in corrections of quirks of the reference (default 0: every quirk kept,
+ * results as the reference's).  No counterpart in the reference, which has
+ * them compiled in.
+ *   GEGS_FIX_Q1: hownear_phantom measures from the particle's own voxel,
+ *   region - 6, where the reference takes region - 5, the next voxel
+ *   (geom.cu:1487-1490); electron steps are then limited by the true
+ *   distance to the voxel boundary. */
+#define GEGS_FIX_Q1 1u
+#define GEGS_FIX_ALL 1u
+int gegs_set_fixes(gegs_ctx *ctx, uint32_t mask);
+
 /* Leaf positions of n_sets fields, 120 values per set, in the order of the
  * .mlcinfo leaf files (init.cu:1480-1500), cm at the isocentre plane: leaf
  * pair j has its +x end at +pos[j] and its -x end at -pos[j+60] (the

@@ -164,6 +164,7 @@ typedef struct octx {
     int jawx_next;                  /* module after the X jaws */
     const gegs_block_wedge *bw;     /* block and wedge (NULL: neither) */
     int mlc_next, block_next;       /* modules after the MLC and after the block */
+    uint32_t fixes;                 /* GEGS_FIX_* corrections (oracle_set_fixes) */
     uint64_t seed;
     double *dose, *dose2, *hist_edep;
     uint32_t *hist_steps;
@@ -178,6 +179,10 @@ typedef struct octx {
     int sp;
     gegs_counters cnt;
 } octx;
+
+/* GEGS_FIX_* mask that every context made after oracle_set_fixes uses */
+static uint32_t g_fixes = 0;
+void oracle_set_fixes(uint32_t mask) { g_fixes = mask; }
 
 static void push(octx *c, const opart *p) {
     if (c->sp >= STACK_MAX) {
@@ -331,7 +336,8 @@ static uint32_t howfar_phantom(const octx *c, const opart *p0, float *t) {
     return 0;
 }
 
-/* hownear_phantom, geom.cu:1422-1543, with quirk Q1 (region - 5) */
+/* hownear_phantom, geom.cu:1422-1543, with quirk Q1 (region - 5); with
+ * GEGS_FIX_Q1 set by oracle_set_fixes, the particle's own voxel (region - 6) */
 static float hownear_phantom(const octx *c, const opart *p0) {
     float q[6];
     to_phantom(c, p0, q);
@@ -339,8 +345,9 @@ static float hownear_phantom(const octx *c, const opart *p0) {
     const uint32_t nx = c->nx, ny = c->ny, nz = c->nz;
     const float *xb = c->xb, *yb = c->yb, *zb = c->zb;
     float tperp = INFI_DIST;
-    if (p0->region > 4) {
-        uint32_t ir = p0->region - 1 - 2 - 2;
+    const uint32_t first = (c->fixes & GEGS_FIX_Q1) ? 6u : 5u;
+    if (p0->region >= first) {
+        uint32_t ir = p0->region - first;
         int iz = (int)(ir / (nx * ny));
         ir -= (uint32_t)iz * nx * ny;
         int iy = (int)(ir / nx);
@@ -728,6 +735,7 @@ static float hownear(octx *c, opart *p) {
 /* module order jaws -> MLC -> block -> wedge -> phantom over the loaded
  * modules (CUDA_EGS.cu:1279-1298) */
 static void set_module_order(octx *c, const gegs_mlc *mlc, const gegs_block_wedge *bw) {
+    c->fixes = g_fixes;
     c->bw = bw;
     c->block_next = bw && bw->has_wedge ? GEGS_M_WEDGE : GEGS_M_PHANTOM;
     c->mlc_next = bw && bw->has_block ? GEGS_M_BLOCK : c->block_next;
@@ -2278,6 +2286,7 @@ float oracle_hownear_phantom(uint32_t nx, uint32_t ny, uint32_t nz, const float 
     octx c;
     memset(&c, 0, sizeof(c));
     c.xb = xb; c.yb = yb; c.zb = zb; c.nx = nx; c.ny = ny; c.nz = nz; c.f = *field;
+    c.fixes = g_fixes;
     opart p;
     memset(&p, 0, sizeof(p));
     p.x = pos[0]; p.y = pos[1]; p.z = pos[2];

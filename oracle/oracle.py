@@ -47,6 +47,7 @@ def lib():
         L.oracle_jaw_polys.argtypes = [vp, vp, ctypes.c_float, vp]
         L.oracle_howfar_jaws.restype = ctypes.c_uint32
         L.oracle_howfar_jaws.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, vp]
+        L.oracle_set_fixes.argtypes = [ctypes.c_uint32]
         L.oracle_howfar_block_wedge.restype = ctypes.c_uint32
         L.oracle_howfar_block_wedge.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp]
         L.oracle_leaf_ends.argtypes = [vp, vp, ctypes.c_float, vp]
@@ -137,6 +138,12 @@ def howfar_jaws(polys, is_y, jawx_next, particle, t):
     r = lib().oracle_howfar_jaws(_p(np.ascontiguousarray(polys, dtype=np.float32)), int(is_y), int(jawx_next),
                                  _p(part), ctypes.byref(mod), ctypes.byref(tt))
     return int(r), tt.value, mod.value
+
+
+def set_fixes(mask):
+    """GEGS_FIX_* corrections of reference quirks for every later oracle call
+    (0: the reference's behaviour)."""
+    lib().oracle_set_fixes(int(mask))
 
 
 def howfar_block_wedge(bw, is_wedge, particle, t):

@@ -197,6 +197,34 @@ def test_hownear_keeps_reference_offset():
         assert t <= expect + 1e-6
 
 
+def test_hownear_with_q1_fix_measures_the_own_voxel():
+    """With GEGS_FIX_Q1 hownear decodes region-6, the particle's own voxel:
+    the distance is the smallest of the six distances to its faces (> 0 inside)."""
+    L = oracle.lib()
+    w = workloads.ct(nx=8, ny=6, nz=5, voxel=0.7, seed=3)
+    ph = w.phantom
+    xb, yb, zb = ph.padded_bounds()
+    fld = workloads.make_field(False, 100.0)
+    rng = np.random.default_rng(12)
+    oracle.set_fixes(1)
+    try:
+        for _ in range(500):
+            ix, iy, iz = rng.integers(ph.nx), rng.integers(ph.ny), rng.integers(ph.nz)
+            f = rng.uniform(0.05, 0.95, 3)
+            pos = np.array([ph.xb[ix] + f[0] * 0.7, ph.yb[iy] + f[1] * 0.7, ph.zb[iz] + f[2] * 0.7 + 100.0],
+                           dtype=np.float32)
+            region = 6 + ix + iy * ph.nx + iz * ph.nx * ph.ny
+            t = L.oracle_hownear_phantom(ph.nx, ph.ny, ph.nz, xb.ctypes.data, yb.ctypes.data, zb.ctypes.data,
+                                         ctypes.addressof(fld), pos.ctypes.data, region)
+            q = pos.astype(np.float64) - np.array([0.0, 0.0, 100.0])
+            expect = min(q[0] - ph.xb[ix], ph.xb[ix + 1] - q[0], q[1] - ph.yb[iy], ph.yb[iy + 1] - q[1],
+                         q[2] - ph.zb[iz], ph.zb[iz + 1] - q[2])
+            assert t > 0.0
+            assert abs(t - expect) < 2e-5
+    finally:
+        oracle.set_fixes(0)
+
+
 @pytest.mark.parametrize("change_xyz", [0, 1])
 def test_phantom_frame_within_2ulp_of_fp64(change_xyz):
     """The head -> phantom transform (geom.cu:59-109) runs in binary32 as in
