@@ -64,8 +64,12 @@ __device__ __forceinline__ uint32_t howfar_jaws(const Env &env, Part &p, float &
             const float dopen = ray_polygon(pj, dj, geom + 2, 4, dummy);
             if (dopen < INFI_DIST) {
                 // the X pair steps by the boundary distance of the (x, z)
-                // plane here (quirk of geom.cu:992, kept)
-                if (t > dopen) { inew = REG_AIR; t = (Y ? dopen : dis_yz) + EPSTEP; }
+                // plane here (quirk Q8 of geom.cu:992, kept unless
+                // GEGS_FIX_Q8 is set: then by the distance to the opening)
+                if (t > dopen) {
+                    inew = REG_AIR;
+                    t = ((Y || (env.fixes & GEGS_FIX_Q8)) ? dopen : dis_yz) + EPSTEP;
+                }
             } else if (t > bound) {
                 inew = 0;
                 t = bound + EPSTEP;

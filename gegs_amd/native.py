@@ -147,6 +147,7 @@ def _p(a):
 
 
 FIX_Q1 = 1                 # GEGS_FIX_Q1: hownear_phantom from the particle's own voxel
+FIX_Q8 = 2                 # GEGS_FIX_Q8: X-jaw step into the opening by the opening's distance
 
 
 class GegsError(RuntimeError):

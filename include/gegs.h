@@ -232,7 +232,11 @@ int gegs_load_block_wedge(gegs_ctx *ctx, const gegs_block_wedge *bw);
  *   (geom.cu:1487-1490); electron steps are then limited by the true
  *   distance to the voxel boundary. */
 #define GEGS_FIX_Q1 1u
-#define GEGS_FIX_ALL 1u
+/*   GEGS_FIX_Q8: a particle in the X jaw's tungsten heading into the opening
+ *   steps by its distance to the opening, where the reference steps by the
+ *   boundary distance of the other plane (geom.cu:992). */
+#define GEGS_FIX_Q8 2u
+#define GEGS_FIX_ALL 3u
 int gegs_set_fixes(gegs_ctx *ctx, uint32_t mask);
 
 /* Leaf positions of n_sets fields, 120 values per set, in the order of the

@@ -639,7 +639,9 @@ static uint32_t howfar_jaws(const octx *c, opart *p, float *t, int is_y) {
             float d0 = ray_poly(pp, vv, g + 2, 4, &dummy);
             if (d0 < INFI_DIST) {
                 /* geom.cu:992: the X pair steps by dis_yz here */
-                if (*t > d0) { r = 1u; *t = (is_y ? d0 : dyz) + EPSTEP; }
+                /* the X pair steps by the (x, z) boundary distance (quirk Q8, geom.cu:992);
+                 * GEGS_FIX_Q8 steps by the distance to the opening */
+                if (*t > d0) { r = 1u; *t = ((is_y || (c->fixes & GEGS_FIX_Q8)) ? d0 : dyz) + EPSTEP; }
             } else if (*t > bound) {
                 r = 0u;
                 *t = bound + EPSTEP;
@@ -2094,6 +2096,7 @@ uint32_t oracle_howfar_jaws(const gegs_float2 *polys, int is_y, int jawx_next, c
     memset(&c, 0, sizeof(c));
     c.jaws = polys;
     c.jawx_next = jawx_next;
+    c.fixes = g_fixes;
     opart p;
     memset(&p, 0, sizeof(p));
     p.x = in->x; p.y = in->y; p.z = in->z; p.u = in->u; p.v = in->v; p.w = in->w;

@@ -72,3 +72,31 @@ def test_q1_fix_dose_equivalent_to_reference(ct_small):
     assert nv > 200
     assert 0.75 < chi2 < 1.35, (chi2, meanz, nv)
     assert abs(meanz) < 0.2
+
+
+def test_all_fixes_jaws_mlc_bit_exact():
+    """GEGS_FIX_Q1 | GEGS_FIX_Q8 on the jaws -> MLC -> CT workload: the GPU
+    reproduces the oracle with the same mask history by history, and the
+    event path reproduces the plain kernel."""
+    wl = workloads.ct_jaws_mlc()
+    src = wl.source(4000, 19)
+    print("fixes Q1|Q8: jaws + MLC workload, 4000 histories, seed 43", flush=True)
+    ctx = native.Context(0)
+    try:
+        ctx.load_workload(wl)
+        ctx.set_source(src)
+        ctx.set_fixes(native.FIX_Q1 | native.FIX_Q8)
+        g = ctx.run(seed=43, n_batches=2, method="plain", per_history=True)
+        e = ctx.run(seed=43, n_batches=2, method="event", per_history=True)
+    finally:
+        ctx.close()
+    oracle.set_fixes(native.FIX_Q1 | native.FIX_Q8)
+    try:
+        o = oracle.run(wl, src, seed=43, per_history=True, threads=8)
+    finally:
+        oracle.set_fixes(0)
+    assert np.array_equal(g["steps"], o["steps"])
+    assert np.array_equal(g["edep"], o["edep"])
+    assert e["counters"]["overflow"] == 0
+    assert np.array_equal(e["steps"], g["steps"])
+    assert np.allclose(e["edep"], g["edep"], rtol=1e-12, atol=1e-12)

@@ -125,3 +125,46 @@ def test_ct_jaws_mlc_transport_conserves_and_collimates():
     narrow.jaw_positions = np.array([-1.0, 1.0, -1.0, 1.0], dtype=np.float32)
     o2 = oracle.run(narrow, src, seed=3, per_history=True, threads=4)
     assert o2["dose"].sum() < 0.5 * o["dose"].sum()
+
+
+def test_x_jaw_step_into_the_opening_with_q8_fix(geom):
+    """Quirk Q8 (geom.cu:992): from the X pair's tungsten towards the opening
+    the reference steps by the boundary distance of the other plane, so the
+    step can end short of the opening or beyond it; with GEGS_FIX_Q8 the step
+    is the distance to the opening's focused face (closed form) plus EPSTEP
+    every time."""
+    j, xy, poly = geom
+    rng = np.random.default_rng(21)
+    ends = {0: [], 1: []}
+    for mask in (0, 2):
+        oracle.set_fixes(mask)
+        try:
+            for _ in range(3000):
+                z = rng.uniform(j.zminx + 0.3, j.zmaxx - 0.3)
+                x = rng.uniform(-6, 6)
+                y = rng.uniform(-8, 8)
+                if classify(j, xy, 0, x, y, z) != 2:
+                    continue
+                d = rng.normal(size=3)
+                d /= np.linalg.norm(d)
+                part = np.zeros(1, dtype=PARTICLE_DTYPE)
+                part["x"], part["y"], part["z"] = x, y, z
+                part["u"], part["v"], part["w"] = d
+                part["region"] = 2
+                r, t, module = oracle.howfar_jaws(poly, 0, 3, part, 1e8)
+                if r != 1:
+                    continue
+                # distance to the opening's focused faces y = k z / SID, k = x1, x2
+                best = np.inf
+                for k in (float(xy[0]), float(xy[1])):
+                    den = d[1] - k * d[2] / SID
+                    if den != 0.0:
+                        sk = (k * z / SID - y) / den
+                        zk = z + sk * d[2]
+                        if sk > 0 and j.zminx <= zk <= j.zmaxx:
+                            best = min(best, sk)
+                ends[mask // 2].append(abs((t - 5e-4) - best) < 1e-4 * max(1.0, best))
+        finally:
+            oracle.set_fixes(0)
+    assert len(ends[1]) > 100 and all(ends[1])
+    assert len(ends[0]) > 100 and not all(ends[0])
