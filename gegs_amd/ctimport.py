@@ -256,11 +256,12 @@ def read_ct_ramps(path, max_media=6):
 
 
 def ctcreate(list_path, thickness, box_min=(0, 0, 0), box_max=(0, 0, 0), ramp=ph.DEFAULT_RAMP,
-             lower=ph.DEFAULT_RAMP_LOWER):
-    """DICOM series -> Phantom: read, crop, resample, CT ramp."""
+             lower=ph.DEFAULT_RAMP_LOWER, fix_q10=False):
+    """DICOM series -> Phantom: read, crop, resample, CT ramp (fix_q10: CT
+    values below the ramp take its first medium instead of vacuum)."""
     vol = subsize(read_dicom_series(list_path), box_min, box_max)
     hu, xb, yb, zb = resample(vol, thickness)
-    med, dens = ph.ct_to_medium(hu.reshape(-1), ramp=ramp, lower=lower)
+    med, dens = ph.ct_to_medium(hu.reshape(-1), ramp=ramp, lower=lower, fix_q10=fix_q10)
     return ph.Phantom([r[0] for r in ramp], xb, yb, zb, med, dens)
 
 

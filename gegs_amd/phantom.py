@@ -122,8 +122,11 @@ def region_table(phantom, tables, other_media=None, flags=DEFAULT_FLAGS):
 
 
 # ------------------------------------------------------------------ ctcreate
-def ct_to_medium(hu, ramp=DEFAULT_RAMP, lower=DEFAULT_RAMP_LOWER):
-    """CTToMedium (ctcreate.c:824-1140): HU -> (1-based medium, density)."""
+def ct_to_medium(hu, ramp=DEFAULT_RAMP, lower=DEFAULT_RAMP_LOWER, fix_q10=False):
+    """CTToMedium (ctcreate.c:824-1140): HU -> (1-based medium, density).
+    CT values below the ramp become medium 0, vacuum, as in the reference
+    (quirk Q10, ctcreate.c:1060-1066); with fix_q10 they take the first ramp
+    medium at its lowest density, as values above the ramp take the last."""
     hu = np.asarray(hu, dtype=np.int32)
     med = np.zeros(hu.shape, dtype=np.int32)
     dens = np.zeros(hu.shape, dtype=np.float32)
@@ -132,6 +135,9 @@ def ct_to_medium(hu, ramp=DEFAULT_RAMP, lower=DEFAULT_RAMP_LOWER):
     above = hu > upper[-1]
     med[above] = len(ramp)
     dens[above] = np.float32(ramp[-1][3])
+    if fix_q10:
+        med[below] = 1
+        dens[below] = np.float32(ramp[0][2])
     rest = ~(below | above)
     assigned = np.zeros(hu.shape, dtype=bool)
     for i, (_, up, dlo, dhi) in enumerate(ramp):

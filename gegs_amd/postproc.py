@@ -124,8 +124,9 @@ def _cell(b, n, p):
     return i
 
 
-def abs_dose(dose, xb, yb, zb, pos):
-    """(dose at pos, (i, j, k) of the voxel taken as nearest, its dose)."""
+def abs_dose(dose, xb, yb, zb, pos, fix_q9=False):
+    """(dose at pos, (i, j, k) of the voxel taken as nearest, its dose).
+    fix_q9 takes the z midpoint from the z bounds, (zb[k] + zb[k + 1]) / 2."""
     d = np.asarray(dose, dtype=np.float32)
     nz, ny, nx = d.shape
     xb, yb, zb = (np.asarray(b, dtype=np.float32) for b in (xb, yb, zb))
@@ -144,7 +145,8 @@ def abs_dose(dose, xb, yb, zb, pos):
         i += 1
     if py > f32(f32(yb[j] + yb[j + 1]) / f32(2)):
         j += 1
-    if pz > f32(f32(zb[k] + xb[k + 1]) / f32(2)):      # the reference's z midpoint uses x_bounds (Q9)
+    zhi = zb[k + 1] if fix_q9 else xb[k + 1]           # the reference's z midpoint uses x_bounds (Q9)
+    if pz > f32(f32(zb[k] + zhi) / f32(2)):
         k += 1
     return float(a0), (i, j, k), float(d[k, j, i])
 

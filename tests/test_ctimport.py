@@ -254,6 +254,11 @@ def test_ct_ramps_file(tmp_path):
                     ("H2O700ICRU", 3000, float(np.float32(0.9)), 2.5)]
     med, dens = phantom.ct_to_medium(np.array([-1000, -950, -900, 0, 3000, 3001]), ramp=ramp, lower=lower)
     assert list(med) == [1, 1, 1, 2, 2, 2]
+    # below the ramp: vacuum as the reference (Q10); with the fix, the first medium at its lowest density
+    med, dens = phantom.ct_to_medium(np.array([-1024, -1001, -1000]), ramp=ramp, lower=lower)
+    assert list(med) == [0, 0, 1] and list(dens[:2]) == [0.0, 0.0]
+    med, dens = phantom.ct_to_medium(np.array([-1024, -1001, -1000]), ramp=ramp, lower=lower, fix_q10=True)
+    assert list(med) == [1, 1, 1] and dens[0] == np.float32(0.001) and dens[2] == np.float32(0.001)
     with open(p, "w") as f:
         f.write("0,0\n")
     assert ctimport.read_ct_ramps(p)[0] == list(phantom.DEFAULT_RAMP)

@@ -104,3 +104,12 @@ def test_gamma_pass_rate_closed_forms():
     # every voxel but the first, whose matching neighbour lies outside the grid
     assert postproc.gamma_pass_rate(ramp, shifted, fine, dose_crit=0.005) == 39 / 40
     assert postproc.gamma_pass_rate(ramp, shifted, fine, dose_crit=0.005, dta=0.01) < 0.5
+
+
+def test_abs_dose_q9_fix_takes_the_z_midpoint():
+    dose, xb, yb, zb, _ = _linear_phantom()
+    # z = 1.6 in voxel 3 ([1.5, 2.0]), below its midpoint 1.75: the voxel itself with the fix
+    _, (i, j, k), vox = postproc.abs_dose(dose, xb, yb, zb, (0.1, 0.1, 1.6), fix_q9=True)
+    assert k == 3 and vox == dose[3, j, i]
+    _, (i, j, k), _ = postproc.abs_dose(dose, xb, yb, zb, (0.1, 0.1, 1.9), fix_q9=True)
+    assert k == 4
