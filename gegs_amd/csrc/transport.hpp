@@ -1343,7 +1343,10 @@ __device__ __forceinline__ bool electron_iter(const Env &env, Stream &s, Part &p
         tustep = fminf(tstep, fminf(tmxs_val, range));
         tperp = hownear_module(env, p);
         p.dnear = tperp;
-        if (mt->i_do_rr[medium] == 1 && p.e < mt->e_max_rr[medium] && tperp >= range) {
+        // range rejection compares the total energy with e_max_rr (quirk Q7, electron.cu:1523);
+        // GEGS_FIX_Q7 compares the kinetic energy
+        const float e_rr = (env.fixes & GEGS_FIX_Q7) ? eke : p.e;
+        if (mt->i_do_rr[medium] == 1 && e_rr < mt->e_max_rr[medium] && tperp >= range) {
             t.score(p.region, to_kinetic(p.e) * p.wt);
             if (lelec > 0) annih_at_rest(env, s, p, rd, sink, t);
             return false;

@@ -148,6 +148,7 @@ def _p(a):
 
 FIX_Q1 = 1                 # GEGS_FIX_Q1: hownear_phantom from the particle's own voxel
 FIX_Q8 = 2                 # GEGS_FIX_Q8: X-jaw step into the opening by the opening's distance
+FIX_Q7 = 4                 # GEGS_FIX_Q7: range rejection on kinetic energy
 
 
 class GegsError(RuntimeError):

@@ -236,7 +236,11 @@ int gegs_load_block_wedge(gegs_ctx *ctx, const gegs_block_wedge *bw);
  *   steps by its distance to the opening, where the reference steps by the
  *   boundary distance of the other plane (geom.cu:992). */
 #define GEGS_FIX_Q8 2u
-#define GEGS_FIX_ALL 3u
+/*   GEGS_FIX_Q7: range rejection (media with i_do_rr) compares the kinetic
+ *   energy with e_max_rr, where the reference compares the total energy
+ *   (electron.cu:1523). */
+#define GEGS_FIX_Q7 4u
+#define GEGS_FIX_ALL 7u
 int gegs_set_fixes(gegs_ctx *ctx, uint32_t mask);
 
 /* Leaf positions of n_sets fields, 120 values per set, in the order of the

@@ -1472,7 +1472,9 @@ static void electron_step(octx *c, ostream *s, opart *p) {
             tustep = fminf(tstep, fminf(tmxs_val, range));
             tperp = hownear(c, p);
             p->dnear = tperp;
-            if (mt->i_do_rr[medium] == 1 && p->e < mt->e_max_rr[medium]) {
+            /* total energy against e_max_rr (Q7); kinetic with GEGS_FIX_Q7 */
+            const float e_rr = (c->fixes & GEGS_FIX_Q7) ? eke : p->e;
+            if (mt->i_do_rr[medium] == 1 && e_rr < mt->e_max_rr[medium]) {
                 if (tperp >= range) {
                     edep1 = (float)((double)p->e - PRM);
                     escore(c, p->region, edep1 * p->wt);

@@ -75,22 +75,23 @@ def test_q1_fix_dose_equivalent_to_reference(ct_small):
 
 
 def test_all_fixes_jaws_mlc_bit_exact():
-    """GEGS_FIX_Q1 | GEGS_FIX_Q8 on the jaws -> MLC -> CT workload: the GPU
+    """GEGS_FIX_Q1 | GEGS_FIX_Q8 | GEGS_FIX_Q7 on the jaws -> MLC -> CT
+    workload (range rejection is on for the MLC medium): the GPU
     reproduces the oracle with the same mask history by history, and the
     event path reproduces the plain kernel."""
     wl = workloads.ct_jaws_mlc()
     src = wl.source(4000, 19)
-    print("fixes Q1|Q8: jaws + MLC workload, 4000 histories, seed 43", flush=True)
+    print("fixes Q1|Q8|Q7: jaws + MLC workload, 4000 histories, seed 43", flush=True)
     ctx = native.Context(0)
     try:
         ctx.load_workload(wl)
         ctx.set_source(src)
-        ctx.set_fixes(native.FIX_Q1 | native.FIX_Q8)
+        ctx.set_fixes(native.FIX_Q1 | native.FIX_Q8 | native.FIX_Q7)
         g = ctx.run(seed=43, n_batches=2, method="plain", per_history=True)
         e = ctx.run(seed=43, n_batches=2, method="event", per_history=True)
     finally:
         ctx.close()
-    oracle.set_fixes(native.FIX_Q1 | native.FIX_Q8)
+    oracle.set_fixes(native.FIX_Q1 | native.FIX_Q8 | native.FIX_Q7)
     try:
         o = oracle.run(wl, src, seed=43, per_history=True, threads=8)
     finally:
