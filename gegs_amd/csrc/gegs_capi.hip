@@ -176,6 +176,16 @@ int gegs_load_media(gegs_ctx *c, const gegs_media_tables *t) {
     return GEGS_OK;
 }
 
+int gegs_validate_regions(const gegs_region *regions, uint64_t n) {
+    if (!regions) return GEGS_ERR_ARG;
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint16_t m = regions[i].med;
+        if (m != GEGS_VACUUM && m >= GEGS_MXMED) return GEGS_ERR_PHANTOM;
+        if (regions[i].flags & ~GEGS_F_SUPPORTED) return GEGS_ERR_ARG;
+    }
+    return GEGS_OK;
+}
+
 int gegs_load_phantom(gegs_ctx *c, uint32_t nx, uint32_t ny, uint32_t nz, const float *xb, const float *yb,
                       const float *zb, const gegs_region *regions) {
     if (!c || !xb || !yb || !zb || !regions) return fail(c, GEGS_ERR_ARG, "gegs_load_phantom: null argument");
@@ -184,11 +194,10 @@ int gegs_load_phantom(gegs_ctx *c, uint32_t nx, uint32_t ny, uint32_t nz, const 
         return fail(c, GEGS_ERR_PHANTOM, "gegs_load_phantom: more than 4096 voxels along an axis");
     const size_t nvox = (size_t)nx * ny * nz;
     if (nvox + VAC_AND_OTH > 0xFFFFFFFFull) return fail(c, GEGS_ERR_PHANTOM, "gegs_load_phantom: too many voxels");
-    // voxel media must exist in the loaded tables
-    for (size_t i = 0; i < nvox + VAC_AND_OTH; ++i) {
-        const uint16_t m = regions[i].med;
-        if (m != GEGS_VACUUM && m >= GEGS_MXMED) return fail(c, GEGS_ERR_PHANTOM, "gegs_load_phantom: region medium out of range");
-    }
+    // voxel media must exist in the loaded tables; region flags must name physics the kernels implement
+    const int vrc = gegs_validate_regions(regions, nvox + VAC_AND_OTH);
+    if (vrc == GEGS_ERR_PHANTOM) return fail(c, vrc, "gegs_load_phantom: region medium out of range");
+    if (vrc) return fail(c, vrc, "gegs_load_phantom: region flags request physics this build does not implement");
     const float *b[3] = {xb, yb, zb};
     const uint32_t n[3] = {nx, ny, nz};
     float **d[3] = {&c->d_xb, &c->d_yb, &c->d_zb};

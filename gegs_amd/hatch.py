@@ -570,11 +570,16 @@ class MediaTables:
     """Device physics tables for an ordered list of media."""
 
     def __init__(self, names, data_dir=DATA_DIR, pegs="pegslite700.pegs4dat", prefix="pegslite",
-                 mlc_medium=None, eii_flag=1):
+                 mlc_medium=None, eii_flag=1, edit=None):
+        """edit: optional callable applied to the list of PegsMedium records
+        read from the PEGS file before any table is derived from them (test
+        media with synthetic stopping powers or cross sections)."""
         if len(names) > MXMED:
             raise ValueError("at most %d media (MXMED)" % MXMED)
         self.names = list(names)
         media = read_pegs_file(os.path.join(data_dir, pegs), self.names)
+        if edit is not None:
+            edit(media)
         ge, ptabs = init_user_photon(media, os.path.join(data_dir, prefix))
         ray, i_arr, pmax = init_rayleigh_data(media, os.path.join(data_dir, prefix + "_ff.data"))
         ums, fms, wms, ims, dllambi, dqmsi = init_ms_sr(os.path.join(data_dir, prefix + "_msnew.data"))

@@ -145,6 +145,11 @@ def abs_dose(dose, xb, yb, zb, pos, fix_q9=False):
         i += 1
     if py > f32(f32(yb[j] + yb[j + 1]) / f32(2)):
         j += 1
+    if not fix_q9 and k + 1 >= len(xb):
+        # the reference reads x_bounds[k + 1] here (output.c:835), past the end of
+        # the x bounds when the point's z cell is beyond nx - 1: undefined there
+        raise ValueError("abs_dose: z cell %d needs x bound %d of %d (reference quirk Q9 reads x_bounds "
+                         "for the z midpoint); pass fix_q9=True" % (k, k + 1, len(xb)))
     zhi = zb[k + 1] if fix_q9 else xb[k + 1]           # the reference's z midpoint uses x_bounds (Q9)
     if pz > f32(f32(zb[k] + zhi) / f32(2)):
         k += 1

@@ -200,6 +200,13 @@ int gegs_load_phantom(gegs_ctx *ctx, uint32_t nx, uint32_t ny, uint32_t nz,
                       const float *x_bounds, const float *y_bounds, const float *z_bounds,
                       const gegs_region *regions);
 
+/* Host-only check of n region records, run by gegs_load_phantom: GEGS_OK, or
+ * GEGS_ERR_PHANTOM for a medium index >= GEGS_MXMED (other than vacuum), or
+ * GEGS_ERR_ARG for a flag bit outside GEGS_F_SUPPORTED.  The reference
+ * (init_regions, init.cu:909-1051) sets the flags itself and checks nothing;
+ * a caller may validate a region table without a device. */
+int gegs_validate_regions(const gegs_region *regions, uint64_t n);
+
 /* Field geometry (gantry symbols, CUDA_EGS.cu:430-432).  A field whose
  * module_first is GEGS_M_MLC starts its particles in the MLC, which needs
  * gegs_load_mlc and a leaf-position row (row 0 for gegs_run_plain/_event);

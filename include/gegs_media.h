@@ -117,5 +117,9 @@ typedef struct gegs_region {
 #define GEGS_F_ATOMIC_RELAXATION 0x0010u
 #define GEGS_F_PHOTOELECTRON_ANGLE 0x0020u
 #define GEGS_F_RANGE_REJECTION 0x0040u
+/* Flags the kernels act on; gegs_load_phantom rejects a region carrying any
+ * other bit (bound Compton and atomic relaxation are not built yet, so a
+ * region asking for them is an error rather than a silent Klein-Nishina). */
+#define GEGS_F_SUPPORTED (GEGS_F_RAYLEIGH | GEGS_F_PHOTOELECTRON_ANGLE | GEGS_F_RANGE_REJECTION)
 
 #endif

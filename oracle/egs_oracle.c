@@ -175,6 +175,7 @@ typedef struct octx {
     uint32_t cur_steps;
     double escaped;                  /* energy leaving the phantom or scored outside it */
     double dropped;                  /* energy of particles the transport abandons */
+    double path;                     /* summed electron path length (tvstep) of the handler call */
     opart stack[STACK_MAX];
     int sp;
     gegs_counters cnt;
@@ -1578,6 +1579,7 @@ static void electron_step(octx *c, ostream *s, opart *p) {
             if (!callmsdist) de = compute_eloss(mt, qel, medium, range, rhof, tvstep, eke, elke, lelke);
         }
         float save_de = de;
+        c->path += (double)tvstep;
         float edep = de;
         float ekef = eke - de;
         if (!callmsdist) {
@@ -2195,6 +2197,8 @@ int oracle_run(const gegs_media_tables *mt, const gegs_region *regions,
  * in->status runs on one particle with the stream a transport run would give
  * it (lineage in->lin, block in->blk); the particles it pushes are returned
  * in push order, and *edep receives the energy it scored. */
+static double g_last_path = 0.0;
+
 int oracle_interaction(const gegs_media_tables *mt, const gegs_region *regions,
                        uint32_t nx, uint32_t ny, uint32_t nz,
                        const float *xb, const float *yb, const float *zb,
@@ -2244,9 +2248,14 @@ int oracle_interaction(const gegs_media_tables *mt, const gegs_region *regions,
     double tot = 0.0;
     for (size_t v = 0; v < nvox; ++v) tot += dose[v];
     *edep_out = tot;
+    g_last_path = c->path;
     free(c->tmp); free(c->last); free(dose); free(dose2); free(c);
     return n;
 }
+
+/* electron path length (sum of the condensed-history steps' tvstep) of the
+ * last oracle_interaction call; for the CSDA range checks (single-threaded use) */
+double oracle_last_path(void) { return g_last_path; }
 
 /* elementary functions, exported for their accuracy tests */
 /* mscat probe: n draws of the multiple-scattering angle for fixed (lambda,
@@ -2263,6 +2272,54 @@ void oracle_mscat(const gegs_media_tables *mt, float lambda, float chia2, float 
         mscat(mt, &s, lambda, chia2, q1, &c, &sn);
         cost[i] = c;
     }
+}
+
+/* msdist_pI probe: n draws of one PRESTA-I step (path tustep, energy loss
+ * eloss) of an electron of total energy e starting at the origin with
+ * direction (u, v, w) in a region of `medium` at relative density rhof, stream
+ * i keyed by lineage lin0 + i; out[7 i ...] = displacement x, y, z, final
+ * direction u, v, w and the returned straight-line distance ustep */
+void oracle_msdist(const gegs_media_tables *mt, int medium, float rhof, float e, float eloss, float tustep,
+                   const float *dir, uint64_t seed, uint64_t lin0, uint32_t n, float *out) {
+    gegs_region rd;
+    memset(&rd, 0, sizeof(rd));
+    rd.med = (uint16_t)medium;
+    rd.rhof = rhof;
+    for (uint32_t i = 0; i < n; ++i) {
+        opart p;
+        memset(&p, 0, sizeof(p));
+        p.e = e;
+        p.u = dir[0]; p.v = dir[1]; p.w = dir[2];
+        p.lin = lin0 + i;
+        ostream s;
+        st_begin(&s, &p, seed);
+        float us = msdist_pI(mt, &s, eloss, tustep, &p, &rd);
+        float *o = out + 7 * (size_t)i;
+        o[0] = p.x; o[1] = p.y; o[2] = p.z; o[3] = p.u; o[4] = p.v; o[5] = p.w; o[6] = us;
+    }
+}
+
+/* compute_eloss probe: energy lost over a step of length `step` by an electron
+ * (qel 0) or positron (qel 1) of kinetic energy eke with range `range` */
+float oracle_compute_eloss(const gegs_media_tables *mt, int qel, int medium, float range, float rhof, float step,
+                           float eke) {
+    float elke = om_logf(eke);
+    gegs_float2 ek = mt->eke01[medium];
+    int lelke = (int)(ek.x + ek.y * elke);
+    return compute_eloss(mt, qel, medium, range, rhof, step, eke, elke, lelke);
+}
+
+/* range of an electron of kinetic energy eke, as electr_step forms it
+ * (electron.cu:1414-1432): drange to the bin's lower edge plus range_ep */
+float oracle_range(const gegs_media_tables *mt, int qel, int medium, float rhof, float eke) {
+    float elke = om_logf(eke);
+    gegs_float2 ek = mt->eke01[medium];
+    int lelke = (int)(ek.x + ek.y * elke);
+    const gegs_float2 *dedx = mt->dedx + qel * MXEKE * MXMED + medium * MXEKE;
+    float ekei = mt->e_array[medium * MXEKE + lelke - 1];
+    float elkei = ((float)lelke - ek.x) / ek.y;
+    float range = drange(dedx, eke, elke, ekei, elkei, lelke);
+    return (range + mt->range_ep[(medium * MXEKE + lelke - 1) * 2 + qel]) / rhof;
 }
 
 float oracle_logf(float x) { return om_logf(x); }

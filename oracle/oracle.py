@@ -62,6 +62,14 @@ def lib():
                                          vp, vp, vp, vp, vp, ctypes.c_uint64, vp, ctypes.c_int, vp]
         L.oracle_mscat.argtypes = [vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_uint64,
                                    ctypes.c_uint64, ctypes.c_uint32, vp]
+        L.oracle_last_path.restype = ctypes.c_double
+        L.oracle_msdist.argtypes = [vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                    ctypes.c_float, vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, vp]
+        L.oracle_compute_eloss.restype = ctypes.c_float
+        L.oracle_compute_eloss.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                           ctypes.c_float, ctypes.c_float]
+        L.oracle_range.restype = ctypes.c_float
+        L.oracle_range.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float]
         L.oracle_logf.restype = ctypes.c_float
         L.oracle_logf.argtypes = [ctypes.c_float]
         L.oracle_expf.restype = ctypes.c_float
@@ -208,6 +216,28 @@ def interaction(workload, particle, seed, max_out=64):
                              _p(xb), _p(yb), _p(zb), ctypes.c_void_p(ctypes.addressof(workload.field)),
                              _p(inp), seed, _p(out), max_out, _p(ed))
     return out[:k].copy(), float(ed[0])
+
+
+def last_path():
+    """Electron path length (sum of step lengths) of the last interaction() call."""
+    return float(lib().oracle_last_path())
+
+
+def msdist(tables, medium, rhof, e, eloss, tustep, direction, n, seed=1, lin0=1):
+    """n PRESTA-I steps (msdist_pI) from the origin: array n x 7 of displacement
+    (x, y, z), final direction (u, v, w) and the returned straight-line distance."""
+    out = np.zeros((n, 7), dtype=np.float32)
+    d = np.ascontiguousarray(direction, dtype=np.float32)
+    lib().oracle_msdist(ctypes.c_void_p(tables.ptr()), medium, rhof, e, eloss, tustep, _p(d), seed, lin0, n, _p(out))
+    return out
+
+
+def compute_eloss(tables, qel, medium, range_, rhof, step, eke):
+    return float(lib().oracle_compute_eloss(ctypes.c_void_p(tables.ptr()), qel, medium, range_, rhof, step, eke))
+
+
+def electron_range(tables, qel, medium, rhof, eke):
+    return float(lib().oracle_range(ctypes.c_void_p(tables.ptr()), qel, medium, rhof, eke))
 
 
 def to_phantom(field, pos_dir):

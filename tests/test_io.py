@@ -104,3 +104,29 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(L, n), n
     L.gegs_sizeof_media_tables.restype = ctypes.c_size_t
     assert L.gegs_sizeof_media_tables() == ctypes.sizeof(hatch.CMediaTables)
+
+
+def test_validate_regions_rejects_unbuilt_physics_flags():
+    """A region asking for physics the kernels do not act on is an error, not a
+    silent fallback (the reference turns bound Compton and relaxation on for
+    every region, init.cu:935-1000)."""
+    lib = os.path.join(ROOT, "gegs_amd", "libgegs_hip.so")
+    if not os.path.exists(lib):
+        pytest.skip("libgegs_hip.so not built")
+    L = ctypes.CDLL(lib)
+    L.gegs_validate_regions.argtypes = [ctypes.c_void_p, ctypes.c_uint64]
+    reg = np.zeros(10, dtype=ph.REGION_DTYPE)
+    reg["med"] = 0
+    reg["flags"] = ph.DEFAULT_FLAGS
+    assert L.gegs_validate_regions(reg.ctypes.data, len(reg)) == 0
+    supported = ph.F_RAYLEIGH | ph.F_PHOTOELECTRON_ANGLE | 0x0040
+    for bit in range(16):
+        r = reg.copy()
+        r["flags"][3] = 1 << bit
+        want = 0 if (1 << bit) & supported else 2001
+        assert L.gegs_validate_regions(r.ctypes.data, len(r)) == want, hex(1 << bit)
+    r = reg.copy()
+    r["med"][4] = 6
+    assert L.gegs_validate_regions(r.ctypes.data, len(r)) == 8001
+    r["med"][4] = ph.VACUUM
+    assert L.gegs_validate_regions(r.ctypes.data, len(r)) == 0

@@ -113,3 +113,16 @@ def test_abs_dose_q9_fix_takes_the_z_midpoint():
     assert k == 3 and vox == dose[3, j, i]
     _, (i, j, k), _ = postproc.abs_dose(dose, xb, yb, zb, (0.1, 0.1, 1.9), fix_q9=True)
     assert k == 4
+
+
+def test_abs_dose_q9_past_the_x_bounds_is_a_clear_error():
+    # a 4 x 4 x 10 phantom: the point's z cell (8) needs x_bounds[9] on the
+    # reference's path (output.c:835), which has only 5 x bounds
+    xb = np.linspace(-1.0, 1.0, 5).astype(np.float32)
+    yb = np.linspace(-1.0, 1.0, 5).astype(np.float32)
+    zb = np.linspace(0.0, 5.0, 11).astype(np.float32)
+    dose = np.ones((10, 4, 4), dtype=np.float32)
+    with pytest.raises(ValueError, match="Q9"):
+        postproc.abs_dose(dose, xb, yb, zb, (0.1, 0.1, 4.2))
+    val, (i, j, k), _ = postproc.abs_dose(dose, xb, yb, zb, (0.1, 0.1, 4.2), fix_q9=True)
+    assert k == 8 and abs(val - 1.0) < 1e-6
