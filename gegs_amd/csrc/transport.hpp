@@ -527,7 +527,8 @@ __device__ __forceinline__ void azimuth_box(Stream &s, float &cphi, float &sphi)
     sphi = 2.0f * xp * yp * r2;
 }
 
-__device__ __forceinline__ float to_kinetic(float etot) { return (float)((double)etot - PRM); }
+// the reference's rest mass prm is a float symbol (commonBlocks dev.h:220): float arithmetic
+__device__ __forceinline__ float to_kinetic(float etot) { return etot - RM; }
 
 // ------------------------------------------------------------------ annihilation at rest
 template <class Sink, class T>
@@ -535,7 +536,7 @@ __device__ __forceinline__ void annih_at_rest(const Env &env, Stream &s, Part &p
     const int nsplit = env.mt->nbr_split;
     float edep = 0.0f;
     if (nsplit > 1) p.wt = p.wt / (float)nsplit;
-    p.e = (float)PRM;
+    p.e = RM;
     p.charge = 0;
     for (int i = 1; i <= nsplit; ++i) {
         float costhe = s.next();
@@ -763,125 +764,487 @@ __device__ __forceinline__ void photon_step(const Env &env, Stream &s, Part &p, 
     }
 }
 
-// ------------------------------------------------------------------ Compton (Klein-Nishina)
+// ------------------------------------------------------------------ atomic relaxation (relax.cu:14-238)
+constexpr int MXSHELL = GEGS_MXSHELL;
+constexpr int MXTRANS = GEGS_MXTRANS;
+constexpr int MXEDGE = GEGS_MXEDGE;
+constexpr int MXMDSH = GEGS_MXMDSH;
+constexpr int MXVAC = GEGS_MXVAC;
+constexpr int N_EII_BINS = GEGS_N_EII_BINS;
+constexpr double RELAX_CUTOFF = 0.001;
+
+// transitions of each shell (commonBlocks dev.h:260-262): [first, last]
+// indices into the element's probabilities, and the final state of each:
+// < 10 fluorescence into that shell, 1x Coster-Kronig into shell x, ab Auger
+// leaving vacancies in a and b
+__device__ __forceinline__ int first_transition(int shell) {
+    return shell == 1 ? 1 : (shell == 2 ? 20 : (shell == 3 ? 27 : (shell == 4 ? 33 : 38)));
+}
+__device__ __forceinline__ int last_transition(int shell) {
+    return shell == 1 ? 19 : (shell == 2 ? 26 : (shell == 3 ? 32 : (shell == 4 ? 37 : 39)));
+}
+__device__ __forceinline__ int final_state(int k) {
+    // two entries per 32-bit word of a wavefront-uniform table
+    constexpr uint32_t FS[20] = {
+        4u | 3u << 16, 5u | 6u << 16, 202u | 302u << 16, 402u | 404u << 16, 403u | 303u << 16,
+        502u | 503u << 16, 504u | 602u << 16, 603u | 604u << 16, 505u | 605u << 16, 606u | 13u << 16,
+        14u | 5u << 16, 6u | 505u << 16, 605u | 606u << 16, 14u | 5u << 16, 6u | 505u << 16,
+        605u | 606u << 16, 5u | 6u << 16, 505u | 605u << 16, 606u | 6u << 16, 606u};
+    const uint32_t w = FS[(k - 1) >> 1];
+    return (int)(((k - 1) & 1) ? (w >> 16) : (w & 0xffffu));
+}
+
+// isotropic direction of a relaxation particle (relax.cu:62-90)
+__device__ __forceinline__ void relax_direction(Stream &s, Part &p) {
+    float eta = s.next();
+    eta = 2 * eta - 1;
+    p.w = eta;
+    eta = (1 - eta) * (1 + eta);
+    if (eta > 1e-20f) {
+        eta = sqrtf(eta);
+        float xphi, yphi, rhophi2;
+        do {
+            xphi = s.next();
+            xphi = 2 * xphi - 1;
+            yphi = s.next();
+            rhophi2 = xphi * xphi + yphi * yphi;
+        } while (rhophi2 > 1);
+        rhophi2 = 1 / rhophi2;
+        p.u = eta * (xphi * xphi - yphi * yphi) * rhophi2;
+        p.v = eta * 2 * xphi * yphi * rhophi2;
+    } else {
+        p.u = 0.0f;
+        p.v = 0.0f;
+        p.w = 1.0f;
+    }
+}
+
+// fill a vacancy in shell n (1 K .. 6 N) of element iz carrying `energy`:
+// fluorescence photons and Auger / Coster-Kronig electrons above the cuts are
+// created, the rest goes to edep; a cascade deeper than MXVAC vacancies (an
+// out-of-bounds write in the reference) deposits the vacancy locally
+template <class Sink>
+__device__ __forceinline__ void relax(const Env &env, Stream &s, float energy, int n, int iz, float &edep, Part p,
+                                      const gegs_region &rd, Sink &sink) {
+    const gegs_media_tables *mt = env.mt;
+    if (n < 1 || n > MXSHELL) return;
+    float e_cut = fminf(rd.ecut - RM, rd.pcut);
+    e_cut = ((double)e_cut > RELAX_CUTOFF) ? e_cut : (float)RELAX_CUTOFF;
+    if (energy <= e_cut) {
+        edep += energy;
+        return;
+    }
+    // vacancy stack, shells and energies, in registers: 4 bits per shell
+    uint32_t vac = (uint32_t)n;
+    float ea[MXVAC];
+    int nvac = 1;
+    ea[0] = energy;
+    const float *be = mt->binding_energies + MXSHELL * (iz - 1);
+    for (;;) {
+        const int shell = (int)((vac >> (4 * (nvac - 1))) & 15u);
+        const float ei = ea[nvac - 1];
+        nvac -= 1;
+        if (ei <= e_cut) {
+            edep += ei;
+            if (nvac > 0) continue;
+            break;
+        }
+        if (shell == MXSHELL) {
+            p.e = ei + RM;
+            p.charge = -1;
+            relax_direction(s, p);
+            if (p.e < rd.ecut) edep += p.e - RM;
+            else create(sink, s, p, E_ELECTRON_STEP);
+            if (nvac > 0) continue;
+            break;
+        }
+        float eta = s.next();
+        int k;
+        const int klast = last_transition(shell) - 1;
+        for (k = first_transition(shell); k <= klast; k++) {
+            eta = eta - mt->relaxation_prob[MXTRANS * (iz - 1) + k - 1];
+            if (eta <= 0.0f) break;
+        }
+        int fin = final_state(k);
+        int iql;
+        float elcut, ex;
+        if (nvac + (fin < 100 ? 1 : 2) > MXVAC) {
+            edep += ei;
+            if (nvac > 0) continue;
+            break;
+        }
+        if (fin < 100) {
+            if (fin < 10) {
+                iql = 0;
+                elcut = rd.pcut;
+            } else {
+                fin = fin - 10;
+                iql = -1;
+                elcut = rd.ecut - RM;
+            }
+            const float ef = be[fin - 1];
+            ex = ei - ef;
+            nvac += 1;
+            vac = (vac & ~(15u << (4 * (nvac - 1)))) | ((uint32_t)fin << (4 * (nvac - 1)));
+            ea[nvac - 1] = ef;
+        } else {
+            const int f1 = fin / 100;
+            const int f2 = fin - f1 * 100;
+            nvac += 1;
+            vac = (vac & ~(15u << (4 * (nvac - 1)))) | ((uint32_t)f1 << (4 * (nvac - 1)));
+            ea[nvac - 1] = be[f1 - 1];
+            nvac += 1;
+            vac = (vac & ~(15u << (4 * (nvac - 1)))) | ((uint32_t)f2 << (4 * (nvac - 1)));
+            ea[nvac - 1] = be[f2 - 1];
+            iql = -1;
+            ex = ei - ea[nvac - 1] - ea[nvac - 2];
+            elcut = rd.ecut - RM;
+        }
+        if (ex <= elcut) {
+            edep += ex;
+        } else {
+            p.charge = iql;
+            p.e = iql == 0 ? ex : ex + RM;
+            relax_direction(s, p);
+            if (iql == 0) {
+                if (p.e < rd.pcut) edep += p.e;
+                else create(sink, s, p, P_PHOTON_STEP);
+            } else {
+                if (p.e < rd.ecut) edep += p.e - RM;
+                else create(sink, s, p, E_ELECTRON_STEP);
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ Compton (compton.cu:17-492)
+// ibcmpl from the region flags (compton.cu:87-95)
+__device__ __forceinline__ int compton_mode(uint32_t flags) {
+    const uint32_t f = flags & 0x000Eu;
+    return f == 0x0002u ? 1 : (f == 0x0006u ? 2 : (f == 0x000Au ? 3 : 0));
+}
+
 template <class Sink, class T>
 __device__ __forceinline__ void compton(const Env &env, Stream &s, Part &p, Sink &sink, T &t) {
+    const gegs_media_tables *mt = env.mt;
     const gegs_region rd = region_rec(env, p.region);
     float edep = 0.0f;
     Part e1 = p;
+    const int medium = rd.med;
     const float peig = p.e;
     const float ko = peig / RM;
-    const float broi = 1.0f + 2.0f * ko;
-    float bro = 0.0f, br = 0.0f, temp = 0.0f, sinthe = 0.0f;
-    float alph1 = 0.0f, alpha = 0.0f, rejmax = 0.0f;
+    const float broi = 1 + 2 * ko;
+    const int ibcmpl = compton_mode(rd.flags);
     bool first = true;
-    do {
-        if (ko > 2.0f) {
-            if (first) {
-                alph1 = glog(broi);
-                bro = 1.0f / broi;
-                alpha = alph1 + ko * (broi + 1.0f) * bro * bro;
+    // declared inside the resample loop in the reference and read again on a
+    // resample without being set: kept from the previous pass
+    float bro = 0.0f, br = 0.0f, temp = 0.0f, sinthe = 0.0f, costhe = 0.0f;
+    float alph1 = 0.0f, alpha = 0.0f, rejmax = 0.0f;
+    for (int loops = 0;; loops++) {
+        if (loops > 100) break;
+        float uj = 0.0f, jo = 0.0f;
+        int j = 0;
+        if (ibcmpl > 0) {
+            float rnno17 = s.next();
+            rnno17 = 1 + rnno17 * mt->n_shell[medium];
+            int i = (int)rnno17;
+            if (rnno17 > mt->eno_array[medium * MXMDSH + i - 1]) i = mt->eno_atbin_array[medium * MXMDSH + i - 1];
+            j = mt->shell_array[medium * MXMDSH + i - 1];
+            uj = mt->be_array[j - 1];
+            if (ko <= uj) {
+                if (ibcmpl == 1) { forward(sink, s, p, P_PHOTON_STEP); break; }
+                continue;
             }
-            float aux, rejf3, r19;
-            int n = 0;
-            do {
-                const float r15 = s.next();
-                const float r16 = s.next();
-                if (r15 * alpha < alph1) br = gexp(alph1 * r16) * bro;
-                else br = sqrtf(r16 * broi * broi + (1.0f - r16)) * bro;
-                temp = (1.0f - br) / (ko * br);
-                sinthe = fmaxf(0.0f, temp * (2.0f - temp));
-                aux = 1.0f + br * br;
-                rejf3 = aux - br * sinthe;
-                r19 = s.next();
-                if (++n > 100) break;
-            } while (r19 * aux > rejf3);
-        } else {
-            if (first) {
-                bro = 1.0f / broi;
-                rejmax = broi + bro;
-            }
-            float r16, rejf3;
-            int n = 0;
-            do {
-                const float r15 = s.next();
-                r16 = s.next();
-                br = bro + (1.0f - bro) * r15;
-                temp = (1.0f - br) / (ko * br);
-                sinthe = fmaxf(0.0f, temp * (2.0f - temp));
-                rejf3 = 1.0f + br * br - br * sinthe;
-                if (n++ > 100) break;
-            } while (r16 * br * rejmax > rejf3);
+            jo = mt->jo_array[j - 1];
         }
-        first = false;
-    } while (br < bro || br > 1.0f);
-    float costhe = 1.0f - temp;
-    const float pesg = br * peig;
-    const float pese = (float)((double)(peig - pesg) + PRM);
-    sinthe = sqrtf(sinthe);
-    p.e = pesg;
-    float sinphi, cosphi;
-    uphi21(s, costhe, sinthe, cosphi, sinphi, p);
-    if (p.e < rd.pcut) edep += p.e; else create(sink, s, p, P_PHOTON_STEP);
-    const float aux = 1.0f + br * br - 2.0f * br * costhe;
-    if (aux > 1e-8f) {
-        costhe = (1.0f - br * costhe) / sqrtf(aux);
-        sinthe = (1.0f - costhe) * (1.0f + costhe);
-        sinthe = sinthe > 0.0f ? -sqrtf(sinthe) : 0.0f;
-    } else {
-        costhe = 0.0f;
-        sinthe = -1.0f;
+        do {
+            if (ko > 2) {
+                if (first) {
+                    alph1 = glog(broi);
+                    bro = 1 / broi;
+                    alpha = alph1 + ko * (broi + 1) * bro * bro;
+                }
+                float aux, rejf3, r19;
+                int n = 0;
+                do {
+                    const float r15 = s.next();
+                    const float r16 = s.next();
+                    if ((r15 * alpha) < alph1) br = gexp(alph1 * r16) * bro;
+                    else br = sqrtf(r16 * broi * broi + (1 - r16)) * bro;
+                    temp = (1 - br) / (ko * br);
+                    sinthe = fmaxf(0.0f, temp * (2 - temp));
+                    aux = 1 + br * br;
+                    rejf3 = aux - br * sinthe;
+                    r19 = s.next();
+                    if (++n > 100) break;
+                } while (r19 * aux > rejf3);
+            } else {
+                if (first) {
+                    bro = (float)(1.0 / (double)broi);
+                    rejmax = broi + bro;
+                }
+                float r16, rejf3;
+                int n = 0;
+                do {
+                    const float r15 = s.next();
+                    r16 = s.next();
+                    br = bro + (1 - bro) * r15;
+                    temp = (1 - br) / (ko * br);
+                    sinthe = fmaxf(0.0f, temp * (2 - temp));
+                    rejf3 = 1 + br * br - br * sinthe;
+                    if (n++ > 100) break;
+                } while (r16 * br * rejmax > rejf3);
+            }
+            first = false;
+        } while (br < bro || br > 1);
+        costhe = 1 - temp;
+        if (ibcmpl == 0) {
+            uj = 0;
+        } else {
+            // p_z range rejection and sampling (compton.cu:228-405)
+            float aux = ko * (ko - uj) * temp;
+            float aux1 = 2 * aux + uj * uj;
+            float pzmax = aux - uj;
+            if ((pzmax < 0) && (pzmax * pzmax >= aux1)) {
+                if (ibcmpl == 1) { forward(sink, s, p, P_PHOTON_STEP); break; }
+                continue;
+            }
+            pzmax = pzmax / sqrtf(aux1);
+            const float qc2 = 1 + br * br - 2 * br * costhe;
+            const float qc = sqrtf(qc2);
+            float af, fmax1, fpz;
+            bool rejected = false;
+            if (pzmax > 1) {
+                pzmax = 1;
+                af = 0;
+                fmax1 = 1;
+                fpz = 1;
+            } else {
+                float aux3 = 1 + 2 * jo * fabsf(pzmax);
+                float aux4 = (float)(0.5 * (double)(1 - aux3 * aux3));
+                fpz = (float)(0.5 * (double)gexp(aux4));
+                af = qc * (1 + br * (br - costhe) / qc2);
+                if (af < 0) {
+                    if (pzmax > 0) fpz = 1.0f - fpz;
+                    const float eta = s.next();
+                    if (eta > fpz) rejected = true;
+                    af = 0;
+                    fmax1 = 1;
+                } else {
+                    float fpz1;
+                    if ((double)pzmax < -0.15) {
+                        fmax1 = (float)(1.0 - (double)af * 0.15);
+                        fpz1 = fpz * fmax1 * jo;
+                    } else if ((double)pzmax < 0.15) {
+                        fmax1 = 1 + af * pzmax;
+                        aux3 = (float)(1.0 / (1.0 + 0.33267252734 * (double)aux3));
+                        aux4 = (float)((double)(fpz * aux3) *
+                                       (0.3480242 + (double)aux3 * (-0.0958798 + (double)aux3 * 0.7478556)) +
+                                       (double)mt->erfjo_array[j - 1]);
+                        if (pzmax > 0) {
+                            fpz1 = (float)((double)((1 - fmax1 * fpz) * jo) - 0.62665706866 * (double)af * (double)aux4);
+                            fpz = 1 - fpz;
+                        } else {
+                            fpz1 = (float)((double)(fmax1 * fpz * jo) - 0.62665706866 * (double)af * (double)aux4);
+                        }
+                    } else {
+                        fmax1 = (float)(1.0 + (double)af * 0.15);
+                        fpz1 = (1 - fmax1 * fpz) * jo;
+                        fpz = 1 - fpz;
+                    }
+                    const float eta = s.next();
+                    if (eta * jo > fpz1) rejected = true;
+                }
+            }
+            if (rejected) {
+                if (ibcmpl == 1) { forward(sink, s, p, P_PHOTON_STEP); break; }
+                continue;
+            }
+            float pz = 0.0f;
+            for (int n = 0;; n++) {
+                if (n > 100) break;
+                if (ibcmpl != 2) {
+                    float r18 = s.next();
+                    r18 = r18 * fpz;
+                    if (r18 < 0.5f) {
+                        r18 = (double)(2 * r18) > 1e-30 ? 2 * r18 : (float)1e-30;
+                        pz = (float)(0.5 * (double)(1 - sqrtf(1 - 2 * glog(r18))) / (double)jo);
+                    } else {
+                        r18 = 2 * (1 - r18);
+                        pz = (float)(0.5 * (double)(sqrtf(1 - 2 * glog(r18)) - 1) / (double)jo);
+                    }
+                    if (fabsf(pz) > 1) continue;
+                    if ((double)pz < 0.15) {
+                        const float frej = (double)pz < -0.15 ? (float)(1.0 - (double)af * 0.15) : 1 + af * pz;
+                        const float eta = s.next();
+                        if (eta * fmax1 > frej) continue;
+                    }
+                } else {
+                    pz = 0;
+                    uj = 0;
+                }
+                break;
+            }
+            if ((double)fabsf(pz) < 0.01) {
+                br = br * (1 + pz * (qc + (br * br - costhe) * pz));
+            } else {
+                aux = 1 - pz * pz * br * costhe;
+                aux1 = 1 - pz * pz * br * br;
+                const float aux2 = qc2 - br * br * pz * pz * sinthe;
+                if ((double)aux2 > 1e-10) br = br / aux1 * (aux + pz * sqrtf(aux2));
+            }
+            uj = uj * RM;
+        }
+        const float pesg = br * peig;
+        const float pese = peig - pesg - uj + RM;
+        sinthe = sqrtf(sinthe);
+        p.e = pesg;
+        float sinphi, cosphi;
+        uphi21(s, costhe, sinthe, cosphi, sinphi, p);
+        if (p.e < rd.pcut) edep += p.e; else create(sink, s, p, P_PHOTON_STEP);
+        const float aux = 1 + br * br - 2 * br * costhe;
+        if ((double)aux > 1e-8) {
+            costhe = (1 - br * costhe) / sqrtf(aux);
+            sinthe = (1 - costhe) * (1 + costhe);
+            sinthe = sinthe > 0 ? -sqrtf(sinthe) : 0.0f;
+        } else {
+            costhe = 0.0f;
+            sinthe = -1.0f;
+        }
+        e1.e = pese;
+        e1.charge = -1;
+        turn(costhe, sinthe, cosphi, sinphi, e1);
+        if (e1.e < rd.ecut) edep += e1.e - RM; else create(sink, s, e1, E_ELECTRON_STEP);
+        if (ibcmpl == 1 || ibcmpl == 3) {
+            if ((double)uj > RELAX_CUTOFF) relax(env, s, uj, mt->shn_array[j - 1], mt->iz_array[j - 1], edep, p, rd, sink);
+            else edep += uj;
+        }
+        break;
     }
-    e1.e = pese;
-    e1.charge = -1;
-    turn(costhe, sinthe, cosphi, sinphi, e1);
-    if (e1.e < rd.ecut) edep += to_kinetic(e1.e); else create(sink, s, e1, E_ELECTRON_STEP);
     t.score(p.region, edep * p.wt);
 }
 
-// ------------------------------------------------------------------ photo-electric
+// ------------------------------------------------------------------ photo-electric (photo.cu:1-279)
+__device__ __forceinline__ void sauter_angle(Stream &s, Part &p) {
+    // Sauter distribution (photo.cu:197-244)
+    const float beta = sqrtf((p.e - RM) * (p.e + RM)) / p.e;
+    const float gamma = p.e / RM;
+    const float alpha = 0.5f * gamma - 0.5f + 1.0f / gamma;
+    const float ratio = beta / alpha;
+    float xi, r2, sinth2, costhe;
+    do {
+        float rn = s.next();
+        rn = 2.0f * rn - 1.0f;
+        if (ratio <= 0.2f) {
+            const float fk = rn + 0.5f * ratio * (1.0f - rn) * (1.0f + rn);
+            if (gamma < 100.0f) {
+                costhe = (beta + fk) / (1.0f + beta * fk);
+            } else if (fk > 0.0f) {
+                costhe = 1.0f - (1.0f - fk) * (gamma - 3.0f) /
+                         (2.0f * (1.0f + fk) * (gamma - 1.0f) * (gamma - 1.0f) * (gamma - 1.0f));
+            } else {
+                costhe = (beta + fk) / (1.0f + beta * fk);
+            }
+            xi = (1.0f + beta * fk) * gamma * gamma;
+        } else {
+            xi = gamma * gamma * (1.0f + alpha * (sqrtf(1.0f + ratio * (2.0f * rn + ratio)) - 1.0f));
+            costhe = (1.0f - 1.0f / xi) / beta;
+        }
+        sinth2 = fmaxf(0.0f, (1.0f - costhe) * (1.0f + costhe));
+        r2 = s.next();
+    } while (r2 > 0.5f * (1.0f + gamma) * sinth2 * xi / gamma);
+    float sinphi, cosphi;
+    uphi21(s, costhe, sqrtf(sinth2), cosphi, sinphi, p);
+}
+
 template <class Sink, class T>
 __device__ __forceinline__ void photo(const Env &env, Stream &s, Part &p, Sink &sink, T &t) {
+    const gegs_media_tables *mt = env.mt;
     const gegs_region rd = region_rec(env, p.region);
     float edep = 0.0f;
-    p.e = (float)((double)p.e + PRM);
-    p.charge = -1;
-    if (p.e > rd.ecut) {
-        if (rd.flags & GEGS_F_PHOTOELECTRON_ANGLE) {
-            // Sauter distribution (photo.cu:197-244)
-            const float beta = sqrtf((p.e - RM) * (p.e + RM)) / p.e;
-            const float gamma = p.e / RM;
-            const float alpha = 0.5f * gamma - 0.5f + 1.0f / gamma;
-            const float ratio = beta / alpha;
-            float xi, r2, sinth2, costhe;
-            do {
-                float rn = s.next();
-                rn = 2.0f * rn - 1.0f;
-                if (ratio <= 0.2f) {
-                    const float fk = rn + 0.5f * ratio * (1.0f - rn) * (1.0f + rn);
-                    if (gamma < 100.0f) {
-                        costhe = (beta + fk) / (1.0f + beta * fk);
-                    } else if (fk > 0.0f) {
-                        costhe = 1.0f - (1.0f - fk) * (gamma - 3.0f) /
-                                 (2.0f * (1.0f + fk) * (gamma - 1.0f) * (gamma - 1.0f) * (gamma - 1.0f));
-                    } else {
-                        costhe = (beta + fk) / (1.0f + beta * fk);
-                    }
-                    xi = (1.0f + beta * fk) * gamma * gamma;
-                } else {
-                    xi = gamma * gamma * (1.0f + alpha * (sqrtf(1.0f + ratio * (2.0f * rn + ratio)) - 1.0f));
-                    costhe = (1.0f - 1.0f / xi) / beta;
-                }
-                sinth2 = fmaxf(0.0f, (1.0f - costhe) * (1.0f + costhe));
-                r2 = s.next();
-            } while (r2 > 0.5f * (1.0f + gamma) * sinth2 * xi / gamma);
-            float sinphi, cosphi;
-            uphi21(s, costhe, sqrtf(sinth2), cosphi, sinphi, p);
-        }
-        create(sink, s, p, E_ELECTRON_STEP);
-    } else {
-        edep += to_kinetic(p.e);
+    const int medium = rd.med;
+    const float peig = p.e;
+    if (peig < mt->edge_energies[1]) {   // quirk Q6: the second edge of hydrogen
+        p.charge = -1;
+        p.e = peig + RM;
+        if (p.e < rd.ecut) t.score(p.region, (p.e - RM) * p.wt);
+        else create(sink, s, p, E_ELECTRON_STEP);
+        return;
     }
+    const float gle = glog(p.e);
+    bool do_relax = false;
+    float e_vac = 0.0f;
+    int iz = 0, k = 0;
+    if (rd.flags & GEGS_F_ATOMIC_RELAXATION) {
+        // the element (photo.cu:87-154), then the shell (photo.cu:160-183)
+        const int nne = mt->nne[medium];
+        if (nne == 1) {
+            iz = (int)((double)mt->zelem[medium] + 0.5);
+        } else {
+            float probs[GEGS_MXEL];
+            float aux = peig * peig;
+            const float aux1 = aux * peig;
+            aux = aux * sqrtf(peig);
+            float sigtot = 0;
+            for (k = 0; k < nne; k++) {
+                const int z = (int)((double)mt->zelem[MXMED * k + medium] + 0.5);
+                const int e0 = MXEDGE * (z - 1);
+                float sigma;
+                if (peig > mt->edge_energies[e0]) {
+                    sigma = (mt->edge_a[e0] + mt->edge_b[e0] / peig + mt->edge_c[e0] / aux + mt->edge_d[e0] / aux1) / peig;
+                } else {
+                    int j;
+                    const int ne = mt->edge_number[z - 1];
+                    for (j = 2; j <= ne; j++)
+                        if (peig >= mt->edge_energies[e0 + j - 1]) break;
+                    sigma = mt->edge_a[e0 + j - 1] +
+                            gle * (mt->edge_b[e0 + j - 1] + gle * (mt->edge_c[e0 + j - 1] + gle * mt->edge_d[e0 + j - 1]));
+                    sigma = gexp(sigma);
+                }
+                sigma = sigma * mt->pz[MXMED * k + medium];
+                sigtot = sigtot + sigma;
+                probs[k] = sigma;
+            }
+            float br = s.next();
+            br = br * sigtot;
+            for (k = 0; k < nne; k++) {
+                br = br - probs[k];
+                if (br <= 0) break;
+            }
+            // past the last element only by rounding (an out-of-range read in the reference)
+            if (k == nne) k = nne - 1;
+            iz = (int)((double)mt->zelem[MXMED * k + medium] + 0.5);
+        }
+        const float *be = mt->binding_energies + MXSHELL * (iz - 1);
+        if (peig <= be[5]) {
+            edep += peig;
+        } else {
+            float br = s.next();
+            for (k = 1; k <= 5; k++) {
+                if (peig > be[k - 1]) {
+                    const float tmp = mt->interaction_prob[MXSHELL * (iz - 1) + k - 1];
+                    if (br < tmp) break;
+                    br = (br - tmp) / (1 - tmp);
+                }
+            }
+            e_vac = be[k - 1];
+            p.e = peig - e_vac + RM;
+            do_relax = true;
+            p.charge = -1;
+        }
+    } else {
+        p.e = peig + RM;
+        p.charge = -1;
+    }
+    if (p.charge == -1) {
+        if (p.e > rd.ecut) {
+            if (rd.flags & GEGS_F_PHOTOELECTRON_ANGLE) sauter_angle(s, p);
+            create(sink, s, p, E_ELECTRON_STEP);
+        } else {
+            edep += p.e - RM;
+        }
+    }
+    if (do_relax) relax(env, s, e_vac, k, iz, edep, p, rd, sink);
     t.score(p.region, edep * p.wt);
 }
 
@@ -939,7 +1302,8 @@ __device__ __forceinline__ void pair(const Env &env, Stream &s, Part &p, Sink &s
     int q1, q2;
     if (eig <= 2.1f) {
         float r = s.next();
-        ese2 = (float)(PRM + (double)(0.5f * r * (eig - 2.0f * RM)));
+        // prm + 0.5*rnno30*(eig - 2*prm) with the double literal 0.5 (pair.cu:89)
+        ese2 = (float)((double)RM + 0.5 * (double)r * (double)(eig - 2.0f * RM));
         r = s.next();
         ese1 = eig - ese2;
         q1 = r < 0.5f ? -1 : 1;
@@ -1404,7 +1768,7 @@ __device__ __forceinline__ bool electron_iter(const Env &env, Stream &s, Part &p
             return false;
         }
         if (ustep != 0.0f && idisc < 0) {
-            t.score(p.region, (float)((double)p.e + (double)p.charge * PRM) * p.wt);
+            t.score(p.region, (p.e + (float)p.charge * RM) * p.wt);
             return false;
         }
         is_tstep = true;
@@ -1478,7 +1842,7 @@ __device__ __forceinline__ bool electron_iter(const Env &env, Stream &s, Part &p
         return false;
     }
     if (idisc < 0) {
-        t.score(p.region, (float)((double)p.e + (double)p.charge * PRM) * p.wt);
+        t.score(p.region, (p.e + (float)p.charge * RM) * p.wt);
         return false;
     }
     if (medium != medold) {
@@ -1533,6 +1897,116 @@ __device__ __forceinline__ void electron_step(const Env &env, Stream &s, Part &p
     }
 }
 
+// ------------------------------------------------------------------ electron-impact ionisation
+// eii_sample (eii_sample.cu:4-96): the energy transfer to an electron of
+// shell ish (binding uj) of element iz from the mixture of distant and close
+// collisions, the secondary created above AE, then the vacancy relaxed;
+// edep is set, not accumulated, as the reference does
+template <class Sink>
+__device__ __forceinline__ void eii_sample(const Env &env, Stream &s, int ish, int iz, float uj, Part &p,
+                                           const gegs_region &rd, float &edep, Sink &sink) {
+    const gegs_media_tables *mt = env.mt;
+    Part p1 = p;
+    const float eie = p.e;
+    const float t = eie - RM;
+    const float tau = t / RM;
+    const float tau12 = (tau + 1) * (tau + 1);
+    const float p2 = tau * tau + 2 * tau;
+    const float beta2 = p2 / tau12;
+    const float wmax = (float)(0.5 * (double)(t + uj));
+    const float xmax = uj / wmax;
+    const float c1 = (wmax / eie) * (wmax / eie);
+    const float c2 = (2 * tau + 1) / tau12;
+    const float fm_s = glog(RMT2 * p2 / uj) - beta2 - 0.5f;
+    const float prob_s = (float)(0.66666667 * (double)fm_s * (double)(1 + xmax + xmax * xmax));
+    float fm_h = 2 + c1 - c2;
+    if (fm_h < 1) fm_h = 1;
+    const float prob = fm_h + prob_s;
+    float r1, frej, wx;
+    do {
+        r1 = s.next();
+        if ((r1 * prob) < fm_h) {
+            r1 = s.next();
+            wx = 1 / (r1 * xmax + 1 - r1);
+            const float wxx = wx * xmax;
+            const float aux = wxx / (2 - wxx);
+            frej = (1 + aux * (aux - c2) + c1 * wxx * wxx) / fm_h;
+        } else {
+            r1 = s.next();
+            // powf(x, 0.333333333f) through the library's own log and exp
+            wx = 1.0f / gexp(glog(r1 * xmax * xmax * xmax + 1 - r1) * 0.333333333f);
+            frej = 1 - glog(wx) / fm_s;
+        }
+        r1 = s.next();
+    } while (r1 >= frej);
+    wx = wx * uj;
+    const float h1 = (eie + RM) / t;
+    const float ese1 = eie - wx;
+    p.e = ese1;
+    float dcosth = h1 * (ese1 - RM) / (ese1 + RM);
+    float sinthe = sqrtf(1 - dcosth);
+    float costhe = sqrtf(dcosth);
+    float sinphi, cosphi;
+    uphi21(s, costhe, sinthe, cosphi, sinphi, p);
+    const float ese2 = wx - uj + RM;
+    if (ese2 > mt->ae[rd.med]) {
+        p1.e = ese2;
+        dcosth = h1 * (ese2 - RM) / (ese2 + RM);
+        sinthe = -sqrtf(1 - dcosth);
+        costhe = sqrtf(dcosth);
+        turn(costhe, sinthe, cosphi, sinphi, p1);
+        create(sink, s, p1, E_ELECTRON_STEP);
+        edep = 0;
+    } else {
+        edep = wx - uj;
+    }
+    relax(env, s, uj, ish, iz, edep, p, rd, sink);
+}
+
+// the EII branch of moller (moller.cu:78-121): true when the interaction was
+// an ionisation of an EII shell (then handled completely)
+template <class Sink, class T>
+__device__ __forceinline__ bool eii_select(const Env &env, Stream &s, Part &p, const gegs_region &rd, Sink &sink, T &t) {
+    const gegs_media_tables *mt = env.mt;
+    const int m = rd.med;
+    const float ekin = p.e - RM;
+    const float elke = glog(ekin);
+    const uint32_t lelke = (uint32_t)(mt->eke01[m].y * elke + mt->eke01[m].x);
+    const gegs_float2 sg = mt->sig[m * MXEKE + lelke - 1];
+    const gegs_float2 eb = mt->ebr1[m * MXEKE + lelke - 1];
+    float sigm = sg.y * elke + sg.x;
+    const float pbrem = eb.y * elke + eb.x;
+    sigm = sigm * (1 - pbrem);
+    float rsh = s.next();
+    rsh = sigm * rsh;
+    const int nne = mt->nne[m];
+    for (int iele = 0; iele < nne; iele++) {
+        const int iz = (int)((double)mt->zelem[iele * MXMED + m] + 0.5);
+        const int nsh = mt->eii_no[iele * MXMED + m];
+        if (nsh <= 0) continue;
+        const int ifirst = mt->eii_first[iele * MXMED + m];
+        for (int ish = 1; ish <= nsh; ish++) {
+            const float uj = mt->binding_energies[(iz - 1) * MXSHELL + ish - 1];
+            if ((ekin > uj) && (uj > mt->te[m] || uj > mt->ap[m])) {
+                const int jj = ifirst + ish - 1;
+                const int i = (int)(mt->eii[jj - 1].x * elke + mt->eii[jj - 1].y + (float)((jj - 1) * N_EII_BINS));
+                const gegs_float2 xs = mt->eii_xsection[i - 1];
+                float sig_j = xs.x * elke + xs.y;
+                sig_j = sig_j * mt->pz[MXMED * iele + m] * mt->eii_cons[m];
+                rsh -= sig_j;
+                if (rsh < 0.0f) {
+                    float edep = 0.0f;
+                    eii_sample(env, s, ish, iz, uj, p, rd, edep, sink);
+                    if (p.e < rd.ecut) edep += p.e - RM; else create(sink, s, p, E_ELECTRON_STEP);
+                    t.score(p.region, edep * p.wt);
+                    return true;
+                }
+            }
+        }
+    }
+    return false;
+}
+
 // ------------------------------------------------------------------ Moller
 template <class Sink, class T>
 __device__ __forceinline__ void moller(const Env &env, Stream &s, Part &p, Sink &sink, T &t) {
@@ -1544,6 +2018,7 @@ __device__ __forceinline__ void moller(const Env &env, Stream &s, Part &p, Sink 
     const float eie = p.e;
     const float ekin = to_kinetic(eie);
     const float te = mt->te[m];
+    if (mt->eii_flag > 0 && mt->eii_nsh[m] > 0 && eii_select(env, s, p, rd, sink, t)) return;
     if (ekin <= 2.0f * te) {
         t.score(p.region, to_kinetic(p.e) * p.wt);
         return;
@@ -1566,17 +2041,17 @@ __device__ __forceinline__ void moller(const Env &env, Stream &s, Part &p, Sink 
     } while (r > rejf4);
     const float ekse2 = br * ekin;
     const float ese1 = eie - ekse2;
-    const float ese2 = (float)((double)ekse2 + PRM);
+    const float ese2 = ekse2 + RM;
     p.e = ese1;
     p1.e = ese2;
-    const float h1 = (float)(((double)eie + PRM) / (double)ekin);
-    float dcosth = (float)((double)h1 * ((double)ese1 - PRM) / ((double)ese1 + PRM));
+    const float h1 = (eie + RM) / ekin;
+    float dcosth = h1 * (ese1 - RM) / (ese1 + RM);
     float sinthe = sqrtf(1.0f - dcosth);
     float costhe = sqrtf(dcosth);
     float sinphi, cosphi;
     uphi21(s, costhe, sinthe, cosphi, sinphi, p);
     if (p.e < rd.ecut) edep += to_kinetic(p.e); else create(sink, s, p, E_ELECTRON_STEP);
-    dcosth = (float)((double)h1 * ((double)ese2 - PRM) / ((double)ese2 + PRM));
+    dcosth = h1 * (ese2 - RM) / (ese2 + RM);
     sinthe = -sqrtf(1.0f - dcosth);
     costhe = sqrtf(dcosth);
     turn(costhe, sinthe, cosphi, sinphi, p1);
@@ -1624,11 +2099,11 @@ __device__ __forceinline__ void bhabha(const Env &env, Stream &s, Part &p, Sink 
     br = fmaxf(br, 0.0f);
     const float ekse2 = br * ekin;
     const float ese1 = eip - ekse2;
-    const float ese2 = (float)((double)ekse2 + PRM);
+    const float ese2 = ekse2 + RM;
     p.e = ese1;
     p1.e = ese2;
-    const float h1 = (float)(((double)eip + PRM) / (double)ekin);
-    float dcosth = (float)fmin(1.0, (double)h1 * ((double)ese1 - PRM) / ((double)ese1 + PRM));
+    const float h1 = (eip + RM) / ekin;
+    float dcosth = fminf(1.0f, h1 * (ese1 - RM) / (ese1 + RM));
     float sinthe = sqrtf(1.0f - dcosth);
     float costhe = sqrtf(dcosth);
     float sinphi, cosphi;
@@ -1639,7 +2114,7 @@ __device__ __forceinline__ void bhabha(const Env &env, Stream &s, Part &p, Sink 
     } else {
         create(sink, s, p, E_ELECTRON_STEP);
     }
-    dcosth = (float)((double)h1 * ((double)ese2 - PRM) / ((double)ese2 + PRM));
+    dcosth = h1 * (ese2 - RM) / (ese2 + RM);
     sinthe = -sqrtf(1.0f - dcosth);
     costhe = sqrtf(dcosth);
     turn(costhe, sinthe, cosphi, sinphi, p1);
@@ -1761,7 +2236,7 @@ __device__ __forceinline__ void annih(const Env &env, Stream &s, Part &p, Sink &
     const int nsplit = mt->nbr_split;
     float edep = 0.0f;
     if (nsplit <= 0) return;
-    const float avip = (float)((double)p.e + PRM);
+    const float avip = p.e + RM;
     const float a = avip / RM;
     const float g = a - 1.0f;
     const float tt = g - 1.0f;

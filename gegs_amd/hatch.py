@@ -38,6 +38,9 @@ LAMBMIN_MS, LAMBMAX_MS, QMAX_MS = 1.0, 1e5, 0.5
 RM_F = float(np.float32(0.5110034))       # ELECTRON_REST_MASS_FLOAT promoted to double
 E_MAX_RR = 0.8
 MAX_SMAX = 1e10
+MXELEMENT, MXSHELL, MXTRANS, MXEDGE = 100, 6, 39, 16
+MXTOTSH, MXMDSH = 1538, 50
+MAX_EII_SHELLS, N_EII_BINS = 40, 250
 
 DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
 
@@ -91,6 +94,25 @@ class CMediaTables(ctypes.Structure):
         ("i_array", _arr(ctypes.c_uint32, MXMED * RAYCDFSIZE)),
         ("ums", _arr(ctypes.c_float, MS_SIZE)), ("fms", _arr(ctypes.c_float, MS_SIZE)),
         ("wms", _arr(ctypes.c_float, MS_SIZE)), ("ims", _arr(ctypes.c_int16, MS_SIZE)),
+        ("binding_energies", _arr(ctypes.c_float, MXSHELL * MXELEMENT)),
+        ("interaction_prob", _arr(ctypes.c_float, MXSHELL * MXELEMENT)),
+        ("relaxation_prob", _arr(ctypes.c_float, MXTRANS * MXELEMENT)),
+        ("edge_energies", _arr(ctypes.c_float, MXEDGE * MXELEMENT)),
+        ("edge_a", _arr(ctypes.c_float, MXEDGE * MXELEMENT)), ("edge_b", _arr(ctypes.c_float, MXEDGE * MXELEMENT)),
+        ("edge_c", _arr(ctypes.c_float, MXEDGE * MXELEMENT)), ("edge_d", _arr(ctypes.c_float, MXEDGE * MXELEMENT)),
+        ("edge_number", _arr(ctypes.c_int32, MXELEMENT)),
+        ("n_shell", _arr(ctypes.c_int32, MXMED)),
+        ("shell_array", _arr(ctypes.c_int32, MXMDSH * MXMED)),
+        ("eno_array", _arr(ctypes.c_float, MXMDSH * MXMED)),
+        ("eno_atbin_array", _arr(ctypes.c_int32, MXMDSH * MXMED)),
+        ("iz_array", _arr(ctypes.c_int32, MXTOTSH)), ("shn_array", _arr(ctypes.c_int32, MXTOTSH)),
+        ("be_array", _arr(ctypes.c_float, MXTOTSH)), ("jo_array", _arr(ctypes.c_float, MXTOTSH)),
+        ("erfjo_array", _arr(ctypes.c_float, MXTOTSH)),
+        ("eii_nsh", _arr(ctypes.c_int32, MXMED)),
+        ("eii_no", _arr(ctypes.c_int32, MXEL * MXMED)), ("eii_first", _arr(ctypes.c_int32, MXEL * MXMED)),
+        ("eii_cons", _arr(ctypes.c_float, MXMED)),
+        ("eii", _arr(F2, MAX_EII_SHELLS)),
+        ("eii_xsection", _arr(F2, N_EII_BINS * MAX_EII_SHELLS)),
     ]
 
 
@@ -566,6 +588,293 @@ def fix_brems(m):
 
 
 # ------------------------------------------------------------------ assembly
+# ------------------------------------------------------------------ atomic data
+def _numbers(path):
+    with open(path) as f:
+        return f.read().split()
+
+
+def edgset(relax_path, cs_path):
+    """PhysicsInit.cu:1184-1334: binding energies (eV -> MeV), shell
+    interaction probabilities (the sixth set to 1.01), the 38 relaxation
+    probabilities per element, and the photo-absorption edges and fits."""
+    tok = _numbers(relax_path)
+    k = 0
+
+    def take(n):
+        nonlocal k
+        out = tok[k:k + n]
+        k += n
+        return out
+    be = np.zeros((MXELEMENT, MXSHELL))
+    ip = np.zeros((MXELEMENT, MXSHELL))
+    rp = np.zeros((MXELEMENT, MXTRANS))
+    for i in range(MXELEMENT):
+        take(1)
+        be[i] = [float(v) * 1e-6 for v in take(MXSHELL)]
+    for i in range(MXELEMENT):
+        take(1)
+        ip[i, :MXSHELL - 1] = [float(v) for v in take(MXSHELL - 1)]
+        ip[i, MXSHELL - 1] = 1.01
+    for lo, hi in ((0, 19), (19, 26), (26, 32), (32, 37), (37, 38)):
+        for i in range(MXELEMENT):
+            take(1)
+            rp[i, lo:hi] = [float(v) for v in take(hi - lo)]
+    tok = _numbers(cs_path)
+    k = 0
+    ne = np.zeros(MXELEMENT, dtype=np.int32)
+    ed = np.zeros((5, MXELEMENT, MXEDGE))        # a, b, c, d, energy
+    for i in range(MXELEMENT):
+        _, n = take(2)
+        ne[i] = int(n)
+        for j in range(ne[i]):
+            ed[:, i, j] = [float(v) for v in take(5)]
+    return be, ip, rp, ne, ed
+
+
+def _erf_jo(jo):
+    """0.82436063535 erf(0.70710678119 (1 + 0.3 jo)) as init_compton stores it
+    (PhysicsInit.cu:1397: the reference drops the -1 of EGSnrc's erf1(x) - 1)."""
+    return 0.82436063535 * math.erf(0.70710678119 * (1.0 + 0.3 * jo))
+
+
+def init_compton(media, path):
+    """PhysicsInit.cu:1336-1519: the sub-shell data (J0 x 137, binding in
+    units of m) and per medium Walker's alias table over the sub-shells of
+    its elements weighted by pz x occupation, built as the reference does."""
+    with open(path) as f:
+        lines = f.read().split("\n")[18:]
+    tok = " ".join(lines).split()
+    iz = np.zeros(MXTOTSH, dtype=np.int32)
+    shn = np.zeros(MXTOTSH, dtype=np.int32)
+    ne = np.zeros(MXTOTSH, dtype=np.int32)
+    jo = np.zeros(MXTOTSH, dtype=np.float32)
+    be = np.zeros(MXTOTSH, dtype=np.float32)
+    erfjo = np.zeros(MXTOTSH, dtype=np.float32)
+    f32 = np.float32
+    for j in range(MXTOTSH):
+        a = tok[5 * j:5 * j + 5]
+        iz[j], shn[j], ne[j] = int(a[0]), int(a[1]), int(a[2])
+        jo[j] = f32(f32(a[3]) * f32(137.0))
+        be[j] = f32(float(f32(a[4])) * 1e-6 / float(f32(0.5110034)))
+        erfjo[j] = _erf_jo(float(jo[j]))
+    nshell = np.zeros(MXMED, dtype=np.int32)
+    shell = np.zeros((MXMED, MXMDSH), dtype=np.int32)
+    eno = np.zeros((MXMED, MXMDSH), dtype=np.float32)
+    atbin = np.zeros((MXMED, MXMDSH), dtype=np.int32)
+    for im, m in enumerate(media):
+        pztot = f32(0.0)
+        nsh = 0
+        for iele in range(m.ne):
+            z = int(m.z[iele])
+            for j in range(MXTOTSH):
+                if iz[j] == z:
+                    nsh += 1
+                    if nsh > MXMDSH:
+                        raise RuntimeError("medium %s has more than MXMDSH Compton sub-shells" % m.name)
+                    shell[im, nsh - 1] = j + 1
+                    aux = f32(f32(m.pz[iele]) * f32(ne[j]))
+                    eno[im, nsh - 1] = aux
+                    pztot = f32(pztot + aux)
+        if nsh == 0:
+            raise RuntimeError("medium %s has no Compton sub-shells" % m.name)
+        nshell[im] = nsh
+        e = eno[im]
+        for i in range(nsh):
+            e[i] = f32(-f32(e[i] / pztot))
+            atbin[im, i] = i + 1
+        atav = f32(f32(1.0) / f32(nsh))
+        for i in range(nsh - 1):
+            jh = nsh - 1
+            for t in range(nsh - 1):
+                if e[t] < 0 and abs(e[t]) > atav:
+                    jh = t
+                    break
+            jl = nsh - 1
+            for t in range(nsh - 1):
+                if e[t] < 0 and abs(e[t]) < atav:
+                    jl = t
+                    break
+            aux = f32(atav - abs(e[jl]))
+            e[jh] = f32(e[jh] + aux)
+            e[jl] = f32(f32(-e[jl] / atav) + f32(jl + 1))
+            atbin[im, jl] = jh + 1
+            if i == nsh - 2:
+                e[jh] = f32(jh + 1 + 1)
+        for i in range(nsh):
+            if e[i] < 0:
+                e[i] = f32(1 + i + 1)
+    return dict(iz=iz, shn=shn, ne=ne, jo=jo, be=be, erfjo=erfjo, n_shell=nshell, shell=shell, eno=eno,
+                atbin=atbin)
+
+
+def eii_init(media, c, binding, path, eii_flag):
+    """PhysicsInit.cu:1643-2047: EII shells (K..L3 with binding energy above
+    the lowest electron/photon cut of the media), their cross-section fits,
+    and for media with such shells the electron tables corrected for EII
+    (dedx, sig, ebr1, sig_e, sig_ismonotone; after mscati, as the reference
+    calls it).  Returns the EII flag in force."""
+    rm = RM_F
+    nshells = np.zeros(MXELEMENT, dtype=np.int32)
+    if not eii_flag:
+        return 0
+    e_min = min(min(m.ae - rm for m in media), min(m.ap for m in media))
+    for m in media:
+        for iele in range(m.ne):
+            z = int(m.z[iele] + 0.5)
+            if nshells[z - 1] == 0:
+                nshells[z - 1] = sum(1 for ish in range(4) if binding[z - 1, ish] > e_min)
+    nsh_tot = int(nshells.sum())
+    if nsh_tot == 0:
+        return 0
+    if nsh_tot > MAX_EII_SHELLS:
+        raise RuntimeError("more than MAX_EII_SHELLS EII shells")
+    first = np.concatenate([[0], np.cumsum(nshells)[:-1]])
+    for im, m in enumerate(media):
+        nsh = 0
+        for iele in range(m.ne):
+            z = int(m.z[iele] + 0.5)
+            c.eii_no[iele * MXMED + im] = int(nshells[z - 1])
+            nsh += int(nshells[z - 1])
+            c.eii_first[MXMED * iele + im] = int(first[z - 1]) + 1 if nshells[z - 1] > 0 else 0
+        c.eii_nsh[im] = nsh
+    tok = _numbers(path)
+    nskip = int(tok[0])
+    with open(path) as f:
+        lines = f.read().split("\n")
+    tok = " ".join(lines[1 + nskip:]).split()
+    emax, nbin = float(tok[0]), int(tok[1])
+    if nbin != N_EII_BINS:
+        raise RuntimeError("inconsistent EII data file")
+    k = 2
+    ii = 0
+    eii_z, eii_sh = [], []
+    for _ in range(MXELEMENT):
+        z, nsh = int(tok[k]), int(tok[k + 1])
+        k += 2
+        if nsh < nshells[z - 1]:
+            raise RuntimeError("EII data lack shells of element %d" % z)
+        for ish in range(1, nsh + 1):
+            fmax = float(np.float32(tok[k]))
+            aux = np.array(tok[k + 1:k + 1 + nbin], dtype=np.float32).astype(np.float64)
+            k += 1 + nbin
+            if ish <= nshells[z - 1]:
+                ii += 1
+                eii_z.append(z)
+                eii_sh.append(ish)
+                u = float(binding[z - 1, ish - 1])
+                ex = nbin / math.log(emax / u)
+                ey = 1.0 - ex * math.log(u)
+                c.eii[ii - 1].x, c.eii[ii - 1].y = ex, ey
+                exf, eyf = float(c.eii[ii - 1].x), float(c.eii[ii - 1].y)
+                for kk in range(1, nbin + 1):
+                    sigo = fmax * aux[kk - 2] if kk > 1 else 0.0
+                    loge = (kk - eyf) / exf
+                    xs = (fmax * aux[kk - 1] - sigo) * exf
+                    q = nbin * (ii - 1) + kk - 1
+                    c.eii_xsection[q].x = xs
+                    c.eii_xsection[q].y = sigo - float(np.float32(xs)) * loge
+        if ii == nsh_tot:
+            break
+    # electron tables with EII (the reference's shell lookup below reads
+    # eii[] at the shell's index within its element, jjj, where the shell's
+    # own entry is jj: the same when the first EII element is the only one)
+    occn = (2, 2, 2, 4)
+    for im, m in enumerate(media):
+        ec = m.ae - rm
+        ecc = min(ec, m.ap)
+        sum_a = sum(m.pz[i] * m.wa[i] for i in range(m.ne))
+        con_med = float(np.float32(m.rho / 1.6605655 / sum_a))
+        c.eii_cons[im] = con_med
+        if c.eii_nsh[im] <= 0:
+            continue
+        e0, e1 = float(c.eke01[im].x), float(c.eke01[im].y)
+        is_monotone = True
+        sigma_max = 0.0
+        dedx_old = sigma_old = wbrem_old = 0.0
+        base = im * MXEKE
+        for j in range(m.meke):
+            loge = (j + 1 - e0) / e1
+            e = math.exp(loge)
+            tau = e / rm
+            beta2 = tau * (tau + 2) / ((tau + 1) * (tau + 1))
+            p2 = 2 * rm * tau * (tau + 2)
+            dd, sg, eb = c.dedx[base + j], c.sig[base + j], c.ebr1[base + j]
+            dedx = float(dd.y) * loge + float(dd.x)
+            sig = float(sg.y) * loge + float(sg.x)             # the reference's condition is always true
+            if e > 2 * ec:
+                wbrem = float(eb.y) * loge + float(eb.x)
+                sigm = sig * (1 - wbrem)
+            else:
+                sigm = 0.0
+                wbrem = 1.0
+            sum_occn = sum_sigma = sum_dedx = 0.0
+            for iele in range(m.ne):
+                z = int(m.z[iele] + 0.5)
+                sum_sh = 0.0
+                for ish in range(c.eii_no[MXMED * iele + im]):
+                    jj = c.eii_first[MXMED * iele + im] + ish
+                    jjj = eii_sh[jj - 1]
+                    u = float(binding[z - 1, jjj - 1])
+                    wmax = (e + u) / 2
+                    uwm = u / wmax
+                    if u < e and u > ecc:
+                        sum_sh += occn[jjj - 1]
+                        ss_0 = 2 * (math.log(p2 / u) - uwm ** 3 * math.log(p2 / wmax)
+                                    - (beta2 + 0.833333) * (1 - uwm ** 3)) / 3 / u
+                        sh_0 = ((1 - uwm) * (1 + uwm / (2 - uwm)) + u * (wmax - u) / (e + rm) ** 2
+                                - (2 * tau + 1) / (tau + 1) ** 2 * uwm / 2 * math.log((2 - uwm) / uwm)) / u
+                        ss_1 = math.log(p2 / u) - uwm * uwm * math.log(p2 / wmax) - (beta2 + 1) * (1 - uwm * uwm)
+                        sh_1 = (math.log(wmax / u / (2 - uwm)) + 2 * (wmax - u) / (2 * wmax - u)
+                                + (wmax * wmax - u * u) / (e + rm) ** 2 / 2
+                                - (2 * tau + 1) / (tau + 1) ** 2 * math.log((2 * wmax - u) / wmax))
+                        av_e = (ss_1 + sh_1) / (ss_0 + sh_0)
+                        ex, ey = float(c.eii[jjj - 1].x), float(c.eii[jjj - 1].y)
+                        i = int(ex * loge + ey)
+                        i = (jj - 1) * N_EII_BINS + i
+                        xs = c.eii_xsection[i - 1]
+                        sig_j = (float(xs.x) * loge + float(xs.y)) * m.pz[iele] * con_med
+                        sum_sigma += sig_j
+                        sum_dedx += sig_j * av_e
+                sum_occn += sum_sh * m.pz[iele]
+            sigm += sum_sigma
+            dedx -= sum_dedx
+            aux = ec / e
+            if e > 2 * ec:
+                r1 = tau / (tau + 1)
+                r2 = tau + 1
+                k0 = float(np.float32(np.float32(0.153536) * np.float32(m.rho)))
+                sigo = (sum_occn * k0 / (beta2 * ec) * ((1 - aux * 2) * (1 + aux / (1 - aux) + r1 * r1 * aux / 2)
+                        - (tau * 2 + 1) / (r2 * r2) * aux * math.log((1 - aux) / aux)) / sum_a)
+                de = (sum_occn * k0 / beta2 * (math.log(0.25 / aux / (1 - aux)) + (1 - aux * 2) / (1 - aux)
+                      + r1 * r1 * (1 - aux * 4 * aux) / 8 - (tau * 2 + 1) / (r2 * r2) * math.log((1 - aux) * 2))
+                      / sum_a)
+                sigm -= sigo
+                dedx += de
+            sigma = sigm + wbrem * sig
+            sigma_max = max(sigma_max, sigma / dedx)
+            wbrem = wbrem * sig / sigma if sigma > 0 else 1.0
+            if j > 0:
+                d = c.dedx[base + j - 1]
+                d.y = (dedx - dedx_old) * e1
+                d.x = dedx - float(np.float32(d.y)) * loge
+                q = c.sig[base + j - 1]
+                q.y = (sigma - sigma_old) * e1
+                q.x = sigma - float(np.float32(q.y)) * loge
+                b = c.ebr1[base + j - 1]
+                b.y = (wbrem - wbrem_old) * e1
+                b.x = wbrem - float(np.float32(b.y)) * loge
+                if sigma / dedx < sigma_old / dedx_old:
+                    is_monotone = False
+            dedx_old, sigma_old, wbrem_old = dedx, sigma, wbrem
+        last, prev = base + m.meke - 1, base + m.meke - 2
+        for arr in (c.dedx, c.sig, c.ebr1):
+            arr[last].x, arr[last].y = arr[prev].x, arr[prev].y
+        c.sig_ismonotone[2 * im] = int(is_monotone)
+        c.sig_e[im] = sigma_max
+    return 1
+
+
 class MediaTables:
     """Device physics tables for an ordered list of media."""
 
@@ -585,6 +894,9 @@ class MediaTables:
         ums, fms, wms, ims, dllambi, dqmsi = init_ms_sr(os.path.join(data_dir, prefix + "_msnew.data"))
         # every region cut equals its medium's AE (init.cu:1005-1006)
         skin = mscati(media, [m.ae for m in media])
+        be, ip, rp, ne_edges, ed = edgset(os.path.join(data_dir, prefix + "_photo_relax.data"),
+                                          os.path.join(data_dir, prefix + "_photo_cs.data"))
+        comp = init_compton(media, os.path.join(data_dir, prefix + "_incoh.data"))
         for m in media:
             fix_brems(m)
         c = CMediaTables()
@@ -661,9 +973,23 @@ class MediaTables:
             for j in range(RAYCDFSIZE):
                 c.i_array[im * RAYCDFSIZE + j] = int(i_arr[im, j])
         for name, arr, dt in (("ums", ums, np.float32), ("fms", fms, np.float32),
-                              ("wms", wms, np.float32), ("ims", ims, np.int16)):
+                              ("wms", wms, np.float32), ("ims", ims, np.int16),
+                              ("binding_energies", be, np.float32), ("interaction_prob", ip, np.float32),
+                              ("relaxation_prob", rp, np.float32), ("edge_number", ne_edges, np.int32),
+                              ("edge_a", ed[0], np.float32), ("edge_b", ed[1], np.float32),
+                              ("edge_c", ed[2], np.float32), ("edge_d", ed[3], np.float32),
+                              ("edge_energies", ed[4], np.float32),
+                              ("n_shell", comp["n_shell"], np.int32), ("shell_array", comp["shell"], np.int32),
+                              ("eno_array", comp["eno"], np.float32), ("eno_atbin_array", comp["atbin"], np.int32),
+                              ("iz_array", comp["iz"], np.int32), ("shn_array", comp["shn"], np.int32),
+                              ("be_array", comp["be"], np.float32), ("jo_array", comp["jo"], np.float32),
+                              ("erfjo_array", comp["erfjo"], np.float32)):
             buf = np.ascontiguousarray(arr, dtype=dt)     # keep alive across the copy
+            assert buf.nbytes == ctypes.sizeof(getattr(c, name)), name
             ctypes.memmove(getattr(c, name), buf.ctypes.data, buf.nbytes)
+        # EII last: it corrects the electron tables written above (init.cu:1691-1695 order)
+        c.eii_flag = eii_init(media, c, be, os.path.join(data_dir, prefix + "_eii_ik.data"), eii_flag)
+        self.compton = comp
         self.c = c
         self.media = media
 

@@ -26,10 +26,14 @@ F_BOUND_COMPTON = 0x0002
 F_ATOMIC_RELAXATION = 0x0010
 F_PHOTOELECTRON_ANGLE = 0x0020
 
-# Region flags of the transport: Rayleigh and photo-electron angular
-# distribution on.  Bound Compton and atomic relaxation need the incoh and
-# photo-relax data sets, which pegs-lite does not provide.
-DEFAULT_FLAGS = F_RAYLEIGH | F_PHOTOELECTRON_ANGLE
+# Region flags of the transport, as the reference sets them for every region
+# (CUDA_EGS.h:44-51, init.cu:935-1000): Rayleigh scattering, atomic
+# relaxation, photo-electron angular distribution and bound Compton
+# scattering with Doppler broadening.
+DEFAULT_FLAGS = F_RAYLEIGH | F_ATOMIC_RELAXATION | F_PHOTOELECTRON_ANGLE | F_BOUND_COMPTON
+# The configuration round-1 results were quoted in (Klein-Nishina Compton, no
+# relaxation, no shell selection in photo-absorption).
+KN_FLAGS = F_RAYLEIGH | F_PHOTOELECTRON_ANGLE
 
 REGION_DTYPE = np.dtype([("med", "<u2"), ("flags", "<u2"), ("rhof", "<f4"),
                          ("pcut", "<f4"), ("ecut", "<f4")])

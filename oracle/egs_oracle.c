@@ -795,7 +795,7 @@ static void annih_at_rest(octx *c, ostream *s, opart *p, const gegs_region *rd) 
     const gegs_media_tables *mt = c->mt;
     float edep = 0.0f;
     if (mt->nbr_split > 1) p->wt = p->wt / (float)mt->nbr_split;
-    p->e = (float)PRM;
+    p->e = RM;
     p->charge = 0;
     for (int ibr = 1; ibr <= mt->nbr_split; ++ibr) {
         float costhe = rnd(s);
@@ -878,133 +878,487 @@ static void photon_step(octx *c, ostream *s, opart *p) {
     }
 }
 
-/* ------------------------------------------------------------ compton (compton.cu:17-492), Klein-Nishina branch */
+/* ------------------------------------------------------------ relax (relax.cu:14-238) */
+#define MXSHELL GEGS_MXSHELL
+#define MXTRANS GEGS_MXTRANS
+#define MXEDGE GEGS_MXEDGE
+#define MXMDSH GEGS_MXMDSH
+#define MXVAC GEGS_MXVAC
+#define N_EII_BINS GEGS_N_EII_BINS
+#define RELAX_CUTOFF 0.001
+static const int first_transition[5] = {1, 20, 27, 33, 38};
+static const int last_transition[5] = {19, 26, 32, 37, 39};
+static const int final_state[39] = {4, 3, 5, 6, 202, 302, 402, 404, 403, 303, 502, 503, 504, 602, 603, 604,
+                                    505, 605, 606, 13, 14, 5, 6, 505, 605, 606, 14, 5, 6, 505, 605, 606,
+                                    5, 6, 505, 605, 606, 6, 606};
+
+/* isotropic direction of a relaxation particle (relax.cu:62-90): cos(theta)
+ * uniform, azimuth by the box method on the half disk */
+static void relax_direction(ostream *s, opart *p) {
+    float eta = rnd(s);
+    eta = 2 * eta - 1;
+    p->w = eta;
+    eta = (1 - eta) * (1 + eta);
+    if (eta > 1e-20f) {
+        eta = sqrtf(eta);
+        float xphi, yphi, rhophi2;
+        do {
+            xphi = rnd(s);
+            xphi = 2 * xphi - 1;
+            yphi = rnd(s);
+            rhophi2 = xphi * xphi + yphi * yphi;
+        } while (rhophi2 > 1);
+        rhophi2 = 1 / rhophi2;
+        p->u = eta * (xphi * xphi - yphi * yphi) * rhophi2;
+        p->v = eta * 2 * xphi * yphi * rhophi2;
+    } else {
+        p->u = 0.0f;
+        p->v = 0.0f;
+        p->w = 1.0f;
+    }
+}
+
+/* fill a vacancy in shell n (1 K .. 6 N) of element iz carrying `energy`:
+ * fluorescence photons, Auger and Coster-Kronig electrons above the cuts are
+ * created, the rest is added to *edep.  A cascade deeper than MXVAC
+ * vacancies (an out-of-bounds write in the reference) deposits the vacancy
+ * locally instead. */
+static void relax(octx *c, ostream *s, float energy, int n, int iz, float *edep, opart p, const gegs_region *rd) {
+    const gegs_media_tables *mt = c->mt;
+    int vac[MXVAC];
+    float ea[MXVAC];
+    if (n < 1 || n > MXSHELL) return;
+    float e_cut = fminf(rd->ecut - RM, rd->pcut);
+    e_cut = ((double)e_cut > RELAX_CUTOFF) ? e_cut : (float)RELAX_CUTOFF;
+    if (energy <= e_cut) { *edep += energy; return; }
+    int nvac = 1;
+    vac[0] = n;
+    ea[0] = energy;
+    const float *be = mt->binding_energies + MXSHELL * (iz - 1);
+    for (;;) {
+        int shell = vac[nvac - 1];
+        float ei = ea[nvac - 1];
+        nvac -= 1;
+        if (ei <= e_cut) {
+            *edep += ei;
+            if (nvac > 0) continue;
+            break;
+        }
+        if (shell == MXSHELL) {          /* N-shell vacancy: an Auger electron */
+            p.e = ei + RM;
+            p.charge = -1;
+            relax_direction(s, &p);
+            if (p.e < rd->ecut) *edep += p.e - RM;
+            else push_child(c, s, &p, E_ELECTRON_STEP);
+            if (nvac > 0) continue;
+            break;
+        }
+        float eta = rnd(s);
+        int k;
+        for (k = first_transition[shell - 1]; k <= last_transition[shell - 1] - 1; k++) {
+            eta = eta - mt->relaxation_prob[MXTRANS * (iz - 1) + k - 1];
+            if (eta <= 0.0f) break;
+        }
+        int fin = final_state[k - 1];
+        int iql;
+        float elcut, ex;
+        if (nvac + (fin < 100 ? 1 : 2) > MXVAC) {
+            *edep += ei;
+            if (nvac > 0) continue;
+            break;
+        }
+        if (fin < 100) {
+            if (fin < 10) {              /* fluorescence */
+                iql = 0;
+                elcut = rd->pcut;
+            } else {                     /* Coster-Kronig */
+                fin = fin - 10;
+                iql = -1;
+                elcut = rd->ecut - RM;
+            }
+            float ef = be[fin - 1];
+            ex = ei - ef;
+            nvac += 1;
+            vac[nvac - 1] = fin;
+            ea[nvac - 1] = ef;
+        } else {                         /* Auger: two new vacancies */
+            int f1 = fin / 100;
+            int f2 = fin - f1 * 100;
+            nvac += 1;
+            vac[nvac - 1] = f1;
+            ea[nvac - 1] = be[f1 - 1];
+            nvac += 1;
+            vac[nvac - 1] = f2;
+            ea[nvac - 1] = be[f2 - 1];
+            iql = -1;
+            ex = ei - ea[nvac - 1] - ea[nvac - 2];
+            elcut = rd->ecut - RM;
+        }
+        if (ex <= elcut) {
+            *edep += ex;
+        } else {
+            p.charge = iql;
+            p.e = iql == 0 ? ex : ex + RM;
+            relax_direction(s, &p);
+            if (iql == 0) {
+                if (p.e < rd->pcut) *edep += p.e;
+                else push_child(c, s, &p, P_PHOTON_STEP);
+            } else {
+                if (p.e < rd->ecut) *edep += p.e - RM;
+                else push_child(c, s, &p, E_ELECTRON_STEP);
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------ compton (compton.cu:17-492) */
+/* ibcmpl from the region flags (compton.cu:87-95): 0x2 bound Compton with
+ * rejections returning the photon to the photon step, 0x6 without Doppler
+ * broadening and binding, 0xA rejections resampled; anything else Klein-Nishina */
+static int compton_mode(uint16_t flags) {
+    switch (flags & 0x000E) {
+    case 0x0002: return 1;
+    case 0x0006: return 2;
+    case 0x000A: return 3;
+    default: return 0;
+    }
+}
+
 static void compton(octx *c, ostream *s, opart *p) {
+    const gegs_media_tables *mt = c->mt;
     gegs_region rd = c->reg[p->region];
     float edep = 0.0f;
     opart p1 = *p;
+    int medium = rd.med;
     float peig = p->e;
     float ko = peig / RM;
-    float broi = 1.0f + 2.0f * ko;
-    float bro = 0.0f, br = 0.0f, temp = 0.0f, sinthe = 0.0f, costhe;
-    float alph1 = 0.0f, alpha = 0.0f, rejmax = 0.0f;
+    float broi = 1 + 2 * ko;
     int first_time = 1;
-    /* bound Compton (ibcmpl > 0) needs the incoh shell data, which the
-     * pegs-lite data set does not carry; regions request KN (flags 0). */
-    do {
-        if (ko > 2.0f) {
-            if (first_time) {
-                alph1 = om_logf(broi);
-                bro = 1.0f / broi;
-                float alph2 = ko * (broi + 1.0f) * bro * bro;
-                alpha = alph1 + alph2;
+    int ibcmpl = compton_mode(rd.flags);
+    /* declared inside the resample loop in the reference and read again on a
+     * resample without being set: kept from the previous pass */
+    float bro = 0.0f, br = 0.0f, temp = 0.0f, sinthe = 0.0f, costhe = 0.0f;
+    float alph1 = 0.0f, alpha = 0.0f, rejmax = 0.0f;
+    for (int compton_loops = 0;; compton_loops++) {
+        if (compton_loops > 100) break;
+        float uj = 0.0f, jo = 0.0f;
+        int j = 0;
+        if (ibcmpl > 0) {
+            /* shell: Walker alias table over the medium's sub-shells */
+            float rnno17 = rnd(s);
+            rnno17 = 1 + rnno17 * mt->n_shell[medium];
+            int i = (int)rnno17;
+            if (rnno17 > mt->eno_array[medium * MXMDSH + i - 1]) i = mt->eno_atbin_array[medium * MXMDSH + i - 1];
+            j = mt->shell_array[medium * MXMDSH + i - 1];
+            uj = mt->be_array[j - 1];
+            if (ko <= uj) {
+                if (ibcmpl == 1) { push_cont(c, s, p, P_PHOTON_STEP); break; }
+                continue;
             }
-            float aux, rejf3, rnno19;
-            int loops = 0;
-            do {
-                float rnno15 = rnd(s);
-                float rnno16 = rnd(s);
-                if (rnno15 * alpha < alph1) br = om_expf(alph1 * rnno16) * bro;
-                else br = sqrtf(rnno16 * broi * broi + (1.0f - rnno16)) * bro;
-                temp = (1.0f - br) / (ko * br);
-                sinthe = fmaxf(0.0f, temp * (2.0f - temp));
-                aux = 1.0f + br * br;
-                rejf3 = aux - br * sinthe;
-                rnno19 = rnd(s);
-                loops++;
-                if (loops > 100) break;
-            } while (rnno19 * aux > rejf3);
-        } else {
-            if (first_time) {
-                bro = 1.0f / broi;
-                rejmax = broi + bro;
-            }
-            float rnno16, rejf3;
-            int loops = 0;
-            do {
-                float rnno15 = rnd(s);
-                rnno16 = rnd(s);
-                br = bro + (1.0f - bro) * rnno15;
-                temp = (1.0f - br) / (ko * br);
-                sinthe = fmaxf(0.0f, temp * (2.0f - temp));
-                rejf3 = 1.0f + br * br - br * sinthe;
-                if (loops++ > 100) break;
-            } while (rnno16 * br * rejmax > rejf3);
+            jo = mt->jo_array[j - 1];
         }
-        first_time = 0;
-    } while (br < bro || br > 1.0f);
-    costhe = 1.0f - temp;
-    float pesg = br * peig;
-    float pese = (float)((double)(peig - pesg) + PRM);
-    sinthe = sqrtf(sinthe);
-    p->e = pesg;
-    float sinphi, cosphi;
-    uphi21(s, costhe, sinthe, &cosphi, &sinphi, p);
-    if (p->e < rd.pcut) edep += p->e;
-    else push_child(c, s, p, P_PHOTON_STEP);
-    float aux = 1.0f + br * br - 2.0f * br * costhe;
-    if (aux > 1e-8f) {
-        costhe = (1.0f - br * costhe) / sqrtf(aux);
-        sinthe = (1.0f - costhe) * (1.0f + costhe);
-        if (sinthe > 0.0f) sinthe = -sqrtf(sinthe); else sinthe = 0.0f;
-    } else {
-        costhe = 0.0f;
-        sinthe = -1.0f;
+        /* scattering angle from Klein-Nishina */
+        do {
+            if (ko > 2) {
+                if (first_time) {
+                    alph1 = om_logf(broi);
+                    bro = 1 / broi;
+                    float alph2 = ko * (broi + 1) * bro * bro;
+                    alpha = alph1 + alph2;
+                }
+                float aux, rejf3, rnno19;
+                int rnno19_loops = 0;
+                do {
+                    float rnno15 = rnd(s);
+                    float rnno16 = rnd(s);
+                    if ((rnno15 * alpha) < alph1) br = om_expf(alph1 * rnno16) * bro;
+                    else br = sqrtf(rnno16 * broi * broi + (1 - rnno16)) * bro;
+                    temp = (1 - br) / (ko * br);
+                    sinthe = fmaxf(0.0f, temp * (2 - temp));
+                    aux = 1 + br * br;
+                    rejf3 = aux - br * sinthe;
+                    rnno19 = rnd(s);
+                    rnno19_loops++;
+                    if (rnno19_loops > 100) break;
+                } while (rnno19 * aux > rejf3);
+            } else {
+                if (first_time) {
+                    bro = (float)(1.0 / (double)broi);
+                    rejmax = broi + bro;
+                }
+                float rnno16, rejf3;
+                int rnno16_loops = 0;
+                do {
+                    float rnno15 = rnd(s);
+                    rnno16 = rnd(s);
+                    br = bro + (1 - bro) * rnno15;
+                    temp = (1 - br) / (ko * br);
+                    sinthe = fmaxf(0.0f, temp * (2 - temp));
+                    rejf3 = 1 + br * br - br * sinthe;
+                    if (rnno16_loops++ > 100) break;
+                } while (rnno16 * br * rejmax > rejf3);
+            }
+            first_time = 0;
+        } while (br < bro || br > 1);
+        costhe = 1 - temp;
+        if (ibcmpl == 0) {
+            uj = 0;
+        } else {
+            /* rejection by the limited range of p_z, then p_z itself (compton.cu:228-405) */
+            float aux = ko * (ko - uj) * temp;
+            float aux1 = 2 * aux + uj * uj;
+            float pzmax = aux - uj;
+            if ((pzmax < 0) && (pzmax * pzmax >= aux1)) {
+                if (ibcmpl == 1) { push_cont(c, s, p, P_PHOTON_STEP); break; }
+                continue;
+            }
+            pzmax = pzmax / sqrtf(aux1);
+            float qc2 = 1 + br * br - 2 * br * costhe;
+            float qc = sqrtf(qc2);
+            float af, fmax1, fpz;
+            int rejected = 0;
+            if (pzmax > 1) {
+                pzmax = 1;
+                af = 0;
+                fmax1 = 1;
+                fpz = 1;
+            } else {
+                float aux3 = 1 + 2 * jo * fabsf(pzmax);
+                float aux4 = (float)(0.5 * (double)(1 - aux3 * aux3));
+                fpz = (float)(0.5 * (double)om_expf(aux4));
+                af = qc * (1 + br * (br - costhe) / qc2);
+                if (af < 0) {
+                    if (pzmax > 0) fpz = 1.0f - fpz;
+                    float eta_incoh = rnd(s);
+                    if (eta_incoh > fpz) rejected = 1;
+                    af = 0;
+                    fmax1 = 1;
+                } else {
+                    float fpz1;
+                    if ((double)pzmax < -0.15) {
+                        fmax1 = (float)(1.0 - (double)af * 0.15);
+                        fpz1 = fpz * fmax1 * jo;
+                    } else if ((double)pzmax < 0.15) {
+                        fmax1 = 1 + af * pzmax;
+                        aux3 = (float)(1.0 / (1.0 + 0.33267252734 * (double)aux3));
+                        aux4 = (float)((double)(fpz * aux3) *
+                                       (0.3480242 + (double)aux3 * (-0.0958798 + (double)aux3 * 0.7478556)) +
+                                       (double)mt->erfjo_array[j - 1]);
+                        if (pzmax > 0) {
+                            fpz1 = (float)((double)((1 - fmax1 * fpz) * jo) - 0.62665706866 * (double)af * (double)aux4);
+                            fpz = 1 - fpz;
+                        } else {
+                            fpz1 = (float)((double)(fmax1 * fpz * jo) - 0.62665706866 * (double)af * (double)aux4);
+                        }
+                    } else {
+                        fmax1 = (float)(1.0 + (double)af * 0.15);
+                        fpz1 = (1 - fmax1 * fpz) * jo;
+                        fpz = 1 - fpz;
+                    }
+                    float eta_incoh = rnd(s);
+                    if (eta_incoh * jo > fpz1) rejected = 1;
+                }
+            }
+            if (rejected) {
+                if (ibcmpl == 1) { push_cont(c, s, p, P_PHOTON_STEP); break; }
+                continue;
+            }
+            float pz = 0.0f;
+            for (int pz_loops = 0;; pz_loops++) {
+                if (pz_loops > 100) break;
+                if (ibcmpl != 2) {
+                    float rnno18 = rnd(s);
+                    rnno18 = rnno18 * fpz;
+                    if (rnno18 < 0.5f) {
+                        rnno18 = (double)(2 * rnno18) > 1e-30 ? 2 * rnno18 : (float)1e-30;
+                        pz = (float)(0.5 * (double)(1 - sqrtf(1 - 2 * om_logf(rnno18))) / (double)jo);
+                    } else {
+                        rnno18 = 2 * (1 - rnno18);
+                        pz = (float)(0.5 * (double)(sqrtf(1 - 2 * om_logf(rnno18)) - 1) / (double)jo);
+                    }
+                    if (fabsf(pz) > 1) continue;
+                    if ((double)pz < 0.15) {
+                        float frej;
+                        if ((double)pz < -0.15) frej = (float)(1.0 - (double)af * 0.15);
+                        else frej = 1 + af * pz;
+                        float eta = rnd(s);
+                        if (eta * fmax1 > frej) continue;
+                    }
+                } else {
+                    pz = 0;
+                    uj = 0;
+                }
+                break;
+            }
+            /* energy of the scattered photon */
+            if ((double)fabsf(pz) < 0.01) {
+                br = br * (1 + pz * (qc + (br * br - costhe) * pz));
+            } else {
+                aux = 1 - pz * pz * br * costhe;
+                aux1 = 1 - pz * pz * br * br;
+                float aux2 = qc2 - br * br * pz * pz * sinthe;
+                if ((double)aux2 > 1e-10) br = br / aux1 * (aux + pz * sqrtf(aux2));
+            }
+            uj = uj * RM;
+        }
+        float pesg = br * peig;
+        float pese = peig - pesg - uj + RM;
+        sinthe = sqrtf(sinthe);
+        p->e = pesg;
+        float sinphi, cosphi;
+        uphi21(s, costhe, sinthe, &cosphi, &sinphi, p);
+        if (p->e < rd.pcut) edep += p->e;
+        else push_child(c, s, p, P_PHOTON_STEP);
+        float aux = 1 + br * br - 2 * br * costhe;
+        if ((double)aux > 1e-8) {
+            costhe = (1 - br * costhe) / sqrtf(aux);
+            sinthe = (1 - costhe) * (1 + costhe);
+            if (sinthe > 0) sinthe = -sqrtf(sinthe);
+            else sinthe = 0;
+        } else {
+            costhe = 0.0f;
+            sinthe = -1.0f;
+        }
+        p1.e = pese;
+        p1.charge = -1;
+        uphi32(costhe, sinthe, cosphi, sinphi, &p1);
+        if (p1.e < rd.ecut) edep += p1.e - RM;
+        else push_child(c, s, &p1, E_ELECTRON_STEP);
+        if (ibcmpl == 1 || ibcmpl == 3) {
+            if ((double)uj > RELAX_CUTOFF) relax(c, s, uj, mt->shn_array[j - 1], mt->iz_array[j - 1], &edep, *p, &rd);
+            else edep += uj;
+        }
+        break;
     }
-    p1.e = pese;
-    p1.charge = -1;
-    uphi32(costhe, sinthe, cosphi, sinphi, &p1);
-    if (p1.e < rd.ecut) edep += (float)((double)p1.e - PRM);
-    else push_child(c, s, &p1, E_ELECTRON_STEP);
     escore(c, p->region, edep * p->wt);
 }
 
-/* ------------------------------------------------------------ photo (photo.cu:1-279), iedgfl = 0 branch */
+/* ------------------------------------------------------------ photo (photo.cu:1-279) */
+static void sauter_angle(ostream *s, opart *p) {
+    /* Sauter photo-electron direction (photo.cu:197-244) */
+    float beta = sqrtf((p->e - RM) * (p->e + RM)) / p->e;
+    float gamma = p->e / RM;
+    float alpha = 0.5f * gamma - 0.5f + 1.0f / gamma;
+    float ratio = beta / alpha;
+    float xi, rnpht2, sinth2, costhe;
+    do {
+        float rnpht = rnd(s);
+        rnpht = 2.0f * rnpht - 1.0f;
+        if (ratio <= 0.2f) {
+            float fkappa = rnpht + 0.5f * ratio * (1.0f - rnpht) * (1.0f + rnpht);
+            if (gamma < 100.0f) {
+                costhe = (beta + fkappa) / (1.0f + beta * fkappa);
+            } else {
+                if (fkappa > 0.0f)
+                    costhe = 1.0f - (1.0f - fkappa) * (gamma - 3.0f) /
+                             (2.0f * (1.0f + fkappa) * (gamma - 1.0f) * (gamma - 1.0f) * (gamma - 1.0f));
+                else
+                    costhe = (beta + fkappa) / (1.0f + beta * fkappa);
+            }
+            xi = (1.0f + beta * fkappa) * gamma * gamma;
+        } else {
+            xi = gamma * gamma * (1.0f + alpha * (sqrtf(1.0f + ratio * (2.0f * rnpht + ratio)) - 1.0f));
+            costhe = (1.0f - 1.0f / xi) / beta;
+        }
+        sinth2 = fmaxf(0.0f, (1.0f - costhe) * (1.0f + costhe));
+        rnpht2 = rnd(s);
+    } while (rnpht2 > 0.5f * (1.0f + gamma) * sinth2 * xi / gamma);
+    float sinthe = sqrtf(sinth2);
+    float sinphi, cosphi;
+    uphi21(s, costhe, sinthe, &cosphi, &sinphi, p);
+}
+
 static void photo(octx *c, ostream *s, opart *p) {
+    const gegs_media_tables *mt = c->mt;
     gegs_region rd = c->reg[p->region];
     float edep = 0.0f;
+    int medium = rd.med;
     float peig = p->e;
-    /* atomic relaxation (iedgfl != 0) needs the photo_relax/photo_cs data;
-     * regions request the no-relaxation branch */
-    p->e = (float)((double)peig + PRM);
-    p->charge = -1;
-    if (p->e > rd.ecut) {
-        if (rd.flags & GEGS_F_PHOTOELECTRON_ANGLE) {
-            float beta = sqrtf((p->e - RM) * (p->e + RM)) / p->e;
-            float gamma = p->e / RM;
-            float alpha = 0.5f * gamma - 0.5f + 1.0f / gamma;
-            float ratio = beta / alpha;
-            float xi, rnpht2, sinth2, costhe;
-            do {
-                float rnpht = rnd(s);
-                rnpht = 2.0f * rnpht - 1.0f;
-                if (ratio <= 0.2f) {
-                    float fkappa = rnpht + 0.5f * ratio * (1.0f - rnpht) * (1.0f + rnpht);
-                    if (gamma < 100.0f) {
-                        costhe = (beta + fkappa) / (1.0f + beta * fkappa);
-                    } else {
-                        if (fkappa > 0.0f)
-                            costhe = 1.0f - (1.0f - fkappa) * (gamma - 3.0f) /
-                                     (2.0f * (1.0f + fkappa) * (gamma - 1.0f) * (gamma - 1.0f) * (gamma - 1.0f));
-                        else
-                            costhe = (beta + fkappa) / (1.0f + beta * fkappa);
-                    }
-                    xi = (1.0f + beta * fkappa) * gamma * gamma;
-                } else {
-                    xi = gamma * gamma * (1.0f + alpha * (sqrtf(1.0f + ratio * (2.0f * rnpht + ratio)) - 1.0f));
-                    costhe = (1.0f - 1.0f / xi) / beta;
-                }
-                sinth2 = fmaxf(0.0f, (1.0f - costhe) * (1.0f + costhe));
-                rnpht2 = rnd(s);
-            } while (rnpht2 > 0.5f * (1.0f + gamma) * sinth2 * xi / gamma);
-            float sinthe = sqrtf(sinth2);
-            float sinphi, cosphi;
-            uphi21(s, costhe, sinthe, &cosphi, &sinphi, p);
+    int iedgfl = rd.flags & GEGS_F_ATOMIC_RELAXATION;
+    int iphter = rd.flags & GEGS_F_PHOTOELECTRON_ANGLE;
+    if (peig < mt->edge_energies[1]) {   /* quirk Q6: the second edge of hydrogen */
+        p->charge = -1;
+        p->e = peig + RM;
+        if (p->e < rd.ecut) {
+            edep = p->e - RM;
+            escore(c, p->region, edep * p->wt);
+        } else {
+            push_child(c, s, p, E_ELECTRON_STEP);
         }
-        push_child(c, s, p, E_ELECTRON_STEP);
-    } else {
-        edep += (float)((double)p->e - PRM);
+        return;
     }
+    float gle = om_logf(p->e);
+    int do_relax = 0;
+    float e_vac = 0.0f;
+    int iz = 0, k = 0;
+    if (iedgfl != 0) {
+        /* the element (photo.cu:87-154), then the shell (photo.cu:160-183) */
+        if (mt->nne[medium] == 1) {
+            iz = (int)((double)mt->zelem[medium] + 0.5);
+        } else {
+            float probs[GEGS_MXEL];
+            float aux = peig * peig;
+            float aux1 = aux * peig;
+            aux = aux * sqrtf(peig);
+            float sigtot = 0;
+            for (k = 0; k < mt->nne[medium]; k++) {
+                iz = (int)((double)mt->zelem[MXMED * k + medium] + 0.5);
+                const int e0 = MXEDGE * (iz - 1);
+                float sigma;
+                if (peig > mt->edge_energies[e0]) {
+                    sigma = (mt->edge_a[e0] + mt->edge_b[e0] / peig + mt->edge_c[e0] / aux + mt->edge_d[e0] / aux1) / peig;
+                } else {
+                    int j;
+                    for (j = 2; j <= mt->edge_number[iz - 1]; j++)
+                        if (peig >= mt->edge_energies[e0 + j - 1]) break;
+                    sigma = mt->edge_a[e0 + j - 1] +
+                            gle * (mt->edge_b[e0 + j - 1] + gle * (mt->edge_c[e0 + j - 1] + gle * mt->edge_d[e0 + j - 1]));
+                    sigma = om_expf(sigma);
+                }
+                sigma = sigma * mt->pz[MXMED * k + medium];
+                sigtot = sigtot + sigma;
+                probs[k] = sigma;
+            }
+            float br = rnd(s);
+            br = br * sigtot;
+            for (k = 0; k < mt->nne[medium]; k++) {
+                br = br - probs[k];
+                if (br <= 0) break;
+            }
+            /* past the last element only by rounding (an out-of-range read in the reference) */
+            if (k == mt->nne[medium]) k = mt->nne[medium] - 1;
+            iz = (int)((double)mt->zelem[MXMED * k + medium] + 0.5);
+        }
+        const float *be = mt->binding_energies + MXSHELL * (iz - 1);
+        if (peig <= be[5]) {             /* below the N shell: absorbed locally */
+            edep += peig;
+        } else {
+            float br = rnd(s);
+            for (k = 1; k <= 5; k++) {
+                if (peig > be[k - 1]) {
+                    float tmp = mt->interaction_prob[MXSHELL * (iz - 1) + k - 1];
+                    if (br < tmp) break;
+                    br = (br - tmp) / (1 - tmp);
+                }
+            }
+            e_vac = be[k - 1];
+            p->e = peig - e_vac + RM;
+            do_relax = 1;
+            p->charge = -1;
+        }
+    } else {
+        p->e = peig + RM;
+        p->charge = -1;
+    }
+    if (p->charge == -1) {
+        if (p->e > rd.ecut) {
+            if (iphter) sauter_angle(s, p);
+            push_child(c, s, p, E_ELECTRON_STEP);
+        } else {
+            edep += p->e - RM;
+        }
+    }
+    if (do_relax) relax(c, s, e_vac, k, iz, &edep, *p, &rd);
     escore(c, p->region, edep * p->wt);
 }
 
@@ -1065,7 +1419,8 @@ static void pair(octx *c, ostream *s, opart *p) {
     int iq1, iq2;
     if (eig <= 2.1f) {
         float rnno30 = rnd(s);
-        ese2 = (float)(PRM + (double)(0.5f * rnno30 * (eig - 2.0f * RM)));
+        /* prm + 0.5*rnno30*(eig - 2*prm) with the double literal 0.5 (pair.cu:89) */
+        ese2 = (float)((double)RM + 0.5 * (double)rnno30 * (double)(eig - 2.0f * RM));
         rnno30 = rnd(s);
         ese1 = eig - ese2;
         if (rnno30 < 0.5f) { iq1 = -1; iq2 = 1; } else { iq1 = 1; iq2 = -1; }
@@ -1143,7 +1498,7 @@ static void pair(octx *c, ostream *s, opart *p) {
         if (ichrg == 1) {
             uphi21(s, costhe, sinthe, &cosphi, &sinphi, p);
             if (p->e < rd.ecut) {
-                edep += (float)((double)p->e - PRM);
+                edep += (p->e - RM);
                 if (p->charge > 0) annih_at_rest(c, s, p, &rd);
             } else {
                 push_child(c, s, p, E_ELECTRON_STEP);
@@ -1152,7 +1507,7 @@ static void pair(octx *c, ostream *s, opart *p) {
             sinthe = -sinthe;
             uphi32(costhe, sinthe, cosphi, sinphi, &p1);
             if (p1.e < rd.ecut) {
-                edep += (float)((double)p1.e - PRM);
+                edep += (p1.e - RM);
                 if (p1.charge > 0) annih_at_rest(c, s, &p1, &rd);
             } else {
                 push_child(c, s, &p1, E_ELECTRON_STEP);
@@ -1387,7 +1742,7 @@ static void electron_step(octx *c, ostream *s, opart *p) {
         c->cnt.electron_steps++;
         c->cur_steps++;
         if (eie <= rd.ecut) {
-            edep1 = (float)((double)p->e - PRM);
+            edep1 = (p->e - RM);
             escore(c, p->region, edep1 * p->wt);
             if (p->charge > 0) annih_at_rest(c, s, p, &rd);
             return;
@@ -1477,7 +1832,7 @@ static void electron_step(octx *c, ostream *s, opart *p) {
             const float e_rr = (c->fixes & GEGS_FIX_Q7) ? eke : p->e;
             if (mt->i_do_rr[medium] == 1 && e_rr < mt->e_max_rr[medium]) {
                 if (tperp >= range) {
-                    edep1 = (float)((double)p->e - PRM);
+                    edep1 = (p->e - RM);
                     escore(c, p->region, edep1 * p->wt);
                     if (lelec > 0) annih_at_rest(c, s, p, &rd);
                     return;
@@ -1539,13 +1894,13 @@ static void electron_step(octx *c, ostream *s, opart *p) {
             irl = irnew;
             rd = c->reg[irl];
             if (eie <= rd.ecut) {
-                edep1 = (float)((double)p->e - PRM);
+                edep1 = (p->e - RM);
                 escore(c, p->region, edep1 * p->wt);
                 if (p->charge > 0) annih_at_rest(c, s, p, &rd);
                 return;
             }
             if (ustep != 0.0f && idisc < 0) {
-                edep1 = (float)((double)p->e + (double)p->charge * PRM);
+                edep1 = (p->e + (float)p->charge * RM);
                 escore(c, p->region, edep1 * p->wt);
                 return;
             }
@@ -1603,7 +1958,7 @@ static void electron_step(octx *c, ostream *s, opart *p) {
         p->e = eie;
         escore(c, irold, edep * p->wt);
         if (irnew == irl && eie <= rd.ecut) {
-            edep1 = (float)((double)p->e - PRM);
+            edep1 = (p->e - RM);
             escore(c, p->region, edep1 * p->wt);
             if (p->charge > 0) annih_at_rest(c, s, p, &rd);
             return;
@@ -1623,13 +1978,13 @@ static void electron_step(octx *c, ostream *s, opart *p) {
             c->cnt.region_gathers++;
         }
         if (eie <= rd.ecut) {
-            edep1 = (float)((double)p->e - PRM);
+            edep1 = (p->e - RM);
             escore(c, p->region, edep1 * p->wt);
             if (p->charge > 0) annih_at_rest(c, s, p, &rd);
             return;
         }
         if (idisc < 0) {
-            edep1 = (float)((double)p->e + (double)p->charge * PRM);
+            edep1 = (p->e + (float)p->charge * RM);
             escore(c, p->region, edep1 * p->wt);
             return;
         }
@@ -1671,6 +2026,111 @@ static void electron_step(octx *c, ostream *s, opart *p) {
     }
 }
 
+/* ------------------------------------------------------------ eii_sample (eii_sample.cu:4-96) */
+/* ionisation of shell ish (binding uj) of element iz by the electron p: the
+ * energy transfer from a mixture of the distant (Bethe) and close (Moller)
+ * collision shapes, the secondary created above AE, then relaxation of the
+ * vacancy; *edep is set (not accumulated), as the reference does */
+static void eii_sample(octx *c, ostream *s, int ish, int iz, float uj, opart *p, const gegs_region *rd, float *edep) {
+    const gegs_media_tables *mt = c->mt;
+    int medium = rd->med;
+    opart p1 = *p;
+    float eie = p->e;
+    float t = eie - RM;
+    float tau = t / RM;
+    float tau12 = (tau + 1) * (tau + 1);
+    float p2 = tau * tau + 2 * tau;
+    float beta2 = p2 / tau12;
+    float wmax = (float)(0.5 * (double)(t + uj));
+    float xmax = uj / wmax;
+    float c1 = (wmax / eie) * (wmax / eie);
+    float c2 = (2 * tau + 1) / tau12;
+    float fm_s = om_logf(RMT2 * p2 / uj) - beta2 - 0.5f;
+    float prob_s = (float)(0.66666667 * (double)fm_s * (double)(1 + xmax + xmax * xmax));
+    float fm_h = 2 + c1 - c2;
+    if (fm_h < 1) fm_h = 1;
+    float prob = fm_h + prob_s;
+    float r1, frej, wx;
+    do {
+        r1 = rnd(s);
+        if ((r1 * prob) < fm_h) {
+            r1 = rnd(s);
+            wx = 1 / (r1 * xmax + 1 - r1);
+            float wxx = wx * xmax;
+            float aux = wxx / (2 - wxx);
+            frej = (1 + aux * (aux - c2) + c1 * wxx * wxx) / fm_h;
+        } else {
+            r1 = rnd(s);
+            /* powf(x, 0.333333333f) through the library's own log and exp */
+            wx = 1.0f / om_expf(om_logf(r1 * xmax * xmax * xmax + 1 - r1) * 0.333333333f);
+            frej = 1 - om_logf(wx) / fm_s;
+        }
+        r1 = rnd(s);
+    } while (r1 >= frej);
+    wx = wx * uj;
+    float h1 = (eie + RM) / t;
+    float ese1 = eie - wx;
+    p->e = ese1;
+    float dcosth = h1 * (ese1 - RM) / (ese1 + RM);
+    float sinthe = sqrtf(1 - dcosth);
+    float costhe = sqrtf(dcosth);
+    float sinphi, cosphi;
+    uphi21(s, costhe, sinthe, &cosphi, &sinphi, p);
+    float ese2 = wx - uj + RM;
+    if (ese2 > mt->ae[medium]) {
+        p1.e = ese2;
+        dcosth = h1 * (ese2 - RM) / (ese2 + RM);
+        sinthe = -sqrtf(1 - dcosth);
+        costhe = sqrtf(dcosth);
+        uphi32(costhe, sinthe, cosphi, sinphi, &p1);
+        push_child(c, s, &p1, E_ELECTRON_STEP);
+        *edep = 0;
+    } else {
+        *edep = wx - uj;
+    }
+    relax(c, s, uj, ish, iz, edep, *p, rd);
+}
+
+/* is the Moller interaction an ionisation of an EII shell? (moller.cu:78-121)
+ * 1 with the shell, its element and binding energy, or 0 (one uniform drawn
+ * whenever the medium has EII shells) */
+static int eii_choose(const gegs_media_tables *mt, ostream *s, int medium, float ekin, int *ish_out, int *iz_out,
+                      float *uj_out) {
+    if (!(mt->eii_flag > 0 && mt->eii_nsh[medium] > 0)) return 0;
+    float elke = om_logf(ekin);
+    unsigned lelke = (unsigned)(mt->eke01[medium].y * elke + mt->eke01[medium].x);
+    gegs_float2 sg = mt->sig[medium * MXEKE + lelke - 1];
+    gegs_float2 eb = mt->ebr1[medium * MXEKE + lelke - 1];
+    float sigm = sg.y * elke + sg.x;
+    float pbrem = eb.y * elke + eb.x;
+    sigm = sigm * (1 - pbrem);
+    float rsh = rnd(s);
+    rsh = sigm * rsh;
+    for (int iele = 0; iele < mt->nne[medium]; iele++) {
+        int iz = (int)((double)mt->zelem[iele * MXMED + medium] + 0.5);
+        int nsh = mt->eii_no[iele * MXMED + medium];
+        if (nsh <= 0) continue;
+        int ifirst = mt->eii_first[iele * MXMED + medium];
+        for (int ish = 1; ish <= nsh; ish++) {
+            float uj = mt->binding_energies[(iz - 1) * MXSHELL + ish - 1];
+            if ((ekin > uj) && (uj > mt->te[medium] || uj > mt->ap[medium])) {
+                int jj = ifirst + ish - 1;
+                int i = (int)(mt->eii[jj - 1].x * elke + mt->eii[jj - 1].y + (float)((jj - 1) * N_EII_BINS));
+                float sig_j = mt->eii_xsection[i - 1].x * elke + mt->eii_xsection[i - 1].y;
+                sig_j = sig_j * mt->pz[MXMED * iele + medium] * mt->eii_cons[medium];
+                rsh -= sig_j;
+                if (rsh < 0.0f) {
+                    *ish_out = ish;
+                    *iz_out = iz;
+                    *uj_out = uj;
+                    return 1;
+                }
+            }
+        }
+    }
+    return 0;
+}
+
 /* ------------------------------------------------------------ moller (moller.cu:16-210) */
 static void moller(octx *c, ostream *s, opart *p) {
     const gegs_media_tables *mt = c->mt;
@@ -1679,10 +2139,18 @@ static void moller(octx *c, ostream *s, opart *p) {
     float edep = 0.0f;
     opart p1 = *p;
     float eie = p->e;
-    float ekin = (float)((double)eie - PRM);
-    /* EII branch: no shell participates in the pegs-lite data (eii_nsh = 0) */
+    float ekin = eie - RM;
+    int ish, iz;
+    float uj;
+    if (eii_choose(mt, s, medium, ekin, &ish, &iz, &uj)) {
+        eii_sample(c, s, ish, iz, uj, p, &rd, &edep);
+        if (p->e < rd.ecut) edep += p->e - RM;
+        else push_child(c, s, p, E_ELECTRON_STEP);
+        escore(c, p->region, edep * p->wt);
+        return;
+    }
     if (ekin <= 2.0f * mt->te[medium]) {
-        edep = (float)((double)p->e - PRM);
+        edep = (p->e - RM);
         escore(c, p->region, edep * p->wt);
         return;
     }
@@ -1704,22 +2172,22 @@ static void moller(octx *c, ostream *s, opart *p) {
     } while (rnno27 > rejf4);
     float ekse2 = br * ekin;
     float ese1 = eie - ekse2;
-    float ese2 = (float)((double)ekse2 + PRM);
+    float ese2 = ekse2 + RM;
     p->e = ese1;
     p1.e = ese2;
-    float h1 = (float)(((double)eie + PRM) / (double)ekin);
-    float dcosth = (float)((double)h1 * ((double)ese1 - PRM) / ((double)ese1 + PRM));
+    float h1 = (eie + RM) / ekin;
+    float dcosth = h1 * (ese1 - RM) / (ese1 + RM);
     float sinthe = sqrtf(1.0f - dcosth);
     float costhe = sqrtf(dcosth);
     float sinphi, cosphi;
     uphi21(s, costhe, sinthe, &cosphi, &sinphi, p);
-    if (p->e < rd.ecut) edep += (float)((double)p->e - PRM);
+    if (p->e < rd.ecut) edep += (p->e - RM);
     else push_child(c, s, p, E_ELECTRON_STEP);
-    dcosth = (float)((double)h1 * ((double)ese2 - PRM) / ((double)ese2 + PRM));
+    dcosth = h1 * (ese2 - RM) / (ese2 + RM);
     sinthe = -sqrtf(1.0f - dcosth);
     costhe = sqrtf(dcosth);
     uphi32(costhe, sinthe, cosphi, sinphi, &p1);
-    if (p1.e < rd.ecut) edep += (float)((double)p1.e - PRM);
+    if (p1.e < rd.ecut) edep += (p1.e - RM);
     else push_child(c, s, &p1, E_ELECTRON_STEP);
     escore(c, p->region, edep * p->wt);
 }
@@ -1732,7 +2200,7 @@ static void bhabha(octx *c, ostream *s, opart *p) {
     float edep = 0.0f;
     opart p1 = *p;
     float eip = p->e;
-    float ekin = (float)((double)eip - PRM);
+    float ekin = eip - RM;
     float t0 = ekin / RM;
     float e0 = t0 + 1.0f;
     float yy = 1.0f / (t0 + 2.0f);
@@ -1764,27 +2232,27 @@ static void bhabha(octx *c, ostream *s, opart *p) {
     br = fmaxf(br, 0.0f);
     float ekse2 = br * ekin;
     float ese1 = eip - ekse2;
-    float ese2 = (float)((double)ekse2 + PRM);
+    float ese2 = ekse2 + RM;
     p->e = ese1;
     p1.e = ese2;
-    float h1 = (float)(((double)eip + PRM) / (double)ekin);
-    float dcosth = (float)fmin(1.0, (double)h1 * ((double)ese1 - PRM) / ((double)ese1 + PRM));
+    float h1 = (eip + RM) / ekin;
+    float dcosth = fminf(1.0f, h1 * (ese1 - RM) / (ese1 + RM));
     float sinthe = sqrtf(1.0f - dcosth);
     float costhe = sqrtf(dcosth);
     float sinphi, cosphi;
     uphi21(s, costhe, sinthe, &cosphi, &sinphi, p);
     if (p->e < rd.ecut) {
-        edep += (float)((double)p->e - PRM);
+        edep += (p->e - RM);
         if (p->charge > 0) annih_at_rest(c, s, p, &rd);
     } else {
         push_child(c, s, p, E_ELECTRON_STEP);
     }
-    dcosth = (float)((double)h1 * ((double)ese2 - PRM) / ((double)ese2 + PRM));
+    dcosth = h1 * (ese2 - RM) / (ese2 + RM);
     sinthe = -sqrtf(1.0f - dcosth);
     costhe = sqrtf(dcosth);
     uphi32(costhe, sinthe, cosphi, sinphi, &p1);
     if (p1.e < rd.ecut) {
-        edep += (float)((double)p1.e - PRM);
+        edep += (p1.e - RM);
         if (p1.charge > 0) annih_at_rest(c, s, &p1, &rd);
     } else {
         push_child(c, s, &p1, E_ELECTRON_STEP);
@@ -1803,7 +2271,7 @@ static void brems(octx *c, ostream *s, opart *p) {
     float eie = p->e;
     int l = eie < 50.0f ? 1 : 3;
     int l1 = l + 1;
-    float ekin = (float)((double)eie - PRM);
+    float ekin = eie - RM;
     float elke = om_logf(ekin);
     float brmin = mt->ap[medium] / ekin;
     float waux = elke - mt->log_ap[medium];
@@ -1892,7 +2360,7 @@ static void brems(octx *c, ostream *s, opart *p) {
         else push_child(c, s, &p1, P_PHOTON_STEP);
     }
     p->e = ese;
-    if (p->e < rd.ecut) edep += (float)((double)p->e - PRM);
+    if (p->e < rd.ecut) edep += (p->e - RM);
     else push_child(c, s, p, E_ELECTRON_STEP);
     escore(c, p->region, edep * p->wt);
 }
@@ -1903,7 +2371,7 @@ static void annih(octx *c, ostream *s, opart *p) {
     gegs_region rd = c->reg[p->region];
     float edep = 0.0f;
     if (mt->nbr_split <= 0) return;      /* Q5 */
-    float avip = (float)((double)p->e + PRM);
+    float avip = p->e + RM;
     float a = avip / RM;
     float g = a - 1.0f;
     float t = g - 1.0f;
@@ -2320,6 +2788,54 @@ float oracle_range(const gegs_media_tables *mt, int qel, int medium, float rhof,
     float elkei = ((float)lelke - ek.x) / ek.y;
     float range = drange(dedx, eke, elke, ekei, elkei, lelke);
     return (range + mt->range_ep[(medium * MXEKE + lelke - 1) * 2 + qel]) / rhof;
+}
+
+/* relax probe: the cascade of a vacancy in `shell` of element iz with energy
+ * `energy`, in a region with the given record; returns the particles created
+ * (at most max_out) and the energy deposited locally */
+int oracle_relax(const gegs_media_tables *mt, const gegs_region *rd, int iz, int shell, float energy, uint64_t seed,
+                 uint64_t lin, gegs_particle *out, int max_out, float *edep_out) {
+    octx *c = (octx *)calloc(1, sizeof(octx));
+    c->mt = mt;
+    opart p;
+    memset(&p, 0, sizeof(p));
+    p.wt = 1.0f;
+    p.region = VAC_AND_OTH;
+    p.lin = lin;
+    ostream s;
+    st_begin(&s, &p, seed);
+    float edep = 0.0f;
+    relax(c, &s, energy, shell, iz, &edep, p, rd);
+    int n = c->sp < max_out ? c->sp : max_out;
+    for (int i = 0; i < n; ++i) {
+        const opart *q = &c->stack[i];
+        gegs_particle *o = out + i;
+        memset(o, 0, sizeof(*o));
+        o->e = q->e; o->u = q->u; o->v = q->v; o->w = q->w; o->wt = q->wt;
+        o->charge = q->charge; o->status = q->status; o->lin = q->lin; o->blk = q->blk;
+    }
+    *edep_out = edep;
+    free(c);
+    return n;
+}
+
+/* EII shell choice probe: for n electrons of kinetic energy ekin in `medium`,
+ * streams keyed by lineage lin0 + i: the chosen shell (1 K .. 4 L3) and
+ * element, or 0 for a Moller interaction */
+void oracle_eii_shell(const gegs_media_tables *mt, int medium, float ekin, uint64_t seed, uint64_t lin0, uint32_t n,
+                      int32_t *shell, int32_t *z) {
+    for (uint32_t i = 0; i < n; ++i) {
+        opart p;
+        memset(&p, 0, sizeof(p));
+        p.lin = lin0 + i;
+        ostream s;
+        st_begin(&s, &p, seed);
+        int ish = 0, iz = 0;
+        float uj;
+        if (!eii_choose(mt, &s, medium, ekin, &ish, &iz, &uj)) ish = iz = 0;
+        shell[i] = ish;
+        z[i] = iz;
+    }
 }
 
 float oracle_logf(float x) { return om_logf(x); }

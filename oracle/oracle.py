@@ -63,6 +63,11 @@ def lib():
         L.oracle_mscat.argtypes = [vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_uint64,
                                    ctypes.c_uint64, ctypes.c_uint32, vp]
         L.oracle_last_path.restype = ctypes.c_double
+        L.oracle_relax.restype = ctypes.c_int
+        L.oracle_relax.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64,
+                                   ctypes.c_uint64, vp, ctypes.c_int, vp]
+        L.oracle_eii_shell.argtypes = [vp, ctypes.c_int, ctypes.c_float, ctypes.c_uint64, ctypes.c_uint64,
+                                       ctypes.c_uint32, vp, vp]
         L.oracle_msdist.argtypes = [vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                     ctypes.c_float, vp, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, vp]
         L.oracle_compute_eloss.restype = ctypes.c_float
@@ -216,6 +221,25 @@ def interaction(workload, particle, seed, max_out=64):
                              _p(xb), _p(yb), _p(zb), ctypes.c_void_p(ctypes.addressof(workload.field)),
                              _p(inp), seed, _p(out), max_out, _p(ed))
     return out[:k].copy(), float(ed[0])
+
+
+def relax(tables, region, iz, shell, energy, seed, lin, max_out=32):
+    """The relaxation cascade of one vacancy: (particles created, local deposit)."""
+    from gegs_amd.native import PARTICLE_DTYPE
+    out = np.zeros(max_out, dtype=PARTICLE_DTYPE)
+    ed = np.zeros(1, dtype=np.float32)
+    reg = np.ascontiguousarray(np.asarray(region).reshape(1))
+    k = lib().oracle_relax(ctypes.c_void_p(tables.ptr()), _p(reg), iz, shell, energy, seed, lin, _p(out), max_out,
+                           _p(ed))
+    return out[:k].copy(), float(ed[0])
+
+
+def eii_shell(tables, medium, ekin, n, seed=1, lin0=1):
+    """(shell, Z) chosen by the EII branch of moller for n electrons (0: Moller)."""
+    sh = np.zeros(n, dtype=np.int32)
+    z = np.zeros(n, dtype=np.int32)
+    lib().oracle_eii_shell(ctypes.c_void_p(tables.ptr()), medium, ekin, seed, lin0, n, _p(sh), _p(z))
+    return sh, z
 
 
 def last_path():

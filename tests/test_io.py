@@ -106,10 +106,11 @@ def test_library_exports_every_declared_symbol():
     assert L.gegs_sizeof_media_tables() == ctypes.sizeof(hatch.CMediaTables)
 
 
-def test_validate_regions_rejects_unbuilt_physics_flags():
+def test_validate_regions_rejects_unknown_flags():
     """A region asking for physics the kernels do not act on is an error, not a
-    silent fallback (the reference turns bound Compton and relaxation on for
-    every region, init.cu:935-1000)."""
+    silent fallback; the reference's default flags (Rayleigh, relaxation,
+    photo-electron angles, bound Compton; init.cu:935-1000) and the bound
+    Compton variants pass."""
     lib = os.path.join(ROOT, "gegs_amd", "libgegs_hip.so")
     if not os.path.exists(lib):
         pytest.skip("libgegs_hip.so not built")
@@ -119,12 +120,16 @@ def test_validate_regions_rejects_unbuilt_physics_flags():
     reg["med"] = 0
     reg["flags"] = ph.DEFAULT_FLAGS
     assert L.gegs_validate_regions(reg.ctypes.data, len(reg)) == 0
-    supported = ph.F_RAYLEIGH | ph.F_PHOTOELECTRON_ANGLE | 0x0040
+    supported = 0x0001 | 0x000E | 0x0010 | 0x0020 | 0x0040
     for bit in range(16):
         r = reg.copy()
         r["flags"][3] = 1 << bit
         want = 0 if (1 << bit) & supported else 2001
         assert L.gegs_validate_regions(r.ctypes.data, len(r)) == want, hex(1 << bit)
+    for f in (0x0033, 0x0037, 0x003B, 0x0002):
+        r = reg.copy()
+        r["flags"][:] = f
+        assert L.gegs_validate_regions(r.ctypes.data, len(r)) == 0
     r = reg.copy()
     r["med"][4] = 6
     assert L.gegs_validate_regions(r.ctypes.data, len(r)) == 8001

@@ -15,6 +15,7 @@ import pytest
 from scipy import integrate, special, stats
 
 from gegs_amd import hatch, native, workloads
+from gegs_amd import phantom as ph
 from oracle import oracle
 
 RM = 0.5110034
@@ -27,11 +28,15 @@ def wl():
     return workloads.water(n=20, voxel=1.5)
 
 
-def low_cut(wl, ecut):
-    """The workload with every voxel's electron cut lowered to ecut (total energy)."""
+def with_regions(wl, ecut=None, flags=None):
+    """The workload with every voxel's electron cut (total energy) and/or
+    region flags replaced."""
     reg = wl.regions.copy()
     vox = reg["med"] != 0xFFFF
-    reg["ecut"][vox] = np.float32(ecut)
+    if ecut is not None:
+        reg["ecut"][vox] = np.float32(ecut)
+    if flags is not None:
+        reg["flags"][vox] = flags
     return types.SimpleNamespace(phantom=wl.phantom, tables=wl.tables, field=wl.field, regions=reg)
 
 
@@ -91,7 +96,8 @@ def test_sauter_photoelectron_angle(wl, t, mistake):
     the Sauter distribution.  5 keV and 80 MeV take the small beta/alpha
     branch (an approximation of Sauter good to O((beta/alpha)^2)), 80 MeV
     also the gamma > 100 form of cos(theta); 0.3 and 3 MeV the exact branch."""
-    w = low_cut(wl, RM + 0.0009) if t < 0.1 else wl
+    # without atomic relaxation the photo-electron takes the whole photon energy
+    w = with_regions(wl, RM + 0.0009 if t < 0.1 else None, ph.KN_FLAGS)
     n = 6000
     cs, gam = [], None
     for i in range(n):

@@ -21,6 +21,18 @@ def wl():
     return workloads.water(n=20, voxel=1.5)
 
 
+@pytest.fixture(scope="module")
+def wl_kn(wl):
+    """The same water workload with Klein-Nishina Compton and no atomic
+    relaxation (bound Compton and relaxation have their own tests)."""
+    import types
+    from gegs_amd import phantom as ph
+    reg = wl.regions.copy()
+    vox = reg["med"] != 0xFFFF
+    reg["flags"][vox] = ph.KN_FLAGS
+    return types.SimpleNamespace(phantom=wl.phantom, tables=wl.tables, field=wl.field, regions=reg)
+
+
 def particle(status, e, charge=0, region=None, lin=1, wl=None):
     p = np.zeros(1, dtype=native.PARTICLE_DTYPE)[0]
     p["e"] = e
@@ -54,9 +66,9 @@ def chi2_pvalue(samples, cdf, bins):
     return stats.chi2.sf(c2, keep.sum() - 1)
 
 
-def test_compton_klein_nishina(wl):
+def test_compton_klein_nishina(wl_kn):
     e0 = 1.25
-    outs, ed = run_many(wl, P_COMPTON, e0, 6000)
+    outs, ed = run_many(wl_kn, P_COMPTON, e0, 6000)
     frac = []
     for o, e in zip(outs, ed):
         ph = o[o["charge"] == 0]
@@ -200,11 +212,23 @@ def test_inflight_annihilation_conserves_energy(wl):
         assert abs(o["e"].sum() + e - (t0 + 2 * RM)) < 1e-5
 
 
-def test_photo_electron_energy(wl):
-    outs, ed = run_many(wl, P_PHOTO, 0.3, 500)
+def test_photo_electron_energy(wl, wl_kn):
+    # without relaxation the photo-electron takes the photon's energy
+    outs, ed = run_many(wl_kn, P_PHOTO, 0.3, 500)
     for o in outs:
         assert len(o) == 1 and o["charge"][0] == -1
         assert abs(o["e"][0] - (0.3 + RM)) < 1e-6
+    # with relaxation (the default) it takes the photon's energy less the
+    # binding energy of a shell of H or O; the binding energy is deposited
+    # (every vacancy of water lies below the 10 keV relaxation cut)
+    outs, ed = run_many(wl, P_PHOTO, 0.3, 500)
+    c = wl.tables.c
+    be = {float(np.float32(c.binding_energies[6 * (z - 1) + k])) for z in (1, 8) for k in range(6)}
+    for o, e in zip(outs, ed):
+        assert len(o) == 1 and o["charge"][0] == -1
+        t = float(o["e"][0]) - RM
+        assert min(abs((0.3 - t) - b) for b in be) < 2e-6
+        assert abs(t + e - 0.3) < 2e-6
 
 
 def test_photon_free_path_is_exponential(wl):
