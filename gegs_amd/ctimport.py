@@ -265,6 +265,45 @@ def ctcreate(list_path, thickness, box_min=(0, 0, 0), box_max=(0, 0, 0), ramp=ph
     return ph.Phantom([r[0] for r in ramp], xb, yb, zb, med, dens)
 
 
+def write_egsphant_ctcreate(path, ph):
+    """The .egsphant file ctcreate writes (ctcreate.c:1249-1311): the media
+    count "%2d", the names, a dummy ESTEPE line " 1.0" per medium, " %4d" x3
+    voxel counts, each axis's bounds as " %g" eight to a line, the media
+    digits a row per line with an empty line after each slice, then the
+    densities as "  %g" eight to a line (counted over a slice) with an empty
+    line after each slice.  The reference reads the phantom back from this
+    text, so its bounds and densities keep the 6 significant digits of %g."""
+    nx, ny, nz = ph.nx, ph.ny, ph.nz
+    out = ["%2d\n" % len(ph.media_names)]
+    out += ["%s\n" % n for n in ph.media_names]
+    out.append("".join(" %3.1f" % 1.0 for _ in ph.media_names) + "\n")
+    out.append(" %4d %4d %4d\n" % (nx, ny, nz))
+    for b, n in ((ph.xb, nx), (ph.yb, ny), (ph.zb, nz)):
+        for ii in range(n + 1):
+            out.append(" %g" % float(b[ii]))
+            if ii % 8 == 7:
+                out.append("\n")
+        if n % 8 != 7:
+            out.append("\n")
+    med = np.asarray(ph.media_idx).reshape(nz, ny, nx)
+    for k in range(nz):
+        for j in range(ny):
+            out.append("".join("%1d" % v for v in med[k, j]) + "\n")
+        out.append("\n")
+    rho = np.asarray(ph.density, dtype=np.float32).reshape(nz, ny, nx)
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                out.append("  %g" % float(rho[k, j, i]))
+                if (i + j * nx) % 8 == 7:
+                    out.append("\n")
+        if (nx * ny) % 8 != 0:
+            out.append("\n")
+        out.append("\n")
+    with open(path, "w") as f:
+        f.write("".join(out))
+
+
 def phantom_from_run_input(ri, base_dir=None):
     """The phantom a run input asks for with "use ctcreate = Yes" ([CTcreate]:
     CT ramps file, dicom file list, xyz boundaries x1 x2 y1 y2 z1 z2, 3d voxel

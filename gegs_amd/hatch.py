@@ -875,28 +875,56 @@ def eii_init(media, c, binding, path, eii_flag):
     return 1
 
 
+def data_files(data_dir=DATA_DIR, pegs="pegslite700.pegs4dat", prefix="pegslite"):
+    """Paths of the data set: the PEGS4 file, the photon cross-section prefix
+    (``<prefix>_photo.data`` ...), the atomic form factors, the
+    multiple-scattering surface, the relaxation and photo-absorption edge
+    data, the bound-Compton shells and the EII cross sections, in the
+    pegs-lite naming of this package."""
+    j = lambda n: os.path.join(data_dir, n)
+    return dict(pegs=j(pegs), xsections=j(prefix), ff=j(prefix + "_ff.data"), msnew=j(prefix + "_msnew.data"),
+                photo_relax=j(prefix + "_photo_relax.data"), photo_cs=j(prefix + "_photo_cs.data"),
+                incoh=j(prefix + "_incoh.data"), eii=j(prefix + "_eii_ik.data"))
+
+
+def reference_data_files(data_dir, pegs_file, photon_xsections, atomic_ff_file):
+    """The same set as a run input names it, in the reference's naming
+    (media.cu:151, :844-848; PhysicsInit.cu:865, :1209, :1311, :1377,
+    :1800): <data dir>/<pegs file>, <data dir>/<photon xsections>_*.data,
+    <data dir>/<atomic ff file>, msnew.data, photo_relax.data,
+    photo_cs.data, incoh.data and eii_ik.data in the data directory."""
+    j = lambda n: os.path.join(data_dir, n)
+    return dict(pegs=j(pegs_file), xsections=j(photon_xsections), ff=j(atomic_ff_file), msnew=j("msnew.data"),
+                photo_relax=j("photo_relax.data"), photo_cs=j("photo_cs.data"), incoh=j("incoh.data"),
+                eii=j("eii_ik.data"))
+
+
 class MediaTables:
     """Device physics tables for an ordered list of media."""
 
     def __init__(self, names, data_dir=DATA_DIR, pegs="pegslite700.pegs4dat", prefix="pegslite",
-                 mlc_medium=None, eii_flag=1, edit=None):
+                 mlc_medium=None, eii_flag=1, edit=None, files=None):
         """edit: optional callable applied to the list of PegsMedium records
         read from the PEGS file before any table is derived from them (test
-        media with synthetic stopping powers or cross sections)."""
+        media with synthetic stopping powers or cross sections).  files:
+        optional paths that replace the pegs-lite names, with the keys of
+        ``data_files`` (a run input's EGSnrc data set)."""
         if len(names) > MXMED:
             raise ValueError("at most %d media (MXMED)" % MXMED)
         self.names = list(names)
-        media = read_pegs_file(os.path.join(data_dir, pegs), self.names)
+        fp = data_files(data_dir, pegs, prefix)
+        fp.update(files or {})
+        self.files = fp
+        media = read_pegs_file(fp["pegs"], self.names)
         if edit is not None:
             edit(media)
-        ge, ptabs = init_user_photon(media, os.path.join(data_dir, prefix))
-        ray, i_arr, pmax = init_rayleigh_data(media, os.path.join(data_dir, prefix + "_ff.data"))
-        ums, fms, wms, ims, dllambi, dqmsi = init_ms_sr(os.path.join(data_dir, prefix + "_msnew.data"))
+        ge, ptabs = init_user_photon(media, fp["xsections"])
+        ray, i_arr, pmax = init_rayleigh_data(media, fp["ff"])
+        ums, fms, wms, ims, dllambi, dqmsi = init_ms_sr(fp["msnew"])
         # every region cut equals its medium's AE (init.cu:1005-1006)
         skin = mscati(media, [m.ae for m in media])
-        be, ip, rp, ne_edges, ed = edgset(os.path.join(data_dir, prefix + "_photo_relax.data"),
-                                          os.path.join(data_dir, prefix + "_photo_cs.data"))
-        comp = init_compton(media, os.path.join(data_dir, prefix + "_incoh.data"))
+        be, ip, rp, ne_edges, ed = edgset(fp["photo_relax"], fp["photo_cs"])
+        comp = init_compton(media, fp["incoh"])
         for m in media:
             fix_brems(m)
         c = CMediaTables()
@@ -988,7 +1016,7 @@ class MediaTables:
             assert buf.nbytes == ctypes.sizeof(getattr(c, name)), name
             ctypes.memmove(getattr(c, name), buf.ctypes.data, buf.nbytes)
         # EII last: it corrects the electron tables written above (init.cu:1691-1695 order)
-        c.eii_flag = eii_init(media, c, be, os.path.join(data_dir, prefix + "_eii_ik.data"), eii_flag)
+        c.eii_flag = eii_init(media, c, be, fp["eii"], eii_flag)
         self.compton = comp
         self.c = c
         self.media = media

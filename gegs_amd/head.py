@@ -339,6 +339,43 @@ def read_dynvmlc(path) -> Dynvmlc:
     return Dynvmlc(rmax, zmin, zthick, full, target, iso, n_full, n_half, x_start, gap, rad, openings, medium)
 
 
+# ------------------------------------------------------------------ egsinp JAWS block (SECJAWS)
+SECJAWS_TITLE = "*********** start of CM JAWS with identifier SECJAWS"   # the 52 characters compared
+
+
+def read_egsinp_secjaws(path):
+    """(Jaws, medium) of the JAWS component SECJAWS (read_egsinp_secjaws,
+    init.cu:612-689): after the line that starts with the 52-character CM
+    title, RMAX on the next line, three lines skipped, "ZMIN, ZMAX" of the
+    first (Y) pair, one line skipped, "ZMIN, ZMAX" of the second (X) pair, two
+    lines skipped, then the jaw medium's name.  None when the file has no such
+    block (the reference then keeps its zero-initialised jaws)."""
+    with open(path, errors="replace") as f:
+        lines = f.read().splitlines()
+    try:
+        k = next(i for i, l in enumerate(lines) if l[:52] == SECJAWS_TITLE)
+    except StopIteration:
+        return None
+    rmax = _nums(lines[k + 1], 1)[0]
+    zminy, zmaxy = _nums(lines[k + 5], 2)
+    zminx, zmaxx = _nums(lines[k + 7], 2)
+    medium = lines[k + 10].strip() if k + 10 < len(lines) else ""
+    return build_jaws(rmax, zminy, zmaxy, zminx, zmaxx), medium
+
+
+def write_egsinp_secjaws(path, jaws, medium="W700ICRU", append=False):
+    """A BEAMnrc JAWS component with the lines read_egsinp_secjaws takes
+    (written after an existing DYNVMLC block when append is set: the
+    reference reads both from one .egsinp file)."""
+    lines = [SECJAWS_TITLE + "  ***********",
+             "%g, RMAX" % jaws.rmax, "SECJAWS (synthetic)", "2, # PAIRED BARS OR JAWS", "Y",
+             "%g, %g, 0, 0, 0, 0," % (jaws.zminy, jaws.zmaxy), "X",
+             "%g, %g, 0, 0, 0, 0," % (jaws.zminx, jaws.zmaxx),
+             "0.7, 0.01, 0, 0,", "0.7, 0.01, 0, 0, 0", medium]
+    with open(path, "a" if append else "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
 # ------------------------------------------------------------------ fields (.mlcinfo)
 @dataclasses.dataclass
 class MlcField:

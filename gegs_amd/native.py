@@ -183,22 +183,36 @@ class Context:
         except Exception:
             pass
 
-    def load_workload(self, wl):
-        t = wl.tables
-        self._check(self.L.gegs_load_media(self.h, ctypes.c_void_p(t.ptr())))
-        ph = wl.phantom
-        reg = np.ascontiguousarray(wl.regions)
+    def load_media(self, tables):
+        """The physics tables (hatch.MediaTables) of the media the regions use."""
+        self._check(self.L.gegs_load_media(self.h, ctypes.c_void_p(tables.ptr())))
+
+    def load_phantom(self, ph, regions):
+        """The voxel phantom's bounds and its region records (phantom.region_table)."""
+        reg = np.ascontiguousarray(regions)
         self._check(self.L.gegs_load_phantom(self.h, ph.nx, ph.ny, ph.nz, _p(ph.xb), _p(ph.yb), _p(ph.zb), _p(reg)))
+        self.nvox = ph.nvox
+
+    def load_mlc(self, mlc):
+        """The MLC description (head.Mlc)."""
+        self._check(self.L.gegs_load_mlc(self.h, ctypes.c_void_p(ctypes.addressof(mlc))))
+
+    def load_jaws(self, jaws):
+        """The secondary jaws (head.Jaws)."""
+        self._check(self.L.gegs_load_jaws(self.h, ctypes.c_void_p(ctypes.addressof(jaws))))
+
+    def load_workload(self, wl):
+        self.load_media(wl.tables)
+        self.load_phantom(wl.phantom, wl.regions)
         if getattr(wl, "mlc", None) is not None:
-            self._check(self.L.gegs_load_mlc(self.h, ctypes.c_void_p(ctypes.addressof(wl.mlc))))
+            self.load_mlc(wl.mlc)
             self.set_leaf_positions(wl.leaf_positions)
         if getattr(wl, "jaws", None) is not None:
-            self._check(self.L.gegs_load_jaws(self.h, ctypes.c_void_p(ctypes.addressof(wl.jaws))))
+            self.load_jaws(wl.jaws)
             self.set_jaw_positions(wl.jaw_positions)
         if getattr(wl, "block_wedge", None) is not None:
             self.load_block_wedge(wl.block_wedge)
         self._check(self.L.gegs_set_field(self.h, ctypes.c_void_p(ctypes.addressof(wl.field))))
-        self.nvox = ph.nvox
 
     def set_source(self, src):
         src = np.ascontiguousarray(src)

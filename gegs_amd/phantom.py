@@ -78,12 +78,14 @@ class Phantom:
         return out
 
     def voxel_mass(self):
-        """Voxel masses (g) as in init.cu:1256-1271 (mass 0 replaced by 1)."""
-        dx = np.abs(np.diff(self.xb.astype(np.float64)))
-        dy = np.abs(np.diff(self.yb.astype(np.float64)))
-        dz = np.abs(np.diff(self.zb.astype(np.float64)))
-        vol = (dz[:, None, None] * dy[None, :, None] * dx[None, None, :]).reshape(-1)
-        m = (vol * self.density).astype(np.float32)
+        """Voxel masses (g) as in init.cu:1256-1271, in float: the volume
+        (dx * dy) * dz, its absolute value, times the density; a mass of 0
+        becomes 1."""
+        dx = np.diff(self.xb.astype(np.float32))
+        dy = np.diff(self.yb.astype(np.float32))
+        dz = np.diff(self.zb.astype(np.float32))
+        vol = np.abs((dx[None, None, :] * dy[None, :, None]) * dz[:, None, None]).reshape(-1)
+        m = (vol * np.asarray(self.density, dtype=np.float32).reshape(-1)).astype(np.float32)
         m[m == 0] = 1.0
         return m
 

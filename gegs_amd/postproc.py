@@ -26,6 +26,8 @@ computes in float:
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 f32 = np.float32
@@ -184,3 +186,209 @@ def gamma_pass_rate(ref, evl, spacing, dose_crit=0.01, dta=0.1, cutoff=0.1):
                 best = np.fmin(best, g2)
     sel = ref > cutoff * dmax
     return float(np.mean(best[sel] <= 1.0))
+
+
+# ------------------------------------------------------------------ what write_dose reads
+def host_bounds(b):
+    """The voxel coordinates the reference's output steps use: after the
+    bounds go to the device, entry k < n of the host array becomes the
+    centre (b[k] + b[k + 1]) / 2 of voxel k, in float; entry n keeps the last
+    boundary (init.cu:1289-1294)."""
+    b = np.asarray(b, dtype=np.float32).copy()
+    n = len(b) - 1
+    b[:n] = (b[:n] + b[1:]) / f32(2)
+    return b
+
+
+def voxel_dose(edep, mass, dose_unit):
+    """write_dose's values (output.c:888-891): (float) E / m * dose_unit in
+    float, per voxel."""
+    e = np.asarray(edep, dtype=np.float64).astype(np.float32)
+    return ((e / np.asarray(mass, dtype=np.float32)) * f32(dose_unit)).astype(np.float32)
+
+
+def dose_file_name(prefix, nx, ny, nz):
+    return "%s_DoseResults_%03dx%03dx%03d.bin" % (prefix, nx, ny, nz)
+
+
+def abs_dose_report(dose, xb, yb, zb, pos, calibrated, measured, fix_q9=False):
+    """The lines output_AbsDos logs (output.c:780-872) for the point pos of
+    the dose (float32, [z, y, x]) on the host coordinates xb, yb, zb."""
+    val, (i, j, k), vox = abs_dose(dose, xb, yb, zb, pos, fix_q9=fix_q9)
+    val, vox, meas = f32(val), f32(vox), f32(measured)
+    px, py, pz = (float(f32(v)) for v in pos)
+    if calibrated:
+        if meas == 0:
+            return ["Absolute dose at (%.2f, %.2f, %.2f) = %.1f cGy" % (px, py, pz, val),
+                    "   voxel dose of (i=%2d, j=%2d, k=%2d) = %.1f cGy" % (i, j, k, vox)]
+        d0 = f32(abs(f32(val - meas)) / meas)
+        d1 = f32(abs(f32(vox - meas)) / meas)
+        return ["Absolute dose at (%.2f, %.2f, %.2f):" % (px, py, pz),
+                "  measured = %.1f cGy, calculated = %.1f cGy, difference = %.1f %%" % (meas, val, f32(d0 * f32(100))),
+                "  voxel dose of index (%2d, %2d, %2d) = %.1f cGy, difference = %.1f %%" % (i, j, k, vox,
+                                                                                              f32(d1 * f32(100)))]
+    return ["Absolute dose at (%.2f, %.2f, %.2f) = %.3E cGy" % (px, py, pz, val),
+            "   voxel dose of (i=%2d, j=%2d, k=%2d) = %.3E cGy" % (i, j, k, vox)]
+
+
+def reldose_points_nearest(dose, xb, yb, zb, start, sizes, dimen):
+    """output_RelDos_II's sampling (output.c:603-624): point (i, j, k) at
+    start + sizes * index takes the dose of voxel int((p - b[0]) / (b[1] -
+    b[0]) + 0.1) per axis, in float, on the host coordinates (voxel
+    centres).  A point whose voxel lies outside the phantom is an error (the
+    reference reads outside its dose array there)."""
+    d = np.asarray(dose, dtype=np.float32)
+    nz, ny, nx = d.shape
+    b = [np.asarray(v, dtype=np.float32) for v in (xb, yb, zb)]
+    n = (nx, ny, nz)
+    out = np.zeros(int(dimen[0]) * int(dimen[1]) * int(dimen[2]), dtype=np.float32)
+    for k in range(dimen[2]):
+        for j in range(dimen[1]):
+            for i in range(dimen[0]):
+                idx = []
+                for a, t in enumerate((i, j, k)):
+                    p = f32(f32(start[a]) + f32(f32(sizes[a]) * f32(t)))
+                    q = f32(f32(p - b[a][0]) / f32(b[a][1] - b[a][0]))
+                    v = int(float(q) + 0.1)            # float + double, truncated toward zero
+                    if not 0 <= v < n[a]:
+                        raise ValueError("relative dose point (%d, %d, %d) lies outside the phantom on axis %d"
+                                         % (i, j, k, a))
+                    idx.append(v)
+                out[i + dimen[0] * j + dimen[0] * dimen[1] * k] = d[idx[2], idx[1], idx[0]]
+    return out
+
+
+def reldose_points_interp(dose, xb, yb, zb, start, sizes, dimen):
+    """output_RelDos's sampling (output.c:317-424): separable linear
+    interpolation along z, y, x of the voxel doses placed at the host
+    coordinates; None when a point lies at or outside the first or the
+    last-but-one coordinate (the reference then writes no file)."""
+    g = np.asarray(dose, dtype=np.float32)
+    b = [np.asarray(v, dtype=np.float32) for v in (xb, yb, zb)]
+    nz, ny, nx = g.shape
+    n = (nx, ny, nz)
+    for a in range(3):
+        end = f32(f32(start[a]) + f32(f32(sizes[a]) * f32(dimen[a] - 1)))
+        if f32(start[a]) <= b[a][0] or end >= b[a][n[a] - 1]:
+            return None
+    grid = interp_dose(g, b[0], b[1], b[2], start, sizes, dimen)
+    return np.ascontiguousarray(grid, dtype=np.float32).reshape(-1)
+
+
+def _read_measured(path, n):
+    vals = []
+    with open(path) as fh:
+        for line in fh:
+            t = line.split()
+            if t:
+                vals.append(float(np.float32(float(t[0]))))
+            if len(vals) == n:
+                break
+    if len(vals) < n:
+        raise ValueError('the measured data file "%s" holds %d of the %d values' % (path, len(vals), n))
+    return np.array(vals, dtype=np.float32)
+
+
+def compare_measured(calc, meas, dimen, start, sizes, isocenter, norm_type, array_type, gamma_len, gamma_dose):
+    """The comparison both relative-dose outputs make (output.c:444-593):
+    normalisation (1: each to its maximum; 2: each to its value at the
+    isocentre's point; else the measured values x 100), then every point
+    whose measured value exceeds 10% of the measured maximum is compared:
+    PTW729 by the dose difference alone, otherwise by the dose difference or,
+    failing that, a gamma-like test against the calculated values of the
+    neighbouring points along x and z (distance 0.25 cm).  Returns (rows of
+    (measured, calculated, pass), total, passed, percent)."""
+    calc = np.asarray(calc, dtype=np.float32).copy()
+    meas = np.asarray(meas, dtype=np.float32).copy()
+    n = len(calc)
+    mmax = f32(0)
+    if norm_type == 1:
+        cmax = f32(0)
+        for i in range(n):
+            mmax = max(mmax, meas[i])
+            cmax = max(cmax, calc[i])
+        meas = (meas / mmax).astype(np.float32)
+        calc = (calc / cmax).astype(np.float32)
+        mmax = f32(1)
+    elif norm_type == 2:
+        for i in range(n):
+            mmax = max(mmax, meas[i])
+        center = 0
+        if f32(sizes[0]) != 0:
+            center += int(float(f32(f32(f32(isocenter[0]) - f32(start[0])) / f32(sizes[0]))) + 0.1)
+        if f32(sizes[1]) != 0:
+            center += int(float(f32(f32(f32(isocenter[1]) - f32(start[1])) / f32(sizes[1]))) + 0.1) * dimen[0]
+        if f32(sizes[2]) != 0:
+            center += int(float(f32(f32(f32(isocenter[2]) - f32(start[2])) / f32(sizes[2]))) + 0.1) * dimen[0] * dimen[1]
+        if not 0 <= center < n:
+            raise ValueError("the isocentre's point %d lies outside the %d relative-dose points" % (center, n))
+        mc, cc = meas[center], calc[center]
+        meas = (meas / mc).astype(np.float32)
+        calc = (calc / cc).astype(np.float32)
+        mmax = f32(mmax / mc)
+    else:
+        meas = (meas * f32(100)).astype(np.float32)
+        for i in range(n):
+            mmax = max(mmax, meas[i])
+    thr = f32(mmax * f32(0.1))
+    glen, gdos = f32(gamma_len), f32(gamma_dose)
+    rows, total, count = [], 0, 0
+
+    def near(c, m):
+        v = f32(f32(abs(f32(c - m)) / m) / gdos)
+        v = f32(v * v)
+        return float(np.sqrt(float(v) + 0.25 * 0.25 / float(glen) / float(glen))) <= 1.0
+
+    for i in range(n):
+        p = 0
+        m = meas[i]
+        if m > thr:
+            total += 1
+            if array_type == 1:
+                if f32(abs(f32(calc[i] - m)) / m) <= gdos:
+                    p = 1
+                    count += 1
+                else:
+                    p = -1
+            else:
+                if f32(abs(f32(calc[i] - m)) / m) <= gdos:
+                    p += 1
+                iz = i // dimen[0]
+                ix = i - iz * dimen[0]
+                if ix > 0 and p <= 0 and near(calc[i - 1], m):
+                    p += 1
+                if ix + 1 < dimen[0] and p <= 0 and near(calc[i + 1], m):
+                    p += 1
+                if iz > 0 and near(calc[i - dimen[0]], m):         # no p <= 0 test here (output.c:570)
+                    p += 1
+                if iz + 1 < dimen[2] and p <= 0 and near(calc[i + dimen[0]], m):
+                    p += 1
+                if p > 0:
+                    count += 1
+                else:
+                    p = -1
+        rows.append((float(m), float(calc[i]), p))
+    percent = f32(f32(count) / f32(total)) if total else f32(np.nan)
+    return rows, total, count, percent
+
+
+def write_relative_dose(prefix, calc, dimen, measured_path=None, **compare):
+    """<prefix>_RelativeDose_XXXxYYYxZZZ.txt: "measured\\t calculated\\t pass"
+    rows and the pass-rate line, or the calculated values alone when the
+    measured file cannot be opened (output.c:430-436, :585-592).  Returns
+    (file name, log lines)."""
+    fname = "%s_RelativeDose_%03dx%03dx%03d.txt" % (prefix, dimen[0], dimen[1], dimen[2])
+    log = [fname]
+    if measured_path is None or not os.path.exists(measured_path):
+        with open(fname, "w") as f:
+            for v in np.asarray(calc, dtype=np.float32):
+                f.write("%f\n" % v)
+        return fname, log + ["Error: dose data file reading error!"]
+    meas = _read_measured(measured_path, len(calc))
+    rows, total, count, percent = compare_measured(calc, meas, dimen, **compare)
+    summary = "Totally %d points compared, %d passed, percentage = %.0f %%" % (total, count, f32(percent * f32(100)))
+    with open(fname, "w") as f:
+        for m, c, p in rows:
+            f.write("%f\t %f\t %d\n" % (m, c, p))
+        f.write(summary + "\n")
+    return fname, log + [summary]

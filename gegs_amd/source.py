@@ -11,6 +11,7 @@ sources of the benchmark workloads.
 from __future__ import annotations
 
 import math
+import os
 import struct
 
 import numpy as np
@@ -192,3 +193,76 @@ def stratified_parallel_beam(n_side, energy, half_width, z0, seed):
     s["w"] = np.float32(1.0)
     s["wt"] = np.float32(1.0)
     return s
+
+
+# ------------------------------------------------------------------ the run input's phase-space files
+class PhaseSpaceFiles:
+    """The phase-space input of a run: ``n_files`` MODE0 files
+    ``<base>_p<i>.egsphsp1`` (i = 0 .. n_files - 1), or one file, read as one
+    record sequence (read_egsphsp_header / read_egsphsp_data,
+    read_egsphsp.cpp:60-232).
+
+    Kept from the reference (each with a switch that corrects it):
+
+    * Q4: with one file the header (its record count) comes from
+      ``<base>.egsphsp1`` but the records from ``<base>_p0.egsphsp1``
+      (read_egsphsp.cpp:113 against :202-216); ``fix_q4`` reads both from
+      ``<base>.egsphsp1``.
+    * Q15: a read that runs past the end of file i takes its remaining
+      records from the start of file i again, not of file i + 1
+      (read_egsphsp.cpp:208-212 open the same ``_p<i>`` name twice);
+      ``fix_q15`` continues in file i + 1.
+
+    ``read(istart, length)`` returns None where the reference returns -1:
+    ``istart`` at or past the total, or fewer than ``length`` records left."""
+
+    def __init__(self, base, n_files, fix_q4=False, fix_q15=False):
+        self.base = base
+        self.n_files = int(n_files)
+        self.fix_q15 = fix_q15
+        if self.n_files > 1:
+            self.header_paths = ["%s_p%d.egsphsp1" % (base, i) for i in range(self.n_files)]
+            self.data_paths = list(self.header_paths)
+        else:
+            self.header_paths = [base + ".egsphsp1"]
+            self.data_paths = [base + ".egsphsp1" if fix_q4 else base + "_p0.egsphsp1"]
+        self.counts = []
+        for p in self.header_paths:
+            try:
+                with open(p, "rb") as f:
+                    hdr = f.read(28)
+            except OSError:
+                raise ValueError('Could not open the egsphsp file "%s".' % p)
+            if len(hdr) < 28:
+                raise ValueError('The egsphsp file "%s" has no complete header.' % p)
+            self.counts.append(struct.unpack("<5s3xI", hdr[:12])[1])
+        self.total = int(sum(self.counts))
+
+    def _records(self, i, start, count):
+        path = self.data_paths[i]
+        if not os.path.exists(path):
+            raise ValueError('Could not open the egsphsp file "%s"%s.' % (
+                path, " (the reference reads the records of a single phase-space file from <name>_p0.egsphsp1,"
+                      " quirk Q4; fix_q4 reads <name>.egsphsp1)" if self.n_files <= 1 else ""))
+        with open(path, "rb") as f:
+            f.seek(28 + 28 * start)
+            raw = f.read(28 * count)
+        if len(raw) != 28 * count:
+            raise ValueError('The egsphsp file "%s" holds fewer than %d records from record %d.' % (path, count, start))
+        return np.frombuffer(raw, dtype=PHSP_DTYPE).copy()
+
+    def read(self, istart, length):
+        istart, length = int(istart), int(length)
+        if istart >= self.total or self.total - istart < length:
+            return None
+        i = 0
+        while i < len(self.counts) and istart >= self.counts[i]:
+            istart -= self.counts[i]
+            i += 1
+        if i >= len(self.counts):
+            return None
+        remain = self.counts[i] - istart
+        if remain < length:
+            second = i + 1 if self.fix_q15 else i
+            return np.concatenate([self._records(i, istart, remain), self._records(second, 0, length - remain)])
+        return self._records(i, istart, length)

@@ -221,6 +221,10 @@ default medium = AIR700ICRU
 phsp file number = 1
 phase space file = "ph"
 z position = 10
+[SecJaws]
+secjaws file = "a.egsinp"
+[MLC]
+mlcinfo file = "f.mlcinfo"
 [Block]
 module Block = On
 block file = "WedgeBlock.egsinp"
