@@ -29,6 +29,14 @@
 #ifndef GEGS_WAVES_PER_EU
 #define GEGS_WAVES_PER_EU(n) __attribute__((amdgpu_waves_per_eu(n)))
 #endif
+// Marks a function whose wavefront operations are shared by exactly the
+// lanes that call it together -- a subset of the wavefront when the call is
+// in divergent code -- unlike the kernels' other wavefront operations, which
+// every lane of the wavefront reaches together.  Nothing on the GPU; the CPU
+// build with 64-lane wavefronts groups lanes by it.
+#ifndef GEGS_WAVE_SUBSET_OP
+#define GEGS_WAVE_SUBSET_OP(id)
+#endif
 #include <vector>
 
 #include "transport.hpp"
@@ -275,6 +283,7 @@ struct BankSink {
     // compiler may assume the first iteration exits and drop the test, which
     // sends every lane's particle to the first lane's destination count.
     __device__ __forceinline__ void push(const Part &p) {
+        GEGS_WAVE_SUBSET_OP(1);
         const uint32_t seg = blockIdx.x % QSEG;
         const int e = p.status;
         const int dest = bank_slot(e, flip, e == P_PHOTON_STEP ? photons_now : electrons_now);
@@ -298,6 +307,7 @@ struct BankSink {
     // an electron suspended between two condensed-history steps carries the
     // step state (electron_iter's EState) to the sweep that resumes it
     __device__ __forceinline__ void push_suspended(const Part &p, const EState &es) {
+        GEGS_WAVE_SUBSET_OP(2);
         const uint32_t seg = blockIdx.x % QSEG;
         const int dest = bank_slot(p.status, flip, electrons_now);
         const uint64_t m = __ballot(1);

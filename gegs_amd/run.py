@@ -372,8 +372,12 @@ def _run(ri, path, base, data_dir, root, log, method, seed, fixes, n_batches, de
                 ctx.set_leaf_positions(fields[i].positions)
                 ctx.set_jaw_positions([fields[i].x1, fields[i].x2, fields[i].y1, fields[i].y2])
                 ctx.set_field(j["field"])
-                out = ctx.run(seed, n_batches=n_batches, history_offset=j["history_offset"], first=j["first"],
-                              count=j["count"])
+                # the plain kernel keys history k of the range by history_offset
+                # + record index, the batched loop by the job's history_offset
+                # + k: the same histories need the offset less the first record
+                # (modulo 2^64, as the library adds it)
+                off = (j["history_offset"] - j["first"]) % (1 << 64)
+                out = ctx.run(seed, n_batches=n_batches, history_offset=off, first=j["first"], count=j["count"])
                 field_edep[i] += out["dose"]
         else:
             raise RunError("method must be batched or plain")

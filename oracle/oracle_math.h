@@ -15,7 +15,7 @@
  *   sin/cos(2 pi t): t reduced modulo 1 exactly, then to an octant
  *          |y| <= 1/8 so that 2 pi y <= pi/4; Taylor series of sin and cos.
  * All functions are accurate to a few ulp over the ranges the transport uses
- * (tests/test_oracle_math.py checks them against double-precision libm).
+ * (tests/test_oracle_core.py checks them against double-precision libm).
  */
 #ifndef ORACLE_MATH_H
 #define ORACLE_MATH_H

@@ -1,7 +1,9 @@
 // CPU build of the event-queued transport: the production kernels of
 // gegs_amd/csrc/event.hpp, compiled as host C++ against the one-lane
-// wavefront stand-in of include/hip/hip_runtime.h, run sweep after sweep
-// until the banks and the source are empty.  Inputs and outputs are raw
+// wavefront stand-in of include/hip/hip_runtime.h (or, with -DGEGS_SIMT,
+// the 64-lane stand-in of include_simt/hip/hip_runtime.h, with the launch
+// shape and refill policy of the GPU), run sweep after sweep until the
+// banks and the source are empty.  Inputs and outputs are raw
 // files in one directory (written and read by tests/test_event_cpu.py).
 // Built with AddressSanitizer, with the LDS array sized exactly to what the
 // flight kernels stage, so that any out-of-bounds access of the queue logic
@@ -49,6 +51,9 @@ static void spill(const std::string &path, const void *p, size_t n) {
 struct Meta {
     uint32_t nx, ny, nz, nsrc, nbatch, pchunk, echunk, have_mlc, have_jaws, cap;
     uint64_t seed, hist_offset;
+    // 64-lane build: flight block size, refill threshold and reservation,
+    // blocks per flight launch, primaries per injection and the injection threshold
+    uint32_t flight_block, refill_min, refill_slots, flight_grid, inject, threshold;
 };
 
 int main(int argc, char **argv) {
@@ -158,17 +163,28 @@ int main(int argc, char **argv) {
     a.adj = nullptr;
     a.grad = nullptr;
     a.tally_batches = m.nbatch;
+    EventConfig cfg;
+#ifdef GEGS_SIMT
+    // 64-lane wavefronts with the GPU's launch shape and refill policy
+    cfg.block = 256;
+    cfg.grid = 4;
+    cfg.flight_block = m.flight_block;
+    cfg.photon_grid = m.flight_grid;
+    cfg.electron_grid = m.flight_grid;
+    cfg.refill_min = m.refill_min;
+    cfg.refill_slots = m.refill_slots;
+#else
     // one-lane wavefronts: a refill needs only one idle lane; the flight
     // kernels run one thread per block so that each stages all of LDS
-    EventConfig cfg;
     cfg.block = 64;
     cfg.grid = 2;
     cfg.flight_block = 1;
     cfg.photon_grid = 3;
     cfg.electron_grid = 3;
+    cfg.refill_min = 1;
+#endif
     cfg.photon_chunk = m.pchunk;
     cfg.electron_chunk = m.echunk;
-    cfg.refill_min = 1;
     uint64_t sweeps = 0;
     bp->njob = 1;
     bp->total = m.nsrc;
@@ -178,8 +194,8 @@ int main(int argc, char **argv) {
     bp->slot0[0] = 0;
     bp->hist0[0] = m.hist_offset;
     // a small injection size keeps several generations of particles in flight
-    cfg.inject = 256;
-    cfg.threshold = 512;
+    cfg.inject = m.inject ? m.inject : 256;
+    cfg.threshold = m.threshold ? m.threshold : 512;
     cpu_launch(batch_begin, dim3(1), dim3(1), st, 0u, m.nsrc);
     for (;;) {
         launch_sweep(a, cfg, nullptr);
@@ -191,6 +207,10 @@ int main(int argc, char **argv) {
     spill(d + "/out_edep.bin", hist_edep, m.nsrc * sizeof(double));
     spill(d + "/out_steps.bin", hist_steps, m.nsrc * sizeof(uint32_t));
     spill(d + "/out_dose.bin", dose, nvox * m.nbatch * sizeof(double));
+#ifdef GEGS_SIMT
+    printf("sweeps %llu wavefront-ops %llu\n", (unsigned long long)sweeps, (unsigned long long)simt::S.ops);
+#else
     printf("sweeps %llu\n", (unsigned long long)sweeps);
+#endif
     return 0;
 }
