@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -76,6 +77,7 @@ struct gegs_ctx {
     float last_ms = 0.0f;
     EventBanks banks;
     EventConfig tuning;
+    hipError_t scratch_err = hipSuccess; // address of this device's no-alloc scratch
 };
 
 static int fail(gegs_ctx *c, int code, const char *what, hipError_t e = hipSuccess) {
@@ -94,6 +96,34 @@ static int fail(gegs_ctx *c, int code, const char *what, hipError_t e = hipSucce
         hipError_t e_ = (call);                                        \
         if (e_ != hipSuccess) return fail((c), GEGS_ERR_DEVICE, (what), e_); \
     } while (0)
+
+// Device memory of gegs_run_plain_noalloc: one copy of this array per GPU,
+// held from the library's load on that GPU until the process ends.  In host
+// code the name `scratch` is not a device address; the address of each GPU's
+// copy is taken once per device with hipGetSymbolAddress (gegs_create) and
+// only that address is used.  Calls on one GPU share its copy: concurrent
+// no-alloc runs on one device race.
+static __device__ __align__(256) unsigned char scratch[GEGS_NOALLOC_SCRATCH_BYTES];
+static constexpr int MAX_DEVICES = 64;
+static std::mutex scratch_mutex;
+static unsigned char *scratch_addr[MAX_DEVICES];   // per device, nullptr until resolved
+
+// address of this device's scratch copy (the device must be current)
+static hipError_t resolve_scratch(int device) {
+    if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (scratch_addr[device]) return hipSuccess;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    scratch_addr[device] = static_cast<unsigned char *>(ptr);
+    return hipSuccess;
+}
+
+static unsigned char *scratch_of(int device) {
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    return (device >= 0 && device < MAX_DEVICES) ? scratch_addr[device] : nullptr;
+}
 
 template <class T>
 static hipError_t realloc_dev(T *&p, size_t &have, size_t need_elems) {
@@ -119,6 +149,8 @@ gegs_ctx *gegs_create(int device) {
     if (hipSetDevice(device) != hipSuccess) return nullptr;
     gegs_ctx *c = new gegs_ctx();
     c->device = device;
+    // the no-alloc entry point reports a failure here when it is called
+    c->scratch_err = resolve_scratch(device);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&c->d_next, 4096 * sizeof(uint32_t)) != hipSuccess ||
         hipMalloc(&c->d_counters, 16 * sizeof(unsigned long long)) != hipSuccess) {
@@ -434,21 +466,19 @@ static void jaw_polys(const gegs_jaws &g, const float *xy, float sid, gegs_float
     }
 }
 
-// beam geometry, dose buffer and leaf ends of each job, in device memory
-static int upload_jobs(gegs_ctx *c, const std::vector<gegs_field> &fields, const std::vector<uint32_t> &plans,
-                       const std::vector<uint32_t> &leaf_sets = {0u}) {
-    HIPCHECK(c, realloc_dev(c->d_fields, c->fields_cap, fields.size()), "hipMalloc fields");
-    HIPCHECK(c, realloc_dev(c->d_job_plan, c->job_plan_cap, plans.size()), "hipMalloc job plans");
-    HIPCHECK(c, hipMemcpyAsync(c->d_fields, fields.data(), fields.size() * sizeof(gegs_field), hipMemcpyHostToDevice,
-                               c->stream), "copy fields");
-    HIPCHECK(c, hipMemcpyAsync(c->d_job_plan, plans.data(), plans.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
-                               c->stream), "copy job plans");
+// Head geometry of each job: jaw polygons (32 per job) of the fields that
+// start in the jaws and leaf ends (60 x 34 per job) of the fields that pass
+// the MLC.  With polys == ends == nullptr only validates and reports which
+// of the two the jobs need.
+static int job_geometry(gegs_ctx *c, const gegs_field *fields, size_t nf, const uint32_t *leaf_sets, size_t n_given,
+                        bool &any_jaws, bool &any_mlc, gegs_float2 *polys, gegs_float2 *ends) {
     // a field starting in the jaws passes the MLC too when one is loaded
     auto uses_mlc = [&](const gegs_field &f) {
         return f.module_first == GEGS_M_MLC || (f.module_first == GEGS_M_SECJAWY && c->have_mlc);
     };
-    bool any_mlc = false, any_jaws = false;
-    for (const gegs_field &f : fields) {
+    any_jaws = any_mlc = false;
+    for (size_t j = 0; j < nf; ++j) {
+        const gegs_field &f = fields[j];
         any_mlc |= uses_mlc(f);
         any_jaws |= f.module_first == GEGS_M_SECJAWY;
         if (f.module_first == GEGS_M_BLOCK && !c->h_bw.has_block)
@@ -459,32 +489,54 @@ static int upload_jobs(gegs_ctx *c, const std::vector<gegs_field> &fields, const
     if (any_jaws) {
         if (!c->have_jaws) return fail(c, GEGS_ERR_STATE, "run: a field starts in the jaws but no jaws are loaded");
         const size_t nsets = c->jaw_pos.size() / 4;
-        std::vector<gegs_float2> polys(fields.size() * 32);
-        for (size_t j = 0; j < fields.size(); ++j) {
+        for (size_t j = 0; j < nf; ++j) {
             if (fields[j].module_first != GEGS_M_SECJAWY) continue;
-            const uint32_t set = j < leaf_sets.size() ? leaf_sets[j] : 0u;
+            const uint32_t set = j < n_given ? leaf_sets[j] : 0u;
             if (set >= nsets) return fail(c, GEGS_ERR_ARG, "run: a field's jaw set is not loaded");
             if (!(fields[j].source_to_isocenter > 0.0f)) return fail(c, GEGS_ERR_ARG, "run: source_to_isocenter must be > 0");
-            jaw_polys(c->h_jaws, c->jaw_pos.data() + set * 4, fields[j].source_to_isocenter, polys.data() + j * 32);
+            if (polys) jaw_polys(c->h_jaws, c->jaw_pos.data() + set * 4, fields[j].source_to_isocenter, polys + j * 32);
         }
-        HIPCHECK(c, realloc_dev(c->d_jaws, c->jaws_cap, polys.size()), "hipMalloc jaw polygons");
-        HIPCHECK(c, hipMemcpyAsync(c->d_jaws, polys.data(), polys.size() * sizeof(gegs_float2), hipMemcpyHostToDevice,
-                                   c->stream), "copy jaw polygons");
-        HIPCHECK(c, hipStreamSynchronize(c->stream), "copy jaw polygons");
     }
-    std::vector<gegs_float2> ends;
     if (any_mlc) {
         if (!c->have_mlc) return fail(c, GEGS_ERR_STATE, "run: a field starts in the MLC but no MLC is loaded");
         const size_t nsets = c->leaf_pos.size() / (2 * GEGS_MLC_LEAVES);
-        ends.resize(fields.size() * GEGS_MLC_LEAVES * 34);
-        for (size_t j = 0; j < fields.size(); ++j) {
+        for (size_t j = 0; j < nf; ++j) {
             if (!uses_mlc(fields[j])) continue;
-            const uint32_t set = j < leaf_sets.size() ? leaf_sets[j] : 0u;
+            const uint32_t set = j < n_given ? leaf_sets[j] : 0u;
             if (set >= nsets) return fail(c, GEGS_ERR_ARG, "run: a field's leaf set is not loaded");
             if (!(fields[j].source_to_isocenter > 0.0f)) return fail(c, GEGS_ERR_ARG, "run: source_to_isocenter must be > 0");
-            leaf_ends(c->h_mlc, c->leaf_pos.data() + set * 2 * GEGS_MLC_LEAVES, fields[j].source_to_isocenter,
-                      ends.data() + j * GEGS_MLC_LEAVES * 34);
+            if (ends)
+                leaf_ends(c->h_mlc, c->leaf_pos.data() + set * 2 * GEGS_MLC_LEAVES, fields[j].source_to_isocenter,
+                          ends + j * GEGS_MLC_LEAVES * 34);
         }
+    }
+    return GEGS_OK;
+}
+
+// beam geometry, dose buffer and leaf ends of each job, in device memory
+static int upload_jobs(gegs_ctx *c, const std::vector<gegs_field> &fields, const std::vector<uint32_t> &plans,
+                       const std::vector<uint32_t> &leaf_sets = {0u}) {
+    bool any_jaws = false, any_mlc = false;
+    int rc = job_geometry(c, fields.data(), fields.size(), leaf_sets.data(), leaf_sets.size(), any_jaws, any_mlc,
+                          nullptr, nullptr);
+    if (rc) return rc;
+    std::vector<gegs_float2> polys(any_jaws ? fields.size() * 32 : 0);
+    std::vector<gegs_float2> ends(any_mlc ? fields.size() * GEGS_MLC_LEAVES * 34 : 0);
+    rc = job_geometry(c, fields.data(), fields.size(), leaf_sets.data(), leaf_sets.size(), any_jaws, any_mlc,
+                      any_jaws ? polys.data() : nullptr, any_mlc ? ends.data() : nullptr);
+    if (rc) return rc;
+    HIPCHECK(c, realloc_dev(c->d_fields, c->fields_cap, fields.size()), "hipMalloc fields");
+    HIPCHECK(c, realloc_dev(c->d_job_plan, c->job_plan_cap, plans.size()), "hipMalloc job plans");
+    HIPCHECK(c, hipMemcpyAsync(c->d_fields, fields.data(), fields.size() * sizeof(gegs_field), hipMemcpyHostToDevice,
+                               c->stream), "copy fields");
+    HIPCHECK(c, hipMemcpyAsync(c->d_job_plan, plans.data(), plans.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                               c->stream), "copy job plans");
+    if (any_jaws) {
+        HIPCHECK(c, realloc_dev(c->d_jaws, c->jaws_cap, polys.size()), "hipMalloc jaw polygons");
+        HIPCHECK(c, hipMemcpyAsync(c->d_jaws, polys.data(), polys.size() * sizeof(gegs_float2), hipMemcpyHostToDevice,
+                                   c->stream), "copy jaw polygons");
+    }
+    if (any_mlc) {
         HIPCHECK(c, realloc_dev(c->d_endnodes, c->endnodes_cap, ends.size()), "hipMalloc leaf ends");
         HIPCHECK(c, hipMemcpyAsync(c->d_endnodes, ends.data(), ends.size() * sizeof(gegs_float2),
                                    hipMemcpyHostToDevice, c->stream), "copy leaf ends");
@@ -602,6 +654,117 @@ int gegs_run_plain(gegs_ctx *c, const gegs_run_options *o, double *dose, double 
     hipEventRecord(t1, c->stream);
     HIPCHECK(c, hipGetLastError(), "launch plain_kernel");
     return finish_run(c, dose, dose2, t0, t1);
+}
+
+// Layout of the no-alloc scratch for a run on nvox voxels: counters, batch
+// work counters, the field, its plan index, jaw polygons, leaf ends, the
+// batch dose row, then the particle stacks.
+struct NoallocLayout {
+    size_t counters = 0, next = 256, field, plan, polys, ends, chunk, stack;
+    explicit NoallocLayout(size_t nvox) {
+        auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+        field = up(next + 4096 * sizeof(uint32_t));
+        plan = up(field + sizeof(gegs_field));
+        polys = up(plan + sizeof(uint32_t));
+        ends = up(polys + 32 * sizeof(gegs_float2));
+        chunk = up(ends + (size_t)GEGS_MLC_LEAVES * 34 * sizeof(gegs_float2));
+        stack = up(chunk + nvox * sizeof(double));
+    }
+};
+
+int gegs_run_plain_noalloc(gegs_ctx *c, const gegs_run_options *o, double *dose, double *dose2) {
+    int rc = check_ready(c, o);
+    if (rc) return rc;
+    if (!dose || !dose2) return fail(c, GEGS_ERR_ARG, "run_plain_noalloc: dose and dose2 are device buffers of nvox doubles");
+    if (c->hist_tally) return fail(c, GEGS_ERR_STATE, "run_plain_noalloc: per-history tallies need gegs_run_plain");
+    if (c->scratch_err != hipSuccess)
+        return fail(c, GEGS_ERR_DEVICE, "run_plain_noalloc: address of the scratch array", c->scratch_err);
+    HIPCHECK(c, hipSetDevice(c->device), "set device");
+    unsigned char *base = scratch_of(c->device);
+    if (!base) return fail(c, GEGS_ERR_DEVICE, "run_plain_noalloc: no scratch address for this device");
+    const uint32_t block = o->block_size ? o->block_size : 128;
+    if (block != 128) return fail(c, GEGS_ERR_ARG, "run_plain_noalloc: block_size must be 128");
+    const int depth = o->stack_depth ? (int)o->stack_depth : 40;
+    const NoallocLayout L(c->nvox);
+    const size_t per_block = (size_t)block * depth * PART_WORDS * sizeof(float);
+    const size_t cap = GEGS_NOALLOC_SCRATCH_BYTES;
+    if (L.stack + per_block > cap)
+        return fail(c, GEGS_ERR_ARG, "run_plain_noalloc: the phantom's dose row and one block of particle stacks "
+                                     "exceed the scratch size (GEGS_NOALLOC_SCRATCH_BYTES)");
+    const size_t fit = (cap - L.stack) / per_block;
+    uint32_t grid = 2048;
+    if (o->grid_size) {
+        if (o->grid_size > fit)
+            return fail(c, GEGS_ERR_ARG, "run_plain_noalloc: grid_size x stack_depth exceeds the scratch size");
+        grid = o->grid_size;
+    } else if (grid > fit) {
+        grid = (uint32_t)fit;   // histories do not depend on the grid
+    }
+    // field, plan 0 and head geometry of leaf/jaw set 0, as gegs_run_plain
+    bool any_jaws = false, any_mlc = false;
+    gegs_float2 polys[32];
+    gegs_float2 ends[GEGS_MLC_LEAVES * 34];
+    rc = job_geometry(c, &c->field, 1, nullptr, 0, any_jaws, any_mlc, polys, ends);
+    if (rc) return rc;
+    const uint32_t plan0 = 0;
+    HIPCHECK(c, hipMemcpyAsync(base + L.field, &c->field, sizeof(gegs_field), hipMemcpyHostToDevice, c->stream),
+             "copy field");
+    HIPCHECK(c, hipMemcpyAsync(base + L.plan, &plan0, sizeof(plan0), hipMemcpyHostToDevice, c->stream), "copy plan");
+    if (any_jaws)
+        HIPCHECK(c, hipMemcpyAsync(base + L.polys, polys, sizeof(polys), hipMemcpyHostToDevice, c->stream),
+                 "copy jaw polygons");
+    if (any_mlc)
+        HIPCHECK(c, hipMemcpyAsync(base + L.ends, ends, sizeof(ends), hipMemcpyHostToDevice, c->stream),
+                 "copy leaf ends");
+    unsigned long long *counters = reinterpret_cast<unsigned long long *>(base + L.counters);
+    uint32_t *next = reinterpret_cast<uint32_t *>(base + L.next);
+    double *chunk = reinterpret_cast<double *>(base + L.chunk);
+    // the scratch keeps the previous call's contents: clear what starts at zero
+    hipLaunchKernelGGL(fill_zero_u64, dim3(1), dim3(64), 0, c->stream, counters, (size_t)16);
+    hipLaunchKernelGGL(fill_zero_u32, dim3(16), dim3(256), 0, c->stream, next, (size_t)4096);
+    hipLaunchKernelGGL(fill_zero, dim3(2048), dim3(256), 0, c->stream, chunk, c->nvox);
+    hipLaunchKernelGGL(fill_zero, dim3(2048), dim3(256), 0, c->stream, dose, c->nvox);
+    hipLaunchKernelGGL(fill_zero, dim3(2048), dim3(256), 0, c->stream, dose2, c->nvox);
+    PlainArgs a;
+    a.env = make_env(c, o->seed, 1);
+    a.env.fields = reinterpret_cast<const gegs_field *>(base + L.field);
+    a.env.jaws = reinterpret_cast<const gegs_float2 *>(base + L.polys);
+    a.env.endnodes = reinterpret_cast<const gegs_float2 *>(base + L.ends);
+    a.src = c->d_src;
+    a.hist_offset = o->history_offset;
+    a.stack = reinterpret_cast<float *>(base + L.stack);
+    a.depth = depth;
+    a.dose = chunk;
+    a.hist_edep = nullptr;
+    a.hist_steps = nullptr;
+    a.counters = counters;
+    const bool tab = c->nclass <= LDS_MAX_CLASSES;
+    const size_t lds = env_lds_bytes(c->nx, c->ny, c->nz, c->nclass, c->nmed, 1, tab);
+    const uint32_t nb = o->n_batches;
+    const uint64_t r0 = o->first, nr = o->count ? o->count : c->nsrc - o->first;
+    for (uint32_t b = 0; b < nb; ++b) {
+        a.first = (uint32_t)(r0 + (nr * b) / nb);
+        a.last = (uint32_t)(r0 + (nr * (b + 1)) / nb);
+        a.next = next + b;
+        if (tab) hipLaunchKernelGGL(plain_kernel<true>, dim3(grid), dim3(block), lds, c->stream, a);
+        else hipLaunchKernelGGL(plain_kernel<false>, dim3(grid), dim3(block), lds, c->stream, a);
+        hipLaunchKernelGGL(fold_batch, dim3(2048), dim3(256), 0, c->stream, chunk, dose, dose2, c->nvox);
+    }
+    HIPCHECK(c, hipGetLastError(), "launch plain_kernel");
+    HIPCHECK(c, hipStreamSynchronize(c->stream), "run");
+    unsigned long long h[16];
+    HIPCHECK(c, hipMemcpy(h, counters, sizeof(h), hipMemcpyDeviceToHost), "read counters");
+    c->counters.photon_steps = h[0];
+    c->counters.electron_steps = h[1];
+    c->counters.region_gathers = h[2];
+    c->counters.deposits = h[3];
+    for (int k = 0; k < 10; ++k) c->counters.interactions[k] = h[4 + k];
+    c->counters.histories = h[14];
+    c->counters.overflow = h[15];
+    c->last_ms = 0.0f;
+    c->hist_len = 0;
+    if (c->counters.overflow) return fail(c, GEGS_ERR_QUEUE_OVERFLOW, "particle stack overflow: particles were lost");
+    return GEGS_OK;
 }
 
 // One job of an event-queued run: a field, its dose buffer and its records.

@@ -25,6 +25,7 @@ SYMBOLS = [
     "gegs_set_tuning", "gegs_get_tuning", "gegs_run_batched", "gegs_set_beamlets", "gegs_adjoint",
     "gegs_set_kernel_timing", "gegs_kernel_timing", "gegs_load_mlc", "gegs_set_leaf_positions",
     "gegs_load_jaws", "gegs_set_jaw_positions", "gegs_load_block_wedge", "gegs_set_fixes",
+    "gegs_run_plain_noalloc",
 ]
 
 PARTICLE_DTYPE = np.dtype([("e", "<f4"), ("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("u", "<f4"),
@@ -111,6 +112,7 @@ def load():
     L.gegs_set_source.argtypes = [vp, vp, ctypes.c_uint64]
     L.gegs_run_plain.argtypes = [vp, vp, vp, vp]
     L.gegs_run_event.argtypes = [vp, vp, vp, vp]
+    L.gegs_run_plain_noalloc.argtypes = [vp, vp, vp, vp]
     L.gegs_set_history_tally.argtypes = [vp, ctypes.c_int]
     L.gegs_history_tally.argtypes = [vp, vp, vp]
     L.gegs_get_counters.argtypes = [vp, vp]
@@ -357,6 +359,20 @@ class Context:
         n = ctypes.c_uint64()
         self._check(self.L.gegs_kernel_timing(self.h, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+    def run_plain_noalloc(self, seed, dose_ptr, dose2_ptr, n_batches=10, history_offset=0, grid=0, depth=0,
+                          first=0, count=0):
+        """gegs_run_plain_noalloc: the plain kernel with no device allocation.
+        dose_ptr, dose2_ptr: device addresses of nvox float64 each, written in
+        place (the caller's writes to them must be complete).  Returns the
+        run's counters."""
+        self._check(self.L.gegs_set_history_tally(self.h, 0))
+        opt = RunOptions(seed, history_offset, n_batches, 0, grid, depth, first, count)
+        self._check(self.L.gegs_run_plain_noalloc(self.h, ctypes.c_void_p(ctypes.addressof(opt)),
+                                                  ctypes.c_void_p(dose_ptr), ctypes.c_void_p(dose2_ptr)))
+        cnt = Counters()
+        self._check(self.L.gegs_get_counters(self.h, ctypes.byref(cnt)))
+        return cnt.as_dict()
 
     def set_field(self, field):
         self._check(self.L.gegs_set_field(self.h, ctypes.c_void_p(ctypes.addressof(field))))

@@ -268,6 +268,23 @@ int gegs_set_source(gegs_ctx *ctx, const gegs_source *src, uint64_t n);
  * caller forms the batch variance. */
 int gegs_run_plain(gegs_ctx *ctx, const gegs_run_options *opt, double *dose, double *dose2);
 
+/* gegs_run_plain without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  dose and dose2 are DEVICE
+ * buffers of nx*ny*nz doubles each (same contents as gegs_run_plain's); the
+ * field, the head geometry, the batch dose row, the work counters and the
+ * per-thread particle stacks live in a fixed device array of
+ * GEGS_NOALLOC_SCRATCH_BYTES per GPU, held from the library's load until the
+ * process ends.  Limit: the batch row (8 B per voxel) plus one block of
+ * stacks (128 x stack_depth x 64 B) must fit; the grid shrinks to what fits
+ * (histories do not depend on it), and an explicit grid_size that does not
+ * fit is an error.  Per-history tallies are not available (GEGS_ERR_STATE if
+ * enabled) and gegs_last_kernel_ms reports 0.  Calls on one GPU share its
+ * array: calls that overlap on one GPU race.  Nothing is allocated on the
+ * device by this call. */
+#define GEGS_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int gegs_run_plain_noalloc(gegs_ctx *ctx, const gegs_run_options *opt, double *dose, double *dose2);
+
 /* Same transport through the event-queued path: SoA particle banks in HBM,
  * one kernel per event type, wavefront-compacted queues. */
 int gegs_run_event(gegs_ctx *ctx, const gegs_run_options *opt, double *dose, double *dose2);
