@@ -24,11 +24,6 @@
 // exactly the trajectory the plain kernel and the oracle give it.
 #pragma once
 
-// Occupancy floor of a flight kernel: at most 512 / n VGPRs, so that n waves
-// fit on a SIMD (the electron kernel sits at the 3-wave limit of 168 VGPRs).
-#ifndef GEGS_WAVES_PER_EU
-#define GEGS_WAVES_PER_EU(n) __attribute__((amdgpu_waves_per_eu(n)))
-#endif
 // Marks a function whose wavefront operations are shared by exactly the
 // lanes that call it together -- a subset of the wavefront when the call is
 // in divergent code -- unlike the kernels' other wavefront operations, which

@@ -577,7 +577,9 @@ static Env make_env(gegs_ctx *c, uint64_t seed, uint32_t njob) {
     return env;
 }
 
-static int finish_run(gegs_ctx *c, double *dose, double *dose2, hipEvent_t t0, hipEvent_t t1, uint32_t n_plans = 1) {
+// dose and dose2 are host buffers, or device buffers with dev_out
+static int finish_run(gegs_ctx *c, double *dose, double *dose2, hipEvent_t t0, hipEvent_t t1, uint32_t n_plans = 1,
+                      bool dev_out = false) {
     HIPCHECK(c, hipEventSynchronize(t1), "run");
     hipEventElapsedTime(&c->last_ms, t0, t1);
     hipEventDestroy(t0);
@@ -592,8 +594,18 @@ static int finish_run(gegs_ctx *c, double *dose, double *dose2, hipEvent_t t0, h
     c->counters.histories = h[14];
     c->counters.overflow = h[15];
     const size_t n = c->nvox * n_plans;
-    if (dose) HIPCHECK(c, hipMemcpy(dose, c->d_dose, n * sizeof(double), hipMemcpyDeviceToHost), "read dose");
-    if (dose2) HIPCHECK(c, hipMemcpy(dose2, c->d_dose2, n * sizeof(double), hipMemcpyDeviceToHost), "read dose2");
+    if (dev_out) {
+        if (dose)
+            HIPCHECK(c, hipMemcpyAsync(dose, c->d_dose, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream),
+                     "copy dose");
+        if (dose2)
+            HIPCHECK(c, hipMemcpyAsync(dose2, c->d_dose2, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream),
+                     "copy dose2");
+        HIPCHECK(c, hipStreamSynchronize(c->stream), "copy dose");
+    } else {
+        if (dose) HIPCHECK(c, hipMemcpy(dose, c->d_dose, n * sizeof(double), hipMemcpyDeviceToHost), "read dose");
+        if (dose2) HIPCHECK(c, hipMemcpy(dose2, c->d_dose2, n * sizeof(double), hipMemcpyDeviceToHost), "read dose2");
+    }
     if (c->counters.overflow) return fail(c, GEGS_ERR_QUEUE_OVERFLOW, "particle stack overflow: particles were lost");
     return GEGS_OK;
 }
@@ -782,7 +794,7 @@ static constexpr size_t CONTINUOUS_TALLY_BYTES = (size_t)16 << 30;
 
 static int run_event_jobs(gegs_ctx *c, const gegs_run_options *o, const std::vector<gegs_field> &fields,
                           const std::vector<JobSpec> &jobs, uint32_t n_plans, uint64_t n_slots, double *dose,
-                          double *dose2, const BeamletArgs &bl = BeamletArgs()) {
+                          double *dose2, const BeamletArgs &bl = BeamletArgs(), bool dev_out = false) {
     hipSetDevice(c->device);
     // Continuous batching: all statistical batches go through one event loop
     // (one drain at the end instead of one per batch), each deposit tallied
@@ -843,7 +855,7 @@ static int run_event_jobs(gegs_ctx *c, const gegs_run_options *o, const std::vec
     hipLaunchKernelGGL(report_overflow, dim3(1), dim3(1), 0, c->stream, c->banks.st, c->d_counters);
     (void)hipEventRecord(t1, c->stream);
     HIPCHECK(c, hipGetLastError(), "launch event kernels");
-    return finish_run(c, dose, dose2, t0, t1, n_plans);
+    return finish_run(c, dose, dose2, t0, t1, n_plans, dev_out);
 }
 
 int gegs_run_event(gegs_ctx *c, const gegs_run_options *o, double *dose, double *dose2) {
@@ -881,8 +893,8 @@ static int parse_jobs(gegs_ctx *c, const gegs_job *jobs, uint32_t n_jobs, uint32
     return GEGS_OK;
 }
 
-int gegs_run_batched(gegs_ctx *c, const gegs_job *jobs, uint32_t n_jobs, uint32_t n_plans, const gegs_run_options *o,
-                     double *dose, double *dose2) {
+static int run_batched(gegs_ctx *c, const gegs_job *jobs, uint32_t n_jobs, uint32_t n_plans, const gegs_run_options *o,
+                       double *dose, double *dose2, bool dev_out) {
     std::vector<gegs_field> fields;
     std::vector<JobSpec> specs;
     uint64_t slots = 0;
@@ -894,7 +906,17 @@ int gegs_run_batched(gegs_ctx *c, const gegs_job *jobs, uint32_t n_jobs, uint32_
         bl.beamlet = c->d_beamlet;
         bl.bweight = c->d_bweight;
     }
-    return run_event_jobs(c, o, fields, specs, n_plans, slots, dose, dose2, bl);
+    return run_event_jobs(c, o, fields, specs, n_plans, slots, dose, dose2, bl, dev_out);
+}
+
+int gegs_run_batched(gegs_ctx *c, const gegs_job *jobs, uint32_t n_jobs, uint32_t n_plans, const gegs_run_options *o,
+                     double *dose, double *dose2) {
+    return run_batched(c, jobs, n_jobs, n_plans, o, dose, dose2, false);
+}
+
+int gegs_run_batched_device(gegs_ctx *c, const gegs_job *jobs, uint32_t n_jobs, uint32_t n_plans,
+                            const gegs_run_options *o, double *dose, double *dose2) {
+    return run_batched(c, jobs, n_jobs, n_plans, o, dose, dose2, true);
 }
 
 int gegs_set_beamlets(gegs_ctx *c, const uint32_t *beamlet, uint64_t n_slots, const float *weight, uint32_t n_beamlets) {
@@ -922,8 +944,9 @@ int gegs_set_beamlets(gegs_ctx *c, const uint32_t *beamlet, uint64_t n_slots, co
     return GEGS_OK;
 }
 
-int gegs_adjoint(gegs_ctx *c, const gegs_job *jobs, uint32_t n_jobs, uint32_t n_plans, const gegs_run_options *o,
-                 const double *dLdD, double *grad) {
+// dLdD and grad are host buffers, or device buffers with dev_io
+static int adjoint(gegs_ctx *c, const gegs_job *jobs, uint32_t n_jobs, uint32_t n_plans, const gegs_run_options *o,
+                   const double *dLdD, double *grad, bool dev_io) {
     if (!c || !dLdD || !grad) return fail(c, GEGS_ERR_ARG, "gegs_adjoint: null argument");
     std::vector<gegs_field> fields;
     std::vector<JobSpec> specs;
@@ -936,7 +959,9 @@ int gegs_adjoint(gegs_ctx *c, const gegs_job *jobs, uint32_t n_jobs, uint32_t n_
     const size_t n = c->nvox * n_plans;
     HIPCHECK(c, realloc_dev(c->d_adj, c->adj_cap, n), "hipMalloc dL/dD");
     HIPCHECK(c, realloc_dev(c->d_grad, c->grad_cap, c->n_beamlets), "hipMalloc gradient");
-    HIPCHECK(c, hipMemcpy(c->d_adj, dLdD, n * sizeof(double), hipMemcpyHostToDevice), "copy dL/dD");
+    // dL/dD goes to the context's buffer, whose address the captured sweeps keep
+    HIPCHECK(c, hipMemcpyAsync(c->d_adj, dLdD, n * sizeof(double),
+                               dev_io ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream), "copy dL/dD");
     hipLaunchKernelGGL(fill_zero, dim3(64), dim3(256), 0, c->stream, c->d_grad, (size_t)c->n_beamlets);
     BeamletArgs bl;
     bl.beamlet = c->d_beamlet;     // replay at unit beamlet weight: no bweight
@@ -944,7 +969,30 @@ int gegs_adjoint(gegs_ctx *c, const gegs_job *jobs, uint32_t n_jobs, uint32_t n_
     bl.grad = c->d_grad;
     rc = run_event_jobs(c, o, fields, specs, n_plans, slots, nullptr, nullptr, bl);
     if (rc) return rc;
-    HIPCHECK(c, hipMemcpy(grad, c->d_grad, c->n_beamlets * sizeof(double), hipMemcpyDeviceToHost), "read gradient");
+    HIPCHECK(c, hipMemcpyAsync(grad, c->d_grad, c->n_beamlets * sizeof(double),
+                               dev_io ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->stream), "read gradient");
+    HIPCHECK(c, hipStreamSynchronize(c->stream), "read gradient");
+    return GEGS_OK;
+}
+
+int gegs_adjoint(gegs_ctx *c, const gegs_job *jobs, uint32_t n_jobs, uint32_t n_plans, const gegs_run_options *o,
+                 const double *dLdD, double *grad) {
+    return adjoint(c, jobs, n_jobs, n_plans, o, dLdD, grad, false);
+}
+
+int gegs_adjoint_device(gegs_ctx *c, const gegs_job *jobs, uint32_t n_jobs, uint32_t n_plans,
+                        const gegs_run_options *o, const double *dLdD, double *grad) {
+    return adjoint(c, jobs, n_jobs, n_plans, o, dLdD, grad, true);
+}
+
+int gegs_set_beamlet_weights_device(gegs_ctx *c, const float *weight, uint32_t n_beamlets) {
+    if (!c || !weight) return fail(c, GEGS_ERR_ARG, "set_beamlet_weights_device: null argument");
+    if (!c->d_beamlet || n_beamlets != c->n_beamlets)
+        return fail(c, GEGS_ERR_STATE, "set_beamlet_weights_device: set a beamlet map with as many beamlets first");
+    hipSetDevice(c->device);
+    HIPCHECK(c, hipMemcpyAsync(c->d_bweight, weight, n_beamlets * sizeof(float), hipMemcpyDeviceToDevice, c->stream),
+             "copy beamlet weights");
+    HIPCHECK(c, hipStreamSynchronize(c->stream), "copy beamlet weights");
     return GEGS_OK;
 }
 

@@ -107,7 +107,7 @@ __device__ __forceinline__ Part source_particle(const gegs_source &r, int module
 }
 
 template <bool TAB>
-__global__ void __launch_bounds__(128) plain_kernel(PlainArgs a) {
+__global__ void __launch_bounds__(128) GEGS_WAVES_PER_EU(2) plain_kernel(PlainArgs a) {
     extern __shared__ float lds_env[];
     Env env = a.env;
     stage_env<TAB>(env, lds_env);

@@ -30,6 +30,13 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Occupancy floor of a kernel: at most 512 / n registers per lane (VGPRs and
+// AGPRs together), so that n waves fit on a SIMD (the electron kernel sits
+// at the 3-wave limit of 168 VGPRs, the plain kernel at the 2-wave limit).
+#ifndef GEGS_WAVES_PER_EU
+#define GEGS_WAVES_PER_EU(n) __attribute__((amdgpu_waves_per_eu(n)))
+#endif
+
 #include "../../include/gegs.h"
 #include "gmath.hpp"
 

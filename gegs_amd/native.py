@@ -25,7 +25,8 @@ SYMBOLS = [
     "gegs_set_tuning", "gegs_get_tuning", "gegs_run_batched", "gegs_set_beamlets", "gegs_adjoint",
     "gegs_set_kernel_timing", "gegs_kernel_timing", "gegs_load_mlc", "gegs_set_leaf_positions",
     "gegs_load_jaws", "gegs_set_jaw_positions", "gegs_load_block_wedge", "gegs_set_fixes",
-    "gegs_run_plain_noalloc",
+    "gegs_run_plain_noalloc", "gegs_run_batched_device", "gegs_adjoint_device",
+    "gegs_set_beamlet_weights_device",
 ]
 
 PARTICLE_DTYPE = np.dtype([("e", "<f4"), ("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("u", "<f4"),
@@ -135,6 +136,9 @@ def load():
     L.gegs_set_beamlets.argtypes = [vp, vp, ctypes.c_uint64, vp, ctypes.c_uint32]
     L.gegs_adjoint.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp]
     L.gegs_get_tuning.argtypes = [vp, vp]
+    L.gegs_run_batched_device.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp]
+    L.gegs_adjoint_device.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp]
+    L.gegs_set_beamlet_weights_device.argtypes = [vp, vp, ctypes.c_uint32]
     for name in SYMBOLS:
         if getattr(L, name, None) is None:
             raise RuntimeError("symbol %s missing from %s" % (name, LIB_PATH))
@@ -300,6 +304,7 @@ class Context:
     def set_beamlets(self, beamlet=None, weight=None):
         """Beamlet index per history slot (job after job) and weight per
         beamlet; None removes the map."""
+        self._beamlet_owner = None
         if beamlet is None:
             self._check(self.L.gegs_set_beamlets(self.h, None, 0, None, 0))
             return
@@ -318,6 +323,25 @@ class Context:
         self._check(self.L.gegs_adjoint(self.h, self._jobs(jobs), len(jobs), n_plans, ctypes.byref(opt),
                                         _p(g), _p(grad)))
         return grad
+
+    def set_beamlet_weights_device(self, weight_ptr, n_beamlets):
+        """Weights of the current beamlet map from device memory (float32)."""
+        self._check(self.L.gegs_set_beamlet_weights_device(self.h, ctypes.c_void_p(weight_ptr), n_beamlets))
+
+    def run_batched_device(self, jobs, n_plans, seed, n_batches, dose_ptr, dose2_ptr=0):
+        """run_batched writing dose (and dose2 unless 0) into device buffers
+        of n_plans * nvox float64."""
+        opt = RunOptions(seed, 0, n_batches, 0, 0, 0, 0, 0)
+        self._check(self.L.gegs_set_history_tally(self.h, 0))
+        self._check(self.L.gegs_run_batched_device(self.h, self._jobs(jobs), len(jobs), n_plans, ctypes.byref(opt),
+                                                   ctypes.c_void_p(dose_ptr), ctypes.c_void_p(dose2_ptr or None)))
+
+    def adjoint_device(self, jobs, n_plans, seed, dLdD_ptr, grad_ptr, n_batches=10):
+        """adjoint with dL/dD (n_plans * nvox float64) and the gradient (one
+        float64 per beamlet of the current map) in device memory."""
+        opt = RunOptions(seed, 0, n_batches, 0, 0, 0, 0, 0)
+        self._check(self.L.gegs_adjoint_device(self.h, self._jobs(jobs), len(jobs), n_plans, ctypes.byref(opt),
+                                               ctypes.c_void_p(dLdD_ptr), ctypes.c_void_p(grad_ptr)))
 
     def run_batched(self, jobs, n_plans, seed, n_batches=10, per_history=False):
         """Event-queued transport of several fields/plans in one loop.

@@ -316,6 +316,21 @@ int gegs_set_beamlets(gegs_ctx *ctx, const uint32_t *beamlet, uint64_t n_slots, 
 int gegs_adjoint(gegs_ctx *ctx, const gegs_job *jobs, uint32_t n_jobs, uint32_t n_plans,
                  const gegs_run_options *opt, const double *dLdD, double *grad);
 
+/* Device-resident forms of gegs_run_batched, gegs_adjoint and the beamlet
+ * weights, for callers that keep dose, dL/dD, weights and gradients in
+ * device memory (the torch.autograd.Function binding): dose, dose2, dLdD,
+ * grad and weight are DEVICE buffers of the same sizes; the calls return
+ * once the results are in them.  The caller's writes to the inputs must be
+ * complete before the call (the library works on its own stream).  No
+ * counterpart in the reference, which has no differentiable path.
+ * gegs_set_beamlet_weights_device replaces the weights of the map set by
+ * gegs_set_beamlets (same n_beamlets). */
+int gegs_run_batched_device(gegs_ctx *ctx, const gegs_job *jobs, uint32_t n_jobs, uint32_t n_plans,
+                            const gegs_run_options *opt, double *dose, double *dose2);
+int gegs_adjoint_device(gegs_ctx *ctx, const gegs_job *jobs, uint32_t n_jobs, uint32_t n_plans,
+                        const gegs_run_options *opt, const double *dLdD, double *grad);
+int gegs_set_beamlet_weights_device(gegs_ctx *ctx, const float *weight, uint32_t n_beamlets);
+
 /* Tuning of gegs_run_event; NULL restores the defaults.  gegs_get_tuning
  * returns the values in force with defaults filled in (grid sizes that fill
  * the device are resolved at run time and reported as 0). */

@@ -113,4 +113,67 @@ def test_torch_autograd_function(gpu_ctx, ct):
     g_direct = plan.adjoint(2.0 * m_np * (d.detach().cpu().numpy() - p_np))
     assert w.grad is not None and w.grad.shape == w.shape
     assert np.allclose(w.grad.cpu().numpy(), g_direct.astype(np.float32), rtol=1e-5, atol=1e-6 * np.abs(g_direct).max())
+    # the device-resident forward gives the host path's dose
+    d_host = plan.forward(w0)
+    assert d.device.type == "cuda" and d.dtype == torch.float64
+    assert np.allclose(d.detach().cpu().numpy(), d_host, rtol=1e-12, atol=1e-12 * d_host.max())
     gpu_ctx.set_beamlets(None)
+
+
+def test_torch_autograd_plan_queue(gpu_ctx, ct):
+    """Three plans of one or two fields in one BeamletDose: dose (3, nvox) on
+    the device, and the gradient of a sum of per-plan objectives equals the
+    host-side adjoint with one dL/dD row per plan."""
+    import torch
+    sid = ct.field.source_to_isocenter
+    f = [workloads.make_field(True, sid, g) for g in (0.0, 90.0, 180.0, 270.0)]
+    n = 15000
+    jobs = [dict(field=f[0], plan=0, first=0, count=n), dict(field=f[1], plan=1, first=n, count=n),
+            dict(field=f[2], plan=1, first=2 * n, count=n), dict(field=f[3], plan=2, first=3 * n, count=n)]
+    src = ct.source(4 * n, 48)
+    gpu_ctx.load_workload(ct)
+    gpu_ctx.set_source(src)
+    q = beamlets.BeamletPlan.from_jobs(gpu_ctx, jobs, 3, src, n_side=20, pitch=0.5, seed=5, n_batches=2)
+    assert q.n_beamlets == 1600 and list(q.plan_beamlets(1)) == list(range(400, 1200))
+    w0 = np.random.default_rng(7).uniform(0.5, 1.5, q.n_beamlets)
+    w = torch.tensor(w0, dtype=torch.float32, device="cuda", requires_grad=True)
+    d = beamlets.BeamletDose.apply(w, q)
+    assert tuple(d.shape) == (3, gpu_ctx.nvox)
+    dn = d.detach().cpu().numpy()
+    ms, ps = zip(*(_objective(ct, dn[k]) for k in range(3)))
+    m = torch.from_numpy(np.stack(ms)).cuda()
+    p = torch.from_numpy(np.stack(ps)).cuda()
+    ((m * (d - p) ** 2).sum()).backward()
+    g_host = q.adjoint(2.0 * np.stack(ms) * (dn - np.stack(ps)))
+    assert np.allclose(w.grad.cpu().numpy(), g_host.astype(np.float32), rtol=1e-5, atol=1e-6 * np.abs(g_host).max())
+    gpu_ctx.set_beamlets(None)
+
+
+def test_phase_space_weight_gradient(gpu_ctx, ct):
+    """dL/dwt of single source records (each history its own beamlet) against
+    central differences of L when one record's weight changes, with common
+    random numbers (the dose is linear in each record's weight)."""
+    n = 20000
+    src = ct.source(n, 47)
+    gpu_ctx.load_workload(ct)
+    gpu_ctx.set_source(src)
+    jobs = [dict(field=ct.field, plan=0, first=0, count=n)]
+    gpu_ctx.set_beamlets(None)
+    d = gpu_ctx.run_batched(jobs, 1, seed=8, n_batches=2)["dose"][0]
+    m, p = _objective(ct, d)
+    g = beamlets.phase_space_weight_gradient(gpu_ctx, jobs, 1, src, 8, 2.0 * m * (d - p), n_batches=2)
+    assert g.shape == (n,)
+    gpu_ctx.set_beamlets(None)
+    rng = np.random.default_rng(9)
+    picks = rng.choice(np.flatnonzero(np.abs(g) > 1e-2 * np.abs(g).max()), 6, replace=False)
+    for i in picks:
+        h = 0.25 * float(src["wt"][i])
+        ls = []
+        for sgn in (1.0, -1.0):
+            s2 = src.copy()
+            s2["wt"][i] = np.float32(float(src["wt"][i]) + sgn * h)
+            gpu_ctx.set_source(s2)
+            ls.append(_loss(gpu_ctx.run_batched(jobs, 1, seed=8, n_batches=2)["dose"][0], m, p))
+        fd = (ls[0] - ls[1]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-4 * abs(g[i]) + 1e-9 * np.abs(g).max(), (i, fd, g[i])
+    gpu_ctx.set_source(src)

@@ -224,3 +224,59 @@ def test_plan_queue_1e8_batched_equals_per_field_plain(plan_queue):
     _same_counters(cnt, b["counters"])
     for k in range(len(mp.plans)):
         _close(dose[k], b["dose"][k])
+
+
+def test_plan_queue_per_plan_adjoint(plan_queue):
+    """The 16-plan queue on the 256 x 256 x 180 CT, 20 x 20 beamlets of 5 mm
+    per field (every field of every plan), history budgets scaled to 1/100:
+    one replay of all plans with one dL/dD row per plan (target sphere and
+    OAR shell of each plan's own dose).  Each plan's gradient equals the
+    adjoint of that plan's fields alone, and central differences of its
+    objective with common random numbers on two beamlets agree."""
+    from gegs_amd import beamlets
+    mp = plan_queue
+    wl = mp.workload
+    jobs, first = [], 0
+    for jb in mp.jobs():
+        n = max(2000, jb["count"] // 100)
+        jobs.append(dict(field=jb["field"], plan=jb["plan"], first=first, count=n))
+        first += n
+    src = wl.source(first, 61)
+    n_plans = len(mp.plans)
+    rng = np.random.default_rng(12)
+    ctx = native.Context(0)
+    try:
+        ctx.load_workload(wl)
+        ctx.set_source(src)
+        print("plan-queue adjoint: %d jobs, %d plans, %d histories" % (len(jobs), n_plans, first), flush=True)
+        q = beamlets.BeamletPlan.from_jobs(ctx, jobs, n_plans, src, n_side=20, pitch=0.5, seed=31, n_batches=2)
+        assert q.n_beamlets == 400 * len(jobs)
+        w = rng.uniform(0.5, 1.5, q.n_beamlets)
+        d = q.forward(w)
+        assert d.shape == (n_plans, wl.phantom.nvox)
+        target, oar = workloads.target_and_oar(wl.phantom)
+        m = (target | oar).astype(np.float64)
+        P = np.where(target[None, :], 1.2 * d[:, target].mean(axis=1)[:, None], 0.0)
+        dLdD = 2.0 * m[None, :] * (d - P)
+        g = q.adjoint(dLdD)
+        for k in range(n_plans):
+            bk = q.plan_beamlets(k)
+            jk = [dict(j, plan=0, history_offset=j["first"]) for j in jobs if j["plan"] == k]
+            pk = beamlets.BeamletPlan.from_jobs(ctx, jk, 1, src, n_side=20, pitch=0.5, seed=31, n_batches=2)
+            gk = pk.adjoint(dLdD[k])
+            scale = np.abs(gk).max()
+            assert scale > 0, k
+            np.testing.assert_allclose(g[bk], gk, rtol=1e-9, atol=1e-12 * scale)
+            hit = np.flatnonzero(np.abs(gk) > 1e-2 * scale)
+            h = 0.1
+            for b in rng.choice(hit, 2, replace=False):
+                wk = w[bk]
+                wp, wm = wk.copy(), wk.copy()
+                wp[b] += h
+                wm[b] -= h
+                lp = float(np.sum(m * (pk.forward(wp) - P[k]) ** 2))
+                lm = float(np.sum(m * (pk.forward(wm) - P[k]) ** 2))
+                fd = (lp - lm) / (2 * h)
+                assert abs(fd - gk[b]) <= 1e-4 * abs(gk[b]) + 1e-9 * scale, (k, b, fd, gk[b])
+    finally:
+        ctx.close()

@@ -253,8 +253,10 @@ class FakeContext:
         return dict(dose=dose, counters=dict(overflow=0))
 
     def run(self, seed, n_batches=10, history_offset=0, first=0, count=0):
+        # the plain kernel's history index is history_offset + record index
+        # (gegs_run_plain), added modulo 2^64
         dose = np.zeros(self.nvox)
-        self._deposit(dose, first, count, history_offset)
+        self._deposit(dose, first, count, (history_offset + first) % (1 << 64))
         return dict(dose=dose)
 
 
