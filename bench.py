@@ -53,17 +53,23 @@ ROWS_PER_ELECTRON_STEP = 6
 PHOTON_STEP_BYTES = REGION_BYTES + ROWS_PER_PHOTON_STEP * TABLE_ROW_BYTES
 
 
-def measured_traffic_per_photon_step():
-    """HBM bytes per photon step of the photon kernel from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
+def profile_figure(key):
+    """A figure of the committed rocprofv3 summary of the photon kernel
+    (profiles/photon_kernel_ct128_event.md, a line `key: value`), or None:
+    hbm_bytes_per_photon_step (FETCH_SIZE / WRITE_SIZE, copy-calibrated) and
+    valu_issue_frac (SQ_ACTIVE_INST_VALU over the SIMDs' cycles)."""
     import re
     path = os.path.join(ROOT, "profiles", "photon_kernel_ct128_event.md")
     try:
         with open(path) as f:
-            m = re.search(r"^hbm_bytes_per_photon_step: ([0-9.]+)$", f.read(), re.M)
+            m = re.search(r"^%s: ([0-9.]+)$" % re.escape(key), f.read(), re.M)
     except OSError:
         return None
     return float(m.group(1)) if m else None
+
+
+def measured_traffic_per_photon_step():
+    return profile_figure("hbm_bytes_per_photon_step")
 
 
 def algorithmic_bytes(counters, n_hist):
@@ -74,7 +80,18 @@ def algorithmic_bytes(counters, n_hist):
 
 
 def cpu_baseline(wl, budget_s, threads):
+    """The oracle, built here for this host's CPU (gcc -O3 -march=native, no
+    fast-math), on `threads` OpenMP threads over a bounded sample of the
+    workload (about budget_s seconds)."""
+    import subprocess
+    import tempfile
     from oracle import oracle
+    build = "gcc -O3 -march=native"
+    try:
+        oracle.use_library(oracle.build_native(tempfile.mkdtemp(prefix="gegs_oracle_")))
+    except (OSError, subprocess.CalledProcessError):
+        oracle.use_library(None)
+        build = "gcc -O3 -march=x86-64-v3, prebuilt (no compiler on this host)"
     probe = wl.source(50000 * threads, 991)
     t0 = time.perf_counter()
     oracle.run(wl, probe, seed=5, threads=threads)
@@ -85,8 +102,8 @@ def cpu_baseline(wl, budget_s, threads):
     oracle.run(wl, sample, seed=6, threads=threads)
     dt = time.perf_counter() - t0
     return dict(value=n / dt, unit="histories/s", cores=threads, kind="port",
-                sample="%d histories of the same CT workload (seed 992), oracle/liboracle.so "
-                       "(C restatement, gcc -O2, OpenMP over histories), %.1f s" % (n, dt))
+                sample="%d histories of the same CT workload (seed 992), the oracle "
+                       "(C restatement, %s, OpenMP over histories), %.1f s" % (n, build, dt))
 
 
 def cpu_detail(wl, threads, seconds=4.0):
@@ -246,6 +263,57 @@ def mlc_plan_bench(n_total, n_batches, seed, autotune_first=True, fixed_tuning=N
     return out
 
 
+def mlc_plan_at_one_percent(n_target, n_records, n_batches, seed, tuning, plain_efficiency):
+    """The batched 5-field MLC plan run to the stated uncertainty: its
+    n_records phase-space records recycled R times (R even, each pass with
+    its own Philox history range, as the reference's nRepeat recycling), in
+    two independent halves of R/2 passes and n_batches/2 statistical
+    batches each.  sigma comes from the pooled n_batches batches; its spread
+    from the two halves' own estimates, each scaled to the whole run.  The
+    plain kernel's dose efficiency 1/(sigma^2 T) does not depend on the
+    number of histories (sigma^2 ~ 1/N, T ~ N), so the ratio takes the plain
+    runs' efficiency at their own scale."""
+    import math
+    from gegs_amd import native, workloads
+    plan = workloads.mlc_plan()
+    src = plan.source(n_records, 3)
+    reps = 2 * int(math.ceil(n_target / (2.0 * n_records)))
+    ctx = native.Context(0)
+    ctx.load_workload(plan.workload)
+    ctx.set_source(src)
+    if tuning:
+        ctx.set_tuning(tuning)
+    ctx.set_leaf_positions(plan.leaf_positions)
+    base = plan.jobs(n_records)
+    half_b = n_batches // 2
+    # warm-up: graph capture on a small slice
+    ctx.run_batched(plan.jobs(n_records, 0, 100), 1, seed=seed, n_batches=half_b)
+    halves = []
+    for h in range(2):
+        jobs = [dict(jb, history_offset=r * n_records + jb["first"])
+                for r in range(h * reps // 2, (h + 1) * reps // 2) for jb in base]
+        _sync()
+        t0 = time.perf_counter()
+        out = ctx.run_batched(jobs, 1, seed=seed, n_batches=half_b)
+        _sync()
+        halves.append((time.perf_counter() - t0, out["dose"][0], out["dose2"][0], out["counters"]["overflow"]))
+    ctx.close()
+    t = halves[0][0] + halves[1][0]
+    dose = halves[0][1] + halves[1][1]
+    dose2 = halves[0][2] + halves[1][2]
+    sig, nsel, _ = plan_sigma(dose, dose2, 2 * half_b)
+    half_sig = [plan_sigma(hv[1], hv[2], half_b)[0] / math.sqrt(2.0) for hv in halves]
+    n_hist = reps * n_records
+    eff = 1.0 / (sig * sig * t)
+    return dict(histories=n_hist, record_passes=reps, n_batches=2 * half_b, seconds=t, histories_per_s=n_hist / t,
+                mean_rel_sigma=sig, voxels=nsel, dose_efficiency=eff,
+                sigma_from_each_half_scaled=half_sig,
+                sigma_spread=abs(half_sig[0] - half_sig[1]) / sig,
+                overflow=int(halves[0][3] + halves[1][3]),
+                batched_vs_plain_efficiency=eff / plain_efficiency if plain_efficiency else None,
+                plain_reference="plain per-field runs of mlc_plan at their own scale (efficiency is N-independent)")
+
+
 def adjoint_rate(n_hist, seed, repeat=2):
     """Gradients/s of the beamlet adjoint on the dose-optimisation workload:
     the 128^3 CT at 2.5 mm, 5 fields (gantry 0/72/144/216/288) x 20x20
@@ -284,8 +352,12 @@ def plan_queue_bench(total, n_batches, seed, tuning=None):
     the 256 x 256 x 180 CT at 2 mm: all 93 fields in one batched run (one
     dose buffer per plan) against the plain kernel field after field.  Same
     histories and streams on both sides; the per-plan uncertainty comes from
-    each method's batch variance."""
-    from gegs_amd import native, workloads
+    each method's batch variance.  Then the queue's beamlet-weight gradient
+    (20 x 20 beamlets of 5 mm per field): one replay of all `total`
+    histories with one dL/dD row per plan (target sphere and OAR shell of
+    each plan's own dose), dL/dD and the gradient in device memory."""
+    import torch
+    from gegs_amd import beamlets, native, workloads
     mp = workloads.multi_plan(total=total)
     src = mp.source(5)
     jobs = mp.jobs()
@@ -293,6 +365,7 @@ def plan_queue_bench(total, n_batches, seed, tuning=None):
     ctx = native.Context(0)
     ctx.load_workload(mp.workload)
     ctx.set_source(src)
+    queue = beamlets.BeamletPlan.from_jobs(ctx, jobs, nplan, src, n_side=20, pitch=0.5, seed=seed, n_batches=2)
     del src
     if tuning:
         ctx.set_tuning(tuning)
@@ -318,6 +391,19 @@ def plan_queue_bench(total, n_batches, seed, tuning=None):
         var[jb["plan"]] += v
     _sync()
     t_p = time.perf_counter() - t0
+    target, oar = workloads.target_and_oar(mp.workload.phantom)
+    m = (target | oar).astype(np.float64)
+    P = np.where(target[None, :], 1.2 * b["dose"][:, target].mean(axis=1)[:, None], 0.0)
+    dLdD = torch.from_numpy(2.0 * m[None, :] * (b["dose"] - P)).cuda()
+    del P
+    queue.adjoint_device(dLdD)              # graph capture / warm caches
+    _sync()
+    t0 = time.perf_counter()
+    g = queue.adjoint_device(dLdD)
+    _sync()
+    t_a = time.perf_counter() - t0
+    nonzero = int(torch.count_nonzero(g).item())
+    del dLdD, g
     ctx.close()
     ratio, sig_b, sig_p = [], [], []
     for k in range(nplan):
@@ -333,7 +419,10 @@ def plan_queue_bench(total, n_batches, seed, tuning=None):
                 plain=dict(seconds=t_p, histories_per_s=mp.n_histories / t_p),
                 batched_vs_plain_time=t_p / t_b, batched_vs_plain_efficiency_mean_over_plans=float(np.mean(ratio)),
                 mean_rel_sigma_batched=float(np.mean(sig_b)), mean_rel_sigma_plain=float(np.mean(sig_p)),
-                max_rel_dose_difference=rel)
+                max_rel_dose_difference=rel,
+                adjoint=dict(seconds=t_a, queue_gradients_per_s=1.0 / t_a, plan_gradients_per_s=nplan / t_a,
+                             histories_per_s=mp.n_histories / t_a, beamlets=queue.n_beamlets, nonzero=nonzero,
+                             objective="per plan: target sphere r 3 cm (1.2 x its mean dose) + OAR shell 4-5 cm"))
 
 
 def main():
@@ -352,6 +441,8 @@ def main():
                     help="histories per adjoint gradient (0: skip)")
     ap.add_argument("--mlc-plan-histories", type=int, default=20_000_000,
                     help="histories of the 5-field MLC plan comparison (0: skip)")
+    ap.add_argument("--mlc-plan-1pct-histories", type=int, default=1_700_000_000,
+                    help="histories of the batched 5-field MLC plan run to about 1%% mean uncertainty (0: skip)")
     ap.add_argument("--plan-queue-histories", type=int, default=100_000_000,
                     help="histories of the 16-plan queue comparison on the 256x256x180 CT (0: skip)")
     ap.add_argument("--seed", type=int, default=20240)
@@ -432,6 +523,11 @@ def main():
     if args.mlc_plan_histories:
         mlc_plan = mlc_plan_bench(args.mlc_plan_histories, args.steps, args.seed, not args.no_autotune,
                                   json.loads(args.tuning) if args.tuning else None)
+    one_pct = None
+    if args.mlc_plan_1pct_histories and mlc_plan:
+        one_pct = mlc_plan_at_one_percent(args.mlc_plan_1pct_histories, args.mlc_plan_histories, 100, args.seed,
+                                          mlc_plan.get("tuning"), mlc_plan["plain"]["dose_efficiency"])
+        mlc_plan["at_1pct"] = one_pct
     queue = None
     if args.plan_queue_histories:
         queue = plan_queue_bench(args.plan_queue_histories, args.steps, args.seed, tuning)
@@ -447,7 +543,10 @@ def main():
                 "photon_kernel_share": rf["photon_kernel_share"],
                 "timed_pass_histories_per_s": rf["timed_pass_histories_per_s"],
                 "timing": "HIP events around every photon kernel launch over a second pass of the same steps "
-                          "(sweeps launched on the stream; the value pass replays graphs)"}
+                          "(sweeps launched on the stream; the value pass replays graphs)",
+                "valu_issue_frac": profile_figure("valu_issue_frac"),
+                "limiter": "VALU issue: the SIMDs issue VALU most of the kernel's cycles (valu_issue_frac, "
+                           "SQ_ACTIVE_INST_VALU in the committed profile); real HBM traffic is traffic/launch"}
     else:
         roof = None
     line = {

@@ -483,7 +483,7 @@ struct HoldSink {
 // crossings is suspended into the next sweep's bank.  The photons of ended
 // flights are appended together when the wavefront refills (or runs dry),
 // so that one atomic per destination serves many lanes.
-template <bool TAB, bool ADJ>
+template <bool TAB, bool ADJ, bool GEN>
 __global__ void __launch_bounds__(512) photon_kernel(EvArgs a, uint32_t chunk, uint32_t refill_min,
                                                      uint32_t refill_slots) {
     extern __shared__ float lds_env[];
@@ -494,6 +494,7 @@ __global__ void __launch_bounds__(512) photon_kernel(EvArgs a, uint32_t chunk, u
     const uint32_t n = view.total();
     if (n == 0) return;
     Env env = a.env;
+    env.fixes = kernel_fixes<GEN>(env.fixes);
     stage_env<TAB>(env, lds_env);
     TallyT<ADJ> t = fresh_tally<ADJ>(a);
     BankSink sink{&a, 0, flip, false, true};
@@ -546,7 +547,7 @@ __global__ void __launch_bounds__(512) photon_kernel(EvArgs a, uint32_t chunk, u
 
 // electron / positron condensed-history steps with lane refill; an electron
 // still stepping after `chunk` steps is suspended into the next sweep's bank
-template <bool TAB, bool ADJ>
+template <bool TAB, bool ADJ, bool GEN>
 __global__ void __launch_bounds__(512) GEGS_WAVES_PER_EU(3) electron_kernel(EvArgs a, uint32_t chunk, uint32_t refill_min,
                                                        uint32_t refill_slots) {
     extern __shared__ float lds_env[];
@@ -557,6 +558,7 @@ __global__ void __launch_bounds__(512) GEGS_WAVES_PER_EU(3) electron_kernel(EvAr
     const uint32_t n = view.total();
     if (n == 0) return;
     Env env = a.env;
+    env.fixes = kernel_fixes<GEN>(env.fixes);
     stage_env<TAB>(env, lds_env);
     TallyT<ADJ> t = fresh_tally<ADJ>(a);
     BankSink sink{&a, 0, flip, false, false};
@@ -837,7 +839,7 @@ inline size_t flight_lds(const Env &env) {
     return env_lds_bytes(env.nx, env.ny, env.nz, env.nclass, env.nmed, env.njob, tab_in_lds(env));
 }
 
-template <bool TAB, bool ADJ>
+template <bool TAB, bool ADJ, bool GEN>
 inline void launch_sweep_t(const EvArgs &a, const EventConfig &cfg, hipStream_t stream, hipEvent_t t0,
                            hipEvent_t t1) {
     const dim3 g(cfg.grid), b(cfg.block), fb(cfg.flight_block);
@@ -845,14 +847,14 @@ inline void launch_sweep_t(const EvArgs &a, const EventConfig &cfg, hipStream_t 
     hipLaunchKernelGGL(inject_plan, dim3(1), dim3(1), 0, stream, a.st, cfg.threshold, cfg.inject);
     hipLaunchKernelGGL(inject_fill, g, b, 0, stream, a);
     if (t0) (void)hipEventRecord(t0, stream);
-    hipLaunchKernelGGL((photon_kernel<TAB, ADJ>), dim3(cfg.photon_grid), fb, lds, stream, a, cfg.photon_chunk,
+    hipLaunchKernelGGL((photon_kernel<TAB, ADJ, GEN>), dim3(cfg.photon_grid), fb, lds, stream, a, cfg.photon_chunk,
                        cfg.refill_min, cfg.refill_slots);
     if (t1) (void)hipEventRecord(t1, stream);
     hipLaunchKernelGGL((interaction_kernel<P_RAYLEIGH, ADJ>), g, b, 0, stream, a);
     hipLaunchKernelGGL((interaction_kernel<P_COMPTON, ADJ>), g, b, 0, stream, a);
     hipLaunchKernelGGL((interaction_kernel<P_PHOTO, ADJ>), g, b, 0, stream, a);
     hipLaunchKernelGGL((interaction_kernel<P_PAIR, ADJ>), g, b, 0, stream, a);
-    hipLaunchKernelGGL((electron_kernel<TAB, ADJ>), dim3(cfg.electron_grid), fb, lds, stream, a, cfg.electron_chunk,
+    hipLaunchKernelGGL((electron_kernel<TAB, ADJ, GEN>), dim3(cfg.electron_grid), fb, lds, stream, a, cfg.electron_chunk,
                        cfg.refill_min, cfg.refill_slots);
     hipLaunchKernelGGL((interaction_kernel<E_MOLLER, ADJ>), g, b, 0, stream, a);
     hipLaunchKernelGGL((interaction_kernel<E_BREMS, ADJ>), g, b, 0, stream, a);
@@ -861,34 +863,47 @@ inline void launch_sweep_t(const EvArgs &a, const EventConfig &cfg, hipStream_t 
     hipLaunchKernelGGL(sweep_end, dim3(1), dim3(1), 0, stream, a.st);
 }
 
-inline void launch_sweep(const EvArgs &a, const EventConfig &cfg, hipStream_t stream, hipEvent_t t0 = nullptr,
-                         hipEvent_t t1 = nullptr) {
+template <bool GEN>
+inline void launch_sweep_f(const EvArgs &a, const EventConfig &cfg, hipStream_t stream, hipEvent_t t0,
+                           hipEvent_t t1) {
     const bool adj = a.adj != nullptr;
     if (tab_in_lds(a.env)) {
-        if (adj) launch_sweep_t<true, true>(a, cfg, stream, t0, t1);
-        else launch_sweep_t<true, false>(a, cfg, stream, t0, t1);
+        if (adj) launch_sweep_t<true, true, GEN>(a, cfg, stream, t0, t1);
+        else launch_sweep_t<true, false, GEN>(a, cfg, stream, t0, t1);
     } else {
-        if (adj) launch_sweep_t<false, true>(a, cfg, stream, t0, t1);
-        else launch_sweep_t<false, false>(a, cfg, stream, t0, t1);
+        if (adj) launch_sweep_t<false, true, GEN>(a, cfg, stream, t0, t1);
+        else launch_sweep_t<false, false, GEN>(a, cfg, stream, t0, t1);
     }
 }
 
-template <bool TAB, bool ADJ>
+inline void launch_sweep(const EvArgs &a, const EventConfig &cfg, hipStream_t stream, hipEvent_t t0 = nullptr,
+                         hipEvent_t t1 = nullptr) {
+    if (a.env.fixes & ENV_GENERAL) launch_sweep_f<true>(a, cfg, stream, t0, t1);
+    else launch_sweep_f<false>(a, cfg, stream, t0, t1);
+}
+
+template <bool TAB, bool ADJ, bool GEN>
 inline void flight_grids_t(EventConfig &cfg, size_t lds) {
-    if (!cfg.photon_grid) cfg.photon_grid = resident_grid(photon_kernel<TAB, ADJ>, cfg.flight_block, lds);
-    if (!cfg.electron_grid) cfg.electron_grid = resident_grid(electron_kernel<TAB, ADJ>, cfg.flight_block, lds);
+    if (!cfg.photon_grid) cfg.photon_grid = resident_grid(photon_kernel<TAB, ADJ, GEN>, cfg.flight_block, lds);
+    if (!cfg.electron_grid) cfg.electron_grid = resident_grid(electron_kernel<TAB, ADJ, GEN>, cfg.flight_block, lds);
+}
+
+template <bool GEN>
+inline void flight_grids_f(EventConfig &cfg, const Env &env, bool adj) {
+    const size_t lds = flight_lds(env);
+    if (tab_in_lds(env)) {
+        if (adj) flight_grids_t<true, true, GEN>(cfg, lds);
+        else flight_grids_t<true, false, GEN>(cfg, lds);
+    } else {
+        if (adj) flight_grids_t<false, true, GEN>(cfg, lds);
+        else flight_grids_t<false, false, GEN>(cfg, lds);
+    }
 }
 
 // grid sizes of the persistent flight kernels that fill the device once
 inline void flight_grids(EventConfig &cfg, const Env &env, bool adj) {
-    const size_t lds = flight_lds(env);
-    if (tab_in_lds(env)) {
-        if (adj) flight_grids_t<true, true>(cfg, lds);
-        else flight_grids_t<true, false>(cfg, lds);
-    } else {
-        if (adj) flight_grids_t<false, true>(cfg, lds);
-        else flight_grids_t<false, false>(cfg, lds);
-    }
+    if (env.fixes & ENV_GENERAL) flight_grids_f<true>(cfg, env, adj);
+    else flight_grids_f<false>(cfg, env, adj);
 }
 
 inline bool same_args(const EvArgs &x, const EvArgs &y) {

@@ -52,6 +52,7 @@ struct gegs_ctx {
     gegs_block_wedge h_bw{};             // block and wedge geometry
     gegs_float2 *d_bw = nullptr;         // their 18 vertices on the device
     uint32_t fixes = 0;                  // GEGS_FIX_* mask
+    uint32_t media_opts = 0;             // ENV_PRESTA2 / ENV_EXACT_BCA of the loaded tables
     size_t jaws_cap = 0;
     uint32_t *d_beamlet = nullptr;       // beamlet of each history slot
     float *d_bweight = nullptr;          // weight of each beamlet
@@ -199,11 +200,18 @@ const char *gegs_last_error(gegs_ctx *c) { return c ? c->err.c_str() : "no conte
 int gegs_load_media(gegs_ctx *c, const gegs_media_tables *t) {
     if (!c || !t) return fail(c, GEGS_ERR_ARG, "gegs_load_media: null argument");
     if (t->nmed < 1 || t->nmed > GEGS_MXMED) return fail(c, GEGS_ERR_MEDIA, "gegs_load_media: nmed out of range");
+    // the reference drops in-flight annihilations when nbr_split <= 0 (quirk
+    // Q5, annih.cu:71-75) and loses their energy: such tables are refused
+    if (t->nbr_split < 1) return fail(c, GEGS_ERR_MEDIA, "gegs_load_media: nbr_split must be >= 1 (quirk Q5)");
+    if (t->spin_effects) return fail(c, GEGS_ERR_MEDIA, "gegs_load_media: spin effects are not built (no spin data)");
+    if (t->transport_algorithm != 0 && t->transport_algorithm != 1)
+        return fail(c, GEGS_ERR_MEDIA, "gegs_load_media: transport_algorithm is 0 (PRESTA-II) or 1 (PRESTA-I)");
     hipSetDevice(c->device);
     if (!c->d_media) HIPCHECK(c, hipMalloc(&c->d_media, sizeof(gegs_media_tables)), "hipMalloc media");
     HIPCHECK(c, hipMemcpyAsync(c->d_media, t, sizeof(gegs_media_tables), hipMemcpyHostToDevice, c->stream), "copy media");
     HIPCHECK(c, hipStreamSynchronize(c->stream), "sync media");
     c->nmed = (uint32_t)t->nmed;
+    c->media_opts = (t->transport_algorithm == 0 ? ENV_PRESTA2 : 0u) | (t->exact_bca ? ENV_EXACT_BCA : 0u);
     c->have_media = true;
     return GEGS_OK;
 }
@@ -572,7 +580,7 @@ static Env make_env(gegs_ctx *c, uint64_t seed, uint32_t njob) {
     env.jawx_next = c->have_mlc ? GEGS_M_MLC : env.mlc_next;
     env.bw = c->d_bw;
     env.wedge_nver = c->h_bw.wedge_nver;
-    env.fixes = c->fixes;
+    env.fixes = c->fixes | c->media_opts;
     env.seed = seed;
     return env;
 }
@@ -621,6 +629,16 @@ static int check_ready(gegs_ctx *c, const gegs_run_options *o) {
     return GEGS_OK;
 }
 
+// the plain kernel instantiation for the record-table placement and the
+// options of the run (the general kernel for any of ENV_GENERAL)
+static void launch_plain(const PlainArgs &a, bool tab, uint32_t grid, uint32_t block, size_t lds, hipStream_t st) {
+    const bool gen = (a.env.fixes & ENV_GENERAL) != 0;
+    if (tab && gen) hipLaunchKernelGGL((plain_kernel<true, true>), dim3(grid), dim3(block), lds, st, a);
+    else if (tab) hipLaunchKernelGGL((plain_kernel<true, false>), dim3(grid), dim3(block), lds, st, a);
+    else if (gen) hipLaunchKernelGGL((plain_kernel<false, true>), dim3(grid), dim3(block), lds, st, a);
+    else hipLaunchKernelGGL((plain_kernel<false, false>), dim3(grid), dim3(block), lds, st, a);
+}
+
 int gegs_run_plain(gegs_ctx *c, const gegs_run_options *o, double *dose, double *dose2) {
     int rc = check_ready(c, o);
     if (rc) return rc;
@@ -659,8 +677,7 @@ int gegs_run_plain(gegs_ctx *c, const gegs_run_options *o, double *dose, double 
         a.first = (uint32_t)(r0 + (nr * b) / nb);
         a.last = (uint32_t)(r0 + (nr * (b + 1)) / nb);
         a.next = c->d_next + b;
-        if (tab) hipLaunchKernelGGL(plain_kernel<true>, dim3(grid), dim3(block), lds, c->stream, a);
-        else hipLaunchKernelGGL(plain_kernel<false>, dim3(grid), dim3(block), lds, c->stream, a);
+        launch_plain(a, tab, grid, block, lds, c->stream);
         hipLaunchKernelGGL(fold_batch, dim3(2048), dim3(256), 0, c->stream, c->d_chunk, c->d_dose, c->d_dose2, c->nvox);
     }
     hipEventRecord(t1, c->stream);
@@ -758,8 +775,7 @@ int gegs_run_plain_noalloc(gegs_ctx *c, const gegs_run_options *o, double *dose,
         a.first = (uint32_t)(r0 + (nr * b) / nb);
         a.last = (uint32_t)(r0 + (nr * (b + 1)) / nb);
         a.next = next + b;
-        if (tab) hipLaunchKernelGGL(plain_kernel<true>, dim3(grid), dim3(block), lds, c->stream, a);
-        else hipLaunchKernelGGL(plain_kernel<false>, dim3(grid), dim3(block), lds, c->stream, a);
+        launch_plain(a, tab, grid, block, lds, c->stream);
         hipLaunchKernelGGL(fold_batch, dim3(2048), dim3(256), 0, c->stream, chunk, dose, dose2, c->nvox);
     }
     HIPCHECK(c, hipGetLastError(), "launch plain_kernel");

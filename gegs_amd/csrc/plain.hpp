@@ -106,10 +106,11 @@ __device__ __forceinline__ Part source_particle(const gegs_source &r, int module
     return p;
 }
 
-template <bool TAB>
+template <bool TAB, bool GEN>
 __global__ void __launch_bounds__(128) GEGS_WAVES_PER_EU(2) plain_kernel(PlainArgs a) {
     extern __shared__ float lds_env[];
     Env env = a.env;
+    env.fixes = kernel_fixes<GEN>(env.fixes);
     stage_env<TAB>(env, lds_env);
     StackSink sink;
     sink.base = a.stack;

@@ -1,8 +1,8 @@
 // transport.hpp -- device-side coupled electron/photon transport for gfx950.
 //
 // Event handlers of the reference's event kernels, one device function per
-// event type, shared by the plain per-history kernel (plain.hip) and the
-// event-queued kernels (event.hip):
+// event type, shared by the plain per-history kernel (plain.hpp) and the
+// event-queued kernels (event.hpp):
 //   photon_step   photon.cu:6-292       compton   compton.cu:17-492
 //   photo         photo.cu:1-279        rayleigh  rayleigh.cu:4-157
 //   pair          pair.cu:14-431        electron  electron.cu:888-2622
@@ -334,7 +334,56 @@ struct Local {
     float x, y, z, u, v, w;
 };
 
-__device__ __forceinline__ Local phantom_frame(const gegs_field &f, const Part &p) {
+// the transform in binary64 (GEGS_FIX_FP64_FRAME): the field's binary32
+// cos/sin pair normalised to unit length, products and sums in binary64,
+// one rounding per coordinate
+__device__ __forceinline__ void frame64_rotation(const gegs_field &f, double &c, double &s) {
+    c = (double)f.cos_angle;
+    s = (double)f.sin_angle;
+    const double r = sqrt(c * c + s * s);
+    c = c / r;
+    s = s / r;
+}
+
+__device__ __forceinline__ Local phantom_frame64(const gegs_field &f, const Part &p) {
+    double x, y, z, u, v, w, c, s;
+    frame64_rotation(f, c, s);
+    if (f.change_xyz) {
+        x = p.y; y = (double)p.z - (double)f.source_to_isocenter; z = p.x;
+        u = p.v; v = p.w; w = p.u;
+    } else {
+        x = p.x; y = p.y; z = (double)p.z - (double)f.source_to_isocenter;
+        u = p.u; v = p.v; w = p.w;
+    }
+    Local q;
+    q.x = (float)(x * c - y * s + (double)f.iso_x);
+    q.y = (float)(x * s + y * c + (double)f.iso_y);
+    q.z = (float)(z + (double)f.iso_z);
+    q.u = (float)(u * c - v * s);
+    q.v = (float)(u * s + v * c);
+    q.w = (float)w;
+    return q;
+}
+
+// Options of the run that change the flight loops, beside the public
+// GEGS_FIX_* bits of Env::fixes: the electron-step algorithms the loaded
+// tables select (the reference's compile-time ESA_PII and BCA_EXACT,
+// CUDA_EGS.h:35-38), set by the host from the tables.
+constexpr uint32_t ENV_PRESTA2 = 1u << 16;     // msdist_pII instead of msdist_pI
+constexpr uint32_t ENV_EXACT_BCA = 1u << 17;   // exact boundary crossing (single scattering)
+constexpr uint32_t ENV_GENERAL = GEGS_FIX_FP64_FRAME | ENV_PRESTA2 | ENV_EXACT_BCA;
+
+// The flight and plain kernels are instantiated twice: GEN = false for the
+// reference's defaults (binary32 frame, PRESTA-I, inexact BCA) with these
+// bits known to be zero, so that kernel carries none of their code, and
+// GEN = true, which tests them at run time.
+template <bool GEN>
+__device__ __forceinline__ uint32_t kernel_fixes(uint32_t fixes) {
+    return GEN ? fixes : (fixes & ~ENV_GENERAL);
+}
+
+__device__ __forceinline__ Local phantom_frame(const gegs_field &f, const Part &p, uint32_t fixes) {
+    if (fixes & GEGS_FIX_FP64_FRAME) return phantom_frame64(f, p);
     float x, y, z, u, v, w;
     if (f.change_xyz) {
         x = p.y; y = p.z - f.source_to_isocenter; z = p.x;
@@ -371,7 +420,7 @@ __device__ __forceinline__ int locate(float p, uint32_t n, const float *b) {
 // distance to the next voxel face along the flight direction; returns the
 // region entered (0 = outside) and shortens t (geom.cu:54-307)
 __device__ __forceinline__ uint32_t howfar_phantom(const Env &env, const Part &p, float &t) {
-    const Local q = phantom_frame(field_of(env, p), p);
+    const Local q = phantom_frame(field_of(env, p), p, env.fixes);
     const uint32_t nx = env.nx, ny = env.ny, nz = env.nz, nxy = nx * ny;
     const float *xb = env.xb, *yb = env.yb, *zb = env.zb;
     if (p.region >= VAC_AND_OTH) {
@@ -443,7 +492,7 @@ __device__ __forceinline__ uint32_t howfar_phantom(const Env &env, const Part &p
 // perpendicular distance to the closest face; keeps the reference's
 // region-5 voxel offset (geom.cu:1487-1490), reading the neighbouring voxel
 __device__ __forceinline__ float hownear_phantom(const Env &env, const Part &p) {
-    const Local q = phantom_frame(field_of(env, p), p);
+    const Local q = phantom_frame(field_of(env, p), p, env.fixes);
     const uint32_t nx = env.nx, ny = env.ny, nz = env.nz;
     const float *xb = env.xb, *yb = env.yb, *zb = env.zb;
     // the reference takes the voxel of region r as r - 5 (quirk Q1, geom.cu:1487-1490): the
@@ -585,8 +634,16 @@ struct PhotonWalk {
     int32_t ix, iy, iz;      // voxel indices, valid when p.region >= VAC_AND_OTH
 };
 
-__device__ __forceinline__ void local_position(const gegs_field &f, const Part &p, float &qx, float &qy, float &qz) {
+__device__ __forceinline__ void local_position(const gegs_field &f, const Part &p, uint32_t fixes, float &qx, float &qy,
+                                               float &qz) {
     // the position part of phantom_frame(), operation for operation
+    if (fixes & GEGS_FIX_FP64_FRAME) {
+        const Local q = phantom_frame64(f, p);
+        qx = q.x;
+        qy = q.y;
+        qz = q.z;
+        return;
+    }
     float x, y, z;
     if (f.change_xyz) {
         x = p.y; y = p.z - f.source_to_isocenter; z = p.x;
@@ -615,7 +672,7 @@ __device__ __forceinline__ void photon_walk_begin(const Env &env, const Part &p,
     w.cmed = -1;
     w.lgle = 0;
     w.gmfpr0 = w.cohf = 0.0f;
-    const Local q = phantom_frame(field_of(env, p), p);
+    const Local q = phantom_frame(field_of(env, p), p, env.fixes);
     w.lu = q.u;
     w.lv = q.v;
     w.lw = q.w;
@@ -646,7 +703,7 @@ __device__ __forceinline__ bool photon_walk_iter(const Env &env, Stream &s, Part
     uint16_t cls_next = 0;
     if (inside) {
         float qx, qy, qz;
-        local_position(field_of(env, p), p, qx, qy, qz);
+        local_position(field_of(env, p), p, env.fixes, qx, qy, qz);
         const uint32_t nx = env.nx, nxy = env.nx * env.ny;
         if (w.lu > 0.0f) {
             dmin = (env.xb[w.ix + 1] - qx) / w.lu;
@@ -964,7 +1021,11 @@ __device__ __forceinline__ void compton(const Env &env, Stream &s, Part &p, Sink
             }
             jo = mt->jo_array[j - 1];
         }
+        // the pass counter br_loops is never incremented in the reference
+        // (quirk Q3, compton.cu:134-141); GEGS_FIX_Q3 applies its cap
+        int br_loops = 0;
         do {
+            if ((env.fixes & GEGS_FIX_Q3) && br_loops++ > 100) break;
             if (ko > 2) {
                 if (first) {
                     alph1 = glog(broi);
@@ -1336,7 +1397,9 @@ __device__ __forceinline__ void pair(const Env &env, Stream &s, Part &p, Sink &s
             l = 7;
             amax = mt->dl1[m8 + l - 1];
             bmax = mt->dl1[m8 + l];
-            aux1 = mt->bpar[2 * rd.med + 1] * (1.0f - mt->bpar[8 * rd.med] * RM / eig);
+            // bpar(1, medium) read at 8 * medium (quirk Q2, pair.cu:137); GEGS_FIX_Q2 reads 2 * medium
+            const int b1 = ((env.fixes & GEGS_FIX_Q2) ? 2 : 8) * (int)rd.med;
+            aux1 = mt->bpar[2 * rd.med + 1] * (1.0f - mt->bpar[b1] * RM / eig);
         }
         const float del0 = eig * mt->delcm[rd.med];
         const float eavail = eig - RMT2;
@@ -1490,6 +1553,100 @@ __device__ __forceinline__ void mscat(const gegs_media_tables *mt, Stream &s, fl
     }
     cost = 1.0f;
     sint = 0.0f;
+}
+
+// msdist_pII (electron.cu:282-614), spin off: two multiple-scattering
+// angles, one per half step, combined with random substep lengths b, c and
+// the energy-loss corrections of PRESTA-II (Kawrakow & Bielajew 1998)
+__device__ __forceinline__ float msdist_pII(const gegs_media_tables *mt, Stream &s, float eloss, float tustep, Part &p,
+                                            const gegs_region &rd) {
+    const int m = rd.med;
+    const float e0 = p.e - RM;
+    const float blc = mt->blcc[m];
+    const float xcc = mt->xcc[m];
+    float e = e0 - 0.5f * eloss;
+    const float tau = e / RM;
+    const float tau2 = tau * tau;
+    const float epsilon = eloss / e0;
+    const float epsilonp = eloss / e;
+    // the reference's denominator (24 tau2 + 48 tau + 72), kept
+    e = e * (1.0f - epsilonp * epsilonp * (6.0f + 10.0f * tau + 5.0f * tau2) / (24.0f * tau2 + 48.0f * tau + 72.0f));
+    const float p2 = e * (e + RMT2);
+    const float beta2 = p2 / (p2 + RMSQ);
+    const float chia2 = xcc / p2 / (4.0f * blc);
+    float lambda = 0.5f * tustep * rd.rhof * blc / beta2;
+    float temp = epsilonp / ((tau + 1.0f) * (tau + 2.0f));
+    float temp1 = 0.166666f * (4.0f + tau * (6.0f + tau * (7.0f + tau * (4.0f + tau)))) * temp * temp;
+    lambda = lambda * (1.0f - temp1);
+    float gamma = 1.0f;
+    lambda = lambda / (1.0f + chia2);
+    const float chilog = glog(1.0f + 1.0f / chia2);
+    const float q1 = 2.0f * chia2 * (chilog * (1.0f + chia2) - 1.0f);
+    gamma = 6.0f * chia2 * (1.0f + chia2) * (chilog * (1.0f + 2.0f * chia2) - 2.0f) / q1 * gamma;
+    float xi = q1 * lambda;
+    float w1, sint1, w2, sint2, cphi1, sphi1, cphi2, sphi2;
+    mscat(mt, s, lambda, chia2, xi, w1, sint1);
+    azimuth_box(s, cphi1, sphi1);
+    mscat(mt, s, lambda, chia2, xi, w2, sint2);
+    azimuth_box(s, cphi2, sphi2);
+    const float u2 = sint2 * cphi2;
+    const float v2 = sint2 * sphi2;
+    float u2p = w1 * u2 + sint1 * w2;
+    float us = u2p * cphi1 - v2 * sphi1;
+    float vs = u2p * sphi1 + v2 * cphi1;
+    float ws = w1 * w2 - sint1 * u2;
+    xi = 2.0f * xi;
+    float eta = s.next();
+    eta = sqrtf(eta);
+    const float eta1 = 0.5f * (1.0f - eta);
+    float delta = 0.9082483f - (0.1020621f - 0.0263747f * gamma) * xi;
+    temp1 = 2.0f + tau;
+    temp = (2.0f + tau * temp1) / ((tau + 1.0f) * temp1);
+    temp = temp - (tau + 1.0f) / ((tau + 2.0f) * (chilog * (1.0f + chia2) - 1.0f));
+    temp = temp * epsilonp;
+    temp1 = 1.0f - temp;
+    delta = delta + 0.40824829f * (epsilon * (tau + 1.0f) /
+                                   ((tau + 2.0f) * (chilog * (1.0f + chia2) - 1.0f) * (chilog * (1.0f + 2.0f * chia2) - 2.0f)) -
+                                   0.25f * temp * temp);
+    const float b = eta * delta;
+    const float cc = eta * (1.0f - delta);
+    const float w1v2 = w1 * v2;
+    float ut = b * sint1 * cphi1 + cc * (cphi1 * u2 - sphi1 * w1v2) + eta1 * us * temp1;
+    float vt = b * sint1 * sphi1 + cc * (sphi1 * u2 + cphi1 * w1v2) + eta1 * vs * temp1;
+    float wt = eta1 * (1.0f + temp) + b * w1 + cc * w2 + eta1 * ws * temp1;
+    const float ustep = tustep * sqrtf(ut * ut + vt * vt + wt * wt);
+    const float sint02 = p.u * p.u + p.v * p.v;
+    if (sint02 > 1e-20f) {
+        const float sint0 = sqrtf(sint02);
+        const float cphi0 = p.u / sint0;
+        const float sphi0 = p.v / sint0;
+        u2p = p.w * us + sint0 * ws;
+        ws = p.w * ws - sint0 * us;
+        us = u2p * cphi0 - vs * sphi0;
+        vs = u2p * sphi0 + vs * cphi0;
+        u2p = p.w * ut + sint0 * wt;
+        wt = p.w * wt - sint0 * ut;
+        ut = u2p * cphi0 - vt * sphi0;
+        vt = u2p * sphi0 + vt * cphi0;
+    } else {
+        wt = p.w * wt;
+        ws = p.w * ws;
+    }
+    p.x = p.x + tustep * ut;
+    p.y = p.y + tustep * vt;
+    p.z = p.z + tustep * wt;
+    p.u = us;
+    p.v = vs;
+    p.w = ws;
+    return ustep;
+}
+
+// sscat (electron.cu:245-279), spin off: one screened-Rutherford scattering
+__device__ __forceinline__ void sscat(Stream &s, float chia2, float &cost, float &sint) {
+    float xi = s.next();
+    xi = 2.0f * chia2 * xi / (1.0f - xi + chia2);
+    cost = 1.0f - xi;
+    sint = sqrtf(xi * (2.0f - xi));
 }
 
 __device__ __forceinline__ float msdist_pI(const gegs_media_tables *mt, Stream &s, float eloss, float tustep, Part &p,
@@ -1663,7 +1820,10 @@ __device__ __forceinline__ bool electron_iter(const Env &env, Stream &s, Part &p
     float sig_val = 0.0f, dedx_val = 0.0f, tstep = 0.0f, ustep = 0.0f, tustep = 0.0f;
     float tvstep = 0.0f, vstep = 0.0f, range = 0.0f, de = 0.0f;
     float blccl = 0.0f, xccl = 0.0f, beta2 = 0.0f, chia2 = 0.0f, xi = 0.0f, tperp = 0.0f;
-    bool callhowfar = false, domultiple = false, callmsdist = false;
+    bool callhowfar = false, domultiple = false, dosingle = false, callmsdist = false;
+    // electron-step options of the loaded tables (known zero in the
+    // reference-default kernels)
+    const bool exact_bca = (env.fixes & ENV_EXACT_BCA) != 0;
     if (medium == (int)VACUUM) {
         tstep = VACUUM_STEP;
         ustep = tstep;
@@ -1729,11 +1889,33 @@ __device__ __forceinline__ bool electron_iter(const Env &env, Stream &s, Part &p
         const float ssmfp = beta2 / blccl;
         const float skindepth = mt->skindepth_for_bca * ssmfp;
         tustep = fminf(tustep, fmaxf(tperp, skindepth));
-        if (tustep <= tperp && (!mt->exact_bca || tustep > skindepth)) {
+        if (tustep <= tperp && (!exact_bca || tustep > skindepth)) {
             callmsdist = true;
             de = eloss_over(mt, dedx, qel, medium, range, rhof, tustep, eke, elke, lelke);
             tvstep = tustep;
-            ustep = msdist_pI(mt, s, de, tustep, p, rd);
+            if (env.fixes & ENV_PRESTA2) ustep = msdist_pII(mt, s, de, tustep, p, rd);
+            else ustep = msdist_pI(mt, s, de, tustep, p, rd);
+        } else if (exact_bca) {
+            // exact BCA (electron.cu:1718-1752): cross the boundary in
+            // single-scattering mode; the distance to the next elastic
+            // scattering, in mean free paths, shortens the step
+            const float rnnoss = s.next();
+            const float lambda = rnnoss < 1.0f ? -glog(1.0f - rnnoss) : __builtin_inff();
+            const float tmp = eke / RM + 1.0f;
+            const float lambda_max = 0.5f * blccl * RM / dedx_val * tmp * tmp * tmp;
+            if (lambda >= 0.0f && lambda_max > 0.0f) {
+                float tuss;
+                if (lambda < lambda_max) tuss = lambda * ssmfp * (1.0f - 0.5f * lambda / lambda_max);
+                else tuss = 0.5f * lambda * ssmfp;
+                if (tuss < tustep) {
+                    tustep = tuss;
+                    dosingle = true;
+                }
+            } else {
+                return false;      // the reference drops the electron (electron.cu:1742-1748)
+            }
+            ustep = tustep;
+            callhowfar = !(ustep < tperp);
         } else {
             // inexact boundary-crossing algorithm
             domultiple = true;
@@ -1782,7 +1964,13 @@ __device__ __forceinline__ bool electron_iter(const Env &env, Stream &s, Part &p
         return true;
     }
     vstep = ustep;
-    if (callhowfar) {
+    if (callhowfar && exact_bca) {
+        // single-scattering mode: a crossed boundary ends the step there and
+        // shuts off the single scattering
+        tvstep = vstep;
+        if (tvstep != tustep) dosingle = false;
+        de = eloss_over(mt, dedx, qel, medium, range, rhof, tvstep, eke, elke, lelke);
+    } else if (callhowfar) {
         if (vstep < ustep0) {
             const float ekems = eke - 0.5f * tustep * vstep / ustep0 * dedx_val;
             const float p2m = ekems * (ekems + RMT2);
@@ -1811,11 +1999,17 @@ __device__ __forceinline__ bool electron_iter(const Env &env, Stream &s, Part &p
         if (domultiple) {
             const float lambda = blccl * tvstep / beta2 / (1.0f + chia2);
             mscat(mt, s, lambda, chia2, xi, costhe, sinthe);
+        } else if (dosingle) {
+            // single scattering at the end of the step (electron.cu:2247-2276)
+            const float ekems = fmaxf(eke - de, rd.ecut - RM);
+            const float p2s = ekems * (ekems + RMT2);
+            chia2 = mt->xcc[medium] / (4.0f * mt->blcc[medium] * p2s);
+            sscat(s, chia2, costhe, sinthe);
         }
         p.x = p.x + p.u * vstep;
         p.y = p.y + p.v * vstep;
         p.z = p.z + p.w * vstep;
-        if (domultiple) {
+        if (domultiple || dosingle) {
             float sinphi, cosphi;
             uphi21(s, costhe, sinthe, cosphi, sinphi, p);
         }

@@ -903,12 +903,21 @@ class MediaTables:
     """Device physics tables for an ordered list of media."""
 
     def __init__(self, names, data_dir=DATA_DIR, pegs="pegslite700.pegs4dat", prefix="pegslite",
-                 mlc_medium=None, eii_flag=1, edit=None, files=None):
+                 mlc_medium=None, eii_flag=1, edit=None, files=None, transport="presta-I", bca="inexact"):
         """edit: optional callable applied to the list of PegsMedium records
         read from the PEGS file before any table is derived from them (test
         media with synthetic stopping powers or cross sections).  files:
         optional paths that replace the pegs-lite names, with the keys of
-        ``data_files`` (a run input's EGSnrc data set)."""
+        ``data_files`` (a run input's EGSnrc data set).  transport
+        ("presta-I", the reference's default, or "presta-II") and bca
+        ("inexact", the default, or "exact") are the electron-step options
+        the reference selects at compile time (ESA_PII and BCA_EXACT,
+        CUDA_EGS.h:35-38; PhysicsInit.cu:263-282, :923-996).  The inexact
+        crossing takes the skin depth mscati computes; the exact one EGSnrc's
+        3 elastic mean free paths (the reference's SKIN_DEPTH_FOR_BCA = 0
+        would hold an electron on a boundary with zero-length steps)."""
+        if transport not in ("presta-I", "presta-II") or bca not in ("inexact", "exact"):
+            raise ValueError("transport is presta-I or presta-II, bca inexact or exact")
         if len(names) > MXMED:
             raise ValueError("at most %d media (MXMED)" % MXMED)
         self.names = list(names)
@@ -933,10 +942,13 @@ class MediaTables:
         c.nbr_split = 1
         c.ibrdst = 1
         c.iprdst = 1
-        c.exact_bca = 0
+        c.exact_bca = 1 if bca == "exact" else 0
         c.spin_effects = 0
-        c.transport_algorithm = 1
-        c.skindepth_for_bca = skin
+        c.transport_algorithm = 0 if transport == "presta-II" else 1
+        # the exact crossing takes EGSnrc's default of 3 elastic mean free
+        # paths: with the reference's SKIN_DEPTH_FOR_BCA = 0 an electron on a
+        # boundary (tperp = 0) gets tustep = 0 and never leaves it
+        c.skindepth_for_bca = 3.0 if bca == "exact" else skin
         c.dllambi = dllambi
         c.dqmsi = dqmsi
         c.rm = RM_F

@@ -155,6 +155,10 @@ def _p(a):
 FIX_Q1 = 1                 # GEGS_FIX_Q1: hownear_phantom from the particle's own voxel
 FIX_Q8 = 2                 # GEGS_FIX_Q8: X-jaw step into the opening by the opening's distance
 FIX_Q7 = 4                 # GEGS_FIX_Q7: range rejection on kinetic energy
+FIX_FP64_FRAME = 8         # GEGS_FIX_FP64_FRAME: head -> phantom transform in binary64
+FIX_Q2 = 16                # GEGS_FIX_Q2: pair bpar(1, medium) at 2 * medium
+FIX_Q3 = 32                # GEGS_FIX_Q3: Compton Klein-Nishina resampling capped
+FIX_ALL = 63
 
 
 class GegsError(RuntimeError):

@@ -190,7 +190,9 @@ const char *gegs_last_error(gegs_ctx *ctx);
 size_t gegs_sizeof_media_tables(void);
 
 /* Physics tables (copyToDevice, copy.cu:3-277; init_user_photon upload,
- * media.cu:1030-1048). */
+ * media.cu:1030-1048).  GEGS_ERR_MEDIA for nmed outside 1..GEGS_MXMED and
+ * for nbr_split < 1 (the reference then drops in-flight annihilations and
+ * their energy, quirk Q5 at annih.cu:71-75). */
 int gegs_load_media(gegs_ctx *ctx, const gegs_media_tables *tables);
 
 /* Voxel phantom and region table (init_regions, init.cu:909-1051; phantom
@@ -247,7 +249,22 @@ int gegs_load_block_wedge(gegs_ctx *ctx, const gegs_block_wedge *bw);
  *   energy with e_max_rr, where the reference compares the total energy
  *   (electron.cu:1523). */
 #define GEGS_FIX_Q7 4u
-#define GEGS_FIX_ALL 7u
+/*   GEGS_FIX_FP64_FRAME: the head -> phantom frame transform (change_xyz swap,
+ *   gantry rotation, isocentre shift; geom.cu:59-109, binary32 in the
+ *   reference) in binary64, the rotation from the field's cos/sin pair
+ *   normalised to unit length, each coordinate rounded once to binary32.
+ *   Not a quirk: an accuracy option (voxel indices can then differ from the
+ *   reference's where a binary32 rounding crosses a voxel boundary). */
+#define GEGS_FIX_FP64_FRAME 8u
+/*   GEGS_FIX_Q2: Bethe-Heitler Coulomb-corrected pair sampling (E >= 50 MeV)
+ *   reads bpar(1, medium) at bpar[2 * medium], not bpar[8 * medium]
+ *   (pair.cu:137, a read past the medium's pair of values). */
+#define GEGS_FIX_Q2 16u
+/*   GEGS_FIX_Q3: the Klein-Nishina resampling loop of Compton scattering
+ *   stops after 101 passes, as its br_loops cap intends (compton.cu:134-141,
+ *   where the counter is never incremented). */
+#define GEGS_FIX_Q3 32u
+#define GEGS_FIX_ALL 63u
 int gegs_set_fixes(gegs_ctx *ctx, uint32_t mask);
 
 /* Leaf positions of n_sets fields, 120 values per set, in the order of the

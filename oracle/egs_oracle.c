@@ -230,6 +230,29 @@ static void escore(octx *c, uint32_t region, float energy) {
 /* ------------------------------------------------------------ geometry */
 static void to_phantom(const octx *c, const opart *p0, float *q) {
     const gegs_field *f = &c->f;
+    if (c->fixes & GEGS_FIX_FP64_FRAME) {
+        /* binary64 transform: the cos/sin pair normalised to unit length, one
+         * rounding per coordinate */
+        double cs = (double)f->cos_angle, sn = (double)f->sin_angle;
+        const double r = sqrt(cs * cs + sn * sn);
+        cs = cs / r;
+        sn = sn / r;
+        double x, y, z, u, v, w;
+        if (f->change_xyz) {
+            x = p0->y; y = (double)p0->z - (double)f->source_to_isocenter; z = p0->x;
+            u = p0->v; v = p0->w; w = p0->u;
+        } else {
+            x = p0->x; y = p0->y; z = (double)p0->z - (double)f->source_to_isocenter;
+            u = p0->u; v = p0->v; w = p0->w;
+        }
+        q[0] = (float)(x * cs - y * sn + (double)f->iso_x);
+        q[1] = (float)(x * sn + y * cs + (double)f->iso_y);
+        q[2] = (float)(z + (double)f->iso_z);
+        q[3] = (float)(u * cs - v * sn);
+        q[4] = (float)(u * sn + v * cs);
+        q[5] = (float)w;
+        return;
+    }
     float x, y, z, u, v, w;
     if (f->change_xyz) {
         x = p0->y; y = p0->z - f->source_to_isocenter; z = p0->x;
@@ -1057,8 +1080,11 @@ static void compton(octx *c, ostream *s, opart *p) {
             }
             jo = mt->jo_array[j - 1];
         }
-        /* scattering angle from Klein-Nishina */
+        /* scattering angle from Klein-Nishina; br_loops is never incremented
+         * in the reference (Q3), GEGS_FIX_Q3 applies its cap */
+        int br_loops = 0;
         do {
+            if ((c->fixes & GEGS_FIX_Q3) && br_loops++ > 100) break;
             if (ko > 2) {
                 if (first_time) {
                     alph1 = om_logf(broi);
@@ -1447,7 +1473,8 @@ static void pair(octx *c, ostream *s, opart *p) {
             l = 7;
             amax = mt->dl1[8 * medium + l - 1];
             bmax = mt->dl1[8 * medium + l];
-            aux1 = mt->bpar[2 * medium + 1] * (1.0f - mt->bpar[8 * medium] * RM / eig);   /* Q2 */
+            aux1 = mt->bpar[2 * medium + 1] *
+                   (1.0f - mt->bpar[((c->fixes & GEGS_FIX_Q2) ? 2 : 8) * medium] * RM / eig);   /* Q2 */
         }
         float del0 = eig * mt->delcm[medium];
         float eavail = eig - RMT2;
@@ -1672,6 +1699,104 @@ static float msdist_pI(const gegs_media_tables *mt, ostream *s, float eloss, flo
     return ustep;
 }
 
+/* msdist_pII (electron.cu:282-614), spin off: the step is split in two
+ * halves with one multiple-scattering angle each; the transport distance
+ * combines the two directions with the random substep lengths b, c and the
+ * energy-loss corrections of the PRESTA-II paper (Kawrakow & Bielajew 1998) */
+static float msdist_pII(const gegs_media_tables *mt, ostream *s, float eloss, float tustep, opart *p0,
+                        const gegs_region *rd) {
+    int medium = rd->med;
+    float e0 = p0->e - RM;
+    float blccc = mt->blcc[medium];
+    float xcccc = mt->xcc[medium];
+    float e = e0 - 0.5f * eloss;
+    float tau = e / RM;
+    float tau2 = tau * tau;
+    float epsilon = eloss / e0;
+    float epsilonp = eloss / e;
+    /* the reference's denominator (24 tau2 + 48 tau + 72), kept */
+    e = e * (1.0f - epsilonp * epsilonp * (6.0f + 10.0f * tau + 5.0f * tau2) / (24.0f * tau2 + 48.0f * tau + 72.0f));
+    float p2 = e * (e + RMT2);
+    float beta2 = p2 / (p2 + RMSQ);
+    float chia2 = xcccc / p2 / (4.0f * blccc);
+    float lambda = 0.5f * tustep * rd->rhof * blccc / beta2;
+    float temp = epsilonp / ((tau + 1.0f) * (tau + 2.0f));
+    float temp1 = 0.166666f * (4.0f + tau * (6.0f + tau * (7.0f + tau * (4.0f + tau)))) * temp * temp;
+    lambda = lambda * (1.0f - temp1);
+    /* spin off: etap = xi_corr = gamma = ms_corr = 1 */
+    float gamma = 1.0f;
+    lambda = lambda / (1.0f + chia2);
+    float chilog = om_logf(1.0f + 1.0f / chia2);
+    float q1 = 2.0f * chia2 * (chilog * (1.0f + chia2) - 1.0f);
+    gamma = 6.0f * chia2 * (1.0f + chia2) * (chilog * (1.0f + 2.0f * chia2) - 2.0f) / q1 * gamma;
+    float xi = q1 * lambda;
+    float w1, sint1, w2, sint2, cphi1, sphi1, cphi2, sphi2;
+    mscat(mt, s, lambda, chia2, xi, &w1, &sint1);
+    select_azimuth(s, &cphi1, &sphi1);
+    mscat(mt, s, lambda, chia2, xi, &w2, &sint2);
+    select_azimuth(s, &cphi2, &sphi2);
+    /* final direction relative to motion along z */
+    float u2 = sint2 * cphi2;
+    float v2 = sint2 * sphi2;
+    float u2p = w1 * u2 + sint1 * w2;
+    float us = u2p * cphi1 - v2 * sphi1;
+    float vs = u2p * sphi1 + v2 * cphi1;
+    float ws = w1 * w2 - sint1 * u2;
+    xi = 2.0f * xi;      /* xi was for half the step; xi_corr = 1 */
+    float eta = rnd(s);
+    eta = sqrtf(eta);    /* sampled from 2 eta d(eta) */
+    float eta1 = 0.5f * (1.0f - eta);
+    float delta = 0.9082483f - (0.1020621f - 0.0263747f * gamma) * xi;
+    /* energy-loss corrections of the substep coefficients */
+    temp1 = 2.0f + tau;
+    temp = (2.0f + tau * temp1) / ((tau + 1.0f) * temp1);
+    temp = temp - (tau + 1.0f) / ((tau + 2.0f) * (chilog * (1.0f + chia2) - 1.0f));
+    temp = temp * epsilonp;
+    temp1 = 1.0f - temp;
+    delta = delta + 0.40824829f * (epsilon * (tau + 1.0f) /
+                                   ((tau + 2.0f) * (chilog * (1.0f + chia2) - 1.0f) * (chilog * (1.0f + 2.0f * chia2) - 2.0f)) -
+                                   0.25f * temp * temp);
+    float b = eta * delta;
+    float cc = eta * (1.0f - delta);
+    float w1v2 = w1 * v2;
+    float ut = b * sint1 * cphi1 + cc * (cphi1 * u2 - sphi1 * w1v2) + eta1 * us * temp1;
+    float vt = b * sint1 * sphi1 + cc * (sphi1 * u2 + cphi1 * w1v2) + eta1 * vs * temp1;
+    float wt = eta1 * (1.0f + temp) + b * w1 + cc * w2 + eta1 * ws * temp1;
+    float ustep = tustep * sqrtf(ut * ut + vt * vt + wt * wt);
+    float sint02 = p0->u * p0->u + p0->v * p0->v;
+    if (sint02 > 1e-20f) {
+        float sint0 = sqrtf(sint02);
+        float cphi0 = p0->u / sint0;
+        float sphi0 = p0->v / sint0;
+        u2p = p0->w * us + sint0 * ws;
+        ws = p0->w * ws - sint0 * us;
+        us = u2p * cphi0 - vs * sphi0;
+        vs = u2p * sphi0 + vs * cphi0;
+        u2p = p0->w * ut + sint0 * wt;
+        wt = p0->w * wt - sint0 * ut;
+        ut = u2p * cphi0 - vt * sphi0;
+        vt = u2p * sphi0 + vt * cphi0;
+    } else {
+        wt = p0->w * wt;
+        ws = p0->w * ws;
+    }
+    p0->x = p0->x + tustep * ut;
+    p0->y = p0->y + tustep * vt;
+    p0->z = p0->z + tustep * wt;
+    p0->u = us;
+    p0->v = vs;
+    p0->w = ws;
+    return ustep;
+}
+
+/* sscat (electron.cu:245-279), spin off: one screened-Rutherford scattering */
+static void sscat(ostream *s, float chia2, float *cost, float *sint) {
+    float xi = rnd(s);
+    xi = 2.0f * chia2 * xi / (1.0f - xi + chia2);
+    *cost = 1.0f - xi;
+    *sint = sqrtf(xi * (2.0f - xi));
+}
+
 /* $COMPUTE-ELOSS-G: energy loss over path `step` (electron.cu:1649-1701) */
 static float compute_eloss(const gegs_media_tables *mt, int qel, int medium, float range, float rhof,
                            float step, float eke, float elke, int lelke) {
@@ -1852,7 +1977,36 @@ static void electron_step(octx *c, ostream *s, opart *p) {
                 callmsdist = 1;
                 de = compute_eloss(mt, qel, medium, range, rhof, tustep, eke, elke, lelke);
                 tvstep = tustep;
-                ustep = msdist_pI(mt, s, de, tustep, p, &rd);
+                /* transport_algorithm: 0 PRESTA-II, 1 PRESTA-I (the reference's default) */
+                if (mt->transport_algorithm == 0) ustep = msdist_pII(mt, s, de, tustep, p, &rd);
+                else ustep = msdist_pI(mt, s, de, tustep, p, &rd);
+            } else if (mt->exact_bca) {
+                /* exact BCA (electron.cu:1718-1752): cross the boundary in
+                 * single-scattering mode; the distance to the next elastic
+                 * scattering, in mean free paths, shortens the step */
+                callmsdist = 0;
+                domultiple = 0;
+                float rnnoss = rnd(s);
+                float lambda = rnnoss < 1.0f ? -om_logf(1.0f - rnnoss) : INFINITY;
+                float tmp = eke / RM + 1.0f;
+                float lambda_max = 0.5f * blccl * RM / dedx_val * tmp * tmp * tmp;
+                if (lambda >= 0.0f && lambda_max > 0.0f) {
+                    float tuss;
+                    if (lambda < lambda_max) tuss = lambda * ssmfp * (1.0f - 0.5f * lambda / lambda_max);
+                    else tuss = 0.5f * lambda * ssmfp;
+                    if (tuss < tustep) {
+                        tustep = tuss;
+                        dosingle = 1;
+                    } else {
+                        dosingle = 0;
+                    }
+                } else {
+                    /* the reference drops the electron (electron.cu:1742-1748) */
+                    c->dropped += ((double)p->e + (double)p->charge * PRM) * (double)p->wt;
+                    return;
+                }
+                ustep = tustep;
+                callhowfar = ustep < tperp ? 0 : 1;
             } else {
                 callmsdist = 0;
                 /* inexact BCA (exact_bca = 0 in the reference defaults) */
@@ -1908,7 +2062,13 @@ static void electron_step(octx *c, ostream *s, opart *p) {
             continue;
         }
         vstep = ustep;
-        if (callhowfar) {
+        if (callhowfar && mt->exact_bca) {
+            /* single-scattering mode: a crossed boundary ends the step there
+             * and shuts off the single scattering */
+            tvstep = vstep;
+            if (tvstep != tustep) dosingle = 0;
+            de = compute_eloss(mt, qel, medium, range, rhof, tvstep, eke, elke, lelke);
+        } else if (callhowfar) {
             if (vstep < ustep0) {
                 ekems = eke - 0.5f * tustep * vstep / ustep0 * dedx_val;
                 float p2m = ekems * (ekems + RMT2);
@@ -1942,6 +2102,12 @@ static void electron_step(octx *c, ostream *s, opart *p) {
             if (domultiple) {
                 float lambda = blccl * tvstep / beta2 / (1.0f + chia2);
                 mscat(mt, s, lambda, chia2, xi, &costhe, &sinthe);
+            } else if (dosingle) {
+                /* single scattering at the end of the step (electron.cu:2247-2276) */
+                ekems = fmaxf(ekef, rd.ecut - RM);
+                float p2s = ekems * (ekems + RMT2);
+                chia2 = mt->xcc[medium] / (4.0f * mt->blcc[medium] * p2s);
+                sscat(s, chia2, &costhe, &sinthe);
             }
             p->x = p->x + p->u * vstep;
             p->y = p->y + p->v * vstep;
@@ -2679,6 +2845,7 @@ int oracle_interaction(const gegs_media_tables *mt, const gegs_region *regions,
     double *dose2 = (double *)calloc(nvox, sizeof(double));
     c->mt = mt; c->reg = regions; c->xb = xb; c->yb = yb; c->zb = zb;
     c->nx = nx; c->ny = ny; c->nz = nz; c->f = *field; c->seed = seed;
+    c->fixes = g_fixes;
     c->dose = dose; c->dose2 = dose2;
     c->tmp = (double *)calloc(nvox, sizeof(double));
     c->last = (uint64_t *)calloc(nvox, sizeof(uint64_t));
@@ -2761,7 +2928,9 @@ void oracle_msdist(const gegs_media_tables *mt, int medium, float rhof, float e,
         p.lin = lin0 + i;
         ostream s;
         st_begin(&s, &p, seed);
-        float us = msdist_pI(mt, &s, eloss, tustep, &p, &rd);
+        /* the tables' transport algorithm: 0 PRESTA-II, 1 PRESTA-I */
+        float us = mt->transport_algorithm == 0 ? msdist_pII(mt, &s, eloss, tustep, &p, &rd)
+                                                : msdist_pI(mt, &s, eloss, tustep, &p, &rd);
         float *o = out + 7 * (size_t)i;
         o[0] = p.x; o[1] = p.y; o[2] = p.z; o[3] = p.u; o[4] = p.v; o[5] = p.w; o[6] = us;
     }
@@ -2878,6 +3047,7 @@ void oracle_to_phantom(const gegs_field *field, const float *pos_dir, float *out
     octx c;
     memset(&c, 0, sizeof(c));
     c.f = *field;
+    c.fixes = g_fixes;
     opart p;
     memset(&p, 0, sizeof(p));
     p.x = pos_dir[0]; p.y = pos_dir[1]; p.z = pos_dir[2];

@@ -22,6 +22,21 @@ def build(force=False):
     return LIB
 
 
+def build_native(dest_dir):
+    """The oracle built for the CPU of this host (gcc -O3 -march=native) into
+    dest_dir; returns its path."""
+    out = os.path.join(dest_dir, "liboracle_native.so")
+    subprocess.check_call(["make", "-s", "-C", HERE, "-B", "native", "NATIVE_OUT=" + out])
+    return out
+
+
+def use_library(path):
+    """Load the oracle from `path` from now on (None: oracle/liboracle.so)."""
+    global _lib, _path
+    _lib = None
+    _path = path
+
+
 class Counters(ctypes.Structure):
     _fields_ = [("photon_steps", ctypes.c_uint64), ("electron_steps", ctypes.c_uint64),
                 ("region_gathers", ctypes.c_uint64), ("deposits", ctypes.c_uint64),
@@ -30,13 +45,15 @@ class Counters(ctypes.Structure):
 
 
 _lib = None
+_path = None
 
 
 def lib():
     global _lib
     if _lib is None:
-        build()
-        L = ctypes.CDLL(LIB)
+        if _path is None:
+            build()
+        L = ctypes.CDLL(_path or LIB)
         vp = ctypes.c_void_p
         L.oracle_run.restype = ctypes.c_int
         L.oracle_run.argtypes = [vp, vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,

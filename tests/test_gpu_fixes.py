@@ -74,24 +74,27 @@ def test_q1_fix_dose_equivalent_to_reference(ct_small):
     assert abs(meanz) < 0.2
 
 
-def test_all_fixes_jaws_mlc_bit_exact():
-    """GEGS_FIX_Q1 | GEGS_FIX_Q8 | GEGS_FIX_Q7 on the jaws -> MLC -> CT
-    workload (range rejection is on for the MLC medium): the GPU
-    reproduces the oracle with the same mask history by history, and the
-    event path reproduces the plain kernel."""
-    wl = workloads.ct_jaws_mlc()
+@pytest.mark.parametrize("mask,gantry", [(native.FIX_Q1 | native.FIX_Q8 | native.FIX_Q7, 0.0),
+                                         (native.FIX_ALL, 72.0), (native.FIX_FP64_FRAME, 216.0)])
+def test_all_fixes_jaws_mlc_bit_exact(mask, gantry):
+    """GEGS_FIX_Q1 | GEGS_FIX_Q8 | GEGS_FIX_Q7, every fix with the binary64
+    frame transform, and that transform alone, on the jaws -> MLC -> CT
+    workload (range rejection is on for the MLC medium) at three gantry
+    angles: the GPU reproduces the oracle with the same mask history by
+    history, and the event path reproduces the plain kernel."""
+    wl = workloads.ct_jaws_mlc(gantry_deg=gantry)
     src = wl.source(4000, 19)
-    print("fixes Q1|Q8|Q7: jaws + MLC workload, 4000 histories, seed 43", flush=True)
+    print("fixes mask %d, gantry %.0f: jaws + MLC workload, 4000 histories, seed 43" % (mask, gantry), flush=True)
     ctx = native.Context(0)
     try:
         ctx.load_workload(wl)
         ctx.set_source(src)
-        ctx.set_fixes(native.FIX_Q1 | native.FIX_Q8 | native.FIX_Q7)
+        ctx.set_fixes(mask)
         g = ctx.run(seed=43, n_batches=2, method="plain", per_history=True)
         e = ctx.run(seed=43, n_batches=2, method="event", per_history=True)
     finally:
         ctx.close()
-    oracle.set_fixes(native.FIX_Q1 | native.FIX_Q8 | native.FIX_Q7)
+    oracle.set_fixes(mask)
     try:
         o = oracle.run(wl, src, seed=43, per_history=True, threads=8)
     finally:

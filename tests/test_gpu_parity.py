@@ -63,6 +63,25 @@ EVENTS = [(0, 0, (0.02, 6.0)), (1, 0, (0.02, 6.0)), (2, 0, (0.012, 0.5)), (3, 0,
 def test_samplers_uniform_for_uniform(gpu_ctx, ct_small, status, charge, erange):
     """Every event handler, fed the same stream, returns bit-identical
     particles and scored energy on the GPU and in the oracle."""
+    _samplers_agree(gpu_ctx, ct_small, status, charge, erange)
+
+
+@pytest.mark.parametrize("mask,status,charge,erange", [(0, 4, 0, (50.5, 80.0)), (native.FIX_Q2, 4, 0, (50.5, 80.0)),
+                                                       (native.FIX_Q3, 1, 0, (0.02, 6.0))])
+def test_quirk_switch_samplers_uniform_for_uniform(gpu_ctx, ct_small, mask, status, charge, erange):
+    """The pair sampler above 50 MeV (the Coulomb-corrected branch, where
+    quirk Q2 reads bpar) with and without GEGS_FIX_Q2, and Compton with
+    GEGS_FIX_Q3: GPU and oracle agree uniform for uniform under each mask."""
+    gpu_ctx.set_fixes(mask)
+    oracle.set_fixes(mask)
+    try:
+        _samplers_agree(gpu_ctx, ct_small, status, charge, erange)
+    finally:
+        gpu_ctx.set_fixes(0)
+        oracle.set_fixes(0)
+
+
+def _samplers_agree(gpu_ctx, ct_small, status, charge, erange):
     gpu_ctx.load_workload(ct_small)
     ph = ct_small.phantom
     rng = np.random.default_rng(status * 10 + charge + 3)
@@ -246,3 +265,17 @@ def test_atomic_probe_rate_and_bounds(gpu_ctx):
     with pytest.raises(native.GegsError) as e:
         gpu_ctx.atomic_probe(bad, 1000)
     assert e.value.code == 2001
+
+
+def test_media_with_nbr_split_below_one_are_refused(gpu_ctx, ct_small):
+    """nbr_split < 1 would drop in-flight annihilations with their energy in
+    the reference (quirk Q5): gegs_load_media refuses such tables."""
+    t = ct_small.tables
+    old = t.c.nbr_split
+    t.c.nbr_split = 0
+    try:
+        with pytest.raises(native.GegsError, match="nbr_split"):
+            gpu_ctx.load_media(t)
+    finally:
+        t.c.nbr_split = old
+    gpu_ctx.load_media(t)

@@ -226,6 +226,40 @@ def test_hownear_with_q1_fix_measures_the_own_voxel():
 
 
 @pytest.mark.parametrize("change_xyz", [0, 1])
+def test_phantom_frame_fp64_option_is_correctly_rounded(change_xyz):
+    """GEGS_FIX_FP64_FRAME: the transform from the unit-normalised cos/sin
+    pair is its binary64 evaluation (recomputed here with numpy in the same
+    operation order) rounded once to binary32 per coordinate, where the
+    binary32 path of the reference is within two ULP."""
+    from gegs_amd import native, workloads
+    rng = np.random.default_rng(12)
+    oracle.set_fixes(native.FIX_FP64_FRAME)
+    try:
+        for gantry in (0.0, 33.0, 90.0, 217.5):
+            f = workloads.make_field(change_xyz, 100.0, gantry, iso=(1.25, -0.5, 2.0))
+            pd = np.zeros((500, 6), dtype=np.float32)
+            pd[:, :3] = rng.uniform(-30, 130, (500, 3))
+            d = rng.normal(size=(500, 3))
+            pd[:, 3:] = d / np.linalg.norm(d, axis=1)[:, None]
+            got = oracle.to_phantom(f, pd)
+            c, s = np.float64(np.float32(f.cos_angle)), np.float64(np.float32(f.sin_angle))
+            r = np.sqrt(c * c + s * s)
+            c, s = c / r, s / r
+            p = pd.astype(np.float64)
+            sid = np.float64(np.float32(f.source_to_isocenter))
+            if change_xyz:
+                x, y, z, u, v, w = p[:, 1], p[:, 2] - sid, p[:, 0], p[:, 4], p[:, 5], p[:, 3]
+            else:
+                x, y, z, u, v, w = p[:, 0], p[:, 1], p[:, 2] - sid, p[:, 3], p[:, 4], p[:, 5]
+            iso = [np.float64(np.float32(t)) for t in (f.iso_x, f.iso_y, f.iso_z)]
+            exact = np.stack([x * c - y * s + iso[0], x * s + y * c + iso[1], z + iso[2],
+                              u * c - v * s, u * s + v * c, w], axis=1)
+            assert np.array_equal(got, exact.astype(np.float32)), gantry
+    finally:
+        oracle.set_fixes(0)
+
+
+@pytest.mark.parametrize("change_xyz", [0, 1])
 def test_phantom_frame_within_2ulp_of_fp64(change_xyz):
     """The head -> phantom transform (geom.cu:59-109) runs in binary32 as in
     the reference, so that voxel indexing matches it bit for bit; against the

@@ -340,3 +340,35 @@ def test_multiple_scattering_mean_cosine(wl, lam, chia2):
     expect = math.exp(-g1)
     err = cost.std() / math.sqrt(n)
     assert abs(cost.mean() - expect) < 4 * err + 3e-3 * (1 - expect), (cost.mean(), expect, err)
+
+
+def test_q2_switch_reads_the_medium_pair_parameter():
+    """Pair production above 50 MeV takes the Coulomb-corrected branch, where
+    the reference reads bpar(1, medium) at bpar[8 * medium] (quirk Q2) and
+    GEGS_FIX_Q2 at bpar[2 * medium]: the two agree in medium 0 and differ in
+    medium 1, uniform for uniform."""
+    from gegs_amd import native, workloads
+    wl = workloads.ct()
+    reg = wl.regions
+    assert wl.tables.c.bpar[8] != wl.tables.c.bpar[2]
+    rng = np.random.default_rng(31)
+    same = {0: 0, 1: 0}
+    for med in (0, 1):
+        regions = np.flatnonzero(reg["med"] == med)[:40]
+        for r in regions:
+            p = np.zeros(1, dtype=native.PARTICLE_DTYPE)
+            p["e"] = rng.uniform(50.5, 80.0)
+            p["w"] = 1.0
+            p["wt"] = 1.0
+            p["region"] = r
+            p["status"] = 4
+            p["lin"] = rng.integers(1, 2 ** 62, dtype=np.uint64)
+            a = oracle.interaction(wl, p[0], 77)
+            oracle.set_fixes(native.FIX_Q2)
+            try:
+                b = oracle.interaction(wl, p[0], 77)
+            finally:
+                oracle.set_fixes(0)
+            same[med] += int(a[0].tobytes() == b[0].tobytes() and a[1] == b[1])
+    assert same[0] == 40
+    assert same[1] < 40

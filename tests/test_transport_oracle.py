@@ -65,3 +65,32 @@ def test_depth_dose_builds_up_then_attenuates():
     d = r["dose"].reshape(30, 30, 30)[:, 12:18, 12:18].sum(axis=(1, 2))
     # beyond the build-up region the central-axis dose falls off with depth
     assert d[3:8].mean() > d[20:25].mean() * 1.3
+
+
+import pytest  # noqa: E402
+
+
+@pytest.mark.parametrize("transport,bca", [("presta-II", "inexact"), ("presta-I", "exact"), ("presta-II", "exact")])
+def test_electron_step_options_conserve_energy_and_agree_in_dose(transport, bca):
+    """PRESTA-II and the exact boundary crossing (the reference's ESA_PII and
+    BCA_EXACT builds): every history's energy is accounted for, and the depth
+    dose agrees with the reference's default (PRESTA-I, inexact BCA) within
+    its statistical uncertainty -- condensed-history algorithms that converge
+    to the same transport solution."""
+    from gegs_amd import hatch
+    w = workloads.water(n=24, voxel=1.25)
+    opt = workloads.water(n=24, voxel=1.25)
+    opt._tables = hatch.MediaTables(w.media_names, mlc_medium=w.mlc_medium, transport=transport, bca=bca)
+    assert opt.tables.c.transport_algorithm == (0 if transport == "presta-II" else 1)
+    src = w.source(40000, 22)
+    r = oracle.run(opt, src, seed=78, threads=8)
+    e_in = float(np.sum(src["e"].astype(np.float64) * src["wt"]))
+    e_out = r["dose"].sum() + r["escaped"] + r["dropped"]
+    assert abs(e_out - e_in) / e_in < 1e-5
+    ref = oracle.run(w, src, seed=79, threads=8)
+    d, d_ref = r["dose"].reshape(24, 24, 24), ref["dose"].reshape(24, 24, 24)
+    v, v_ref = r["dose2"].reshape(24, 24, 24), ref["dose2"].reshape(24, 24, 24)
+    depth, depth_ref = d.sum(axis=(1, 2)), d_ref.sum(axis=(1, 2))
+    sig = np.sqrt(v.sum(axis=(1, 2)) + v_ref.sum(axis=(1, 2)))
+    z = (depth - depth_ref) / np.maximum(sig, 1e-30)
+    assert np.mean(z ** 2) < 2.0, z
