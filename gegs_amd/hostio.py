@@ -1,0 +1,270 @@
+"""ctypes binding of include/gegs_host.h: the run's file formats as the C++
+library reads and writes them (egsphant, egsphsp, media-table and head files,
+the batches of a field, the dose file).  Host only: no GPU is touched."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import hatch, head as headmod, native
+from . import phantom as ph
+from . import source as srcmod
+
+NAME_LEN = 64
+MXMED = 6
+HOST_FIX_Q4 = 1
+HOST_FIX_Q15 = 2
+HOST_FIX_Q16 = 4
+PHSP_REFUSED = 1
+
+HOST_SYMBOLS = [
+    "gegs_host_last_error", "gegs_media_file_write", "gegs_media_file_read", "gegs_egsphant_read",
+    "gegs_egsphant_free", "gegs_voxel_mass", "gegs_region_table", "gegs_phsp_open", "gegs_phsp_total",
+    "gegs_phsp_header_path", "gegs_phsp_read", "gegs_phsp_close", "gegs_field_schedule", "gegs_write_dose",
+    "gegs_head_file_write", "gegs_head_file_read", "gegs_head_free",
+]
+
+
+class CEgsphant(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_uint32), ("ny", ctypes.c_uint32), ("nz", ctypes.c_uint32), ("nmed", ctypes.c_uint32),
+                ("names", ctypes.c_void_p), ("xb", ctypes.POINTER(ctypes.c_float)),
+                ("yb", ctypes.POINTER(ctypes.c_float)), ("zb", ctypes.POINTER(ctypes.c_float)),
+                ("media", ctypes.POINTER(ctypes.c_int32)), ("density", ctypes.POINTER(ctypes.c_float))]
+
+
+class CSchedule(ctypes.Structure):
+    _fields_ = [("histories", ctypes.c_uint64), ("n_repeat", ctypes.c_uint32), ("batch_size", ctypes.c_uint32),
+                ("n_batch", ctypes.c_uint64), ("last_batch", ctypes.c_uint64), ("n_loads", ctypes.c_uint64),
+                ("n_dropped", ctypes.c_uint64), ("dropped_records", ctypes.c_uint64), ("records", ctypes.c_uint64),
+                ("repeat_counts", ctypes.c_uint64 * 256)]
+
+
+class CHead(ctypes.Structure):
+    _fields_ = [("has_mlc", ctypes.c_uint32), ("has_jaws", ctypes.c_uint32), ("has_block_wedge", ctypes.c_uint32),
+                ("n_sets", ctypes.c_uint32), ("mlc", headmod.Mlc), ("jaws", headmod.Jaws),
+                ("block_wedge", headmod.BlockWedge), ("leaf_positions", ctypes.POINTER(ctypes.c_float)),
+                ("jaw_positions", ctypes.POINTER(ctypes.c_float))]
+
+
+class HostError(ValueError):
+    def __init__(self, code, msg):
+        super().__init__("error %d: %s" % (code, msg))
+        self.code = code
+
+
+_L = None
+
+
+def lib():
+    global _L
+    if _L is not None:
+        return _L
+    L = native.load()
+    vp, u32, u64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64
+    for name in HOST_SYMBOLS:
+        if getattr(L, name, None) is None:
+            raise RuntimeError("symbol %s missing from %s" % (name, native.LIB_PATH))
+    L.gegs_host_last_error.restype = ctypes.c_char_p
+    L.gegs_media_file_write.argtypes = [ctypes.c_char_p, vp, vp]
+    L.gegs_media_file_read.argtypes = [ctypes.c_char_p, vp, vp]
+    L.gegs_egsphant_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(CEgsphant)]
+    L.gegs_egsphant_free.argtypes = [ctypes.POINTER(CEgsphant)]
+    L.gegs_voxel_mass.argtypes = [ctypes.POINTER(CEgsphant), vp]
+    L.gegs_region_table.argtypes = [ctypes.POINTER(CEgsphant), vp, vp, vp, ctypes.c_uint16, vp]
+    L.gegs_phsp_open.argtypes = [ctypes.c_char_p, u32, u32, ctypes.POINTER(vp)]
+    L.gegs_phsp_total.argtypes = [vp]
+    L.gegs_phsp_total.restype = u64
+    L.gegs_phsp_header_path.argtypes = [vp, u32]
+    L.gegs_phsp_header_path.restype = ctypes.c_char_p
+    L.gegs_phsp_read.argtypes = [vp, u64, u64, ctypes.c_float, vp]
+    L.gegs_phsp_close.argtypes = [vp]
+    L.gegs_field_schedule.argtypes = [u64, u64, u32, ctypes.POINTER(CSchedule), vp, u64]
+    L.gegs_write_dose.argtypes = [ctypes.c_char_p, u32, u32, u32, vp, vp, ctypes.c_float, vp, ctypes.c_char_p,
+                                  ctypes.c_size_t]
+    L.gegs_head_file_write.argtypes = [ctypes.c_char_p, ctypes.POINTER(CHead)]
+    L.gegs_head_file_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(CHead)]
+    L.gegs_head_free.argtypes = [ctypes.POINTER(CHead)]
+    _L = L
+    return L
+
+
+def _check(rc):
+    if rc:
+        raise HostError(rc, lib().gegs_host_last_error().decode())
+
+
+def _names_block(names, n=MXMED):
+    buf = ctypes.create_string_buffer(n * NAME_LEN)
+    for i, s in enumerate(names):
+        b = s.encode()
+        if len(b) >= NAME_LEN:
+            raise ValueError("medium name %r longer than %d bytes" % (s, NAME_LEN - 1))
+        ctypes.memmove(ctypes.addressof(buf) + i * NAME_LEN, b, len(b))
+    return buf
+
+
+def _decode_names(buf, n):
+    return [bytes(buf[i * NAME_LEN:(i + 1) * NAME_LEN]).split(b"\0", 1)[0].decode() for i in range(n)]
+
+
+# ------------------------------------------------------------------ media-table file
+def write_media_file(path, tables):
+    """The media tables (hatch.MediaTables) and their names in one file."""
+    _check(lib().gegs_media_file_write(path.encode(), ctypes.addressof(tables.c), _names_block(tables.names)))
+
+
+def read_media_file(path):
+    """(hatch.CMediaTables, names) of a media-table file."""
+    c = hatch.CMediaTables()
+    names = ctypes.create_string_buffer(MXMED * NAME_LEN)
+    _check(lib().gegs_media_file_read(path.encode(), ctypes.addressof(c), names))
+    return c, _decode_names(names.raw, c.nmed)
+
+
+# ------------------------------------------------------------------ egsphant, masses, regions
+class Egsphant:
+    """A phantom read by the C++ reader; arrays copied out, C memory freed."""
+
+    def __init__(self, path):
+        c = CEgsphant()
+        _check(lib().gegs_egsphant_read(path.encode(), ctypes.byref(c)))
+        try:
+            self.nx, self.ny, self.nz, nmed = c.nx, c.ny, c.nz, c.nmed
+            raw = ctypes.string_at(c.names, nmed * NAME_LEN)
+            self.media_names = _decode_names(raw, nmed)
+            self.xb = np.ctypeslib.as_array(c.xb, (self.nx + 1,)).copy()
+            self.yb = np.ctypeslib.as_array(c.yb, (self.ny + 1,)).copy()
+            self.zb = np.ctypeslib.as_array(c.zb, (self.nz + 1,)).copy()
+            n = self.nx * self.ny * self.nz
+            self.media_idx = np.ctypeslib.as_array(c.media, (n,)).copy()
+            self.density = np.ctypeslib.as_array(c.density, (n,)).copy()
+            self.voxel_mass = np.zeros(n, dtype=np.float32)
+            _check(lib().gegs_voxel_mass(ctypes.byref(c), self.voxel_mass.ctypes.data))
+            self._c = c
+        except Exception:
+            lib().gegs_egsphant_free(ctypes.byref(c))
+            raise
+
+    def region_table(self, tables_c, names, other_media, flags=ph.DEFAULT_FLAGS):
+        out = np.zeros(self.nx * self.ny * self.nz + ph.VAC_AND_OTH, dtype=ph.REGION_DTYPE)
+        _check(lib().gegs_region_table(ctypes.byref(self._c), ctypes.addressof(tables_c), _names_block(names),
+                                       _names_block(other_media, 5), int(flags), out.ctypes.data))
+        return out
+
+    def close(self):
+        if self._c is not None:
+            lib().gegs_egsphant_free(ctypes.byref(self._c))
+            self._c = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ egsphsp
+class PhaseSpace:
+    """gegs_phsp: the run's phase-space files through the C++ reader."""
+
+    def __init__(self, base, n_files, fixes=0):
+        h = ctypes.c_void_p()
+        _check(lib().gegs_phsp_open(base.encode(), int(n_files), int(fixes), ctypes.byref(h)))
+        self.h = h
+        self.total = int(lib().gegs_phsp_total(h))
+        self.header_paths = []
+        i = 0
+        while True:
+            p = lib().gegs_phsp_header_path(h, i)
+            if p is None:
+                break
+            self.header_paths.append(p.decode())
+            i += 1
+
+    def read(self, istart, length, z):
+        """Decoded source records (source.SOURCE_DTYPE), or None where the
+        reference's read refuses."""
+        out = np.zeros(int(length), dtype=srcmod.SOURCE_DTYPE)
+        rc = lib().gegs_phsp_read(self.h, int(istart), int(length), float(z), out.ctypes.data)
+        if rc == PHSP_REFUSED:
+            return None
+        _check(rc)
+        return out
+
+    def close(self):
+        if self.h:
+            lib().gegs_phsp_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ batches, dose file, head file
+def field_schedule(histories, total_records, fixes=0):
+    """(CSchedule, loads as an (n, 2) array of first record, count)."""
+    s = CSchedule()
+    _check(lib().gegs_field_schedule(int(histories), int(total_records), int(fixes), ctypes.byref(s), None, 0))
+    loads = np.zeros((s.n_loads, 2), dtype=np.uint64)
+    _check(lib().gegs_field_schedule(int(histories), int(total_records), int(fixes), ctypes.byref(s),
+                                     loads.ctypes.data if s.n_loads else None, s.n_loads))
+    return s, loads
+
+
+def write_dose(root, nx, ny, nz, edep, mass, dose_unit):
+    """(dose float32, file name) as the C++ writer makes them."""
+    edep = np.ascontiguousarray(edep, dtype=np.float64)
+    mass = np.ascontiguousarray(mass, dtype=np.float32)
+    out = np.zeros(nx * ny * nz, dtype=np.float32)
+    fname = ctypes.create_string_buffer(len(root) + 64)
+    _check(lib().gegs_write_dose(root.encode(), nx, ny, nz, edep.ctypes.data, mass.ctypes.data,
+                                 float(np.float32(dose_unit)), out.ctypes.data, fname, len(fname)))
+    return out, fname.value.decode()
+
+
+def write_head_file(path, mlc=None, jaws=None, block_wedge=None, leaf_positions=None, jaw_positions=None):
+    h = CHead()
+    keep = []
+    n_sets = 0
+    if mlc is not None:
+        lp = np.ascontiguousarray(leaf_positions, dtype=np.float32).reshape(-1, 120)
+        keep.append(lp)
+        h.has_mlc, h.mlc = 1, mlc
+        h.leaf_positions = lp.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        n_sets = len(lp)
+    if jaws is not None:
+        jp = np.ascontiguousarray(jaw_positions, dtype=np.float32).reshape(-1, 4)
+        keep.append(jp)
+        if mlc is not None and len(jp) != n_sets:
+            raise ValueError("leaf and jaw position sets differ in number")
+        h.has_jaws, h.jaws = 1, jaws
+        h.jaw_positions = jp.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        n_sets = len(jp)
+    if block_wedge is not None:
+        h.has_block_wedge, h.block_wedge = 1, block_wedge
+    h.n_sets = n_sets
+    _check(lib().gegs_head_file_write(path.encode(), ctypes.byref(h)))
+
+
+def read_head_file(path):
+    """dict(mlc, jaws, block_wedge, leaf_positions, jaw_positions); absent
+    parts are None."""
+    h = CHead()
+    _check(lib().gegs_head_file_read(path.encode(), ctypes.byref(h)))
+    try:
+        out = dict(mlc=None, jaws=None, block_wedge=None, leaf_positions=None, jaw_positions=None)
+        if h.has_mlc:
+            out["mlc"] = headmod.Mlc.from_buffer_copy(h.mlc)
+            out["leaf_positions"] = np.ctypeslib.as_array(h.leaf_positions, (h.n_sets, 120)).copy()
+        if h.has_jaws:
+            out["jaws"] = headmod.Jaws.from_buffer_copy(h.jaws)
+            out["jaw_positions"] = np.ctypeslib.as_array(h.jaw_positions, (h.n_sets, 4)).copy()
+        if h.has_block_wedge:
+            out["block_wedge"] = headmod.BlockWedge.from_buffer_copy(h.block_wedge)
+        return out
+    finally:
+        lib().gegs_head_free(ctypes.byref(h))

@@ -35,6 +35,10 @@ exactly MU x histories-per-MU histories), Q9 (absolute-dose voxel) and Q10
 (CT values below the ramp).  Device-side corrections (Q1, Q7, Q8) are the
 ``gegs_set_fixes`` mask.
 
+``--prepare <run file>`` stops after step 5 and writes what the C++ driver
+gegs_amd/gegs_run needs to do steps 2, 6, 7 and the dose file without Python:
+the run file, the media tables and the head geometry (``prepare``).
+
 The reference's RNG is cuRAND; this transport uses Philox streams keyed by
 a history index, so histories are independent of the reference's but each
 (record, repeat) pair of the schedule is transported once, as there.
@@ -213,17 +217,10 @@ def run(path, base_dir=None, method="batched", seed=1, fixes=(), n_batches=10, d
         log.close()
 
 
-def _run(ri, path, base, data_dir, root, log, method, seed, fixes, n_batches, device, tuning, context, t0):
+def _setup(ri, base, data_dir, root, log, fixes):
+    """Steps 2-5: phase-space headers, phantom, head components, media tables
+    and region records, fields with leaf files and output factors."""
     d = lambda name: os.path.join(data_dir, name.replace("\\", "/"))
-    log("\nGEGS (MI355X) started at %s" % time.strftime("%Y-%m-%d %H:%M:%S"))
-    log("\nSimulation")
-    log("  Input file  . . . . . . . . %s" % os.path.abspath(path))
-    for w in ri.warnings:
-        log("WARNING: " + w)
-    log("  Output prefix . . . . . . . %s" % ri.output_prefix)
-    log("  Histories per MU  . . . . . %.3E" % ri.histories_input)
-    log("  Data directory  . . . . . . %s" % data_dir)
-
     # phase space (read_egsphsp_header)
     phsp = srcmod.PhaseSpaceFiles(d(ri.phase_space_file), ri.phsp_file_number, fix_q4="Q4" in fixes,
                                   fix_q15="Q15" in fixes)
@@ -312,6 +309,26 @@ def _run(ri, path, base, data_dir, root, log, method, seed, fixes, n_batches, de
     total_mu = sum(int(f.mu) for f in fields)
     log("  Total number of MU  . . . . %d" % total_mu)
     log("  Histories per MU  . . . . . %d" % ri.histories)
+
+    return dict(phsp=phsp, egsphant=egsphant, phan=phan, jaws=jaws, dyn=dyn, block=block, wedge=wedge,
+                others=others, names=names, module_first=module_first, tables=tables, regions=regions,
+                fields=fields, ofs=ofs)
+
+
+def _run(ri, path, base, data_dir, root, log, method, seed, fixes, n_batches, device, tuning, context, t0):
+    d = lambda name: os.path.join(data_dir, name.replace("\\", "/"))
+    log("\nGEGS (MI355X) started at %s" % time.strftime("%Y-%m-%d %H:%M:%S"))
+    log("\nSimulation")
+    log("  Input file  . . . . . . . . %s" % os.path.abspath(path))
+    for w in ri.warnings:
+        log("WARNING: " + w)
+    log("  Output prefix . . . . . . . %s" % ri.output_prefix)
+    log("  Histories per MU  . . . . . %.3E" % ri.histories_input)
+    log("  Data directory  . . . . . . %s" % data_dir)
+
+    S = _setup(ri, base, data_dir, root, log, fixes)
+    phsp, phan, dyn, jaws, block, wedge = S["phsp"], S["phan"], S["dyn"], S["jaws"], S["block"], S["wedge"]
+    module_first, tables, regions, fields, ofs = S["module_first"], S["tables"], S["regions"], S["fields"], S["ofs"]
 
     # schedules and source records
     scheds, seqs, seq_of = [], {}, []
@@ -425,6 +442,81 @@ def _run(ri, path, base, data_dir, root, log, method, seed, fixes, n_batches, de
                      sum(s.transported for s in scheds))
 
 
+def prepare(path, run_file, base_dir=None, method="batched", seed=1, fixes=(), n_batches=10, echo=True):
+    """Steps 1-5 of the run input at `path`, written for the C++ driver
+    (gegs_amd/gegs_run, gegs_amd/csrc/gegs_run.cpp), which does the rest
+    without Python: the run file `run_file` (key = value lines), the media
+    tables (<run file stem>.gegsmed) and the head geometry with one aperture
+    set per field (<run file stem>.gegshead).  The driver reads the egsphant
+    and phase-space files itself.  Returns the run file's path."""
+    from . import hostio
+    fixes = set(fixes)
+    unknown = fixes - set(FIXES)
+    if unknown:
+        raise RunError("unknown fixes %s (known: %s)" % (sorted(unknown), ", ".join(FIXES)))
+    base = base_dir or os.getcwd()
+    ri = runinput.read_run_input(path)
+    data_dir = host_path(base, ri.data_directory)
+    out_dir = host_path(base, ri.output_directory)
+    os.makedirs(out_dir, exist_ok=True)
+    root = os.path.join(out_dir, ri.output_prefix)
+    stem = os.path.splitext(os.path.abspath(run_file))[0]
+    log = RunLog(stem + "_prepare_log.txt", echo)
+    try:
+        log("\nGEGS (MI355X) run preparation at %s" % time.strftime("%Y-%m-%d %H:%M:%S"))
+        log("  Input file  . . . . . . . . %s" % os.path.abspath(path))
+        S = _setup(ri, base, data_dir, root, log, fixes)
+    finally:
+        log.close()
+    fields = S["fields"]
+    media_file = stem + ".gegsmed"
+    hostio.write_media_file(media_file, S["tables"])
+    head_file = None
+    if S["dyn"] is not None or S["jaws"] is not None or S["block"] is not None or S["wedge"] is not None:
+        head_file = stem + ".gegshead"
+        hostio.write_head_file(
+            head_file,
+            mlc=head.build_mlc(S["dyn"]) if S["dyn"] is not None else None,
+            jaws=S["jaws"],
+            block_wedge=head.build_block_wedge(S["block"], S["wedge"])
+            if S["block"] is not None or S["wedge"] is not None else None,
+            leaf_positions=np.stack([f.positions for f in fields]) if S["dyn"] is not None else None,
+            jaw_positions=np.array([[f.x1, f.x2, f.y1, f.y2] for f in fields], dtype=np.float32)
+            if S["jaws"] is not None else None)
+    g = lambda v: "%.9g" % float(np.float32(v))
+    host_fixes = ((1 if "Q4" in fixes else 0) | (2 if "Q15" in fixes else 0) | (4 if "Q16" in fixes else 0))
+    lines = ["# gegs run file: %s prepared for gegs_amd/gegs_run" % os.path.abspath(path),
+             "media file = %s" % media_file]
+    if head_file:
+        lines.append("head file = %s" % head_file)
+    lines += ["egsphant file = %s" % os.path.abspath(S["egsphant"])]
+    lines += ["region medium %d = %s" % (i + 1, n) for i, n in enumerate(S["others"])]
+    lines += ["region flags = 0x%04x" % ph.DEFAULT_FLAGS,
+              "phsp base = %s" % os.path.abspath(os.path.join(data_dir, ri.phase_space_file.replace("\\", "/"))),
+              "phsp files = %d" % ri.phsp_file_number,
+              "phsp z = %s" % g(ri.phsp_z),
+              "host fixes = %d" % host_fixes,
+              "device fixes = %d" % _fix_mask(fixes),
+              "change xyz = %d" % int(ri.change_coordinate),
+              "module first = %d" % S["module_first"],
+              "source to isocenter = %s" % g(ri.source_to_isocenter),
+              "isocenter = %s" % " ".join(g(v) for v in ri.isocenter),
+              "histories per mu = %d" % ri.histories,
+              "dose unit = %s" % g(ri.dose_unit),
+              "output root = %s" % os.path.abspath(root),
+              "seed = %d" % seed,
+              "batches = %d" % n_batches,
+              "method = %s" % method]
+    for i, f in enumerate(fields):
+        fl = make_field(ri.change_coordinate, ri.source_to_isocenter, float(np.float32(f.angle)),
+                        tuple(ri.isocenter), S["module_first"])
+        of = g(S["ofs"][i]) if S["ofs"] is not None else "-"
+        lines.append("field = %d %s %s %s %s" % (int(f.mu), g(f.angle), g(fl.cos_angle), g(fl.sin_angle), of))
+    with open(run_file, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    return run_file
+
+
 def _load(ctx, tables, phan, regions, dyn, jaws, block, wedge, fields):
     """Media, phantom and head geometry of the run on the context; leaf set
     and jaw set i belong to field i."""
@@ -449,8 +541,20 @@ def main(argv=None):
     ap.add_argument("--device", type=int, default=0)
     ap.add_argument("--base-dir", default=None, help="folder the input's relative paths start from (default: cwd)")
     ap.add_argument("--fix", default="", help="comma list of reference quirks to correct: " + ", ".join(FIXES))
+    ap.add_argument("--prepare", default=None, metavar="RUN_FILE",
+                    help="write the run file, media tables and head geometry for the C++ driver gegs_amd/gegs_run "
+                         "instead of running")
     a = ap.parse_args(argv)
     fixes = [f.strip().upper() for f in a.fix.split(",") if f.strip()]
+    if a.prepare:
+        try:
+            rf = prepare(a.input, a.prepare, base_dir=a.base_dir, method=a.method, seed=a.seed, fixes=fixes,
+                         n_batches=a.batches)
+        except (runinput.RunInputError, RunError, ValueError) as e:
+            print("ERROR: %s" % e, file=sys.stderr)
+            return getattr(e, "code", 1) if isinstance(getattr(e, "code", None), int) else 1
+        print("run file %s: gegs_amd/gegs_run %s" % (rf, rf))
+        return 0
     try:
         r = run(a.input, base_dir=a.base_dir, method=a.method, seed=a.seed, fixes=fixes, n_batches=a.batches,
                 device=a.device)

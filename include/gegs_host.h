@@ -1,0 +1,159 @@
+/*
+ * gegs_host.h -- host-side C ABI of libgegs_hip.so: the file formats a run
+ * reads and writes, restated in C++ so that a C or C++ host drives a run of
+ * the reference's input files without Python.  Nothing here touches the GPU;
+ * every function runs on a machine without one.
+ *
+ * Pieces of the reference replaced:
+ *   egsphant reader and voxel masses   init.cu:1164-1271 (init_phantom_and_MLC)
+ *   region records                     init.cu:909-1051  (init_regions)
+ *   egsphsp headers and record reads   read_egsphsp.cpp:60-232
+ *   record decoding                    CUDA_EGS.cu:40-106 (load_stack_from_phspfile)
+ *   batches of one field               CUDA_EGS.cu:377-411, :462-476, :635-650; init.cu:29-41
+ *   field sum and dose file            CUDA_EGS.cu:1712-1735, output.c:873-903 (write_dose)
+ * and two files of this library with no counterpart in the reference: the
+ * media-table file (the physics tables the HATCH builders produce, so a
+ * host loads them without rebuilding them) and the head file (MLC, jaws,
+ * block/wedge descriptions and per-field aperture sets).
+ *
+ * Return values are GEGS_OK or a status code (the reference's 2xxx input and
+ * 8xxx phantom codes where it has one); gegs_host_last_error() gives the
+ * message of the calling thread's last failure.
+ */
+#ifndef GEGS_HOST_H
+#define GEGS_HOST_H
+#include <stddef.h>
+#include <stdint.h>
+#include "gegs.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GEGS_NAME_LEN 64             /* medium name, NUL-terminated */
+
+/* Reference quirks of the host side with a switch that corrects them.
+ * Q4: with one phase-space file the record count comes from <base>.egsphsp1
+ *     but the records from <base>_p0.egsphsp1 (read_egsphsp.cpp:113 against
+ *     :202-216); the fix reads both from <base>.egsphsp1.
+ * Q15: a read running past the end of file i continues at the start of file
+ *     i again (read_egsphsp.cpp:208-212); the fix continues in file i + 1.
+ * Q16: the last batch of a field holds floor(rest / (nRepeat + 1)) + 1
+ *     records, so a field whose histories are a multiple of nRepeat + 1
+ *     reads one record past the file and the read refuses the batch
+ *     (CUDA_EGS.cu:462-476); the fix transports exactly the field's
+ *     histories. */
+#define GEGS_HOST_FIX_Q4 1u
+#define GEGS_HOST_FIX_Q15 2u
+#define GEGS_HOST_FIX_Q16 4u
+
+const char *gegs_host_last_error(void);
+
+/* ---------------------------------------------------------------- media-table file
+ * "GEGSMT01", sizeof(gegs_media_tables), nmed, nmed names of GEGS_NAME_LEN
+ * bytes, the table block, an FNV-1a 64 checksum of the block.  names[m] is
+ * the medium of table index m. */
+int gegs_media_file_write(const char *path, const gegs_media_tables *tables, const char *names);
+int gegs_media_file_read(const char *path, gegs_media_tables *tables, char names[GEGS_MXMED][GEGS_NAME_LEN]);
+
+/* ---------------------------------------------------------------- egsphant
+ * gegs_egsphant_read restates the reference's reader (init.cu:1164-1251):
+ * media count and names, a skipped line, the voxel numbers, nx+1, ny+1, nz+1
+ * bounds, one media digit per voxel (x fastest) and one density per voxel,
+ * each z slice followed by a blank line; errors 8001-8013 as there.  The
+ * arrays are allocated by the library and released by gegs_egsphant_free. */
+typedef struct gegs_egsphant {
+    uint32_t nx, ny, nz, nmed;
+    char (*names)[GEGS_NAME_LEN];    /* nmed medium names */
+    float *xb, *yb, *zb;             /* nx + 1, ny + 1, nz + 1 bounds (cm) */
+    int32_t *media;                  /* 1-based medium per voxel, 0 = vacuum */
+    float *density;                  /* g/cm^3 per voxel */
+} gegs_egsphant;
+
+int gegs_egsphant_read(const char *path, gegs_egsphant *out);
+void gegs_egsphant_free(gegs_egsphant *ph);
+
+/* Voxel masses (init.cu:1256-1271): |(dx * dy) * dz| * density in binary32,
+ * 0 replaced by 1; nx * ny * nz entries. */
+int gegs_voxel_mass(const gegs_egsphant *ph, float *mass);
+
+/* Region records (init_regions): region 0 vacuum; regions 1..5 the media
+ * other_media[0..4] (air, jaws, MLC, block, wedge) with flags, rhof 1 and the
+ * medium's cuts; voxel regions from 6 on with the phantom's media, rhof =
+ * density / medium density (1 where the density is 0) and the medium's
+ * cuts; vacuum voxels all zero.  Media are found by name in `names` (the
+ * media-table file's list); a name not there is error 10029. */
+int gegs_region_table(const gegs_egsphant *ph, const gegs_media_tables *tables,
+                      const char names[GEGS_MXMED][GEGS_NAME_LEN], const char other_media[5][GEGS_NAME_LEN],
+                      uint16_t flags, gegs_region *out /* nx * ny * nz + 6 */);
+
+/* ---------------------------------------------------------------- egsphsp
+ * The phase-space input of a run: n_files MODE0 files <base>_p<i>.egsphsp1,
+ * or one file (host fixes Q4 and Q15 above), read as one record sequence. */
+typedef struct gegs_phsp gegs_phsp;
+
+int gegs_phsp_open(const char *base, uint32_t n_files, uint32_t host_fixes, gegs_phsp **out);
+uint64_t gegs_phsp_total(const gegs_phsp *ph);
+/* Path of file i's header (the file whose record count was read). */
+const char *gegs_phsp_header_path(const gegs_phsp *ph, uint32_t i);
+/* Records [istart, istart + length) decoded into source particles at the
+ * plane z (LATCH bit 30 electron, bit 29 positron, else photon; |E|; W =
+ * sqrt(max(0, 1 - U^2 - V^2)) with the sign of WT; |WT|).  Returns
+ * GEGS_PHSP_REFUSED where the reference's read returns -1 (istart at or past
+ * the total, or fewer than length records left). */
+#define GEGS_PHSP_REFUSED 1
+int gegs_phsp_read(gegs_phsp *ph, uint64_t istart, uint64_t length, float z, gegs_source *out);
+void gegs_phsp_close(gegs_phsp *ph);
+
+/* ---------------------------------------------------------------- batches of a field
+ * The reference's batches of a field of `histories` histories over
+ * `total_records` records: nRepeat = (histories - 1) / total_records extra
+ * uses of every record, batch size from init_stack, n_batch full batches and
+ * a last one.  loads[2 i], loads[2 i + 1] = first record and count of every
+ * load the read accepts (up to *n_loads; pass max_loads = 0 to learn the
+ * count), repeat_counts[q] = records transported with repeat q. */
+typedef struct gegs_schedule {
+    uint64_t histories;
+    uint32_t n_repeat;
+    uint32_t batch_size;
+    uint64_t n_batch;
+    uint64_t last_batch;
+    uint64_t n_loads;                /* loads the read accepts */
+    uint64_t n_dropped;              /* loads it refuses (Q16) */
+    uint64_t dropped_records;
+    uint64_t records;                /* records of the accepted loads */
+    uint64_t repeat_counts[256];
+} gegs_schedule;
+
+int gegs_field_schedule(uint64_t histories, uint64_t total_records, uint32_t host_fixes, gegs_schedule *out,
+                        uint64_t *loads, uint64_t max_loads);
+
+/* ---------------------------------------------------------------- dose file
+ * write_dose: the plan's deposit (FP64, per voxel) as (float)E / mass *
+ * dose_unit in binary32, written to <root>_DoseResults_NNNxNNNxNNN.bin;
+ * the file name is returned in fname (at least 32 + strlen(root) bytes). */
+int gegs_write_dose(const char *root, uint32_t nx, uint32_t ny, uint32_t nz, const double *edep,
+                    const float *mass, float dose_unit, float *dose_out, char *fname, size_t fname_len);
+
+/* ---------------------------------------------------------------- head file
+ * "GEGSHD01": which of MLC, jaws and block/wedge are present, their
+ * descriptions and n_sets aperture sets (120 leaf positions and x1 x2 y1 y2
+ * jaw openings per set). */
+typedef struct gegs_head {
+    uint32_t has_mlc, has_jaws, has_block_wedge, n_sets;
+    gegs_mlc mlc;
+    gegs_jaws jaws;
+    gegs_block_wedge block_wedge;
+    float *leaf_positions;           /* n_sets x 120 (has_mlc) */
+    float *jaw_positions;            /* n_sets x 4 (has_jaws) */
+} gegs_head;
+
+int gegs_head_file_write(const char *path, const gegs_head *head);
+/* Allocates leaf_positions / jaw_positions; release with gegs_head_free. */
+int gegs_head_file_read(const char *path, gegs_head *head);
+void gegs_head_free(gegs_head *head);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

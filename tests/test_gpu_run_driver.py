@@ -54,3 +54,31 @@ def test_run_input_plain_equals_batched(tmp_path):
     p = grun.run(ini, base_dir=str(tmp_path), method="plain", seed=5, echo=False)
     # the same histories: deposits differ only by the order of the FP64 sums
     assert np.allclose(p.field_edep, b.field_edep, rtol=1e-9, atol=1e-9 * b.field_edep.max())
+
+
+RUNNER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gegs_amd", "gegs_run")
+
+
+def test_cpp_driver_matches_the_python_driver(tmp_path):
+    """The prepared run (run.prepare) transported by the C++ driver
+    gegs_amd/gegs_run -- its own egsphant and phase-space readers, batch
+    schedule, region records and dose file -- gives the Python driver's dose
+    file; the deposits differ only by the order of the FP64 atomic sums."""
+    import subprocess
+    ini, _ = make_dataset(str(tmp_path))
+    doses = {}
+    for method in ("batched", "plain"):
+        rf = str(tmp_path / ("plan_%s.gegsrun" % method))
+        grun.prepare(ini, rf, base_dir=str(tmp_path), method=method, seed=31, n_batches=10, echo=False)
+        print("C++ run driver: %s" % method, flush=True)
+        r = subprocess.run([RUNNER, rf], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        out = os.path.join(str(tmp_path), "out", "Syn_DoseResults_032x032x024.bin")
+        doses[method] = np.fromfile(out, dtype=np.float32)
+        log = open(os.path.join(str(tmp_path), "out", "Syn_log.txt")).read()
+        assert "repeat times 2" in log and "Output factor" in log
+    print("python run driver: batched", flush=True)
+    res = grun.run(ini, base_dir=str(tmp_path), method="batched", seed=31, echo=False)
+    assert doses["batched"].max() > 0
+    assert np.allclose(doses["batched"], res.dose, rtol=3e-7, atol=0)
+    assert np.allclose(doses["plain"], res.dose, rtol=3e-7, atol=0)

@@ -89,8 +89,13 @@ def test_dose_bin_writer(tmp_path):
 
 
 def _declared_functions():
-    text = open(os.path.join(ROOT, "include", "gegs.h")).read()
-    return sorted(set(re.findall(r"\b(gegs_[a-z_]+)\s*\(", text)))
+    """Every function declared in include/*.h (gegs.h, gegs_host.h)."""
+    names = set()
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if h.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", h)).read()
+            names |= set(re.findall(r"\b(gegs_[a-z_0-9]+)\s*\(", text))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_symbol():
