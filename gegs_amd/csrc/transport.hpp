@@ -429,27 +429,23 @@ __device__ __forceinline__ uint32_t howfar_phantom(const Env &env, const Part &p
         ir -= (uint32_t)iz * nxy;
         int iy = (int)(ir / nx);
         int ix = (int)(ir - (uint32_t)iy * nx);
+        // one division per axis, the face ahead chosen by the sign of the
+        // direction cosine (the same operands as the reference's two branches)
         uint32_t inew = p.region;
-        if (q.u > 0.0f) {
-            const float d = (xb[ix + 1] - q.x) / q.u;
-            if (d <= t) { t = d; inew = (++ix < (int)nx) ? p.region + 1 : 0u; }
-        } else if (q.u < 0.0f) {
-            const float d = (xb[ix] - q.x) / q.u;
-            if (d <= t) { t = d; inew = (--ix >= 0) ? p.region - 1 : 0u; }
+        if (q.u != 0.0f) {
+            const bool up = q.u > 0.0f;
+            const float d = (xb[ix + (up ? 1 : 0)] - q.x) / q.u;
+            if (d <= t) { t = d; inew = up ? ((ix + 1 < (int)nx) ? p.region + 1 : 0u) : ((ix - 1 >= 0) ? p.region - 1 : 0u); }
         }
-        if (q.v > 0.0f) {
-            const float d = (yb[iy + 1] - q.y) / q.v;
-            if (d <= t) { t = d; inew = (++iy < (int)ny) ? p.region + nx : 0u; }
-        } else if (q.v < 0.0f) {
-            const float d = (yb[iy] - q.y) / q.v;
-            if (d <= t) { t = d; inew = (--iy >= 0) ? p.region - nx : 0u; }
+        if (q.v != 0.0f) {
+            const bool up = q.v > 0.0f;
+            const float d = (yb[iy + (up ? 1 : 0)] - q.y) / q.v;
+            if (d <= t) { t = d; inew = up ? ((iy + 1 < (int)ny) ? p.region + nx : 0u) : ((iy - 1 >= 0) ? p.region - nx : 0u); }
         }
-        if (q.w > 0.0f) {
-            const float d = (zb[iz + 1] - q.z) / q.w;
-            if (d <= t) { t = d; inew = (++iz < (int)nz) ? p.region + nxy : 0u; }
-        } else if (q.w < 0.0f) {
-            const float d = (zb[iz] - q.z) / q.w;
-            if (d <= t) { t = d; inew = (--iz >= 0) ? p.region - nxy : 0u; }
+        if (q.w != 0.0f) {
+            const bool up = q.w > 0.0f;
+            const float d = (zb[iz + (up ? 1 : 0)] - q.z) / q.w;
+            if (d <= t) { t = d; inew = up ? ((iz + 1 < (int)nz) ? p.region + nxy : 0u) : ((iz - 1 >= 0) ? p.region - nxy : 0u); }
         }
         if (t < 0.0f) t = EPSTEP;
         return inew;
@@ -704,29 +700,34 @@ __device__ __forceinline__ bool photon_walk_iter(const Env &env, Stream &s, Part
     if (inside) {
         float qx, qy, qz;
         local_position(field_of(env, p), p, env.fixes, qx, qy, qz);
+        // One division per axis: the face ahead is bound index + 1 for a
+        // positive direction cosine and + 0 for a negative one, the same
+        // operands the reference divides in either case, so lanes moving
+        // either way share one instruction stream instead of two branches.
         const uint32_t nx = env.nx, nxy = env.nx * env.ny;
-        if (w.lu > 0.0f) {
-            dmin = (env.xb[w.ix + 1] - qx) / w.lu;
-            ncand = (w.ix + 1 < (int)nx) ? p.region + 1 : 0u;
-            axis = 0;
-        } else if (w.lu < 0.0f) {
-            dmin = (env.xb[w.ix] - qx) / w.lu;
-            ncand = (w.ix - 1 >= 0) ? p.region - 1 : 0u;
-            axis = 1;
+        if (w.lu != 0.0f) {
+            const bool up = w.lu > 0.0f;
+            dmin = (env.xb[w.ix + (up ? 1 : 0)] - qx) / w.lu;
+            ncand = up ? ((w.ix + 1 < (int)nx) ? p.region + 1 : 0u) : ((w.ix - 1 >= 0) ? p.region - 1 : 0u);
+            axis = up ? 0 : 1;
         }
-        if (w.lv > 0.0f) {
-            const float d = (env.yb[w.iy + 1] - qy) / w.lv;
-            if (axis < 0 || d <= dmin) { dmin = d; ncand = (w.iy + 1 < (int)env.ny) ? p.region + nx : 0u; axis = 2; }
-        } else if (w.lv < 0.0f) {
-            const float d = (env.yb[w.iy] - qy) / w.lv;
-            if (axis < 0 || d <= dmin) { dmin = d; ncand = (w.iy - 1 >= 0) ? p.region - nx : 0u; axis = 3; }
+        if (w.lv != 0.0f) {
+            const bool up = w.lv > 0.0f;
+            const float d = (env.yb[w.iy + (up ? 1 : 0)] - qy) / w.lv;
+            if (axis < 0 || d <= dmin) {
+                dmin = d;
+                ncand = up ? ((w.iy + 1 < (int)env.ny) ? p.region + nx : 0u) : ((w.iy - 1 >= 0) ? p.region - nx : 0u);
+                axis = up ? 2 : 3;
+            }
         }
-        if (w.lw > 0.0f) {
-            const float d = (env.zb[w.iz + 1] - qz) / w.lw;
-            if (axis < 0 || d <= dmin) { dmin = d; ncand = (w.iz + 1 < (int)env.nz) ? p.region + nxy : 0u; axis = 4; }
-        } else if (w.lw < 0.0f) {
-            const float d = (env.zb[w.iz] - qz) / w.lw;
-            if (axis < 0 || d <= dmin) { dmin = d; ncand = (w.iz - 1 >= 0) ? p.region - nxy : 0u; axis = 5; }
+        if (w.lw != 0.0f) {
+            const bool up = w.lw > 0.0f;
+            const float d = (env.zb[w.iz + (up ? 1 : 0)] - qz) / w.lw;
+            if (axis < 0 || d <= dmin) {
+                dmin = d;
+                ncand = up ? ((w.iz + 1 < (int)env.nz) ? p.region + nxy : 0u) : ((w.iz - 1 >= 0) ? p.region - nxy : 0u);
+                axis = up ? 4 : 5;
+            }
         }
         if (axis >= 0) cls_next = env.rcls[ncand];
     }
