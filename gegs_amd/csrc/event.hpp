@@ -500,7 +500,7 @@ __global__ void __launch_bounds__(512) photon_kernel(EvArgs a, uint32_t chunk, u
     BankSink sink{&a, 0, flip, false, true};
     HoldSink hold{false};
     uint32_t pops = 0;
-    bool have = false;
+    bool have = false, interact = false;
     Part p;
     Stream s;
     PhotonWalk w;
@@ -513,6 +513,12 @@ __global__ void __launch_bounds__(512) photon_kernel(EvArgs a, uint32_t chunk, u
         const uint32_t nidle = (uint32_t)__popcll(__ballot(!have));
         if (nidle && (nidle >= refill_min || nidle == 64u || rf.exhausted) && __any(hold.held)) {
             if (hold.held) {
+                // flights that reached their interaction point choose it now,
+                // together, from their untouched streams
+                if (interact) {
+                    photon_interaction(env, s, p, w, hold);
+                    interact = false;
+                }
                 sink.hist = hist;
                 sink.push(p);
                 hold.held = false;
@@ -535,9 +541,16 @@ __global__ void __launch_bounds__(512) photon_kernel(EvArgs a, uint32_t chunk, u
                 suspend(hold, s, p, P_PHOTON_STEP);
                 have = false;
                 flush_particle(a, hist, t);
-            } else if (!photon_walk_iter(env, s, p, w, hold, t)) {
-                have = false;
-                flush_particle(a, hist, t);
+            } else {
+                const int r = photon_walk_iter(env, s, p, w, hold, t);
+                if (r != PW_ON) {
+                    have = false;
+                    flush_particle(a, hist, t);
+                    if (r == PW_INTERACT) {
+                        interact = true;
+                        hold.held = true;
+                    }
+                }
             }
             iters++;
         }

@@ -628,7 +628,14 @@ struct PhotonWalk {
     float gmfpr0, cohf;      // interpolated gmfp and cohe of medium cmed
     float lu, lv, lw;        // direction in the phantom frame
     int32_t ix, iy, iz;      // voxel indices, valid when p.region >= VAC_AND_OTH
+    float cohfac;            // Rayleigh factor of the last step (read by photon_interaction)
 };
+
+// photon_walk_iter's outcomes: the flight ended (absorbed below the cut,
+// discarded, or handed on), goes on, or reached its interaction point
+constexpr int PW_END = 0;
+constexpr int PW_ON = 1;
+constexpr int PW_INTERACT = 2;
 
 __device__ __forceinline__ void local_position(const gegs_field &f, const Part &p, uint32_t fixes, float &qx, float &qy,
                                                float &qz) {
@@ -676,15 +683,18 @@ __device__ __forceinline__ void photon_walk_begin(const Env &env, const Part &p,
     if (p.region >= VAC_AND_OTH) decode_voxel(env, p.region, w.ix, w.iy, w.iz);
 }
 
-// one free-flight segment; true: the photon crossed a boundary and flies on
+// One free-flight segment: PW_ON when the photon crossed a boundary and flies
+// on, PW_INTERACT at the interaction point (photon_interaction then chooses
+// the interaction from the same stream; the event path defers that choice
+// until the wavefront hands its ended flights on together), PW_END otherwise.
 template <class Sink, class T>
-__device__ __forceinline__ bool photon_walk_iter(const Env &env, Stream &s, Part &p, PhotonWalk &w, Sink &sink, T &t) {
+__device__ __forceinline__ int photon_walk_iter(const Env &env, Stream &s, Part &p, PhotonWalk &w, Sink &sink, T &t) {
     const gegs_media_tables *mt = env.mt;
     t.photon_steps++;
     t.steps++;
     gegs_region rd = w.rd;
-    if (p.e <= rd.pcut) { t.score(p.region, p.e * p.wt); return false; }
-    if (p.wt <= 0.0f) return false;
+    if (p.e <= rd.pcut) { t.score(p.region, p.e * p.wt); return PW_END; }
+    if (p.wt <= 0.0f) return PW_END;
     // Inside the phantom the face the ray leaves the voxel through does not
     // depend on the sampled path, so it is found first and the class of the
     // voxel behind it is requested before the uniform is drawn; the load is
@@ -771,7 +781,7 @@ __device__ __forceinline__ bool photon_walk_iter(const Env &env, Stream &s, Part
     }
     int idisc = 0;
     if (nreg == 0) idisc = (rd.med == VACUUM || p.leaf == PASSMLC) ? 1 : -1;
-    if (idisc > 0) { p.region = 0; return false; }
+    if (idisc > 0) { p.region = 0; return PW_END; }
     p.x += p.u * tstep;
     p.y += p.v * tstep;
     p.z += p.w * tstep;
@@ -791,21 +801,36 @@ __device__ __forceinline__ bool photon_walk_iter(const Env &env, Stream &s, Part
         w.rd = rd;
         t.gathers++;
     }
-    if (p.e <= rd.pcut || idisc < 0) { t.score(p.region, p.e * p.wt); return false; }
-    if (rd.med != old_med || rd.med == VACUUM || (double)dpmfp >= EPSGMFP) return true;
+    if (p.e <= rd.pcut || idisc < 0) { t.score(p.region, p.e * p.wt); return PW_END; }
+    if (rd.med != old_med || rd.med == VACUUM || (double)dpmfp >= EPSGMFP) return PW_ON;
+    // the interaction point: the medium is the one the step started in, so
+    // w.lgle and w.gle index its rows; the Rayleigh factor is the step's
+    w.cohfac = cohfac;
+    return PW_INTERACT;
+}
+
+// The interaction at the end of a flight (photon.cu:229-271): Rayleigh with
+// probability 1 - cohfac in a region with Rayleigh on, else pair, Compton or
+// photo-absorption from the branching ratios at ln E.  Called right after
+// photon_walk_iter returned PW_INTERACT, or later with the lane's stream,
+// particle and walk state untouched in between: the uniforms are the same.
+template <class Sink>
+__device__ __forceinline__ void photon_interaction(const Env &env, Stream &s, Part &p, const PhotonWalk &w, Sink &sink) {
+    const gegs_media_tables *mt = env.mt;
+    const gegs_region &rd = w.rd;
+    float r1;
     if (rd.flags & GEGS_F_RAYLEIGH) {
         r1 = s.next();
-        if (r1 < 1.0f - cohfac) { forward(sink, s, p, P_RAYLEIGH); return false; }
+        if (r1 < 1.0f - w.cohfac) { forward(sink, s, p, P_RAYLEIGH); return; }
     }
-    const gegs_float2 g1 = mt->gbr1[rd.med * MXGE + lgle];
-    const gegs_float2 g2 = mt->gbr2[rd.med * MXGE + lgle];
-    const float gbr1 = g1.x + g1.y * gle;
-    const float gbr2 = g2.x + g2.y * gle;
+    const gegs_float2 g1 = mt->gbr1[rd.med * MXGE + w.lgle];
+    const gegs_float2 g2 = mt->gbr2[rd.med * MXGE + w.lgle];
+    const float gbr1 = g1.x + g1.y * w.gle;
+    const float gbr2 = g2.x + g2.y * w.gle;
     r1 = s.next();
     if (r1 <= gbr1 && p.e > 2.0f * RM) forward(sink, s, p, P_PAIR);
     else if (r1 < gbr2) forward(sink, s, p, P_COMPTON);
     else forward(sink, s, p, P_PHOTO);
-    return false;
 }
 
 // a flight interrupted between two crossings: the particle keeps its stream
@@ -825,7 +850,10 @@ __device__ __forceinline__ void photon_step(const Env &env, Stream &s, Part &p, 
     photon_walk_begin(env, p, w);
     for (;;) {
         wave_lookahead(s, true);
-        if (!photon_walk_iter(env, s, p, w, sink, t)) break;
+        const int r = photon_walk_iter(env, s, p, w, sink, t);
+        if (r == PW_ON) continue;
+        if (r == PW_INTERACT) photon_interaction(env, s, p, w, sink);
+        break;
     }
 }
 
