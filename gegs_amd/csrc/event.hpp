@@ -600,7 +600,7 @@ __global__ void __launch_bounds__(512) GEGS_WAVES_PER_EU(3) electron_kernel(EvAr
             iters = 0;
         }
         if (!__any(have)) break;
-        wave_lookahead(s, have);
+        wave_lookahead_eager(s, have);
         if (have) {
             sink.hist = hist;
             if (chunk && iters == chunk) {

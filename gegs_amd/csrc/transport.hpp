@@ -191,6 +191,18 @@ __device__ __forceinline__ void wave_lookahead(Stream &s, bool have) {
     }
 }
 
+// For steps that draw several uniforms (a condensed-history step draws five
+// on average): every lane without a block generated ahead makes it as soon as
+// any lane lacks one, so a step starts with its current block's rest plus a
+// whole block in registers and seldom has to generate one alone mid-step.
+// Each block is still generated exactly once; only the lanes that share the
+// evaluation change.  All lanes of the wavefront must reach this together.
+__device__ __forceinline__ void wave_lookahead_eager(Stream &s, bool have) {
+    if (__any(have && !s.ahead)) {
+        if (have) s.lookahead();
+    }
+}
+
 __device__ __forceinline__ uint64_t child_lineage(Stream &s) { return splitmix(s.child()); }
 
 // ------------------------------------------------------------------ environment
