@@ -1,13 +1,25 @@
 // gegs_run -- a prepared run of the reference's input, transported from C++.
 //
+//   gegs_run -i <file.ini> --media <tables.gegsmed> [--base-dir D] [--method batched|plain]
+//            [--seed S] [--batches B] [--fix Q1,Q4,Q7,Q8,Q15,Q16] [--dry-run <run file>] [--device N]
 //   gegs_run <run file> [--device N]
 //
-// The run file (key = value lines, written by `python -m gegs_amd.run -i
-// <file.ini> --prepare <run file>`) names the media-table file, the egsphant
-// phantom, the head file and the phase-space files of the run input, with its
-// field list and options.  This program does what the reference's main() and
-// runSimulation() do from there (CUDA_EGS.cu:367-900, :1712-1735), through the
-// library's C ABI only:
+// With -i it reads the reference's run input itself (CUDA_EGS.cu:936-1480):
+// the [General], [Data], [Phase space], [SecJaws], [MLC], [Block], [Wedge],
+// [Phantom], [Calibration] and [Output factor] keys with the reference's
+// defaults and error codes, the .egsinp components (SECJAWS, DYNVMLC,
+// PYRAMIDS), the .mlcinfo field list with its leaf files and the output-factor
+// table; the physics tables come from a media-table file (the HATCH builders'
+// output, like the reference's PEGS data: `python -m gegs_amd.hatch`).  A run
+// input that asks for ctcreate from DICOM needs the package's DICOM reader
+// (`python -m gegs_amd.run -i <file.ini> --prepare <run file>`, then the run
+// file form).  --dry-run writes the run file and head file instead of running.
+//
+// The run file (key = value lines) names the media-table file, the egsphant
+// phantom, the head file and the phase-space files, with the field list and
+// options.  From there this program does what the reference's main() and
+// runSimulation() do (CUDA_EGS.cu:367-900, :1712-1735), through the library's
+// C ABI only:
 //   1. media tables, egsphant phantom and its region records, head geometry;
 //   2. the phase-space headers, the batches of every field (nRepeat, batch
 //      size, the last batch) and the records they load, decoded once per
@@ -17,7 +29,9 @@
 //      field;
 //   4. the plan's deposit (each field times its output factor), the dose file
 //      <root>_DoseResults_NNNxNNNxNNN.bin and <root>_log.txt.
+#include <cctype>
 #include <chrono>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -25,6 +39,9 @@
 #include <map>
 #include <string>
 #include <vector>
+
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include "../../include/gegs.h"
 #include "../../include/gegs_host.h"
@@ -40,6 +57,9 @@ struct FieldSpec {
 
 struct RunFile {
     std::string dir, media, head, egsphant, phsp_base, root, method = "batched";
+    bool head_in_memory = false;     // the -i form builds the head here instead of reading a head file
+    gegs_head head_mem{};
+    std::vector<float> leaf_mem, jaw_mem;
     char other[5][GEGS_NAME_LEN] = {};
     uint32_t flags = GEGS_F_REFERENCE_DEFAULT, phsp_files = 1, host_fixes = 0, device_fixes = 0, batches = 10;
     int32_t change_xyz = 0, module_first = GEGS_M_PHANTOM;
@@ -48,8 +68,11 @@ struct RunFile {
     std::vector<FieldSpec> fields;
 };
 
+// logstr (output.c:41-49): every line to stdout and the log file; lines
+// logged before the log file is open are written to it when it opens
 struct Log {
     FILE *f = nullptr;
+    std::vector<std::string> pending;
     void operator()(const char *fmt, ...) {
         char buf[4096];
         va_list ap;
@@ -63,7 +86,17 @@ struct Log {
             fputs(buf, f);
             fputc('\n', f);
             fflush(f);
+        } else {
+            pending.push_back(buf);
         }
+    }
+    bool open(const std::string &path) {
+        f = fopen(path.c_str(), "w");
+        if (!f) return false;
+        for (const std::string &l : pending) fprintf(f, "%s\n", l.c_str());
+        pending.clear();
+        fflush(f);
+        return true;
     }
 };
 
@@ -74,14 +107,16 @@ std::string trim(const std::string &s) {
     return s.substr(a, s.find_last_not_of(ws) - a + 1);
 }
 
+// the reference's error(): the code and the message, then the exit (status 1;
+// the code itself does not fit an exit status)
 [[noreturn]] void die(int code, const char *fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
-    fputs("ERROR: ", stderr);
+    fprintf(stderr, "ERROR (%d): ", code);
     vfprintf(stderr, fmt, ap);
     fputc('\n', stderr);
     va_end(ap);
-    exit(code ? code : 1);
+    exit(1);
 }
 
 std::string resolve(const std::string &dir, const std::string &p) {
@@ -161,6 +196,436 @@ RunFile read_run_file(const char *path) {
     return r;
 }
 
+// ------------------------------------------------------------------ paths (posix rules)
+bool is_abs(const std::string &p) { return !p.empty() && p[0] == '/'; }
+
+std::string join(const std::string &a, const std::string &b) {
+    if (is_abs(b) || a.empty()) return b;
+    return a.back() == '/' ? a + b : a + "/" + b;
+}
+
+// normalised path: "." and empty parts dropped, ".." folded where it can be
+std::string normpath(const std::string &p) {
+    if (p.empty()) return ".";
+    const bool abs = is_abs(p);
+    std::vector<std::string> parts;
+    size_t i = 0;
+    while (i <= p.size()) {
+        size_t j = p.find('/', i);
+        if (j == std::string::npos) j = p.size();
+        const std::string c = p.substr(i, j - i);
+        if (c.empty() || c == ".") {
+        } else if (c == "..") {
+            if (!parts.empty() && parts.back() != "..") parts.pop_back();
+            else if (!abs) parts.push_back(c);
+        } else {
+            parts.push_back(c);
+        }
+        i = j + 1;
+    }
+    std::string out = abs ? "/" : "";
+    for (size_t k = 0; k < parts.size(); ++k) out += (k ? "/" : "") + parts[k];
+    return out.empty() ? "." : out;
+}
+
+std::string cwd() {
+    char buf[4096];
+    return getcwd(buf, sizeof buf) ? std::string(buf) : std::string(".");
+}
+
+std::string abspath(const std::string &p) { return normpath(is_abs(p) ? p : join(cwd(), p)); }
+
+std::string slashes(std::string s) {
+    for (char &c : s)
+        if (c == '\\') c = '/';
+    return s;
+}
+
+// a file name of the run input on this host: backslashes as separators,
+// relative names against base (the reference resolves them with
+// GetFullPathName against its working directory)
+std::string host_path(const std::string &base, const std::string &name) {
+    const std::string n = slashes(name);
+    return is_abs(n) ? n : normpath(join(base, n));
+}
+
+bool exists(const std::string &p) {
+    FILE *f = fopen(p.c_str(), "rb");
+    if (!f) return false;
+    fclose(f);
+    return true;
+}
+
+// ------------------------------------------------------------------ the run input (.ini)
+const char *kMissing = "<<missing>>";
+
+// GetPrivateProfileString's reading: section and key names compare
+// case-insensitively, blanks around them and around the value are dropped,
+// one pair of enclosing double quotes is removed; a repeated key keeps its
+// first value
+struct Ini {
+    std::map<std::string, std::map<std::string, std::string>> sec;
+    std::string path;
+    static std::string lower(std::string s) {
+        for (char &c : s) c = (char)tolower((unsigned char)c);
+        return s;
+    }
+    std::string get(const char *section, const char *key) const {
+        auto a = sec.find(lower(section));
+        if (a == sec.end()) return kMissing;
+        auto b = a->second.find(lower(key));
+        return b == a->second.end() ? std::string(kMissing) : b->second;
+    }
+    std::string req(const char *section, const char *key, int code) const {
+        const std::string v = get(section, key);
+        if (v == kMissing)
+            die(code, "Property \"%s\" is not specified in the section \"%s\" in the input file \"%s\".", key, section,
+                path.c_str());
+        return v;
+    }
+};
+
+Ini read_ini(const char *path) {
+    Ini ini;
+    ini.path = path;
+    FILE *f = fopen(path, "rb");
+    if (!f) die(1001, "The input file \"%s\" does not exist.", path);
+    std::string text;
+    char buf[65536];
+    size_t n;
+    while ((n = fread(buf, 1, sizeof buf, f)) > 0) text.append(buf, n);
+    fclose(f);
+    std::string cur;
+    bool have = false;
+    size_t i = 0;
+    while (i < text.size()) {
+        size_t j = text.find('\n', i);
+        if (j == std::string::npos) j = text.size();
+        const std::string line = trim(text.substr(i, j - i));
+        i = j + 1;
+        if (line.empty() || line[0] == ';' || line[0] == '#') continue;
+        if (line[0] == '[' && line.find(']') != std::string::npos) {
+            cur = Ini::lower(trim(line.substr(1, line.find(']') - 1)));
+            ini.sec[cur];
+            have = true;
+            continue;
+        }
+        const size_t eq = line.find('=');
+        if (!have || eq == std::string::npos) continue;
+        const std::string k = Ini::lower(trim(line.substr(0, eq)));
+        std::string v = trim(line.substr(eq + 1));
+        if (v.size() >= 2 && v.front() == '"' && v.back() == '"') v = v.substr(1, v.size() - 2);
+        ini.sec[cur].emplace(k, v);
+    }
+    return ini;
+}
+
+// sscanf's "%f" / "%d" conversions of n blank-separated numbers: each takes
+// the longest number at its position (floats rounded to binary32 through
+// binary64); false when fewer than n convert
+bool scan_numbers(const std::string &v, size_t n, bool integer, std::vector<double> &out) {
+    out.clear();
+    size_t p = 0;
+    for (size_t k = 0; k < n; ++k) {
+        while (p < v.size() && isspace((unsigned char)v[p])) ++p;
+        const size_t s0 = p;
+        if (p < v.size() && (v[p] == '+' || v[p] == '-')) ++p;
+        const size_t d0 = p;
+        while (p < v.size() && isdigit((unsigned char)v[p])) ++p;
+        bool digits = p > d0;
+        if (!integer) {
+            if (p < v.size() && v[p] == '.') {
+                ++p;
+                const size_t f0 = p;
+                while (p < v.size() && isdigit((unsigned char)v[p])) ++p;
+                digits = digits || p > f0;
+            }
+            if (digits && p < v.size() && (v[p] == 'e' || v[p] == 'E')) {
+                size_t q = p + 1;
+                if (q < v.size() && (v[q] == '+' || v[q] == '-')) ++q;
+                const size_t e0 = q;
+                while (q < v.size() && isdigit((unsigned char)v[q])) ++q;
+                if (q > e0) p = q;
+            }
+        }
+        if (!digits) return false;
+        const std::string t = v.substr(s0, p - s0);
+        out.push_back(integer ? (double)strtoll(t.c_str(), nullptr, 10) : (double)(float)strtod(t.c_str(), nullptr));
+    }
+    return true;
+}
+
+std::vector<double> need_numbers(const Ini &ini, const std::string &v, size_t n, bool integer, int code,
+                                 const char *what) {
+    std::vector<double> out;
+    if (!scan_numbers(v, n, integer, out))
+        die(code, "Could not parse the %s \"%s\" in the input file \"%s\".", what, v.c_str(), ini.path.c_str());
+    return out;
+}
+
+uint32_t parse_fixes(const std::string &list, uint32_t &host) {
+    uint32_t dev = 0;
+    host = 0;
+    size_t i = 0;
+    while (i <= list.size()) {
+        size_t j = list.find(',', i);
+        if (j == std::string::npos) j = list.size();
+        std::string f = Ini::lower(trim(list.substr(i, j - i)));
+        i = j + 1;
+        if (f.empty()) continue;
+        if (f == "q1") dev |= GEGS_FIX_Q1;
+        else if (f == "q7") dev |= GEGS_FIX_Q7;
+        else if (f == "q8") dev |= GEGS_FIX_Q8;
+        else if (f == "q4") host |= GEGS_HOST_FIX_Q4;
+        else if (f == "q15") host |= GEGS_HOST_FIX_Q15;
+        else if (f == "q16") host |= GEGS_HOST_FIX_Q16;
+        else die(2001, "unknown fix \"%s\" (known here: Q1, Q4, Q7, Q8, Q15, Q16)", f.c_str());
+    }
+    return dev;
+}
+
+// The run input's transport part (CUDA_EGS.cu:936-1480 for the keys,
+// init.cu:345-905 and :1377-1502 for the head and fields), as run.prepare
+// restates it.
+RunFile from_ini(const char *path, const std::string &media, const std::string &base_dir, const std::string &method,
+                 uint64_t seed, uint32_t batches, const std::string &fixes, Log &log) {
+    const Ini ini = read_ini(path);
+    RunFile r;
+    r.media = media;
+    r.method = method;
+    r.seed = seed;
+    r.batches = batches;
+    r.device_fixes = parse_fixes(fixes, r.host_fixes);
+    const std::string base = base_dir.empty() ? cwd() : base_dir;
+    std::string prefix = ini.get("General", "output prefix");
+    if (prefix == kMissing) {
+        prefix = "GEGS";
+        log("WARNING: No output prefix was specified. Default value is used.");
+    }
+    std::string outdir = ini.get("General", "output directory");
+    if (outdir == kMissing || outdir.empty()) {
+        outdir = ".";
+        log("WARNING: No output directory was specified. Current Working Directory is used.");
+    }
+    for (const char *k : {"GPU number", "GPU id"}) {
+        const std::string g = ini.get("General", k);
+        std::vector<double> tmp;
+        if (g != kMissing && !scan_numbers(g, 1, true, tmp))
+            die(2004, "The given %s \"%s\" could not be parsed as a 4-byte integer.", k, g.c_str());
+    }
+    const std::string h = ini.req("General", "histories", 2013);
+    std::vector<double> hv;
+    if (!scan_numbers(h, 1, false, hv)) die(2014, "Could not parse the number of histories \"%s\".", h.c_str());
+    const float hist = (float)hv[0];
+    if (!(hist > 0)) die(2015, "The number of histories must be positive, but %g was specified.", (double)hist);
+    r.histories_per_mu = (uint64_t)hist;
+    const std::string data_dir = host_path(base, ini.req("Data", "data directory", 2017));
+    const std::string egsinp = ini.req("Data", "egsinp file", 2011);
+    ini.req("Data", "pegs file", 2012);
+    ini.req("Data", "photon xsections", 2019);
+    ini.req("Data", "atomic ff file", 2020);
+    const std::string dmed = ini.req("Data", "default medium", 2013);
+    const std::string nph = ini.req("Phase space", "phsp file number", 5013);
+    std::vector<double> v;
+    if (!scan_numbers(nph, 1, true, v)) die(5014, "Could not parse phsp file number \"%s\".", nph.c_str());
+    r.phsp_files = (uint32_t)v[0];
+    const std::string phsp = ini.req("Phase space", "phase space file", 2012);
+    const std::string zs = ini.req("Phase space", "z position", 5013);
+    if (!scan_numbers(zs, 1, false, v)) die(5014, "Could not parse the z position \"%s\".", zs.c_str());
+    r.phsp_z = (float)v[0];
+    const bool secjaws = ini.get("SecJaws", "module SecJaws") == "On";
+    const std::string secjaws_file = ini.req("SecJaws", "secjaws file", 2011);
+    const bool mlc_on = ini.get("MLC", "module MLC") == "On";
+    const std::string mlcinfo = ini.req("MLC", "mlcinfo file", 2011);
+    const bool block_on = ini.get("Block", "module Block") == "On";
+    const bool wedge_on = ini.get("Wedge", "module Wedge") == "On";
+    if (block_on) ini.req("Block", "block file", 2011);
+    if (wedge_on) ini.req("Wedge", "wedge file", 2011);
+    if (ini.get("CTcreate", "use ctcreate") == "Yes")
+        die(2011, "\"%s\" creates its phantom from DICOM (use ctcreate = Yes): prepare it with `python -m gegs_amd.run "
+                  "-i %s --prepare <run file>` and run the run file",
+            path, path);
+    const std::string egsphant = ini.req("Phantom", "egsphant file", 2011);
+    r.change_xyz = ini.get("Phantom", "change coordinate") != "No" ? 1 : 0;
+    v = need_numbers(ini, ini.req("Phantom", "isocenter location", 2011), 3, false, 2011, "isocenter location");
+    for (int k = 0; k < 3; ++k) r.iso[k] = (float)v[k];
+    v = need_numbers(ini, ini.req("Phantom", "source to isocenter", 2011), 1, false, 2011, "source to isocenter");
+    r.sid = (float)v[0];
+    float dose_unit = 1.6e-8f;                                             // CUDA_EGS.h:536, MeV/g -> cGy
+    if (ini.get("Calibration", "dose calibration") == "Yes") {
+        const std::string hp = ini.req("Calibration", "histories per cGy", 5013);
+        if (!scan_numbers(hp, 1, false, v)) die(5014, "Could not parse histories per cGy \"%s\".", hp.c_str());
+        float du = dose_unit * (float)v[0];
+        dose_unit = du / hist;                                             // CUDA_EGS.cu:1478, in float
+    }
+    r.dose_unit = dose_unit;
+    const bool of_on = ini.get("Output factor", "use output factor") == "Yes";
+    std::string of_file;
+    if (of_on) of_file = ini.req("Output factor", "output factor file", 2012);
+    if (ini.get("Interpolation", "do interpolate") == "Yes" || ini.get("Dose output", "absolute dose") == "Yes" ||
+        ini.get("Dose output", "relative dose") == "Yes")
+        log("NOTE: interpolated, absolute and relative dose outputs are made from the dose file by gegs_amd.postproc");
+    auto d = [&](const std::string &name) { return join(data_dir, slashes(name)); };
+    const std::string out_dir = host_path(base, outdir);
+    r.root = abspath(join(out_dir, prefix));
+    r.egsphant = abspath(d(egsphant));
+    r.phsp_base = abspath(join(data_dir, slashes(phsp)));
+
+    // head components, read whether or not the modules are on
+    gegs_head &hd = r.head_mem;
+    memset(&hd, 0, sizeof hd);
+    char jaw_med[GEGS_NAME_LEN], mlc_med[GEGS_NAME_LEN], block_med[GEGS_NAME_LEN], wedge_med[GEGS_NAME_LEN];
+    for (char *m : {jaw_med, mlc_med, block_med, wedge_med}) {
+        memset(m, 0, GEGS_NAME_LEN);
+        strncpy(m, dmed.c_str(), GEGS_NAME_LEN - 1);
+    }
+    if (exists(d(secjaws_file))) {
+        const int rc = gegs_read_secjaws(d(secjaws_file).c_str(), &hd.jaws, jaw_med);
+        if (rc != GEGS_OK && rc != GEGS_NOT_FOUND) die(rc, "%s", gegs_input_last_error());
+        hd.has_jaws = rc == GEGS_OK;
+        if (rc == GEGS_NOT_FOUND) strncpy(jaw_med, dmed.c_str(), GEGS_NAME_LEN - 1);
+    }
+    if (secjaws && !hd.has_jaws) die(2011, "no JAWS component SECJAWS in \"%s\"", d(secjaws_file).c_str());
+    if (exists(d(egsinp))) {
+        char med[GEGS_NAME_LEN];
+        if (gegs_read_dynvmlc(d(egsinp).c_str(), &hd.mlc, med) == GEGS_OK) {
+            hd.has_mlc = 1;
+            memcpy(mlc_med, med, GEGS_NAME_LEN);
+        } else {
+            memset(&hd.mlc, 0, sizeof hd.mlc);
+        }
+    }
+    if (mlc_on && !hd.has_mlc) die(2011, "no DYNVMLC component in \"%s\"", d(egsinp).c_str());
+    float block_coord[10], wedge_rmax = 0, layers[24];
+    uint32_t n_layers = 0;
+    if (block_on) {
+        const std::string bf = d(trim(ini.get("Block", "block file")));
+        const int rc = gegs_read_block(bf.c_str(), block_coord, block_med);
+        if (rc == GEGS_NOT_FOUND) die(2011, "no PYRAMIDS component BLOCK in the block file");
+        if (rc) die(rc, "%s", gegs_input_last_error());
+    }
+    if (wedge_on) {
+        const std::string wf = d(trim(ini.get("Wedge", "wedge file")));
+        const int rc = gegs_read_wedge(wf.c_str(), &wedge_rmax, layers, &n_layers, wedge_med);
+        if (rc == GEGS_NOT_FOUND) die(2011, "no PYRAMIDS component WEDGE in the wedge file");
+        if (rc) die(rc, "%s", gegs_input_last_error());
+    }
+    if (block_on || wedge_on) {
+        const int rc = gegs_build_block_wedge(block_on ? block_coord : nullptr, wedge_rmax, wedge_on ? layers : nullptr,
+                                              wedge_on ? n_layers : 0, &hd.block_wedge);
+        if (rc) die(rc, "%s", gegs_input_last_error());
+        hd.has_block_wedge = 1;
+    }
+    const char *others[5] = {dmed.c_str(), jaw_med, mlc_med, block_med, wedge_med};
+    for (int i = 0; i < 5; ++i) {
+        memset(r.other[i], 0, GEGS_NAME_LEN);
+        strncpy(r.other[i], others[i], GEGS_NAME_LEN - 1);
+    }
+    r.module_first = secjaws ? GEGS_M_SECJAWY : mlc_on ? GEGS_M_MLC : block_on ? GEGS_M_BLOCK
+                   : wedge_on ? GEGS_M_WEDGE : GEGS_M_PHANTOM;
+
+    // fields (read_mlcinp)
+    const std::string mi = d(mlcinfo);
+    if (!exists(mi)) die(2011, "Error: MLC info  file reading error! No simulation will be done!");
+    uint32_t nf = 0;
+    if (gegs_read_mlcinfo(mi.c_str(), nullptr, 0, &nf)) die(2011, "%s", gegs_input_last_error());
+    std::vector<gegs_mlc_field> mf(nf ? nf : 1);
+    if (gegs_read_mlcinfo(mi.c_str(), mf.data(), nf, &nf)) die(2011, "%s", gegs_input_last_error());
+    const uint32_t kMaxFields = 20;                                        // CUDA_EGS.h:266
+    if (nf > kMaxFields) {
+        log("Warning! number of MLC fields is larger than the MAXFIELDS=%u", kMaxFields);
+        nf = kMaxFields;
+    }
+    std::vector<float> sizes(64), factors(64);
+    uint32_t nof = 0;
+    if (of_on) {
+        if (gegs_read_output_factors(d(of_file).c_str(), nullptr, nullptr, 0, &nof)) die(2011, "%s", gegs_input_last_error());
+        sizes.resize(nof ? nof : 1);
+        factors.resize(nof ? nof : 1);
+        if (gegs_read_output_factors(d(of_file).c_str(), sizes.data(), factors.data(), nof, &nof))
+            die(2011, "%s", gegs_input_last_error());
+    }
+    r.leaf_mem.assign((size_t)nf * 2 * GEGS_MLC_LEAVES, 0.0f);
+    r.jaw_mem.assign((size_t)nf * 4, 0.0f);
+    for (uint32_t i = 0; i < nf; ++i) {
+        const gegs_mlc_field &f = mf[i];
+        if (gegs_read_leaf_file(d(f.leaf_file).c_str(), r.leaf_mem.data() + (size_t)i * 2 * GEGS_MLC_LEAVES))
+            die(2011, "%s", gegs_input_last_error());
+        const float jp[4] = {(float)f.x1, (float)f.x2, (float)f.y1, (float)f.y2};
+        memcpy(r.jaw_mem.data() + (size_t)i * 4, jp, sizeof jp);
+        FieldSpec fs;
+        fs.mu = (uint64_t)f.mu;
+        fs.angle = (float)f.angle;
+        // CUDA_EGS.cu:424: the angle in radians as binary32, its cos and sin
+        // in binary64 rounded once
+        const float a = (float)((double)fs.angle * 3.1415926 / 180.0);
+        fs.cos_a = (float)std::cos((double)a);
+        fs.sin_a = (float)std::sin((double)a);
+        if (of_on) {
+            float of = 0;
+            const int rc = gegs_field_output_factor(f.x1, f.x2, f.y1, f.y2, sizes.data(), factors.data(), nof, &of);
+            if (rc == GEGS_NOT_FOUND)
+                die(2011, "field #%u: its equivalent square lies outside the output-factor table (the reference "
+                          "leaves its factor unset)", i);
+            fs.has_of = true;
+            fs.output_factor = of;
+        }
+        r.fields.push_back(fs);
+    }
+    hd.n_sets = nf;
+    hd.leaf_positions = hd.has_mlc ? r.leaf_mem.data() : nullptr;
+    hd.jaw_positions = hd.has_jaws ? r.jaw_mem.data() : nullptr;
+    r.head_in_memory = true;
+    return r;
+}
+
+// the run file of a run (the format read_run_file reads and run.prepare writes)
+void write_run_file(const RunFile &r, const char *path, const char *source) {
+    const std::string stem = [&] {
+        std::string p = abspath(path);
+        const size_t slash = p.rfind('/'), dot = p.rfind('.');
+        return (dot != std::string::npos && (slash == std::string::npos || dot > slash)) ? p.substr(0, dot) : p;
+    }();
+    const bool any_head = r.head_mem.has_mlc || r.head_mem.has_jaws || r.head_mem.has_block_wedge;
+    const std::string head_file = stem + ".gegshead";
+    if (any_head) {
+        const int rc = gegs_head_file_write(head_file.c_str(), &r.head_mem);
+        if (rc) die(rc, "%s", gegs_host_last_error());
+    }
+    FILE *f = fopen(path, "w");
+    if (!f) die(2001, "Could not write the run file \"%s\".", path);
+    auto g = [](float x) {
+        char b[64];
+        snprintf(b, sizeof b, "%.9g", (double)x);
+        return std::string(b);
+    };
+    fprintf(f, "# gegs run file: %s prepared for gegs_amd/gegs_run\n", abspath(source).c_str());
+    fprintf(f, "media file = %s\n", abspath(r.media).c_str());
+    if (any_head) fprintf(f, "head file = %s\n", head_file.c_str());
+    fprintf(f, "egsphant file = %s\n", r.egsphant.c_str());
+    for (int i = 0; i < 5; ++i) fprintf(f, "region medium %d = %s\n", i + 1, r.other[i]);
+    fprintf(f, "region flags = 0x%04x\n", r.flags);
+    fprintf(f, "phsp base = %s\n", r.phsp_base.c_str());
+    fprintf(f, "phsp files = %u\n", r.phsp_files);
+    fprintf(f, "phsp z = %s\n", g(r.phsp_z).c_str());
+    fprintf(f, "host fixes = %u\n", r.host_fixes);
+    fprintf(f, "device fixes = %u\n", r.device_fixes);
+    fprintf(f, "change xyz = %d\n", r.change_xyz);
+    fprintf(f, "module first = %d\n", r.module_first);
+    fprintf(f, "source to isocenter = %s\n", g(r.sid).c_str());
+    fprintf(f, "isocenter = %s %s %s\n", g(r.iso[0]).c_str(), g(r.iso[1]).c_str(), g(r.iso[2]).c_str());
+    fprintf(f, "histories per mu = %llu\n", (unsigned long long)r.histories_per_mu);
+    fprintf(f, "dose unit = %s\n", g(r.dose_unit).c_str());
+    fprintf(f, "output root = %s\n", r.root.c_str());
+    fprintf(f, "seed = %llu\n", (unsigned long long)r.seed);
+    fprintf(f, "batches = %u\n", r.batches);
+    fprintf(f, "method = %s\n", r.method.c_str());
+    for (const FieldSpec &fs : r.fields)
+        fprintf(f, "field = %llu %s %s %s %s\n", (unsigned long long)fs.mu, g(fs.angle).c_str(), g(fs.cos_a).c_str(),
+                g(fs.sin_a).c_str(), fs.has_of ? g(fs.output_factor).c_str() : "-");
+    fclose(f);
+}
+
 void check(gegs_ctx *ctx, int rc, const char *what) {
     if (rc) die(rc, "%s: %s", what, ctx ? gegs_last_error(ctx) : "");
 }
@@ -172,16 +637,55 @@ void check_host(int rc, const char *what) {
 }  // namespace
 
 int main(int argc, char **argv) {
-    if (argc < 2) die(2001, "usage: gegs_run <run file> [--device N]");
+    const char *usage = "usage: gegs_run -i <file.ini> --media <tables.gegsmed> [--base-dir D] [--method batched|plain] "
+                        "[--seed S] [--batches B] [--fix Q1,Q4,...] [--dry-run <run file>] [--device N]\n"
+                        "       gegs_run <run file> [--device N]";
+    if (argc < 2) die(2001, "%s", usage);
     int device = 0;
-    for (int i = 2; i < argc; ++i)
-        if (!strcmp(argv[i], "--device") && i + 1 < argc) device = atoi(argv[++i]);
+    const char *ini_path = nullptr, *dry_run = nullptr;
+    std::string media, base_dir, method = "batched", fixes;
+    uint64_t seed = 1;
+    uint32_t batches = 10;
+    for (int i = 1; i < argc; ++i) {
+        const bool more = i + 1 < argc;
+        if (!strcmp(argv[i], "--device") && more) device = atoi(argv[++i]);
+        else if (!strcmp(argv[i], "-i") && more) ini_path = argv[++i];
+        else if (!strcmp(argv[i], "--media") && more) media = argv[++i];
+        else if (!strcmp(argv[i], "--base-dir") && more) base_dir = argv[++i];
+        else if (!strcmp(argv[i], "--method") && more) method = argv[++i];
+        else if (!strcmp(argv[i], "--seed") && more) seed = strtoull(argv[++i], nullptr, 10);
+        else if (!strcmp(argv[i], "--batches") && more) batches = (uint32_t)strtoul(argv[++i], nullptr, 10);
+        else if (!strcmp(argv[i], "--fix") && more) fixes = argv[++i];
+        else if (!strcmp(argv[i], "--dry-run") && more) dry_run = argv[++i];
+        else if (i > 1 || argv[i][0] == '-') die(2001, "unknown argument \"%s\"\n%s", argv[i], usage);
+    }
     const auto t0 = std::chrono::steady_clock::now();
-    RunFile rf = read_run_file(argv[1]);
     Log log;
-    log.f = fopen((rf.root + "_log.txt").c_str(), "w");
-    if (!log.f) die(2001, "Could not open \"%s_log.txt\".", rf.root.c_str());
-    log("GEGS (MI355X, C++ host) run file %s", argv[1]);
+    RunFile rf;
+    if (ini_path) {
+        if (media.empty()) die(2001, "-i needs --media <tables.gegsmed> (python -m gegs_amd.hatch writes one)");
+        if (method != "batched" && method != "plain") die(2001, "method must be batched or plain");
+        log("GEGS (MI355X, C++ host) run input %s", ini_path);
+        rf = from_ini(ini_path, media, base_dir, method, seed, batches, fixes, log);
+        if (dry_run) {
+            write_run_file(rf, dry_run, ini_path);
+            printf("run file %s\n", dry_run);
+            return 0;
+        }
+        const std::string out_dir = rf.root.substr(0, rf.root.rfind('/'));
+        if (!out_dir.empty()) {
+            std::string acc;
+            for (size_t k = 0; k <= out_dir.size(); ++k)
+                if (k == out_dir.size() || (out_dir[k] == '/' && k > 0)) {
+                    acc = out_dir.substr(0, k);
+                    mkdir(acc.c_str(), 0777);
+                }
+        }
+    } else {
+        rf = read_run_file(argv[1]);
+        log("GEGS (MI355X, C++ host) run file %s", argv[1]);
+    }
+    if (!log.open(rf.root + "_log.txt")) die(2001, "Could not open \"%s_log.txt\".", rf.root.c_str());
 
     // media tables
     std::vector<unsigned char> tbuf(sizeof(gegs_media_tables));
@@ -206,7 +710,8 @@ int main(int argc, char **argv) {
     // head
     gegs_head head;
     memset(&head, 0, sizeof head);
-    if (!rf.head.empty()) check_host(gegs_head_file_read(rf.head.c_str(), &head), "head file");
+    if (rf.head_in_memory) head = rf.head_mem;
+    else if (!rf.head.empty()) check_host(gegs_head_file_read(rf.head.c_str(), &head), "head file");
     const uint32_t nf = (uint32_t)rf.fields.size();
     if ((head.has_mlc || head.has_jaws) && head.n_sets != nf)
         die(2001, "the head file holds %u aperture sets for %u fields", head.n_sets, nf);
@@ -374,7 +879,7 @@ int main(int argc, char **argv) {
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     log("Total CPU/GPU . . . . . . . . %.2f ms (%.2f min)", secs * 1e3, secs / 60.0);
     printf("%llu histories in %.2f s -> %s\n", (unsigned long long)transported, secs, fname);
-    gegs_head_free(&head);
+    if (!rf.head_in_memory) gegs_head_free(&head);
     gegs_egsphant_free(&ph);
     fclose(log.f);
     return 0;

@@ -1042,3 +1042,33 @@ class MediaTables:
 
     def index(self, name):
         return self.names.index(name)
+
+
+def main(argv=None):
+    """``python -m gegs_amd.hatch -o tables.gegsmed [--mlc-medium M] MEDIUM ...``:
+    the media tables of the named media written as a media-table file, the
+    physics input of the C++ run driver (gegs_amd/gegs_run -i <file.ini>
+    --media tables.gegsmed).  With --data-dir and the run input's --pegs,
+    --xsections and --ff names the EGSnrc data set there is used (as
+    ``python -m gegs_amd.run`` does when it finds it), else pegs-lite."""
+    import argparse
+    from . import hostio
+    ap = argparse.ArgumentParser(prog="python -m gegs_amd.hatch", description=main.__doc__.split("\n")[0])
+    ap.add_argument("media", nargs="+", help="medium names, in table order")
+    ap.add_argument("-o", dest="out", required=True, help="media-table file to write")
+    ap.add_argument("--mlc-medium", default=None, help="the MLC medium (range rejection on)")
+    ap.add_argument("--data-dir", default=None)
+    ap.add_argument("--pegs", default="700icru.pegs4dat")
+    ap.add_argument("--xsections", default="si")
+    ap.add_argument("--ff", default="pgs4form.dat")
+    a = ap.parse_args(argv)
+    files = reference_data_files(a.data_dir, a.pegs, a.xsections, a.ff) if a.data_dir else None
+    tables = MediaTables(a.media, mlc_medium=a.mlc_medium, files=files)
+    hostio.write_media_file(a.out, tables)
+    print("%s: %s" % (a.out, ", ".join(tables.names)))
+    return 0
+
+
+if __name__ == "__main__":
+    import sys
+    sys.exit(main())

@@ -151,8 +151,10 @@ def leaf_end_profile(zmin, zthick, rmax, rad_end):
     zc = zthick / 2.0
     th = math.asin(min(1.0, zc / rad_end))
     ang = np.linspace(-th, th, 15)
-    ys = rad_end * np.sin(ang)                   # along z, relative to the mid plane
-    xs = rad_end * (1.0 - np.cos(ang))           # retreat of the end from the tip
+    # the C library's sin and cos (as the C++ reader evaluates them), not
+    # numpy's vectorised ones, which may differ in the last binary64 place
+    ys = np.array([rad_end * math.sin(a) for a in ang])          # along z, relative to the mid plane
+    xs = np.array([rad_end * (1.0 - math.cos(a)) for a in ang])  # retreat of the end from the tip
     out = np.zeros((34, 2), dtype=np.float32)
     out[0] = (-rmax, zmin + zthick)
     out[1] = (-rmax, zmin)

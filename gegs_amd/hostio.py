@@ -23,7 +23,11 @@ HOST_SYMBOLS = [
     "gegs_egsphant_free", "gegs_voxel_mass", "gegs_region_table", "gegs_phsp_open", "gegs_phsp_total",
     "gegs_phsp_header_path", "gegs_phsp_read", "gegs_phsp_close", "gegs_field_schedule", "gegs_write_dose",
     "gegs_head_file_write", "gegs_head_file_read", "gegs_head_free",
+    "gegs_input_last_error", "gegs_read_secjaws", "gegs_read_dynvmlc", "gegs_read_block", "gegs_read_wedge",
+    "gegs_build_block_wedge", "gegs_read_mlcinfo", "gegs_read_leaf_file", "gegs_read_output_factors",
+    "gegs_field_output_factor",
 ]
+NOT_FOUND = 1
 
 
 class CEgsphant(ctypes.Structure):
@@ -45,6 +49,12 @@ class CHead(ctypes.Structure):
                 ("n_sets", ctypes.c_uint32), ("mlc", headmod.Mlc), ("jaws", headmod.Jaws),
                 ("block_wedge", headmod.BlockWedge), ("leaf_positions", ctypes.POINTER(ctypes.c_float)),
                 ("jaw_positions", ctypes.POINTER(ctypes.c_float))]
+
+
+class CMlcField(ctypes.Structure):
+    _fields_ = [("angle", ctypes.c_double), ("x1", ctypes.c_double), ("x2", ctypes.c_double),
+                ("y1", ctypes.c_double), ("y2", ctypes.c_double), ("mu", ctypes.c_double),
+                ("leaf_file", ctypes.c_char * 512)]
 
 
 class HostError(ValueError):
@@ -85,6 +95,16 @@ def lib():
     L.gegs_head_file_write.argtypes = [ctypes.c_char_p, ctypes.POINTER(CHead)]
     L.gegs_head_file_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(CHead)]
     L.gegs_head_free.argtypes = [ctypes.POINTER(CHead)]
+    L.gegs_input_last_error.restype = ctypes.c_char_p
+    L.gegs_read_secjaws.argtypes = [ctypes.c_char_p, vp, vp]
+    L.gegs_read_dynvmlc.argtypes = [ctypes.c_char_p, vp, vp]
+    L.gegs_read_block.argtypes = [ctypes.c_char_p, vp, vp]
+    L.gegs_read_wedge.argtypes = [ctypes.c_char_p, vp, vp, vp, vp]
+    L.gegs_build_block_wedge.argtypes = [vp, ctypes.c_float, vp, u32, vp]
+    L.gegs_read_mlcinfo.argtypes = [ctypes.c_char_p, vp, u32, vp]
+    L.gegs_read_leaf_file.argtypes = [ctypes.c_char_p, vp]
+    L.gegs_read_output_factors.argtypes = [ctypes.c_char_p, vp, vp, u32, vp]
+    L.gegs_field_output_factor.argtypes = [ctypes.c_double] * 4 + [vp, vp, u32, vp]
     _L = L
     return L
 
@@ -92,6 +112,15 @@ def lib():
 def _check(rc):
     if rc:
         raise HostError(rc, lib().gegs_host_last_error().decode())
+
+
+def _check_in(rc):
+    """Status of a head-input reader: None for GEGS_NOT_FOUND, raises on errors."""
+    if rc == NOT_FOUND:
+        return None
+    if rc:
+        raise HostError(rc, lib().gegs_input_last_error().decode())
+    return True
 
 
 def _names_block(names, n=MXMED):
@@ -268,3 +297,91 @@ def read_head_file(path):
         return out
     finally:
         lib().gegs_head_free(ctypes.byref(h))
+
+
+# ------------------------------------------------------------------ head inputs of a run
+def read_secjaws(path):
+    """(head.Jaws, medium) of the SECJAWS component, or None."""
+    j = headmod.Jaws()
+    med = ctypes.create_string_buffer(NAME_LEN)
+    if _check_in(lib().gegs_read_secjaws(path.encode(), ctypes.addressof(j), med)) is None:
+        return None
+    return j, med.value.decode()
+
+
+def read_dynvmlc(path):
+    """(head.Mlc built from the DYNVMLC component, leaf medium), or None."""
+    m = headmod.Mlc()
+    med = ctypes.create_string_buffer(NAME_LEN)
+    if _check_in(lib().gegs_read_dynvmlc(path.encode(), ctypes.addressof(m), med)) is None:
+        return None
+    return m, med.value.decode()
+
+
+def read_block(path):
+    c = np.zeros(10, dtype=np.float32)
+    med = ctypes.create_string_buffer(NAME_LEN)
+    if _check_in(lib().gegs_read_block(path.encode(), c.ctypes.data, med)) is None:
+        return None
+    return c, med.raw.split(b"\0", 1)[0].decode()
+
+
+def read_wedge(path):
+    rmax = ctypes.c_float()
+    layers = np.zeros(24, dtype=np.float32)
+    n = ctypes.c_uint32()
+    med = ctypes.create_string_buffer(NAME_LEN)
+    if _check_in(lib().gegs_read_wedge(path.encode(), ctypes.byref(rmax), layers.ctypes.data, ctypes.byref(n),
+                                       med)) is None:
+        return None
+    return rmax.value, layers[:6 * n.value].reshape(-1, 6).copy(), med.raw.split(b"\0", 1)[0].decode()
+
+
+def build_block_wedge(block=None, wedge=None):
+    """head.BlockWedge from block values (10) and/or wedge (rmax, layers)."""
+    bw = headmod.BlockWedge()
+    b = None if block is None else np.ascontiguousarray(block, dtype=np.float32)
+    rmax, layers, n = 0.0, None, 0
+    if wedge is not None:
+        rmax = float(wedge[0])
+        layers = np.ascontiguousarray(wedge[1], dtype=np.float32).reshape(-1)
+        n = len(layers) // 6
+    _check_in(lib().gegs_build_block_wedge(b.ctypes.data if b is not None else None, rmax,
+                                           layers.ctypes.data if layers is not None else None, n,
+                                           ctypes.addressof(bw)))
+    return bw
+
+
+def read_mlcinfo(path):
+    n = ctypes.c_uint32()
+    _check_in(lib().gegs_read_mlcinfo(path.encode(), None, 0, ctypes.byref(n)))
+    arr = (CMlcField * max(1, n.value))()
+    _check_in(lib().gegs_read_mlcinfo(path.encode(), ctypes.addressof(arr), n.value, ctypes.byref(n)))
+    return [dict(angle=f.angle, x1=f.x1, x2=f.x2, y1=f.y1, y2=f.y2, mu=f.mu, leaf_file=f.leaf_file.decode())
+            for f in arr[:n.value]]
+
+
+def read_leaf_file(path):
+    out = np.zeros(120, dtype=np.float32)
+    _check_in(lib().gegs_read_leaf_file(path.encode(), out.ctypes.data))
+    return out
+
+
+def read_output_factors(path):
+    n = ctypes.c_uint32()
+    _check_in(lib().gegs_read_output_factors(path.encode(), None, None, 0, ctypes.byref(n)))
+    sizes = np.zeros(max(1, n.value), dtype=np.float32)
+    factors = np.zeros(max(1, n.value), dtype=np.float32)
+    _check_in(lib().gegs_read_output_factors(path.encode(), sizes.ctypes.data, factors.ctypes.data, n.value,
+                                             ctypes.byref(n)))
+    return sizes[:n.value], factors[:n.value]
+
+
+def field_output_factor(x1, x2, y1, y2, sizes, factors):
+    sizes = np.ascontiguousarray(sizes, dtype=np.float32)
+    factors = np.ascontiguousarray(factors, dtype=np.float32)
+    of = ctypes.c_float()
+    if _check_in(lib().gegs_field_output_factor(float(x1), float(x2), float(y1), float(y2), sizes.ctypes.data,
+                                                factors.ctypes.data, len(sizes), ctypes.byref(of))) is None:
+        return None
+    return np.float32(of.value)

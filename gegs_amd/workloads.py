@@ -30,8 +30,10 @@ class Field(ctypes.Structure):
 
 def make_field(change_xyz, sid, gantry_deg=0.0, iso=(0.0, 0.0, 0.0), module_first=M_PHANTOM):
     a = np.float32(gantry_deg * 3.1415926 / 180.0)     # CUDA_EGS.cu:424
-    return Field(int(change_xyz), int(module_first), float(sid), float(np.cos(a, dtype=np.float32)),
-                 float(np.sin(a, dtype=np.float32)), *map(float, iso))
+    # cos and sin of the binary32 angle in binary64, rounded once (the C
+    # library's, as the C++ run driver evaluates them)
+    return Field(int(change_xyz), int(module_first), float(sid), float(np.float32(math.cos(float(a)))),
+                 float(np.float32(math.sin(float(a)))), *map(float, iso))
 
 
 class Workload:

@@ -153,6 +153,60 @@ int gegs_head_file_write(const char *path, const gegs_head *head);
 int gegs_head_file_read(const char *path, gegs_head *head);
 void gegs_head_free(gegs_head *head);
 
+/* ---------------------------------------------------------------- head inputs of a run
+ * The BEAMnrc component blocks of an .egsinp file, the .mlcinfo field list,
+ * the Varian leaf files and the output-factor table.  Numbers are read as
+ * decimal -> binary64 -> binary32; the geometry is built in the reference's
+ * binary32 operation order.  A file without the component asked for returns
+ * GEGS_NOT_FOUND (the reference then keeps that module's defaults); errors
+ * give a status code and gegs_input_last_error(). */
+#define GEGS_NOT_FOUND 1
+
+const char *gegs_input_last_error(void);
+
+/* JAWS component SECJAWS (read_egsinp_secjaws, init.cu:612-689): RMAX, the Y
+ * and X pairs' z ranges; medium = the jaw medium's name. */
+int gegs_read_secjaws(const char *egsinp, gegs_jaws *jaws, char medium[GEGS_NAME_LEN]);
+
+/* DYNVMLC component (read_egsinp, init.cu:345-611): the three leaf types, the
+ * leaf order, the default openings and the rounded leaf ends, built into the
+ * focused cross-sections of gegs_mlc; medium = the leaf medium. */
+int gegs_read_dynvmlc(const char *egsinp, gegs_mlc *mlc, char medium[GEGS_NAME_LEN]);
+
+/* PYRAMIDS components BLOCK (10 values: zmin, zmax, XFP, XBP, XFN, XBN, YFP,
+ * YBP, YFN, YBN; init.cu:793-905) and WEDGE (RMAX and 1-4 layers of zmin,
+ * zmax, XFP, XBP, XFN, XBN; init.cu:692-791), and the gegs_block_wedge built
+ * from them (block or layers may be absent: null / n_layers 0). */
+int gegs_read_block(const char *egsinp, float coord[10], char medium[GEGS_NAME_LEN]);
+int gegs_read_wedge(const char *egsinp, float *rmax, float layers[24], uint32_t *n_layers,
+                    char medium[GEGS_NAME_LEN]);
+int gegs_build_block_wedge(const float *block_coord, float wedge_rmax, const float *layers, uint32_t n_layers,
+                           gegs_block_wedge *out);
+
+/* One field of a .mlcinfo list (read_mlcinp, init.cu:1377-1502): gantry
+ * angle (degrees), jaw opening x1 x2 y1 y2 (cm at the isocentre), monitor
+ * units and the leaf file as written. */
+typedef struct gegs_mlc_field {
+    double angle, x1, x2, y1, y2, mu;
+    char leaf_file[512];
+} gegs_mlc_field;
+
+/* Fields of the list (*n_fields = the count it declares; at most max_fields
+ * are written). */
+int gegs_read_mlcinfo(const char *path, gegs_mlc_field *fields, uint32_t max_fields, uint32_t *n_fields);
+/* A Varian leaf file: 14 header lines, then 120 positions read after the
+ * first 10 characters of their lines (bank A leaves 1-60, then bank B). */
+int gegs_read_leaf_file(const char *path, float positions[120]);
+
+/* Output-factor table (init.cu:1389-1412): a header line, the entry count,
+ * then "field size  output factor" per line (*n = the count; at most max
+ * entries written).  gegs_field_output_factor interpolates it at the
+ * equivalent square sqrt((x2 - x1)(y2 - y1)) of a jaw opening, in binary32
+ * (init.cu:1430-1455); GEGS_NOT_FOUND outside the table. */
+int gegs_read_output_factors(const char *path, float *sizes, float *factors, uint32_t max, uint32_t *n);
+int gegs_field_output_factor(double x1, double x2, double y1, double y2, const float *sizes, const float *factors,
+                             uint32_t n, float *output_factor);
+
 #ifdef __cplusplus
 }
 #endif

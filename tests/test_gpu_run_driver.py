@@ -82,3 +82,23 @@ def test_cpp_driver_matches_the_python_driver(tmp_path):
     assert doses["batched"].max() > 0
     assert np.allclose(doses["batched"], res.dose, rtol=3e-7, atol=0)
     assert np.allclose(doses["plain"], res.dose, rtol=3e-7, atol=0)
+
+
+def test_cpp_driver_reads_the_run_input(tmp_path):
+    """gegs_run -i <file.ini> --media <tables>: the C++ driver reads the run
+    input, head components, field list, leaf files and output factors itself
+    and gives the Python driver's dose file (to the FP64 atomics' order)."""
+    import subprocess
+    ini, _ = make_dataset(str(tmp_path))
+    rf = str(tmp_path / "tables.gegsrun")
+    grun.prepare(ini, rf, base_dir=str(tmp_path), seed=31, n_batches=10, echo=False)   # writes tables.gegsmed
+    print("C++ run driver: -i run input", flush=True)
+    r = subprocess.run([RUNNER, "-i", ini, "--media", str(tmp_path / "tables.gegsmed"), "--base-dir", str(tmp_path),
+                        "--seed", "31"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    out = os.path.join(str(tmp_path), "out", "Syn_DoseResults_032x032x024.bin")
+    cpp = np.fromfile(out, dtype=np.float32)
+    print("python run driver: batched", flush=True)
+    res = grun.run(ini, base_dir=str(tmp_path), method="batched", seed=31, echo=False)
+    assert cpp.max() > 0
+    assert np.allclose(cpp, res.dose, rtol=3e-7, atol=0)

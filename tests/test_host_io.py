@@ -10,6 +10,7 @@ refuses where the reference's read returns -1; the batches of a field equal
 run.field_schedule (quirk Q16 and its fix); the dose file equals
 postproc.voxel_dose.  Host only: nothing here touches a GPU."""
 import ctypes
+import math
 import os
 import subprocess
 import sys
@@ -319,8 +320,8 @@ def test_prepare_writes_what_the_python_driver_loads(tmp_path):
     assert kv["module first"] == str(head.M_SECJAWY) and kv["host fixes"] == "0"
     assert [int(f[0]) for f in fields] == [f[5] for f in FIELDS]
     # the field geometry's binary32 cos and sin, written to round-trip exactly
-    assert F32(fields[1][2]) == np.cos(F32(90.0 * 3.1415926 / 180.0), dtype=F32)
-    assert F32(fields[1][3]) == np.sin(F32(90.0 * 3.1415926 / 180.0), dtype=F32)
+    a = float(F32(90.0 * 3.1415926 / 180.0))
+    assert F32(fields[1][2]) == F32(math.cos(a)) and F32(fields[1][3]) == F32(math.sin(a))
     c, names = hostio.read_media_file(kv["media file"])
     tables = hatch.MediaTables(names, mlc_medium="W700ICRU")
     assert bytes(c) == bytes(tables.c)
@@ -341,3 +342,153 @@ def test_runner_refuses_a_bad_run_file(tmp_path):
     p.write_text("media file = x.gegsmed\n")
     r = subprocess.run([RUNNER, str(p)], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0 and "required" in r.stderr
+
+
+# ------------------------------------------------------------------ head inputs
+def test_secjaws_reader_matches_python(tmp_path):
+    p = str(tmp_path / "j.egsinp")
+    head.write_egsinp_secjaws(p, head.build_jaws(**head.CLINAC_LIKE_JAWS), "W700ICRU")
+    py_j, py_med = head.read_egsinp_secjaws(p)
+    cc_j, cc_med = hostio.read_secjaws(p)
+    assert bytes(cc_j) == bytes(py_j) and cc_med == py_med == "W700ICRU"
+    q = str(tmp_path / "none.egsinp")
+    open(q, "w").write("no components here\n")
+    assert hostio.read_secjaws(q) is None and head.read_egsinp_secjaws(q) is None
+
+
+def test_dynvmlc_reader_and_builder_match_python(tmp_path):
+    """The whole gegs_mlc (cross-sections, boundaries, leaf ends, default
+    openings) built by the C++ reader equals head.build_mlc's, byte for byte,
+    for the synthetic MLC and for one with tongue-and-groove widths, a shifted
+    start and several opening groups."""
+    import dataclasses
+    p = str(tmp_path / "m.egsinp")
+    d = head.millennium_like()
+    head.write_dynvmlc(p, d)
+    m_py = head.build_mlc(head.read_dynvmlc(p))
+    m_cc, med = hostio.read_dynvmlc(p)
+    assert bytes(m_cc) == bytes(m_py) and med == d.medium
+    lf = head.Leaf(0.474, 0.03, 0.02, 0.05, 0.1, 0.07, 49.1, 50.0, 48.3, 54.0, 51.0, 52.0, 0.0, 50.5, 52.5)
+    d2 = dataclasses.replace(d, full=lf, x_start=-14.1, rad_end=8.5, zthick=6.7,
+                             openings=[(-1.5, 2.0, 20), (0.5, 0.5, 30), (-3.25, 4.0, 10)])
+    p2 = str(tmp_path / "m2.egsinp")
+    head.write_dynvmlc(p2, d2)
+    assert bytes(hostio.read_dynvmlc(p2)[0]) == bytes(head.build_mlc(head.read_dynvmlc(p2)))
+    q = str(tmp_path / "none.egsinp")
+    open(q, "w").write("nothing\n")
+    assert hostio.read_dynvmlc(q) is None
+
+
+def test_pyramids_readers_and_builder_match_python(tmp_path):
+    p = str(tmp_path / "bw.egsinp")
+    head.write_egsinp_pyramids(p, block=workloads.BLOCK_PYRAMIDS, wedge=workloads.WEDGE_PYRAMIDS)
+    b_py, bmed_py = head.read_egsinp_block(p)
+    r_py, l_py, wmed_py = head.read_egsinp_wedge(p)
+    b_cc, bmed_cc = hostio.read_block(p)
+    r_cc, l_cc, wmed_cc = hostio.read_wedge(p)
+    assert np.array_equal(b_cc, np.asarray(b_py, dtype=F32)) and bmed_cc == bmed_py
+    assert F32(r_cc) == F32(r_py) and np.array_equal(l_cc, l_py) and wmed_cc == wmed_py
+    for block, wedge in ((b_py, (r_py, l_py)), (b_py, None), (None, (r_py, l_py))):
+        want = head.build_block_wedge(block, wedge)
+        got = hostio.build_block_wedge(block, wedge)
+        assert bytes(got) == bytes(want)
+    # every slanted-side case of the wedge tracer
+    for layers in ([[0, 1, 2, 2, -3, -1]], [[0, 1, 2, 2, -1, -3], [1, 2, 2, 2, -1.5, -3]],
+                   [[0, 1, 1, 3, -2, -2]], [[0, 1, 3, 1, -2, -2], [1, 2, 3, 0.5, -2, -2]]):
+        w = (7.5, np.asarray(layers, dtype=F32))
+        assert bytes(hostio.build_block_wedge(None, w)) == bytes(head.build_block_wedge(None, w))
+
+
+def test_mlcinfo_leaf_files_and_output_factors_match_python(tmp_path):
+    sys.path.insert(0, os.path.dirname(__file__))
+    from rundata import make_dataset
+    ini, data = make_dataset(str(tmp_path))
+    mi = os.path.join(data, "HEAD", "plan.mlcinfo")
+    py = head.read_mlcinfo(mi)
+    cc = hostio.read_mlcinfo(mi)
+    assert len(cc) == len(py) == 2
+    for a, b in zip(cc, py):
+        assert (a["angle"], a["x1"], a["x2"], a["y1"], a["y2"], a["mu"]) == (b.angle, b.x1, b.x2, b.y1, b.y2, b.mu)
+        assert a["leaf_file"] == b.leaf_file
+        lp = os.path.join(data, b.leaf_file.replace("\\", "/"))
+        assert np.array_equal(hostio.read_leaf_file(lp), head.read_leaf_file(lp))
+    of = os.path.join(data, "output_factor.txt")
+    s_py, f_py = postproc.read_output_factor_table(of)
+    s_cc, f_cc = hostio.read_output_factors(of)
+    assert np.array_equal(s_cc, s_py) and np.array_equal(f_cc, f_py)
+    for x1, x2, y1, y2 in ((-4, 4, -4, 4), (-3, 5, -4, 3), (-1.1, 0.7, -2.3, 5.9), (-0.5, 0.5, -0.5, 0.5),
+                           (-12, 12, -12, 12)):
+        want = postproc.field_output_factor(x1, x2, y1, y2, s_py, f_py)
+        got = hostio.field_output_factor(x1, x2, y1, y2, s_cc, f_cc)
+        assert (got is None and want is None) or got == want, (x1, x2, y1, y2)
+
+
+def _run_file_dict(path):
+    kv, fields = {}, []
+    for line in open(path):
+        line = line.strip()
+        if not line or line.startswith("#"):
+            continue
+        k, v = (s.strip() for s in line.split("=", 1))
+        if k == "field":
+            fields.append(v)
+        else:
+            kv[k] = v
+    return kv, fields
+
+
+@pytest.mark.skipif(not os.path.exists(RUNNER), reason="gegs_run not built")
+@pytest.mark.parametrize("variant", ["plain", "fixes", "block_wedge"])
+def test_cpp_run_input_reader_matches_prepare(tmp_path, variant):
+    """gegs_run -i <file.ini> --dry-run reads the run input, the .egsinp
+    components, the field list, the leaf files and the output factors in
+    C++ and writes the run file and head file that run.prepare writes from
+    the package's Python readers: every key and every byte of the head
+    geometry equal."""
+    sys.path.insert(0, os.path.dirname(__file__))
+    from rundata import make_dataset
+    ini, data = make_dataset(str(tmp_path))
+    fixes = ()
+    if variant == "fixes":
+        fixes = ("Q1", "Q7", "Q15", "Q16")
+    if variant == "block_wedge":
+        head.write_egsinp_pyramids(os.path.join(data, "bw.egsinp"), block=workloads.BLOCK_PYRAMIDS,
+                                   wedge=workloads.WEDGE_PYRAMIDS)
+        text = open(ini).read() + "[Block]\nmodule Block = On\nblock file = \"bw.egsinp\"\n" \
+                                  "[Wedge]\nmodule Wedge = On\nwedge file = \"bw.egsinp\"\n"
+        open(ini, "w").write(text)
+    rf_py = str(tmp_path / "py.gegsrun")
+    grun.prepare(ini, rf_py, base_dir=str(tmp_path), seed=9, n_batches=4, fixes=fixes, echo=False)
+    rf_cc = str(tmp_path / "cc.gegsrun")
+    cmd = [RUNNER, "-i", ini, "--media", str(tmp_path / "py.gegsmed"), "--base-dir", str(tmp_path), "--seed", "9",
+           "--batches", "4", "--dry-run", rf_cc]
+    if fixes:
+        cmd += ["--fix", ",".join(fixes)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    kv_py, f_py = _run_file_dict(rf_py)
+    kv_cc, f_cc = _run_file_dict(rf_cc)
+    h_py, h_cc = kv_py.pop("head file"), kv_cc.pop("head file")
+    assert kv_cc == kv_py
+    assert f_cc == f_py
+    assert open(h_cc, "rb").read() == open(h_py, "rb").read()
+    if variant == "block_wedge":
+        assert hostio.read_head_file(h_cc)["block_wedge"] is not None
+
+
+@pytest.mark.skipif(not os.path.exists(RUNNER), reason="gegs_run not built")
+def test_cpp_run_input_errors_keep_the_reference_codes(tmp_path):
+    sys.path.insert(0, os.path.dirname(__file__))
+    from rundata import make_dataset
+    ini, data = make_dataset(str(tmp_path))
+    text = open(ini).read()
+    cases = [(text.replace("histories = ", "histories = x"), "ERROR (2014)"),
+             (text.replace("[Phase space]\nphsp file number = 2\n", "[Phase space]\n"), "ERROR (5013)"),
+             (text + "[CTcreate]\nuse ctcreate = Yes\n", "--prepare"),
+             (text.replace("plan.mlcinfo", "nothere.mlcinfo"), "MLC info  file reading error")]
+    for k, (t, want) in enumerate(cases):
+        p = str(tmp_path / ("bad%d.ini" % k))
+        open(p, "w").write(t)
+        r = subprocess.run([RUNNER, "-i", p, "--media", "m.gegsmed", "--base-dir", str(tmp_path), "--dry-run",
+                            str(tmp_path / "x.gegsrun")], capture_output=True, text=True, timeout=60)
+        assert r.returncode != 0 and want in r.stderr, (k, r.stderr)
