@@ -66,6 +66,13 @@ struct RunFile {
     float phsp_z = 0, sid = 100, iso[3] = {0, 0, 0}, dose_unit = 1.6e-8f;
     uint64_t histories_per_mu = 0, seed = 1;
     std::vector<FieldSpec> fields;
+    // outputs of write_dose besides the dose file
+    bool interp = false;             // interpolated dose: start (with the half voxel), sizes, points per axis
+    float interp_start[3] = {0, 0, 0}, interp_sizes[3] = {0, 0, 0};
+    uint32_t interp_dimen[3] = {0, 0, 0};
+    bool absdose = false;            // absolute dose at a point, calibrated or not, against a measured dose
+    float abs_pos[3] = {0, 0, 0}, abs_measured = 0;
+    int abs_calibrated = 0;
 };
 
 // logstr (output.c:41-49): every line to stdout and the log file; lines
@@ -164,6 +171,18 @@ RunFile read_run_file(const char *path) {
         } else if (k == "dose unit") r.dose_unit = strtof(c, nullptr);
         else if (k == "histories per mu") r.histories_per_mu = strtoull(c, nullptr, 10);
         else if (k == "seed") r.seed = strtoull(c, nullptr, 10);
+        else if (k == "interpolation") {
+            float *a = r.interp_start, *b = r.interp_sizes;
+            uint32_t *d = r.interp_dimen;
+            if (sscanf(c, "%f %f %f %f %f %f %u %u %u", &a[0], &a[1], &a[2], &b[0], &b[1], &b[2], &d[0], &d[1], &d[2]) != 9)
+                die(2001, "%s:%d: interpolation = <start x y z> <sizes x y z> <points x y z>", path, n);
+            r.interp = true;
+        } else if (k == "absolute dose") {
+            if (sscanf(c, "%f %f %f %d %f", &r.abs_pos[0], &r.abs_pos[1], &r.abs_pos[2], &r.abs_calibrated,
+                       &r.abs_measured) != 5)
+                die(2001, "%s:%d: absolute dose = <x y z> <calibrated 0|1> <measured>", path, n);
+            r.absdose = true;
+        }
         else if (k == "field") {
             // MU, gantry angle (deg), cos and sin of the angle as the run input's
             // field geometry has them, output factor or "-"
@@ -379,7 +398,8 @@ uint32_t parse_fixes(const std::string &list, uint32_t &host) {
         else if (f == "q4") host |= GEGS_HOST_FIX_Q4;
         else if (f == "q15") host |= GEGS_HOST_FIX_Q15;
         else if (f == "q16") host |= GEGS_HOST_FIX_Q16;
-        else die(2001, "unknown fix \"%s\" (known here: Q1, Q4, Q7, Q8, Q15, Q16)", f.c_str());
+        else if (f == "q9") host |= GEGS_HOST_FIX_Q9;
+        else die(2001, "unknown fix \"%s\" (known here: Q1, Q4, Q7, Q8, Q9, Q15, Q16)", f.c_str());
     }
     return dev;
 }
@@ -462,9 +482,33 @@ RunFile from_ini(const char *path, const std::string &media, const std::string &
     const bool of_on = ini.get("Output factor", "use output factor") == "Yes";
     std::string of_file;
     if (of_on) of_file = ini.req("Output factor", "output factor file", 2012);
-    if (ini.get("Interpolation", "do interpolate") == "Yes" || ini.get("Dose output", "absolute dose") == "Yes" ||
-        ini.get("Dose output", "relative dose") == "Yes")
-        log("NOTE: interpolated, absolute and relative dose outputs are made from the dose file by gegs_amd.postproc");
+    if (ini.get("Interpolation", "do interpolate") == "Yes") {
+        // CUDA_EGS.cu:1545-1560: the starting point moves by half a voxel
+        const auto st = need_numbers(ini, ini.req("Interpolation", "starting point", 2011), 3, false, 2011,
+                                     "starting point of interpolation");
+        const auto dm = need_numbers(ini, ini.req("Interpolation", "xyz dimensions", 2011), 3, true, 2011,
+                                     "xyz dimensions of interpolation");
+        const auto sz = need_numbers(ini, ini.req("Interpolation", "3d voxel sizes", 2011), 3, false, 2011,
+                                     "3d voxel sizes of interpolation");
+        for (int k = 0; k < 3; ++k) {
+            r.interp_sizes[k] = (float)sz[k];
+            r.interp_dimen[k] = (uint32_t)dm[k];
+            r.interp_start[k] = (float)st[k] + r.interp_sizes[k] / 2.0f;
+        }
+        r.interp = true;
+    }
+    if (ini.get("Dose output", "absolute dose") == "Yes") {
+        const auto ps = need_numbers(ini, ini.req("Dose output", "dose position", 2011), 3, false, 2011,
+                                     "dose position of absolute dose");
+        const auto ms = need_numbers(ini, ini.req("Dose output", "dose measured", 2011), 1, false, 2011,
+                                     "dose measured of absolute dose");
+        for (int k = 0; k < 3; ++k) r.abs_pos[k] = (float)ps[k];
+        r.abs_measured = (float)ms[0];
+        r.abs_calibrated = ini.get("Calibration", "dose calibration") == "Yes" ? 1 : 0;
+        r.absdose = true;
+    }
+    if (ini.get("Dose output", "relative dose") == "Yes")
+        log("NOTE: the relative dose output is made from the dose file by gegs_amd.postproc");
     auto d = [&](const std::string &name) { return join(data_dir, slashes(name)); };
     const std::string out_dir = host_path(base, outdir);
     r.root = abspath(join(out_dir, prefix));
@@ -620,6 +664,13 @@ void write_run_file(const RunFile &r, const char *path, const char *source) {
     fprintf(f, "seed = %llu\n", (unsigned long long)r.seed);
     fprintf(f, "batches = %u\n", r.batches);
     fprintf(f, "method = %s\n", r.method.c_str());
+    if (r.interp)
+        fprintf(f, "interpolation = %s %s %s %s %s %s %u %u %u\n", g(r.interp_start[0]).c_str(), g(r.interp_start[1]).c_str(),
+                g(r.interp_start[2]).c_str(), g(r.interp_sizes[0]).c_str(), g(r.interp_sizes[1]).c_str(),
+                g(r.interp_sizes[2]).c_str(), r.interp_dimen[0], r.interp_dimen[1], r.interp_dimen[2]);
+    if (r.absdose)
+        fprintf(f, "absolute dose = %s %s %s %d %s\n", g(r.abs_pos[0]).c_str(), g(r.abs_pos[1]).c_str(),
+                g(r.abs_pos[2]).c_str(), r.abs_calibrated, g(r.abs_measured).c_str());
     for (const FieldSpec &fs : r.fields)
         fprintf(f, "field = %llu %s %s %s %s\n", (unsigned long long)fs.mu, g(fs.angle).c_str(), g(fs.cos_a).c_str(),
                 g(fs.sin_a).c_str(), fs.has_of ? g(fs.output_factor).c_str() : "-");
@@ -872,10 +923,47 @@ int main(int argc, char **argv) {
             for (size_t v = 0; v < nvox; ++v) edep[v] += d[v];
     }
     char fname[4200];
-    check_host(gegs_write_dose(rf.root.c_str(), ph.nx, ph.ny, ph.nz, edep.data(), mass.data(), rf.dose_unit, nullptr,
-                               fname, sizeof fname),
+    std::vector<float> dose(nvox);
+    check_host(gegs_write_dose(rf.root.c_str(), ph.nx, ph.ny, ph.nz, edep.data(), mass.data(), rf.dose_unit,
+                               dose.data(), fname, sizeof fname),
                "dose file");
     log("%s", fname);
+    // the other outputs of write_dose (output.c:873-918) on the host's voxel coordinates
+    std::vector<float> xc(ph.nx + 1), yc(ph.ny + 1), zc(ph.nz + 1);
+    gegs_host_bounds(ph.xb, ph.nx, xc.data());
+    gegs_host_bounds(ph.yb, ph.ny, yc.data());
+    gegs_host_bounds(ph.zb, ph.nz, zc.data());
+    if (rf.interp) {
+        std::vector<float> grid((size_t)rf.interp_dimen[0] * rf.interp_dimen[1] * rf.interp_dimen[2]);
+        check_host(gegs_interp_dose(dose.data(), ph.nx, ph.ny, ph.nz, xc.data(), yc.data(), zc.data(), rf.interp_start,
+                                    rf.interp_sizes, rf.interp_dimen, grid.data()),
+                   "interpolated dose");
+        char iname[4200];
+        check_host(gegs_write_interp_dose(rf.root.c_str(), grid.data(), rf.interp_dimen, iname, sizeof iname),
+                   "interpolated dose file");
+        log("%s", iname);
+    }
+    if (rf.absdose) {
+        float val, vox;
+        uint32_t ijk[3];
+        check_host(gegs_abs_dose(dose.data(), ph.nx, ph.ny, ph.nz, xc.data(), yc.data(), zc.data(), rf.abs_pos,
+                                 (rf.host_fixes & GEGS_HOST_FIX_Q9) != 0, &val, ijk, &vox),
+                   "absolute dose");
+        const float *p = rf.abs_pos, meas = rf.abs_measured;
+        if (rf.abs_calibrated && meas == 0.0f) {
+            log("Absolute dose at (%.2f, %.2f, %.2f) = %.1f cGy", p[0], p[1], p[2], val);
+            log("   voxel dose of (i=%2u, j=%2u, k=%2u) = %.1f cGy", ijk[0], ijk[1], ijk[2], vox);
+        } else if (rf.abs_calibrated) {
+            const float d0 = std::fabs(val - meas) / meas, d1 = std::fabs(vox - meas) / meas;
+            log("Absolute dose at (%.2f, %.2f, %.2f):", p[0], p[1], p[2]);
+            log("  measured = %.1f cGy, calculated = %.1f cGy, difference = %.1f %%", meas, val, d0 * 100.0f);
+            log("  voxel dose of index (%2u, %2u, %2u) = %.1f cGy, difference = %.1f %%", ijk[0], ijk[1], ijk[2], vox,
+                d1 * 100.0f);
+        } else {
+            log("Absolute dose at (%.2f, %.2f, %.2f) = %.3E cGy", p[0], p[1], p[2], val);
+            log("   voxel dose of (i=%2u, j=%2u, k=%2u) = %.3E cGy", ijk[0], ijk[1], ijk[2], vox);
+        }
+    }
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     log("Total CPU/GPU . . . . . . . . %.2f ms (%.2f min)", secs * 1e3, secs / 60.0);
     printf("%llu histories in %.2f s -> %s\n", (unsigned long long)transported, secs, fname);

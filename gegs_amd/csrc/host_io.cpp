@@ -558,3 +558,133 @@ void gegs_head_free(gegs_head *h) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ dose outputs of write_dose
+namespace {
+
+// one pass of interp_dose (output.c:222-312): values v[.., k] placed at
+// b[k] linearly interpolated onto the coordinates c along the last axis, in
+// binary32; points outside (b[0], b[n - 1]) stay 0
+void axis_interp(const float *v, size_t outer, uint32_t n, const float *b, const float *c, uint32_t m, float *out) {
+    for (size_t o = 0; o < outer; ++o) {
+        const float *vo = v + o * n;
+        float *w = out + o * m;
+        for (uint32_t q = 0; q < m; ++q) {
+            const float t = c[q];
+            w[q] = 0.0f;
+            if (t <= b[0] || t >= b[n - 1]) continue;
+            for (uint32_t k = 0; k + 1 < n; ++k)
+                if (t > b[k] && t <= b[k + 1]) {
+                    float d = (t - b[k]) * (vo[k + 1] - vo[k]);
+                    d = d / (b[k + 1] - b[k]);
+                    w[q] = vo[k] + d;
+                    break;
+                }
+        }
+    }
+}
+
+uint32_t nearest_cell(const float *b, uint32_t n, float p) {
+    uint32_t i = 1;
+    while (i < n - 2) {
+        if (p > b[i] && p <= b[i + 1]) break;
+        ++i;
+    }
+    if (i == n - 2) i = p <= b[1] ? 0 : n - 2;
+    return i;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gegs_host_bounds(const float *b, uint32_t n, float *out) {
+    if (!b || !out || n < 1) return fail(GEGS_ERR_ARG, "gegs_host_bounds: bad argument");
+    for (uint32_t k = 0; k < n; ++k) out[k] = (b[k] + b[k + 1]) / 2.0f;
+    out[n] = b[n];
+    return GEGS_OK;
+}
+
+int gegs_interp_dose(const float *dose, uint32_t nx, uint32_t ny, uint32_t nz, const float *xc, const float *yc,
+                     const float *zc, const float start[3], const float sizes[3], const uint32_t dimen[3], float *out) {
+    if (!dose || !xc || !yc || !zc || !start || !sizes || !dimen || !out)
+        return fail(GEGS_ERR_ARG, "gegs_interp_dose: null argument");
+    if (nx < 2 || ny < 2 || nz < 2) return fail(GEGS_ERR_ARG, "gegs_interp_dose: at least 2 voxels per axis");
+    std::vector<float> pts[3];
+    for (int a = 0; a < 3; ++a) {
+        pts[a].resize(dimen[a]);
+        for (uint32_t i = 0; i < dimen[a]; ++i) pts[a][i] = start[a] + sizes[a] * (float)i;
+    }
+    const uint32_t dx = dimen[0], dy = dimen[1], dz = dimen[2];
+    // z pass over every (y, x) column, then y, then x, as the reference does
+    std::vector<float> col((size_t)ny * nx * nz), vz((size_t)ny * nx * dz);
+    for (uint32_t j = 0; j < ny; ++j)
+        for (uint32_t i = 0; i < nx; ++i)
+            for (uint32_t k = 0; k < nz; ++k) col[((size_t)j * nx + i) * nz + k] = dose[((size_t)k * ny + j) * nx + i];
+    axis_interp(col.data(), (size_t)ny * nx, nz, zc, pts[2].data(), dz, vz.data());      // [ny][nx][dz]
+    std::vector<float> cy((size_t)nx * dz * ny), vy((size_t)nx * dz * dy);
+    for (uint32_t j = 0; j < ny; ++j)
+        for (uint32_t i = 0; i < nx; ++i)
+            for (uint32_t k = 0; k < dz; ++k) cy[((size_t)i * dz + k) * ny + j] = vz[((size_t)j * nx + i) * dz + k];
+    axis_interp(cy.data(), (size_t)nx * dz, ny, yc, pts[1].data(), dy, vy.data());       // [nx][dz][dy]
+    std::vector<float> cx((size_t)dz * dy * nx);
+    for (uint32_t i = 0; i < nx; ++i)
+        for (uint32_t k = 0; k < dz; ++k)
+            for (uint32_t j = 0; j < dy; ++j) cx[((size_t)k * dy + j) * nx + i] = vy[((size_t)i * dz + k) * dy + j];
+    axis_interp(cx.data(), (size_t)dz * dy, nx, xc, pts[0].data(), dx, out);             // [dz][dy][dx]
+    return GEGS_OK;
+}
+
+int gegs_write_interp_dose(const char *root, const float *grid, const uint32_t dimen[3], char *fname, size_t fname_len) {
+    if (!root || !grid || !dimen) return fail(GEGS_ERR_ARG, "gegs_write_interp_dose: null argument");
+    char name[4096];
+    snprintf(name, sizeof name, "%s_Interpolate_%03ux%03ux%03u.bin", root, dimen[0], dimen[1], dimen[2]);
+    File f(name, "wb");
+    const int32_t d[3] = {(int32_t)dimen[0], (int32_t)dimen[1], (int32_t)dimen[2]};
+    const size_t n = (size_t)dimen[0] * dimen[1] * dimen[2];
+    if (!f.f || fwrite(d, sizeof d, 1, f.f) != 1 || fwrite(grid, sizeof(float), n, f.f) != n)
+        return fail(GEGS_ERR_ARG, "Could not write \"%s\"", name);
+    if (fname && fname_len) {
+        strncpy(fname, name, fname_len - 1);
+        fname[fname_len - 1] = 0;
+    }
+    return GEGS_OK;
+}
+
+int gegs_abs_dose(const float *dose, uint32_t nx, uint32_t ny, uint32_t nz, const float *xc, const float *yc,
+                  const float *zc, const float pos[3], int fix_q9, float *value, uint32_t ijk[3], float *voxel_dose) {
+    if (!dose || !xc || !yc || !zc || !pos || !value || !ijk || !voxel_dose)
+        return fail(GEGS_ERR_ARG, "gegs_abs_dose: null argument");
+    if (nx < 3 || ny < 3 || nz < 3) return fail(GEGS_ERR_ARG, "gegs_abs_dose: at least 3 voxels per axis");
+    const float px = pos[0], py = pos[1], pz = pos[2];
+    uint32_t i = nearest_cell(xc, nx, px), j = nearest_cell(yc, ny, py), k = nearest_cell(zc, nz, pz);
+    auto at = [&](uint32_t kk, uint32_t jj, uint32_t ii) { return dose[((size_t)kk * ny + jj) * nx + ii]; };
+    auto lin = [](float a, float b, float t, float t0, float t1) {
+        float d = (t - t0) * (b - a);
+        d = d / (t1 - t0);
+        return a + d;
+    };
+    float a0 = lin(at(k, j, i), at(k + 1, j, i), pz, zc[k], zc[k + 1]);
+    float a1 = lin(at(k, j, i + 1), at(k + 1, j, i + 1), pz, zc[k], zc[k + 1]);
+    float a2 = lin(at(k, j + 1, i), at(k + 1, j + 1, i), pz, zc[k], zc[k + 1]);
+    float a3 = lin(at(k, j + 1, i + 1), at(k + 1, j + 1, i + 1), pz, zc[k], zc[k + 1]);
+    a0 = lin(a0, a2, py, yc[j], yc[j + 1]);
+    a1 = lin(a1, a3, py, yc[j], yc[j + 1]);
+    a0 = lin(a0, a1, px, xc[i], xc[i + 1]);
+    if (px > (xc[i] + xc[i + 1]) / 2.0f) ++i;
+    if (py > (yc[j] + yc[j + 1]) / 2.0f) ++j;
+    // the reference takes the z midpoint from x_bounds[k + 1] (output.c:835, quirk Q9)
+    if (!fix_q9 && k + 1 > nx)
+        return fail(GEGS_ERR_ARG, "abs_dose: z cell %u needs x bound %u of %u (reference quirk Q9 reads x_bounds for "
+                                  "the z midpoint); fix Q9", k, k + 1, nx + 1);
+    const float zhi = fix_q9 ? zc[k + 1] : xc[k + 1];
+    if (pz > (zc[k] + zhi) / 2.0f) ++k;
+    *value = a0;
+    ijk[0] = i;
+    ijk[1] = j;
+    ijk[2] = k;
+    *voxel_dose = at(k, j, i);
+    return GEGS_OK;
+}
+
+}  // extern "C"

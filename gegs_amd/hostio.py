@@ -25,7 +25,7 @@ HOST_SYMBOLS = [
     "gegs_head_file_write", "gegs_head_file_read", "gegs_head_free",
     "gegs_input_last_error", "gegs_read_secjaws", "gegs_read_dynvmlc", "gegs_read_block", "gegs_read_wedge",
     "gegs_build_block_wedge", "gegs_read_mlcinfo", "gegs_read_leaf_file", "gegs_read_output_factors",
-    "gegs_field_output_factor",
+    "gegs_field_output_factor", "gegs_host_bounds", "gegs_interp_dose", "gegs_write_interp_dose", "gegs_abs_dose",
 ]
 NOT_FOUND = 1
 
@@ -385,3 +385,50 @@ def field_output_factor(x1, x2, y1, y2, sizes, factors):
                                                 factors.ctypes.data, len(sizes), ctypes.byref(of))) is None:
         return None
     return np.float32(of.value)
+
+
+# ------------------------------------------------------------------ dose outputs
+def _dose_outputs_lib():
+    L = lib()
+    vp, u32 = ctypes.c_void_p, ctypes.c_uint32
+    L.gegs_host_bounds.argtypes = [vp, u32, vp]
+    L.gegs_interp_dose.argtypes = [vp, u32, u32, u32, vp, vp, vp, vp, vp, vp, vp]
+    L.gegs_abs_dose.argtypes = [vp, u32, u32, u32, vp, vp, vp, vp, ctypes.c_int, vp, vp, vp]
+    return L
+
+
+def host_bounds(b):
+    b = np.ascontiguousarray(b, dtype=np.float32)
+    out = np.zeros_like(b)
+    _check(_dose_outputs_lib().gegs_host_bounds(b.ctypes.data, len(b) - 1, out.ctypes.data))
+    return out
+
+
+def interp_dose(dose, xc, yc, zc, start, sizes, dimen):
+    """The interpolated dose [dimen_z, dimen_y, dimen_x] of dose [nz, ny, nx]
+    on the host coordinates xc, yc, zc (host_bounds)."""
+    d = np.ascontiguousarray(dose, dtype=np.float32)
+    nz, ny, nx = d.shape
+    xc, yc, zc = (np.ascontiguousarray(b, dtype=np.float32) for b in (xc, yc, zc))
+    st = np.ascontiguousarray(start, dtype=np.float32)
+    sz = np.ascontiguousarray(sizes, dtype=np.float32)
+    dm = np.ascontiguousarray(dimen, dtype=np.uint32)
+    out = np.zeros((dm[2], dm[1], dm[0]), dtype=np.float32)
+    _check(_dose_outputs_lib().gegs_interp_dose(d.ctypes.data, nx, ny, nz, xc.ctypes.data, yc.ctypes.data,
+                                                zc.ctypes.data, st.ctypes.data, sz.ctypes.data, dm.ctypes.data,
+                                                out.ctypes.data))
+    return out
+
+
+def abs_dose(dose, xc, yc, zc, pos, fix_q9=False):
+    """(value, (i, j, k), voxel dose) as postproc.abs_dose."""
+    d = np.ascontiguousarray(dose, dtype=np.float32)
+    nz, ny, nx = d.shape
+    xc, yc, zc = (np.ascontiguousarray(b, dtype=np.float32) for b in (xc, yc, zc))
+    p = np.ascontiguousarray(pos, dtype=np.float32)
+    val, vox = ctypes.c_float(), ctypes.c_float()
+    ijk = np.zeros(3, dtype=np.uint32)
+    _check(_dose_outputs_lib().gegs_abs_dose(d.ctypes.data, nx, ny, nz, xc.ctypes.data, yc.ctypes.data,
+                                             zc.ctypes.data, p.ctypes.data, 1 if fix_q9 else 0, ctypes.byref(val),
+                                             ijk.ctypes.data, ctypes.byref(vox)))
+    return val.value, tuple(int(v) for v in ijk), vox.value

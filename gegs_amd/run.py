@@ -484,7 +484,8 @@ def prepare(path, run_file, base_dir=None, method="batched", seed=1, fixes=(), n
             jaw_positions=np.array([[f.x1, f.x2, f.y1, f.y2] for f in fields], dtype=np.float32)
             if S["jaws"] is not None else None)
     g = lambda v: "%.9g" % float(np.float32(v))
-    host_fixes = ((1 if "Q4" in fixes else 0) | (2 if "Q15" in fixes else 0) | (4 if "Q16" in fixes else 0))
+    host_fixes = ((1 if "Q4" in fixes else 0) | (2 if "Q15" in fixes else 0) | (4 if "Q16" in fixes else 0)
+                  | (8 if "Q9" in fixes else 0))
     lines = ["# gegs run file: %s prepared for gegs_amd/gegs_run" % os.path.abspath(path),
              "media file = %s" % media_file]
     if head_file:
@@ -507,6 +508,12 @@ def prepare(path, run_file, base_dir=None, method="batched", seed=1, fixes=(), n
               "seed = %d" % seed,
               "batches = %d" % n_batches,
               "method = %s" % method]
+    if ri.interpolate:
+        lines.append("interpolation = %s %s" % (" ".join(g(v) for v in list(ri.interp_start) + list(ri.interp_sizes)),
+                                                " ".join("%d" % v for v in ri.interp_dimen)))
+    if ri.absolute_dose:
+        lines.append("absolute dose = %s %d %s" % (" ".join(g(v) for v in ri.dose_position),
+                                                    1 if ri.dose_calibration else 0, g(ri.dose_measured)))
     for i, f in enumerate(fields):
         fl = make_field(ri.change_coordinate, ri.source_to_isocenter, float(np.float32(f.angle)),
                         tuple(ri.isocenter), S["module_first"])

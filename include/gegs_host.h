@@ -46,6 +46,9 @@ extern "C" {
 #define GEGS_HOST_FIX_Q4 1u
 #define GEGS_HOST_FIX_Q15 2u
 #define GEGS_HOST_FIX_Q16 4u
+/* Q9: the absolute-dose voxel's z midpoint is taken with x_bounds[k + 1]
+ * (output.c:835); the fix takes z_bounds[k + 1]. */
+#define GEGS_HOST_FIX_Q9 8u
 
 const char *gegs_host_last_error(void);
 
@@ -134,6 +137,26 @@ int gegs_field_schedule(uint64_t histories, uint64_t total_records, uint32_t hos
  * the file name is returned in fname (at least 32 + strlen(root) bytes). */
 int gegs_write_dose(const char *root, uint32_t nx, uint32_t ny, uint32_t nz, const double *edep,
                     const float *mass, float dose_unit, float *dose_out, char *fname, size_t fname_len);
+
+/* ---------------------------------------------------------------- dose outputs of write_dose
+ * The coordinates these outputs use are the host's bounds after the upload
+ * (init.cu:1289-1294): entry k < n is the centre of voxel k, entry n the last
+ * boundary (gegs_host_bounds, n + 1 entries in and out).
+ * gegs_interp_dose (interp_dose, output.c:222-312): the dose (binary32,
+ * x fastest) linearly interpolated in z, then y, then x onto the points
+ * start + sizes * index (dimen per axis); points outside the centres get 0.
+ * gegs_write_interp_dose writes <root>_Interpolate_XXXxYYYxZZZ.bin: int32
+ * dimensions x, y, z, then the values x fastest.
+ * gegs_abs_dose (output_AbsDos, output.c:780-872): the dose at pos
+ * (trilinear between voxel centres) and the voxel taken as nearest with its
+ * dose; quirk Q9 kept unless fix_q9, and refused where it would read past
+ * x_bounds. */
+int gegs_host_bounds(const float *bounds, uint32_t n, float *out);
+int gegs_interp_dose(const float *dose, uint32_t nx, uint32_t ny, uint32_t nz, const float *xc, const float *yc,
+                     const float *zc, const float start[3], const float sizes[3], const uint32_t dimen[3], float *out);
+int gegs_write_interp_dose(const char *root, const float *grid, const uint32_t dimen[3], char *fname, size_t fname_len);
+int gegs_abs_dose(const float *dose, uint32_t nx, uint32_t ny, uint32_t nz, const float *xc, const float *yc,
+                  const float *zc, const float pos[3], int fix_q9, float *value, uint32_t ijk[3], float *voxel_dose);
 
 /* ---------------------------------------------------------------- head file
  * "GEGSHD01": which of MLC, jaws and block/wedge are present, their

@@ -90,15 +90,24 @@ def test_cpp_driver_reads_the_run_input(tmp_path):
     and gives the Python driver's dose file (to the FP64 atomics' order)."""
     import subprocess
     ini, _ = make_dataset(str(tmp_path))
+    open(ini, "a").write("[Interpolation]\ndo interpolate = Yes\nstarting point = -2.5 -1.0 -3.0\n"
+                         "xyz dimensions = 21 11 9\n3d voxel sizes = 0.25 0.2 0.5\n")
     rf = str(tmp_path / "tables.gegsrun")
     grun.prepare(ini, rf, base_dir=str(tmp_path), seed=31, n_batches=10, echo=False)   # writes tables.gegsmed
     print("C++ run driver: -i run input", flush=True)
     r = subprocess.run([RUNNER, "-i", ini, "--media", str(tmp_path / "tables.gegsmed"), "--base-dir", str(tmp_path),
                         "--seed", "31"], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
-    out = os.path.join(str(tmp_path), "out", "Syn_DoseResults_032x032x024.bin")
-    cpp = np.fromfile(out, dtype=np.float32)
+    outdir = os.path.join(str(tmp_path), "out")
+    cpp = np.fromfile(os.path.join(outdir, "Syn_DoseResults_032x032x024.bin"), dtype=np.float32)
+    cpp_interp = open(os.path.join(outdir, "Syn_Interpolate_021x011x009.bin"), "rb").read()
+    cpp_log = open(os.path.join(outdir, "Syn_log.txt")).read()
     print("python run driver: batched", flush=True)
     res = grun.run(ini, base_dir=str(tmp_path), method="batched", seed=31, echo=False)
     assert cpp.max() > 0
     assert np.allclose(cpp, res.dose, rtol=3e-7, atol=0)
+    py_interp = open(res.files["interpolate"], "rb").read()
+    assert cpp_interp[:12] == py_interp[:12]                                # int32 dimensions x, y, z
+    a, b = np.frombuffer(cpp_interp[12:], dtype=np.float32), np.frombuffer(py_interp[12:], dtype=np.float32)
+    assert a.any() and np.allclose(a, b, rtol=1e-6, atol=1e-6 * b.max())
+    assert "Absolute dose at (0.30, 1.20, -0.40):" in cpp_log and "voxel dose of index" in cpp_log

@@ -438,7 +438,7 @@ def _run_file_dict(path):
 
 
 @pytest.mark.skipif(not os.path.exists(RUNNER), reason="gegs_run not built")
-@pytest.mark.parametrize("variant", ["plain", "fixes", "block_wedge"])
+@pytest.mark.parametrize("variant", ["plain", "fixes", "block_wedge", "interpolation"])
 def test_cpp_run_input_reader_matches_prepare(tmp_path, variant):
     """gegs_run -i <file.ini> --dry-run reads the run input, the .egsinp
     components, the field list, the leaf files and the output factors in
@@ -450,7 +450,10 @@ def test_cpp_run_input_reader_matches_prepare(tmp_path, variant):
     ini, data = make_dataset(str(tmp_path))
     fixes = ()
     if variant == "fixes":
-        fixes = ("Q1", "Q7", "Q15", "Q16")
+        fixes = ("Q1", "Q7", "Q9", "Q15", "Q16")
+    if variant == "interpolation":
+        open(ini, "a").write("[Interpolation]\ndo interpolate = Yes\nstarting point = -2.5 -1.0 -3.0\n"
+                             "xyz dimensions = 21 11 9\n3d voxel sizes = 0.25 0.2 0.5\n")
     if variant == "block_wedge":
         head.write_egsinp_pyramids(os.path.join(data, "bw.egsinp"), block=workloads.BLOCK_PYRAMIDS,
                                    wedge=workloads.WEDGE_PYRAMIDS)
@@ -492,3 +495,33 @@ def test_cpp_run_input_errors_keep_the_reference_codes(tmp_path):
         r = subprocess.run([RUNNER, "-i", p, "--media", "m.gegsmed", "--base-dir", str(tmp_path), "--dry-run",
                             str(tmp_path / "x.gegsrun")], capture_output=True, text=True, timeout=60)
         assert r.returncode != 0 and want in r.stderr, (k, r.stderr)
+
+
+def test_dose_outputs_match_postproc():
+    """Interpolated and absolute dose of the C++ host equal postproc's (the
+    reference's binary32 order), on host coordinates from host_bounds, with
+    quirk Q9 and its fix; Q9 past the x bounds is refused by both."""
+    rng = np.random.default_rng(8)
+    nx, ny, nz = 9, 7, 12
+    dose = rng.exponential(2.0, (nz, ny, nx)).astype(F32)
+    xb = np.cumsum(np.r_[-4.0, rng.uniform(0.4, 1.1, nx)]).astype(F32)
+    yb = np.cumsum(np.r_[-3.0, rng.uniform(0.4, 1.1, ny)]).astype(F32)
+    zb = np.cumsum(np.r_[-5.0, rng.uniform(0.4, 1.1, nz)]).astype(F32)
+    xc, yc, zc = (postproc.host_bounds(b) for b in (xb, yb, zb))
+    for b, c in ((xb, xc), (yb, yc), (zb, zc)):
+        assert np.array_equal(hostio.host_bounds(b), c)
+    start = [F32(F32(-4.3) + F32(0.37) / F32(2)), F32(F32(-3.1) + F32(0.5) / F32(2)), F32(F32(-5.2) + F32(0.61) / F32(2))]
+    sizes, dimen = [F32(0.37), F32(0.5), F32(0.61)], [23, 13, 19]
+    want = postproc.interp_dose(dose, xc, yc, zc, start, sizes, dimen)
+    got = hostio.interp_dose(dose, xc, yc, zc, start, sizes, dimen)
+    assert got.tobytes() == want.tobytes() and got.any()
+    for pos in ((0.1, 0.2, -1.0), (-3.3, 2.0, 3.9), (1.5, -1.9, -4.1)):
+        for fix in (False, True):
+            try:
+                w = postproc.abs_dose(dose, xc, yc, zc, pos, fix_q9=fix)
+            except ValueError:
+                with pytest.raises(hostio.HostError, match="Q9"):
+                    hostio.abs_dose(dose, xc, yc, zc, pos, fix_q9=fix)
+                continue
+            g = hostio.abs_dose(dose, xc, yc, zc, pos, fix_q9=fix)
+            assert F32(g[0]) == F32(w[0]) and g[1] == w[1] and F32(g[2]) == F32(w[2]), (pos, fix)
