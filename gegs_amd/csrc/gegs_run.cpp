@@ -73,6 +73,12 @@ struct RunFile {
     bool absdose = false;            // absolute dose at a point, calibrated or not, against a measured dose
     float abs_pos[3] = {0, 0, 0}, abs_measured = 0;
     int abs_calibrated = 0;
+    bool reldose = false;            // relative dose: points, the measured profile and the comparison
+    float rel_start[3] = {0, 0, 0}, rel_sizes[3] = {0, 0, 0}, rel_iso[3] = {0, 0, 0}, rel_glen = 0, rel_gdose = 0;
+    float rel_measured = 0;          // the absolute dose's measured value (a single point compares with it)
+    uint32_t rel_dimen[3] = {0, 0, 0};
+    int rel_norm = 0, rel_array = 0;
+    std::string rel_file;            // measured profile
 };
 
 // logstr (output.c:41-49): every line to stdout and the log file; lines
@@ -182,6 +188,17 @@ RunFile read_run_file(const char *path) {
                        &r.abs_measured) != 5)
                 die(2001, "%s:%d: absolute dose = <x y z> <calibrated 0|1> <measured>", path, n);
             r.absdose = true;
+        } else if (k == "relative dose") {
+            float *a = r.rel_start, *b = r.rel_sizes, *q = r.rel_iso;
+            uint32_t *d = r.rel_dimen;
+            char file[4096] = {0};
+            if (sscanf(c, "%f %f %f %f %f %f %u %u %u %f %f %f %d %d %f %f %d %f %4095s", &a[0], &a[1], &a[2], &b[0], &b[1],
+                       &b[2], &d[0], &d[1], &d[2], &q[0], &q[1], &q[2], &r.rel_norm, &r.rel_array, &r.rel_glen,
+                       &r.rel_gdose, &r.abs_calibrated, &r.rel_measured, file) != 19)
+                die(2001, "%s:%d: relative dose = <start> <sizes> <points> <isocentre> <norm> <array> <gamma cm> "
+                          "<gamma fraction> <calibrated> <measured dose> <measured file>", path, n);
+            r.rel_file = resolve(r.dir, file);
+            r.reldose = true;
         }
         else if (k == "field") {
             // MU, gantry angle (deg), cos and sin of the angle as the run input's
@@ -507,9 +524,35 @@ RunFile from_ini(const char *path, const std::string &media, const std::string &
         r.abs_calibrated = ini.get("Calibration", "dose calibration") == "Yes" ? 1 : 0;
         r.absdose = true;
     }
-    if (ini.get("Dose output", "relative dose") == "Yes")
-        log("NOTE: the relative dose output is made from the dose file by gegs_amd.postproc");
+    if (ini.get("Dose output", "relative dose") == "Yes") {
+        // CUDA_EGS.cu:1600-1680: the starting point is relative to the isocentre
+        const auto st = need_numbers(ini, ini.req("Dose output", "starting point", 2011), 3, false, 2011,
+                                     "starting point of relative dose");
+        const auto dm = need_numbers(ini, ini.req("Dose output", "xyz dimensions", 2011), 3, true, 2011,
+                                     "xyz dimensions of relative dose");
+        const auto sz = need_numbers(ini, ini.req("Dose output", "3d voxel sizes", 2011), 3, false, 2011,
+                                     "3d voxel sizes of relative dose");
+        for (int k = 0; k < 3; ++k) {
+            r.rel_start[k] = (float)st[k] + r.iso[k];
+            r.rel_dimen[k] = (uint32_t)dm[k];
+            r.rel_sizes[k] = (float)sz[k];
+            r.rel_iso[k] = r.iso[k];
+        }
+        const std::string at = ini.get("Dose output", "PTW729/PTW1000");
+        r.rel_array = at == "PTW729" ? 1 : (at == "PTW1000" ? 2 : 0);
+        const std::string nt = ini.get("Dose output", "norm to Max/Center");
+        r.rel_norm = nt == "Max" ? 1 : (nt == "Center" ? 2 : 0);
+        const auto gm = need_numbers(ini, ini.req("Dose output", "gamma index (mm,%)", 2011), 2, false, 2011,
+                                     "gamma index paremeters");
+        r.rel_glen = (float)gm[0] * 0.1f;
+        r.rel_gdose = (float)gm[1] * 0.01f;
+        r.rel_file = ini.req("Dose output", "measured data", 2012);     // resolved in the data directory below
+        r.rel_measured = r.absdose ? r.abs_measured : 0.0f;
+        r.abs_calibrated = ini.get("Calibration", "dose calibration") == "Yes" ? 1 : 0;
+        r.reldose = true;
+    }
     auto d = [&](const std::string &name) { return join(data_dir, slashes(name)); };
+    if (r.reldose) r.rel_file = abspath(d(r.rel_file));
     const std::string out_dir = host_path(base, outdir);
     r.root = abspath(join(out_dir, prefix));
     r.egsphant = abspath(d(egsphant));
@@ -671,6 +714,13 @@ void write_run_file(const RunFile &r, const char *path, const char *source) {
     if (r.absdose)
         fprintf(f, "absolute dose = %s %s %s %d %s\n", g(r.abs_pos[0]).c_str(), g(r.abs_pos[1]).c_str(),
                 g(r.abs_pos[2]).c_str(), r.abs_calibrated, g(r.abs_measured).c_str());
+    if (r.reldose)
+        fprintf(f, "relative dose = %s %s %s %s %s %s %u %u %u %s %s %s %d %d %s %s %d %s %s\n", g(r.rel_start[0]).c_str(),
+                g(r.rel_start[1]).c_str(), g(r.rel_start[2]).c_str(), g(r.rel_sizes[0]).c_str(),
+                g(r.rel_sizes[1]).c_str(), g(r.rel_sizes[2]).c_str(), r.rel_dimen[0], r.rel_dimen[1], r.rel_dimen[2],
+                g(r.rel_iso[0]).c_str(), g(r.rel_iso[1]).c_str(), g(r.rel_iso[2]).c_str(), r.rel_norm, r.rel_array,
+                g(r.rel_glen).c_str(), g(r.rel_gdose).c_str(), r.abs_calibrated, g(r.rel_measured).c_str(),
+                r.rel_file.c_str());
     for (const FieldSpec &fs : r.fields)
         fprintf(f, "field = %llu %s %s %s %s\n", (unsigned long long)fs.mu, g(fs.angle).c_str(), g(fs.cos_a).c_str(),
                 g(fs.sin_a).c_str(), fs.has_of ? g(fs.output_factor).c_str() : "-");
@@ -943,13 +993,12 @@ int main(int argc, char **argv) {
                    "interpolated dose file");
         log("%s", iname);
     }
-    if (rf.absdose) {
+    auto abs_report = [&](const float *p, float meas) {
         float val, vox;
         uint32_t ijk[3];
-        check_host(gegs_abs_dose(dose.data(), ph.nx, ph.ny, ph.nz, xc.data(), yc.data(), zc.data(), rf.abs_pos,
+        check_host(gegs_abs_dose(dose.data(), ph.nx, ph.ny, ph.nz, xc.data(), yc.data(), zc.data(), p,
                                  (rf.host_fixes & GEGS_HOST_FIX_Q9) != 0, &val, ijk, &vox),
                    "absolute dose");
-        const float *p = rf.abs_pos, meas = rf.abs_measured;
         if (rf.abs_calibrated && meas == 0.0f) {
             log("Absolute dose at (%.2f, %.2f, %.2f) = %.1f cGy", p[0], p[1], p[2], val);
             log("   voxel dose of (i=%2u, j=%2u, k=%2u) = %.1f cGy", ijk[0], ijk[1], ijk[2], vox);
@@ -962,6 +1011,24 @@ int main(int argc, char **argv) {
         } else {
             log("Absolute dose at (%.2f, %.2f, %.2f) = %.3E cGy", p[0], p[1], p[2], val);
             log("   voxel dose of (i=%2u, j=%2u, k=%2u) = %.3E cGy", ijk[0], ijk[1], ijk[2], vox);
+        }
+    };
+    if (rf.absdose) abs_report(rf.abs_pos, rf.abs_measured);
+    if (rf.reldose) {
+        if (rf.rel_dimen[0] == 1 && rf.rel_dimen[1] == 1 && rf.rel_dimen[2] == 1) {
+            abs_report(rf.rel_start, rf.rel_measured);
+        } else {
+            std::vector<float> calc((size_t)rf.rel_dimen[0] * rf.rel_dimen[1] * rf.rel_dimen[2]);
+            check_host(gegs_reldose_points(dose.data(), ph.nx, ph.ny, ph.nz, xc.data(), yc.data(), zc.data(),
+                                           rf.rel_start, rf.rel_sizes, rf.rel_dimen, calc.data()),
+                       "relative dose points");
+            char rname[4200], summary[512];
+            check_host(gegs_write_relative_dose(rf.root.c_str(), calc.data(), rf.rel_dimen, rf.rel_file.c_str(),
+                                                rf.rel_start, rf.rel_sizes, rf.rel_iso, rf.rel_norm, rf.rel_array,
+                                                rf.rel_glen, rf.rel_gdose, rname, sizeof rname, summary, sizeof summary),
+                       "relative dose");
+            log("%s", rname);
+            log("%s", summary);
         }
     }
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

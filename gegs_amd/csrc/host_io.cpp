@@ -688,3 +688,171 @@ int gegs_abs_dose(const float *dose, uint32_t nx, uint32_t ny, uint32_t nz, cons
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ relative dose (output.c:317-767)
+extern "C" {
+
+int gegs_reldose_points(const float *dose, uint32_t nx, uint32_t ny, uint32_t nz, const float *xc, const float *yc,
+                        const float *zc, const float start[3], const float sizes[3], const uint32_t dimen[3],
+                        float *out) {
+    if (!dose || !xc || !yc || !zc || !start || !sizes || !dimen || !out)
+        return fail(GEGS_ERR_ARG, "gegs_reldose_points: null argument");
+    const float *b[3] = {xc, yc, zc};
+    const uint32_t n[3] = {nx, ny, nz};
+    for (uint32_t k = 0; k < dimen[2]; ++k)
+        for (uint32_t j = 0; j < dimen[1]; ++j)
+            for (uint32_t i = 0; i < dimen[0]; ++i) {
+                const uint32_t t[3] = {i, j, k};
+                long v[3];
+                for (int a = 0; a < 3; ++a) {
+                    const float p = start[a] + sizes[a] * (float)t[a];
+                    const float q = (p - b[a][0]) / (b[a][1] - b[a][0]);
+                    v[a] = (long)((double)q + 0.1);               // float + double, truncated toward zero
+                    if (v[a] < 0 || v[a] >= (long)n[a])
+                        return fail(GEGS_ERR_ARG, "relative dose point (%u, %u, %u) lies outside the phantom on axis %d",
+                                    i, j, k, a);
+                }
+                out[i + (size_t)dimen[0] * j + (size_t)dimen[0] * dimen[1] * k] =
+                    dose[((size_t)v[2] * ny + v[1]) * nx + v[0]];
+            }
+    return GEGS_OK;
+}
+
+int gegs_reldose_compare(const float *calc_in, const float *meas_in, uint32_t n, const uint32_t dimen[3],
+                         const float start[3], const float sizes[3], const float iso[3], int norm_type, int array_type,
+                         float gamma_len, float gamma_dose, float *meas, float *calc, int32_t *pass, uint32_t *total,
+                         uint32_t *passed) {
+    if (!calc_in || !meas_in || !dimen || !start || !sizes || !iso || !meas || !calc || !pass || !total || !passed)
+        return fail(GEGS_ERR_ARG, "gegs_reldose_compare: null argument");
+    memcpy(meas, meas_in, sizeof(float) * n);
+    memcpy(calc, calc_in, sizeof(float) * n);
+    float mmax = 0.0f;
+    if (norm_type == 1) {                        // each to its maximum
+        float cmax = 0.0f;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (meas[i] > mmax) mmax = meas[i];
+            if (calc[i] > cmax) cmax = calc[i];
+        }
+        for (uint32_t i = 0; i < n; ++i) {
+            meas[i] = meas[i] / mmax;
+            calc[i] = calc[i] / cmax;
+        }
+        mmax = 1.0f;
+    } else if (norm_type == 2) {                 // each to its value at the isocentre's point
+        for (uint32_t i = 0; i < n; ++i)
+            if (meas[i] > mmax) mmax = meas[i];
+        long center = 0;
+        if (sizes[0] != 0.0f) center += (long)((double)((iso[0] - start[0]) / sizes[0]) + 0.1);
+        if (sizes[1] != 0.0f) center += (long)((double)((iso[1] - start[1]) / sizes[1]) + 0.1) * (long)dimen[0];
+        if (sizes[2] != 0.0f)
+            center += (long)((double)((iso[2] - start[2]) / sizes[2]) + 0.1) * (long)dimen[0] * (long)dimen[1];
+        if (center < 0 || center >= (long)n)
+            return fail(GEGS_ERR_ARG, "the isocentre's point %ld lies outside the %u relative-dose points", center, n);
+        const float mc = meas[center], cc = calc[center];
+        for (uint32_t i = 0; i < n; ++i) {
+            meas[i] = meas[i] / mc;
+            calc[i] = calc[i] / cc;
+        }
+        mmax = mmax / mc;
+    } else {                                     // the measured values x 100
+        for (uint32_t i = 0; i < n; ++i) {
+            meas[i] = meas[i] * 100.0f;
+            if (meas[i] > mmax) mmax = meas[i];
+        }
+    }
+    const float thr = mmax * 0.1f;
+    auto near = [&](float c, float m) {
+        float v = std::fabs(c - m) / m;
+        v = v / gamma_dose;
+        v = v * v;
+        return std::sqrt((double)v + 0.25 * 0.25 / (double)gamma_len / (double)gamma_len) <= 1.0;
+    };
+    uint32_t tot = 0, cnt = 0;
+    const uint32_t dx = dimen[0];
+    for (uint32_t i = 0; i < n; ++i) {
+        int p = 0;
+        const float m = meas[i];
+        if (m > thr) {
+            ++tot;
+            const bool close = std::fabs(calc[i] - m) / m <= gamma_dose;
+            if (array_type == 1) {               // PTW729: the dose difference alone
+                if (close) {
+                    p = 1;
+                    ++cnt;
+                } else {
+                    p = -1;
+                }
+            } else {                             // or a gamma-like test against the x and z neighbours
+                if (close) p += 1;
+                const uint32_t iz = i / dx, ix = i - iz * dx;
+                if (ix > 0 && p <= 0 && near(calc[i - 1], m)) p += 1;
+                if (ix + 1 < dx && p <= 0 && near(calc[i + 1], m)) p += 1;
+                if (iz > 0 && near(calc[i - dx], m)) p += 1;                 // no p <= 0 test (output.c:570)
+                if (iz + 1 < dimen[2] && p <= 0 && near(calc[i + dx], m)) p += 1;
+                if (p > 0) ++cnt;
+                else p = -1;
+            }
+        }
+        pass[i] = p;
+    }
+    *total = tot;
+    *passed = cnt;
+    return GEGS_OK;
+}
+
+int gegs_write_relative_dose(const char *root, const float *calc, const uint32_t dimen[3], const char *measured_path,
+                             const float start[3], const float sizes[3], const float iso[3], int norm_type,
+                             int array_type, float gamma_len, float gamma_dose, char *fname, size_t fname_len,
+                             char *summary, size_t summary_len) {
+    if (!root || !calc || !dimen || !start || !sizes || !iso) return fail(GEGS_ERR_ARG, "gegs_write_relative_dose: null argument");
+    char name[4096];
+    snprintf(name, sizeof name, "%s_RelativeDose_%03ux%03ux%03u.txt", root, dimen[0], dimen[1], dimen[2]);
+    if (fname && fname_len) {
+        strncpy(fname, name, fname_len - 1);
+        fname[fname_len - 1] = 0;
+    }
+    const uint32_t n = dimen[0] * dimen[1] * dimen[2];
+    File mf(measured_path ? measured_path : "", "r");
+    File out(name, "w");
+    if (!out.f) return fail(GEGS_ERR_ARG, "Could not write \"%s\"", name);
+    auto say = [&](const char *s) {
+        if (summary && summary_len) {
+            strncpy(summary, s, summary_len - 1);
+            summary[summary_len - 1] = 0;
+        }
+    };
+    if (!measured_path || !mf.f) {               // the calculated values alone (output.c:430-436)
+        for (uint32_t i = 0; i < n; ++i) fprintf(out.f, "%f\n", (double)calc[i]);
+        say("Error: dose data file reading error!");
+        return GEGS_OK;
+    }
+    std::vector<float> meas;
+    std::string line;
+    while (meas.size() < n && read_line(mf.f, line)) {
+        char tok[256];
+        if (sscanf(line.c_str(), "%255s", tok) == 1) {
+            char *end = nullptr;
+            const double v = strtod(tok, &end);
+            if (!end || *end) return fail(GEGS_ERR_ARG, "\"%s\": \"%s\" is not a number", measured_path, tok);
+            meas.push_back((float)v);
+        }
+    }
+    if (meas.size() < n)
+        return fail(GEGS_ERR_ARG, "the measured data file \"%s\" holds %zu of the %u values", measured_path, meas.size(), n);
+    std::vector<float> m2(n), c2(n);
+    std::vector<int32_t> pass(n);
+    uint32_t total = 0, passed = 0;
+    const int rc = gegs_reldose_compare(calc, meas.data(), n, dimen, start, sizes, iso, norm_type, array_type, gamma_len,
+                                        gamma_dose, m2.data(), c2.data(), pass.data(), &total, &passed);
+    if (rc) return rc;
+    for (uint32_t i = 0; i < n; ++i) fprintf(out.f, "%f\t %f\t %d\n", (double)m2[i], (double)c2[i], pass[i]);
+    const float percent = total ? (float)passed / (float)total : std::nanf("");
+    char s[256];
+    snprintf(s, sizeof s, "Totally %u points compared, %u passed, percentage = %.0f %%", total, passed,
+             (double)(percent * 100.0f));
+    fprintf(out.f, "%s\n", s);
+    say(s);
+    return GEGS_OK;
+}
+
+}  // extern "C"

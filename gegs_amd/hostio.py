@@ -26,6 +26,7 @@ HOST_SYMBOLS = [
     "gegs_input_last_error", "gegs_read_secjaws", "gegs_read_dynvmlc", "gegs_read_block", "gegs_read_wedge",
     "gegs_build_block_wedge", "gegs_read_mlcinfo", "gegs_read_leaf_file", "gegs_read_output_factors",
     "gegs_field_output_factor", "gegs_host_bounds", "gegs_interp_dose", "gegs_write_interp_dose", "gegs_abs_dose",
+    "gegs_reldose_points", "gegs_reldose_compare", "gegs_write_relative_dose",
 ]
 NOT_FOUND = 1
 
@@ -432,3 +433,39 @@ def abs_dose(dose, xc, yc, zc, pos, fix_q9=False):
                                              zc.ctypes.data, p.ctypes.data, 1 if fix_q9 else 0, ctypes.byref(val),
                                              ijk.ctypes.data, ctypes.byref(vox)))
     return val.value, tuple(int(v) for v in ijk), vox.value
+
+
+def reldose_points(dose, xc, yc, zc, start, sizes, dimen):
+    L = lib()
+    vp, u32 = ctypes.c_void_p, ctypes.c_uint32
+    L.gegs_reldose_points.argtypes = [vp, u32, u32, u32, vp, vp, vp, vp, vp, vp, vp]
+    d = np.ascontiguousarray(dose, dtype=np.float32)
+    nz, ny, nx = d.shape
+    xc, yc, zc = (np.ascontiguousarray(b, dtype=np.float32) for b in (xc, yc, zc))
+    st, sz = (np.ascontiguousarray(v, dtype=np.float32) for v in (start, sizes))
+    dm = np.ascontiguousarray(dimen, dtype=np.uint32)
+    out = np.zeros(int(dm.prod()), dtype=np.float32)
+    _check(L.gegs_reldose_points(d.ctypes.data, nx, ny, nz, xc.ctypes.data, yc.ctypes.data, zc.ctypes.data,
+                                 st.ctypes.data, sz.ctypes.data, dm.ctypes.data, out.ctypes.data))
+    return out
+
+
+def write_relative_dose(root, calc, dimen, measured_path, start, sizes, isocenter, norm_type, array_type, gamma_len,
+                        gamma_dose):
+    """(file name, summary line) as the C++ writer makes them."""
+    L = lib()
+    vp, u32 = ctypes.c_void_p, ctypes.c_uint32
+    L.gegs_write_relative_dose.argtypes = [ctypes.c_char_p, vp, vp, ctypes.c_char_p, vp, vp, vp, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_char_p,
+                                           ctypes.c_size_t, ctypes.c_char_p, ctypes.c_size_t]
+    c = np.ascontiguousarray(calc, dtype=np.float32)
+    dm = np.ascontiguousarray(dimen, dtype=np.uint32)
+    st, sz, iso = (np.ascontiguousarray(v, dtype=np.float32) for v in (start, sizes, isocenter))
+    fname = ctypes.create_string_buffer(len(root) + 64)
+    summary = ctypes.create_string_buffer(512)
+    _check(L.gegs_write_relative_dose(root.encode(), c.ctypes.data, dm.ctypes.data,
+                                      measured_path.encode() if measured_path else None, st.ctypes.data,
+                                      sz.ctypes.data, iso.ctypes.data, int(norm_type), int(array_type),
+                                      float(gamma_len), float(gamma_dose), fname, len(fname), summary,
+                                      len(summary)))
+    return fname.value.decode(), summary.value.decode()

@@ -514,6 +514,12 @@ def prepare(path, run_file, base_dir=None, method="batched", seed=1, fixes=(), n
     if ri.absolute_dose:
         lines.append("absolute dose = %s %d %s" % (" ".join(g(v) for v in ri.dose_position),
                                                     1 if ri.dose_calibration else 0, g(ri.dose_measured)))
+    if ri.relative_dose:
+        lines.append("relative dose = %s %s %s %d %d %s %s %d %s %s" % (
+            " ".join(g(v) for v in list(ri.reldose_start) + list(ri.reldose_sizes)),
+            " ".join("%d" % v for v in ri.reldose_dimen), " ".join(g(v) for v in ri.isocenter),
+            ri.norm_type, ri.array_type, g(ri.gamma_len), g(ri.gamma_dose), 1 if ri.dose_calibration else 0,
+            g(ri.dose_measured or 0.0), os.path.abspath(os.path.join(data_dir, ri.measured_data.replace("\\", "/")))))
     for i, f in enumerate(fields):
         fl = make_field(ri.change_coordinate, ri.source_to_isocenter, float(np.float32(f.angle)),
                         tuple(ri.isocenter), S["module_first"])

@@ -158,6 +158,31 @@ int gegs_write_interp_dose(const char *root, const float *grid, const uint32_t d
 int gegs_abs_dose(const float *dose, uint32_t nx, uint32_t ny, uint32_t nz, const float *xc, const float *yc,
                   const float *zc, const float pos[3], int fix_q9, float *value, uint32_t ijk[3], float *voxel_dose);
 
+/* Relative dose (output_RelDos_II, output.c:603-767).  gegs_reldose_points:
+ * point (i, j, k) at start + sizes * index takes the dose of voxel
+ * int((p - c[0]) / (c[1] - c[0]) + 0.1) per axis (x fastest; a point outside
+ * the phantom is refused).  gegs_reldose_compare normalises (norm_type 1:
+ * each profile to its maximum, 2: to its value at the isocentre's point,
+ * else the measured values x 100) and marks every point whose measured
+ * value exceeds 10% of the measured maximum: 1 or more passed, -1 failed,
+ * 0 not compared; PTW729 arrays (array_type 1) by the dose difference alone,
+ * others also by a gamma-like test against the x and z neighbours 0.25 cm
+ * away (gamma_len in cm, gamma_dose a fraction).  gegs_write_relative_dose
+ * writes <root>_RelativeDose_XXXxYYYxZZZ.txt ("measured\t calculated\t pass"
+ * rows and the pass-rate line, or the calculated values alone when the
+ * measured file cannot be opened) and returns the line to log in summary. */
+int gegs_reldose_points(const float *dose, uint32_t nx, uint32_t ny, uint32_t nz, const float *xc, const float *yc,
+                        const float *zc, const float start[3], const float sizes[3], const uint32_t dimen[3],
+                        float *out);
+int gegs_reldose_compare(const float *calc, const float *measured, uint32_t n, const uint32_t dimen[3],
+                         const float start[3], const float sizes[3], const float isocenter[3], int norm_type,
+                         int array_type, float gamma_len, float gamma_dose, float *measured_out, float *calc_out,
+                         int32_t *pass, uint32_t *total, uint32_t *passed);
+int gegs_write_relative_dose(const char *root, const float *calc, const uint32_t dimen[3], const char *measured_path,
+                             const float start[3], const float sizes[3], const float isocenter[3], int norm_type,
+                             int array_type, float gamma_len, float gamma_dose, char *fname, size_t fname_len,
+                             char *summary, size_t summary_len);
+
 /* ---------------------------------------------------------------- head file
  * "GEGSHD01": which of MLC, jaws and block/wedge are present, their
  * descriptions and n_sets aperture sets (120 leaf positions and x1 x2 y1 y2

@@ -102,6 +102,7 @@ def test_cpp_driver_reads_the_run_input(tmp_path):
     cpp = np.fromfile(os.path.join(outdir, "Syn_DoseResults_032x032x024.bin"), dtype=np.float32)
     cpp_interp = open(os.path.join(outdir, "Syn_Interpolate_021x011x009.bin"), "rb").read()
     cpp_log = open(os.path.join(outdir, "Syn_log.txt")).read()
+    cpp_rel = open(os.path.join(outdir, "Syn_RelativeDose_005x001x005.txt")).read().splitlines()
     print("python run driver: batched", flush=True)
     res = grun.run(ini, base_dir=str(tmp_path), method="batched", seed=31, echo=False)
     assert cpp.max() > 0
@@ -111,3 +112,6 @@ def test_cpp_driver_reads_the_run_input(tmp_path):
     a, b = np.frombuffer(cpp_interp[12:], dtype=np.float32), np.frombuffer(py_interp[12:], dtype=np.float32)
     assert a.any() and np.allclose(a, b, rtol=1e-6, atol=1e-6 * b.max())
     assert "Absolute dose at (0.30, 1.20, -0.40):" in cpp_log and "voxel dose of index" in cpp_log
+    py_rel = open(res.files["relative"]).read().splitlines()
+    assert len(cpp_rel) == len(py_rel) == 26 and cpp_rel[-1].startswith("Totally 25 points compared")
+    assert cpp_rel[-1] in cpp_log

@@ -525,3 +525,36 @@ def test_dose_outputs_match_postproc():
                 continue
             g = hostio.abs_dose(dose, xc, yc, zc, pos, fix_q9=fix)
             assert F32(g[0]) == F32(w[0]) and g[1] == w[1] and F32(g[2]) == F32(w[2]), (pos, fix)
+
+
+@pytest.mark.parametrize("norm_type,array_type", [(1, 2), (2, 2), (0, 2), (2, 1), (1, 1)])
+def test_relative_dose_matches_postproc(tmp_path, norm_type, array_type):
+    """Points, normalisation, pass marks, the file and its summary line equal
+    postproc's (output_RelDos_II's rules) for every normalisation and both
+    array types; without the measured file both write the calculated values
+    alone."""
+    rng = np.random.default_rng(norm_type * 10 + array_type)
+    nx, ny, nz = 14, 9, 16
+    dose = (rng.exponential(1.0, (nz, ny, nx)) + 2.0).astype(F32)
+    xb, yb, zb = (np.linspace(-a, a, n + 1).astype(F32) for a, n in ((3.5, nx), (2.25, ny), (4.0, nz)))
+    xc, yc, zc = (postproc.host_bounds(b) for b in (xb, yb, zb))
+    iso = [F32(0.1), F32(0.0), F32(-0.2)]
+    start = [F32(F32(-2.0) + iso[0]), F32(F32(0.0) + iso[1]), F32(F32(-2.5) + iso[2])]
+    sizes, dimen = [F32(0.5), F32(0.0), F32(0.5)], [9, 1, 11]
+    want = postproc.reldose_points_nearest(dose, xc, yc, zc, start, sizes, dimen)
+    got = hostio.reldose_points(dose, xc, yc, zc, start, sizes, dimen)
+    assert got.tobytes() == want.tobytes()
+    meas_path = str(tmp_path / "measured.txt")
+    meas = (want * F32(1.0) + rng.normal(0, 0.08, len(want)).astype(F32)).astype(F32)
+    with open(meas_path, "w") as f:
+        f.write("".join("%g\n" % v for v in meas))
+    kw = dict(start=start, sizes=sizes, isocenter=iso, norm_type=norm_type, array_type=array_type,
+              gamma_len=F32(F32(3) * F32(0.1)), gamma_dose=F32(F32(3) * F32(0.01)))
+    fn_py, lines = postproc.write_relative_dose(str(tmp_path / "py"), want, dimen, meas_path, **kw)
+    fn_cc, summary = hostio.write_relative_dose(str(tmp_path / "cc"), got, dimen, meas_path, **kw)
+    assert summary == lines[-1] and "points compared" in summary
+    assert open(fn_cc).read() == open(fn_py).read()
+    fn_py, lines = postproc.write_relative_dose(str(tmp_path / "py2"), want, dimen, str(tmp_path / "none.txt"), **kw)
+    fn_cc, summary = hostio.write_relative_dose(str(tmp_path / "cc2"), got, dimen, str(tmp_path / "none.txt"), **kw)
+    assert summary == lines[-1] == "Error: dose data file reading error!"
+    assert open(fn_cc).read() == open(fn_py).read()
