@@ -13,8 +13,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgegs_hip.so")
 RUNNER = os.path.join(HERE, "gegs_run")
 SOURCES = ["gegs_capi.hip"]
-HOST_SOURCES = ["host_io.cpp", "host_input.cpp"]
-HEADERS = ["host_io.cpp", "host_input.cpp", "gegs_run.cpp", "transport.hpp", "gmath.hpp", "plain.hpp", "event.hpp", "mlc.hpp", "jaws.hpp", "blockwedge.hpp", "ctimport.hpp"]
+HOST_SOURCES = ["host_io.cpp", "host_input.cpp", "host_ct.cpp"]
+HEADERS = ["host_io.cpp", "host_input.cpp", "host_ct.cpp", "gegs_run.cpp", "transport.hpp", "gmath.hpp", "plain.hpp", "event.hpp", "mlc.hpp", "jaws.hpp", "blockwedge.hpp", "ctimport.hpp"]
 
 HIPCC_FLAGS = [
     "-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-shared",

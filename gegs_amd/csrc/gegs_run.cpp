@@ -1,19 +1,20 @@
 // gegs_run -- a prepared run of the reference's input, transported from C++.
 //
 //   gegs_run -i <file.ini> --media <tables.gegsmed> [--base-dir D] [--method batched|plain]
-//            [--seed S] [--batches B] [--fix Q1,Q4,Q7,Q8,Q15,Q16] [--dry-run <run file>] [--device N]
+//            [--seed S] [--batches B] [--fix Q1,Q4,Q7,Q8,Q9,Q10,Q15,Q16] [--dry-run <run file>] [--device N]
 //   gegs_run <run file> [--device N]
 //
 // With -i it reads the reference's run input itself (CUDA_EGS.cu:936-1480):
 // the [General], [Data], [Phase space], [SecJaws], [MLC], [Block], [Wedge],
-// [Phantom], [Calibration] and [Output factor] keys with the reference's
-// defaults and error codes, the .egsinp components (SECJAWS, DYNVMLC,
-// PYRAMIDS), the .mlcinfo field list with its leaf files and the output-factor
-// table; the physics tables come from a media-table file (the HATCH builders'
-// output, like the reference's PEGS data: `python -m gegs_amd.hatch`).  A run
-// input that asks for ctcreate from DICOM needs the package's DICOM reader
-// (`python -m gegs_amd.run -i <file.ini> --prepare <run file>`, then the run
-// file form).  --dry-run writes the run file and head file instead of running.
+// [CTcreate], [Phantom], [Calibration], [Output factor], [Interpolation] and
+// [Dose output] keys with the reference's defaults and error codes, the
+// .egsinp components (SECJAWS, DYNVMLC, PYRAMIDS), the .mlcinfo field list
+// with its leaf files and the output-factor table; with "use ctcreate = Yes"
+// it builds the phantom from the DICOM series and the CT ramps file and writes
+// it to <output root>.egsphant (ctcreate.c:1342-1600); the physics tables come
+// from a media-table file (the HATCH builders' output, like the reference's
+// PEGS data: `python -m gegs_amd.hatch`).  --dry-run writes the run file and
+// head file (and the CT phantom) instead of running.
 //
 // The run file (key = value lines) names the media-table file, the egsphant
 // phantom, the head file and the phase-space files, with the field list and
@@ -399,9 +400,16 @@ std::vector<double> need_numbers(const Ini &ini, const std::string &v, size_t n,
     return out;
 }
 
-uint32_t parse_fixes(const std::string &list, uint32_t &host) {
+// mkdir -p
+void make_dirs(const std::string &dir) {
+    for (size_t k = 0; k <= dir.size(); ++k)
+        if (k == dir.size() || (dir[k] == '/' && k > 0)) mkdir(dir.substr(0, k).c_str(), 0777);
+}
+
+uint32_t parse_fixes(const std::string &list, uint32_t &host, bool &q10) {
     uint32_t dev = 0;
     host = 0;
+    q10 = false;
     size_t i = 0;
     while (i <= list.size()) {
         size_t j = list.find(',', i);
@@ -416,7 +424,8 @@ uint32_t parse_fixes(const std::string &list, uint32_t &host) {
         else if (f == "q15") host |= GEGS_HOST_FIX_Q15;
         else if (f == "q16") host |= GEGS_HOST_FIX_Q16;
         else if (f == "q9") host |= GEGS_HOST_FIX_Q9;
-        else die(2001, "unknown fix \"%s\" (known here: Q1, Q4, Q7, Q8, Q9, Q15, Q16)", f.c_str());
+        else if (f == "q10") q10 = true;                                   // the CT ramp, applied while reading
+        else die(2001, "unknown fix \"%s\" (known here: Q1, Q4, Q7, Q8, Q9, Q10, Q15, Q16)", f.c_str());
     }
     return dev;
 }
@@ -432,7 +441,8 @@ RunFile from_ini(const char *path, const std::string &media, const std::string &
     r.method = method;
     r.seed = seed;
     r.batches = batches;
-    r.device_fixes = parse_fixes(fixes, r.host_fixes);
+    bool fix_q10 = false;
+    r.device_fixes = parse_fixes(fixes, r.host_fixes, fix_q10);
     const std::string base = base_dir.empty() ? cwd() : base_dir;
     std::string prefix = ini.get("General", "output prefix");
     if (prefix == kMissing) {
@@ -478,11 +488,21 @@ RunFile from_ini(const char *path, const std::string &media, const std::string &
     const bool wedge_on = ini.get("Wedge", "module Wedge") == "On";
     if (block_on) ini.req("Block", "block file", 2011);
     if (wedge_on) ini.req("Wedge", "wedge file", 2011);
-    if (ini.get("CTcreate", "use ctcreate") == "Yes")
-        die(2011, "\"%s\" creates its phantom from DICOM (use ctcreate = Yes): prepare it with `python -m gegs_amd.run "
-                  "-i %s --prepare <run file>` and run the run file",
-            path, path);
-    const std::string egsphant = ini.req("Phantom", "egsphant file", 2011);
+    // the phantom: ctcreate from a DICOM series (CUDA_EGS.cu:1327-1383), or an egsphant file
+    const bool use_ct = ini.get("CTcreate", "use ctcreate") == "Yes";
+    std::string egsphant, ct_ramps, dicom_list;
+    std::vector<double> ct_box, ct_voxel;
+    if (use_ct) {
+        ct_ramps = ini.req("CTcreate", "CT ramps file", 2012);
+        dicom_list = ini.get("CTcreate", "dicom file list");
+        if (dicom_list == kMissing) dicom_list = ini.req("CTcreate", "dicom directory", 2012);
+        ct_box = need_numbers(ini, ini.req("CTcreate", "xyz boundaries", 2011), 6, false, 2011,
+                              "boundaries of the subset of CT");
+        ct_voxel = need_numbers(ini, ini.req("CTcreate", "3d voxel sizes", 2011), 3, false, 2011,
+                                "voxel dimensions of CT");
+    } else {
+        egsphant = ini.req("Phantom", "egsphant file", 2011);
+    }
     r.change_xyz = ini.get("Phantom", "change coordinate") != "No" ? 1 : 0;
     v = need_numbers(ini, ini.req("Phantom", "isocenter location", 2011), 3, false, 2011, "isocenter location");
     for (int k = 0; k < 3; ++k) r.iso[k] = (float)v[k];
@@ -555,7 +575,28 @@ RunFile from_ini(const char *path, const std::string &media, const std::string &
     if (r.reldose) r.rel_file = abspath(d(r.rel_file));
     const std::string out_dir = host_path(base, outdir);
     r.root = abspath(join(out_dir, prefix));
-    r.egsphant = abspath(d(egsphant));
+    if (use_ct) {
+        // ctcreate writes <output root>.egsphant and the run reads the phantom back from it
+        gegs_ct_ramp ramp[GEGS_CT_MAX_RAMP];
+        uint32_t n_ramp = 0;
+        int32_t ct_lower = 0;
+        int rc = gegs_read_ct_ramps(d(ct_ramps).c_str(), ramp, &n_ramp, &ct_lower);
+        if (rc) die(rc, "%s", gegs_ct_last_error());
+        const float thick[3] = {(float)ct_voxel[0], (float)ct_voxel[1], (float)ct_voxel[2]};
+        const float lo[3] = {(float)ct_box[0], (float)ct_box[2], (float)ct_box[4]};
+        const float hi[3] = {(float)ct_box[1], (float)ct_box[3], (float)ct_box[5]};
+        gegs_egsphant ct;
+        rc = gegs_ctcreate(host_path(base, dicom_list).c_str(), thick, lo, hi, ramp, n_ramp, ct_lower, fix_q10 ? 1 : 0,
+                           &ct);
+        if (rc) die(rc, "%s", gegs_ct_last_error());
+        make_dirs(out_dir);
+        r.egsphant = abspath(join(out_dir, prefix) + ".egsphant");
+        rc = gegs_egsphant_write_ctcreate(r.egsphant.c_str(), &ct);
+        gegs_egsphant_free(&ct);
+        if (rc) die(rc, "%s", gegs_ct_last_error());
+    } else {
+        r.egsphant = abspath(d(egsphant));
+    }
     r.phsp_base = abspath(join(data_dir, slashes(phsp)));
 
     // head components, read whether or not the modules are on
@@ -773,15 +814,7 @@ int main(int argc, char **argv) {
             printf("run file %s\n", dry_run);
             return 0;
         }
-        const std::string out_dir = rf.root.substr(0, rf.root.rfind('/'));
-        if (!out_dir.empty()) {
-            std::string acc;
-            for (size_t k = 0; k <= out_dir.size(); ++k)
-                if (k == out_dir.size() || (out_dir[k] == '/' && k > 0)) {
-                    acc = out_dir.substr(0, k);
-                    mkdir(acc.c_str(), 0777);
-                }
-        }
+        make_dirs(rf.root.substr(0, rf.root.rfind('/')));
     } else {
         rf = read_run_file(argv[1]);
         log("GEGS (MI355X, C++ host) run file %s", argv[1]);

@@ -22,6 +22,8 @@ EGSnrc_Parameters.h:137-138) -- this is the reference's layout.
 from __future__ import annotations
 
 import ctypes
+import io
+import math
 import os
 import struct
 
@@ -65,6 +67,8 @@ def elements(data):
                 length = struct.unpack_from("<I", data, pos + 4)[0]
                 pos += 8
             else:
+                if pos + 4 > n:
+                    return
                 length = struct.unpack_from("<H", data, pos + 2)[0]
                 pos += 4
         else:
@@ -84,12 +88,41 @@ def elements(data):
             pos += length
 
 
+def _bad(what):
+    return ValueError("ERROR (5008): DICOM series: %s" % what)
+
+
+def _decimal(text):
+    """One finite decimal (surrounding blanks allowed), else error 5008."""
+    try:
+        v = float(text)
+    except ValueError:
+        raise _bad("%r is not a number" % text) from None
+    if not math.isfinite(v):
+        raise _bad("%r is not finite" % text)
+    return v
+
+
 def _doubles(value, count):
     """Backslash-separated decimal strings (string_to_doubles)."""
     parts = value.rstrip(b"\x00 ").split(b"\\")
     if len(parts) < count:
-        raise ValueError("expected %d values in %r" % (count, value))
-    return [float(p) for p in parts[:count]]
+        raise _bad("expected %d values in %r" % (count, value))
+    return [_decimal(p) for p in parts[:count]]
+
+
+def _read(path, mode="rb"):
+    try:
+        with open(path, mode) as f:
+            return f.read()
+    except OSError:
+        raise _bad("cannot read %s" % path) from None
+
+
+def _u16(val):
+    if len(val) < 2:
+        raise _bad("a rows or columns element shorter than 2 bytes")
+    return struct.unpack_from("<H", val)[0]
 
 
 class CTVolume:
@@ -115,15 +148,13 @@ def read_dicom_series(list_path):
     rescale applied only when slope and intercept are both non-zero, lower
     corner = first pixel centre - half a voxel."""
     folder = os.path.dirname(os.path.abspath(list_path))
-    with open(list_path, "r") as f:
-        names = [line.rstrip("\r\n") for line in f]
+    names = [line.rstrip("\r\n") for line in io.StringIO(_read(list_path, "r"))]
     names = [nm for nm in names if nm.strip()]
     slices = []                     # (z, order, path)
     seen = set()
     for order, nm in enumerate(names):
         path = os.path.join(folder, nm)
-        with open(path, "rb") as f:
-            data = f.read()
+        data = _read(path)
         z = None
         for tag, val in elements(data):
             if tag == TAG_IMAGE_TYPE and b"LOCALIZER" in val:
@@ -138,7 +169,7 @@ def read_dicom_series(list_path):
         seen.add(float(z))
         slices.append((float(z), order, path))
     if len(slices) < 2:
-        raise ValueError("need at least two CT slices at distinct z")
+        raise _bad("need at least two CT slices at distinct z")
     slices.sort()
     dz = np.float32(np.float32(slices[1][0]) - np.float32(slices[0][0]))
     rows = cols = None
@@ -147,35 +178,44 @@ def read_dicom_series(list_path):
     slope = intercept = np.float32(0.0)
     planes = []
     for n, (_, _, path) in enumerate(slices):
-        with open(path, "rb") as f:
-            data = f.read()
+        data = _read(path)
         pixels = None
         for tag, val in elements(data):
             if n == 0:
                 if tag == TAG_ROWS:
-                    rows = struct.unpack_from("<H", val)[0]
+                    rows = _u16(val)
                 elif tag == TAG_COLUMNS:
-                    cols = struct.unpack_from("<H", val)[0]
+                    cols = _u16(val)
                 elif tag == TAG_IMAGE_POSITION:
                     p = _doubles(val, 3)
                     offset = np.array([p[0] / 10.0, p[1] / 10.0, p[2] / 10.0], dtype=np.float32)
                 elif tag == TAG_PIXEL_SPACING:
                     spacing = _doubles(val, 2)
                 elif tag == TAG_RESCALE_INTERCEPT:
-                    intercept = np.float32(float(val.rstrip(b"\x00 ")))
+                    intercept = np.float32(_decimal(val.rstrip(b"\x00 ")))
                 elif tag == TAG_RESCALE_SLOPE:
-                    slope = np.float32(float(val.rstrip(b"\x00 ")))
+                    slope = np.float32(_decimal(val.rstrip(b"\x00 ")))
             if tag == TAG_PIXEL_DATA:
+                if len(val) % 2:
+                    raise _bad("%s: pixel data of odd length" % path)
                 pixels = np.frombuffer(val, dtype="<i2")
         if rows is None or cols is None or spacing is None or pixels is None:
-            raise ValueError("%s: missing rows, columns, pixel spacing or pixel data" % path)
+            raise _bad("%s: missing rows, columns, pixel spacing or pixel data" % path)
+        if rows == 0 or cols == 0:
+            raise _bad("%s: no rows or no columns" % path)
         if pixels.size < rows * cols:
-            raise ValueError("%s: short pixel data" % path)
+            raise _bad("%s: short pixel data" % path)
         planes.append(pixels[:rows * cols].reshape(rows, cols))
     hu = np.stack(planes).astype(np.int16)
     if slope != 0 and intercept != 0:
-        v = hu.astype(np.float32) * slope + intercept
-        hu = np.trunc(v).astype(np.int16)
+        # binary32 product and sum, stored to a 16-bit integer through a
+        # truncating conversion to 32 bits, as the reference's float to short
+        # assignment (ReadCT_DICOM.c:879-882) compiles on x86: values past
+        # 32 bits become INT_MIN, the 16-bit store keeps the low half
+        v = (hu.astype(np.float32) * slope + intercept).astype(np.float64)
+        t = np.trunc(v)
+        ok = np.isfinite(t) & (t >= -2.0 ** 31) & (t < 2.0 ** 31)
+        hu = np.where(ok, t, -2.0 ** 31).astype(np.int64).astype(np.int32).astype(np.int16)
     voxel = np.array([spacing[1] / 10.0, spacing[0] / 10.0, 0.0], dtype=np.float32)
     voxel[2] = dz
     corner = np.array([np.float64(offset[a]) - np.float64(voxel[a]) / 2.0 for a in range(3)], dtype=np.float32)
@@ -238,20 +278,40 @@ def read_ct_ramps(path, max_media=6):
     density at the lower end, density at the upper end, unused".  n = 0
     selects the built-in 4-media ramp.  Returns (ramp, lowest CT number);
     error 5005 for n outside [0, max_media] (the reference's MXMED is 6,
-    EGSnrc_Parameters.h:5)."""
-    with open(path) as f:
-        lines = f.read().splitlines()
-    head = lines[0].replace(" ", "").split(",")
-    n, lower = int(head[0]), int(head[1])
+    EGSnrc_Parameters.h:5), 5006 for a medium whose CT upper bound is below
+    the lowest CT number, whose lower density is negative or whose upper
+    density is below its lower one (ctcreate.c:1036-1043), 5007 for a file
+    that cannot be read or parsed (the reference reads it unchecked)."""
+    def bad(what):
+        return ValueError("ERROR (5007): CT ramps file %s: %s" % (path, what))
+
+    try:
+        with open(path) as f:
+            lines = f.read().splitlines()
+    except OSError:
+        raise bad("cannot be read") from None
+    try:
+        head = lines[0].replace(" ", "").split(",")
+        n, lower = int(head[0]), int(head[1])
+    except (IndexError, ValueError):
+        raise bad("no \"number of media, lowest CT number\" line") from None
     if n < 0 or n > max_media:
         raise ValueError("ERROR (5005): number of materials out of range")
     if n == 0:
         return list(ph.DEFAULT_RAMP), ph.DEFAULT_RAMP_LOWER
     ramp = []
     for m in range(n):
-        name = lines[1 + 2 * m][:24].strip()
-        vals = [v for v in lines[2 + 2 * m].replace(" ", "").split(",") if v]
-        ramp.append((name, int(vals[0]), float(np.float32(vals[1])), float(np.float32(vals[2]))))
+        try:
+            name = lines[1 + 2 * m][:24].strip()
+            vals = [v for v in lines[2 + 2 * m].replace(" ", "").split(",") if v]
+            up, dlo, dhi = int(vals[0]), float(np.float32(float(vals[1]))), float(np.float32(float(vals[2])))
+        except (IndexError, ValueError):
+            raise bad("medium %d needs a name line and a \"CT upper bound, density low, density high\" line"
+                      % (m + 1)) from None
+        if up < lower or dlo < 0 or dhi < dlo:
+            raise ValueError("ERROR (5006): CT no. or density out of range, or density upper bound < "
+                             "density lower bound (medium %d)" % (m + 1))
+        ramp.append((name, up, dlo, dhi))
     return ramp, lower
 
 

@@ -27,6 +27,8 @@ HOST_SYMBOLS = [
     "gegs_build_block_wedge", "gegs_read_mlcinfo", "gegs_read_leaf_file", "gegs_read_output_factors",
     "gegs_field_output_factor", "gegs_host_bounds", "gegs_interp_dose", "gegs_write_interp_dose", "gegs_abs_dose",
     "gegs_reldose_points", "gegs_reldose_compare", "gegs_write_relative_dose",
+    "gegs_ct_last_error", "gegs_dicom_read_series", "gegs_ct_volume_free", "gegs_read_ct_ramps",
+    "gegs_ct_to_medium", "gegs_ctcreate", "gegs_egsphant_write_ctcreate",
 ]
 NOT_FOUND = 1
 
@@ -36,6 +38,16 @@ class CEgsphant(ctypes.Structure):
                 ("names", ctypes.c_void_p), ("xb", ctypes.POINTER(ctypes.c_float)),
                 ("yb", ctypes.POINTER(ctypes.c_float)), ("zb", ctypes.POINTER(ctypes.c_float)),
                 ("media", ctypes.POINTER(ctypes.c_int32)), ("density", ctypes.POINTER(ctypes.c_float))]
+
+
+class CCtVolume(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_int32 * 3), ("voxel", ctypes.c_float * 3), ("offset", ctypes.c_float * 3),
+                ("hu", ctypes.POINTER(ctypes.c_int16))]
+
+
+class CCtRamp(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * NAME_LEN), ("ct_upper", ctypes.c_int32), ("density_lower", ctypes.c_float),
+                ("density_upper", ctypes.c_float)]
 
 
 class CSchedule(ctypes.Structure):
@@ -106,6 +118,15 @@ def lib():
     L.gegs_read_leaf_file.argtypes = [ctypes.c_char_p, vp]
     L.gegs_read_output_factors.argtypes = [ctypes.c_char_p, vp, vp, u32, vp]
     L.gegs_field_output_factor.argtypes = [ctypes.c_double] * 4 + [vp, vp, u32, vp]
+    L.gegs_ct_last_error.restype = ctypes.c_char_p
+    L.gegs_dicom_read_series.argtypes = [ctypes.c_char_p, ctypes.POINTER(CCtVolume)]
+    L.gegs_ct_volume_free.argtypes = [ctypes.POINTER(CCtVolume)]
+    L.gegs_ct_volume_free.restype = None
+    L.gegs_read_ct_ramps.argtypes = [ctypes.c_char_p, vp, vp, vp]
+    L.gegs_ct_to_medium.argtypes = [vp, ctypes.c_size_t, vp, u32, ctypes.c_int32, ctypes.c_int, vp, vp]
+    L.gegs_ctcreate.argtypes = [ctypes.c_char_p, vp, vp, vp, vp, u32, ctypes.c_int32, ctypes.c_int,
+                                ctypes.POINTER(CEgsphant)]
+    L.gegs_egsphant_write_ctcreate.argtypes = [ctypes.c_char_p, ctypes.POINTER(CEgsphant)]
     _L = L
     return L
 
@@ -159,6 +180,16 @@ class Egsphant:
     def __init__(self, path):
         c = CEgsphant()
         _check(lib().gegs_egsphant_read(path.encode(), ctypes.byref(c)))
+        self._adopt(c)
+
+    @classmethod
+    def _owning(cls, c):
+        self = cls.__new__(cls)
+        self._c = None
+        self._adopt(c)
+        return self
+
+    def _adopt(self, c):
         try:
             self.nx, self.ny, self.nz, nmed = c.nx, c.ny, c.nz, c.nmed
             raw = ctypes.string_at(c.names, nmed * NAME_LEN)
@@ -469,3 +500,80 @@ def write_relative_dose(root, calc, dimen, measured_path, start, sizes, isocente
                                       float(gamma_len), float(gamma_dose), fname, len(fname), summary,
                                       len(summary)))
     return fname.value.decode(), summary.value.decode()
+
+
+# ------------------------------------------------------------------ CT front end
+def _check_ct(rc):
+    if rc:
+        raise HostError(rc, lib().gegs_ct_last_error().decode())
+
+
+def read_dicom_series(list_path):
+    """gegs_dicom_read_series as a ctimport.CTVolume."""
+    from .ctimport import CTVolume
+    v = CCtVolume()
+    _check_ct(lib().gegs_dicom_read_series(list_path.encode(), ctypes.byref(v)))
+    try:
+        nx, ny, nz = v.size
+        hu = np.ctypeslib.as_array(v.hu, (nz * ny * nx,)).reshape(nz, ny, nx).copy()
+        return CTVolume(hu, list(v.voxel), list(v.offset))
+    finally:
+        lib().gegs_ct_volume_free(ctypes.byref(v))
+
+
+def _ramp_array(ramp):
+    arr = (CCtRamp * MXMED)()
+    if not 1 <= len(ramp) <= MXMED:
+        raise ValueError("a CT ramp has 1 to %d media" % MXMED)
+    for i, (name, up, dlo, dhi) in enumerate(ramp):
+        arr[i].name = name.encode()
+        arr[i].ct_upper = int(up)
+        arr[i].density_lower = float(dlo)
+        arr[i].density_upper = float(dhi)
+    return arr
+
+
+def read_ct_ramps(path):
+    """gegs_read_ct_ramps: ([(name, CT upper bound, density low, density high)], lowest CT number),
+    densities as the binary32 values the library holds."""
+    arr = (CCtRamp * MXMED)()
+    n = ctypes.c_uint32()
+    lower = ctypes.c_int32()
+    _check_ct(lib().gegs_read_ct_ramps(path.encode(), arr, ctypes.byref(n), ctypes.byref(lower)))
+    return [(arr[i].name.decode(), arr[i].ct_upper, arr[i].density_lower, arr[i].density_upper)
+            for i in range(n.value)], lower.value
+
+
+def ct_to_medium(hu, ramp, lower, fix_q10=False):
+    """gegs_ct_to_medium: (1-based media, binary32 densities) of int16 CT numbers."""
+    hu = np.ascontiguousarray(hu, dtype=np.int16).reshape(-1)
+    med = np.zeros(hu.size, dtype=np.int32)
+    dens = np.zeros(hu.size, dtype=np.float32)
+    _check_ct(lib().gegs_ct_to_medium(hu.ctypes.data, hu.size, _ramp_array(ramp), len(ramp), int(lower),
+                                      int(bool(fix_q10)), med.ctypes.data, dens.ctypes.data))
+    return med, dens
+
+
+def ctcreate(list_path, thickness, box_min, box_max, ramp, lower, fix_q10=False):
+    """gegs_ctcreate: the phantom of a DICOM series as an Egsphant (C memory owned)."""
+    f3 = lambda v: (ctypes.c_float * 3)(*[float(x) for x in v])
+    c = CEgsphant()
+    _check_ct(lib().gegs_ctcreate(list_path.encode(), f3(thickness), f3(box_min), f3(box_max), _ramp_array(ramp),
+                                  len(ramp), int(lower), int(bool(fix_q10)), ctypes.byref(c)))
+    return Egsphant._owning(c)
+
+
+def write_egsphant_ctcreate(path, phantom):
+    """gegs_egsphant_write_ctcreate of any phantom with media_names, bounds,
+    media_idx and density (x fastest)."""
+    nx, ny, nz = len(phantom.xb) - 1, len(phantom.yb) - 1, len(phantom.zb) - 1
+    keep = [np.ascontiguousarray(phantom.xb, dtype=np.float32), np.ascontiguousarray(phantom.yb, dtype=np.float32),
+            np.ascontiguousarray(phantom.zb, dtype=np.float32),
+            np.ascontiguousarray(np.asarray(phantom.media_idx).reshape(-1), dtype=np.int32),
+            np.ascontiguousarray(np.asarray(phantom.density).reshape(-1), dtype=np.float32)]
+    names = _names_block(phantom.media_names, max(1, len(phantom.media_names)))
+    fp = lambda a, t: a.ctypes.data_as(ctypes.POINTER(t))
+    c = CEgsphant(nx, ny, nz, len(phantom.media_names), ctypes.addressof(names), fp(keep[0], ctypes.c_float),
+                  fp(keep[1], ctypes.c_float), fp(keep[2], ctypes.c_float), fp(keep[3], ctypes.c_int32),
+                  fp(keep[4], ctypes.c_float))
+    _check_ct(lib().gegs_egsphant_write_ctcreate(path.encode(), ctypes.byref(c)))

@@ -132,7 +132,13 @@ def ct_to_medium(hu, ramp=DEFAULT_RAMP, lower=DEFAULT_RAMP_LOWER, fix_q10=False)
     """CTToMedium (ctcreate.c:824-1140): HU -> (1-based medium, density).
     CT values below the ramp become medium 0, vacuum, as in the reference
     (quirk Q10, ctcreate.c:1060-1066); with fix_q10 they take the first ramp
-    medium at its lowest density, as values above the ramp take the last."""
+    medium at its lowest density, as values above the ramp take the last.
+    The reference holds the ramp's densities as binary32 (ctcreate.c:862-863)
+    and computes dlo + (CT - lo) * (dhi - dlo) / (up - lo) in binary32 with
+    the density difference taken between the binary32 values
+    (ctcreate.c:1074-1095); for the built-in ramp's 0.302..1.101 and
+    1.101..2.088 that difference is one unit in the last place away from the
+    binary32 rounding of the decimal difference."""
     hu = np.asarray(hu, dtype=np.int32)
     med = np.zeros(hu.shape, dtype=np.int32)
     dens = np.zeros(hu.shape, dtype=np.float32)
@@ -149,7 +155,8 @@ def ct_to_medium(hu, ramp=DEFAULT_RAMP, lower=DEFAULT_RAMP_LOWER, fix_q10=False)
     for i, (_, up, dlo, dhi) in enumerate(ramp):
         sel = rest & ~assigned & (hu <= up)
         lo_ct = lower if i == 0 else upper[i - 1]
-        d = np.float32(dlo) + (hu[sel] - lo_ct).astype(np.float32) * np.float32(dhi - dlo) / np.float32(up - lo_ct)
+        span = np.float32(np.float32(dhi) - np.float32(dlo))
+        d = np.float32(dlo) + (hu[sel] - lo_ct).astype(np.float32) * span / np.float32(up - lo_ct)
         dens[sel] = d.astype(np.float32)
         med[sel] = i + 1
         assigned |= sel

@@ -90,6 +90,81 @@ int gegs_region_table(const gegs_egsphant *ph, const gegs_media_tables *tables,
                       const char names[GEGS_MXMED][GEGS_NAME_LEN], const char other_media[5][GEGS_NAME_LEN],
                       uint16_t flags, gegs_region *out /* nx * ny * nz + 6 */);
 
+/* ---------------------------------------------------------------- CT front end
+ * The phantom a run builds from a DICOM CT series with "use ctcreate = Yes"
+ * (ctcreate.c:1342-1600, called from CUDA_EGS.cu:1327-1383): read the
+ * series, crop it to a box, resample it onto transport voxels, map CT numbers
+ * to media and densities through a ramp, write the egsphant text the run then
+ * reads back.  Errors: status codes with gegs_ct_last_error(). */
+#define GEGS_CT_MAX_RAMP 6           /* media in a ramp (MXMED, EGSnrc_Parameters.h:5) */
+
+/* A CT volume: hu[i + nx * (j + ny * k)], x along a row (the column index), y
+ * along the columns, z over slices; voxel sizes and the lower corner of voxel
+ * (0, 0, 0) in cm.  hu is allocated by the library (gegs_ct_volume_free). */
+typedef struct gegs_ct_volume {
+    int32_t size[3];
+    float voxel[3];
+    float offset[3];
+    int16_t *hu;
+} gegs_ct_volume;
+
+/* readct_dicom (ReadCT_DICOM.c:316-909): the slices named one per line in
+ * list_path (relative to the list's folder), little-endian, explicit or
+ * implicit VR element by element, with or without the 128-byte preamble;
+ * ordered by the z of their image position, localizers and slices at a z
+ * already read dropped; slice thickness z[1] - z[0] in binary32; rows,
+ * columns, pixel spacing and rescale from the first slice, the rescale
+ * applied in binary32 when slope and intercept are both non-zero; lower
+ * corner = first pixel centre - half a voxel.  Error 5008 (this library's)
+ * for an unreadable file, fewer than two slices, a missing or malformed
+ * element or short pixel data; the reference exits there. */
+int gegs_dicom_read_series(const char *list_path, gegs_ct_volume *out);
+void gegs_ct_volume_free(gegs_ct_volume *v);
+
+/* One medium of a CT ramp: CT numbers up to ct_upper map to it, its density
+ * rising linearly from density_lower to density_upper over its CT range. */
+typedef struct gegs_ct_ramp {
+    char name[GEGS_NAME_LEN];
+    int32_t ct_upper;
+    float density_lower, density_upper;
+} gegs_ct_ramp;
+
+/* The CT ramps file (ctcreate.c:886-1043): "n,lowest CT number", then per
+ * medium a name line (its first 24 characters, trimmed) and "CT upper
+ * bound, density low, density high, unused"; n = 0 gives the built-in
+ * 4-media ramp with lowest CT number -1024 (ctcreate.c:904-929).  Errors
+ * 5005 (n outside [0, 6]), 5006 (an upper bound below the lowest CT number,
+ * a negative density or densities out of order), 5007 (this library's: the
+ * file cannot be read or parsed). */
+int gegs_read_ct_ramps(const char *path, gegs_ct_ramp ramp[GEGS_CT_MAX_RAMP], uint32_t *n, int32_t *ct_lower);
+
+/* CTToMedium (ctcreate.c:1052-1104): 1-based medium and density per CT
+ * number in binary32; below the ramp vacuum (0, 0.0; quirk Q10,
+ * ctcreate.c:1060-1066) or, with fix_q10, the first medium at its lowest
+ * density; above the ramp the last medium at its highest density. */
+int gegs_ct_to_medium(const int16_t *hu, size_t n, const gegs_ct_ramp *ramp, uint32_t n_ramp, int32_t ct_lower,
+                      int fix_q10, int32_t *media, float *density);
+
+/* The whole chain: the series of list_path cropped to [box_min, box_max]
+ * (gegs_ct_subsize; all zero = the whole series), resampled on voxels of
+ * about `thickness` cm (gegs_ct_resample_grid / gegs_ct_resample), media and
+ * densities by gegs_ct_to_medium, the ramp's names as the phantom's media.
+ * The phantom is allocated as gegs_egsphant_read allocates it
+ * (gegs_egsphant_free). */
+int gegs_ctcreate(const char *list_path, const float thickness[3], const float box_min[3], const float box_max[3],
+                  const gegs_ct_ramp *ramp, uint32_t n_ramp, int32_t ct_lower, int fix_q10, gegs_egsphant *out);
+
+/* The egsphant text ctcreate writes (ctcreate.c:1249-1311): the media count
+ * "%2d", the names, " 1.0" per medium, " %4d" x 3 voxel counts, each axis's
+ * bounds " %g" eight to a line, the media digits a row per line with an
+ * empty line after each slice, the densities "  %g" eight to a line
+ * (counted over a slice) with an empty line after each slice.  The run reads
+ * its phantom back from this text, so bounds and densities keep the six
+ * significant digits of %g. */
+int gegs_egsphant_write_ctcreate(const char *path, const gegs_egsphant *ph);
+
+const char *gegs_ct_last_error(void);
+
 /* ---------------------------------------------------------------- egsphsp
  * The phase-space input of a run: n_files MODE0 files <base>_p<i>.egsphsp1,
  * or one file (host fixes Q4 and Q15 above), read as one record sequence. */

@@ -5,7 +5,7 @@ import os
 
 import numpy as np
 
-from gegs_amd import hatch, head, phantom as ph, postproc, source, workloads
+from gegs_amd import ctimport, hatch, head, phantom as ph, postproc, source, workloads
 
 F32 = np.float32
 N0, N1 = 6000, 4000
@@ -86,6 +86,35 @@ norm to Max/Center = Max
 measured data = "PTW1000.txt"
 """ % (HPM, HPC))
     return ini, data
+
+
+def add_ctcreate(tmp, ini, ramp_text="0,0\n", box="0 0 0 0 0 0", voxel="0.25 0.25 0.25"):
+    """Switch the run input to a phantom built from DICOM (use ctcreate =
+    Yes): a 40 x 40 x 16 CT series of 1.5 mm pixels and 3 mm slices (air
+    around an elliptic water body with a lung and a bone insert, stored as
+    CT number + 1024 with slope 1 and intercept -1024), listed in reverse
+    order in dicom/list.dlt, and the CT ramps file data/ramps.inp."""
+    d = os.path.join(tmp, "dicom")
+    os.makedirs(d)
+    rng = np.random.default_rng(8)
+    yy, xx = np.mgrid[0:40, 0:40]
+    names = []
+    for k in range(16):
+        hu = np.full((40, 40), -1000)
+        hu[((xx - 19.5) / 18) ** 2 + ((yy - 19.5) / 15) ** 2 < 1] = 0
+        hu[((xx - 12) / 5) ** 2 + ((yy - 18) / 7) ** 2 < 1] = -800
+        hu[(xx - 28) ** 2 + (yy - 24) ** 2 < 16] = 1000
+        pix = (hu + 1024 + rng.integers(-20, 21, hu.shape)).astype(np.int16)
+        nm = "ct%02d.dcm" % k
+        ctimport.write_ct_slice(os.path.join(d, nm), pix, (-29.25, -29.25, -22.5 + 3.0 * k), (1.5, 1.5))
+        names.append(nm)
+    with open(os.path.join(d, "list.dlt"), "w") as f:
+        f.write("\n".join(names[::-1]) + "\n")
+    with open(os.path.join(tmp, "data", "ramps.inp"), "w") as f:
+        f.write(ramp_text)
+    with open(ini, "a") as f:
+        f.write('[CTcreate]\nuse ctcreate = Yes\nCT ramps file = "ramps.inp"\ndicom file list = "dicom\\list.dlt"\n'
+                "xyz boundaries = %s\n3d voxel sizes = %s\n" % (box, voxel))
 
 
 def by_hand(data, seed, ctx):

@@ -115,3 +115,32 @@ def test_cpp_driver_reads_the_run_input(tmp_path):
     py_rel = open(res.files["relative"]).read().splitlines()
     assert len(cpp_rel) == len(py_rel) == 26 and cpp_rel[-1].startswith("Totally 25 points compared")
     assert cpp_rel[-1] in cpp_log
+
+
+def test_cpp_driver_builds_the_ct_phantom(tmp_path):
+    """gegs_run -i with "use ctcreate = Yes": the C++ driver builds the
+    phantom from the DICOM series and the CT ramps file itself, writes it to
+    <output root>.egsphant and transports the plan on it; the dose file equals
+    the Python driver's on its own ctcreate phantom (to the FP64 atomics'
+    order)."""
+    import glob
+    import subprocess
+    from rundata import add_ctcreate
+    ini, _ = make_dataset(str(tmp_path))
+    add_ctcreate(str(tmp_path), ini)
+    rf = str(tmp_path / "tables.gegsrun")
+    grun.prepare(ini, rf, base_dir=str(tmp_path), seed=17, n_batches=10, echo=False)   # writes tables.gegsmed
+    print("C++ run driver: -i run input with ctcreate", flush=True)
+    r = subprocess.run([RUNNER, "-i", ini, "--media", str(tmp_path / "tables.gegsmed"), "--base-dir", str(tmp_path),
+                        "--seed", "17"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    outdir = os.path.join(str(tmp_path), "out")
+    (dose_file,) = glob.glob(os.path.join(outdir, "Syn_DoseResults_*.bin"))
+    assert dose_file.endswith("_024x024x019.bin")
+    cpp = np.fromfile(dose_file, dtype=np.float32)
+    ct_cpp = open(os.path.join(outdir, "Syn.egsphant"), "rb").read()
+    print("python run driver: batched with ctcreate", flush=True)
+    res = grun.run(ini, base_dir=str(tmp_path), method="batched", seed=17, echo=False)
+    assert open(os.path.join(outdir, "Syn.egsphant"), "rb").read() == ct_cpp
+    assert cpp.max() > 0
+    assert np.allclose(cpp, res.dose, rtol=3e-7, atol=0)

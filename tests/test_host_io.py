@@ -438,19 +438,29 @@ def _run_file_dict(path):
 
 
 @pytest.mark.skipif(not os.path.exists(RUNNER), reason="gegs_run not built")
-@pytest.mark.parametrize("variant", ["plain", "fixes", "block_wedge", "interpolation"])
+@pytest.mark.parametrize("variant", ["plain", "fixes", "block_wedge", "interpolation", "ctcreate", "ctcreate_q10"])
 def test_cpp_run_input_reader_matches_prepare(tmp_path, variant):
     """gegs_run -i <file.ini> --dry-run reads the run input, the .egsinp
     components, the field list, the leaf files and the output factors in
     C++ and writes the run file and head file that run.prepare writes from
     the package's Python readers: every key and every byte of the head
-    geometry equal."""
+    geometry equal; with "use ctcreate = Yes" the phantom built from the
+    DICOM series (the built-in ramp over the whole series, or a ramp file
+    whose lowest CT number leaves the air below it, with and without the Q10
+    fix, over a cropped box) is the same egsphant text byte for byte."""
     sys.path.insert(0, os.path.dirname(__file__))
-    from rundata import make_dataset
+    from rundata import add_ctcreate, make_dataset
     ini, data = make_dataset(str(tmp_path))
     fixes = ()
     if variant == "fixes":
         fixes = ("Q1", "Q7", "Q9", "Q15", "Q16")
+    if variant == "ctcreate":
+        add_ctcreate(str(tmp_path), ini)
+    if variant == "ctcreate_q10":
+        add_ctcreate(str(tmp_path), ini, ramp_text="2,-900\nAIR700ICRU\n-500,0.001,0.5,0\nICRUTISSUE700ICRU\n"
+                                                   "3000,0.9,2.0,0\n",
+                     box="-2.2 2.4 -1.9 2.6 -1.0 1.5", voxel="0.3 0.2 0.25")
+        fixes = ("Q10",)
     if variant == "interpolation":
         open(ini, "a").write("[Interpolation]\ndo interpolate = Yes\nstarting point = -2.5 -1.0 -3.0\n"
                              "xyz dimensions = 21 11 9\n3d voxel sizes = 0.25 0.2 0.5\n")
@@ -467,8 +477,23 @@ def test_cpp_run_input_reader_matches_prepare(tmp_path, variant):
            "--batches", "4", "--dry-run", rf_cc]
     if fixes:
         cmd += ["--fix", ",".join(fixes)]
+    ct_py = None
+    if variant.startswith("ctcreate"):
+        ct_py = open(os.path.join(str(tmp_path), "out", "Syn.egsphant"), "rb").read()
+        os.remove(os.path.join(str(tmp_path), "out", "Syn.egsphant"))
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
+    if ct_py is not None:
+        ct_cc = open(os.path.join(str(tmp_path), "out", "Syn.egsphant"), "rb").read()
+        assert ct_cc == ct_py
+        phan = ph.read_egsphant(os.path.join(str(tmp_path), "out", "Syn.egsphant"))
+        assert 0 not in phan.media_idx and len(set(phan.media_idx.tolist())) >= 2
+        if variant == "ctcreate_q10":                                 # without the fix the air is vacuum
+            from gegs_amd import ctimport
+            ramp, lower = ctimport.read_ct_ramps(os.path.join(data, "ramps.inp"))
+            bare = ctimport.ctcreate(os.path.join(str(tmp_path), "dicom", "list.dlt"), (0.3, 0.2, 0.25),
+                                     (-2.2, -1.9, -1.0), (2.4, 2.6, 1.5), ramp=ramp, lower=lower)
+            assert 0 in bare.media_idx
     kv_py, f_py = _run_file_dict(rf_py)
     kv_cc, f_cc = _run_file_dict(rf_cc)
     h_py, h_cc = kv_py.pop("head file"), kv_cc.pop("head file")
@@ -487,7 +512,11 @@ def test_cpp_run_input_errors_keep_the_reference_codes(tmp_path):
     text = open(ini).read()
     cases = [(text.replace("histories = ", "histories = x"), "ERROR (2014)"),
              (text.replace("[Phase space]\nphsp file number = 2\n", "[Phase space]\n"), "ERROR (5013)"),
-             (text + "[CTcreate]\nuse ctcreate = Yes\n", "--prepare"),
+             (text + "[CTcreate]\nuse ctcreate = Yes\n", "ERROR (2012)"),
+             (text + "[CTcreate]\nuse ctcreate = Yes\nCT ramps file = \"r.inp\"\ndicom file list = \"l.dlt\"\n"
+                     "xyz boundaries = 0 0 0 0 0\n3d voxel sizes = 0.2 0.2 0.2\n", "ERROR (2011)"),
+             (text + "[CTcreate]\nuse ctcreate = Yes\nCT ramps file = \"r.inp\"\ndicom file list = \"l.dlt\"\n"
+                     "xyz boundaries = 0 0 0 0 0 0\n3d voxel sizes = 0.2 0.2 0.2\n", "ERROR (5007)"),
              (text.replace("plan.mlcinfo", "nothere.mlcinfo"), "MLC info  file reading error")]
     for k, (t, want) in enumerate(cases):
         p = str(tmp_path / ("bad%d.ini" % k))
