@@ -504,6 +504,21 @@ def main():
                                        achieved=pbytes / (photon_ms * 1e-3) / 1e9,
                                        photon_kernel_share=photon_ms * 1e-3 / el2,
                                        timed_pass_histories_per_s=hist / el2)
+                # the photon kernel alone: the same steps with the electron
+                # side of each sweep after the photon side instead of beside it
+                tun = ctx.get_tuning()
+                if tun.get("sweep_streams", 2) == 2:
+                    ctx.set_tuning(dict(tun, sweep_streams=1))
+                    el3, _, _, cnt3, timing3 = run_method(ctx, method, args.steps, 0, b, args.seed,
+                                                          kernel_timing=True)
+                    ctx.set_tuning(tun)
+                    if timing3 and timing3[1]:
+                        ms3, n3 = timing3
+                        b3 = PHOTON_STEP_BYTES * cnt3["photon_steps"]
+                        res["roofline"]["isolated"] = dict(
+                            sweep_streams=1, launches=n3, avg_launch_us=ms3 / n3 * 1e3,
+                            achieved=b3 / (ms3 * 1e-3) / 1e9, frac=b3 / (ms3 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                            timed_pass_histories_per_s=hist / el3)
         results[method] = res
 
     head = "event" if "event" in results else methods[0]
@@ -543,7 +558,9 @@ def main():
                 "photon_kernel_share": rf["photon_kernel_share"],
                 "timed_pass_histories_per_s": rf["timed_pass_histories_per_s"],
                 "timing": "HIP events around every photon kernel launch over a second pass of the same steps "
-                          "(sweeps launched on the stream; the value pass replays graphs)",
+                          "(sweeps launched on the stream; the value pass replays graphs); the photon kernel "
+                          "shares the GPU with the electron side of its sweep, which runs on a second stream",
+                "isolated": rf.get("isolated"),
                 "valu_issue_frac": profile_figure("valu_issue_frac"),
                 "limiter": "VALU issue: the SIMDs issue VALU most of the kernel's cycles (valu_issue_frac, "
                            "SQ_ACTIVE_INST_VALU in the committed profile); real HBM traffic is traffic/launch"}

@@ -82,6 +82,7 @@ struct EventConfig {
     uint32_t electron_chunk = 16;    // condensed-history steps per electron and sweep (0: no limit)
     uint32_t refill_min = 16;        // idle lanes of a wavefront that trigger a refill
     uint32_t refill_slots = 64;      // bank slots a wavefront reserves per refill atomic
+    uint32_t sweep_streams = 2;      // 2: photon and electron sides of a sweep concurrently; 1: one after the other
     size_t capacity = 0;             // slots per bank (0: twice what can be in flight, at least 4 Mi)
     uint32_t inject = 6u << 20;      // primaries injected per refill
     uint32_t threshold = 6u << 20;   // refill when fewer particles are in flight
@@ -263,8 +264,11 @@ __device__ __forceinline__ void bind_tally(const EvArgs &a, const Part &p, uint3
 // Appends to the bank named by p.status.  The lanes that reach a push
 // together are grouped by destination; each group reserves its slots with
 // one atomic issued by its first lane.  Photon-step and electron-step pushes
-// go to the buffer drained in this sweep when their kernel has not run yet
-// in the sweep (photons_now / electrons_now), else to the next sweep's.
+// go to the buffer drained in this sweep when their kernel has not started
+// in the sweep (photons_now / electrons_now), else to the next sweep's.  The
+// photon side of a sweep (photon kernel, photon interactions) and its
+// electron side run concurrently, so only the injection, which precedes
+// both, pushes to the current buffers.
 struct BankSink {
     const EvArgs *a;
     uint32_t hist;
@@ -384,43 +388,96 @@ __device__ __forceinline__ TallyT<ADJ> fresh_tally(const EvArgs &a) {
     return t;
 }
 
-// single-pass events: compton, photo, rayleigh, pair, moller, brems, bhabha,
-// annih.  Wavefront w takes slots [64 w, 64 w + 64) and then strides by the
-// grid; an empty bank costs one load per wave.
+// One particle of single-pass event E: the interaction's sampler.
 template <int E, bool ADJ>
+__device__ __forceinline__ void interact(const EvArgs &a, Stream &s, Part &p, BankSink &sink, TallyT<ADJ> &t) {
+    if (E == P_COMPTON) compton(a.env, s, p, sink, t);
+    else if (E == P_PHOTO) photo(a.env, s, p, sink, t);
+    else if (E == P_RAYLEIGH) rayleigh(a.env, s, p, sink, t);
+    else if (E == P_PAIR) pair(a.env, s, p, sink, t);
+    else if (E == E_MOLLER) moller(a.env, s, p, sink, t);
+    else if (E == E_BREMS) brems(a.env, s, p, sink, t);
+    else if (E == E_BHABHA) bhabha(a.env, s, p, sink, t);
+    else annih(a.env, s, p, sink, t);
+}
+
+// particles in a bank (what BankView::total gives, without the prefix)
+__device__ __forceinline__ uint32_t bank_total(const EvArgs &a, int slot) {
+    uint32_t n = 0;
+#pragma unroll
+    for (int g = 0; g < QSEG; ++g) {
+        const uint32_t c = a.st->count[slot][g].v;
+        n += c < a.seg_cap ? c : a.seg_cap;
+    }
+    return n;
+}
+
+// The four single-pass events of one particle type in one launch: E0 = 1
+// (compton, photo, rayleigh, pair) or E0 = 6 (moller, brems, bhabha, annih).
+// The banks' particles are dealt in chunks of 64, bank after bank, and
+// wavefront w takes chunks w, w + nwaves, ...; a chunk holds particles of one
+// bank only, so a wavefront runs one sampler at a time.  Interactions create
+// particles only in the photon-step and electron-step banks, so the four
+// banks' counts hold for the whole launch.  One launch instead of four
+// removes three kernel boundaries from every sweep; in the last sweeps of a
+// run, when the banks hold a few particles each, the boundaries are most of
+// the time these events take.
+template <int E0, bool ADJ>
 __global__ void __launch_bounds__(256) interaction_kernel(EvArgs a) {
-    BankView view;
-    view.load(a, E);
-    const uint32_t n = view.total();
-    if (n == 0) return;
+    static_assert(E0 == P_COMPTON || E0 == E_MOLLER, "a group is the four single-pass events of one type");
+    uint32_t c1, c2, c3, c4;                 // chunks of the banks before bank E0 + 1, + 2, + 3, and in all
+    c1 = (bank_total(a, E0) + 63u) >> 6;
+    c2 = c1 + ((bank_total(a, E0 + 1) + 63u) >> 6);
+    c3 = c2 + ((bank_total(a, E0 + 2) + 63u) >> 6);
+    c4 = c3 + ((bank_total(a, E0 + 3) + 63u) >> 6);
+    if (c4 == 0) return;
     TallyT<ADJ> t = fresh_tally<ADJ>(a);
-    // photon interactions run before the electron kernel of the sweep
-    BankSink sink{&a, 0, a.st->flip, false, E < E_ELECTRON_STEP};
-    uint32_t pops = 0;
+    BankSink sink{&a, 0, a.st->flip, false, false};
+    uint32_t pops0 = 0, pops1 = 0, pops2 = 0, pops3 = 0;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    for (uint32_t base = wave * 64u; base < n; base += nwaves * 64u) {
-        const uint32_t idx = base + lane_id();
+    BankView view;
+    int cur = -1;
+    uint32_t n = 0;
+    for (uint32_t c = wave; c < c4; c += nwaves) {
+        const int k = (c >= c1) + (c >= c2) + (c >= c3);     // wave-uniform
+        const uint32_t first = k == 0 ? 0u : (k == 1 ? c1 : (k == 2 ? c2 : c3));
+        if (k != cur) {
+            view.load(a, E0 + k);
+            n = view.total();
+            cur = k;
+        }
+        const uint32_t idx = (c - first) * 64u + lane_id();
         if (idx < n) {
             uint32_t hist;
-            Part p = bank_load(a, E, E, view.slot_of(a, idx), hist);
+            Part p = bank_load(a, E0 + k, E0 + k, view.slot_of(a, idx), hist);
             bind_tally(a, p, hist, t);
             sink.hist = hist;
             Stream s;
             s.begin(p, a.env.seed);
-            if (E == P_COMPTON) compton(a.env, s, p, sink, t);
-            else if (E == P_PHOTO) photo(a.env, s, p, sink, t);
-            else if (E == P_RAYLEIGH) rayleigh(a.env, s, p, sink, t);
-            else if (E == P_PAIR) pair(a.env, s, p, sink, t);
-            else if (E == E_MOLLER) moller(a.env, s, p, sink, t);
-            else if (E == E_BREMS) brems(a.env, s, p, sink, t);
-            else if (E == E_BHABHA) bhabha(a.env, s, p, sink, t);
-            else annih(a.env, s, p, sink, t);
+            if (k == 0) {
+                interact<E0, ADJ>(a, s, p, sink, t);
+                pops0++;
+            } else if (k == 1) {
+                interact<E0 + 1, ADJ>(a, s, p, sink, t);
+                pops1++;
+            } else if (k == 2) {
+                interact<E0 + 2, ADJ>(a, s, p, sink, t);
+                pops2++;
+            } else {
+                interact<E0 + 3, ADJ>(a, s, p, sink, t);
+                pops3++;
+            }
             flush_particle(a, hist, t);
-            pops++;
         }
     }
-    flush_counters(a, t, E, pops);
+    flush_counters(a, t, E0, pops0);
+    const uint32_t v1 = wave_sum(pops1), v2 = wave_sum(pops2), v3 = wave_sum(pops3);
+    if (first_lane()) {
+        if (v1) atomicAdd(a.counters + 4 + E0 + 1, (unsigned long long)v1);
+        if (v2) atomicAdd(a.counters + 4 + E0 + 2, (unsigned long long)v2);
+        if (v3) atomicAdd(a.counters + 4 + E0 + 3, (unsigned long long)v3);
+    }
 }
 
 // Lane refill for the looping events: a wavefront reserves `slots` (>= 64)
@@ -497,7 +554,7 @@ __global__ void __launch_bounds__(512) photon_kernel(EvArgs a, uint32_t chunk, u
     env.fixes = kernel_fixes<GEN>(env.fixes);
     stage_env<TAB>(env, lds_env);
     TallyT<ADJ> t = fresh_tally<ADJ>(a);
-    BankSink sink{&a, 0, flip, false, true};
+    BankSink sink{&a, 0, flip, false, false};
     HoldSink hold{false};
     uint32_t pops = 0;
     bool have = false, interact = false;
@@ -725,6 +782,15 @@ __global__ void report_overflow(const EvState *st, unsigned long long *counters)
 }
 
 // ------------------------------------------------------------------ host side
+// The second stream of a sweep: the electron side runs on `side` between
+// the events fork (recorded on the sweep's stream after the injection) and
+// join (waited for before the sweep's bookkeeping).  side == nullptr: the
+// whole sweep on one stream, photon side first.
+struct SweepFork {
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+
 struct EventBanks {
     float *bank = nullptr;
     uint32_t cap = 0;
@@ -743,6 +809,7 @@ struct EventBanks {
     bool timing = false;             // measure the photon kernel with HIP events
     double photon_ms = 0.0;          // accumulated over the runs since the last reset
     uint64_t photon_launches = 0;
+    SweepFork fork;                  // the electron side's stream and its fork/join events
 
     // Banks for a run of n_hist histories.  Without an explicit capacity a
     // bank holds twice what can be in flight at an injection (what is left
@@ -774,6 +841,12 @@ struct EventBanks {
                 e = hipEventCreateWithFlags(&done[k], hipEventDisableTiming);
                 if (e != hipSuccess) return e;
             }
+            e = hipStreamCreateWithFlags(&fork.side, hipStreamNonBlocking);
+            if (e != hipSuccess) return e;
+            e = hipEventCreateWithFlags(&fork.fork, hipEventDisableTiming);
+            if (e != hipSuccess) return e;
+            e = hipEventCreateWithFlags(&fork.join, hipEventDisableTiming);
+            if (e != hipSuccess) return e;
         }
         if (cap < want) {
             if (bank) (void)hipFree(bank);
@@ -825,6 +898,10 @@ struct EventBanks {
         for (int k = 0; k < 2; ++k)
             if (done[k]) (void)hipEventDestroy(done[k]);
         done[0] = done[1] = nullptr;
+        if (fork.side) (void)hipStreamDestroy(fork.side);
+        if (fork.fork) (void)hipEventDestroy(fork.fork);
+        if (fork.join) (void)hipEventDestroy(fork.join);
+        fork = SweepFork();
         if (h_plan) (void)hipHostFree(h_plan);
         bank = nullptr;
         st = nullptr;
@@ -854,45 +931,49 @@ inline size_t flight_lds(const Env &env) {
 
 template <bool TAB, bool ADJ, bool GEN>
 inline void launch_sweep_t(const EvArgs &a, const EventConfig &cfg, hipStream_t stream, hipEvent_t t0,
-                           hipEvent_t t1) {
+                           hipEvent_t t1, const SweepFork &f) {
     const dim3 g(cfg.grid), b(cfg.block), fb(cfg.flight_block);
     const size_t lds = flight_lds(a.env);
     hipLaunchKernelGGL(inject_plan, dim3(1), dim3(1), 0, stream, a.st, cfg.threshold, cfg.inject);
     hipLaunchKernelGGL(inject_fill, g, b, 0, stream, a);
+    hipStream_t es = stream;
+    if (f.side) {
+        (void)hipEventRecord(f.fork, stream);
+        (void)hipStreamWaitEvent(f.side, f.fork, 0);
+        es = f.side;
+    }
     if (t0) (void)hipEventRecord(t0, stream);
     hipLaunchKernelGGL((photon_kernel<TAB, ADJ, GEN>), dim3(cfg.photon_grid), fb, lds, stream, a, cfg.photon_chunk,
                        cfg.refill_min, cfg.refill_slots);
     if (t1) (void)hipEventRecord(t1, stream);
-    hipLaunchKernelGGL((interaction_kernel<P_RAYLEIGH, ADJ>), g, b, 0, stream, a);
     hipLaunchKernelGGL((interaction_kernel<P_COMPTON, ADJ>), g, b, 0, stream, a);
-    hipLaunchKernelGGL((interaction_kernel<P_PHOTO, ADJ>), g, b, 0, stream, a);
-    hipLaunchKernelGGL((interaction_kernel<P_PAIR, ADJ>), g, b, 0, stream, a);
-    hipLaunchKernelGGL((electron_kernel<TAB, ADJ, GEN>), dim3(cfg.electron_grid), fb, lds, stream, a, cfg.electron_chunk,
+    hipLaunchKernelGGL((electron_kernel<TAB, ADJ, GEN>), dim3(cfg.electron_grid), fb, lds, es, a, cfg.electron_chunk,
                        cfg.refill_min, cfg.refill_slots);
-    hipLaunchKernelGGL((interaction_kernel<E_MOLLER, ADJ>), g, b, 0, stream, a);
-    hipLaunchKernelGGL((interaction_kernel<E_BREMS, ADJ>), g, b, 0, stream, a);
-    hipLaunchKernelGGL((interaction_kernel<E_BHABHA, ADJ>), g, b, 0, stream, a);
-    hipLaunchKernelGGL((interaction_kernel<E_ANNIH, ADJ>), g, b, 0, stream, a);
+    hipLaunchKernelGGL((interaction_kernel<E_MOLLER, ADJ>), g, b, 0, es, a);
+    if (f.side) {
+        (void)hipEventRecord(f.join, f.side);
+        (void)hipStreamWaitEvent(stream, f.join, 0);
+    }
     hipLaunchKernelGGL(sweep_end, dim3(1), dim3(1), 0, stream, a.st);
 }
 
 template <bool GEN>
 inline void launch_sweep_f(const EvArgs &a, const EventConfig &cfg, hipStream_t stream, hipEvent_t t0,
-                           hipEvent_t t1) {
+                           hipEvent_t t1, const SweepFork &f) {
     const bool adj = a.adj != nullptr;
     if (tab_in_lds(a.env)) {
-        if (adj) launch_sweep_t<true, true, GEN>(a, cfg, stream, t0, t1);
-        else launch_sweep_t<true, false, GEN>(a, cfg, stream, t0, t1);
+        if (adj) launch_sweep_t<true, true, GEN>(a, cfg, stream, t0, t1, f);
+        else launch_sweep_t<true, false, GEN>(a, cfg, stream, t0, t1, f);
     } else {
-        if (adj) launch_sweep_t<false, true, GEN>(a, cfg, stream, t0, t1);
-        else launch_sweep_t<false, false, GEN>(a, cfg, stream, t0, t1);
+        if (adj) launch_sweep_t<false, true, GEN>(a, cfg, stream, t0, t1, f);
+        else launch_sweep_t<false, false, GEN>(a, cfg, stream, t0, t1, f);
     }
 }
 
 inline void launch_sweep(const EvArgs &a, const EventConfig &cfg, hipStream_t stream, hipEvent_t t0 = nullptr,
-                         hipEvent_t t1 = nullptr) {
-    if (a.env.fixes & ENV_GENERAL) launch_sweep_f<true>(a, cfg, stream, t0, t1);
-    else launch_sweep_f<false>(a, cfg, stream, t0, t1);
+                         hipEvent_t t1 = nullptr, const SweepFork &f = SweepFork()) {
+    if (a.env.fixes & ENV_GENERAL) launch_sweep_f<true>(a, cfg, stream, t0, t1, f);
+    else launch_sweep_f<false>(a, cfg, stream, t0, t1, f);
 }
 
 template <bool TAB, bool ADJ, bool GEN>
@@ -939,7 +1020,8 @@ inline bool same_cfg(const EventConfig &x, const EventConfig &y) {
     return x.block == y.block && x.flight_block == y.flight_block && x.grid == y.grid && x.photon_grid == y.photon_grid &&
            x.electron_grid == y.electron_grid && x.photon_chunk == y.photon_chunk &&
            x.electron_chunk == y.electron_chunk && x.refill_min == y.refill_min && x.inject == y.inject &&
-           x.threshold == y.threshold && x.sweeps_per_graph == y.sweeps_per_graph && x.refill_slots == y.refill_slots;
+           x.threshold == y.threshold && x.sweeps_per_graph == y.sweeps_per_graph && x.refill_slots == y.refill_slots &&
+           x.sweep_streams == y.sweep_streams;
 }
 
 // Transport the histories of one batch (bk.h_plan, filled by the caller)
@@ -989,6 +1071,7 @@ inline hipError_t run_event_batch(EventBanks &bk, const Env &env, const gegs_sou
     // elapsed time for events recorded by graph nodes); otherwise the sweep
     // group is captured once per argument set and replayed.
     const bool timed = bk.timing && cfg.sweeps_per_graph <= (uint32_t)EventBanks::MAX_TIMED;
+    const SweepFork fork = cfg.sweep_streams == 2 ? bk.fork : SweepFork();
     if (timed) {
         e = bk.make_timing_events();
         if (e != hipSuccess) return e;
@@ -997,7 +1080,7 @@ inline hipError_t run_event_batch(EventBanks &bk, const Env &env, const gegs_sou
         hipGraph_t g;
         e = hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return e;
-        for (uint32_t k = 0; k < cfg.sweeps_per_graph; ++k) launch_sweep(a, cfg, stream);
+        for (uint32_t k = 0; k < cfg.sweeps_per_graph; ++k) launch_sweep(a, cfg, stream, nullptr, nullptr, fork);
         e = hipStreamEndCapture(stream, &g);
         if (e != hipSuccess) return e;
         e = hipGraphInstantiate(&bk.graph[0], g, nullptr, nullptr, 0);
@@ -1018,7 +1101,7 @@ inline hipError_t run_event_batch(EventBanks &bk, const Env &env, const gegs_sou
         hipError_t r = hipSuccess;
         if (timed) {
             for (uint32_t k = 0; k < cfg.sweeps_per_graph; ++k)
-                launch_sweep(a, cfg, stream, bk.tev[slot][2 * k], bk.tev[slot][2 * k + 1]);
+                launch_sweep(a, cfg, stream, bk.tev[slot][2 * k], bk.tev[slot][2 * k + 1], fork);
             r = hipGetLastError();
         } else {
             r = hipGraphLaunch(bk.graph[0], stream);

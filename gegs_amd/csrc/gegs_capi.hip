@@ -1029,7 +1029,10 @@ int gegs_set_tuning(gegs_ctx *c, const gegs_tuning *t) {
         if (t->sweeps_per_graph) cfg.sweeps_per_graph = t->sweeps_per_graph;
         if (t->capacity) cfg.capacity = (size_t)t->capacity;
         if (t->refill_slots) cfg.refill_slots = t->refill_slots;
+        if (t->sweep_streams) cfg.sweep_streams = t->sweep_streams;
     }
+    if (cfg.sweep_streams != 1 && cfg.sweep_streams != 2)
+        return fail(c, GEGS_ERR_ARG, "gegs_set_tuning: sweep_streams must be 1 or 2");
     if (cfg.refill_slots < 64 || cfg.refill_slots > 4096 || cfg.refill_slots % 64)
         return fail(c, GEGS_ERR_ARG, "gegs_set_tuning: refill_slots must be a multiple of 64 in [64, 4096]");
     if (cfg.block != 256 && cfg.block != 128 && cfg.block != 64)
@@ -1061,7 +1064,8 @@ int gegs_get_tuning(gegs_ctx *c, gegs_tuning *t) {
     t->sweeps_per_graph = cfg.sweeps_per_graph;
     t->flight_block = cfg.flight_block;
     t->refill_min = cfg.refill_min;
-    t->refill_slots = c->tuning.refill_slots;
+    t->refill_slots = cfg.refill_slots;
+    t->sweep_streams = cfg.sweep_streams;
     t->capacity = cfg.capacity;
     return GEGS_OK;
 }

@@ -71,6 +71,7 @@ typedef void *hipGraphExec_t;
 enum {
     hipEventDisableTiming = 2,
     hipStreamCaptureModeThreadLocal = 1,
+    hipStreamNonBlocking = 1,
     hipMemcpyHostToDevice = 1,
     hipMemcpyDeviceToHost = 2,
     hipDeviceAttributeMultiprocessorCount = 3
@@ -114,6 +115,12 @@ inline hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return hipSucces
 inline hipError_t hipStreamBeginCapture(hipStream_t, int) { return hipSuccess; }
 inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *) { return hipSuccess; }
 inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+inline hipError_t hipStreamCreateWithFlags(hipStream_t *s, int) {
+    *s = nullptr;
+    return hipSuccess;
+}
+inline hipError_t hipStreamDestroy(hipStream_t) { return hipSuccess; }
+inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
 inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, int, hipStream_t) {
     memcpy(d, s, n);
     return hipSuccess;

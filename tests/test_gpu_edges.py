@@ -79,3 +79,32 @@ def test_single_history_ragged_and_maximal_batches(gpu_ctx, ct_small, water16, w
     assert e["counters"]["histories"] == count and not e["counters"]["overflow"]
     one = ctx.run(seed=5, n_batches=1, method="event")
     assert np.allclose(e["dose"], one["dose"], rtol=1e-9, atol=1e-12 * max(1.0, one["dose"].max()))
+
+
+def test_sweep_streams_give_the_same_histories(gpu_ctx, ct_small):
+    """The photon and electron sides of a sweep on two streams (the default)
+    or one after the other on one stream: the same histories (energy and
+    steps per history equal bit for bit), the same dose to the order of the
+    FP64 sums; other stream counts are refused."""
+    print("edges: sweep_streams 1 and 2", flush=True)
+    ctx = gpu_ctx
+    ctx.load_workload(ct_small)
+    src = ct_small.source(3000, 12)
+    ctx.set_source(src)
+    base = ctx.get_tuning()
+    assert base["sweep_streams"] == 2
+    runs = {}
+    try:
+        for n in (1, 2):
+            ctx.set_tuning(dict(base, sweep_streams=n))
+            assert ctx.get_tuning()["sweep_streams"] == n
+            runs[n] = ctx.run(seed=3, n_batches=3, method="event", per_history=True)
+        with pytest.raises(native.GegsError) as e:
+            ctx.set_tuning(dict(base, sweep_streams=3))
+        assert e.value.code == 2001
+    finally:
+        ctx.set_tuning(base)
+    assert np.array_equal(runs[1]["steps"], runs[2]["steps"])
+    assert np.allclose(runs[1]["edep"], runs[2]["edep"], rtol=1e-12, atol=1e-12)
+    d1, d2 = runs[1]["dose"], runs[2]["dose"]
+    assert d2.sum() > 0 and np.allclose(d1, d2, rtol=1e-9, atol=1e-12 * d2.max())
