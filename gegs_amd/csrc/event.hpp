@@ -17,9 +17,14 @@
 //     idling until the slowest lane of the wavefront finishes;
 //   * primaries are injected from the resident source records whenever the
 //     number of particles in flight drops below a refill threshold;
-//   * a group of sweeps (inject, ten event kernels, bookkeeping) is captured
-//     in a hipGraph and replayed until no particle and no source record is
-//     left; the host reads one flag per replay.
+//   * a sweep injects primaries, then runs its photon side (the photon
+//     kernel, then one kernel for the four photon interactions) and its
+//     electron side (the electron kernel, then one kernel for the four
+//     electron interactions) concurrently on two streams, each side draining
+//     what the previous sweeps left for it, then does its bookkeeping;
+//   * a group of sweeps is captured in a hipGraph and replayed until no
+//     particle and no source record is left; the host reads one flag per
+//     replay.
 // Particles keep their Philox streams across banks, so every history follows
 // exactly the trajectory the plain kernel and the oracle give it.
 #pragma once
