@@ -39,6 +39,7 @@
 #endif
 #include <vector>
 
+#include "plain.hpp"
 #include "transport.hpp"
 
 namespace gegs {
@@ -88,6 +89,7 @@ struct EventConfig {
     uint32_t refill_min = 16;        // idle lanes of a wavefront that trigger a refill
     uint32_t refill_slots = 64;      // bank slots a wavefront reserves per refill atomic
     uint32_t sweep_streams = 2;      // 2: photon and electron sides of a sweep concurrently; 1: one after the other
+    uint32_t tail = 1u << 18;        // in-flight particles that one tail launch finishes (0: no tail launch)
     size_t capacity = 0;             // slots per bank (0: twice what can be in flight, at least 4 Mi)
     uint32_t inject = 6u << 20;      // primaries injected per refill
     uint32_t threshold = 6u << 20;   // refill when fewer particles are in flight
@@ -765,6 +767,92 @@ __global__ void sweep_end(EvState *st) {
     st->sweeps += 1;
 }
 
+// The last particles of a run, each with all its descendants in one thread.
+// Once every source record is injected and at most `tail` particles are in
+// flight, the sweeps would go on one interaction or one chunk of crossings
+// at a time, each sweep paying its launches and bank round trips for a few
+// particles; this launch takes the particles the last sweep left (photon
+// and electron banks) and transports each to the end of its history branch
+// as the plain kernel does, secondaries on a per-thread stack, interactions
+// in line.  A particle's trajectory does not depend on which kernel moves
+// it (each carries its stream), so the histories are the sweeps' own; a
+// suspended electron resumes from its banked step state.
+template <bool TAB, bool ADJ, bool GEN>
+__global__ void __launch_bounds__(128) GEGS_WAVES_PER_EU(2) tail_kernel(EvArgs a, float *stack, int depth) {
+    extern __shared__ float lds_env[];
+    const uint32_t flip = a.st->flip;
+    const int sp_in = bank_slot(P_PHOTON_STEP, flip, true);
+    const int se_in = bank_slot(E_ELECTRON_STEP, flip, true);
+    BankView vp, ve;
+    vp.load(a, sp_in);
+    ve.load(a, se_in);
+    const uint32_t np = vp.total(), n = np + ve.total();
+    if (n == 0) return;
+    Env env = a.env;
+    env.fixes = kernel_fixes<GEN>(env.fixes);
+    stage_env<TAB>(env, lds_env);
+    TallyT<ADJ> t = fresh_tally<ADJ>(a);
+    StackSink sink;
+    sink.base = stack;
+    sink.stride = gridDim.x * blockDim.x;
+    sink.tid = blockIdx.x * blockDim.x + threadIdx.x;
+    sink.depth = depth;
+    sink.overflow = 0;
+    sink.sp = 0;
+    uint32_t pops[N_EVENTS];
+#pragma unroll
+    for (int k = 0; k < N_EVENTS; ++k) pops[k] = 0;
+    for (uint32_t k = sink.tid; k < n; k += sink.stride) {
+        const bool photon = k < np;
+        const uint32_t at = photon ? vp.slot_of(a, k) : ve.slot_of(a, k - np);
+        uint32_t hist;
+        Part p = photon ? bank_load(a, sp_in, P_PHOTON_STEP, at, hist) : bank_load(a, se_in, E_ELECTRON_STEP, at, hist);
+        bind_tally(a, p, hist, t);
+        Stream s;
+        s.begin(p, env.seed);
+        sink.sp = 0;
+        const bool fresh = (p.spos & 8) == 0;     // a new flight or track, not a resumed one
+        if (photon) {
+            pops[P_PHOTON_STEP] += fresh ? 1u : 0u;
+            photon_step<false>(env, s, p, sink, t);
+        } else {
+            EState es;
+            if (fresh) es.init();
+            else load_estate(a, se_in, at, es);
+            pops[E_ELECTRON_STEP] += fresh ? 1u : 0u;
+            while (electron_iter(env, s, p, es, sink, t)) {
+            }
+        }
+        while (sink.sp > 0) {
+            Part q = sink.pop();
+            Stream sq;
+            sq.begin(q, env.seed);
+#pragma unroll
+            for (int e = 0; e < N_EVENTS; ++e) pops[e] += (q.status == e);
+            handle<false>(env, sq, q, sink, t);
+        }
+        flush_particle(a, hist, t);
+    }
+    if (sink.overflow) atomicAdd(&a.st->overflow, sink.overflow);
+    flush_counters(a, t, P_PHOTON_STEP, pops[P_PHOTON_STEP]);
+    unsigned long long *c = a.counters;
+#pragma unroll
+    for (int e = 1; e < N_EVENTS; ++e) {
+        const uint32_t v = wave_sum(pops[e]);
+        if (first_lane() && v) atomicAdd(c + 4 + e, (unsigned long long)v);
+    }
+}
+
+// after the tail launch: every bank is empty and nothing is left to inject
+__global__ void tail_end(EvState *st) {
+    for (int e = 0; e < N_SLOTS; ++e) {
+        for (int g = 0; g < QSEG; ++g) st->count[e][g].v = 0;
+        st->taken[e].v = 0;
+    }
+    st->inflight = 0;
+    st->active = 0;
+}
+
 __global__ void batch_begin(EvState *st, uint32_t first, uint32_t last) {
     for (int e = 0; e < N_SLOTS; ++e) {
         for (int g = 0; g < QSEG; ++g) st->count[e][g].v = 0;
@@ -787,6 +875,9 @@ __global__ void report_overflow(const EvState *st, unsigned long long *counters)
 }
 
 // ------------------------------------------------------------------ host side
+// EvState words the host reads after each replay: active, sweeps, flip,
+// src_next, src_last, inj_start, inj_n, inflight
+constexpr int FLAG_WORDS = 8;
 // The second stream of a sweep: the electron side runs on `side` between
 // the events fork (recorded on the sweep's stream after the injection) and
 // join (waited for before the sweep's bookkeeping).  side == nullptr: the
@@ -800,7 +891,9 @@ struct EventBanks {
     float *bank = nullptr;
     uint32_t cap = 0;
     EvState *st = nullptr;
-    uint32_t *h_flags = nullptr;     // pinned: active, sweeps, overflow
+    uint32_t *h_flags = nullptr;     // pinned, per replay slot: EvState from `active` to `inflight`
+    float *tail_stack = nullptr;     // per-thread stacks of the tail launch
+    size_t tail_floats = 0;
     hipEvent_t done[2] = {nullptr, nullptr};   // completion of the two replays in flight
     BatchPlan *d_plan = nullptr;     // injection plan of the current batch
     BatchPlan *h_plan = nullptr;     // pinned staging copy
@@ -836,7 +929,7 @@ struct EventBanks {
         if (!st) {
             e = hipMalloc(&st, sizeof(EvState));
             if (e != hipSuccess) return e;
-            e = hipHostMalloc(&h_flags, 4 * sizeof(uint32_t));
+            e = hipHostMalloc(&h_flags, 2 * FLAG_WORDS * sizeof(uint32_t));
             if (e != hipSuccess) return e;
             e = hipMalloc(&d_plan, sizeof(BatchPlan));
             if (e != hipSuccess) return e;
@@ -899,6 +992,9 @@ struct EventBanks {
         if (bank) (void)hipFree(bank);
         if (st) (void)hipFree(st);
         if (h_flags) (void)hipHostFree(h_flags);
+        if (tail_stack) (void)hipFree(tail_stack);
+        tail_stack = nullptr;
+        tail_floats = 0;
         if (d_plan) (void)hipFree(d_plan);
         for (int k = 0; k < 2; ++k)
             if (done[k]) (void)hipEventDestroy(done[k]);
@@ -981,6 +1077,36 @@ inline void launch_sweep(const EvArgs &a, const EventConfig &cfg, hipStream_t st
     else launch_sweep_f<false>(a, cfg, stream, t0, t1, f);
 }
 
+constexpr uint32_t TAIL_BLOCK = 128;
+constexpr int TAIL_DEPTH = 40;       // particles per thread's stack (the plain kernel's default)
+
+// threads of the tail launch: one per particle up to `tail`, whole blocks
+inline uint32_t tail_threads(const EventConfig &cfg) {
+    return (cfg.tail + TAIL_BLOCK - 1) / TAIL_BLOCK * TAIL_BLOCK;
+}
+
+// The tail launch and the bookkeeping after it; `stack` holds
+// tail_threads(cfg) x TAIL_DEPTH particles.
+template <bool GEN>
+inline void launch_tail_g(const EvArgs &a, const EventConfig &cfg, float *stack, hipStream_t stream) {
+    const dim3 g(tail_threads(cfg) / TAIL_BLOCK), b(TAIL_BLOCK);
+    const size_t lds = flight_lds(a.env);
+    const bool adj = a.adj != nullptr;
+    if (tab_in_lds(a.env)) {
+        if (adj) hipLaunchKernelGGL((tail_kernel<true, true, GEN>), g, b, lds, stream, a, stack, TAIL_DEPTH);
+        else hipLaunchKernelGGL((tail_kernel<true, false, GEN>), g, b, lds, stream, a, stack, TAIL_DEPTH);
+    } else {
+        if (adj) hipLaunchKernelGGL((tail_kernel<false, true, GEN>), g, b, lds, stream, a, stack, TAIL_DEPTH);
+        else hipLaunchKernelGGL((tail_kernel<false, false, GEN>), g, b, lds, stream, a, stack, TAIL_DEPTH);
+    }
+    hipLaunchKernelGGL(tail_end, dim3(1), dim3(1), 0, stream, a.st);
+}
+
+inline void launch_tail(const EvArgs &a, const EventConfig &cfg, float *stack, hipStream_t stream) {
+    if (a.env.fixes & ENV_GENERAL) launch_tail_g<true>(a, cfg, stack, stream);
+    else launch_tail_g<false>(a, cfg, stack, stream);
+}
+
 template <bool TAB, bool ADJ, bool GEN>
 inline void flight_grids_t(EventConfig &cfg, size_t lds) {
     if (!cfg.photon_grid) cfg.photon_grid = resident_grid(photon_kernel<TAB, ADJ, GEN>, cfg.flight_block, lds);
@@ -1026,7 +1152,7 @@ inline bool same_cfg(const EventConfig &x, const EventConfig &y) {
            x.electron_grid == y.electron_grid && x.photon_chunk == y.photon_chunk &&
            x.electron_chunk == y.electron_chunk && x.refill_min == y.refill_min && x.inject == y.inject &&
            x.threshold == y.threshold && x.sweeps_per_graph == y.sweeps_per_graph && x.refill_slots == y.refill_slots &&
-           x.sweep_streams == y.sweep_streams;
+           x.sweep_streams == y.sweep_streams && x.tail == y.tail;
 }
 
 // Transport the histories of one batch (bk.h_plan, filled by the caller)
@@ -1112,25 +1238,51 @@ inline hipError_t run_event_batch(EventBanks &bk, const Env &env, const gegs_sou
             r = hipGraphLaunch(bk.graph[0], stream);
         }
         if (r != hipSuccess) return r;
-        r = hipMemcpyAsync(bk.h_flags + 2 * slot, &bk.st->active, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream);
+        r = hipMemcpyAsync(bk.h_flags + FLAG_WORDS * slot, &bk.st->active, FLAG_WORDS * sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, stream);
         if (r != hipSuccess) return r;
         return hipEventRecord(bk.done[slot], stream);
+    };
+    // The tail launch finishes the run once every record is injected and at
+    // most cfg.tail particles are in flight (not in a timed pass, whose
+    // photon steps must all be photon-kernel steps).
+    const bool use_tail = cfg.tail && !timed;
+    if (use_tail) {
+        const size_t need = (size_t)tail_threads(cfg) * TAIL_DEPTH * PART_WORDS;
+        if (bk.tail_floats < need) {
+            if (bk.tail_stack) (void)hipFree(bk.tail_stack);
+            bk.tail_stack = nullptr;
+            bk.tail_floats = 0;
+            e = hipMalloc(&bk.tail_stack, need * sizeof(float));
+            if (e != hipSuccess) return e;
+            bk.tail_floats = need;
+        }
+    }
+    auto tail_ready = [&](int slot) {
+        const uint32_t *f = bk.h_flags + FLAG_WORDS * slot;
+        return use_tail && f[3] >= f[4] && f[7] <= cfg.tail;     // src_next >= src_last, inflight
     };
     if ((e = launch(0)) != hipSuccess) return e;
     if ((e = launch(1)) != hipSuccess) return e;
     for (int k = 0;; ++k) {
-        const int slot = k & 1;
+        const int slot = k & 1, other = slot ^ 1;
         e = hipEventSynchronize(bk.done[slot]);
         if (e != hipSuccess) return e;
         if (timed && (e = bk.add_timing(slot, cfg.sweeps_per_graph)) != hipSuccess) return e;
         sweeps += cfg.sweeps_per_graph;
-        if (!bk.h_flags[2 * slot]) {
-            // the replay queued behind this one finds nothing left to do;
-            // wait for it so that its launches are counted too
-            const int other = slot ^ 1;
+        if (!bk.h_flags[FLAG_WORDS * slot] || tail_ready(slot)) {
+            // the replay queued behind this one finds nothing (or a tail's
+            // worth) left to do; wait for it so that its launches are counted
             e = hipEventSynchronize(bk.done[other]);
             if (e != hipSuccess) return e;
             if (timed && (e = bk.add_timing(other, cfg.sweeps_per_graph)) != hipSuccess) return e;
+            if (bk.h_flags[FLAG_WORDS * other]) {
+                launch_tail(a, cfg, bk.tail_stack, stream);
+                e = hipGetLastError();
+                if (e != hipSuccess) return e;
+                e = hipStreamSynchronize(stream);
+                if (e != hipSuccess) return e;
+            }
             break;
         }
         if (sweeps >= max_sweeps) {

@@ -1030,7 +1030,9 @@ int gegs_set_tuning(gegs_ctx *c, const gegs_tuning *t) {
         if (t->capacity) cfg.capacity = (size_t)t->capacity;
         if (t->refill_slots) cfg.refill_slots = t->refill_slots;
         if (t->sweep_streams) cfg.sweep_streams = t->sweep_streams;
+        if (t->tail_particles) cfg.tail = t->tail_particles == 0xFFFFFFFFu ? 0u : t->tail_particles;
     }
+    if (cfg.tail > (1u << 18)) return fail(c, GEGS_ERR_ARG, "gegs_set_tuning: tail_particles above 262144");
     if (cfg.sweep_streams != 1 && cfg.sweep_streams != 2)
         return fail(c, GEGS_ERR_ARG, "gegs_set_tuning: sweep_streams must be 1 or 2");
     if (cfg.refill_slots < 64 || cfg.refill_slots > 4096 || cfg.refill_slots % 64)
@@ -1066,6 +1068,7 @@ int gegs_get_tuning(gegs_ctx *c, gegs_tuning *t) {
     t->refill_min = cfg.refill_min;
     t->refill_slots = cfg.refill_slots;
     t->sweep_streams = cfg.sweep_streams;
+    t->tail_particles = cfg.tail ? cfg.tail : 0xFFFFFFFFu;
     t->capacity = cfg.capacity;
     return GEGS_OK;
 }

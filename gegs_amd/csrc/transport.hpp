@@ -856,12 +856,15 @@ __device__ __forceinline__ void suspend(Sink &sink, Stream &s, Part &p, int stat
     sink.push(p);
 }
 
-template <class Sink, class T>
+// VOTE: the lanes that step together generate their look-ahead Philox blocks
+// together (a vote among them); false where lanes step different particles
+// unsynchronised and no vote can be shared.  The uniforms are the same.
+template <bool VOTE = true, class Sink, class T>
 __device__ __forceinline__ void photon_step(const Env &env, Stream &s, Part &p, Sink &sink, T &t) {
     PhotonWalk w;
     photon_walk_begin(env, p, w);
     for (;;) {
-        wave_lookahead(s, true);
+        if (VOTE) wave_lookahead(s, true);
         const int r = photon_walk_iter(env, s, p, w, sink, t);
         if (r == PW_ON) continue;
         if (r == PW_INTERACT) photon_interaction(env, s, p, w, sink);
@@ -2129,12 +2132,12 @@ __device__ __forceinline__ bool electron_iter(const Env &env, Stream &s, Part &p
     return false;
 }
 
-template <class Sink, class T>
+template <bool VOTE = true, class Sink, class T>
 __device__ __forceinline__ void electron_step(const Env &env, Stream &s, Part &p, Sink &sink, T &t) {
     EState st;
     st.init();
     for (;;) {
-        wave_lookahead(s, true);
+        if (VOTE) wave_lookahead(s, true);
         if (!electron_iter(env, s, p, st, sink, t)) break;
     }
 }
@@ -2533,15 +2536,15 @@ __device__ __forceinline__ void annih(const Env &env, Stream &s, Part &p, Sink &
 }
 
 // dispatch one particle to its event handler
-template <class Sink, class T>
+template <bool VOTE = true, class Sink, class T>
 __device__ __forceinline__ void handle(const Env &env, Stream &s, Part &p, Sink &sink, T &t) {
     switch (p.status) {
-    case P_PHOTON_STEP: photon_step(env, s, p, sink, t); break;
+    case P_PHOTON_STEP: photon_step<VOTE>(env, s, p, sink, t); break;
     case P_COMPTON: compton(env, s, p, sink, t); break;
     case P_PHOTO: photo(env, s, p, sink, t); break;
     case P_RAYLEIGH: rayleigh(env, s, p, sink, t); break;
     case P_PAIR: pair(env, s, p, sink, t); break;
-    case E_ELECTRON_STEP: electron_step(env, s, p, sink, t); break;
+    case E_ELECTRON_STEP: electron_step<VOTE>(env, s, p, sink, t); break;
     case E_MOLLER: moller(env, s, p, sink, t); break;
     case E_BREMS: brems(env, s, p, sink, t); break;
     case E_BHABHA: bhabha(env, s, p, sink, t); break;

@@ -50,11 +50,12 @@ class Tuning(ctypes.Structure):
                 ("inject", ctypes.c_uint32), ("threshold", ctypes.c_uint32),
                 ("sweeps_per_graph", ctypes.c_uint32), ("flight_block", ctypes.c_uint32),
                 ("refill_min", ctypes.c_uint32), ("refill_slots", ctypes.c_uint32),
-                ("sweep_streams", ctypes.c_uint32), ("capacity", ctypes.c_uint64)]
+                ("sweep_streams", ctypes.c_uint32), ("tail_particles", ctypes.c_uint32),
+                ("capacity", ctypes.c_uint64)]
 
     FIELDS = ("block", "grid", "photon_grid", "electron_grid", "photon_chunk", "electron_chunk",
               "inject", "threshold", "sweeps_per_graph", "flight_block", "refill_min", "refill_slots",
-              "sweep_streams", "capacity")
+              "sweep_streams", "tail_particles", "capacity")
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k in self.FIELDS}

@@ -166,6 +166,9 @@ typedef struct gegs_tuning {
     uint32_t refill_slots;       /* bank slots a wavefront of the photon/electron kernels reserves per atomic: 64 ... 4096, multiple of 64 (64) */
     uint32_t sweep_streams;      /* streams a sweep runs on: 2 = the photon side (photon kernel and interactions) and
                                     the electron side concurrently, 1 = one after the other (2) */
+    uint32_t tail_particles;     /* once every source record is injected and at most this many particles are in
+                                    flight, one launch finishes them, each with its descendants in one thread
+                                    (262144, the most; 0xFFFFFFFF: never, the sweeps run to the end) */
     uint64_t capacity;           /* particles per bank, rounded down to a multiple of 32: each bank is 32 equal
                                     append segments, and a full segment counts as overflow (0: twice the
                                     injection plus the threshold, each at most the run's histories, and at

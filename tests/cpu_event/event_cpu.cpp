@@ -54,6 +54,8 @@ struct Meta {
     // 64-lane build: flight block size, refill threshold and reservation,
     // blocks per flight launch, primaries per injection and the injection threshold
     uint32_t flight_block, refill_min, refill_slots, flight_grid, inject, threshold;
+    // in-flight particles at or below which the tail launch finishes the run (0: never)
+    uint32_t tail;
 };
 
 int main(int argc, char **argv) {
@@ -196,21 +198,33 @@ int main(int argc, char **argv) {
     // a small injection size keeps several generations of particles in flight
     cfg.inject = m.inject ? m.inject : 256;
     cfg.threshold = m.threshold ? m.threshold : 512;
+    cfg.tail = m.tail;
+    float *tail_stack = m.tail ? (float *)calloc((size_t)tail_threads(cfg) * TAIL_DEPTH * PART_WORDS, sizeof(float))
+                               : nullptr;
     cpu_launch(batch_begin, dim3(1), dim3(1), st, 0u, m.nsrc);
+    bool tailed = false;
     for (;;) {
         launch_sweep(a, cfg, nullptr);
         ++sweeps;
         if (st->overflow) return 5;
         if (!st->active) break;
+        if (m.tail && st->src_next >= st->src_last && st->inflight <= m.tail) {
+            launch_tail(a, cfg, tail_stack, nullptr);
+            tailed = true;
+            if (st->overflow) return 5;
+            if (st->active) return 7;
+            break;
+        }
         if (sweeps > 1000000) return 6;
     }
     spill(d + "/out_edep.bin", hist_edep, m.nsrc * sizeof(double));
     spill(d + "/out_steps.bin", hist_steps, m.nsrc * sizeof(uint32_t));
     spill(d + "/out_dose.bin", dose, nvox * m.nbatch * sizeof(double));
 #ifdef GEGS_SIMT
-    printf("sweeps %llu wavefront-ops %llu\n", (unsigned long long)sweeps, (unsigned long long)simt::S.ops);
+    printf("sweeps %llu%s wavefront-ops %llu\n", (unsigned long long)sweeps, tailed ? " tail" : "",
+           (unsigned long long)simt::S.ops);
 #else
-    printf("sweeps %llu\n", (unsigned long long)sweeps);
+    printf("sweeps %llu%s\n", (unsigned long long)sweeps, tailed ? " tail" : "");
 #endif
     return 0;
 }

@@ -55,7 +55,7 @@ def build(tmp, floats, simt=False):
 
 
 def dump(wl, src, seed, d, nbatch, pchunk, echunk, cap=1 << 16, flight_block=256, refill_min=16, refill_slots=64,
-         flight_grid=4, inject=0, threshold=0):
+         flight_grid=4, inject=0, threshold=0, tail=0):
     os.makedirs(d, exist_ok=True)
     ph = wl.phantom
     first = wl.field.module_first
@@ -67,9 +67,9 @@ def dump(wl, src, seed, d, nbatch, pchunk, echunk, cap=1 << 16, flight_block=256
         with open(os.path.join(d, name), "wb") as f:
             f.write(data)
 
-    put("meta.bin", struct.pack("<10I2Q6I", ph.nx, ph.ny, ph.nz, len(src), nbatch, pchunk, echunk, have_mlc,
+    put("meta.bin", struct.pack("<10I2Q7I", ph.nx, ph.ny, ph.nz, len(src), nbatch, pchunk, echunk, have_mlc,
                                 have_jaws, cap, seed, 0, flight_block, refill_min, refill_slots, flight_grid,
-                                inject, threshold))
+                                inject, threshold, tail))
     put("tables.bin", ctypes.string_at(ctypes.addressof(wl.tables.c), ctypes.sizeof(wl.tables.c)))
     put("regions.bin", np.ascontiguousarray(wl.regions).tobytes())
     for n, b in (("xb", ph.xb), ("yb", ph.yb), ("zb", ph.zb)):
@@ -90,20 +90,25 @@ CASES = {
     "ct_suspend_every_step": (lambda: workloads.ct(), 150, 1, 1),
     "ct_mlc": (lambda: workloads.ct_mlc(), 2500, 32, 16),
     "ct_jaws_mlc": (lambda: workloads.ct_jaws_mlc(), 2500, 8, 4),
+    # the tail launch takes over once at most 300 particles are in flight
+    "ct_tail": (lambda: workloads.ct(), 400, 32, 16, 300),
+    "ct_mlc_tail": (lambda: workloads.ct_mlc(), 2500, 16, 8, 300),
 }
 
 
 @pytest.mark.parametrize("case", sorted(CASES))
 def test_event_path_cpu_build_matches_oracle(case, tmp_path):
-    make, n, pchunk, echunk = CASES[case]
+    make, n, pchunk, echunk = CASES[case][:4]
+    tail = CASES[case][4] if len(CASES[case]) > 4 else 0
     wl = make()
     src = wl.source(n, 11)
     d = str(tmp_path / case)
-    dump(wl, src, 77, d, 2, pchunk, echunk)
+    dump(wl, src, 77, d, 2, pchunk, echunk, tail=tail)
     exe = build(str(tmp_path), lds_floats(wl))
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=0")
     r = subprocess.run([exe, d], capture_output=True, text=True, env=env, timeout=600)
     assert r.returncode == 0, r.stderr[-3000:]
+    assert ("tail" in r.stdout) == (tail > 0), r.stdout
     edep = np.fromfile(os.path.join(d, "out_edep.bin"), dtype=np.float64)
     steps = np.fromfile(os.path.join(d, "out_steps.bin"), dtype=np.uint32)
     rows = np.fromfile(os.path.join(d, "out_dose.bin"), dtype=np.float64).reshape(2, -1)
@@ -131,21 +136,24 @@ SIMT_CASES = {
     "ct_refill1_chunk8": (lambda: workloads.ct(), 1200, 7, (8, 4, 1, 64, 3, 512, 256)),
     "ct_mlc": (lambda: workloads.ct_mlc(), 2000, 7, (64, 16, 16, 128, 4, 1024, 2048)),
     "ct_jaws_mlc_chunk4": (lambda: workloads.ct_jaws_mlc(), 1500, 7, (4, 2, 16, 64, 4, 768, 512)),
+    "ct_tail_chunk16": (lambda: workloads.ct(), 1200, 7, (16, 8, 16, 64, 4, 512, 512), 700),
 }
 
 
 @pytest.mark.parametrize("case", sorted(SIMT_CASES))
 def test_event_path_64_lane_build_matches_oracle(case, tmp_path):
-    make, n, seed, (pchunk, echunk, rmin, rslots, fgrid, inject, thr) = SIMT_CASES[case]
+    make, n, seed, (pchunk, echunk, rmin, rslots, fgrid, inject, thr) = SIMT_CASES[case][:4]
+    tail = SIMT_CASES[case][4] if len(SIMT_CASES[case]) > 4 else 0
     wl = make()
     src = wl.source(n, 11)
     d = str(tmp_path / case)
     dump(wl, src, seed, d, 2, pchunk, echunk, flight_block=256, refill_min=rmin, refill_slots=rslots,
-         flight_grid=fgrid, inject=inject, threshold=thr)
+         flight_grid=fgrid, inject=inject, threshold=thr, tail=tail)
     exe = build(str(tmp_path), lds_floats(wl), simt=True)
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=0:detect_stack_use_after_return=0")
     r = subprocess.run([exe, d], capture_output=True, text=True, env=env, timeout=900)
     assert r.returncode == 0, (r.stdout[-2000:], r.stderr[-3000:])
+    assert ("tail" in r.stdout) == (tail > 0), r.stdout
     edep = np.fromfile(os.path.join(d, "out_edep.bin"), dtype=np.float64)
     steps = np.fromfile(os.path.join(d, "out_steps.bin"), dtype=np.uint32)
     rows = np.fromfile(os.path.join(d, "out_dose.bin"), dtype=np.float64).reshape(2, -1)

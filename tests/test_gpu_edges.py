@@ -108,3 +108,31 @@ def test_sweep_streams_give_the_same_histories(gpu_ctx, ct_small):
     assert np.allclose(runs[1]["edep"], runs[2]["edep"], rtol=1e-12, atol=1e-12)
     d1, d2 = runs[1]["dose"], runs[2]["dose"]
     assert d2.sum() > 0 and np.allclose(d1, d2, rtol=1e-9, atol=1e-12 * d2.max())
+
+
+@pytest.mark.parametrize("tail", [0xFFFFFFFF, 64, 1 << 18])
+def test_tail_launch_gives_the_same_histories(gpu_ctx, ct_small, tail):
+    """The run's last particles finished by one launch (each with its
+    descendants in one thread, as the plain kernel transports them) or by
+    the sweeps to the end: the same histories as the oracle, bit for bit in
+    steps, whichever sweep hands them over (a tail of 64 particles starts
+    late in the run, 2^18 right after the first replay of this small run)."""
+    print("edges: tail_particles %d" % tail, flush=True)
+    ctx = gpu_ctx
+    ctx.load_workload(ct_small)
+    src = ct_small.source(4000, 21)
+    ctx.set_source(src)
+    base = ctx.get_tuning()
+    try:
+        ctx.set_tuning(dict(base, tail_particles=tail, photon_chunk=16, electron_chunk=8))
+        assert ctx.get_tuning()["tail_particles"] == tail
+        r = ctx.run(seed=9, n_batches=2, method="event", per_history=True)
+        with pytest.raises(native.GegsError) as e:
+            ctx.set_tuning(dict(base, tail_particles=(1 << 18) + 1))
+        assert e.value.code == 2001
+    finally:
+        ctx.set_tuning(base)
+    o = oracle.run(ct_small, src, seed=9, per_history=True)
+    assert np.array_equal(r["steps"], o["steps"])
+    assert np.allclose(r["edep"], o["edep"], rtol=1e-12, atol=1e-12)
+    assert r["counters"]["histories"] == 4000 and not r["counters"]["overflow"]
