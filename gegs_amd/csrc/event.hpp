@@ -23,8 +23,10 @@
 //     electron interactions) concurrently on two streams, each side draining
 //     what the previous sweeps left for it, then does its bookkeeping;
 //   * a group of sweeps is captured in a hipGraph and replayed until no
-//     particle and no source record is left; the host reads one flag per
-//     replay.
+//     particle and no source record is left, or until every record is
+//     injected and few particles remain, which one tail launch then
+//     finishes, each particle with its descendants in one thread; the host
+//     reads the loop's state once per replay.
 // Particles keep their Philox streams across banks, so every history follows
 // exactly the trajectory the plain kernel and the oracle give it.
 #pragma once
