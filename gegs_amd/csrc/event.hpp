@@ -687,8 +687,8 @@ __global__ void __launch_bounds__(512) GEGS_WAVES_PER_EU(3) electron_kernel(EvAr
     flush_counters(a, t, E_ELECTRON_STEP, pops);
 }
 
-// plan the next injection: refill when few particles are in flight
-__global__ void inject_plan(EvState *st, uint32_t threshold, uint32_t max_inject) {
+// plan the next sweep's injection: refill when few particles are in flight
+__device__ __forceinline__ void plan_injection(EvState *st, uint32_t threshold, uint32_t max_inject) {
     uint32_t n = 0;
     if (st->inflight < threshold && st->src_next < st->src_last) {
         n = st->src_last - st->src_next;
@@ -747,7 +747,7 @@ __global__ void __launch_bounds__(256) inject_fill(EvArgs a) {
     if (first_lane() && tot) atomicAdd(a.counters + 14, (unsigned long long)tot);
 }
 
-__global__ void sweep_end(EvState *st) {
+__global__ void sweep_end(EvState *st, uint32_t threshold, uint32_t max_inject) {
     // every bank drained in this sweep is empty: the eight single-pass banks
     // and the current buffers of banks 0 and 5, which take the next sweep's
     // pushes after the flip
@@ -767,6 +767,7 @@ __global__ void sweep_end(EvState *st) {
     st->inflight = inflight;
     st->active = (inflight > 0 || st->src_next < st->src_last) ? 1u : 0u;
     st->sweeps += 1;
+    plan_injection(st, threshold, max_inject);
 }
 
 // The last particles of a run, each with all its descendants in one thread.
@@ -855,7 +856,7 @@ __global__ void tail_end(EvState *st) {
     st->active = 0;
 }
 
-__global__ void batch_begin(EvState *st, uint32_t first, uint32_t last) {
+__global__ void batch_begin(EvState *st, uint32_t first, uint32_t last, uint32_t threshold, uint32_t max_inject) {
     for (int e = 0; e < N_SLOTS; ++e) {
         for (int g = 0; g < QSEG; ++g) st->count[e][g].v = 0;
         st->taken[e].v = 0;
@@ -868,6 +869,7 @@ __global__ void batch_begin(EvState *st, uint32_t first, uint32_t last) {
     st->inj_n = 0;
     st->active = 1;
     st->sweeps = 0;
+    plan_injection(st, threshold, max_inject);
 }
 
 __global__ void clear_overflow(EvState *st) { st->overflow = 0; }
@@ -1037,7 +1039,6 @@ inline void launch_sweep_t(const EvArgs &a, const EventConfig &cfg, hipStream_t 
                            hipEvent_t t1, const SweepFork &f) {
     const dim3 g(cfg.grid), b(cfg.block), fb(cfg.flight_block);
     const size_t lds = flight_lds(a.env);
-    hipLaunchKernelGGL(inject_plan, dim3(1), dim3(1), 0, stream, a.st, cfg.threshold, cfg.inject);
     hipLaunchKernelGGL(inject_fill, g, b, 0, stream, a);
     hipStream_t es = stream;
     if (f.side) {
@@ -1057,7 +1058,7 @@ inline void launch_sweep_t(const EvArgs &a, const EventConfig &cfg, hipStream_t 
         (void)hipEventRecord(f.join, f.side);
         (void)hipStreamWaitEvent(stream, f.join, 0);
     }
-    hipLaunchKernelGGL(sweep_end, dim3(1), dim3(1), 0, stream, a.st);
+    hipLaunchKernelGGL(sweep_end, dim3(1), dim3(1), 0, stream, a.st, cfg.threshold, cfg.inject);
 }
 
 template <bool GEN>
@@ -1224,7 +1225,7 @@ inline hipError_t run_event_batch(EventBanks &bk, const Env &env, const gegs_sou
     }
     e = hipMemcpyAsync(bk.d_plan, bk.h_plan, sizeof(BatchPlan), hipMemcpyHostToDevice, stream);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(batch_begin, dim3(1), dim3(1), 0, stream, bk.st, 0u, bk.h_plan->total);
+    hipLaunchKernelGGL(batch_begin, dim3(1), dim3(1), 0, stream, bk.st, 0u, bk.h_plan->total, cfg.threshold, cfg.inject);
     // Two replays in flight: while the host waits for the flag of replay k,
     // replay k+1 is already queued, so the GPU does not idle during the host
     // round trip.  The replay queued after the last busy one finds every bank
@@ -1262,7 +1263,8 @@ inline hipError_t run_event_batch(EventBanks &bk, const Env &env, const gegs_sou
     }
     auto tail_ready = [&](int slot) {
         const uint32_t *f = bk.h_flags + FLAG_WORDS * slot;
-        return use_tail && f[3] >= f[4] && f[7] <= cfg.tail;     // src_next >= src_last, inflight
+        // every record injected (src_next >= src_last, no injection planned), inflight
+        return use_tail && f[3] >= f[4] && f[6] == 0 && f[7] <= cfg.tail;
     };
     if ((e = launch(0)) != hipSuccess) return e;
     if ((e = launch(1)) != hipSuccess) return e;

@@ -201,14 +201,14 @@ int main(int argc, char **argv) {
     cfg.tail = m.tail;
     float *tail_stack = m.tail ? (float *)calloc((size_t)tail_threads(cfg) * TAIL_DEPTH * PART_WORDS, sizeof(float))
                                : nullptr;
-    cpu_launch(batch_begin, dim3(1), dim3(1), st, 0u, m.nsrc);
+    cpu_launch(batch_begin, dim3(1), dim3(1), st, 0u, m.nsrc, cfg.threshold, cfg.inject);
     bool tailed = false;
     for (;;) {
         launch_sweep(a, cfg, nullptr);
         ++sweeps;
         if (st->overflow) return 5;
         if (!st->active) break;
-        if (m.tail && st->src_next >= st->src_last && st->inflight <= m.tail) {
+        if (m.tail && st->src_next >= st->src_last && st->inj_n == 0 && st->inflight <= m.tail) {
             launch_tail(a, cfg, tail_stack, nullptr);
             tailed = true;
             if (st->overflow) return 5;
