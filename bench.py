@@ -529,6 +529,10 @@ def main():
         threads = min(os.cpu_count() or 1, 16)
         cpu = cpu_baseline(wl, args.cpu_seconds, threads)
         cpu_more = cpu_detail(wl, threads)
+    if tuning is not None:
+        # the event loop's whole effective tuning, defaults included
+        eff = ctx.get_tuning()
+        tuning = dict(tuning, sweep_streams=eff["sweep_streams"], tail_particles=eff["tail_particles"])
     ctx.close()
     atomics = atomic_peak(r["dose"], r["deposits_per_s"]) if head == "event" else None
     adj = None
